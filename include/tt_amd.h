@@ -1,0 +1,130 @@
+/*
+ * tt_amd.h -- C ABI of the MI355X (gfx950) TT-rounding library
+ * (libtt_amd.so, built from tensor_train_amd/csrc).
+ *
+ * Every entry point returns an int status (TTA_OK == 0); on failure the
+ * message is available from tta_last_error(ctx).  The header-only C++ layer
+ * (include/tensor_train_amd.hpp) maps a non-zero status to
+ * fprintf(stderr) + abort(), the error behaviour of the reference
+ * (round.hpp:124-134, matvec_round.hpp:47-52, shape_check.hpp:29-150 of
+ * bdaszuta/tensor_train).
+ *
+ * Data layout (identical to the reference, storage_backend.hpp:237-253):
+ *   TT core k          (r_k, n_k, r_{k+1})       row-major, (a*n + i)*rp + b
+ *   TT-matrix core k   (r_k, m_k, n_k, r_{k+1})  row-major, ((a*m + i)*n + j)*rp + b
+ * Host buffers passed to upload/download hold the cores concatenated in
+ * core order.  Device tensors are owned by the library (tta_tt); outputs are
+ * fresh tensors freed with tta_tt_free.
+ *
+ * Threading: a context owns one HIP stream; calls on one context are
+ * serialised by the caller.  All calls are synchronous with respect to the
+ * host unless documented otherwise.
+ */
+#ifndef TT_AMD_H
+#define TT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tta_ctx tta_ctx;
+typedef struct tta_tt tta_tt;
+
+enum tta_status
+{
+  TTA_OK = 0,
+  TTA_ERR_ARG = 1,    /* invalid argument (null handle, negative size) */
+  TTA_ERR_SHAPE = 2,  /* rank / mode mismatch, boundary ranks != 1 */
+  TTA_ERR_METHOD = 3, /* method not valid for this entry point */
+  TTA_ERR_HIP = 4,    /* HIP runtime failure */
+  TTA_ERR_NOMEM = 5,  /* device allocation failed */
+  TTA_ERR_UNSUPPORTED = 6
+};
+
+/* round_method, same order as include/core/round/round.hpp:33-44. */
+enum tta_method
+{
+  TTA_METHOD_SVD = 0,
+  TTA_METHOD_SVD_STREAMING = 1,
+  TTA_METHOD_SVD_NAIVE = 2,
+  TTA_METHOD_ALS = 3,
+  TTA_METHOD_DMRG = 4
+};
+
+/* round_gauge, include/core/round/round.hpp:49-53. */
+enum tta_gauge
+{
+  TTA_GAUGE_NONE = 0,
+  TTA_GAUGE_RIGHT_CANONICAL = 1
+};
+
+/* ---- context ------------------------------------------------------------ */
+int tta_ctx_create(int device, tta_ctx** out);
+int tta_ctx_destroy(tta_ctx* ctx);
+const char* tta_last_error(const tta_ctx* ctx);
+int tta_synchronize(tta_ctx* ctx);
+/* The context's hipStream_t, for callers that enqueue their own work. */
+void* tta_stream(tta_ctx* ctx);
+
+/* ---- device tensors ----------------------------------------------------- */
+/* Allocate a zero-filled TT with d cores, modes[d], ranks[d+1]
+ * (ranks[0] == ranks[d] == 1 required, as tt::tt validates, tt.hpp:35-70). */
+int tta_tt_create(tta_ctx* ctx, int d, const int* modes, const int* ranks, tta_tt** out);
+/* TT-matrix: row_modes[d] (m_k), col_modes[d] (n_k). */
+int tta_ttm_create(tta_ctx* ctx, int d, const int* row_modes, const int* col_modes, const int* ranks,
+                   tta_tt** out);
+int tta_tt_free(tta_ctx* ctx, tta_tt* t);
+/* d and whether t is a TT-matrix. */
+int tta_tt_dims(const tta_tt* t, int* d, int* is_matrix);
+/* Any output pointer may be null: modes[d] (m_k*n_k for matrices),
+ * row_modes[d] / col_modes[d] (matrices only), ranks[d+1]. */
+int tta_tt_shape(const tta_tt* t, int* modes, int* row_modes, int* col_modes, int* ranks);
+long long tta_tt_numel(const tta_tt* t);
+int tta_tt_upload(tta_ctx* ctx, tta_tt* t, const double* host);
+int tta_tt_download(tta_ctx* ctx, const tta_tt* t, double* host);
+/* Device pointer of core k (valid until the tensor is modified or freed). */
+int tta_tt_core_ptr(const tta_tt* t, int k, double** dev);
+int tta_tt_copy(tta_ctx* ctx, const tta_tt* t, tta_tt** out);
+
+/* ---- the hot path ------------------------------------------------------- */
+/* round(const tt&, round_options{eps, max_rank, method, gauge})
+ * (include/core/round/round.hpp:230-267) and round(const tt_matrix&, ...)
+ * (round.hpp:329-370, via round_matrix, tt_matrix_round.hpp:27-71).
+ * method must be TTA_METHOD_SVD (svd_streaming / svd_naive -> TTA_ERR_METHOD,
+ * round.hpp:124-134; ALS / DMRG -> TTA_ERR_UNSUPPORTED). */
+int tta_round(tta_ctx* ctx, const tta_tt* a, double eps, int max_rank, int method, int gauge, tta_tt** out);
+/* right_orthogonalize(tt&) (include/core/gauge/tt_orthogonalize.hpp:26-36) and
+ * right_orthogonalize(tt_matrix&) (tt_matrix_orthogonalize.hpp:34-85): in place;
+ * ranks may shrink to r_eff = min(n*r_right, r_left). */
+int tta_right_orthogonalize(tta_ctx* ctx, tta_tt* a);
+/* detail::left_orthogonalize (tt_orthogonalize.hpp:47-57), in place. */
+int tta_left_orthogonalize(tta_ctx* ctx, tta_tt* a);
+/* matvec(const tt_matrix&, const tt&) (include/core/apply/tt_matrix_apply.hpp:237-248). */
+int tta_matvec(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, tta_tt** y);
+/* matmat(const tt_matrix&, const tt_matrix&) (tt_matrix_apply.hpp:260-271). */
+int tta_matmat(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, tta_tt** C);
+/* matvec_round (include/core/round/matvec_round.hpp:42-91): method svd =
+ * round(matvec(A, x)) (tt_matrix_apply_round.hpp:32-38); svd_streaming =
+ * fused apply + round (streaming_apply.hpp:39-110); svd_naive -> TTA_ERR_METHOD. */
+int tta_matvec_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
+                     tta_tt** y);
+/* matmat_round (include/core/round/matmat_round.hpp:41-94): method svd =
+ * streaming (streaming_apply.hpp:115-202); svd_naive = round_matrix(matmat);
+ * svd_streaming -> TTA_ERR_METHOD. */
+int tta_matmat_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
+                     tta_tt** C);
+/* inner(a, b) (include/core/algebra/tt_inner.hpp:33-48); frob_inner for
+ * TT-matrices (tt_matrix_inner.hpp:31-36). */
+int tta_inner(tta_ctx* ctx, const tta_tt* a, const tta_tt* b, double* out);
+/* norm(a) = sqrt(max(inner(a, a), 0)) (tt_inner.hpp:55-59; tt_matrix_inner.hpp:44-48). */
+int tta_norm(tta_ctx* ctx, const tta_tt* a, double* out);
+/* axpby(alpha, a, beta, b) (include/core/algebra/tt_ops.hpp:72-110). */
+int tta_axpby(tta_ctx* ctx, double alpha, const tta_tt* a, double beta, const tta_tt* b, tta_tt** out);
+/* scale(a, alpha) (tt_ops.hpp:35-50). */
+int tta_scale(tta_ctx* ctx, const tta_tt* a, double alpha, tta_tt** out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TT_AMD_H */

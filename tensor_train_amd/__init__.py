@@ -1,0 +1,402 @@
+"""tensor_train_amd -- TT rounding and MPO apply-and-round on MI355X (gfx950).
+
+Host-side mirror of the reference's operator interface for the hot path
+(bdaszuta/tensor_train: ``bind_py/tensor_train/__init__.py`` classes
+``TensorTrain`` / ``TensorTrainMatrix`` / ``RoundOptions`` and the C++
+functions ``round``, ``right_orthogonalize``, ``matvec_round``,
+``matmat_round``, ``inner``, ``norm``).  Every call goes through the C ABI of
+``libtt_amd.so`` (include/tt_amd.h) into hand-written HIP kernels; tensors
+live in device memory between calls.  There is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import TTAError
+
+__all__ = [
+    "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
+    "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round", "inner",
+    "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError",
+]
+
+# round_method / round_gauge (include/core/round/round.hpp:33-53).
+METHODS = {"svd": 0, "svd_streaming": 1, "svd_naive": 2, "als": 3, "dmrg": 4}
+GAUGES = {"none": 0, "right_canonical": 1}
+
+
+def _enum(table, v, what):
+    if isinstance(v, str):
+        if v not in table:
+            raise ValueError(f"unknown {what} {v!r}; expected one of {sorted(table)}")
+        return table[v]
+    return int(v)
+
+
+class Context:
+    """One HIP stream + device pool on one GPU (``tta_ctx``)."""
+
+    _default = {}
+
+    def __init__(self, device: int = 0):
+        self.lib = _capi.lib()
+        h = ctypes.c_void_p()
+        st = self.lib.tta_ctx_create(device, ctypes.byref(h))
+        if st != 0:
+            raise TTAError(st, f"tta_ctx_create(device={device}) failed (no usable GPU?)")
+        self.h = h
+        self.device = device
+
+    @classmethod
+    def default(cls, device: int = 0) -> "Context":
+        if device not in cls._default:
+            cls._default[device] = Context(device)
+        return cls._default[device]
+
+    def check(self, st):
+        if st != 0:
+            raise TTAError(st, self.lib.tta_last_error(self.h).decode(errors="replace"))
+
+    def synchronize(self):
+        self.check(self.lib.tta_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.tta_stream(self.h) or 0)
+
+
+class RoundOptions:
+    """Options of ``round`` / ``matvec_round`` / ``matmat_round``.
+
+    Same fields and defaults as ``round_options`` (round.hpp:55-105):
+    eps=0.0 (no truncation), max_rank=0 (no cap), method="svd",
+    gauge="none".  ALS / DMRG knobs are accepted but those engines are not
+    part of this library.
+    """
+
+    def __init__(self, eps=0.0, max_rank=0, method="svd", gauge="none", max_iters=20, tol=1e-12, seed=0,
+                 warm_start=None):
+        self.eps = float(eps)
+        self.max_rank = int(max_rank)
+        self.method = _enum(METHODS, method, "round_method")
+        self.gauge = _enum(GAUGES, gauge, "round_gauge")
+        self.max_iters = int(max_iters)
+        self.tol = float(tol)
+        self.seed = int(seed)
+        self.warm_start = warm_start
+
+    @classmethod
+    def merge(cls, opts=None, **kwargs) -> "RoundOptions":
+        """RoundOptions / dict / kwargs merging of the reference wrapper
+        (bind_py/tensor_train/__init__.py:603-656): kwargs override an
+        options object; a dict is read as keyword arguments."""
+        if isinstance(opts, RoundOptions):
+            out = cls(opts.eps, opts.max_rank, opts.method, opts.gauge, opts.max_iters, opts.tol, opts.seed,
+                      opts.warm_start)
+            for k, v in kwargs.items():
+                if k == "method":
+                    v = _enum(METHODS, v, "round_method")
+                elif k == "gauge":
+                    v = _enum(GAUGES, v, "round_gauge")
+                setattr(out, k, v)
+            return out
+        if isinstance(opts, dict):
+            return cls(**opts)
+        return cls(**kwargs)
+
+
+class RoundResult:
+    """Diagnostics of the svd path: {iters_run=1, final_resid=0, converged=True}
+    (round.hpp:114-119, :196-203)."""
+
+    def __init__(self, iters_run=1, final_resid=0.0, converged=True):
+        self.iters_run = iters_run
+        self.final_resid = final_resid
+        self.converged = converged
+
+
+class _DeviceTT:
+    """Owner of a ``tta_tt`` handle."""
+
+    def __init__(self, handle, ctx: Context):
+        self._h = handle
+        self._ctx = ctx
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._ctx.lib.tta_tt_free(self._ctx.h, h)
+            except Exception:
+                pass
+            self._h = None
+
+    # -- shape -------------------------------------------------------------
+    @property
+    def d(self) -> int:
+        d = ctypes.c_int()
+        m = ctypes.c_int()
+        self._ctx.check(self._ctx.lib.tta_tt_dims(self._h, ctypes.byref(d), ctypes.byref(m)))
+        return d.value
+
+    def _shape_arrays(self):
+        d = self.d
+        modes = np.zeros(max(d, 1), np.int32)
+        rm = np.zeros(max(d, 1), np.int32)
+        cm = np.zeros(max(d, 1), np.int32)
+        rk = np.zeros(d + 1, np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._ctx.check(self._ctx.lib.tta_tt_shape(self._h, modes.ctypes.data_as(ip), rm.ctypes.data_as(ip),
+                                                   cm.ctypes.data_as(ip), rk.ctypes.data_as(ip)))
+        return modes[:d], rm[:d], cm[:d], rk
+
+    @property
+    def ranks(self):
+        return [int(x) for x in self._shape_arrays()[3]]
+
+    @property
+    def max_rank(self) -> int:
+        return max(self.ranks)
+
+    @property
+    def num_params(self) -> int:
+        return int(self._ctx.lib.tta_tt_numel(self._h))
+
+    def _flat(self) -> np.ndarray:
+        out = np.zeros(max(self.num_params, 1))
+        self._ctx.check(self._ctx.lib.tta_tt_download(self._ctx.h, self._h,
+                                                      out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return out[: self.num_params]
+
+    def core_ptr(self, k: int) -> int:
+        p = ctypes.POINTER(ctypes.c_double)()
+        self._ctx.check(self._ctx.lib.tta_tt_core_ptr(self._h, k, ctypes.byref(p)))
+        return ctypes.cast(p, ctypes.c_void_p).value or 0
+
+    @staticmethod
+    def _create(ctx, d, modes, ranks, row_modes=None, col_modes=None):
+        ip = ctypes.POINTER(ctypes.c_int)
+        h = ctypes.c_void_p()
+        rk = np.ascontiguousarray(ranks, np.int32)
+        if row_modes is None:
+            md = np.ascontiguousarray(modes, np.int32)
+            ctx.check(ctx.lib.tta_tt_create(ctx.h, d, md.ctypes.data_as(ip), rk.ctypes.data_as(ip), ctypes.byref(h)))
+        else:
+            rm = np.ascontiguousarray(row_modes, np.int32)
+            cm = np.ascontiguousarray(col_modes, np.int32)
+            ctx.check(ctx.lib.tta_ttm_create(ctx.h, d, rm.ctypes.data_as(ip), cm.ctypes.data_as(ip),
+                                             rk.ctypes.data_as(ip), ctypes.byref(h)))
+        return h
+
+    def _upload(self, flat):
+        flat = np.ascontiguousarray(flat, np.float64)
+        self._ctx.check(self._ctx.lib.tta_tt_upload(self._ctx.h, self._h,
+                                                    flat.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+
+
+class TensorTrain(_DeviceTT):
+    """A TT vector resident on the GPU; cores (r_k, n_k, r_{k+1}) row-major."""
+
+    @classmethod
+    def from_cores(cls, cores, ctx: Context | None = None) -> "TensorTrain":
+        ctx = ctx or Context.default()
+        cores = [np.ascontiguousarray(c, np.float64) for c in cores]
+        d = len(cores)
+        modes = [c.shape[1] for c in cores]
+        ranks = [c.shape[0] for c in cores] + [cores[-1].shape[2]] if d else [1]
+        h = cls._create(ctx, d, modes, ranks)
+        t = cls(h, ctx)
+        if d:
+            t._upload(np.concatenate([c.ravel() for c in cores]))
+        return t
+
+    @property
+    def shape(self):
+        return [int(x) for x in self._shape_arrays()[0]]
+
+    @property
+    def cores(self):
+        modes, _, _, rk = self._shape_arrays()
+        flat = self._flat()
+        out, off = [], 0
+        for k in range(len(modes)):
+            sz = int(rk[k] * modes[k] * rk[k + 1])
+            out.append(flat[off:off + sz].reshape(int(rk[k]), int(modes[k]), int(rk[k + 1])))
+            off += sz
+        return out
+
+    def to_dense(self) -> np.ndarray:
+        """Dense reconstruction on the host (test helper, tt.hpp:145-198)."""
+        cores = self.cores
+        if not cores:
+            return np.ones(1)
+        T = cores[0].reshape(-1, cores[0].shape[2])
+        for c in cores[1:]:
+            T = (T @ c.reshape(c.shape[0], -1)).reshape(-1, c.shape[2])
+        return T.ravel()
+
+    def round(self, opts=None, **kwargs) -> "TensorTrain":
+        return round(self, RoundOptions.merge(opts, **kwargs))
+
+    def round_ex(self, opts=None, **kwargs):
+        return round_ex(self, RoundOptions.merge(opts, **kwargs))
+
+    def right_orthogonalize(self) -> "TensorTrain":
+        """Return a copy with cores[1..d-1] right-orthogonal."""
+        out = self.copy()
+        right_orthogonalize(out)
+        return out
+
+    def copy(self):
+        h = ctypes.c_void_p()
+        self._ctx.check(self._ctx.lib.tta_tt_copy(self._ctx.h, self._h, ctypes.byref(h)))
+        return type(self)(h, self._ctx)
+
+    def norm(self) -> float:
+        return norm(self)
+
+    def dot(self, other) -> float:
+        return inner(self, other)
+
+
+class TensorTrainMatrix(_DeviceTT):
+    """A TT-matrix on the GPU; cores (r_k, m_k, n_k, r_{k+1}) row-major."""
+
+    @classmethod
+    def from_cores(cls, cores, ctx: Context | None = None) -> "TensorTrainMatrix":
+        ctx = ctx or Context.default()
+        cores = [np.ascontiguousarray(c, np.float64) for c in cores]
+        d = len(cores)
+        ranks = [c.shape[0] for c in cores] + [cores[-1].shape[3]] if d else [1]
+        h = cls._create(ctx, d, None, ranks, [c.shape[1] for c in cores], [c.shape[2] for c in cores])
+        t = cls(h, ctx)
+        if d:
+            t._upload(np.concatenate([c.ravel() for c in cores]))
+        return t
+
+    @property
+    def row_shape(self):
+        return [int(x) for x in self._shape_arrays()[1]]
+
+    @property
+    def col_shape(self):
+        return [int(x) for x in self._shape_arrays()[2]]
+
+    @property
+    def cores(self):
+        _, rm, cm, rk = self._shape_arrays()
+        flat = self._flat()
+        out, off = [], 0
+        for k in range(len(rm)):
+            sz = int(rk[k] * rm[k] * cm[k] * rk[k + 1])
+            out.append(flat[off:off + sz].reshape(int(rk[k]), int(rm[k]), int(cm[k]), int(rk[k + 1])))
+            off += sz
+        return out
+
+    def round(self, opts=None, **kwargs) -> "TensorTrainMatrix":
+        return round(self, RoundOptions.merge(opts, **kwargs))
+
+    def matvec(self, x):
+        return matvec(self, x)
+
+    def matvec_round(self, x, opts=None, **kwargs):
+        return matvec_round(self, x, RoundOptions.merge(opts, **kwargs))
+
+    def matmat(self, b):
+        return matmat(self, b)
+
+    def matmat_round(self, b, opts=None, **kwargs):
+        return matmat_round(self, b, RoundOptions.merge(opts, **kwargs))
+
+    def right_orthogonalize(self):
+        out = self.copy()
+        right_orthogonalize(out)
+        return out
+
+    def copy(self):
+        h = ctypes.c_void_p()
+        self._ctx.check(self._ctx.lib.tta_tt_copy(self._ctx.h, self._h, ctypes.byref(h)))
+        return type(self)(h, self._ctx)
+
+
+def _out(ctx, fn, *args, cls=TensorTrain):
+    h = ctypes.c_void_p()
+    ctx.check(fn(ctx.h, *args, ctypes.byref(h)))
+    return cls(h, ctx)
+
+
+def round(a, opts=None, **kwargs):  # noqa: A001 - the reference's name
+    """round(a, round_options) (include/core/round/round.hpp:230-425)."""
+    o = RoundOptions.merge(opts, **kwargs)
+    cls = type(a)
+    return _out(a._ctx, a._ctx.lib.tta_round, a._h, o.eps, o.max_rank, o.method, o.gauge, cls=cls)
+
+
+def round_ex(a, opts=None, **kwargs):
+    return round(a, opts, **kwargs), RoundResult()
+
+
+def right_orthogonalize(a) -> None:
+    """In place (tt_orthogonalize.hpp:26-36, tt_matrix_orthogonalize.hpp:34-85)."""
+    a._ctx.check(a._ctx.lib.tta_right_orthogonalize(a._ctx.h, a._h))
+
+
+def left_orthogonalize(a) -> None:
+    a._ctx.check(a._ctx.lib.tta_left_orthogonalize(a._ctx.h, a._h))
+
+
+def matvec(A, x):
+    return _out(A._ctx, A._ctx.lib.tta_matvec, A._h, x._h)
+
+
+def matmat(A, B):
+    return _out(A._ctx, A._ctx.lib.tta_matmat, A._h, B._h, cls=TensorTrainMatrix)
+
+
+def matvec_round(A, x, opts=None, **kwargs):
+    """matvec_round (include/core/round/matvec_round.hpp:42-91)."""
+    o = RoundOptions.merge(opts, **kwargs)
+    return _out(A._ctx, A._ctx.lib.tta_matvec_round, A._h, x._h, o.eps, o.max_rank, o.method)
+
+
+def matmat_round(A, B, opts=None, **kwargs):
+    """matmat_round (include/core/round/matmat_round.hpp:41-94)."""
+    o = RoundOptions.merge(opts, **kwargs)
+    return _out(A._ctx, A._ctx.lib.tta_matmat_round, A._h, B._h, o.eps, o.max_rank, o.method,
+                cls=TensorTrainMatrix)
+
+
+def inner(a, b) -> float:
+    out = ctypes.c_double()
+    a._ctx.check(a._ctx.lib.tta_inner(a._ctx.h, a._h, b._h, ctypes.byref(out)))
+    return out.value
+
+
+def norm(a) -> float:
+    out = ctypes.c_double()
+    a._ctx.check(a._ctx.lib.tta_norm(a._ctx.h, a._h, ctypes.byref(out)))
+    return out.value
+
+
+frob_inner = inner
+frob_norm = norm
+
+
+def axpby(alpha, a, beta, b):
+    return _out(a._ctx, a._ctx.lib.tta_axpby, ctypes.c_double(alpha), a._h, ctypes.c_double(beta), b._h,
+                cls=type(a))
+
+
+def add(a, b):
+    return axpby(1.0, a, 1.0, b)
+
+
+def sub(a, b):
+    return axpby(1.0, a, -1.0, b)
+
+
+def scale(a, alpha):
+    return _out(a._ctx, a._ctx.lib.tta_scale, a._h, ctypes.c_double(alpha), cls=type(a))

@@ -1,0 +1,81 @@
+"""ctypes binding of ``libtt_amd.so`` (the C ABI declared in include/tt_amd.h).
+
+The library is built in-tree into ``tensor_train_amd/_lib/libtt_amd.so`` by
+``tensor_train_amd/csrc/Makefile``.  There is no fallback: a missing library
+or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libtt_amd.so")
+CSRC = os.path.join(_HERE, "csrc")
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_vpp = ctypes.POINTER(ctypes.c_void_p)
+
+# name -> (restype, argtypes); mirrors include/tt_amd.h.
+SIGNATURES = {
+    "tta_ctx_create": (ctypes.c_int, [ctypes.c_int, _vpp]),
+    "tta_ctx_destroy": (ctypes.c_int, [_vp]),
+    "tta_last_error": (ctypes.c_char_p, [_vp]),
+    "tta_synchronize": (ctypes.c_int, [_vp]),
+    "tta_stream": (_vp, [_vp]),
+    "tta_tt_create": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _vpp]),
+    "tta_ttm_create": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _vpp]),
+    "tta_tt_free": (ctypes.c_int, [_vp, _vp]),
+    "tta_tt_dims": (ctypes.c_int, [_vp, _i32p, _i32p]),
+    "tta_tt_shape": (ctypes.c_int, [_vp, _i32p, _i32p, _i32p, _i32p]),
+    "tta_tt_numel": (ctypes.c_longlong, [_vp]),
+    "tta_tt_upload": (ctypes.c_int, [_vp, _vp, _f64p]),
+    "tta_tt_download": (ctypes.c_int, [_vp, _vp, _f64p]),
+    "tta_tt_core_ptr": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(_f64p)]),
+    "tta_tt_copy": (ctypes.c_int, [_vp, _vp, _vpp]),
+    "tta_round": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vpp]),
+    "tta_right_orthogonalize": (ctypes.c_int, [_vp, _vp]),
+    "tta_left_orthogonalize": (ctypes.c_int, [_vp, _vp]),
+    "tta_matvec": (ctypes.c_int, [_vp, _vp, _vp, _vpp]),
+    "tta_matmat": (ctypes.c_int, [_vp, _vp, _vp, _vpp]),
+    "tta_matvec_round": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vpp]),
+    "tta_matmat_round": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vpp]),
+    "tta_inner": (ctypes.c_int, [_vp, _vp, _vp, _f64p]),
+    "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
+    "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),
+    "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),
+}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile libtt_amd.so for gfx950 with hipcc (no GPU needed)."""
+    subprocess.run(["make", "-s", "-B", "-C", CSRC], check=True)
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libtt_amd.so not built: {LIB_PATH} (run tensor_train_amd._capi.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+class TTAError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[tta status {code}] {msg}")
+        self.code = code
+
+
+STATUS = {0: "OK", 1: "ARG", 2: "SHAPE", 3: "METHOD", 4: "HIP", 5: "NOMEM", 6: "UNSUPPORTED"}

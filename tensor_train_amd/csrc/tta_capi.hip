@@ -1,0 +1,1283 @@
+// C ABI of the gfx950 TT-rounding library: device tensors, the plain FP64
+// round / orthogonalize / apply / inner paths, orchestrated on one HIP
+// stream per context.  Kernels live in tta_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/tt_amd.h"
+#include "tta_internal.hpp"
+#include "tta_kernels.hpp"
+
+using namespace tta;
+
+// ----------------------------------------------------------------------------
+// Context: stream, size-bucketed device pool, workspace, pinned scalars.
+// ----------------------------------------------------------------------------
+namespace tta
+{
+
+[[noreturn]] void fail(int code, const char* fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  throw error(code, buf);
+}
+
+void hip_check(hipError_t e, const char* what)
+{
+  if (e != hipSuccess) fail(TTA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+size_t pool::bucket(size_t bytes)
+{
+  if (bytes < 256) return 256;
+  // Round up to one of 8 sizes per power of two (<= 12.5% slack).
+  size_t p = 1;
+  while ((p << 1) <= bytes) p <<= 1;
+  const size_t step = p >= 8 ? p / 8 : 1;
+  return (bytes + step - 1) / step * step;
+}
+
+void* pool::get(size_t bytes)
+{
+  const size_t b = bucket(bytes);
+  auto it = free_.find(b);
+  if (it != free_.end() && !it->second.empty())
+  {
+    void* p = it->second.back();
+    it->second.pop_back();
+    size_of_[p] = b;
+    return p;
+  }
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, b);
+  if (e != hipSuccess)
+  {
+    // Release cached blocks and retry once.
+    (void)hipGetLastError();
+    trim();
+    e = hipMalloc(&p, b);
+    if (e != hipSuccess)
+    {
+      (void)hipGetLastError();
+      fail(TTA_ERR_NOMEM, "device allocation of %zu bytes failed", b);
+    }
+  }
+  size_of_[p] = b;
+  return p;
+}
+
+void pool::put(void* p)
+{
+  if (!p) return;
+  auto it = size_of_.find(p);
+  if (it == size_of_.end()) return;
+  free_[it->second].push_back(p);
+  size_of_.erase(it);
+}
+
+void pool::trim()
+{
+  (void)hipDeviceSynchronize();
+  for (auto& kv : free_)
+    for (void* p : kv.second) (void)hipFree(p);
+  free_.clear();
+}
+
+pool::~pool()
+{
+  trim();
+  for (auto& kv : size_of_) (void)hipFree(kv.first);
+}
+
+double* ctx_ws(tta_ctx* c, int slot, size_t n)
+{
+  auto& w = c->ws[slot];
+  if (w.cap < n)
+  {
+    if (w.p) c->mem.put(w.p);
+    w.p = static_cast<double*>(c->mem.get(n * sizeof(double)));
+    w.cap = n;
+  }
+  return w.p;
+}
+
+}  // namespace tta
+
+// ----------------------------------------------------------------------------
+// Device tensor helpers.
+// ----------------------------------------------------------------------------
+namespace
+{
+
+tta_tt* new_tt(tta_ctx* c, const std::vector<int>& modes, const std::vector<int>& ranks, bool zero)
+{
+  const int d = (int)modes.size();
+  auto* t = new tta_tt();
+  t->ranks = ranks;
+  t->modes = modes;
+  t->cores.assign(d, nullptr);
+  try
+  {
+    for (int k = 0; k < d; ++k)
+    {
+      const size_t n = (size_t)ranks[k] * modes[k] * ranks[k + 1];
+      t->cores[k] = static_cast<double*>(c->mem.get(std::max<size_t>(n, 1) * sizeof(double)));
+      if (zero) hip_check(hipMemsetAsync(t->cores[k], 0, n * sizeof(double), c->stream), "hipMemsetAsync");
+    }
+  }
+  catch (...)
+  {
+    for (double* p : t->cores) c->mem.put(p);
+    delete t;
+    throw;
+  }
+  return t;
+}
+
+void free_tt(tta_ctx* c, tta_tt* t)
+{
+  if (!t) return;
+  for (double* p : t->cores) c->mem.put(p);
+  delete t;
+}
+
+size_t core_numel(const tta_tt* t, int k) { return (size_t)t->ranks[k] * t->modes[k] * t->ranks[k + 1]; }
+
+tta_tt* copy_tt(tta_ctx* c, const tta_tt* a)
+{
+  tta_tt* t = new_tt(c, a->modes, a->ranks, false);
+  t->row_modes = a->row_modes;
+  t->col_modes = a->col_modes;
+  for (int k = 0; k < a->d(); ++k)
+    hip_check(hipMemcpyAsync(t->cores[k], a->cores[k], core_numel(a, k) * sizeof(double), hipMemcpyDeviceToDevice,
+                             c->stream),
+              "hipMemcpyAsync");
+  return t;
+}
+
+void replace_core(tta_ctx* c, tta_tt* t, int k, double* p)
+{
+  c->mem.put(t->cores[k]);
+  t->cores[k] = p;
+}
+
+double* alloc_core(tta_ctx* c, size_t n) { return static_cast<double*>(c->mem.get(std::max<size_t>(n, 1) * 8)); }
+
+void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const double* B, long ldb, double* C,
+             long ldc)
+{
+  gemm_desc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sar = lda;
+  g.sac = 1;
+  g.B = B;
+  g.sbr = ldb;
+  g.sbc = 1;
+  g.C = C;
+  g.scr = ldc;
+  g.scc = 1;
+  launch_gemm(g, 1, c->stream);
+}
+
+int read_int(tta_ctx* c, const int* dev)
+{
+  hip_check(hipMemcpyAsync(c->h_int, dev, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return *c->h_int;
+}
+
+double read_double(tta_ctx* c, const double* dev)
+{
+  hip_check(hipMemcpyAsync(c->h_dbl, dev, sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  return *c->h_dbl;
+}
+
+// Scratch slots (see tta_internal.hpp).
+enum
+{
+  WS_A = 0,
+  WS_Q,
+  WS_X,
+  WS_J,
+  WS_F,
+  WS_JP,
+  WS_M1,
+  WS_M1P,
+  WS_PART,
+  WS_SMALL
+};
+
+// Small device scalars live in WS_SMALL: [0] norm, [1] delta, [2..] ints.
+double* small_dbl(tta_ctx* c) { return ctx_ws(c, WS_SMALL, 64); }
+int* small_int(tta_ctx* c) { return reinterpret_cast<int*>(small_dbl(c) + 16); }
+
+// Frobenius norm of n doubles at x; also writes delta = eps*norm*inv to small[1].
+double frob_norm_dev(tta_ctx* c, const double* x, size_t n, double eps, double inv)
+{
+  double* part = ctx_ws(c, WS_PART, 1024);
+  double* sm = small_dbl(c);
+  launch_frob_norm(x, (long)n, part, 1024, sm, eps, inv, sm + 1, c->stream);
+  return read_double(c, sm);
+}
+
+// ---------------------------------------------------------------------------
+// right_qr_step (include/core/detail/orthogonalize.hpp:63-100): QR of the
+// column-major (n*rp x r) view of core k (its own bytes, factored in place),
+// thin Q -> new core (r_eff, n, rp), L = R^T absorbed into core k-1.
+// ---------------------------------------------------------------------------
+void right_qr_step(tta_ctx* c, tta_tt* t, int k)
+{
+  const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+  const int m = n * rp;
+  const int r_eff = std::min(m, r);
+  double* A = t->cores[k];
+  double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
+  double* Qn = alloc_core(c, (size_t)r_eff * m);
+  qr_desc q;
+  q.A = A;
+  q.lda = m;
+  q.m = m;
+  q.n = r;
+  q.tau = tau;
+  q.Q = Qn;
+  q.ldq = m;
+  launch_qr(q, 1, c->stream);
+  double* L = ctx_ws(c, WS_X, (size_t)r * r_eff);
+  launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream);
+  const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
+  double* Pn = alloc_core(c, (size_t)rows_p * r_eff);
+  gemm_rm(c, rows_p, r_eff, r, t->cores[k - 1], r, L, r_eff, Pn, r_eff);
+  replace_core(c, t, k - 1, Pn);
+  replace_core(c, t, k, Qn);
+  t->ranks[k] = r_eff;
+}
+
+// left_qr_step (orthogonalize.hpp:35-60): QR of left_unfold(core k), R
+// absorbed into core k+1.
+void left_qr_step(tta_ctx* c, tta_tt* t, int k)
+{
+  const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+  const int m = r * n;
+  const int kk = std::min(m, rp);
+  double* A = ctx_ws(c, WS_A, (size_t)m * rp);
+  launch_transpose(t->cores[k], rp, m, rp, A, m, c->stream);
+  double* tau = ctx_ws(c, WS_F, (size_t)std::max(kk, 1));
+  double* Q = ctx_ws(c, WS_Q, (size_t)m * kk);
+  qr_desc q;
+  q.A = A;
+  q.lda = m;
+  q.m = m;
+  q.n = rp;
+  q.tau = tau;
+  q.Q = Q;
+  q.ldq = m;
+  launch_qr(q, 1, c->stream);
+  double* Cn = alloc_core(c, (size_t)m * kk);
+  launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
+  double* R = ctx_ws(c, WS_X, (size_t)kk * rp);
+  launch_extract_upper(A, m, kk, rp, R, rp, 1, c->stream);
+  const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
+  double* Nn = alloc_core(c, (size_t)kk * ncols_n);
+  gemm_rm(c, kk, ncols_n, rp, R, rp, t->cores[k + 1], ncols_n, Nn, ncols_n);
+  replace_core(c, t, k + 1, Nn);
+  replace_core(c, t, k, Cn);
+  t->ranks[k + 1] = kk;
+}
+
+// Thin SVD of a row-major (rows x cols) matrix M on the device, returning
+// the truncation rank and leaving:
+//   U_head  (rows x r_new) row-major in `U_out` (orthonormal columns, exact
+//           up to rounding since it is a product of orthogonal factors),
+//   F       (r_new x cols) row-major = diag(s_head) Vt_head in `F_out`.
+// rows >= cols: M = Q R (Householder), one-sided Jacobi on R^T gives
+//   R^T J = W, so M = (Q J) W^T.  rows < cols: Jacobi on M^T directly.
+// `M` is clobbered only when `may_clobber`.
+int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int max_rank, double* U_out_alloc_hint,
+             double** U_out, double** F_out)
+{
+  (void)U_out_alloc_hint;
+  int* rk = small_int(c);
+  jacobi_desc jd;
+  jd.delta = delta;
+  jd.max_rank = max_rank;
+  jd.rank_out = rk;
+  if (rows >= cols)
+  {
+    double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+    launch_transpose(M, cols, rows, cols, A, rows, c->stream);
+    double* tau = ctx_ws(c, WS_F, (size_t)cols);
+    double* Q = ctx_ws(c, WS_Q, (size_t)rows * cols);
+    qr_desc q;
+    q.A = A;
+    q.lda = rows;
+    q.m = rows;
+    q.n = cols;
+    q.tau = tau;
+    q.Q = Q;
+    q.ldq = rows;
+    launch_qr(q, 1, c->stream);
+    double* X = ctx_ws(c, WS_X, (size_t)cols * cols);
+    launch_extract_upper(A, rows, cols, cols, X, cols, 1, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)cols * cols);
+    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)cols + 8) + 1024;
+    int* perm = reinterpret_cast<int*>(sv + cols);
+    jd.X = X;
+    jd.ldx = cols;
+    jd.p = cols;
+    jd.q = cols;
+    jd.J = J;
+    jd.ldj = cols;
+    jd.s = sv;
+    jd.perm = perm;
+    launch_jacobi(jd, 1, c->stream);
+    const int r_new = read_int(c, rk);
+    double* Jp = ctx_ws(c, WS_JP, (size_t)cols * r_new);
+    launch_gather_cols(J, cols, cols, perm, r_new, Jp, 1, cols, c->stream);
+    double* U = alloc_core(c, (size_t)rows * r_new);
+    gemm_desc g;
+    g.M = rows;
+    g.N = r_new;
+    g.K = cols;
+    g.A = Q;
+    g.sar = 1;
+    g.sac = rows;
+    g.B = Jp;
+    g.sbr = 1;
+    g.sbc = cols;
+    g.C = U;
+    g.scr = r_new;
+    g.scc = 1;
+    launch_gemm(g, 1, c->stream);
+    double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
+    launch_gather_cols(X, cols, cols, perm, r_new, F, 1, cols, c->stream);
+    *U_out = U;
+    *F_out = F;
+    return r_new;
+  }
+  // rows < cols: X = M^T (cols x rows) column-major == M's bytes.
+  double* X = ctx_ws(c, WS_X, (size_t)rows * cols);
+  hip_check(hipMemcpyAsync(X, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+  double* J = ctx_ws(c, WS_J, (size_t)rows * rows);
+  double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)rows + 8) + 1024;
+  int* perm = reinterpret_cast<int*>(sv + rows);
+  jd.X = X;
+  jd.ldx = cols;
+  jd.p = cols;
+  jd.q = rows;
+  jd.J = J;
+  jd.ldj = rows;
+  jd.s = sv;
+  jd.perm = perm;
+  launch_jacobi(jd, 1, c->stream);
+  const int r_new = read_int(c, rk);
+  double* U = alloc_core(c, (size_t)rows * r_new);
+  launch_gather_cols(J, rows, rows, perm, r_new, U, r_new, 1, c->stream);
+  double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
+  launch_gather_cols(X, cols, cols, perm, r_new, F, 1, cols, c->stream);
+  *U_out = U;
+  *F_out = F;
+  return r_new;
+}
+
+// svd_step (include/core/detail/truncate.hpp:61-99).
+void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
+{
+  const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+  const int rows = r * n, cols = rp;
+  double *U = nullptr, *F = nullptr;
+  const int r_new = svd_left(c, t->cores[k], rows, cols, delta, max_rank, nullptr, &U, &F);
+  const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
+  double* Nn = alloc_core(c, (size_t)r_new * ncols_n);
+  gemm_rm(c, r_new, ncols_n, cols, F, cols, t->cores[k + 1], ncols_n, Nn, ncols_n);
+  replace_core(c, t, k, U);
+  replace_core(c, t, k + 1, Nn);
+  t->ranks[k + 1] = r_new;
+}
+
+// svd_step_right (truncate.hpp:114-166): SVD of right_unfold(core k)
+// (r x n*rp); V kept exactly orthonormal (new core rows), U diag(s)
+// absorbed into core k-1.
+void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
+{
+  const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+  const int rows = r, cols = n * rp;
+  int* rk = small_int(c);
+  jacobi_desc jd;
+  jd.delta = delta;
+  jd.max_rank = max_rank;
+  jd.rank_out = rk;
+  double* Cn = nullptr;
+  double* F = nullptr;
+  int r_new = 0;
+  if (cols >= rows)
+  {
+    // M^T (cols x rows) column-major == the core's bytes; factor in place.
+    double* A = t->cores[k];
+    double* tau = ctx_ws(c, WS_F, (size_t)rows);
+    double* Q = ctx_ws(c, WS_Q, (size_t)rows * cols);
+    qr_desc q;
+    q.A = A;
+    q.lda = cols;
+    q.m = cols;
+    q.n = rows;
+    q.tau = tau;
+    q.Q = Q;
+    q.ldq = cols;
+    launch_qr(q, 1, c->stream);
+    double* X = ctx_ws(c, WS_X, (size_t)rows * rows);
+    launch_extract_upper(A, cols, rows, rows, X, rows, 1, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)rows * rows);
+    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)rows + 8) + 1024;
+    int* perm = reinterpret_cast<int*>(sv + rows);
+    jd.X = X;
+    jd.ldx = rows;
+    jd.p = rows;
+    jd.q = rows;
+    jd.J = J;
+    jd.ldj = rows;
+    jd.s = sv;
+    jd.perm = perm;
+    launch_jacobi(jd, 1, c->stream);
+    r_new = read_int(c, rk);
+    double* Jp = ctx_ws(c, WS_JP, (size_t)rows * r_new);
+    launch_gather_cols(J, rows, rows, perm, r_new, Jp, 1, rows, c->stream);
+    Cn = alloc_core(c, (size_t)r_new * cols);
+    gemm_desc g;
+    g.M = r_new;
+    g.N = cols;
+    g.K = rows;
+    g.A = Jp;
+    g.sar = rows;
+    g.sac = 1;
+    g.B = Q;
+    g.sbr = cols;
+    g.sbc = 1;
+    g.C = Cn;
+    g.scr = cols;
+    g.scc = 1;
+    launch_gemm(g, 1, c->stream);
+    F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
+    launch_gather_cols(X, rows, rows, perm, r_new, F, r_new, 1, c->stream);
+  }
+  else
+  {
+    double* X = ctx_ws(c, WS_X, (size_t)rows * cols);
+    launch_transpose(t->cores[k], cols, rows, cols, X, rows, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)cols * cols);
+    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)cols + 8) + 1024;
+    int* perm = reinterpret_cast<int*>(sv + cols);
+    jd.X = X;
+    jd.ldx = rows;
+    jd.p = rows;
+    jd.q = cols;
+    jd.J = J;
+    jd.ldj = cols;
+    jd.s = sv;
+    jd.perm = perm;
+    launch_jacobi(jd, 1, c->stream);
+    r_new = read_int(c, rk);
+    Cn = alloc_core(c, (size_t)r_new * cols);
+    launch_gather_cols(J, cols, cols, perm, r_new, Cn, 1, cols, c->stream);
+    F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
+    launch_gather_cols(X, rows, rows, perm, r_new, F, r_new, 1, c->stream);
+  }
+  const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
+  double* Pn = alloc_core(c, (size_t)rows_p * r_new);
+  gemm_rm(c, rows_p, r_new, rows, t->cores[k - 1], rows, F, r_new, Pn, r_new);
+  replace_core(c, t, k - 1, Pn);
+  replace_core(c, t, k, Cn);
+  t->ranks[k] = r_new;
+}
+
+void right_orthogonalize_impl(tta_ctx* c, tta_tt* t)
+{
+  const int d = t->d();
+  if (d <= 1) return;
+  for (int k = d - 1; k >= 1; --k) right_qr_step(c, t, k);
+}
+
+void left_orthogonalize_impl(tta_ctx* c, tta_tt* t)
+{
+  const int d = t->d();
+  if (d <= 1) return;
+  for (int k = 0; k < d - 1; ++k) left_qr_step(c, t, k);
+}
+
+void make_rank1_zero(tta_ctx* c, tta_tt* t)
+{
+  for (int k = 0; k < t->d(); ++k)
+  {
+    double* p = alloc_core(c, (size_t)t->modes[k]);
+    hip_check(hipMemsetAsync(p, 0, (size_t)t->modes[k] * 8, c->stream), "hipMemsetAsync");
+    replace_core(c, t, k, p);
+  }
+  std::fill(t->ranks.begin(), t->ranks.end(), 1);
+}
+
+// Norm balancing (include/core/round/tt_round.hpp:76-124).
+void balance(tta_ctx* c, tta_tt* t)
+{
+  const int d = t->d();
+  std::vector<size_t> off(d, 0);
+  size_t total = 0;
+  for (int k = 0; k < d - 1; ++k)
+  {
+    off[k] = total;
+    total += (size_t)t->ranks[k + 1];
+  }
+  double* nrm = ctx_ws(c, WS_JP, total + 1);
+  for (int k = 0; k < d - 1; ++k)
+    launch_col_norms(t->cores[k], t->ranks[k] * t->modes[k], t->ranks[k + 1], nrm + off[k], c->stream);
+  double* geo = small_dbl(c) + 2;
+  launch_geo_mean(nrm, (int)total, geo, c->stream);
+  for (int k = 0; k < d; ++k)
+  {
+    const double* nl = k > 0 ? nrm + off[k - 1] : nullptr;
+    const double* nr = k < d - 1 ? nrm + off[k] : nullptr;
+    launch_balance(t->cores[k], t->ranks[k], t->modes[k], t->ranks[k + 1], nl, nr, geo, c->stream);
+  }
+}
+
+// detail::round (tt_round.hpp:36-127) / round_right_canonical (:151-191) on
+// a working copy `t` (consumed in place).
+void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
+{
+  const int d = t->d();
+  if (d <= 1) return;
+  const double inv = 1.0 / std::sqrt((double)std::max(d - 1, 1));
+  if (gauge == TTA_GAUGE_RIGHT_CANONICAL)
+  {
+    left_orthogonalize_impl(c, t);
+    const double nrm = frob_norm_dev(c, t->cores[d - 1], core_numel(t, d - 1), eps, inv);
+    if (nrm == 0.0)
+    {
+      make_rank1_zero(c, t);
+      return;
+    }
+    const double delta = eps * nrm * inv;
+    for (int k = d - 1; k >= 1; --k) svd_step_right(c, t, k, delta, max_rank);
+    return;
+  }
+  right_orthogonalize_impl(c, t);
+  const double nrm = frob_norm_dev(c, t->cores[0], core_numel(t, 0), eps, inv);
+  if (nrm == 0.0)
+  {
+    make_rank1_zero(c, t);
+    return;
+  }
+  const double delta = eps * nrm * inv;
+  for (int k = 0; k < d - 1; ++k) svd_step(c, t, k, delta, max_rank);
+  balance(c, t);
+}
+
+// matvec (tt_matrix_apply.hpp:237-248) into a fresh tensor.
+tta_tt* matvec_impl(tta_ctx* c, const tta_tt* A, const tta_tt* x)
+{
+  const int d = A->d();
+  std::vector<int> modes(d), ranks(d + 1);
+  for (int k = 0; k < d; ++k) modes[k] = A->row_modes[k];
+  for (int k = 0; k <= d; ++k) ranks[k] = A->ranks[k] * x->ranks[k];
+  tta_tt* y = new_tt(c, modes, ranks, false);
+  for (int k = 0; k < d; ++k)
+    launch_matvec_core(A->cores[k], x->cores[k], y->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
+                       A->ranks[k + 1], x->ranks[k], x->ranks[k + 1], c->stream);
+  return y;
+}
+
+// matmat (tt_matrix_apply.hpp:260-271).
+tta_tt* matmat_impl(tta_ctx* c, const tta_tt* A, const tta_tt* B)
+{
+  const int d = A->d();
+  std::vector<int> modes(d), ranks(d + 1);
+  for (int k = 0; k < d; ++k) modes[k] = A->row_modes[k] * B->col_modes[k];
+  for (int k = 0; k <= d; ++k) ranks[k] = A->ranks[k] * B->ranks[k];
+  tta_tt* C = new_tt(c, modes, ranks, false);
+  C->row_modes = A->row_modes;
+  C->col_modes = B->col_modes;
+  for (int k = 0; k < d; ++k)
+    launch_matmat_core(A->cores[k], B->cores[k], C->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
+                       A->ranks[k + 1], B->ranks[k], B->col_modes[k], B->ranks[k + 1], c->stream);
+  return C;
+}
+
+// fused_matvec_core / fused_matmat_core (include/core/detail/fused_apply_core.hpp:33-131,
+// :136-272).  F (alpha x a*sx) row-major; A_k (a, m, p, b); X_k (sx, p, nx, bx)
+// with nx = 1 for a vector.  Output (alpha, m, nx, b*bx) into a fresh buffer.
+double* fused_core(tta_ctx* c, const double* F, int alpha, const double* Ak, int a, int m, int p, int b,
+                   const double* Xk, int sx, int nx, int bx)
+{
+  const size_t mpb = (size_t)m * p * b;
+  // M1 (alpha, ax, i, q, bb) = sum_aa F(al, aa*sx + ax) A(aa, i, q, bb): batch over al.
+  double* M1 = ctx_ws(c, WS_M1, (size_t)alpha * sx * mpb);
+  gemm_desc g;
+  g.M = sx;
+  g.N = (int)mpb;
+  g.K = a;
+  g.A = F;
+  g.sar = 1;
+  g.sac = sx;
+  g.B = Ak;
+  g.sbr = (long)mpb;
+  g.sbc = 1;
+  g.C = M1;
+  g.scr = (long)mpb;
+  g.scc = 1;
+  g.bsa = (long)a * sx;
+  g.bsb = 0;
+  g.bsc = (long)sx * mpb;
+  launch_gemm(g, alpha, c->stream);
+  // M1p (alpha, i, bb, ax, q) <- M1 (alpha, ax, i, q, bb).
+  double* M1p = ctx_ws(c, WS_M1P, (size_t)alpha * sx * mpb);
+  permute_desc pd;
+  pd.src = M1;
+  pd.dst = M1p;
+  pd.n[0] = 1;
+  pd.n[1] = alpha;
+  pd.n[2] = m;
+  pd.n[3] = b;
+  pd.n[4] = sx;
+  pd.n[5] = p;
+  pd.ss[1] = (long)sx * mpb;
+  pd.ss[2] = (long)p * b;
+  pd.ss[3] = 1;
+  pd.ss[4] = (long)mpb;
+  pd.ss[5] = b;
+  launch_permute(pd, c->stream);
+  // R (alpha*m*b x nx*bx) = M1p (alpha*m*b x sx*p) * X_view (sx*p x nx*bx).
+  const long rowsR = (long)alpha * m * b, colsR = (long)nx * bx;
+  double* out = alloc_core(c, (size_t)rowsR * colsR);
+  if (nx == 1)
+  {
+    gemm_rm(c, (int)rowsR, (int)colsR, sx * p, M1p, sx * p, Xk, colsR, out, colsR);
+    return out;
+  }
+  double* R = ctx_ws(c, WS_A, (size_t)rowsR * colsR);
+  gemm_rm(c, (int)rowsR, (int)colsR, sx * p, M1p, sx * p, Xk, colsR, R, colsR);
+  // out (alpha, i, j, bb, bx) <- R (alpha, i, bb, j, bx).
+  permute_desc p2;
+  p2.src = R;
+  p2.dst = out;
+  p2.n[1] = alpha;
+  p2.n[2] = m;
+  p2.n[3] = nx;
+  p2.n[4] = b;
+  p2.n[5] = bx;
+  p2.ss[1] = (long)m * b * nx * bx;
+  p2.ss[2] = (long)b * nx * bx;
+  p2.ss[3] = bx;
+  p2.ss[4] = (long)nx * bx;
+  p2.ss[5] = 1;
+  launch_permute(p2, c->stream);
+  return out;
+}
+
+// build_streaming_matvec / build_streaming_matmat
+// (include/core/detail/streaming_apply.hpp:39-110, :115-202).
+tta_tt* streaming_impl(tta_ctx* c, const tta_tt* A_in, const tta_tt* X_in, bool is_matmat, double eps, int max_rank)
+{
+  const int d = A_in->d();
+  tta_tt* A = copy_tt(c, A_in);
+  tta_tt* X = copy_tt(c, X_in);
+  tta_tt* out = nullptr;
+  try
+  {
+    right_orthogonalize_impl(c, A);
+    right_orthogonalize_impl(c, X);
+    const double nA = frob_norm_dev(c, A->cores[0], core_numel(A, 0), 0.0, 0.0);
+    const double nX = frob_norm_dev(c, X->cores[0], core_numel(X, 0), 0.0, 0.0);
+    const double norm = nA * nX;
+    std::vector<int> modes(d), ranks(d + 1, 1);
+    for (int k = 0; k < d; ++k)
+      modes[k] = is_matmat ? A_in->row_modes[k] * X_in->col_modes[k] : A_in->row_modes[k];
+    out = new_tt(c, modes, ranks, true);
+    if (is_matmat)
+    {
+      out->row_modes = A_in->row_modes;
+      out->col_modes = X_in->col_modes;
+    }
+    if (norm == 0.0)
+    {
+      free_tt(c, A);
+      free_tt(c, X);
+      return out;
+    }
+    const double delta = eps * norm / std::sqrt((double)std::max(d - 1, 1));
+    // F starts as [[1]].
+    double* Fbuf = ctx_ws(c, WS_JP, 1);
+    const double one = 1.0;
+    hip_check(hipMemcpyAsync(Fbuf, &one, 8, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    // The F environment is carried in its own pool block between steps.
+    double* F = alloc_core(c, 1);
+    hip_check(hipMemcpyAsync(F, Fbuf, 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    int alpha = 1;
+    for (int k = 0; k < d; ++k)
+    {
+      const int a = A->ranks[k], b = A->ranks[k + 1];
+      const int m = A->row_modes[k], p = A->col_modes[k];
+      const int sx = X->ranks[k], bx = X->ranks[k + 1];
+      const int nx = is_matmat ? X->col_modes[k] : 1;
+      double* C = fused_core(c, F, alpha, A->cores[k], a, m, p, b, X->cores[k], sx, nx, bx);
+      const int mk = m * nx, br = b * bx;
+      if (k == d - 1)
+      {
+        replace_core(c, out, k, C);
+        out->ranks[k] = alpha;
+        out->ranks[k + 1] = br;
+        break;
+      }
+      double *U = nullptr, *Fs = nullptr;
+      const int r_new = svd_left(c, C, alpha * mk, br, delta, max_rank, nullptr, &U, &Fs);
+      c->mem.put(C);
+      replace_core(c, out, k, U);
+      out->ranks[k] = alpha;
+      out->ranks[k + 1] = r_new;
+      double* Fn = alloc_core(c, (size_t)r_new * br);
+      hip_check(hipMemcpyAsync(Fn, Fs, (size_t)r_new * br * 8, hipMemcpyDeviceToDevice, c->stream),
+                "hipMemcpyAsync");
+      c->mem.put(F);
+      F = Fn;
+      alpha = r_new;
+    }
+    c->mem.put(F);
+  }
+  catch (...)
+  {
+    free_tt(c, A);
+    free_tt(c, X);
+    free_tt(c, out);
+    throw;
+  }
+  free_tt(c, A);
+  free_tt(c, X);
+  return out;
+}
+
+// inner (tt_inner.hpp:33-48) via contract_inner_step (contract.hpp:155-205).
+double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
+{
+  const int d = a->d();
+  if (d == 0) return 0.0;
+  size_t emax = 1, tmax = 1;
+  for (int k = 0; k <= d; ++k) emax = std::max(emax, (size_t)a->ranks[k] * b->ranks[k]);
+  for (int k = 0; k < d; ++k) tmax = std::max(tmax, (size_t)b->ranks[k] * a->modes[k] * a->ranks[k + 1]);
+  double* E0 = ctx_ws(c, WS_X, emax);
+  double* E1 = ctx_ws(c, WS_J, emax);
+  double* T = ctx_ws(c, WS_M1, tmax);
+  const double one = 1.0;
+  hip_check(hipMemcpyAsync(E0, &one, 8, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (int k = 0; k < d; ++k)
+  {
+    const int n = a->modes[k], ra = a->ranks[k], rap = a->ranks[k + 1], rb = b->ranks[k], rbp = b->ranks[k + 1];
+    // tmp (rb x n*rap) = E^T (rb x ra) * right_unfold(ca) (ra x n*rap)
+    gemm_desc g;
+    g.M = rb;
+    g.N = n * rap;
+    g.K = ra;
+    g.A = E0;
+    g.sar = 1;
+    g.sac = rb;
+    g.B = a->cores[k];
+    g.sbr = (long)n * rap;
+    g.sbc = 1;
+    g.C = T;
+    g.scr = (long)n * rap;
+    g.scc = 1;
+    launch_gemm(g, 1, c->stream);
+    // E' (rap x rbp) = tmp_as^T (rap x rb*n) * left_unfold(cb) (rb*n x rbp)
+    gemm_desc h;
+    h.M = rap;
+    h.N = rbp;
+    h.K = rb * n;
+    h.A = T;
+    h.sar = 1;
+    h.sac = rap;
+    h.B = b->cores[k];
+    h.sbr = rbp;
+    h.sbc = 1;
+    h.C = E1;
+    h.scr = rbp;
+    h.scc = 1;
+    launch_gemm(h, 1, c->stream);
+    std::swap(E0, E1);
+  }
+  return read_double(c, E0);
+}
+
+void check_tt(const tta_tt* t, const char* what)
+{
+  if (!t) fail(TTA_ERR_ARG, "%s: null tensor", what);
+}
+
+void check_same_shape(const tta_tt* a, const tta_tt* b, const char* what)
+{
+  check_tt(a, what);
+  check_tt(b, what);
+  if (a->d() != b->d()) fail(TTA_ERR_SHAPE, "%s: core count mismatch %d vs %d", what, a->d(), b->d());
+  for (int k = 0; k < a->d(); ++k)
+    if (a->modes[k] != b->modes[k])
+      fail(TTA_ERR_SHAPE, "%s: mode size mismatch at core %d (%d vs %d)", what, k, a->modes[k], b->modes[k]);
+}
+
+void check_matvec(const tta_tt* A, const tta_tt* x, const char* what)
+{
+  check_tt(A, what);
+  check_tt(x, what);
+  if (!A->is_matrix()) fail(TTA_ERR_ARG, "%s: A is not a TT-matrix", what);
+  if (x->is_matrix()) fail(TTA_ERR_ARG, "%s: x is a TT-matrix", what);
+  if (A->d() != x->d()) fail(TTA_ERR_SHAPE, "%s: core count mismatch %d vs %d", what, A->d(), x->d());
+  for (int k = 0; k < A->d(); ++k)
+    if (A->col_modes[k] != x->modes[k])
+      fail(TTA_ERR_SHAPE, "%s: mode mismatch at core %d (A col %d vs x %d)", what, k, A->col_modes[k], x->modes[k]);
+}
+
+void check_matmat(const tta_tt* A, const tta_tt* B, const char* what)
+{
+  check_tt(A, what);
+  check_tt(B, what);
+  if (!A->is_matrix() || !B->is_matrix()) fail(TTA_ERR_ARG, "%s: operands must be TT-matrices", what);
+  if (A->d() != B->d()) fail(TTA_ERR_SHAPE, "%s: core count mismatch %d vs %d", what, A->d(), B->d());
+  for (int k = 0; k < A->d(); ++k)
+    if (A->col_modes[k] != B->row_modes[k])
+      fail(TTA_ERR_SHAPE, "%s: mode mismatch at core %d (%d vs %d)", what, k, A->col_modes[k], B->row_modes[k]);
+}
+
+void validate_ranks(int d, const int* modes, const int* ranks)
+{
+  if (d < 0) fail(TTA_ERR_ARG, "negative core count");
+  if (d == 0) return;
+  if (!modes || !ranks) fail(TTA_ERR_ARG, "null modes/ranks");
+  if (ranks[0] != 1 || ranks[d] != 1)
+    fail(TTA_ERR_SHAPE, "boundary ranks must be 1 (r_0=%d, r_d=%d)", ranks[0], ranks[d]);
+  for (int k = 0; k < d; ++k)
+  {
+    if (modes[k] <= 0) fail(TTA_ERR_SHAPE, "mode size %d at core %d is not positive", modes[k], k);
+    if (ranks[k] <= 0) fail(TTA_ERR_SHAPE, "rank %d at bond %d is not positive", ranks[k], k);
+  }
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// extern "C" entry points.
+// ----------------------------------------------------------------------------
+#define TTA_GUARD_BEGIN \
+  try               \
+  {
+#define TTA_GUARD_END(ctx)                                  \
+  }                                                         \
+  catch (const tta::error& e)                               \
+  {                                                         \
+    if (ctx) (ctx)->last_error = e.what();                  \
+    return e.code;                                          \
+  }                                                         \
+  catch (const std::exception& e)                           \
+  {                                                         \
+    if (ctx) (ctx)->last_error = e.what();                  \
+    return TTA_ERR_HIP;                                     \
+  }                                                         \
+  return TTA_OK;
+
+extern "C"
+{
+
+int tta_ctx_create(int device, tta_ctx** out)
+{
+  if (!out) return TTA_ERR_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return TTA_ERR_HIP;
+  if (hipSetDevice(device) != hipSuccess) return TTA_ERR_HIP;
+  auto* c = new tta_ctx();
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  {
+    delete c;
+    return TTA_ERR_HIP;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&c->h_int), 64, hipHostMallocDefault) != hipSuccess)
+  {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TTA_ERR_HIP;
+  }
+  c->h_dbl = reinterpret_cast<double*>(c->h_int + 8);
+  *out = c;
+  return TTA_OK;
+}
+
+int tta_ctx_destroy(tta_ctx* c)
+{
+  if (!c) return TTA_ERR_ARG;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& w : c->ws)
+    if (w.p) c->mem.put(w.p);
+  (void)hipHostFree(c->h_int);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TTA_OK;
+}
+
+const char* tta_last_error(const tta_ctx* c) { return c ? c->last_error.c_str() : "null context"; }
+
+int tta_synchronize(tta_ctx* c)
+{
+  if (!c) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+void* tta_stream(tta_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+int tta_tt_create(tta_ctx* c, int d, const int* modes, const int* ranks, tta_tt** out)
+{
+  if (!c || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  validate_ranks(d, modes, ranks);
+  std::vector<int> md(modes, modes + d), rk(ranks, ranks + d + 1);
+  if (d == 0) rk.assign(1, 1);
+  *out = new_tt(c, md, rk, true);
+  TTA_GUARD_END(c)
+}
+
+int tta_ttm_create(tta_ctx* c, int d, const int* row_modes, const int* col_modes, const int* ranks, tta_tt** out)
+{
+  if (!c || !out || (d > 0 && (!row_modes || !col_modes))) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  std::vector<int> md(d);
+  for (int k = 0; k < d; ++k)
+  {
+    if (row_modes[k] <= 0 || col_modes[k] <= 0) fail(TTA_ERR_SHAPE, "mode sizes must be positive");
+    md[k] = row_modes[k] * col_modes[k];
+  }
+  validate_ranks(d, md.data(), ranks);
+  std::vector<int> rk(ranks, ranks + d + 1);
+  if (d == 0) rk.assign(1, 1);
+  tta_tt* t = new_tt(c, md, rk, true);
+  t->row_modes.assign(row_modes, row_modes + d);
+  t->col_modes.assign(col_modes, col_modes + d);
+  t->matrix = true;
+  *out = t;
+  TTA_GUARD_END(c)
+}
+
+int tta_tt_free(tta_ctx* c, tta_tt* t)
+{
+  if (!c) return TTA_ERR_ARG;
+  free_tt(c, t);
+  return TTA_OK;
+}
+
+int tta_tt_dims(const tta_tt* t, int* d, int* is_matrix)
+{
+  if (!t) return TTA_ERR_ARG;
+  if (d) *d = t->d();
+  if (is_matrix) *is_matrix = t->is_matrix() ? 1 : 0;
+  return TTA_OK;
+}
+
+int tta_tt_shape(const tta_tt* t, int* modes, int* row_modes, int* col_modes, int* ranks)
+{
+  if (!t) return TTA_ERR_ARG;
+  const int d = t->d();
+  for (int k = 0; k < d; ++k)
+  {
+    if (modes) modes[k] = t->modes[k];
+    if (row_modes && t->is_matrix()) row_modes[k] = t->row_modes[k];
+    if (col_modes && t->is_matrix()) col_modes[k] = t->col_modes[k];
+  }
+  if (ranks)
+    for (int k = 0; k <= d; ++k) ranks[k] = t->ranks[k];
+  return TTA_OK;
+}
+
+long long tta_tt_numel(const tta_tt* t)
+{
+  if (!t) return -1;
+  long long s = 0;
+  for (int k = 0; k < t->d(); ++k) s += (long long)core_numel(t, k);
+  return s;
+}
+
+int tta_tt_upload(tta_ctx* c, tta_tt* t, const double* host)
+{
+  if (!c || !t || (!host && t->d() > 0)) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  size_t off = 0;
+  for (int k = 0; k < t->d(); ++k)
+  {
+    const size_t n = core_numel(t, k);
+    hip_check(hipMemcpyAsync(t->cores[k], host + off, n * 8, hipMemcpyHostToDevice, c->stream), "upload");
+    off += n;
+  }
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_tt_download(tta_ctx* c, const tta_tt* t, double* host)
+{
+  if (!c || !t || (!host && t->d() > 0)) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  size_t off = 0;
+  for (int k = 0; k < t->d(); ++k)
+  {
+    const size_t n = core_numel(t, k);
+    hip_check(hipMemcpyAsync(host + off, t->cores[k], n * 8, hipMemcpyDeviceToHost, c->stream), "download");
+    off += n;
+  }
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_tt_core_ptr(const tta_tt* t, int k, double** dev)
+{
+  if (!t || !dev || k < 0 || k >= t->d()) return TTA_ERR_ARG;
+  *dev = t->cores[k];
+  return TTA_OK;
+}
+
+int tta_tt_copy(tta_ctx* c, const tta_tt* t, tta_tt** out)
+{
+  if (!c || !t || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  tta_tt* r = copy_tt(c, t);
+  r->matrix = t->matrix;
+  *out = r;
+  TTA_GUARD_END(c)
+}
+
+int tta_round(tta_ctx* c, const tta_tt* a, double eps, int max_rank, int method, int gauge, tta_tt** out)
+{
+  if (!c || !a || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  if (method == TTA_METHOD_SVD_STREAMING || method == TTA_METHOD_SVD_NAIVE)
+    fail(TTA_ERR_METHOD,
+         "round: round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round "
+         "respectively.");
+  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
+    fail(TTA_ERR_UNSUPPORTED, "round: ALS / DMRG engines are not part of this library");
+  if (method != TTA_METHOD_SVD) fail(TTA_ERR_METHOD, "round: unknown method %d", method);
+  if (std::isnan(eps)) fail(TTA_ERR_ARG, "round: eps is NaN");
+  tta_tt* t = copy_tt(c, a);
+  t->matrix = a->matrix;
+  try
+  {
+    round_inplace(c, t, eps, max_rank, gauge);
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  catch (...)
+  {
+    free_tt(c, t);
+    throw;
+  }
+  *out = t;
+  TTA_GUARD_END(c)
+}
+
+int tta_right_orthogonalize(tta_ctx* c, tta_tt* a)
+{
+  if (!c || !a) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  right_orthogonalize_impl(c, a);
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_left_orthogonalize(tta_ctx* c, tta_tt* a)
+{
+  if (!c || !a) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  left_orthogonalize_impl(c, a);
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_matvec(tta_ctx* c, const tta_tt* A, const tta_tt* x, tta_tt** y)
+{
+  if (!c || !y) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_matvec(A, x, "matvec");
+  *y = matvec_impl(c, A, x);
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_matmat(tta_ctx* c, const tta_tt* A, const tta_tt* B, tta_tt** C)
+{
+  if (!c || !C) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_matmat(A, B, "matmat");
+  tta_tt* r = matmat_impl(c, A, B);
+  r->matrix = true;
+  *C = r;
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_matvec_round(tta_ctx* c, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method, tta_tt** y)
+{
+  if (!c || !y) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  if (method == TTA_METHOD_SVD_NAIVE) fail(TTA_ERR_METHOD, "matvec_round: round_method::svd_naive is matmat-only");
+  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
+    fail(TTA_ERR_UNSUPPORTED, "matvec_round: ALS / DMRG engines are not part of this library");
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_STREAMING)
+    fail(TTA_ERR_METHOD, "matvec_round: unknown round_method %d", method);
+  check_matvec(A, x, "matvec_round");
+  tta_tt* r = nullptr;
+  if (method == TTA_METHOD_SVD_STREAMING && A->d() > 1)
+  {
+    if (A->d() == 0) fail(TTA_ERR_ARG, "build_streaming_matvec: d=0 not supported");
+    r = streaming_impl(c, A, x, false, eps, max_rank);
+  }
+  else
+  {
+    r = matvec_impl(c, A, x);
+    try
+    {
+      round_inplace(c, r, eps, max_rank, TTA_GAUGE_NONE);
+    }
+    catch (...)
+    {
+      free_tt(c, r);
+      throw;
+    }
+  }
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *y = r;
+  TTA_GUARD_END(c)
+}
+
+int tta_matmat_round(tta_ctx* c, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method, tta_tt** C)
+{
+  if (!c || !C) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  if (method == TTA_METHOD_SVD_STREAMING)
+    fail(TTA_ERR_METHOD, "matmat_round: round_method::svd_streaming is matvec-only");
+  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
+    fail(TTA_ERR_UNSUPPORTED, "matmat_round: ALS / DMRG engines are not part of this library");
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_NAIVE)
+    fail(TTA_ERR_METHOD, "matmat_round: unknown round_method %d", method);
+  check_matmat(A, B, "matmat_round");
+  tta_tt* r = nullptr;
+  if (method == TTA_METHOD_SVD && A->d() > 1)
+  {
+    r = streaming_impl(c, A, B, true, eps, max_rank);
+  }
+  else
+  {
+    r = matmat_impl(c, A, B);
+    try
+    {
+      round_inplace(c, r, eps, max_rank, TTA_GAUGE_NONE);
+    }
+    catch (...)
+    {
+      free_tt(c, r);
+      throw;
+    }
+  }
+  r->matrix = true;
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *C = r;
+  TTA_GUARD_END(c)
+}
+
+int tta_inner(tta_ctx* c, const tta_tt* a, const tta_tt* b, double* out)
+{
+  if (!c || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_same_shape(a, b, "inner");
+  *out = inner_impl(c, a, b);
+  TTA_GUARD_END(c)
+}
+
+int tta_norm(tta_ctx* c, const tta_tt* a, double* out)
+{
+  if (!c || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_tt(a, "norm");
+  const double s = inner_impl(c, a, a);
+  *out = std::sqrt(std::max(s, 0.0));
+  TTA_GUARD_END(c)
+}
+
+int tta_axpby(tta_ctx* c, double alpha, const tta_tt* a, double beta, const tta_tt* b, tta_tt** out)
+{
+  if (!c || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_same_shape(a, b, "axpby");
+  const int d = a->d();
+  std::vector<int> ranks(d + 1, 1);
+  for (int k = 1; k < d; ++k) ranks[k] = a->ranks[k] + b->ranks[k];
+  tta_tt* r = new_tt(c, a->modes, ranks, true);
+  r->row_modes = a->row_modes;
+  r->col_modes = a->col_modes;
+  r->matrix = a->matrix;
+  // Block-diagonal concatenation (include/core/detail/block_concat.hpp:40-156)
+  // as scaled 2-D slab copies: the a-slab at (row 0, col 0), the b-slab at
+  // (left rank raL, right rank raR) except on the head (left 0) and the
+  // tail (right 0); alpha / beta folded into the head core only.
+  if (d == 1)
+  {
+    // Single core: elementwise alpha*a + beta*b (tt_ops.hpp:85-101).
+    const int n = a->modes[0];
+    launch_copy2d(a->cores[0], n, r->cores[0], n, 1, n, alpha, 0.0, c->stream);
+    launch_copy2d(b->cores[0], n, r->cores[0], n, 1, n, beta, 1.0, c->stream);
+  }
+  for (int k = 0; k < d && d > 1; ++k)
+  {
+    const int n = a->modes[k];
+    const bool head = k == 0, tail = k == d - 1;
+    const int rR = ranks[k + 1];
+    for (int which = 0; which < 2; ++which)
+    {
+      const tta_tt* s = which == 0 ? a : b;
+      const int sL = s->ranks[k], sR = s->ranks[k + 1];
+      const int row_off = which == 0 ? 0 : (head ? 0 : a->ranks[k]);
+      const int col_off = which == 0 ? 0 : (tail ? 0 : a->ranks[k + 1]);
+      const double f = head ? (which == 0 ? alpha : beta) : 1.0;
+      // Tail core: both slabs share column 0, so the b-slab accumulates.
+      const double keep = (tail && which == 1) ? 1.0 : 0.0;
+      launch_copy2d(s->cores[k], sR, r->cores[k] + ((size_t)row_off * n) * rR + col_off, rR, sL * n, sR, f, keep,
+                    c->stream);
+    }
+  }
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *out = r;
+  TTA_GUARD_END(c)
+}
+
+int tta_scale(tta_ctx* c, const tta_tt* a, double alpha, tta_tt** out)
+{
+  if (!c || !a || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  tta_tt* r = copy_tt(c, a);
+  r->matrix = a->matrix;
+  if (r->d() > 0) launch_scale(r->cores[0], (long)core_numel(r, 0), alpha, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  *out = r;
+  TTA_GUARD_END(c)
+}
+
+}  // extern "C"

@@ -1,0 +1,310 @@
+// Device-side building blocks for TT rounding on gfx950 (CDNA4, wave64).
+//
+// Matrices handed to these routines are column-major with a leading
+// dimension unless stated otherwise.  A TT core (r, n, rp) stored row-major
+// is, byte for byte:
+//   * right_unfold  (r x n*rp) row-major  ==  (n*rp x r) column-major;
+//   * left_unfold   (r*n x rp) row-major  ==  (rp x r*n) column-major.
+// so the right-to-left QR sweep works on the core bytes directly.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+namespace tta
+{
+namespace dev
+{
+
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Block-wide sum; `red` must hold at least NT / 64 doubles of LDS.  Every
+// thread returns the total.  Contains two barriers.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* red)
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wid = threadIdx.x / kWave;
+  v = wave_sum(v);
+  __syncthreads();
+  if (lane == 0) red[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / kWave; ++w) t += red[w];
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Householder QR (unblocked, LAPACK dgeqr2 semantics) of a column-major
+// (m x nc) matrix A with leading dimension lda, by one workgroup of NT
+// threads.  On return the upper trapezoid of A holds R, the part below the
+// diagonal holds the essential Householder vectors (v[0] = 1 implicit) and
+// tau[j] the scalar factors, j < k = min(m, nc).  Sign convention of dlarfg
+// (and Eigen's makeHouseholder): beta = -sign(alpha) * ||x||.
+// `vbuf` is LDS of at least m doubles; `red` LDS of NT/64 doubles.
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ void geqr2(double* __restrict__ A, int lda, int m, int nc, double* __restrict__ tau,
+                      double* vbuf, double* red)
+{
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  constexpr int NW = NT / kWave;
+  const int k = m < nc ? m : nc;
+  for (int j = 0; j < k; ++j)
+  {
+    double* colj = A + (size_t)j * lda + j;
+    const int len = m - j;
+    double acc = 0.0;
+    for (int i = tid; i < len; i += NT)
+    {
+      const double x = colj[i];
+      vbuf[i] = x;
+      if (i > 0) acc += x * x;
+    }
+    const double tail = block_sum<NT>(acc, red);  // barrier also publishes vbuf
+    const double alpha = vbuf[0];
+    double tj, scal, beta;
+    if (tail <= DBL_MIN)
+    {
+      tj = 0.0;
+      scal = 0.0;
+      beta = alpha;
+    }
+    else
+    {
+      beta = sqrt(alpha * alpha + tail);
+      if (alpha >= 0.0) beta = -beta;
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    __syncthreads();
+    for (int i = tid; i < len; i += NT)
+    {
+      const double v = (i == 0) ? 1.0 : vbuf[i] * scal;
+      vbuf[i] = v;
+      colj[i] = (i == 0) ? beta : v;
+    }
+    if (tid == 0) tau[j] = tj;
+    __syncthreads();
+    if (tj != 0.0)
+    {
+      // Trailing update: one wave per column.
+      for (int c = j + 1 + wid; c < nc; c += NW)
+      {
+        double* y = A + (size_t)c * lda + j;
+        double dot = 0.0;
+        for (int i = lane; i < len; i += kWave) dot += vbuf[i] * y[i];
+        dot = wave_sum(dot) * tj;
+        for (int i = lane; i < len; i += kWave) y[i] -= dot * vbuf[i];
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Form the thin Q (m x k, column-major, leading dimension ldq) of a QR
+// factorisation stored by geqr2 (LAPACK dorg2r: backward accumulation of the
+// reflectors on the identity).  `vbuf` LDS of m doubles.
+template <int NT>
+__device__ void org2r(const double* __restrict__ A, int lda, int m, int k, const double* __restrict__ tau,
+                      double* __restrict__ Q, int ldq, double* vbuf)
+{
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  constexpr int NW = NT / kWave;
+  for (int c = 0; c < k; ++c)
+    for (int i = tid; i < m; i += NT) Q[(size_t)c * ldq + i] = (i == c) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int j = k - 1; j >= 0; --j)
+  {
+    const double tj = tau[j];
+    if (tj == 0.0) continue;
+    const int len = m - j;
+    for (int i = tid; i < len; i += NT) vbuf[i] = (i == 0) ? 1.0 : A[(size_t)j * lda + j + i];
+    __syncthreads();
+    for (int c = j + wid; c < k; c += NW)
+    {
+      double* y = Q + (size_t)c * ldq + j;
+      double dot = 0.0;
+      for (int i = lane; i < len; i += kWave) dot += vbuf[i] * y[i];
+      dot = wave_sum(dot) * tj;
+      for (int i = lane; i < len; i += kWave) y[i] -= dot * vbuf[i];
+    }
+    __syncthreads();
+  }
+}
+
+// Circle-method round-robin pairing of qq (even) players: round s in
+// [0, qq-1), pair p in [0, qq/2).  Every unordered pair meets exactly once
+// per qq-1 rounds and the pairs of one round are disjoint.
+__device__ __forceinline__ void rr_pair(int s, int p, int qq, int& i, int& j)
+{
+  const int mdl = qq - 1;
+  if (p == 0)
+  {
+    i = mdl;
+    j = s;
+  }
+  else
+  {
+    i = (s + p) % mdl;
+    j = (s - p + mdl) % mdl;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One-sided (Hestenes) Jacobi on the q columns of X (p x q, column-major,
+// leading dimension ldx), accumulating the rotations into J (q x q,
+// column-major, ldj): on return X_in * J = X_out with mutually orthogonal
+// columns and J orthogonal.  Parallel ordering: each round applies q/2
+// disjoint rotations, one wave per pair.  Convergence: a pair is rotated only
+// while |x_i . x_j| > tol * ||x_i|| ||x_j||, tol = max(1e-15, sqrt(p) * u);
+// stops after a sweep with no rotation or after max_sweeps.
+// Returns the number of sweeps run (to thread 0 via the return value).
+// ---------------------------------------------------------------------------
+template <int NT>
+__device__ int jacobi_sweeps(double* __restrict__ X, int ldx, int p, int q, double* __restrict__ J, int ldj,
+                             int max_sweeps, int* flag_lds)
+{
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  constexpr int NW = NT / kWave;
+  for (int c = 0; c < q; ++c)
+    for (int i = tid; i < q; i += NT) J[(size_t)c * ldj + i] = (i == c) ? 1.0 : 0.0;
+  const double tol = fmax(1.0e-15, sqrt((double)p) * 2.220446049250313e-16);
+  const int qq = (q + 1) & ~1;
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep)
+  {
+    if (tid == 0) *flag_lds = 0;
+    __syncthreads();
+    int rotated = 0;
+    for (int s = 0; s < qq - 1; ++s)
+    {
+      for (int pr = wid; pr < qq / 2; pr += NW)
+      {
+        int ci, cj;
+        rr_pair(s, pr, qq, ci, cj);
+        if (ci >= q || cj >= q) continue;
+        if (ci > cj)
+        {
+          const int t = ci;
+          ci = cj;
+          cj = t;
+        }
+        double* xi = X + (size_t)ci * ldx;
+        double* xj = X + (size_t)cj * ldx;
+        double a = 0.0, b = 0.0, g = 0.0;
+        for (int t = lane; t < p; t += kWave)
+        {
+          const double u = xi[t], v = xj[t];
+          a += u * u;
+          b += v * v;
+          g += u * v;
+        }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        g = wave_sum(g);
+        if (g == 0.0 || fabs(g) <= tol * sqrt(a * b)) continue;
+        rotated = 1;
+        const double zeta = (b - a) / (2.0 * g);
+        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+        const double cs = 1.0 / sqrt(1.0 + tt * tt);
+        const double sn = cs * tt;
+        for (int t = lane; t < p; t += kWave)
+        {
+          const double u = xi[t], v = xj[t];
+          xi[t] = cs * u - sn * v;
+          xj[t] = sn * u + cs * v;
+        }
+        double* ji = J + (size_t)ci * ldj;
+        double* jj = J + (size_t)cj * ldj;
+        for (int t = lane; t < q; t += kWave)
+        {
+          const double u = ji[t], v = jj[t];
+          ji[t] = cs * u - sn * v;
+          jj[t] = sn * u + cs * v;
+        }
+      }
+      __syncthreads();
+    }
+    if (rotated && lane == 0) atomicOr(flag_lds, 1);
+    __syncthreads();
+    const int any = *flag_lds;
+    __syncthreads();
+    if (!any) break;
+  }
+  return sweep;
+}
+
+// Norms of the q columns of W (p x q col-major), then a stable descending
+// order: perm[o] = index of the o-th largest, s[o] its norm.  `nrm` scratch
+// of q doubles (global or LDS).
+template <int NT>
+__device__ void column_norms_sorted(const double* __restrict__ W, int ldw, int p, int q, double* nrm,
+                                    double* __restrict__ s, int* __restrict__ perm)
+{
+  const int tid = threadIdx.x;
+  const int lane = tid & (kWave - 1);
+  const int wid = tid / kWave;
+  constexpr int NW = NT / kWave;
+  for (int c = wid; c < q; c += NW)
+  {
+    const double* w = W + (size_t)c * ldw;
+    double acc = 0.0;
+    for (int t = lane; t < p; t += kWave) acc += w[t] * w[t];
+    acc = wave_sum(acc);
+    if (lane == 0) nrm[c] = sqrt(acc);
+  }
+  __syncthreads();
+  for (int c = tid; c < q; c += NT)
+  {
+    const double v = nrm[c];
+    int rank = 0;
+    for (int o = 0; o < q; ++o)
+    {
+      const double u = nrm[o];
+      rank += (u > v) || (u == v && o < c);
+    }
+    perm[rank] = c;
+    s[rank] = v;
+  }
+  __syncthreads();
+}
+
+// truncate_eps_rank: sequential tail sum from the smallest value, strict '>'
+// stops; at least 1; capped by max_rank.  Thread 0 only.
+__device__ __forceinline__ int truncate_rank(const double* s, int n, double delta, int max_rank)
+{
+  if (n == 0) return 1;
+  const double thresh = delta * delta;
+  double tail2 = 0.0;
+  int r_new = n;
+  for (int i = n - 1; i >= 0; --i)
+  {
+    const double next = tail2 + s[i] * s[i];
+    if (next > thresh) break;
+    tail2 = next;
+    r_new = i;
+  }
+  if (r_new < 1) r_new = 1;
+  if (max_rank > 0 && r_new > max_rank) r_new = max_rank;
+  return r_new;
+}
+
+}  // namespace dev
+}  // namespace tta

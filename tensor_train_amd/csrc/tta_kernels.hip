@@ -1,0 +1,458 @@
+// HIP kernels (gfx950) for the FP64 plain path of TT rounding and MPO
+// apply-and-round.  Launch wrappers are declared in tta_kernels.hpp.
+#include "tta_device.hpp"
+#include "tta_kernels.hpp"
+
+namespace tta
+{
+using namespace dev;
+
+// ---------------------------------------------------------------------------
+// FP64 GEMM with arbitrary element strides:
+//   C(i,j) = alpha * sum_k A(i,k) B(k,j) + beta * C(i,j)
+// A(i,k) = A[i*sar + k*sac], B(k,j) = B[k*sbr + j*sbc], C(i,j) = C[i*scr + j*scc].
+// 64x64 output tile per 256-thread workgroup, K staged through LDS in 16-deep
+// slabs; each thread owns a 4x4 micro-tile strided by 16.  blockIdx.z walks a
+// batch with pointer strides bsa/bsb/bsc.
+// ---------------------------------------------------------------------------
+namespace
+{
+constexpr int GT = 64, GK = 16;
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
+{
+  __shared__ double As[GK][GT + 1];
+  __shared__ double Bs[GK][GT + 1];
+  const int tid = threadIdx.x;
+  const int tx = tid & 15, ty = tid >> 4;
+  const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
+  const double* A = g.A + (size_t)blockIdx.z * g.bsa;
+  const double* B = g.B + (size_t)blockIdx.z * g.bsb;
+  double* C = g.C + (size_t)blockIdx.z * g.bsc;
+  const bool a_kc = (g.sac == 1);  // A contiguous along k
+  const bool b_jc = (g.sbc == 1);  // B contiguous along j
+  double acc[4][4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+  for (int k0 = 0; k0 < g.K; k0 += GK)
+  {
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+    {
+      const int idx = tid + 256 * l;
+      int ii, kk;
+      if (a_kc)
+      {
+        ii = idx >> 4;
+        kk = idx & 15;
+      }
+      else
+      {
+        ii = idx & 63;
+        kk = idx >> 6;
+      }
+      const int gi = i0 + ii, gk = k0 + kk;
+      As[kk][ii] = (gi < g.M && gk < g.K) ? A[(size_t)gi * g.sar + (size_t)gk * g.sac] : 0.0;
+      int jj;
+      if (b_jc)
+      {
+        kk = idx >> 6;
+        jj = idx & 63;
+      }
+      else
+      {
+        jj = idx >> 4;
+        kk = idx & 15;
+      }
+      const int gj = j0 + jj, gk2 = k0 + kk;
+      Bs[kk][jj] = (gj < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sbr + (size_t)gj * g.sbc] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < GK; ++kk)
+    {
+      double a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = As[kk][ty + 16 * u];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) b[v] = Bs[kk][tx + 16 * v];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+  {
+    const int gi = i0 + ty + 16 * u;
+    if (gi >= g.M) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v)
+    {
+      const int gj = j0 + tx + 16 * v;
+      if (gj >= g.N) continue;
+      double* c = C + (size_t)gi * g.scr + (size_t)gj * g.scc;
+      *c = (g.beta == 0.0) ? g.alpha * acc[u][v] : g.alpha * acc[u][v] + g.beta * (*c);
+    }
+  }
+}
+
+// Householder QR (+ optional thin Q) of one column-major matrix per block.
+template <int NT>
+__global__ __launch_bounds__(NT) void qr_kernel(qr_desc q)
+{
+  extern __shared__ double lds[];
+  double* red = lds;
+  double* vbuf = lds + NT / kWave;
+  const size_t b = blockIdx.x;
+  double* A = q.A + b * q.bsA;
+  double* tau = q.tau + b * q.bstau;
+  geqr2<NT>(A, q.lda, q.m, q.n, tau, vbuf, red);
+  if (q.Q)
+  {
+    const int k = q.m < q.n ? q.m : q.n;
+    org2r<NT>(A, q.lda, q.m, k, tau, q.Q + b * q.bsQ, q.ldq, vbuf);
+  }
+}
+
+// One-sided Jacobi + sorted column norms (+ optional truncation rank).
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi_kernel(jacobi_desc jd)
+{
+  __shared__ int flag;
+  extern __shared__ double nrm[];
+  const size_t b = blockIdx.x;
+  double* X = jd.X + b * jd.bsX;
+  double* J = jd.J + b * jd.bsJ;
+  const int sweeps = jacobi_sweeps<NT>(X, jd.ldx, jd.p, jd.q, J, jd.ldj, jd.max_sweeps, &flag);
+  __syncthreads();
+  column_norms_sorted<NT>(X, jd.ldx, jd.p, jd.q, nrm, jd.s + b * jd.bss, jd.perm + b * jd.bss);
+  if (threadIdx.x == 0)
+  {
+    if (jd.sweeps) jd.sweeps[b] = sweeps;
+    if (jd.rank_out)
+    {
+      const double delta = jd.delta_ptr ? *jd.delta_ptr : jd.delta;
+      jd.rank_out[b] = truncate_rank(jd.s + b * jd.bss, jd.q, delta, jd.max_rank);
+    }
+  }
+}
+
+// dst[i*drs + c*dcs] = (i <= c) ? A[c*lda + i] : 0   for i < kr, c < nc.
+__global__ void extract_upper_kernel(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs)
+{
+  const long total = (long)kr * nc;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
+  {
+    const int c = (int)(t / kr), i = (int)(t % kr);
+    dst[i * drs + c * dcs] = (i <= c) ? A[(size_t)c * lda + i] : 0.0;
+  }
+}
+
+// dst[t*drs + o*dcs] = src[t + perm[o]*lds] for t < rows, o < ncols.
+__global__ void gather_cols_kernel(const double* src, int lds_, int rows, const int* perm, int ncols, double* dst,
+                                   long drs, long dcs)
+{
+  const long total = (long)rows * ncols;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
+  {
+    const int o = (int)(x / rows), t = (int)(x % rows);
+    dst[t * drs + o * dcs] = src[t + (size_t)perm[o] * lds_];
+  }
+}
+
+// 2-D transpose through a 32x33 LDS tile: out[j*ldo + i] = in[i*ldi + j]
+// for i < rows, j < cols.
+__global__ __launch_bounds__(256) void transpose_kernel(const double* in, int ldi, int rows, int cols, double* out,
+                                                        int ldo)
+{
+  __shared__ double tile[32][33];
+  const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int i = bi + r, j = bj + tx;
+    if (i < rows && j < cols) tile[r][tx] = in[(size_t)i * ldi + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int j = bj + r, i = bi + tx;
+    if (i < rows && j < cols) out[(size_t)j * ldo + i] = tile[tx][r];
+  }
+}
+
+// Generic strided gather: dst is contiguous over up to 6 dims (sizes in
+// `n`, row-major); src offset = sum idx_d * ss[d].
+__global__ void permute_kernel(permute_desc p)
+{
+  long total = 1;
+  for (int d = 0; d < 6; ++d) total *= p.n[d];
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
+  {
+    long rem = x, off = 0;
+    for (int d = 5; d >= 0; --d)
+    {
+      const long i = rem % p.n[d];
+      rem /= p.n[d];
+      off += i * p.ss[d];
+    }
+    p.dst[x] = p.src[off];
+  }
+}
+
+// y(a*s+ax, i, b*sp+bx) = sum_j A(a,i,j,b) x(ax,j,bx)   (one thread per output).
+__global__ void matvec_core_kernel(const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y,
+                                   int r, int m, int n, int rp, int s, int sp)
+{
+  const long total = (long)r * s * m * rp * sp;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    long t = o;
+    const int bx = (int)(t % sp);
+    t /= sp;
+    const int b = (int)(t % rp);
+    t /= rp;
+    const int i = (int)(t % m);
+    t /= m;
+    const int ax = (int)(t % s);
+    const int a = (int)(t / s);
+    const double* Ar = A + (((size_t)a * m + i) * n) * rp + b;
+    const double* xr = x + ((size_t)ax * n) * sp + bx;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc = fma(Ar[(size_t)j * rp], xr[(size_t)j * sp], acc);
+    y[o] = acc;
+  }
+}
+
+// C(a*s+ax, i, j, b*sp+bx) = sum_q A(a,i,q,b) B(ax,q,j,bx).
+__global__ void matmat_core_kernel(const double* __restrict__ A, const double* __restrict__ B, double* __restrict__ C,
+                                   int r, int m, int p, int rp, int s, int n, int sp)
+{
+  const long total = (long)r * s * m * n * rp * sp;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    long t = o;
+    const int bx = (int)(t % sp);
+    t /= sp;
+    const int b = (int)(t % rp);
+    t /= rp;
+    const int j = (int)(t % n);
+    t /= n;
+    const int i = (int)(t % m);
+    t /= m;
+    const int ax = (int)(t % s);
+    const int a = (int)(t / s);
+    const double* Ar = A + (((size_t)a * m + i) * p) * rp + b;
+    const double* Br = B + (((size_t)ax * p) * n + j) * sp + bx;
+    double acc = 0.0;
+    for (int q = 0; q < p; ++q) acc = fma(Ar[(size_t)q * rp], Br[(size_t)q * n * sp], acc);
+    C[o] = acc;
+  }
+}
+
+// Sum of squares, two-level: per-block partials into `part`.
+__global__ __launch_bounds__(256) void sumsq_kernel(const double* x, long n, double* part)
+{
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (long t = blockIdx.x * 256L + threadIdx.x; t < n; t += (long)gridDim.x * 256L) acc = fma(x[t], x[t], acc);
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+__global__ __launch_bounds__(256) void finish_sumsq_kernel(const double* part, int np, double* out, double eps,
+                                                           double inv_sqrt_dm1, double* delta_out)
+{
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < np; t += 256) acc += part[t];
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0)
+  {
+    const double nrm = sqrt(acc);
+    out[0] = nrm;
+    if (delta_out) delta_out[0] = eps * nrm * inv_sqrt_dm1;
+  }
+}
+
+// Column norms of a row-major (rows x cols) matrix: one wave per column.
+__global__ __launch_bounds__(256) void col_norms_kernel(const double* M, int rows, int cols, double* out)
+{
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= cols) return;
+  double acc = 0.0;
+  for (int i = lane; i < rows; i += 64) acc = fma(M[(size_t)i * cols + c], M[(size_t)i * cols + c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) out[c] = sqrt(acc);
+}
+
+// Geometric mean of the nonzero norms (single block): out = exp(mean(log)).
+__global__ __launch_bounds__(256) void geo_mean_kernel(const double* v, int n, double* out)
+{
+  __shared__ double red[4];
+  double s = 0.0, c = 0.0;
+  for (int t = threadIdx.x; t < n; t += 256)
+    if (v[t] > 0.0)
+    {
+      s += log(v[t]);
+      c += 1.0;
+    }
+  s = block_sum<256>(s, red);
+  c = block_sum<256>(c, red);
+  if (threadIdx.x == 0) out[0] = c > 0.0 ? exp(s / c) : 0.0;
+}
+
+// Balancing of one core (r, n, rp): x(a,i,b) *= f_right(b) / f_left(a), with
+// f(c) = geo / norm(c) for norm(c) > 0 and 1 otherwise; a null norm array
+// means all factors are 1.
+__global__ void balance_kernel(double* x, int r, int n, int rp, const double* nl, const double* nr,
+                               const double* geo)
+{
+  const long total = (long)r * n * rp;
+  const double g = *geo;
+  if (g == 0.0) return;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    const int b = (int)(o % rp);
+    const int a = (int)(o / ((long)n * rp));
+    double f = 1.0;
+    if (nr && nr[b] > 0.0) f *= g / nr[b];
+    if (nl && nl[a] > 0.0) f /= g / nl[a];
+    x[o] *= f;
+  }
+}
+
+// dst[i*ldd + j] = f * src[i*lds + j] + keep * dst[i*ldd + j].
+__global__ void copy2d_kernel(const double* src, int lds_, double* dst, int ldd, int rows, int cols, double f,
+                              double keep)
+{
+  const long total = (long)rows * cols;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
+  {
+    const int i = (int)(t / cols), j = (int)(t % cols);
+    double* o = dst + (size_t)i * ldd + j;
+    const double v = f * src[(size_t)i * lds_ + j];
+    *o = keep == 0.0 ? v : v + keep * (*o);
+  }
+}
+
+__global__ void scale_kernel(double* x, long n, double alpha)
+{
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= alpha;
+}
+
+int grid_for(long n, int bs = 256)
+{
+  long g = (n + bs - 1) / bs;
+  if (g > 65535L * 4) g = 65535L * 4;
+  return (int)(g < 1 ? 1 : g);
+}
+}  // namespace
+
+// ------------------------------- launchers --------------------------------
+void launch_gemm(const gemm_desc& g, int batch, hipStream_t st)
+{
+  if (g.M <= 0 || g.N <= 0 || batch <= 0) return;
+  dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT, batch);
+  hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, st, g);
+}
+
+void launch_qr(const qr_desc& q, int batch, hipStream_t st)
+{
+  constexpr int NT = 256;
+  const size_t lds = sizeof(double) * (NT / kWave + (size_t)q.m);
+  hipLaunchKernelGGL(qr_kernel<NT>, dim3(batch), dim3(NT), lds, st, q);
+}
+
+void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st)
+{
+  constexpr int NT = 1024;
+  const size_t lds = sizeof(double) * (size_t)j.q;
+  hipLaunchKernelGGL(jacobi_kernel<NT>, dim3(batch), dim3(NT), lds, st, j);
+}
+
+void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st)
+{
+  hipLaunchKernelGGL(extract_upper_kernel, dim3(grid_for((long)kr * nc)), dim3(256), 0, st, A, lda, kr, nc, dst, drs,
+                     dcs);
+}
+
+void launch_gather_cols(const double* src, int lds_, int rows, const int* perm, int ncols, double* dst, long drs,
+                        long dcs, hipStream_t st)
+{
+  hipLaunchKernelGGL(gather_cols_kernel, dim3(grid_for((long)rows * ncols)), dim3(256), 0, st, src, lds_, rows, perm,
+                     ncols, dst, drs, dcs);
+}
+
+void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st)
+{
+  dim3 grid((cols + 31) / 32, (rows + 31) / 32);
+  hipLaunchKernelGGL(transpose_kernel, grid, dim3(256), 0, st, in, ldi, rows, cols, out, ldo);
+}
+
+void launch_permute(const permute_desc& p, hipStream_t st)
+{
+  long total = 1;
+  for (int d = 0; d < 6; ++d) total *= p.n[d];
+  hipLaunchKernelGGL(permute_kernel, dim3(grid_for(total)), dim3(256), 0, st, p);
+}
+
+void launch_matvec_core(const double* A, const double* x, double* y, int r, int m, int n, int rp, int s, int sp,
+                        hipStream_t st)
+{
+  const long total = (long)r * s * m * rp * sp;
+  hipLaunchKernelGGL(matvec_core_kernel, dim3(grid_for(total)), dim3(256), 0, st, A, x, y, r, m, n, rp, s, sp);
+}
+
+void launch_matmat_core(const double* A, const double* B, double* C, int r, int m, int p, int rp, int s, int n,
+                        int sp, hipStream_t st)
+{
+  const long total = (long)r * s * m * n * rp * sp;
+  hipLaunchKernelGGL(matmat_core_kernel, dim3(grid_for(total)), dim3(256), 0, st, A, B, C, r, m, p, rp, s, n, sp);
+}
+
+void launch_frob_norm(const double* x, long n, double* part, int nparts, double* out, double eps, double inv_sqrt_dm1,
+                      double* delta_out, hipStream_t st)
+{
+  int g = (int)((n + 255) / 256);
+  if (g > nparts) g = nparts;
+  if (g < 1) g = 1;
+  hipLaunchKernelGGL(sumsq_kernel, dim3(g), dim3(256), 0, st, x, n, part);
+  hipLaunchKernelGGL(finish_sumsq_kernel, dim3(1), dim3(256), 0, st, part, g, out, eps, inv_sqrt_dm1, delta_out);
+}
+
+void launch_col_norms(const double* M, int rows, int cols, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(col_norms_kernel, dim3((cols + 3) / 4), dim3(256), 0, st, M, rows, cols, out);
+}
+
+void launch_geo_mean(const double* v, int n, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(geo_mean_kernel, dim3(1), dim3(256), 0, st, v, n, out);
+}
+
+void launch_balance(double* x, int r, int n, int rp, const double* nl, const double* nr, const double* geo,
+                    hipStream_t st)
+{
+  hipLaunchKernelGGL(balance_kernel, dim3(grid_for((long)r * n * rp)), dim3(256), 0, st, x, r, n, rp, nl, nr, geo);
+}
+
+void launch_copy2d(const double* src, int lds_, double* dst, int ldd, int rows, int cols, double f, double keep,
+                   hipStream_t st)
+{
+  hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for((long)rows * cols)), dim3(256), 0, st, src, lds_, dst, ldd, rows,
+                     cols, f, keep);
+}
+
+void launch_scale(double* x, long n, double alpha, hipStream_t st)
+{
+  hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, n, alpha);
+}
+
+}  // namespace tta

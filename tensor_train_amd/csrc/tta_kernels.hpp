@@ -1,0 +1,84 @@
+// Launch wrappers for the FP64 kernels in tta_kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace tta
+{
+
+// C(i,j) = alpha * sum_k A(i,k) B(k,j) + beta * C(i,j), element strides per
+// operand, batch strides bs* (elements) over blockIdx.z.
+struct gemm_desc
+{
+  int M = 0, N = 0, K = 0;
+  const double* A = nullptr;
+  long sar = 0, sac = 0;
+  const double* B = nullptr;
+  long sbr = 0, sbc = 0;
+  double* C = nullptr;
+  long scr = 0, scc = 0;
+  double alpha = 1.0, beta = 0.0;
+  long bsa = 0, bsb = 0, bsc = 0;
+};
+
+// Householder QR of `batch` column-major (m x n) matrices; thin Q optional.
+struct qr_desc
+{
+  double* A = nullptr;
+  int lda = 0, m = 0, n = 0;
+  double* tau = nullptr;
+  double* Q = nullptr;  // null: do not form Q
+  int ldq = 0;
+  long bsA = 0, bstau = 0, bsQ = 0;
+};
+
+// One-sided Jacobi on X (p x q col-major), J (q x q); sorted norms into s,
+// order into perm; optional truncation rank (delta from delta_ptr if set).
+struct jacobi_desc
+{
+  double* X = nullptr;
+  int ldx = 0, p = 0, q = 0;
+  double* J = nullptr;
+  int ldj = 0;
+  double* s = nullptr;
+  int* perm = nullptr;
+  int* sweeps = nullptr;
+  int* rank_out = nullptr;
+  const double* delta_ptr = nullptr;
+  double delta = 0.0;
+  int max_rank = 0;
+  int max_sweeps = 60;
+  long bsX = 0, bsJ = 0, bss = 0;
+};
+
+struct permute_desc
+{
+  const double* src = nullptr;
+  double* dst = nullptr;
+  long n[6] = {1, 1, 1, 1, 1, 1};
+  long ss[6] = {0, 0, 0, 0, 0, 0};
+};
+
+void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
+void launch_qr(const qr_desc& q, int batch, hipStream_t st);
+void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
+void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
+void launch_gather_cols(const double* src, int lds, int rows, const int* perm, int ncols, double* dst, long drs,
+                        long dcs, hipStream_t st);
+void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st);
+void launch_permute(const permute_desc& p, hipStream_t st);
+void launch_matvec_core(const double* A, const double* x, double* y, int r, int m, int n, int rp, int s, int sp,
+                        hipStream_t st);
+void launch_matmat_core(const double* A, const double* B, double* C, int r, int m, int p, int rp, int s, int n,
+                        int sp, hipStream_t st);
+void launch_frob_norm(const double* x, long n, double* part, int nparts, double* out, double eps, double inv_sqrt_dm1,
+                      double* delta_out, hipStream_t st);
+void launch_col_norms(const double* M, int rows, int cols, double* out, hipStream_t st);
+void launch_geo_mean(const double* v, int n, double* out, hipStream_t st);
+void launch_balance(double* x, int r, int n, int rp, const double* nl, const double* nr, const double* geo,
+                    hipStream_t st);
+void launch_copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double f, double keep,
+                   hipStream_t st);
+void launch_scale(double* x, long n, double alpha, hipStream_t st);
+
+}  // namespace tta
