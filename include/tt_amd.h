@@ -123,6 +123,41 @@ int tta_axpby(tta_ctx* ctx, double alpha, const tta_tt* a, double beta, const tt
 /* scale(a, alpha) (tt_ops.hpp:35-50). */
 int tta_scale(tta_ctx* ctx, const tta_tt* a, double alpha, tta_tt** out);
 
+/* ---- seeded generators (inputs of the benchmark configurations) --------- */
+/* random(shape, max_rank, seed) (include/core/factory/tt_factory.hpp:141-207),
+ * on the host with std::mt19937_64 + std::normal_distribution<double>, drawn
+ * core by core in buffer order.  Host-only variant: ranks_out[d+1],
+ * numel_out and data_out may each be null (size query).  Needs no GPU. */
+int tta_random_host(int d, const int* modes, int max_rank, unsigned long long seed, int* ranks_out,
+                    double* data_out, long long* numel_out);
+int tta_random(tta_ctx* ctx, int d, const int* modes, int max_rank, unsigned long long seed, tta_tt** out);
+/* random(row_shape, col_shape, max_rank, seed) (tt_matrix_factory.hpp:133-191). */
+int tta_random_matrix(tta_ctx* ctx, int d, const int* row_modes, const int* col_modes, int max_rank,
+                      unsigned long long seed, tta_tt** out);
+/* zeros (value 0, tt_factory.hpp:31-52) / ones (value 1, :59-78): rank-1 TT
+ * with every core entry equal to value. */
+int tta_constant(tta_ctx* ctx, int d, const int* modes, double value, tta_tt** out);
+/* Multiply core k of t by alpha in place. */
+int tta_scale_core(tta_ctx* ctx, tta_tt* t, int k, double alpha);
+
+/* ---- kernel timing (HIP events on the context stream) -------------------- */
+enum tta_kernel_class
+{
+  TTA_K_QR = 0,     /* Householder QR + thin Q */
+  TTA_K_JACOBI = 1, /* one-sided Jacobi SVD + sort + truncation rank */
+  TTA_K_GEMM = 2,   /* FP64 contractions (absorb factors, U = Q J, inner) */
+  TTA_K_APPLY = 3,  /* MPO core contraction (matvec / matmat cores) */
+  TTA_K_OTHER = 4,  /* transposes, gathers, norms, balancing */
+  TTA_K_COUNT = 5
+};
+/* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
+int tta_prof_enable(tta_ctx* ctx, int on);
+/* Launch count, summed device milliseconds, and the summed algorithmic
+ * flops and minimum HBM bytes credited to those launches (textbook counts at
+ * each launch's shape, not hardware counters) for one kernel class. */
+int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* total_ms, double* flops,
+                  double* bytes);
+
 #ifdef __cplusplus
 }
 #endif

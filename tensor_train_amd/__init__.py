@@ -20,7 +20,8 @@ from ._capi import TTAError
 __all__ = [
     "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
     "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round", "inner",
-    "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError",
+    "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
+    "zeros", "ones", "random_host",
 ]
 
 # round_method / round_gauge (include/core/round/round.hpp:33-53).
@@ -66,6 +67,20 @@ class Context:
     @property
     def stream(self) -> int:
         return int(self.lib.tta_stream(self.h) or 0)
+
+    def prof_enable(self, on: bool = True):
+        """Per-launch HIP-event timing of the library's kernels (resets totals)."""
+        self.check(self.lib.tta_prof_enable(self.h, 1 if on else 0))
+
+    def prof_read(self, kernel_class: str):
+        """{launches, ms, flops, bytes} of one kernel class (qr, jacobi, gemm,
+        apply, other): summed device time and the algorithmic work credited
+        to those launches."""
+        n = ctypes.c_longlong()
+        ms, fl, by = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.check(self.lib.tta_prof_read(self.h, _capi.KERNEL_CLASSES[kernel_class], ctypes.byref(n),
+                                          ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)))
+        return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
 
 
 class RoundOptions:
@@ -400,3 +415,62 @@ def sub(a, b):
 
 def scale(a, alpha):
     return _out(a._ctx, a._ctx.lib.tta_scale, a._h, ctypes.c_double(alpha), cls=type(a))
+
+
+def random_host(shape, max_rank, seed):
+    """random(shape, max_rank, seed) cores on the host (tt_factory.hpp:141-207);
+    needs no GPU."""
+    L = _capi.lib()
+    d = len(shape)
+    md = np.ascontiguousarray(shape, np.int32)
+    rk = np.zeros(d + 1, np.int32)
+    n = ctypes.c_longlong()
+    ip = ctypes.POINTER(ctypes.c_int)
+    st = L.tta_random_host(d, md.ctypes.data_as(ip), max_rank, seed, rk.ctypes.data_as(ip), None, ctypes.byref(n))
+    if st != 0:
+        raise TTAError(st, "random: invalid shape")
+    data = np.zeros(max(n.value, 1))
+    L.tta_random_host(d, md.ctypes.data_as(ip), max_rank, seed, None,
+                      data.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), None)
+    out, off = [], 0
+    for k in range(d):
+        sz = int(rk[k] * md[k] * rk[k + 1])
+        out.append(data[off:off + sz].reshape(int(rk[k]), int(md[k]), int(rk[k + 1])))
+        off += sz
+    return out
+
+
+def random(shape, max_rank, seed, ctx: Context | None = None) -> TensorTrain:  # noqa: A001
+    ctx = ctx or Context.default()
+    md = np.ascontiguousarray(shape, np.int32)
+    return _out(ctx, ctx.lib.tta_random, len(shape), md.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), max_rank,
+                ctypes.c_ulonglong(seed))
+
+
+def random_matrix(row_shape, col_shape, max_rank, seed, ctx: Context | None = None) -> TensorTrainMatrix:
+    ctx = ctx or Context.default()
+    rm = np.ascontiguousarray(row_shape, np.int32)
+    cm = np.ascontiguousarray(col_shape, np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    return _out(ctx, ctx.lib.tta_random_matrix, len(rm), rm.ctypes.data_as(ip), cm.ctypes.data_as(ip), max_rank,
+                ctypes.c_ulonglong(seed), cls=TensorTrainMatrix)
+
+
+def _constant(shape, value, ctx):
+    ctx = ctx or Context.default()
+    md = np.ascontiguousarray(shape, np.int32)
+    return _out(ctx, ctx.lib.tta_constant, len(shape), md.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                ctypes.c_double(value))
+
+
+def zeros(shape, ctx: Context | None = None) -> TensorTrain:
+    return _constant(shape, 0.0, ctx)
+
+
+def ones(shape, ctx: Context | None = None) -> TensorTrain:
+    return _constant(shape, 1.0, ctx)
+
+
+def scale_core(t, k: int, alpha: float) -> None:
+    """Multiply core k of t by alpha in place."""
+    t._ctx.check(t._ctx.lib.tta_scale_core(t._ctx.h, t._h, k, ctypes.c_double(alpha)))

@@ -47,7 +47,17 @@ SIGNATURES = {
     "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
     "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),
     "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),
+    "tta_random_host": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_ulonglong, _i32p, _f64p,
+                                       ctypes.POINTER(ctypes.c_longlong)]),
+    "tta_random": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_ulonglong, _vpp]),
+    "tta_random_matrix": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, ctypes.c_int, ctypes.c_ulonglong, _vpp]),
+    "tta_constant": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_double, _vpp]),
+    "tta_scale_core": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
+    "tta_prof_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),
 }
+
+KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4}
 
 _lib = None
 

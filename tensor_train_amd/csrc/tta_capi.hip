@@ -176,6 +176,77 @@ void replace_core(tta_ctx* c, tta_tt* t, int k, double* p)
 
 double* alloc_core(tta_ctx* c, size_t n) { return static_cast<double*>(c->mem.get(std::max<size_t>(n, 1) * 8)); }
 
+// Event pair around one launch on the context stream while timing is on.
+hipEvent_t prof_event(tta_ctx* c)
+{
+  if (!c->prof_free.empty())
+  {
+    hipEvent_t e = c->prof_free.back();
+    c->prof_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
+// Flop models (Golub & Van Loan counts).  Householder QR of m x n with
+// k = min(m, n) reflectors plus the thin Q: 4 m k^2 - 4/3 k^3 for m >= n,
+// 2 n m^2 + 2/3 m^3 for m < n.  Thin SVD with U and V of p x q (p >= q):
+// 14 p q^2 + 8 q^3 (22 q^3 for a square matrix).
+double qr_flops(double m, double n)
+{
+  return m >= n ? 4.0 * m * n * n - 4.0 / 3.0 * n * n * n : 2.0 * n * m * m + 2.0 / 3.0 * m * m * m;
+}
+double svd_flops(double p, double q)
+{
+  if (p < q) std::swap(p, q);
+  return 14.0 * p * q * q + 8.0 * q * q * q;
+}
+
+// Algorithmic work credited to one launch (not hardware counters):
+// flops of the textbook algorithm at that shape and the minimum HBM bytes
+// (each operand read once, each result written once).
+struct prof_scope
+{
+  tta_ctx* c;
+  int cls;
+  double flops, bytes;
+  hipEvent_t a = nullptr;
+  prof_scope(tta_ctx* c_, int cls_, double f = 0.0, double by = 0.0) : c(c_), cls(cls_), flops(f), bytes(by)
+  {
+    if (c->prof)
+    {
+      a = prof_event(c);
+      hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+    }
+  }
+  ~prof_scope()
+  {
+    if (!a) return;
+    hipEvent_t b = prof_event(c);
+    if (hipEventRecord(b, c->stream) == hipSuccess) c->prof_pending.push_back({cls, a, b, flops, bytes});
+  }
+};
+
+void prof_collect(tta_ctx* c)
+{
+  if (c->prof_pending.empty()) return;
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (auto& r : c->prof_pending)
+  {
+    float ms = 0.f;
+    hip_check(hipEventElapsedTime(&ms, r.a, r.b), "hipEventElapsedTime");
+    c->prof_ms[r.cls] += ms;
+    c->prof_n[r.cls] += 1;
+    c->prof_flops[r.cls] += r.flops;
+    c->prof_bytes[r.cls] += r.bytes;
+    c->prof_free.push_back(r.a);
+    c->prof_free.push_back(r.b);
+  }
+  c->prof_pending.clear();
+}
+
 void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const double* B, long ldb, double* C,
              long ldc)
 {
@@ -192,6 +263,7 @@ void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const d
   g.C = C;
   g.scr = ldc;
   g.scc = 1;
+  prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
   launch_gemm(g, 1, c->stream);
 }
 
@@ -258,7 +330,10 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k)
   q.tau = tau;
   q.Q = Qn;
   q.ldq = m;
-  launch_qr(q, 1, c->stream);
+  {
+      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
+      launch_qr(q, 1, c->stream);
+  }
   double* L = ctx_ws(c, WS_X, (size_t)r * r_eff);
   launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream);
   const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
@@ -288,7 +363,10 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k)
   q.tau = tau;
   q.Q = Q;
   q.ldq = m;
-  launch_qr(q, 1, c->stream);
+  {
+      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
+      launch_qr(q, 1, c->stream);
+  }
   double* Cn = alloc_core(c, (size_t)m * kk);
   launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
   double* R = ctx_ws(c, WS_X, (size_t)kk * rp);
@@ -332,7 +410,10 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     q.tau = tau;
     q.Q = Q;
     q.ldq = rows;
-    launch_qr(q, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
+      launch_qr(q, 1, c->stream);
+  }
     double* X = ctx_ws(c, WS_X, (size_t)cols * cols);
     launch_extract_upper(A, rows, cols, cols, X, cols, 1, c->stream);
     double* J = ctx_ws(c, WS_J, (size_t)cols * cols);
@@ -346,7 +427,11 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     jd.ldj = cols;
     jd.s = sv;
     jd.perm = perm;
+    {
+    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
+                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
     launch_jacobi(jd, 1, c->stream);
+  }
     const int r_new = read_int(c, rk);
     double* Jp = ctx_ws(c, WS_JP, (size_t)cols * r_new);
     launch_gather_cols(J, cols, cols, perm, r_new, Jp, 1, cols, c->stream);
@@ -364,7 +449,11 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     g.C = U;
     g.scr = r_new;
     g.scc = 1;
-    launch_gemm(g, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
+                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
+      launch_gemm(g, 1, c->stream);
+    }
     double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
     launch_gather_cols(X, cols, cols, perm, r_new, F, 1, cols, c->stream);
     *U_out = U;
@@ -385,7 +474,11 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   jd.ldj = rows;
   jd.s = sv;
   jd.perm = perm;
-  launch_jacobi(jd, 1, c->stream);
+  {
+    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
+                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
+    launch_jacobi(jd, 1, c->stream);
+  }
   const int r_new = read_int(c, rk);
   double* U = alloc_core(c, (size_t)rows * r_new);
   launch_gather_cols(J, rows, rows, perm, r_new, U, r_new, 1, c->stream);
@@ -440,7 +533,10 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     q.tau = tau;
     q.Q = Q;
     q.ldq = cols;
-    launch_qr(q, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
+      launch_qr(q, 1, c->stream);
+  }
     double* X = ctx_ws(c, WS_X, (size_t)rows * rows);
     launch_extract_upper(A, cols, rows, rows, X, rows, 1, c->stream);
     double* J = ctx_ws(c, WS_J, (size_t)rows * rows);
@@ -454,7 +550,11 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     jd.ldj = rows;
     jd.s = sv;
     jd.perm = perm;
+    {
+    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
+                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
     launch_jacobi(jd, 1, c->stream);
+  }
     r_new = read_int(c, rk);
     double* Jp = ctx_ws(c, WS_JP, (size_t)rows * r_new);
     launch_gather_cols(J, rows, rows, perm, r_new, Jp, 1, rows, c->stream);
@@ -472,7 +572,11 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     g.C = Cn;
     g.scr = cols;
     g.scc = 1;
-    launch_gemm(g, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
+                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
+      launch_gemm(g, 1, c->stream);
+    }
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
     launch_gather_cols(X, rows, rows, perm, r_new, F, r_new, 1, c->stream);
   }
@@ -491,7 +595,11 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     jd.ldj = cols;
     jd.s = sv;
     jd.perm = perm;
+    {
+    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
+                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
     launch_jacobi(jd, 1, c->stream);
+  }
     r_new = read_int(c, rk);
     Cn = alloc_core(c, (size_t)r_new * cols);
     launch_gather_cols(J, cols, cols, perm, r_new, Cn, 1, cols, c->stream);
@@ -595,6 +703,13 @@ tta_tt* matvec_impl(tta_ctx* c, const tta_tt* A, const tta_tt* x)
   for (int k = 0; k < d; ++k) modes[k] = A->row_modes[k];
   for (int k = 0; k <= d; ++k) ranks[k] = A->ranks[k] * x->ranks[k];
   tta_tt* y = new_tt(c, modes, ranks, false);
+  double fl = 0.0, by = 0.0;
+  for (int k = 0; k < d; ++k)
+  {
+    fl += 2.0 * ranks[k] * modes[k] * ranks[k + 1] * A->col_modes[k];
+    by += 8.0 * ((double)core_numel(A, k) + (double)core_numel(x, k) + (double)ranks[k] * modes[k] * ranks[k + 1]);
+  }
+  prof_scope ps(c, TTA_K_APPLY, fl, by);
   for (int k = 0; k < d; ++k)
     launch_matvec_core(A->cores[k], x->cores[k], y->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
                        A->ranks[k + 1], x->ranks[k], x->ranks[k + 1], c->stream);
@@ -611,6 +726,13 @@ tta_tt* matmat_impl(tta_ctx* c, const tta_tt* A, const tta_tt* B)
   tta_tt* C = new_tt(c, modes, ranks, false);
   C->row_modes = A->row_modes;
   C->col_modes = B->col_modes;
+  double fl = 0.0, by = 0.0;
+  for (int k = 0; k < d; ++k)
+  {
+    fl += 2.0 * ranks[k] * modes[k] * ranks[k + 1] * A->col_modes[k];
+    by += 8.0 * ((double)core_numel(A, k) + (double)core_numel(B, k) + (double)ranks[k] * modes[k] * ranks[k + 1]);
+  }
+  prof_scope ps(c, TTA_K_APPLY, fl, by);
   for (int k = 0; k < d; ++k)
     launch_matmat_core(A->cores[k], B->cores[k], C->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
                        A->ranks[k + 1], B->ranks[k], B->col_modes[k], B->ranks[k + 1], c->stream);
@@ -642,7 +764,11 @@ double* fused_core(tta_ctx* c, const double* F, int alpha, const double* Ak, int
   g.bsa = (long)a * sx;
   g.bsb = 0;
   g.bsc = (long)sx * mpb;
-  launch_gemm(g, alpha, c->stream);
+  {
+    prof_scope ps(c, TTA_K_GEMM, 2.0 * alpha * g.M * g.N * g.K,
+                  8.0 * alpha * ((double)g.M * g.K + (double)g.M * g.N) + 8.0 * g.K * g.N);
+    launch_gemm(g, alpha, c->stream);
+  }
   // M1p (alpha, i, bb, ax, q) <- M1 (alpha, ax, i, q, bb).
   double* M1p = ctx_ws(c, WS_M1P, (size_t)alpha * sx * mpb);
   permute_desc pd;
@@ -801,7 +927,11 @@ double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
     g.C = T;
     g.scr = (long)n * rap;
     g.scc = 1;
-    launch_gemm(g, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
+                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
+      launch_gemm(g, 1, c->stream);
+    }
     // E' (rap x rbp) = tmp_as^T (rap x rb*n) * left_unfold(cb) (rb*n x rbp)
     gemm_desc h;
     h.M = rap;
@@ -816,7 +946,11 @@ double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
     h.C = E1;
     h.scr = rbp;
     h.scc = 1;
-    launch_gemm(h, 1, c->stream);
+    {
+      prof_scope ps(c, TTA_K_GEMM, 2.0 * h.M * h.N * h.K,
+                    8.0 * ((double)h.M * h.K + (double)h.K * h.N + (double)h.M * h.N));
+      launch_gemm(h, 1, c->stream);
+    }
     std::swap(E0, E1);
   }
   return read_double(c, E0);
@@ -1277,6 +1411,43 @@ int tta_scale(tta_ctx* c, const tta_tt* a, double alpha, tta_tt** out)
   if (r->d() > 0) launch_scale(r->cores[0], (long)core_numel(r, 0), alpha, c->stream);
   hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   *out = r;
+  TTA_GUARD_END(c)
+}
+
+int tta_scale_core(tta_ctx* c, tta_tt* t, int k, double alpha)
+{
+  if (!c || !t || k < 0 || k >= t->d()) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  launch_scale(t->cores[k], (long)core_numel(t, k), alpha, c->stream);
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  TTA_GUARD_END(c)
+}
+
+int tta_prof_enable(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  prof_collect(c);
+  c->prof = on != 0;
+  for (int i = 0; i < 8; ++i)
+  {
+    c->prof_ms[i] = 0.0;
+    c->prof_n[i] = 0;
+    c->prof_flops[i] = 0.0;
+    c->prof_bytes[i] = 0.0;
+  }
+  TTA_GUARD_END(c)
+}
+
+int tta_prof_read(tta_ctx* c, int cls, long long* launches, double* total_ms, double* flops, double* bytes)
+{
+  if (!c || cls < 0 || cls >= TTA_K_COUNT) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  prof_collect(c);
+  if (launches) *launches = c->prof_n[cls];
+  if (total_ms) *total_ms = c->prof_ms[cls];
+  if (flops) *flops = c->prof_flops[cls];
+  if (bytes) *bytes = c->prof_bytes[cls];
   TTA_GUARD_END(c)
 }
 

@@ -55,6 +55,21 @@ struct tta_ctx
   int* h_int = nullptr;     // pinned host scalars
   double* h_dbl = nullptr;
   std::string last_error;
+  // Per-launch event timing (tta_prof_enable): recorded pairs awaiting
+  // tta_prof_read, recycled events, totals per kernel class.
+  bool prof = false;
+  struct prof_rec
+  {
+    int cls;
+    hipEvent_t a, b;
+    double flops, bytes;
+  };
+  std::vector<prof_rec> prof_pending;
+  std::vector<hipEvent_t> prof_free;
+  double prof_ms[8] = {0};
+  long long prof_n[8] = {0};
+  double prof_flops[8] = {0};
+  double prof_bytes[8] = {0};
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])
