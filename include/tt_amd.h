@@ -4,7 +4,7 @@
  *
  * Every entry point returns an int status (TTA_OK == 0); on failure the
  * message is available from tta_last_error(ctx).  The header-only C++ layer
- * (include/tensor_train_amd.hpp) maps a non-zero status to
+ * (include/tensor_train.hpp) maps a non-zero status to
  * fprintf(stderr) + abort(), the error behaviour of the reference
  * (round.hpp:124-134, matvec_round.hpp:47-52, shape_check.hpp:29-150 of
  * bdaszuta/tensor_train).
@@ -123,6 +123,15 @@ int tta_axpby(tta_ctx* ctx, double alpha, const tta_tt* a, double beta, const tt
 /* scale(a, alpha) (tt_ops.hpp:35-50). */
 int tta_scale(tta_ctx* ctx, const tta_tt* a, double alpha, tta_tt** out);
 
+/* ---- dense kernels of the path (host buffers, row-major) --------------- */
+/* qr_thin (include/core/types/tt_eigen_bridge.hpp:42-60): Householder QR,
+ * Q (rows x k) orthonormal columns, R (k x cols) upper trapezoidal,
+ * k = min(rows, cols), dlarfg sign convention. */
+int tta_qr_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* Q, double* R);
+/* svd_thin (tt_eigen_bridge.hpp:67-104): U (rows x k), s (k, descending),
+ * Vt (k x cols). */
+int tta_svd_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* U, double* s, double* Vt);
+
 /* ---- seeded generators (inputs of the benchmark configurations) --------- */
 /* random(shape, max_rank, seed) (include/core/factory/tt_factory.hpp:141-207),
  * on the host with std::mt19937_64 + std::normal_distribution<double>, drawn
@@ -157,6 +166,12 @@ int tta_prof_enable(tta_ctx* ctx, int on);
  * each launch's shape, not hardware counters) for one kernel class. */
 int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* total_ms, double* flops,
                   double* bytes);
+
+/* ---- self-checks --------------------------------------------------------- */
+/* Compares the device's DPP wave reduction with a ds_bpermute reduction on n
+ * host values (rounded up to whole 64-lane waves); *mismatches = 0 when they
+ * agree to rounding.  No reference counterpart: a hardware sanity check. */
+int tta_selfcheck_wave_sum(tta_ctx* ctx, const double* host, int n, int* mismatches);
 
 #ifdef __cplusplus
 }

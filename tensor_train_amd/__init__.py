@@ -474,3 +474,32 @@ def ones(shape, ctx: Context | None = None) -> TensorTrain:
 def scale_core(t, k: int, alpha: float) -> None:
     """Multiply core k of t by alpha in place."""
     t._ctx.check(t._ctx.lib.tta_scale_core(t._ctx.h, t._h, k, ctypes.c_double(alpha)))
+
+
+def qr_thin(M, ctx: Context | None = None):
+    """Thin Householder QR on the GPU (qr_thin, tt_eigen_bridge.hpp:42-60)."""
+    ctx = ctx or Context.default()
+    M = np.ascontiguousarray(M, np.float64)
+    rows, cols = M.shape
+    k = min(rows, cols)
+    Q = np.zeros((rows, k))
+    R = np.zeros((k, cols))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ctx.check(ctx.lib.tta_qr_thin(ctx.h, M.ctypes.data_as(dp), rows, cols, Q.ctypes.data_as(dp),
+                                  R.ctypes.data_as(dp)))
+    return Q, R
+
+
+def svd_thin(M, ctx: Context | None = None):
+    """Thin SVD on the GPU (svd_thin, tt_eigen_bridge.hpp:67-104)."""
+    ctx = ctx or Context.default()
+    M = np.ascontiguousarray(M, np.float64)
+    rows, cols = M.shape
+    k = min(rows, cols)
+    U = np.zeros((rows, k))
+    s = np.zeros(k)
+    Vt = np.zeros((k, cols))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ctx.check(ctx.lib.tta_svd_thin(ctx.h, M.ctypes.data_as(dp), rows, cols, U.ctypes.data_as(dp),
+                                   s.ctypes.data_as(dp), Vt.ctypes.data_as(dp)))
+    return U, s, Vt

@@ -47,6 +47,9 @@ SIGNATURES = {
     "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
     "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),
     "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),
+    "tta_qr_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
+    "tta_svd_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]),
+    "tta_selfcheck_wave_sum": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p]),
     "tta_random_host": (ctypes.c_int, [ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_ulonglong, _i32p, _f64p,
                                        ctypes.POINTER(ctypes.c_longlong)]),
     "tta_random": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_ulonglong, _vpp]),

@@ -267,6 +267,62 @@ void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const d
   launch_gemm(g, 1, c->stream);
 }
 
+// Scratch slots (see tta_internal.hpp).
+enum
+{
+  WS_A = 0,
+  WS_Q,
+  WS_X,
+  WS_J,
+  WS_F,
+  WS_JP,
+  WS_M1,
+  WS_M1P,
+  WS_PART,
+  WS_SMALL,
+  WS_T
+};
+
+// Householder QR of a column-major (m x n) matrix in place, thin Q (m x k)
+// into Q when non-null: blocked compact-WY kernels for m <= 2048, the
+// single-workgroup unblocked kernel beyond.
+void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq)
+{
+  prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
+  if (m <= qr_blocked_max_rows())
+  {
+    const int k = std::min(m, n);
+    qr_batch b;
+    b.one.A = A;
+    b.one.lda = lda;
+    b.one.m = m;
+    b.one.n = n;
+    b.one.tau = tau;
+    b.one.T = ctx_ws(c, WS_T, (size_t)((k + 15) / 16) * 256 + 1);
+    b.one.Q = Q;
+    b.one.ldq = ldq;
+    b.max_m = m;
+    b.max_n = n;
+    launch_qr_blocked(b, c->stream);
+    return;
+  }
+  qr_desc q;
+  q.A = A;
+  q.lda = lda;
+  q.m = m;
+  q.n = n;
+  q.tau = tau;
+  q.Q = Q;
+  q.ldq = ldq;
+  launch_qr(q, 1, c->stream);
+}
+
+void run_gemm(tta_ctx* c, const gemm_desc& g)
+{
+  prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K, 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
+  launch_gemm(g, 1, c->stream);
+}
+
 int read_int(tta_ctx* c, const int* dev)
 {
   hip_check(hipMemcpyAsync(c->h_int, dev, sizeof(int), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
@@ -280,21 +336,6 @@ double read_double(tta_ctx* c, const double* dev)
   hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   return *c->h_dbl;
 }
-
-// Scratch slots (see tta_internal.hpp).
-enum
-{
-  WS_A = 0,
-  WS_Q,
-  WS_X,
-  WS_J,
-  WS_F,
-  WS_JP,
-  WS_M1,
-  WS_M1P,
-  WS_PART,
-  WS_SMALL
-};
 
 // Small device scalars live in WS_SMALL: [0] norm, [1] delta, [2..] ints.
 double* small_dbl(tta_ctx* c) { return ctx_ws(c, WS_SMALL, 64); }
@@ -322,18 +363,7 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k)
   double* A = t->cores[k];
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
-  qr_desc q;
-  q.A = A;
-  q.lda = m;
-  q.m = m;
-  q.n = r;
-  q.tau = tau;
-  q.Q = Qn;
-  q.ldq = m;
-  {
-      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
-      launch_qr(q, 1, c->stream);
-  }
+  qr_factor(c, A, m, m, r, tau, Qn, m);
   double* L = ctx_ws(c, WS_X, (size_t)r * r_eff);
   launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream);
   const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
@@ -355,18 +385,7 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k)
   launch_transpose(t->cores[k], rp, m, rp, A, m, c->stream);
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(kk, 1));
   double* Q = ctx_ws(c, WS_Q, (size_t)m * kk);
-  qr_desc q;
-  q.A = A;
-  q.lda = m;
-  q.m = m;
-  q.n = rp;
-  q.tau = tau;
-  q.Q = Q;
-  q.ldq = m;
-  {
-      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
-      launch_qr(q, 1, c->stream);
-  }
+  qr_factor(c, A, m, m, rp, tau, Q, m);
   double* Cn = alloc_core(c, (size_t)m * kk);
   launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
   double* R = ctx_ws(c, WS_X, (size_t)kk * rp);
@@ -379,113 +398,117 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k)
   t->ranks[k + 1] = kk;
 }
 
-// Thin SVD of a row-major (rows x cols) matrix M on the device, returning
-// the truncation rank and leaving:
-//   U_head  (rows x r_new) row-major in `U_out` (orthonormal columns, exact
-//           up to rounding since it is a product of orthogonal factors),
-//   F       (r_new x cols) row-major = diag(s_head) Vt_head in `F_out`.
-// rows >= cols: M = Q R (Householder), one-sided Jacobi on R^T gives
-//   R^T J = W, so M = (Q J) W^T.  rows < cols: Jacobi on M^T directly.
-// `M` is clobbered only when `may_clobber`.
-int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int max_rank, double* U_out_alloc_hint,
-             double** U_out, double** F_out)
+// Jacobi SVD of the square q x q column-major X (ld q), rotations into J
+// (ld q): X J = W.  Leaves sorted norms s and order perm in scratch and
+// returns the truncation rank decided on the device (or q with keep_all).
+int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_rank, bool keep_all, double** s_out,
+               int** perm_out)
 {
-  (void)U_out_alloc_hint;
-  int* rk = small_int(c);
-  jacobi_desc jd;
-  jd.delta = delta;
-  jd.max_rank = max_rank;
-  jd.rank_out = rk;
+  int* ints = small_int(c);
+  double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)q + 8) + 1024;
+  int* perm = reinterpret_cast<int*>(sv + q);
+  jac_batch b;
+  b.one.X = X;
+  b.one.ldx = q;
+  b.one.q = q;
+  b.one.J = J;
+  b.one.ldj = q;
+  b.one.s = sv;
+  b.one.perm = perm;
+  b.one.rank_out = ints;
+  b.one.delta = delta;
+  b.one.max_rank = max_rank;
+  b.one.flag = ints + 2;
+  b.one.sweeps = ints + 5;
+  {
+    prof_scope ps(c, TTA_K_JACOBI, svd_flops(q, q), 8.0 * 4.0 * (double)q * q);
+    launch_jacobi_svd(b, q, ints + 4, c->h_int + 2, c->stream);
+  }
+  *s_out = sv;
+  *perm_out = perm;
+  return keep_all ? q : read_int(c, ints);
+}
+
+// Thin SVD of a row-major (rows x cols) matrix M, truncated on the device:
+//   U_head (rows x r_new) row-major with orthonormal columns (products of
+//          orthogonal factors, exact up to rounding), in a fresh pool block;
+//   F      (r_new x cols) row-major = diag(s_head) Vt_head, in scratch.
+// The matrix is QR-condensed to its square triangular factor first:
+//   rows >= cols: M = Q R, Jacobi on R^T: R^T J = W -> M = (Q J) W^T;
+//   rows <  cols: M^T = Q R, Jacobi on R:  R J = W  -> M = J (Q W)^T.
+int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int max_rank, bool keep_all,
+             double** U_out, double** F_out, double** s_out = nullptr)
+{
+  double* sv = nullptr;
+  int* perm = nullptr;
   if (rows >= cols)
   {
+    const int q = cols;
     double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
     launch_transpose(M, cols, rows, cols, A, rows, c->stream);
-    double* tau = ctx_ws(c, WS_F, (size_t)cols);
-    double* Q = ctx_ws(c, WS_Q, (size_t)rows * cols);
-    qr_desc q;
-    q.A = A;
-    q.lda = rows;
-    q.m = rows;
-    q.n = cols;
-    q.tau = tau;
-    q.Q = Q;
-    q.ldq = rows;
-    {
-      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
-      launch_qr(q, 1, c->stream);
-  }
-    double* X = ctx_ws(c, WS_X, (size_t)cols * cols);
-    launch_extract_upper(A, rows, cols, cols, X, cols, 1, c->stream);
-    double* J = ctx_ws(c, WS_J, (size_t)cols * cols);
-    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)cols + 8) + 1024;
-    int* perm = reinterpret_cast<int*>(sv + cols);
-    jd.X = X;
-    jd.ldx = cols;
-    jd.p = cols;
-    jd.q = cols;
-    jd.J = J;
-    jd.ldj = cols;
-    jd.s = sv;
-    jd.perm = perm;
-    {
-    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
-                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
-    launch_jacobi(jd, 1, c->stream);
-  }
-    const int r_new = read_int(c, rk);
-    double* Jp = ctx_ws(c, WS_JP, (size_t)cols * r_new);
-    launch_gather_cols(J, cols, cols, perm, r_new, Jp, 1, cols, c->stream);
+    double* tau = ctx_ws(c, WS_F, (size_t)q);
+    double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows);
+    double* X = ctx_ws(c, WS_X, (size_t)q * q);
+    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)q * q);
+    const int r_new = square_svd(c, X, q, J, delta, max_rank, keep_all, &sv, &perm);
+    double* Jp = ctx_ws(c, WS_JP, (size_t)q * r_new);
+    launch_gather_cols(J, q, q, perm, r_new, Jp, 1, q, c->stream);
     double* U = alloc_core(c, (size_t)rows * r_new);
     gemm_desc g;
     g.M = rows;
     g.N = r_new;
-    g.K = cols;
+    g.K = q;
     g.A = Q;
     g.sar = 1;
     g.sac = rows;
     g.B = Jp;
     g.sbr = 1;
-    g.sbc = cols;
+    g.sbc = q;
     g.C = U;
     g.scr = r_new;
     g.scc = 1;
-    {
-      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
-                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
-      launch_gemm(g, 1, c->stream);
-    }
+    run_gemm(c, g);
     double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
-    launch_gather_cols(X, cols, cols, perm, r_new, F, 1, cols, c->stream);
+    launch_gather_cols(X, q, q, perm, r_new, F, 1, q, c->stream);
     *U_out = U;
     *F_out = F;
+    if (s_out) *s_out = sv;
     return r_new;
   }
-  // rows < cols: X = M^T (cols x rows) column-major == M's bytes.
-  double* X = ctx_ws(c, WS_X, (size_t)rows * cols);
-  hip_check(hipMemcpyAsync(X, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
-  double* J = ctx_ws(c, WS_J, (size_t)rows * rows);
-  double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)rows + 8) + 1024;
-  int* perm = reinterpret_cast<int*>(sv + rows);
-  jd.X = X;
-  jd.ldx = cols;
-  jd.p = cols;
-  jd.q = rows;
-  jd.J = J;
-  jd.ldj = rows;
-  jd.s = sv;
-  jd.perm = perm;
-  {
-    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
-                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
-    launch_jacobi(jd, 1, c->stream);
-  }
-  const int r_new = read_int(c, rk);
+  const int q = rows;
+  double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+  hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+  double* tau = ctx_ws(c, WS_F, (size_t)q);
+  double* Q = ctx_ws(c, WS_Q, (size_t)cols * q);
+  qr_factor(c, A, cols, cols, rows, tau, Q, cols);
+  double* Y = ctx_ws(c, WS_X, (size_t)q * q);
+  launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);
+  double* J = ctx_ws(c, WS_J, (size_t)q * q);
+  const int r_new = square_svd(c, Y, q, J, delta, max_rank, keep_all, &sv, &perm);
   double* U = alloc_core(c, (size_t)rows * r_new);
-  launch_gather_cols(J, rows, rows, perm, r_new, U, r_new, 1, c->stream);
+  launch_gather_cols(J, q, q, perm, r_new, U, r_new, 1, c->stream);
+  double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);
+  launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
   double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
-  launch_gather_cols(X, cols, cols, perm, r_new, F, 1, cols, c->stream);
+  gemm_desc g;
+  g.M = r_new;
+  g.N = cols;
+  g.K = q;
+  g.A = Wp;
+  g.sar = q;
+  g.sac = 1;
+  g.B = Q;
+  g.sbr = cols;
+  g.sbc = 1;
+  g.C = F;
+  g.scr = cols;
+  g.scc = 1;
+  run_gemm(c, g);
   *U_out = U;
   *F_out = F;
+  if (s_out) *s_out = sv;
   return r_new;
 }
 
@@ -495,7 +518,7 @@ void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int rows = r * n, cols = rp;
   double *U = nullptr, *F = nullptr;
-  const int r_new = svd_left(c, t->cores[k], rows, cols, delta, max_rank, nullptr, &U, &F);
+  const int r_new = svd_left(c, t->cores[k], rows, cols, delta, max_rank, false, &U, &F);
   const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
   double* Nn = alloc_core(c, (size_t)r_new * ncols_n);
   gemm_rm(c, r_new, ncols_n, cols, F, cols, t->cores[k + 1], ncols_n, Nn, ncols_n);
@@ -505,66 +528,39 @@ void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
 }
 
 // svd_step_right (truncate.hpp:114-166): SVD of right_unfold(core k)
-// (r x n*rp); V kept exactly orthonormal (new core rows), U diag(s)
-// absorbed into core k-1.
+// (r x n*rp) with V exactly orthonormal (the new core's rows) and
+// U diag(s) absorbed into core k-1.
+//   cols >= rows: M^T = Q R, Jacobi on R^T: R^T J = W -> M = W (Q J)^T;
+//   cols <  rows: M = Q R,   Jacobi on R:   R J = W   -> M = (Q W) J^T.
 void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
 {
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int rows = r, cols = n * rp;
-  int* rk = small_int(c);
-  jacobi_desc jd;
-  jd.delta = delta;
-  jd.max_rank = max_rank;
-  jd.rank_out = rk;
+  double* sv = nullptr;
+  int* perm = nullptr;
   double* Cn = nullptr;
   double* F = nullptr;
   int r_new = 0;
   if (cols >= rows)
   {
-    // M^T (cols x rows) column-major == the core's bytes; factor in place.
-    double* A = t->cores[k];
-    double* tau = ctx_ws(c, WS_F, (size_t)rows);
-    double* Q = ctx_ws(c, WS_Q, (size_t)rows * cols);
-    qr_desc q;
-    q.A = A;
-    q.lda = cols;
-    q.m = cols;
-    q.n = rows;
-    q.tau = tau;
-    q.Q = Q;
-    q.ldq = cols;
-    {
-      prof_scope ps(c, TTA_K_QR, qr_flops(q.m, q.n), 8.0 * ((double)q.m * q.n + (double)q.m * std::min(q.m, q.n)));
-      launch_qr(q, 1, c->stream);
-  }
-    double* X = ctx_ws(c, WS_X, (size_t)rows * rows);
-    launch_extract_upper(A, cols, rows, rows, X, rows, 1, c->stream);
-    double* J = ctx_ws(c, WS_J, (size_t)rows * rows);
-    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)rows + 8) + 1024;
-    int* perm = reinterpret_cast<int*>(sv + rows);
-    jd.X = X;
-    jd.ldx = rows;
-    jd.p = rows;
-    jd.q = rows;
-    jd.J = J;
-    jd.ldj = rows;
-    jd.s = sv;
-    jd.perm = perm;
-    {
-    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
-                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
-    launch_jacobi(jd, 1, c->stream);
-  }
-    r_new = read_int(c, rk);
-    double* Jp = ctx_ws(c, WS_JP, (size_t)rows * r_new);
-    launch_gather_cols(J, rows, rows, perm, r_new, Jp, 1, rows, c->stream);
+    const int q = rows;
+    double* A = t->cores[k];  // M^T (cols x rows) column-major; factored in place
+    double* tau = ctx_ws(c, WS_F, (size_t)q);
+    double* Q = ctx_ws(c, WS_Q, (size_t)q * cols);
+    qr_factor(c, A, cols, cols, rows, tau, Q, cols);
+    double* X = ctx_ws(c, WS_X, (size_t)q * q);
+    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)q * q);
+    r_new = square_svd(c, X, q, J, delta, max_rank, false, &sv, &perm);
+    double* Jp = ctx_ws(c, WS_JP, (size_t)q * r_new);
+    launch_gather_cols(J, q, q, perm, r_new, Jp, 1, q, c->stream);
     Cn = alloc_core(c, (size_t)r_new * cols);
     gemm_desc g;
     g.M = r_new;
     g.N = cols;
-    g.K = rows;
+    g.K = q;
     g.A = Jp;
-    g.sar = rows;
+    g.sar = q;
     g.sac = 1;
     g.B = Q;
     g.sbr = cols;
@@ -572,39 +568,41 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     g.C = Cn;
     g.scr = cols;
     g.scc = 1;
-    {
-      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
-                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
-      launch_gemm(g, 1, c->stream);
-    }
+    run_gemm(c, g);
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
-    launch_gather_cols(X, rows, rows, perm, r_new, F, r_new, 1, c->stream);
+    launch_gather_cols(X, q, q, perm, r_new, F, r_new, 1, c->stream);
   }
   else
   {
-    double* X = ctx_ws(c, WS_X, (size_t)rows * cols);
-    launch_transpose(t->cores[k], cols, rows, cols, X, rows, c->stream);
-    double* J = ctx_ws(c, WS_J, (size_t)cols * cols);
-    double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)cols + 8) + 1024;
-    int* perm = reinterpret_cast<int*>(sv + cols);
-    jd.X = X;
-    jd.ldx = rows;
-    jd.p = rows;
-    jd.q = cols;
-    jd.J = J;
-    jd.ldj = cols;
-    jd.s = sv;
-    jd.perm = perm;
-    {
-    prof_scope ps(c, TTA_K_JACOBI, svd_flops(jd.p, jd.q),
-                  8.0 * (2.0 * jd.p * jd.q + 2.0 * (double)jd.q * jd.q));
-    launch_jacobi(jd, 1, c->stream);
-  }
-    r_new = read_int(c, rk);
+    const int q = cols;
+    double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+    launch_transpose(t->cores[k], cols, rows, cols, A, rows, c->stream);
+    double* tau = ctx_ws(c, WS_F, (size_t)q);
+    double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows);
+    double* Y = ctx_ws(c, WS_X, (size_t)q * q);
+    launch_extract_upper(A, rows, q, q, Y, 1, q, c->stream);
+    double* J = ctx_ws(c, WS_J, (size_t)q * q);
+    r_new = square_svd(c, Y, q, J, delta, max_rank, false, &sv, &perm);
     Cn = alloc_core(c, (size_t)r_new * cols);
-    launch_gather_cols(J, cols, cols, perm, r_new, Cn, 1, cols, c->stream);
+    launch_gather_cols(J, q, q, perm, r_new, Cn, 1, q, c->stream);
+    double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);
+    launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
-    launch_gather_cols(X, rows, rows, perm, r_new, F, r_new, 1, c->stream);
+    gemm_desc g;
+    g.M = rows;
+    g.N = r_new;
+    g.K = q;
+    g.A = Q;
+    g.sar = 1;
+    g.sac = rows;
+    g.B = Wp;
+    g.sbr = 1;
+    g.sbc = q;
+    g.C = F;
+    g.scr = r_new;
+    g.scc = 1;
+    run_gemm(c, g);
   }
   const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
   double* Pn = alloc_core(c, (size_t)rows_p * r_new);
@@ -870,7 +868,7 @@ tta_tt* streaming_impl(tta_ctx* c, const tta_tt* A_in, const tta_tt* X_in, bool 
         break;
       }
       double *U = nullptr, *Fs = nullptr;
-      const int r_new = svd_left(c, C, alpha * mk, br, delta, max_rank, nullptr, &U, &Fs);
+      const int r_new = svd_left(c, C, alpha * mk, br, delta, max_rank, false, &U, &Fs);
       c->mem.put(C);
       replace_core(c, out, k, U);
       out->ranks[k] = alpha;
@@ -1448,6 +1446,73 @@ int tta_prof_read(tta_ctx* c, int cls, long long* launches, double* total_ms, do
   if (total_ms) *total_ms = c->prof_ms[cls];
   if (flops) *flops = c->prof_flops[cls];
   if (bytes) *bytes = c->prof_bytes[cls];
+  TTA_GUARD_END(c)
+}
+
+// detail::qr_thin (include/core/types/tt_eigen_bridge.hpp:42-60) on host
+// buffers: M (rows x cols) row-major -> Q (rows x k), R (k x cols), k = min.
+int tta_qr_thin(tta_ctx* c, const double* M, int rows, int cols, double* Q, double* R)
+{
+  if (!c || rows < 0 || cols < 0) return TTA_ERR_ARG;
+  if (rows == 0 || cols == 0) return TTA_OK;
+  if (!M || !Q || !R) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  const int k = std::min(rows, cols);
+  double* Md = alloc_core(c, (size_t)rows * cols);
+  hip_check(hipMemcpyAsync(Md, M, (size_t)rows * cols * 8, hipMemcpyHostToDevice, c->stream), "upload");
+  double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+  launch_transpose(Md, cols, rows, cols, A, rows, c->stream);
+  double* tau = ctx_ws(c, WS_F, (size_t)k);
+  double* Qc = ctx_ws(c, WS_Q, (size_t)rows * k);
+  qr_factor(c, A, rows, rows, cols, tau, Qc, rows);
+  double* Rd = ctx_ws(c, WS_X, (size_t)k * cols);
+  launch_extract_upper(A, rows, k, cols, Rd, cols, 1, c->stream);
+  launch_transpose(Qc, rows, k, rows, Md, k, c->stream);
+  hip_check(hipMemcpyAsync(Q, Md, (size_t)rows * k * 8, hipMemcpyDeviceToHost, c->stream), "download");
+  hip_check(hipMemcpyAsync(R, Rd, (size_t)k * cols * 8, hipMemcpyDeviceToHost, c->stream), "download");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  c->mem.put(Md);
+  TTA_GUARD_END(c)
+}
+
+// detail::svd_thin (tt_eigen_bridge.hpp:67-104) on host buffers: U (rows x
+// k), s (k, descending), Vt (k x cols).  Vt rows of zero singular values are 0.
+int tta_svd_thin(tta_ctx* c, const double* M, int rows, int cols, double* U, double* s, double* Vt)
+{
+  if (!c || rows < 0 || cols < 0) return TTA_ERR_ARG;
+  if (rows == 0 || cols == 0) return TTA_OK;
+  if (!M || !U || !s || !Vt) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  const int k = std::min(rows, cols);
+  double* Md = alloc_core(c, (size_t)rows * cols);
+  hip_check(hipMemcpyAsync(Md, M, (size_t)rows * cols * 8, hipMemcpyHostToDevice, c->stream), "upload");
+  double *Ud = nullptr, *Fd = nullptr, *sd = nullptr;
+  svd_left(c, Md, rows, cols, 0.0, 0, true, &Ud, &Fd, &sd);
+  std::vector<double> F((size_t)k * cols);
+  hip_check(hipMemcpyAsync(U, Ud, (size_t)rows * k * 8, hipMemcpyDeviceToHost, c->stream), "download");
+  hip_check(hipMemcpyAsync(F.data(), Fd, (size_t)k * cols * 8, hipMemcpyDeviceToHost, c->stream), "download");
+  hip_check(hipMemcpyAsync(s, sd, (size_t)k * 8, hipMemcpyDeviceToHost, c->stream), "download");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (int i = 0; i < k; ++i)
+    for (int j = 0; j < cols; ++j) Vt[(size_t)i * cols + j] = s[i] > 0.0 ? F[(size_t)i * cols + j] / s[i] : 0.0;
+  c->mem.put(Md);
+  c->mem.put(Ud);
+  TTA_GUARD_END(c)
+}
+
+// Device self-check of the wave reduction used by every kernel (DPP path
+// against the ds_bpermute path) on n host values; *mismatches = 0 when good.
+int tta_selfcheck_wave_sum(tta_ctx* c, const double* host, int n, int* mismatches)
+{
+  if (!c || !host || n <= 0 || !mismatches) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  double* x = alloc_core(c, (size_t)n);
+  int* bad = small_int(c) + 8;
+  hip_check(hipMemcpyAsync(x, host, (size_t)n * 8, hipMemcpyHostToDevice, c->stream), "upload");
+  hip_check(hipMemsetAsync(bad, 0, sizeof(int), c->stream), "memset");
+  launch_wave_sum_check(x, n, bad, c->stream);
+  *mismatches = read_int(c, bad);
+  c->mem.put(x);
   TTA_GUARD_END(c)
 }
 

@@ -20,7 +20,41 @@ namespace dev
 
 constexpr int kWave = 64;
 
+// One DPP move of a double (two 32-bit halves); `ctrl` is a DPP control
+// word (quad_perm, row_ror, ...), all rows and banks enabled.
+template <int ctrl>
+__device__ __forceinline__ double dpp_mov(double v)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffffLL), ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), ctrl, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double read_lane(double v, int lane)
+{
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over the 64 lanes of a wave, returned (wave-uniform) to every lane.
+// Within each 16-lane row: quad_perm [1,0,3,2] and [2,3,0,1] sum each quad,
+// row_ror:4 and row_ror:8 sum the four quads; the four row sums are then
+// read from lanes 0, 16, 32, 48.  All on the ALU/DPP path (no LDS).
 __device__ __forceinline__ double wave_sum(double v)
+{
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+// Reference reduction through ds_bpermute, kept for cross-checking the DPP
+// path in the tests.
+__device__ __forceinline__ double wave_sum_shfl(double v)
 {
 #pragma unroll
   for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);

@@ -1,0 +1,313 @@
+// One-sided (Hestenes) Jacobi SVD of batches of square q x q matrices,
+// with the rotations accumulated into J:  X_in * J = X_out, X_out having
+// mutually orthogonal columns and J orthogonal.
+//
+// q <= 64: jacobi_small_kernel keeps X and J in LDS and runs whole sweeps in
+//   one workgroup (round-robin ordering, one wave per disjoint pair).
+// q  > 64: block Jacobi.  The columns are cut into blocks of W; each launch
+//   of bj_round_kernel is one round of a round-robin over blocks, one
+//   workgroup per block pair: the pair's 2W columns of X and J are staged
+//   in LDS and orthogonalised among themselves by one inner sweep, then
+//   written back.  A sweep is nb - 1 rounds; the host stops after a sweep
+//   with no rotation.
+// Rotation rule: rotate while |x_i . x_j| > tol ||x_i|| ||x_j|| with
+// tol = max(1e-15, q u), the worst-case rounding bound of a length-q dot
+// product; below it a "rotation" only chases rounding noise and a sweep
+// would never come out clean.  J stays orthogonal to working precision
+// whatever tol is, so the singular vectors built from it do too.
+#include "tta_device.hpp"
+#include "tta_kernels.hpp"
+
+namespace tta
+{
+using namespace dev;
+
+namespace
+{
+
+__device__ __forceinline__ const jac_prob& jpick(const jac_batch& b, int idx)
+{
+  return b.descs ? b.descs[idx] : b.one;
+}
+
+__device__ __forceinline__ double jac_tol(int q) { return fmax(1.0e-15, (double)q * 2.220446049250313e-16); }
+
+// Rotate columns (xi, xj) of length p and (ji, jj) of length q in LDS by the
+// Jacobi rotation that orthogonalises xi and xj.  One wave; returns 1 when a
+// rotation was applied.
+__device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, double* jj, int p, int q, double tol,
+                                           int lane)
+{
+  double a = 0.0, b = 0.0, g = 0.0;
+  for (int t = lane; t < p; t += kWave)
+  {
+    const double u = xi[t], v = xj[t];
+    a = fma(u, u, a);
+    b = fma(v, v, b);
+    g = fma(u, v, g);
+  }
+  a = wave_sum(a);
+  b = wave_sum(b);
+  g = wave_sum(g);
+  if (g == 0.0 || fabs(g) <= tol * sqrt(a * b)) return 0;
+  const double zeta = (b - a) / (2.0 * g);
+  const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+  const double cs = 1.0 / sqrt(1.0 + tt * tt);
+  const double sn = cs * tt;
+  for (int t = lane; t < p; t += kWave)
+  {
+    const double u = xi[t], v = xj[t];
+    xi[t] = cs * u - sn * v;
+    xj[t] = sn * u + cs * v;
+  }
+  for (int t = lane; t < q; t += kWave)
+  {
+    const double u = ji[t], v = jj[t];
+    ji[t] = cs * u - sn * v;
+    jj[t] = sn * u + cs * v;
+  }
+  return 1;
+}
+
+// Whole Jacobi in one workgroup with X, J in LDS (q <= 64).
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_sweeps)
+{
+  const jac_prob& P = jpick(bt, blockIdx.x);
+  const int q = P.q;
+  extern __shared__ double lds[];
+  double* Xs = lds;          // q*q
+  double* Js = lds + q * q;  // q*q
+  __shared__ int flag;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  for (int e = tid; e < q * q; e += NT)
+  {
+    const int c = e / q, r = e % q;
+    Xs[e] = P.X[(size_t)c * P.ldx + r];
+    Js[e] = (r == c) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double tol = jac_tol(q);
+  const int qq = (q + 1) & ~1;
+  int sweep = 0;
+  for (; sweep < max_sweeps; ++sweep)
+  {
+    if (tid == 0) flag = 0;
+    __syncthreads();
+    int rot = 0;
+    for (int s = 0; s < qq - 1; ++s)
+    {
+      for (int pr = wid; pr < qq / 2; pr += NT / 64)
+      {
+        int ci, cj;
+        rr_pair(s, pr, qq, ci, cj);
+        if (ci >= q || cj >= q) continue;
+        if (ci > cj)
+        {
+          const int t = ci;
+          ci = cj;
+          cj = t;
+        }
+        rot |= rotate_pair(Xs + ci * q, Xs + cj * q, Js + ci * q, Js + cj * q, q, q, tol, lane);
+      }
+      __syncthreads();
+    }
+    if (rot && lane == 0) atomicOr(&flag, 1);
+    __syncthreads();
+    const int any = flag;
+    __syncthreads();
+    if (!any) break;
+  }
+  for (int e = tid; e < q * q; e += NT)
+  {
+    const int c = e / q, r = e % q;
+    P.X[(size_t)c * P.ldx + r] = Xs[e];
+    P.J[(size_t)c * P.ldj + r] = Js[e];
+  }
+  if (tid == 0 && P.sweeps) *P.sweeps = sweep;
+}
+
+// One round of block Jacobi: workgroup = block pair (bi, bj) of the
+// round-robin over nbb (even) blocks of W columns.
+template <int NT, int W>
+__global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int nbb, int inner_sweeps)
+{
+  const jac_prob& P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  const int nb = (q + W - 1) / W;
+  if (P.flag && P.flag[1]) return;  // converged in an earlier sweep
+  int bi, bj;
+  rr_pair(s, blockIdx.x, nbb, bi, bj);
+  if (bi >= nb || bj >= nb) return;
+  extern __shared__ double lds[];
+  double* Xs = lds;               // [2W][q]
+  double* Js = lds + 2 * W * q;   // [2W][q]
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  int ncol[2], cbeg[2];
+  cbeg[0] = bi * W;
+  cbeg[1] = bj * W;
+  ncol[0] = min(W, q - cbeg[0]);
+  ncol[1] = min(W, q - cbeg[1]);
+  // Local column l (0..2W-1) -> global column.
+  for (int e = tid; e < 2 * W * q; e += NT)
+  {
+    const int l = e / q, r = e % q;
+    const int h = l / W, cl = l % W;
+    double xv = 0.0, jv = 0.0;
+    if (cl < ncol[h])
+    {
+      const int g = cbeg[h] + cl;
+      xv = P.X[(size_t)g * P.ldx + r];
+      jv = P.J[(size_t)g * P.ldj + r];
+    }
+    Xs[e] = xv;
+    Js[e] = jv;
+  }
+  __syncthreads();
+  const double tol = jac_tol(q);
+  int rot = 0;
+  for (int it = 0; it < inner_sweeps; ++it)
+  {
+    for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+    {
+      for (int pr = wid; pr < W; pr += NT / 64)
+      {
+        int li, lj;
+        rr_pair(r2, pr, 2 * W, li, lj);
+        if (li > lj)
+        {
+          const int t = li;
+          li = lj;
+          lj = t;
+        }
+        if ((li % W) >= ncol[li / W] || (lj % W) >= ncol[lj / W]) continue;
+        rot |= rotate_pair(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, lane);
+      }
+      __syncthreads();
+    }
+  }
+  for (int e = tid; e < 2 * W * q; e += NT)
+  {
+    const int l = e / q, r = e % q;
+    const int h = l / W, cl = l % W;
+    if (cl < ncol[h])
+    {
+      const int g = cbeg[h] + cl;
+      P.X[(size_t)g * P.ldx + r] = Xs[e];
+      P.J[(size_t)g * P.ldj + r] = Js[e];
+    }
+  }
+  if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
+}
+
+// J = I for every problem; flags cleared.
+__global__ void jac_init_kernel(jac_batch bt)
+{
+  const jac_prob& P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  const long total = (long)q * q;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int c = (int)(e / q), r = (int)(e % q);
+    P.J[(size_t)c * P.ldj + r] = (r == c) ? 1.0 : 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+  {
+    if (P.flag)
+    {
+      P.flag[0] = 0;
+      P.flag[1] = 0;
+    }
+    if (P.sweeps) *P.sweeps = 0;
+  }
+}
+
+// After a sweep: flag[1] = (no rotation in the sweep); flag[0] reset;
+// counts converged problems into *done.
+__global__ void jac_sweep_end_kernel(jac_batch bt, int batch, int* done)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < batch; i += gridDim.x * blockDim.x)
+  {
+    const jac_prob& P = jpick(bt, i);
+    if (!P.flag) continue;
+    if (!P.flag[1] && P.flag[0] == 0) P.flag[1] = 1;
+    P.flag[0] = 0;
+    if (P.flag[1]) atomicAdd(done, 1);
+    if (P.sweeps && !P.flag[1]) *P.sweeps += 1;
+  }
+}
+
+// Sorted column norms and truncation rank (one workgroup per problem).
+template <int NT>
+__global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
+{
+  const jac_prob& P = jpick(bt, blockIdx.x);
+  extern __shared__ double nrm[];
+  column_norms_sorted<NT>(P.X, P.ldx, P.q, P.q, nrm, P.s, P.perm);
+  if (threadIdx.x == 0 && P.rank_out)
+  {
+    const double delta = P.delta_ptr ? *P.delta_ptr : P.delta;
+    *P.rank_out = truncate_rank(P.s, P.q, delta, P.max_rank);
+  }
+}
+}  // namespace
+
+int jacobi_block_threshold() { return 64; }
+
+void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st)
+{
+  const int batch = b.descs ? b.batch : 1;
+  if (batch <= 0 || max_q <= 0) return;
+  if (max_q <= 64)
+  {
+    constexpr int NT = 512;
+    const size_t lds = 2 * sizeof(double) * (size_t)max_q * max_q;
+    hipLaunchKernelGGL(jacobi_small_kernel<NT>, dim3(batch), dim3(NT), lds, st, b, 60);
+  }
+  else
+  {
+    // Block width: the staged 2W columns of X and J (32 W q bytes) must fit
+    // in the workgroup's LDS.
+    constexpr int NT = 1024;
+    static int max_lds = 0;
+    if (max_lds == 0)
+    {
+      int dev = 0, v = 0;
+      (void)hipGetDevice(&dev);
+      if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess || v <= 0)
+        v = 65536;
+      max_lds = v;
+    }
+    const int W = (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
+    const int nb = (max_q + W - 1) / W;
+    const int nbb = (nb + 1) & ~1;
+    const size_t lds = 4 * sizeof(double) * (size_t)W * max_q;
+    {
+      long g = ((long)max_q * max_q + 255) / 256;
+      if (g > 1024) g = 1024;
+      hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+    }
+    for (int sweep = 0; sweep < 30; ++sweep)
+    {
+      for (int s = 0; s < nbb - 1; ++s)
+      {
+        if (W == 16)
+          hipLaunchKernelGGL((bj_round_kernel<NT, 16>), dim3(nbb / 2, batch), dim3(NT), lds, st, b, s, nbb, 1);
+        else
+          hipLaunchKernelGGL((bj_round_kernel<NT, 8>), dim3(nbb / 2, batch), dim3(NT), lds, st, b, s, nbb, 1);
+      }
+      (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
+      hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+      if (sweep >= 2)
+      {
+        (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+        (void)hipStreamSynchronize(st);
+        if (*done_host >= batch) break;
+      }
+    }
+  }
+  constexpr int NF = 256;
+  hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);
+}
+
+}  // namespace tta

@@ -347,6 +347,18 @@ __global__ void scale_kernel(double* x, long n, double alpha)
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= alpha;
 }
 
+// Self-check of the DPP wave reduction against the ds_bpermute one: each
+// wave sums 64 values of x; mismatches (beyond 4 ulp-scaled) are counted.
+__global__ __launch_bounds__(256) void wave_sum_check_kernel(const double* x, int n, int* bad)
+{
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  const double v = i < n ? x[i] : 0.0;
+  const double a = wave_sum(v), b = wave_sum_shfl(v);
+  double mag = wave_sum(fabs(v));
+  if ((threadIdx.x & 63) == 0 && fabs(a - b) > 8.0 * 2.220446049250313e-16 * mag) atomicAdd(bad, 1);
+  if (fabs(a - __shfl(a, 0)) != 0.0) atomicAdd(bad, 1000);
+}
+
 int grid_for(long n, int bs = 256)
 {
   long g = (n + bs - 1) / bs;
@@ -448,6 +460,11 @@ void launch_copy2d(const double* src, int lds_, double* dst, int ldd, int rows, 
 {
   hipLaunchKernelGGL(copy2d_kernel, dim3(grid_for((long)rows * cols)), dim3(256), 0, st, src, lds_, dst, ldd, rows,
                      cols, f, keep);
+}
+
+void launch_wave_sum_check(const double* x, int n, int* bad, hipStream_t st)
+{
+  hipLaunchKernelGGL(wave_sum_check_kernel, dim3((n + 255) / 256), dim3(256), 0, st, x, n, bad);
 }
 
 void launch_scale(double* x, long n, double alpha, hipStream_t st)

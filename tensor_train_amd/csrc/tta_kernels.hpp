@@ -51,6 +51,59 @@ struct jacobi_desc
   long bsX = 0, bsJ = 0, bss = 0;
 };
 
+// One blocked-QR problem: column-major A (m x n, leading dimension lda),
+// factored in place; tau (k), T (ceil(k/16) blocks of 16 x 16), optional
+// thin Q (m x k, ldq), k = min(m, n).
+struct qrp
+{
+  double* A = nullptr;
+  int lda = 0, m = 0, n = 0;
+  double* tau = nullptr;
+  double* T = nullptr;
+  double* Q = nullptr;
+  int ldq = 0;
+};
+
+// A batch of problems: `descs` (device array of `batch` entries) or, when
+// null, the single problem `one`.  max_m / max_n bound every problem.
+struct qr_batch
+{
+  const qrp* descs = nullptr;
+  qrp one;
+  int batch = 1;
+  int max_m = 0, max_n = 0;
+  bool form_q = false;
+};
+
+// One square Jacobi SVD problem: X (q x q, column-major, ldx) is replaced by
+// W = X J (orthogonal columns), J (q x q, ldj) accumulates the rotations.
+// Then s (q, descending column norms of W), perm (their order) and, when
+// rank_out is set, the truncation rank for delta (or *delta_ptr).
+// flag: 2 device ints (per-sweep rotation flag, converged flag); sweeps:
+// optional device counter of sweeps that rotated.
+struct jac_prob
+{
+  double* X = nullptr;
+  int ldx = 0, q = 0;
+  double* J = nullptr;
+  int ldj = 0;
+  double* s = nullptr;
+  int* perm = nullptr;
+  int* rank_out = nullptr;
+  const double* delta_ptr = nullptr;
+  double delta = 0.0;
+  int max_rank = 0;
+  int* flag = nullptr;
+  int* sweeps = nullptr;
+};
+
+struct jac_batch
+{
+  const jac_prob* descs = nullptr;
+  jac_prob one;
+  int batch = 1;
+};
+
 struct permute_desc
 {
   const double* src = nullptr;
@@ -60,6 +113,10 @@ struct permute_desc
 };
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
+void launch_qr_blocked(const qr_batch& b, hipStream_t st);
+int qr_blocked_max_rows();
+// done_dev: one device int of scratch; done_host: one pinned host int.
+void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st);
 void launch_qr(const qr_desc& q, int batch, hipStream_t st);
 void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
 void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
@@ -80,5 +137,6 @@ void launch_balance(double* x, int r, int n, int rp, const double* nl, const dou
 void launch_copy2d(const double* src, int lds, double* dst, int ldd, int rows, int cols, double f, double keep,
                    hipStream_t st);
 void launch_scale(double* x, long n, double alpha, hipStream_t st);
+void launch_wave_sum_check(const double* x, int n, int* bad, hipStream_t st);
 
 }  // namespace tta

@@ -1,0 +1,400 @@
+// Blocked Householder QR (compact WY) of batches of column-major matrices.
+//
+// Per panel of QB = 16 columns:
+//   * qr_panel_kernel  -- one workgroup per matrix; the panel lives in
+//     registers (thread t owns rows t, t + NT, ...), unblocked Householder
+//     with the dlarfg sign convention, then the triangular T of the block
+//     reflector H = H_0 ... H_{QB-1} = I - V T V^T (LAPACK dlarft, forward,
+//     columnwise) from the Gram V^T V.
+//   * qr_update_kernel -- many workgroups per matrix (CT = 16 columns each):
+//     C <- (I - V T^T V^T) C on the trailing columns (factorisation) or
+//     C <- (I - V T V^T) C on the thin Q (formation, panels in reverse).
+//     V is staged through LDS in 128-row chunks.
+// The thin Q (m x k, k = min(m, n)) is formed from Q = I by the same update
+// kernel (LAPACK dorgqr order).
+#include "tta_device.hpp"
+#include "tta_kernels.hpp"
+
+namespace tta
+{
+using namespace dev;
+
+namespace
+{
+constexpr int QB = 16;   // panel width
+constexpr int QCT = 8;   // columns per update workgroup (2 per wave)
+constexpr int QCPW = QCT / 4;
+constexpr int QRC = 128; // rows per V chunk in the update kernel
+
+__device__ __forceinline__ const qrp& pick(const qr_batch& b)
+{
+  return b.descs ? b.descs[blockIdx.y] : b.one;
+}
+
+// Sum v[c] for c in [lo, hi) across the block; results to out[c] (LDS),
+// visible to all threads on return.  `red`: LDS of (NT/64) * NV doubles.
+// The loop over c is unrolled so the register array is indexed statically.
+template <int NT, int NV>
+__device__ __forceinline__ void block_vsum(const double (&v)[NV], int lo, int hi, double* red, double* out)
+{
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int c = 0; c < NV; ++c)
+  {
+    if (c >= lo && c < hi)
+    {
+      const double s = wave_sum(v[c]);
+      if (lane == 0) red[wid * NV + c] = s;
+    }
+  }
+  __syncthreads();
+  for (int c = lo + (int)threadIdx.x; c < hi; c += NT)
+  {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < NT / 64; ++w) s += red[w * NV + c];
+    out[c] = s;
+  }
+  __syncthreads();
+}
+
+template <int NT, int RPT>
+__global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
+{
+  const qrp& P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
+  const int m = P.m, n = P.n;
+  const int k = m < n ? m : n;
+  if (j0 >= k) return;
+  const int bw = (k - j0) < QB ? (k - j0) : QB;
+  const int len = m - j0;
+  const int lda = P.lda;
+  double* A = P.A;
+  __shared__ double red[(NT / 64) * QB];
+  __shared__ double bc[QB * QB / 2 + QB];
+  __shared__ double Tl[QB][QB];
+  __shared__ double taus[QB];
+  __shared__ double sc[2];
+  const int tid = threadIdx.x;
+
+  double x[RPT][QB];
+#pragma unroll
+  for (int i = 0; i < RPT; ++i)
+  {
+    const int r = tid + NT * i;
+#pragma unroll
+    for (int c = 0; c < QB; ++c) x[i][c] = (r < len && c < bw) ? A[(size_t)(j0 + c) * lda + j0 + r] : 0.0;
+  }
+  if (tid < QB) taus[tid] = 0.0;
+  __syncthreads();
+  for (int jj = 0; jj < bw; ++jj)
+  {
+    // Column jj of the rows this thread owns (static indices + selects).
+    double xj[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+    {
+      xj[i] = 0.0;
+#pragma unroll
+      for (int c = 0; c < QB; ++c)
+        if (c == jj) xj[i] = x[i][c];
+    }
+    double acc[1] = {0.0};
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+    {
+      const int r = tid + NT * i;
+      if (r > jj && r < len) acc[0] += xj[i] * xj[i];
+      if (r == jj) sc[1] = xj[i];
+    }
+    block_vsum<NT, 1>(acc, 0, 1, red, sc);
+    const double tail = sc[0];
+    const double alpha = sc[1];
+    double tj = 0.0, scal = 0.0, beta = alpha;
+    if (tail > DBL_MIN)
+    {
+      beta = sqrt(alpha * alpha + tail);
+      if (alpha >= 0.0) beta = -beta;
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (tid == 0) taus[jj] = tj;
+    double vv[RPT];
+#pragma unroll
+    for (int i = 0; i < RPT; ++i)
+    {
+      const int r = tid + NT * i;
+      vv[i] = (r == jj) ? 1.0 : ((r > jj && r < len) ? xj[i] * scal : 0.0);
+    }
+    double part[QB];
+#pragma unroll
+    for (int c = 0; c < QB; ++c)
+    {
+      part[c] = 0.0;
+#pragma unroll
+      for (int i = 0; i < RPT; ++i) part[c] += vv[i] * x[i][c];
+    }
+    if (jj + 1 < bw) block_vsum<NT, QB>(part, jj + 1, bw, red, bc);
+#pragma unroll
+    for (int c = 0; c < QB; ++c)
+    {
+      if (c > jj && c < bw)
+      {
+        const double w = tj * bc[c];
+#pragma unroll
+        for (int i = 0; i < RPT; ++i) x[i][c] -= w * vv[i];
+      }
+      else if (c == jj)
+      {
+#pragma unroll
+        for (int i = 0; i < RPT; ++i)
+        {
+          const int r = tid + NT * i;
+          if (r == jj) x[i][c] = beta;
+          else if (r > jj) x[i][c] = vv[i];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // Store the panel (R on/above the diagonal, V below) and tau.
+#pragma unroll
+  for (int i = 0; i < RPT; ++i)
+  {
+    const int r = tid + NT * i;
+    if (r < len)
+    {
+#pragma unroll
+      for (int c = 0; c < QB; ++c)
+        if (c < bw) A[(size_t)(j0 + c) * lda + j0 + r] = x[i][c];
+    }
+  }
+  if (tid < bw) P.tau[j0 + tid] = taus[tid];
+  // Gram V^T V (strict upper part, packed) for T: one wave per pair, read
+  // back from global memory (the panel was just stored by this workgroup).
+  __syncthreads();
+  {
+    const int lane = tid & 63, wid = tid >> 6;
+    for (int idx = wid; idx < QB * (QB - 1) / 2; idx += NT / 64)
+    {
+      int a = 0, rem = idx;
+      while (rem >= QB - 1 - a)
+      {
+        rem -= QB - 1 - a;
+        ++a;
+      }
+      const int b = a + 1 + rem;
+      double s = 0.0;
+      if (b < bw)
+      {
+        const double* Va = A + (size_t)(j0 + a) * lda + j0;
+        const double* Vb = A + (size_t)(j0 + b) * lda + j0;
+        for (int r = b + lane; r < len; r += 64) s += ((r == b) ? 1.0 : Vb[r]) * Va[r];
+      }
+      s = wave_sum(s);
+      if (lane == 0) bc[idx] = s;
+    }
+  }
+  __syncthreads();
+  // T of the block reflector (LAPACK dlarft, forward, columnwise):
+  //   T[a][a] = tau_a,  T[a][j] = -tau_j * sum_{b=a}^{j-1} T[a][b] (V^T V)[b][j].
+  // Row a only needs row a, so thread a builds its row in registers.
+  if (tid < QB)
+  {
+    const int a = tid;
+    double row[QB];
+#pragma unroll
+    for (int j = 0; j < QB; ++j) row[j] = 0.0;
+    if (a < bw)
+    {
+#pragma unroll
+      for (int j = 0; j < QB; ++j)
+      {
+        if (j == a) row[j] = taus[a];
+        if (j > a && j < bw)
+        {
+          double sacc = 0.0;
+#pragma unroll
+          for (int bb = 0; bb < QB; ++bb)
+            if (bb >= a && bb < j) sacc += row[bb] * bc[bb * QB - bb * (bb + 1) / 2 + (j - bb - 1)];
+          row[j] = -taus[j] * sacc;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < QB; ++j) Tl[a][j] = row[j];
+  }
+  __syncthreads();
+  double* Tout = P.T + (size_t)(j0 / QB) * QB * QB;
+  for (int e = tid; e < QB * QB; e += NT) Tout[e] = Tl[e / QB][e % QB];
+}
+
+// C <- (I - V op(T) V^T) C for columns [cbeg, cend) of C, rows j0 .. m-1.
+// mode 0: factorisation (C = A, op(T) = T^T, cbeg = j0 + bw, cend = n).
+// mode 1: Q formation (C = Q, op(T) = T, cbeg = j0, cend = k).
+__global__ __launch_bounds__(256) void qr_update_kernel(qr_batch bt, int j0, int mode)
+{
+  const qrp& P = pick(bt);
+  const int m = P.m, n = P.n;
+  const int k = m < n ? m : n;
+  if (j0 >= k) return;
+  const int bw = (k - j0) < QB ? (k - j0) : QB;
+  const int len = m - j0;
+  const int cbeg = mode == 0 ? j0 + bw : j0;
+  const int cend = mode == 0 ? n : k;
+  const int c0 = cbeg + blockIdx.x * QCT;
+  if (c0 >= cend) return;
+  const int lda = P.lda;
+  const double* A = P.A;
+  double* C = mode == 0 ? P.A : P.Q;
+  const int ldc = mode == 0 ? lda : P.ldq;
+  __shared__ double Vs[QRC][QB + 1];
+  __shared__ double Ts[QB][QB];
+  __shared__ double Ws[QB][QCT];
+  __shared__ double W2[QB][QCT];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double* Tg = P.T + (size_t)(j0 / QB) * QB * QB;
+  Ts[tid / QB][tid % QB] = Tg[tid];
+  // Each wave owns QCPW columns.
+  double acc[QCPW][QB];
+#pragma unroll
+  for (int u = 0; u < QCPW; ++u)
+#pragma unroll
+    for (int jj = 0; jj < QB; ++jj) acc[u][jj] = 0.0;
+  for (int r0 = 0; r0 < len; r0 += QRC)
+  {
+    __syncthreads();
+    for (int e = tid; e < QRC * QB; e += 256)
+    {
+      const int rr = e % QRC, jj = e / QRC;
+      const int r = r0 + rr;
+      double v = 0.0;
+      if (jj < bw && r < len) v = (r == jj) ? 1.0 : (r > jj ? A[(size_t)(j0 + jj) * lda + j0 + r] : 0.0);
+      Vs[rr][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < QCPW; ++u)
+    {
+      const int c = c0 + wid * QCPW + u;
+      if (c >= cend) continue;
+      const double* col = C + (size_t)c * ldc + j0;
+      for (int rr = lane; rr < QRC; rr += 64)
+      {
+        const int r = r0 + rr;
+        if (r >= len) break;
+        const double a = col[r];
+#pragma unroll
+        for (int jj = 0; jj < QB; ++jj) acc[u][jj] = fma(Vs[rr][jj], a, acc[u][jj]);
+      }
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < QCPW; ++u)
+#pragma unroll
+    for (int jj = 0; jj < QB; ++jj)
+    {
+      const double s = wave_sum(acc[u][jj]);
+      if (lane == 0) Ws[jj][wid * QCPW + u] = s;
+    }
+  __syncthreads();
+  if (tid < QB * QCT)
+  {
+    const int jj = tid / QCT, cc = tid % QCT;
+    double s = 0.0;
+    if (mode == 0)
+    {
+#pragma unroll
+      for (int a = 0; a < QB; ++a) s += Ts[a][jj] * Ws[a][cc];
+    }
+    else
+    {
+#pragma unroll
+      for (int a = 0; a < QB; ++a) s += Ts[jj][a] * Ws[a][cc];
+    }
+    W2[jj][cc] = s;
+  }
+  for (int r0 = 0; r0 < len; r0 += QRC)
+  {
+    __syncthreads();
+    for (int e = tid; e < QRC * QB; e += 256)
+    {
+      const int rr = e % QRC, jj = e / QRC;
+      const int r = r0 + rr;
+      double v = 0.0;
+      if (jj < bw && r < len) v = (r == jj) ? 1.0 : (r > jj ? A[(size_t)(j0 + jj) * lda + j0 + r] : 0.0);
+      Vs[rr][jj] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < QCPW; ++u)
+    {
+      const int c = c0 + wid * QCPW + u;
+      if (c >= cend) continue;
+      double* col = C + (size_t)c * ldc + j0;
+      const int cc = wid * QCPW + u;
+      for (int rr = lane; rr < QRC; rr += 64)
+      {
+        const int r = r0 + rr;
+        if (r >= len) break;
+        double s = 0.0;
+#pragma unroll
+        for (int jj = 0; jj < QB; ++jj) s = fma(Vs[rr][jj], W2[jj][cc], s);
+        col[r] -= s;
+      }
+    }
+  }
+}
+
+// Q = I[:, :k] (m x k, leading dimension ldq).
+__global__ void qr_eye_kernel(qr_batch bt)
+{
+  const qrp& P = pick(bt);
+  const int m = P.m, k = P.m < P.n ? P.m : P.n;
+  const long total = (long)m * k;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int c = (int)(e / m), r = (int)(e % m);
+    P.Q[(size_t)c * P.ldq + r] = (r == c) ? 1.0 : 0.0;
+  }
+}
+}  // namespace
+
+int qr_blocked_max_rows() { return 4 * 512; }
+
+void launch_qr_blocked(const qr_batch& b, hipStream_t st)
+{
+  const int batch = b.descs ? b.batch : 1;
+  if (batch <= 0) return;
+  const int kmax = b.max_m < b.max_n ? b.max_m : b.max_n;
+  constexpr int NT = 512;
+  for (int j0 = 0; j0 < kmax; j0 += QB)
+  {
+    if (b.max_m <= NT)
+      hipLaunchKernelGGL((qr_panel_kernel<NT, 1>), dim3(batch), dim3(NT), 0, st, b, j0);
+    else if (b.max_m <= 2 * NT)
+      hipLaunchKernelGGL((qr_panel_kernel<NT, 2>), dim3(batch), dim3(NT), 0, st, b, j0);
+    else
+      hipLaunchKernelGGL((qr_panel_kernel<NT, 4>), dim3(batch), dim3(NT), 0, st, b, j0);
+    const int ncols = b.max_n - j0 - 1;
+    if (ncols > 0)
+    {
+      dim3 grid((ncols + QCT - 1) / QCT, batch);
+      hipLaunchKernelGGL(qr_update_kernel, grid, dim3(256), 0, st, b, j0, 0);
+    }
+  }
+  if (b.descs ? b.form_q : b.one.Q != nullptr)
+  {
+    const long total = (long)b.max_m * kmax;
+    long g = (total + 255) / 256;
+    if (g > 4096) g = 4096;
+    hipLaunchKernelGGL(qr_eye_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+    const int last = ((kmax - 1) / QB) * QB;
+    for (int j0 = last; j0 >= 0; j0 -= QB)
+    {
+      dim3 grid((kmax - j0 + QCT - 1) / QCT, batch);
+      hipLaunchKernelGGL(qr_update_kernel, grid, dim3(256), 0, st, b, j0, 1);
+    }
+  }
+}
+
+}  // namespace tta

@@ -1,0 +1,177 @@
+// Drop-in check of include/tensor_train.hpp: the assertions of the
+// reference's hot-path C++ tests (bdaszuta/tensor_train tests/core/
+// test_tt_round.cpp, test_tt_invariants.cpp, test_tt_orthogonalize.cpp,
+// test_tt_matrix_apply_round.cpp, test_tt_ops.cpp, test_tt_validation.cpp),
+// written against the mva::tensor_train API and run on the GPU.  Exit code =
+// number of failed checks.
+#include <cmath>
+#include <cstdio>
+#include <random>
+#include <vector>
+
+#include "tensor_train.hpp"
+
+namespace tt = mva::tensor_train;
+
+static int failures = 0;
+#define CHECK(cond, msg)                                   \
+  do                                                       \
+  {                                                        \
+    if (!(cond))                                           \
+    {                                                      \
+      std::printf("FAIL: %s (%s:%d)\n", msg, __FILE__, __LINE__); \
+      ++failures;                                          \
+    }                                                      \
+    else                                                   \
+      std::printf("ok:   %s\n", msg);                      \
+  } while (0)
+
+static double max_abs_diff(const std::vector<double>& a, const std::vector<double>& b)
+{
+  double m = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) m = std::fmax(m, std::fabs(a[i] - b[i]));
+  return m;
+}
+
+static double frob(const std::vector<double>& a)
+{
+  double s = 0.0;
+  for (double v : a) s += v * v;
+  return std::sqrt(s);
+}
+
+static tt::round_options opts_svd(double eps, int mr = 0)
+{
+  tt::round_options o;
+  o.eps = eps;
+  o.max_rank = mr;
+  return o;
+}
+
+static bool left_orthogonal(const tt::tt_core& c, double tol)
+{
+  const int rows = c.r_left() * c.n_phys(), cols = c.r_right();
+  double err = 0.0;
+  for (int a = 0; a < cols; ++a)
+    for (int b = 0; b < cols; ++b)
+    {
+      double s = 0.0;
+      for (int i = 0; i < rows; ++i) s += c.data()[(size_t)i * cols + a] * c.data()[(size_t)i * cols + b];
+      s -= (a == b);
+      err += s * s;
+    }
+  return std::sqrt(err) <= tol * cols;
+}
+
+static bool right_orthogonal(const tt::tt_core& c, double tol)
+{
+  const int rows = c.r_left(), cols = c.n_phys() * c.r_right();
+  double err = 0.0;
+  for (int a = 0; a < rows; ++a)
+    for (int b = 0; b < rows; ++b)
+    {
+      double s = 0.0;
+      for (int i = 0; i < cols; ++i) s += c.data()[(size_t)a * cols + i] * c.data()[(size_t)b * cols + i];
+      s -= (a == b);
+      err += s * s;
+    }
+  return std::sqrt(err) <= tol * rows;
+}
+
+int main()
+{
+  // Config 1: round(add(add(a, b), a), 1e-10) recovers 2a + b at ranks <= 16.
+  {
+    const std::vector<int> shape(10, 4);
+    auto a = tt::random(shape, 8, 1);
+    auto b = tt::random(shape, 8, 2);
+    auto c = tt::add(tt::add(a, b), a);
+    auto r = tt::round(c, opts_svd(1e-10));
+    const std::vector<int> want = {1, 4, 16, 16, 16, 16, 16, 16, 16, 4, 1};
+    CHECK(r.ranks() == want, "config 1: ranks [1,4,16,...,16,4,1]");
+    auto da = a.to_dense(), db = b.to_dense(), dr = r.to_dense();
+    std::vector<double> ref(da.size());
+    for (size_t i = 0; i < da.size(); ++i) ref[i] = 2.0 * da[i] + db[i];
+    std::vector<double> diff(ref.size());
+    for (size_t i = 0; i < ref.size(); ++i) diff[i] = dr[i] - ref[i];
+    CHECK(frob(diff) / frob(ref) <= 1e-10, "config 1: rel. error <= 1e-10");
+    tt::round_result info;
+    tt::round(c, opts_svd(1e-10), &info);
+    CHECK(info.iters_run == 1 && info.converged && info.final_resid == 0.0, "round_result {1, 0, true}");
+  }
+  // test_tt_invariants.cpp: gauge after round, right_orthogonalize norm, monotone norm.
+  {
+    auto r = tt::round(tt::random({3, 4, 5, 3}, 6, 42), opts_svd(1e-10));
+    bool ok = true;
+    for (int k = 0; k < r.d() - 1; ++k) ok = ok && left_orthogonal(r.core(k), 1e-13);
+    CHECK(ok, "round: cores[0..d-2] left-orthogonal");
+    auto a = tt::random({3, 4, 5, 3}, 6, 99);
+    const double n1 = tt::norm(a);
+    tt::right_orthogonalize(a);
+    ok = true;
+    for (int k = 1; k < a.d(); ++k) ok = ok && right_orthogonal(a.core(k), 1e-13);
+    CHECK(ok, "right_orthogonalize: cores[1..d-1] right-orthogonal");
+    CHECK(std::fabs(tt::norm(a) - n1) <= 1e-13 * (n1 + 1.0), "right_orthogonalize: norm preserved");
+    auto b = tt::random({2, 3, 4, 2}, 8, 7);
+    CHECK(tt::norm(tt::round(b, opts_svd(1e-2))) <= tt::norm(b) * (1 + 1e-14), "round: norm non-increasing");
+    auto r3 = tt::round(b, opts_svd(0.0, 3));
+    CHECK(r3.max_rank() <= 3, "round: max_rank cap");
+    auto rr = tt::round(tt::random({2, 3, 4, 2, 3}, 6, 123), opts_svd(1e-6));
+    auto rr2 = tt::round(rr, opts_svd(1e-6));
+    CHECK(max_abs_diff(rr.to_dense(), rr2.to_dense()) <= 1e-13 * (frob(rr.to_dense()) + 1), "round idempotence");
+    auto z = tt::round(tt::zeros({3, 4, 2}), opts_svd(1e-10));
+    CHECK(z.ranks() == std::vector<int>({1, 1, 1, 1}) && frob(z.to_dense()) == 0.0, "round(zeros): rank-1 zero");
+    auto o7 = tt::ones({7});
+    CHECK(max_abs_diff(tt::round(o7, opts_svd(1e-10)).to_dense(), o7.to_dense()) == 0.0, "round(d=1) identity");
+  }
+  // right canonical gauge.
+  {
+    tt::round_options o = opts_svd(1e-10);
+    o.gauge = tt::round_gauge::right_canonical;
+    auto r = tt::round(tt::random({3, 4, 5, 3}, 6, 42), o);
+    bool ok = true;
+    for (int k = 1; k < r.d(); ++k) ok = ok && right_orthogonal(r.core(k), 1e-13);
+    CHECK(ok, "round(right_canonical): cores[1..d-1] right-orthogonal");
+  }
+  // test_tt_matrix_apply_round.cpp: matvec_round ~ matvec; streaming vs naive.
+  {
+    auto A = tt::random({2, 3, 2, 3}, {3, 2, 2, 2}, 4, 11);
+    auto x = tt::random({3, 2, 2, 2}, 5, 12);
+    auto y = tt::matvec(A, x);
+    auto yr = tt::matvec_round(A, x, opts_svd(1e-12));
+    const auto dy = y.to_dense();
+    std::vector<double> diff(dy.size());
+    auto dyr = yr.to_dense();
+    for (size_t i = 0; i < dy.size(); ++i) diff[i] = dyr[i] - dy[i];
+    CHECK(frob(diff) <= 1e-9 * frob(dy), "matvec_round ~ matvec");
+    tt::round_options so = opts_svd(1e-6);
+    so.method = tt::round_method::svd_streaming;
+    auto ys = tt::matvec_round(A, x, so);
+    auto yn = tt::matvec_round(A, x, opts_svd(1e-6));
+    auto ds = ys.to_dense(), dn = yn.to_dense();
+    for (size_t i = 0; i < dy.size(); ++i) diff[i] = ds[i] - dn[i];
+    CHECK(frob(diff) <= 10 * 1e-6 * frob(dy), "streaming vs naive within 10 eps ||y||");
+    auto B = tt::random({3, 2, 2, 2}, {2, 2, 3, 2}, 3, 13);
+    auto C = tt::matmat_round(A, B, opts_svd(1e-10));
+    auto Cn = tt::matmat(A, B);
+    CHECK(std::fabs(tt::frob_norm(C) - tt::frob_norm(Cn)) <= 1e-8 * tt::frob_norm(Cn), "matmat_round ~ matmat");
+  }
+  // test_tt_ops.cpp: round(add(A, A)) = 2A, ranks <= ranks(A).
+  {
+    auto a = tt::random({3, 4, 3, 2}, 5, 77);
+    auto r = tt::round(tt::add(a, a), opts_svd(1e-12));
+    auto ra = a.ranks(), rr = r.ranks();
+    bool ok = true;
+    for (size_t k = 0; k < ra.size(); ++k) ok = ok && rr[k] <= ra[k];
+    auto da = a.to_dense(), dr = r.to_dense();
+    for (auto& v : da) v *= 2.0;
+    CHECK(ok && max_abs_diff(da, dr) <= 1e-9, "round(add(A, A)) = 2A, ranks <= ranks(A)");
+  }
+  // test_tt_validation.cpp: d = 0 conventions.
+  {
+    tt::tt e;
+    CHECK(e.ranks() == std::vector<int>({1}) && e.to_dense() == std::vector<double>({1.0}), "d=0 conventions");
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures;
+}
