@@ -32,39 +32,73 @@ __device__ __forceinline__ const jac_prob& jpick(const jac_batch& b, int idx)
 
 __device__ __forceinline__ double jac_tol(int q) { return fmax(1.0e-15, (double)q * 2.220446049250313e-16); }
 
+// Sums of three values over the wave, DPP stages interleaved so the three
+// chains overlap.
+__device__ __forceinline__ void wave_sum3(double& a, double& b, double& g)
+{
+  a += dpp_mov<0xB1>(a);
+  b += dpp_mov<0xB1>(b);
+  g += dpp_mov<0xB1>(g);
+  a += dpp_mov<0x4E>(a);
+  b += dpp_mov<0x4E>(b);
+  g += dpp_mov<0x4E>(g);
+  a += dpp_mov<0x124>(a);
+  b += dpp_mov<0x124>(b);
+  g += dpp_mov<0x124>(g);
+  a += dpp_mov<0x128>(a);
+  b += dpp_mov<0x128>(b);
+  g += dpp_mov<0x128>(g);
+  a = (read_lane(a, 0) + read_lane(a, 16)) + (read_lane(a, 32) + read_lane(a, 48));
+  b = (read_lane(b, 0) + read_lane(b, 16)) + (read_lane(b, 32) + read_lane(b, 48));
+  g = (read_lane(g, 0) + read_lane(g, 16)) + (read_lane(g, 32) + read_lane(g, 48));
+}
+
 // Rotate columns (xi, xj) of length p and (ji, jj) of length q in LDS by the
 // Jacobi rotation that orthogonalises xi and xj.  One wave; returns 1 when a
-// rotation was applied.
+// rotation was applied.  PL = elements per lane held in registers (p and q
+// must be <= 64 * PL).
+template <int PL>
 __device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, double* jj, int p, int q, double tol,
                                            int lane)
 {
+  double u[PL], v[PL];
   double a = 0.0, b = 0.0, g = 0.0;
-  for (int t = lane; t < p; t += kWave)
+#pragma unroll
+  for (int e = 0; e < PL; ++e)
   {
-    const double u = xi[t], v = xj[t];
-    a = fma(u, u, a);
-    b = fma(v, v, b);
-    g = fma(u, v, g);
+    const int t = lane + 64 * e;
+    u[e] = t < p ? xi[t] : 0.0;
+    v[e] = t < p ? xj[t] : 0.0;
+    a = fma(u[e], u[e], a);
+    b = fma(v[e], v[e], b);
+    g = fma(u[e], v[e], g);
   }
-  a = wave_sum(a);
-  b = wave_sum(b);
-  g = wave_sum(g);
+  wave_sum3(a, b, g);
   if (g == 0.0 || fabs(g) <= tol * sqrt(a * b)) return 0;
   const double zeta = (b - a) / (2.0 * g);
   const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
   const double cs = 1.0 / sqrt(1.0 + tt * tt);
   const double sn = cs * tt;
-  for (int t = lane; t < p; t += kWave)
+#pragma unroll
+  for (int e = 0; e < PL; ++e)
   {
-    const double u = xi[t], v = xj[t];
-    xi[t] = cs * u - sn * v;
-    xj[t] = sn * u + cs * v;
+    const int t = lane + 64 * e;
+    if (t < p)
+    {
+      xi[t] = cs * u[e] - sn * v[e];
+      xj[t] = sn * u[e] + cs * v[e];
+    }
   }
-  for (int t = lane; t < q; t += kWave)
+#pragma unroll
+  for (int e = 0; e < PL; ++e)
   {
-    const double u = ji[t], v = jj[t];
-    ji[t] = cs * u - sn * v;
-    jj[t] = sn * u + cs * v;
+    const int t = lane + 64 * e;
+    if (t < q)
+    {
+      const double y1 = ji[t], y2 = jj[t];
+      ji[t] = cs * y1 - sn * y2;
+      jj[t] = sn * y1 + cs * y2;
+    }
   }
   return 1;
 }
@@ -108,7 +142,7 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
           ci = cj;
           cj = t;
         }
-        rot |= rotate_pair(Xs + ci * q, Xs + cj * q, Js + ci * q, Js + cj * q, q, q, tol, lane);
+        rot |= rotate_pair<1>(Xs + ci * q, Xs + cj * q, Js + ci * q, Js + cj * q, q, q, tol, lane);
       }
       __syncthreads();
     }
@@ -129,7 +163,7 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
 
 // One round of block Jacobi: workgroup = block pair (bi, bj) of the
 // round-robin over nbb (even) blocks of W columns.
-template <int NT, int W>
+template <int NT, int W, int PL>
 __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int nbb, int inner_sweeps)
 {
   const jac_prob& P = jpick(bt, blockIdx.y);
@@ -181,7 +215,7 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
           lj = t;
         }
         if ((li % W) >= ncol[li / W] || (lj % W) >= ncol[lj / W]) continue;
-        rot |= rotate_pair(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, lane);
+        rot |= rotate_pair<PL>(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, lane);
       }
       __syncthreads();
     }
@@ -278,6 +312,28 @@ void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_h
         v = 65536;
       max_lds = v;
     }
+    if ((size_t)32 * 8 * max_q > (size_t)max_lds || max_q > 640)
+    {
+      // Too tall to stage a block pair: single-workgroup Jacobi in global memory.
+      for (int i = 0; i < batch && !b.descs; ++i)
+      {
+        jacobi_desc jd;
+        jd.X = b.one.X;
+        jd.ldx = b.one.ldx;
+        jd.p = b.one.q;
+        jd.q = b.one.q;
+        jd.J = b.one.J;
+        jd.ldj = b.one.ldj;
+        jd.s = b.one.s;
+        jd.perm = b.one.perm;
+        jd.rank_out = b.one.rank_out;
+        jd.delta_ptr = b.one.delta_ptr;
+        jd.delta = b.one.delta;
+        jd.max_rank = b.one.max_rank;
+        launch_jacobi(jd, 1, st);
+      }
+      return;
+    }
     const int W = (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
@@ -291,10 +347,15 @@ void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_h
     {
       for (int s = 0; s < nbb - 1; ++s)
       {
-        if (W == 16)
-          hipLaunchKernelGGL((bj_round_kernel<NT, 16>), dim3(nbb / 2, batch), dim3(NT), lds, st, b, s, nbb, 1);
+        const dim3 grid(nbb / 2, batch);
+        if (W == 16 && max_q <= 128)
+          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+        else if (W == 16 && max_q <= 256)
+          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+        else if (W == 16)
+          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 5>), grid, dim3(NT), lds, st, b, s, nbb, 1);
         else
-          hipLaunchKernelGGL((bj_round_kernel<NT, 8>), dim3(nbb / 2, batch), dim3(NT), lds, st, b, s, nbb, 1);
+          hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1);
       }
       (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
       hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);

@@ -79,3 +79,23 @@ def test_wave_sum_dpp_matches_bpermute():
     ctx.check(ctx.lib.tta_selfcheck_wave_sum(ctx.h, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(x),
                                              ctypes.byref(bad)))
     assert bad.value == 0
+
+
+@pytest.mark.parametrize("decades", [4, 12, 20])
+def test_svd_graded_large_dynamic_range(decades):
+    # Block Jacobi with Gram inner solves on spectra spanning many decades,
+    # plus an exactly rank-deficient block: backward-stable accuracy
+    # (absolute errors ~ u * s_max, as LAPACK and Eigen's BDCSVD give).
+    rng = np.random.default_rng(decades)
+    q = 256
+    U0, _ = np.linalg.qr(rng.standard_normal((q, q)))
+    V0, _ = np.linalg.qr(rng.standard_normal((q, q)))
+    sv = 10.0 ** (-decades * np.arange(q) / (q - 1))
+    sv[200:] = 0.0
+    M = (U0 * sv) @ V0.T
+    U, s, Vt = T.svd_thin(M)
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert np.abs(s - s_ref).max() <= 1e-13
+    assert np.abs(U.T @ U - np.eye(q)).max() <= 1e-12
+    keep = s > 1e-12
+    assert np.abs((U[:, keep] * s[keep]) @ Vt[keep] - M).max() <= 1e-13
