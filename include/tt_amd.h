@@ -123,6 +123,20 @@ int tta_axpby(tta_ctx* ctx, double alpha, const tta_tt* a, double beta, const tt
 /* scale(a, alpha) (tt_ops.hpp:35-50). */
 int tta_scale(tta_ctx* ctx, const tta_tt* a, double alpha, tta_tt** out);
 
+/* ---- batched rounding --------------------------------------------------- */
+enum tta_precision
+{
+  TTA_PREC_FP64 = 0,   /* plain: the reference algorithm in FP64 */
+  TTA_PREC_MIXED = 1,  /* FP32 contractions, FP64 Gram / Cholesky / small SVD */
+  TTA_PREC_LEARNED = 2 /* learned-rank randomized rounding with plain fallback */
+};
+/* round(a_i, {eps, max_rank}) for count TTs of any d, modes and ranks
+ * (detail::round, include/core/round/tt_round.hpp:36-127, per TT), the
+ * TTs advanced in lockstep so each launch covers the batch.  out[i] are
+ * fresh tensors.  No reference counterpart for the batching itself. */
+int tta_round_batched(tta_ctx* ctx, int count, tta_tt* const* in, double eps, int max_rank, int precision,
+                      tta_tt** out);
+
 /* ---- dense kernels of the path (host buffers, row-major) --------------- */
 /* qr_thin (include/core/types/tt_eigen_bridge.hpp:42-60): Householder QR,
  * Q (rows x k) orthonormal columns, R (k x cols) upper trapezoidal,

@@ -503,3 +503,19 @@ def svd_thin(M, ctx: Context | None = None):
     ctx.check(ctx.lib.tta_svd_thin(ctx.h, M.ctypes.data_as(dp), rows, cols, U.ctypes.data_as(dp),
                                    s.ctypes.data_as(dp), Vt.ctypes.data_as(dp)))
     return U, s, Vt
+
+
+PRECISIONS = {"fp64": 0, "mixed": 1, "learned": 2}
+
+
+def round_batched(tts, eps=0.0, max_rank=0, precision="fp64"):
+    """Round a list of device TTs in one batched call (tta_round_batched)."""
+    if not tts:
+        return []
+    ctx = tts[0]._ctx
+    n = len(tts)
+    arr_in = (ctypes.c_void_p * n)(*[t._h.value for t in tts])
+    arr_out = (ctypes.c_void_p * n)()
+    ctx.check(ctx.lib.tta_round_batched(ctx.h, n, arr_in, ctypes.c_double(eps), max_rank,
+                                        _enum(PRECISIONS, precision, "precision"), arr_out))
+    return [type(tts[i])(ctypes.c_void_p(arr_out[i]), ctx) for i in range(n)]

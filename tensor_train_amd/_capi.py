@@ -47,6 +47,8 @@ SIGNATURES = {
     "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
     "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),
     "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),
+    "tta_round_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     "tta_qr_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "tta_svd_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]),
     "tta_selfcheck_wave_sum": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p]),

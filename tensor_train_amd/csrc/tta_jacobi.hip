@@ -172,7 +172,9 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   if (P.flag && P.flag[1]) return;  // converged in an earlier sweep
   int bi, bj;
   rr_pair(s, blockIdx.x, nbb, bi, bj);
-  if (bi >= nb || bj >= nb) return;
+  // A block paired with a padding block (odd block count) is still
+  // orthogonalised internally; the padding half holds no columns.
+  if (bi >= nb && bj >= nb) return;
   extern __shared__ double lds[];
   double* Xs = lds;               // [2W][q]
   double* Js = lds + 2 * W * q;   // [2W][q]
@@ -180,8 +182,8 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   int ncol[2], cbeg[2];
   cbeg[0] = bi * W;
   cbeg[1] = bj * W;
-  ncol[0] = min(W, q - cbeg[0]);
-  ncol[1] = min(W, q - cbeg[1]);
+  ncol[0] = bi < nb ? min(W, q - cbeg[0]) : 0;
+  ncol[1] = bj < nb ? min(W, q - cbeg[1]) : 0;
   // Local column l (0..2W-1) -> global column.
   for (int e = tid; e < 2 * W * q; e += NT)
   {

@@ -1,5 +1,7 @@
 // HIP kernels (gfx950) for the FP64 plain path of TT rounding and MPO
 // apply-and-round.  Launch wrappers are declared in tta_kernels.hpp.
+#include <algorithm>
+
 #include "tta_device.hpp"
 #include "tta_kernels.hpp"
 
@@ -19,16 +21,13 @@ namespace
 {
 constexpr int GT = 64, GK = 16;
 
-__global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
+__device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, const double* B, double* C)
 {
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
   const int tid = threadIdx.x;
   const int tx = tid & 15, ty = tid >> 4;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
-  const double* A = g.A + (size_t)blockIdx.z * g.bsa;
-  const double* B = g.B + (size_t)blockIdx.z * g.bsb;
-  double* C = g.C + (size_t)blockIdx.z * g.bsc;
   const bool a_kc = (g.sac == 1);  // A contiguous along k
   const bool b_jc = (g.sbc == 1);  // B contiguous along j
   double acc[4][4];
@@ -99,6 +98,20 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
       *c = (g.beta == 0.0) ? g.alpha * acc[u][v] : g.alpha * acc[u][v] + g.beta * (*c);
     }
   }
+}
+
+__global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
+{
+  gemm_tile(g, g.A + (size_t)blockIdx.z * g.bsa, g.B + (size_t)blockIdx.z * g.bsb, g.C + (size_t)blockIdx.z * g.bsc);
+}
+
+// Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
+// M x N exit.
+__global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
+{
+  const gemm_desc& g = gs[blockIdx.z];
+  if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
+  gemm_tile(g, g.A, g.B, g.C);
 }
 
 // Householder QR (+ optional thin Q) of one column-major matrix per block.
@@ -359,6 +372,119 @@ __global__ __launch_bounds__(256) void wave_sum_check_kernel(const double* x, in
   if (fabs(a - __shfl(a, 0)) != 0.0) atomicAdd(bad, 1000);
 }
 
+// ---- descriptor-batched helpers (problem = blockIdx.y / blockIdx.z) ----
+__global__ __launch_bounds__(256) void transpose_batched_kernel(const tr_desc* ds)
+{
+  const tr_desc& d = ds[blockIdx.z];
+  __shared__ double tile[32][33];
+  const int bi = blockIdx.y * 32, bj = blockIdx.x * 32;
+  if (bi >= d.rows || bj >= d.cols) return;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int i = bi + r, j = bj + tx;
+    if (i < d.rows && j < d.cols) tile[r][tx] = d.in[(size_t)i * d.ldi + j];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8)
+  {
+    const int j = bj + r, i = bi + tx;
+    if (i < d.rows && j < d.cols) d.out[(size_t)j * d.ldo + i] = tile[tx][r];
+  }
+}
+
+__global__ void extract_upper_batched_kernel(const ex_desc* ds)
+{
+  const ex_desc& d = ds[blockIdx.y];
+  const long total = (long)d.kr * d.nc;
+  for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
+  {
+    const int c = (int)(t / d.kr), i = (int)(t % d.kr);
+    d.dst[i * d.drs + c * d.dcs] = (i <= c) ? d.A[(size_t)c * d.lda + i] : 0.0;
+  }
+}
+
+__global__ void gather_cols_batched_kernel(const ga_desc* ds)
+{
+  const ga_desc& d = ds[blockIdx.y];
+  const long total = (long)d.rows * d.ncols;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
+  {
+    const int o = (int)(x / d.rows), t = (int)(x % d.rows);
+    d.dst[t * d.drs + o * d.dcs] = d.src[t + (size_t)d.perm[o] * d.lds];
+  }
+}
+
+__global__ void copy_batched_kernel(const cp_desc* ds)
+{
+  const cp_desc& d = ds[blockIdx.y];
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < d.n; x += (long)gridDim.x * blockDim.x)
+    d.dst[x] = d.src[x];
+}
+
+// One workgroup per problem: out_norm = ||x||_F, out_delta = scale * norm.
+__global__ __launch_bounds__(256) void frob_batched_kernel(const sq_desc* ds)
+{
+  const sq_desc& d = ds[blockIdx.x];
+  __shared__ double red[4];
+  double acc = 0.0;
+  for (long t = threadIdx.x; t < d.n; t += 256) acc = fma(d.x[t], d.x[t], acc);
+  acc = block_sum<256>(acc, red);
+  if (threadIdx.x == 0)
+  {
+    const double nrm = sqrt(acc);
+    d.out_norm[0] = nrm;
+    if (d.out_delta) d.out_delta[0] = d.scale * nrm;
+  }
+}
+
+// Column norms of a row-major (rows x cols) core per problem; one wave per column.
+__global__ __launch_bounds__(256) void col_norms_batched_kernel(const cn_desc* ds)
+{
+  const cn_desc& d = ds[blockIdx.y];
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= d.cols) return;
+  double acc = 0.0;
+  for (int i = lane; i < d.rows; i += 64) acc = fma(d.M[(size_t)i * d.cols + c], d.M[(size_t)i * d.cols + c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) d.out[c] = sqrt(acc);
+}
+
+// Geometric mean of the nonzero entries of v[0..n) per problem.
+__global__ __launch_bounds__(256) void geo_mean_batched_kernel(const gm_desc* ds)
+{
+  const gm_desc& d = ds[blockIdx.x];
+  __shared__ double red[4];
+  double s = 0.0, c = 0.0;
+  for (int t = threadIdx.x; t < d.n; t += 256)
+    if (d.v[t] > 0.0)
+    {
+      s += log(d.v[t]);
+      c += 1.0;
+    }
+  s = block_sum<256>(s, red);
+  c = block_sum<256>(c, red);
+  if (threadIdx.x == 0) d.out[0] = c > 0.0 ? exp(s / c) : 0.0;
+}
+
+__global__ void balance_batched_kernel(const bal_desc* ds)
+{
+  const bal_desc& d = ds[blockIdx.y];
+  const long total = (long)d.r * d.n * d.rp;
+  const double g = *d.geo;
+  if (g == 0.0) return;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    const int b = (int)(o % d.rp);
+    const int a = (int)(o / ((long)d.n * d.rp));
+    double f = 1.0;
+    if (d.nr && d.nr[b] > 0.0) f *= g / d.nr[b];
+    if (d.nl && d.nl[a] > 0.0) f /= g / d.nl[a];
+    d.x[o] *= f;
+  }
+}
+
 int grid_for(long n, int bs = 256)
 {
   long g = (n + bs - 1) / bs;
@@ -470,6 +596,67 @@ void launch_wave_sum_check(const double* x, int n, int* bad, hipStream_t st)
 void launch_scale(double* x, long n, double alpha, hipStream_t st)
 {
   hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, n, alpha);
+}
+
+
+void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st)
+{
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, count);
+  hipLaunchKernelGGL(gemm_f64_grouped_kernel, grid, dim3(256), 0, st, descs_dev);
+}
+
+void launch_transpose_batched(const tr_desc* ds, int count, int max_rows, int max_cols, hipStream_t st)
+{
+  if (count <= 0) return;
+  dim3 grid((max_cols + 31) / 32, (max_rows + 31) / 32, count);
+  hipLaunchKernelGGL(transpose_batched_kernel, grid, dim3(256), 0, st, ds);
+}
+
+void launch_extract_upper_batched(const ex_desc* ds, int count, long max_elems, hipStream_t st)
+{
+  if (count <= 0) return;
+  int g = (int)std::min<long>((max_elems + 255) / 256, 256);
+  hipLaunchKernelGGL(extract_upper_batched_kernel, dim3(g < 1 ? 1 : g, count), dim3(256), 0, st, ds);
+}
+
+void launch_gather_cols_batched(const ga_desc* ds, int count, long max_elems, hipStream_t st)
+{
+  if (count <= 0) return;
+  int g = (int)std::min<long>((max_elems + 255) / 256, 256);
+  hipLaunchKernelGGL(gather_cols_batched_kernel, dim3(g < 1 ? 1 : g, count), dim3(256), 0, st, ds);
+}
+
+void launch_copy_batched(const cp_desc* ds, int count, long max_elems, hipStream_t st)
+{
+  if (count <= 0) return;
+  int g = (int)std::min<long>((max_elems + 255) / 256, 512);
+  hipLaunchKernelGGL(copy_batched_kernel, dim3(g < 1 ? 1 : g, count), dim3(256), 0, st, ds);
+}
+
+void launch_frob_batched(const sq_desc* ds, int count, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(frob_batched_kernel, dim3(count), dim3(256), 0, st, ds);
+}
+
+void launch_col_norms_batched(const cn_desc* ds, int count, int max_cols, hipStream_t st)
+{
+  if (count <= 0 || max_cols <= 0) return;
+  hipLaunchKernelGGL(col_norms_batched_kernel, dim3((max_cols + 3) / 4, count), dim3(256), 0, st, ds);
+}
+
+void launch_geo_mean_batched(const gm_desc* ds, int count, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(geo_mean_batched_kernel, dim3(count), dim3(256), 0, st, ds);
+}
+
+void launch_balance_batched(const bal_desc* ds, int count, long max_elems, hipStream_t st)
+{
+  if (count <= 0) return;
+  int g = (int)std::min<long>((max_elems + 255) / 256, 256);
+  hipLaunchKernelGGL(balance_batched_kernel, dim3(g < 1 ? 1 : g, count), dim3(256), 0, st, ds);
 }
 
 }  // namespace tta

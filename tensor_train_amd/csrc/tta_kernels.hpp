@@ -104,6 +104,65 @@ struct jac_batch
   int batch = 1;
 };
 
+// Descriptor-batched helpers (one entry per problem, device arrays).
+struct tr_desc  // out[j*ldo + i] = in[i*ldi + j], i < rows, j < cols
+{
+  const double* in = nullptr;
+  int ldi = 0, rows = 0, cols = 0;
+  double* out = nullptr;
+  int ldo = 0;
+};
+struct ex_desc  // dst[i*drs + c*dcs] = (i <= c) ? A[c*lda + i] : 0
+{
+  const double* A = nullptr;
+  int lda = 0, kr = 0, nc = 0;
+  double* dst = nullptr;
+  long drs = 0, dcs = 0;
+};
+struct ga_desc  // dst[t*drs + o*dcs] = src[t + perm[o]*lds]
+{
+  const double* src = nullptr;
+  int lds = 0, rows = 0;
+  const int* perm = nullptr;
+  int ncols = 0;
+  double* dst = nullptr;
+  long drs = 0, dcs = 0;
+};
+struct cp_desc
+{
+  const double* src = nullptr;
+  double* dst = nullptr;
+  long n = 0;
+};
+struct sq_desc  // out_norm = ||x||, out_delta = scale * ||x||
+{
+  const double* x = nullptr;
+  long n = 0;
+  double* out_norm = nullptr;
+  double* out_delta = nullptr;
+  double scale = 0.0;
+};
+struct cn_desc
+{
+  const double* M = nullptr;
+  int rows = 0, cols = 0;
+  double* out = nullptr;
+};
+struct gm_desc
+{
+  const double* v = nullptr;
+  int n = 0;
+  double* out = nullptr;
+};
+struct bal_desc
+{
+  double* x = nullptr;
+  int r = 0, n = 0, rp = 0;
+  const double* nl = nullptr;
+  const double* nr = nullptr;
+  const double* geo = nullptr;
+};
+
 struct permute_desc
 {
   const double* src = nullptr;
@@ -113,6 +172,8 @@ struct permute_desc
 };
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
+// Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
+void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
@@ -138,5 +199,13 @@ void launch_copy2d(const double* src, int lds, double* dst, int ldd, int rows, i
                    hipStream_t st);
 void launch_scale(double* x, long n, double alpha, hipStream_t st);
 void launch_wave_sum_check(const double* x, int n, int* bad, hipStream_t st);
+void launch_transpose_batched(const tr_desc* ds, int count, int max_rows, int max_cols, hipStream_t st);
+void launch_extract_upper_batched(const ex_desc* ds, int count, long max_elems, hipStream_t st);
+void launch_gather_cols_batched(const ga_desc* ds, int count, long max_elems, hipStream_t st);
+void launch_copy_batched(const cp_desc* ds, int count, long max_elems, hipStream_t st);
+void launch_frob_batched(const sq_desc* ds, int count, hipStream_t st);
+void launch_col_norms_batched(const cn_desc* ds, int count, int max_cols, hipStream_t st);
+void launch_geo_mean_batched(const gm_desc* ds, int count, hipStream_t st);
+void launch_balance_batched(const bal_desc* ds, int count, long max_elems, hipStream_t st);
 
 }  // namespace tta
