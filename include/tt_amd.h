@@ -29,6 +29,7 @@ extern "C" {
 
 typedef struct tta_ctx tta_ctx;
 typedef struct tta_tt tta_tt;
+typedef struct tta_mlp tta_mlp;
 
 enum tta_status
 {
@@ -137,6 +138,46 @@ enum tta_precision
 int tta_round_batched(tta_ctx* ctx, int count, tta_tt* const* in, double eps, int max_rank, int precision,
                       tta_tt** out);
 
+/* Randomized rounding with given target ranks (Randomize-then-Orthogonalize:
+ * Gaussian sketch TT, right-to-left sketch contractions, one left-to-right
+ * Householder QR sweep).  ranks: for TT i, d_i + 1 entries (ends ignored),
+ * concatenated.  The discarded norm ||y - yhat|| = sqrt(||y||^2 - ||yhat||^2)
+ * is reported in rel_err[i] (relative); with fallback != 0 every TT whose
+ * discarded norm exceeds eps is rounded by the plain path instead
+ * (fell_back[i] = 1).  No reference counterpart. */
+int tta_round_randomized_batched(tta_ctx* ctx, int count, tta_tt* const* in, const int* ranks, double eps,
+                                 unsigned long long seed, int fallback, tta_tt** out, double* rel_err,
+                                 int* fell_back);
+
+/* ---- learned ranks (no reference counterpart) ----------------------------- */
+/* Rank-prediction MLP n_in -> hidden (x layers, ReLU) -> 1, FP32 parameters
+ * (flat layout: per layer W (in x out row-major) then b (out)), He init
+ * from seed.  Trained on the device by the library's forward / backward /
+ * Adam kernels on mean-squared error. */
+int tta_mlp_create(tta_ctx* ctx, int n_in, int hidden, int layers, unsigned long long seed, tta_mlp** out);
+int tta_mlp_free(tta_ctx* ctx, tta_mlp* m);
+long long tta_mlp_num_params(const tta_mlp* m);
+int tta_mlp_get_params(tta_ctx* ctx, const tta_mlp* m, float* host);
+int tta_mlp_set_params(tta_ctx* ctx, tta_mlp* m, const float* host);
+/* MSE loss and its gradient (flat, parameter layout) on n rows X (n x n_in), y (n). */
+int tta_mlp_grad(tta_ctx* ctx, tta_mlp* m, const float* X, const float* y, int n, float* grad, float* loss);
+/* `steps` full-batch Adam steps (lr, 0.9, 0.999); *loss = final loss. */
+int tta_mlp_train(tta_ctx* ctx, tta_mlp* m, const float* X, const float* y, int n, int steps, float lr,
+                  float* loss);
+int tta_mlp_predict(tta_ctx* ctx, tta_mlp* m, const float* X, int n, float* y);
+/* Number of per-bond features and the features of every bond k = 1..d_i-1 of
+ * every TT (rows concatenated; the feature list is documented with
+ * tta_learn.hip). */
+int tta_num_bond_features(void);
+int tta_bond_features(tta_ctx* ctx, int count, tta_tt* const* in, double eps, float* feats);
+/* Learned-rank rounding: target rank of each bond = ceil(prediction * r_k) +
+ * oversample (clipped to [1, r_k]), randomized rounding with those ranks, plain
+ * fallback when the discarded norm exceeds eps.  ranks_used (optional) gets
+ * the d_i + 1 target ranks per TT. */
+int tta_round_learned_batched(tta_ctx* ctx, tta_mlp* m, int count, tta_tt* const* in, double eps, int oversample,
+                              unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
+                              int* ranks_used);
+
 /* ---- dense kernels of the path (host buffers, row-major) --------------- */
 /* qr_thin (include/core/types/tt_eigen_bridge.hpp:42-60): Householder QR,
  * Q (rows x k) orthonormal columns, R (k x cols) upper trapezoidal,
@@ -160,6 +201,19 @@ int tta_random_matrix(tta_ctx* ctx, int d, const int* row_modes, const int* col_
 /* zeros (value 0, tt_factory.hpp:31-52) / ones (value 1, :59-78): rank-1 TT
  * with every core entry equal to value. */
 int tta_constant(tta_ctx* ctx, int d, const int* modes, double value, tta_tt** out);
+/* Batched benchmark inputs built on the device.  Entries are N(0,1) from
+ * the counter-based Philox4x32-10 generator keyed by the given seeds
+ * (reproducible, but not the std::mt19937_64 stream of tta_random).
+ * Sums: c_i = add(u_i, add(scale(u_i, 0.5), v_i)) with
+ * u_i ~ random([n_i]^d_i, ru_i), v_i ~ random([n_i]^d_i, rv_i) (rank rule of
+ * tt_factory.hpp:155-190; block concatenation of block_concat.hpp:40-156). */
+int tta_build_sum_batch(tta_ctx* ctx, int count, const int* d, const int* n, const int* ru, const int* rv,
+                        const unsigned long long* seed_u, const unsigned long long* seed_v, tta_tt** out);
+/* Decaying-spectrum TTs: modes [n]^d, ranks min(n^k, n^(d-k), rmax); core
+ * k < d-1 = Q_k diag(10^(-j s_i)) with Q_k the thin Q of a Gaussian
+ * (r_k n x r_{k+1}) matrix; last core a unit-norm Gaussian. */
+int tta_build_decay_batch(tta_ctx* ctx, int count, int d, int n, int rmax, const double* s,
+                          const unsigned long long* seeds, tta_tt** out);
 /* Multiply core k of t by alpha in place. */
 int tta_scale_core(tta_ctx* ctx, tta_tt* t, int k, double alpha);
 

@@ -519,3 +519,141 @@ def round_batched(tts, eps=0.0, max_rank=0, precision="fp64"):
     ctx.check(ctx.lib.tta_round_batched(ctx.h, n, arr_in, ctypes.c_double(eps), max_rank,
                                         _enum(PRECISIONS, precision, "precision"), arr_out))
     return [type(tts[i])(ctypes.c_void_p(arr_out[i]), ctx) for i in range(n)]
+
+
+def build_sum_batch(d, n, ru, rv, seed_u, seed_v, ctx: Context | None = None):
+    """c_i = add(u_i, add(scale(u_i, 0.5), v_i)) built on the device (Philox N(0,1) entries)."""
+    ctx = ctx or Context.default()
+    cnt = len(d)
+    ip = ctypes.POINTER(ctypes.c_int)
+    arrs = [np.ascontiguousarray(a, np.int32) for a in (d, n, ru, rv)]
+    su = (ctypes.c_ulonglong * cnt)(*[int(x) for x in seed_u])
+    sv = (ctypes.c_ulonglong * cnt)(*[int(x) for x in seed_v])
+    out = (ctypes.c_void_p * cnt)()
+    ctx.check(ctx.lib.tta_build_sum_batch(ctx.h, cnt, *[a.ctypes.data_as(ip) for a in arrs], su, sv, out))
+    return [TensorTrain(ctypes.c_void_p(out[i]), ctx) for i in range(cnt)]
+
+
+def build_decay_batch(count, d, n, rmax, s, seeds, ctx: Context | None = None):
+    """Decaying-spectrum TTs (cores Q_k diag(10^(-j s_i)), unit-norm last core) built on the device."""
+    ctx = ctx or Context.default()
+    sv = np.ascontiguousarray(s, np.float64)
+    sd = (ctypes.c_ulonglong * count)(*[int(x) for x in seeds])
+    out = (ctypes.c_void_p * count)()
+    ctx.check(ctx.lib.tta_build_decay_batch(ctx.h, count, d, n, rmax, sv.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                            sd, out))
+    return [TensorTrain(ctypes.c_void_p(out[i]), ctx) for i in range(count)]
+
+
+def round_randomized_batched(tts, ranks, eps, seed=1, fallback=True):
+    """Randomized rounding with target ranks (list per TT of d+1 ints), plain
+    fallback when the discarded norm exceeds eps.  Returns (outputs,
+    rel_err array, fell_back array)."""
+    if not tts:
+        return [], np.zeros(0), np.zeros(0, np.int32)
+    ctx = tts[0]._ctx
+    n = len(tts)
+    flat = np.ascontiguousarray(np.concatenate([np.asarray(r, np.int32) for r in ranks]), np.int32)
+    arr_in = (ctypes.c_void_p * n)(*[t._h.value for t in tts])
+    arr_out = (ctypes.c_void_p * n)()
+    err = np.zeros(n)
+    fb = np.zeros(n, np.int32)
+    ctx.check(ctx.lib.tta_round_randomized_batched(ctx.h, n, arr_in, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                   ctypes.c_double(eps), ctypes.c_ulonglong(seed), 1 if fallback else 0,
+                                                   arr_out, err.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                   fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    return [type(tts[i])(ctypes.c_void_p(arr_out[i]), ctx) for i in range(n)], err, fb
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, np.float32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class RankMLP:
+    """Rank-prediction MLP on the GPU (tta_mlp_*): n_in -> hidden x layers -> 1."""
+
+    def __init__(self, n_in, hidden=64, layers=3, seed=0, ctx: Context | None = None):
+        self._ctx = ctx or Context.default()
+        h = ctypes.c_void_p()
+        self._ctx.check(self._ctx.lib.tta_mlp_create(self._ctx.h, n_in, hidden, layers, seed, ctypes.byref(h)))
+        self._h = h
+        self.n_in = n_in
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            try:
+                self._ctx.lib.tta_mlp_free(self._ctx.h, h)
+            except Exception:
+                pass
+            self._h = None
+
+    @property
+    def num_params(self):
+        return int(self._ctx.lib.tta_mlp_num_params(self._h))
+
+    def get_params(self):
+        out = np.zeros(self.num_params, np.float32)
+        self._ctx.check(self._ctx.lib.tta_mlp_get_params(self._ctx.h, self._h, _f32(out)[1]))
+        return out
+
+    def set_params(self, p):
+        a, ptr = _f32(p)
+        self._ctx.check(self._ctx.lib.tta_mlp_set_params(self._ctx.h, self._h, ptr))
+
+    def grad(self, X, y):
+        Xa, Xp = _f32(X)
+        ya, yp = _f32(y)
+        g = np.zeros(self.num_params, np.float32)
+        loss = ctypes.c_float()
+        self._ctx.check(self._ctx.lib.tta_mlp_grad(self._ctx.h, self._h, Xp, yp, len(ya), _f32(g)[1],
+                                                   ctypes.byref(loss)))
+        return loss.value, g
+
+    def train(self, X, y, steps=1000, lr=1e-3):
+        Xa, Xp = _f32(X)
+        ya, yp = _f32(y)
+        loss = ctypes.c_float()
+        self._ctx.check(self._ctx.lib.tta_mlp_train(self._ctx.h, self._h, Xp, yp, len(ya), steps, ctypes.c_float(lr),
+                                                    ctypes.byref(loss)))
+        return loss.value
+
+    def predict(self, X):
+        Xa, Xp = _f32(X)
+        out = np.zeros(Xa.shape[0], np.float32)
+        self._ctx.check(self._ctx.lib.tta_mlp_predict(self._ctx.h, self._h, Xp, Xa.shape[0], _f32(out)[1]))
+        return out
+
+
+def bond_features(tts, eps):
+    """Per-bond features (rows: bonds 1..d-1 of each TT in order)."""
+    ctx = tts[0]._ctx
+    nf = ctx.lib.tta_num_bond_features()
+    nb = sum(t.d - 1 for t in tts)
+    out = np.zeros((max(nb, 1), nf), np.float32)
+    arr = (ctypes.c_void_p * len(tts))(*[t._h.value for t in tts])
+    ctx.check(ctx.lib.tta_bond_features(ctx.h, len(tts), arr, ctypes.c_double(eps), _f32(out)[1]))
+    return out[:nb]
+
+
+def round_learned_batched(mlp, tts, eps, oversample=6, seed=1):
+    """Learned-rank randomized rounding with plain fallback; returns
+    (outputs, rel_err, fell_back, target ranks per TT)."""
+    ctx = tts[0]._ctx
+    n = len(tts)
+    arr_in = (ctypes.c_void_p * n)(*[t._h.value for t in tts])
+    arr_out = (ctypes.c_void_p * n)()
+    err = np.zeros(n)
+    fb = np.zeros(n, np.int32)
+    rk = np.zeros(sum(t.d + 1 for t in tts), np.int32)
+    ctx.check(ctx.lib.tta_round_learned_batched(ctx.h, mlp._h, n, arr_in, ctypes.c_double(eps), oversample,
+                                                ctypes.c_ulonglong(seed), arr_out,
+                                                err.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                                rk.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+    ranks, off = [], 0
+    for t in tts:
+        ranks.append(rk[off:off + t.d + 1].tolist())
+        off += t.d + 1
+    return [type(tts[i])(ctypes.c_void_p(arr_out[i]), ctx) for i in range(n)], err, fb, ranks

@@ -16,6 +16,7 @@ CSRC = os.path.join(_HERE, "csrc")
 
 _i32p = ctypes.POINTER(ctypes.c_int)
 _f64p = ctypes.POINTER(ctypes.c_double)
+_f32p = ctypes.POINTER(ctypes.c_float)
 _vp = ctypes.c_void_p
 _vpp = ctypes.POINTER(ctypes.c_void_p)
 
@@ -49,6 +50,22 @@ SIGNATURES = {
     "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),
     "tta_round_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double,
                                          ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "tta_round_randomized_batched": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _i32p,
+                                                    ctypes.c_double, ctypes.c_ulonglong, ctypes.c_int,
+                                                    ctypes.POINTER(ctypes.c_void_p), _f64p, _i32p]),
+    "tta_mlp_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_ulonglong, _vpp]),
+    "tta_mlp_free": (ctypes.c_int, [_vp, _vp]),
+    "tta_mlp_num_params": (ctypes.c_longlong, [_vp]),
+    "tta_mlp_get_params": (ctypes.c_int, [_vp, _vp, _f32p]),
+    "tta_mlp_set_params": (ctypes.c_int, [_vp, _vp, _f32p]),
+    "tta_mlp_grad": (ctypes.c_int, [_vp, _vp, _f32p, _f32p, ctypes.c_int, _f32p, _f32p]),
+    "tta_mlp_train": (ctypes.c_int, [_vp, _vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _f32p]),
+    "tta_mlp_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int, _f32p]),
+    "tta_num_bond_features": (ctypes.c_int, []),
+    "tta_bond_features": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double, _f32p]),
+    "tta_round_learned_batched": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong,
+                                                 ctypes.POINTER(ctypes.c_void_p), _f64p, _i32p, _i32p]),
     "tta_qr_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "tta_svd_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]),
     "tta_selfcheck_wave_sum": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p]),
@@ -57,6 +74,11 @@ SIGNATURES = {
     "tta_random": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_ulonglong, _vpp]),
     "tta_random_matrix": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, ctypes.c_int, ctypes.c_ulonglong, _vpp]),
     "tta_constant": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, ctypes.c_double, _vpp]),
+    "tta_build_sum_batch": (ctypes.c_int, [_vp, ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
+                                           ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_ulonglong),
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "tta_build_decay_batch": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f64p,
+                                             ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_void_p)]),
     "tta_scale_core": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "tta_prof_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),

@@ -48,3 +48,51 @@ def test_round_batched_equals_single_path():
         assert y.ranks == single.ranks
         a, b = y.to_dense(), single.to_dense()
         assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(b)
+
+
+def test_build_sum_batch_structure():
+    d, n, ru, rv = [4, 1, 5], [2, 3, 4], [3, 2, 4], [2, 3, 2]
+    cs = T.build_sum_batch(d, n, ru, rv, [11, 12, 13], [21, 22, 23])
+    for t, c in enumerate(cs):
+        cores = c.cores
+        if d[t] == 1:
+            assert cores[0].shape == (1, n[t], 1)
+            continue
+        Ru = O.OTT.random([n[t]] * d[t], ru[t], 0).ranks
+        Rv = O.OTT.random([n[t]] * d[t], rv[t], 0).ranks
+        assert c.ranks == [1] + [2 * a + b for a, b in zip(Ru[1:-1], Rv[1:-1])] + [1]
+        u, v = [], []
+        for k, ck in enumerate(cores):
+            if k == 0:
+                u.append(ck[:, :, :Ru[1]])
+                assert np.array_equal(ck[:, :, Ru[1]:2 * Ru[1]], 0.5 * u[-1])
+                v.append(ck[:, :, 2 * Ru[1]:])
+            elif k == d[t] - 1:
+                u.append(ck[:Ru[k]])
+                assert np.array_equal(ck[Ru[k]:2 * Ru[k]], u[-1])
+                v.append(ck[2 * Ru[k]:])
+            else:
+                u.append(ck[:Ru[k], :, :Ru[k + 1]])
+                assert np.array_equal(ck[Ru[k]:2 * Ru[k], :, Ru[k + 1]:2 * Ru[k + 1]], u[-1])
+                v.append(ck[2 * Ru[k]:, :, 2 * Ru[k + 1]:])
+                assert np.all(ck[:Ru[k], :, Ru[k + 1]:] == 0) and np.all(ck[2 * Ru[k]:, :, :2 * Ru[k + 1]] == 0)
+        U, V = O.OTT.from_cores(u), O.OTT.from_cores(v)
+        dense = c.to_dense()
+        np.testing.assert_allclose(dense, 1.5 * U.to_dense() + V.to_dense(), rtol=0, atol=1e-12)
+        allv = np.concatenate([x.ravel() for x in u + v])
+        assert abs(allv.mean()) < 0.3 and 0.5 < allv.var() < 1.6
+        r = T.round(c, eps=1e-12)
+        assert all(x <= a + b for x, a, b in zip(r.ranks, Ru, Rv))
+
+
+def test_build_decay_batch_spectrum():
+    s = [0.05, 0.2, 0.3]
+    ts = T.build_decay_batch(3, 6, 4, 16, s, [5, 6, 7])
+    for si, t in zip(s, ts):
+        cores = t.cores
+        assert t.ranks == [1, 4, 16, 16, 16, 4, 1]
+        for k in range(5):
+            M = cores[k].reshape(-1, cores[k].shape[2])
+            want = 10.0 ** (-2.0 * si * np.arange(M.shape[1]))
+            np.testing.assert_allclose(M.T @ M, np.diag(want), rtol=0, atol=1e-13)
+        assert abs(np.linalg.norm(cores[-1]) - 1.0) < 1e-14
