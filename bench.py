@@ -76,6 +76,91 @@ def cpu_baseline(reps: int, y_dev_cores):
     }, rel, yref.ranks
 
 
+def batched_section(T, ctx, args):
+    """Batched workloads: config 4 (learned-rank randomized vs plain FP64
+    batched rounding) and config 3 (plain FP64 batched rounding)."""
+    import numpy as np
+
+    out = {}
+    eps = 1e-6
+    rng = np.random.default_rng(2024)
+
+    # ---- config 4: 2048 TTs, d=32, n=4, ranks <= 128, decaying spectra.
+    n4 = args.cfg4_count
+    s_train = rng.uniform(0.05, 0.3, args.train_tts)
+    train = T.build_decay_batch(args.train_tts, 32, 4, 128, s_train, 10_000_000 + np.arange(args.train_tts))
+    plain_tr = T.round_batched(train, eps=eps)
+    X = T.bond_features(train, eps)
+    y = np.array([p.ranks[k] / t.ranks[k] for t, p in zip(train, plain_tr) for k in range(1, t.d)], np.float32)
+    del plain_tr
+    mlp = T.RankMLP(X.shape[1], 64, 3, seed=1)
+    t0 = time.perf_counter()
+    loss = mlp.train(X, y, steps=args.mlp_steps, lr=2e-3)
+    ctx.synchronize()
+    train_s = time.perf_counter() - t0
+    del train
+    s4 = rng.uniform(0.05, 0.3, n4)
+    batch = T.build_decay_batch(n4, 32, 4, 128, s4, np.arange(n4))
+    ctx.synchronize()
+    # warm up both paths on a slice
+    T.round_batched(batch[:32], eps=eps)
+    T.round_learned_batched(mlp, batch[:32], eps, oversample=6, seed=3)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    plain = T.round_batched(batch, eps=eps)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    learned, err, fb, ranks = T.round_learned_batched(mlp, batch, eps, oversample=6, seed=3)
+    ctx.synchronize()
+    t_learned = time.perf_counter() - t0
+    # Agreement with the plain FP64 result on a sample (difference norm on the
+    # device: right-orthogonalized difference TT, norm = ||core 0||).
+    sample = list(range(0, n4, max(1, n4 // 16)))
+    diffs = []
+    for i in sample:
+        dif = T.sub(learned[i], plain[i])
+        T.right_orthogonalize(dif)
+        pr = plain[i].copy()
+        T.right_orthogonalize(pr)
+        diffs.append(float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(pr.cores[0])))
+    out["config4"] = {
+        "workload": f"{n4} TTs, d=32, n=4, ranks min(4^k, 4^(32-k), 128), cores Q_k diag(10^(-j s)), "
+                    "s~U[0.05,0.3], eps=1e-6",
+        "plain_fp64_rounds_per_s": n4 / t_plain,
+        "learned_rounds_per_s": n4 / t_learned,
+        "speedup_learned_vs_plain": t_plain / t_learned,
+        "fallback_fraction": float(fb.mean()),
+        "max_discarded_rel_norm": float(err[fb == 0].max()) if (fb == 0).any() else None,
+        "max_rel_diff_vs_plain_sampled": max(diffs),
+        "mean_target_rank": float(np.mean([np.mean(r[1:-1]) for r in ranks])),
+        "mean_plain_rank": float(np.mean([np.mean(p.ranks[1:-1]) for p in plain])),
+        "mlp": {"train_bonds": int(len(y)), "steps": args.mlp_steps, "final_mse": loss, "train_s": train_s},
+    }
+    del batch, plain, learned
+
+    # ---- config 3: 4096 TTs, d~U{8..40}, n in {2,4,8}, add(u, add(0.5u, v)).
+    n3 = args.cfg3_count
+    d = rng.integers(8, 41, n3)
+    nn = rng.choice([2, 4, 8], n3)
+    ru = rng.integers(8, 65, n3)
+    rv = rng.integers(8, 65, n3)
+    c3 = T.build_sum_batch(d, nn, ru, rv, 1000 + np.arange(n3), 500000 + np.arange(n3))
+    ctx.synchronize()
+    T.round_batched(c3[:16], eps=eps)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    p3 = T.round_batched(c3, eps=eps)
+    ctx.synchronize()
+    t3 = time.perf_counter() - t0
+    out["config3"] = {
+        "workload": f"{n3} TTs, d~U{{8..40}}, n in {{2,4,8}}, add(u, add(0.5u, v)), r_u, r_v ~ U{{8..64}}, eps=1e-6",
+        "plain_fp64_rounds_per_s": n3 / t3,
+        "mean_out_rank": float(np.mean([np.mean(p.ranks[1:-1]) for p in p3])),
+    }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +168,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-reps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--cfg4-count", type=int, default=2048)
+    ap.add_argument("--cfg3-count", type=int, default=4096)
+    ap.add_argument("--train-tts", type=int, default=650)
+    ap.add_argument("--mlp-steps", type=int, default=1500)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
@@ -140,6 +230,8 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": {k: v["ms"] / args.steps for k, v in stats.items()},
     }
+    if not args.no_batched:
+        out["batched"] = batched_section(T, ctx, args)
     if not args.no_cpu:
         cb, rel, yref_ranks = cpu_baseline(args.cpu_reps, y.cores)
         out["cpu_baseline"] = cb
