@@ -235,6 +235,16 @@ int tta_prof_enable(tta_ctx* ctx, int on);
 int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* total_ms, double* flops,
                   double* bytes);
 
+/* ---- device memory pool --------------------------------------------------- */
+/* Statistics of the context's caching allocator: bytes held by live blocks
+ * (tensor cores and workspaces), bytes cached for reuse, device allocations
+ * made, and cache releases forced by a failed allocation.  No reference
+ * counterpart (the reference allocates host memory per core). */
+int tta_mem_stats(tta_ctx* ctx, unsigned long long* live_bytes, unsigned long long* cached_bytes,
+                  unsigned long long* device_allocs, unsigned long long* trims);
+/* Returns every cached (free) block of the context to the device. */
+int tta_mem_trim(tta_ctx* ctx);
+
 /* ---- self-checks --------------------------------------------------------- */
 /* Compares the device's DPP wave reduction with a ds_bpermute reduction on n
  * host values (rounded up to whole 64-lane waves); *mismatches = 0 when they

@@ -82,6 +82,17 @@ class Context:
                                           ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)))
         return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
 
+    def mem_stats(self):
+        """{live_bytes, cached_bytes, device_allocs, trims} of the context's
+        caching device allocator."""
+        v = [ctypes.c_ulonglong() for _ in range(4)]
+        self.check(self.lib.tta_mem_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("live_bytes", "cached_bytes", "device_allocs", "trims"), (x.value for x in v)))
+
+    def mem_trim(self):
+        """Returns the allocator's cached blocks to the device."""
+        self.check(self.lib.tta_mem_trim(self.h))
+
 
 class RoundOptions:
     """Options of ``round`` / ``matvec_round`` / ``matmat_round``.

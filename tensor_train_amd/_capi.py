@@ -82,6 +82,8 @@ SIGNATURES = {
     "tta_scale_core": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "tta_prof_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),
+    "tta_mem_stats": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_ulonglong)] * 4),
+    "tta_mem_trim": (ctypes.c_int, [_vp]),
 }
 
 KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4}

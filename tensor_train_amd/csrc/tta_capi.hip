@@ -40,68 +40,6 @@ void hip_check(hipError_t e, const char* what)
   if (e != hipSuccess) fail(TTA_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
 
-size_t pool::bucket(size_t bytes)
-{
-  if (bytes < 256) return 256;
-  // Round up to one of 8 sizes per power of two (<= 12.5% slack).
-  size_t p = 1;
-  while ((p << 1) <= bytes) p <<= 1;
-  const size_t step = p >= 8 ? p / 8 : 1;
-  return (bytes + step - 1) / step * step;
-}
-
-void* pool::get(size_t bytes)
-{
-  const size_t b = bucket(bytes);
-  auto it = free_.find(b);
-  if (it != free_.end() && !it->second.empty())
-  {
-    void* p = it->second.back();
-    it->second.pop_back();
-    size_of_[p] = b;
-    return p;
-  }
-  void* p = nullptr;
-  hipError_t e = hipMalloc(&p, b);
-  if (e != hipSuccess)
-  {
-    // Release cached blocks and retry once.
-    (void)hipGetLastError();
-    trim();
-    e = hipMalloc(&p, b);
-    if (e != hipSuccess)
-    {
-      (void)hipGetLastError();
-      fail(TTA_ERR_NOMEM, "device allocation of %zu bytes failed", b);
-    }
-  }
-  size_of_[p] = b;
-  return p;
-}
-
-void pool::put(void* p)
-{
-  if (!p) return;
-  auto it = size_of_.find(p);
-  if (it == size_of_.end()) return;
-  free_[it->second].push_back(p);
-  size_of_.erase(it);
-}
-
-void pool::trim()
-{
-  (void)hipDeviceSynchronize();
-  for (auto& kv : free_)
-    for (void* p : kv.second) (void)hipFree(p);
-  free_.clear();
-}
-
-pool::~pool()
-{
-  trim();
-  for (auto& kv : size_of_) (void)hipFree(kv.first);
-}
-
 double* ctx_ws(tta_ctx* c, int slot, size_t n)
 {
   auto& w = c->ws[slot];
@@ -1446,6 +1384,25 @@ int tta_prof_read(tta_ctx* c, int cls, long long* launches, double* total_ms, do
   if (total_ms) *total_ms = c->prof_ms[cls];
   if (flops) *flops = c->prof_flops[cls];
   if (bytes) *bytes = c->prof_bytes[cls];
+  TTA_GUARD_END(c)
+}
+
+int tta_mem_stats(tta_ctx* c, unsigned long long* live_bytes, unsigned long long* cached_bytes,
+                  unsigned long long* device_allocs, unsigned long long* trims)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (live_bytes) *live_bytes = c->mem.live_bytes;
+  if (cached_bytes) *cached_bytes = c->mem.cached_bytes;
+  if (device_allocs) *device_allocs = c->mem.device_allocs;
+  if (trims) *trims = c->mem.trims;
+  return TTA_OK;
+}
+
+int tta_mem_trim(tta_ctx* c)
+{
+  if (!c) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  c->mem.trim();
   TTA_GUARD_END(c)
 }
 

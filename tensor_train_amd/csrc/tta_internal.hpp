@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -21,21 +22,44 @@ struct error : std::runtime_error
 [[noreturn]] void fail(int code, const char* fmt, ...);
 void hip_check(hipError_t e, const char* what);
 
-// Size-bucketed caching allocator over hipMalloc.  All work of a context is
-// ordered on its one stream, so a block returned here may be handed to the
-// next request without synchronisation.
+// Caching sub-allocator over hipMalloc.  Requests are carved best-fit out of
+// device segments (16 MiB segments for blocks up to 1 MiB, at least 32 MiB
+// for larger ones); freed blocks coalesce with free neighbours of the same
+// segment and are reused without going back to the device.  All work of a
+// context is ordered on its one stream, so a block returned here may be
+// handed to the next request without synchronisation.
 class pool
 {
  public:
   void* get(size_t bytes);
   void put(void* p);
+  // Returns every segment with no live block to the device.
   void trim();
   ~pool();
-  static size_t bucket(size_t bytes);
+
+  size_t live_bytes = 0, cached_bytes = 0;
+  unsigned long long device_allocs = 0, trims = 0;
 
  private:
-  std::map<size_t, std::vector<void*>> free_;
-  std::unordered_map<void*, size_t> size_of_;
+  struct block
+  {
+    char* ptr;
+    size_t size;
+    bool free, small;
+    block* prev;  // neighbours inside the same segment
+    block* next;
+  };
+  struct by_size
+  {
+    bool operator()(const block* a, const block* b) const
+    {
+      return a->size != b->size ? a->size < b->size : a->ptr < b->ptr;
+    }
+  };
+  block* carve(std::set<block*, by_size>& fl, size_t size, bool small);
+  std::set<block*, by_size> small_free_, large_free_;
+  std::unordered_map<void*, block*> live_;
+  std::vector<block*> segments_;  // first block of each segment
 };
 
 struct ws_slot
