@@ -170,12 +170,14 @@ int tta_mlp_predict(tta_ctx* ctx, tta_mlp* m, const float* X, int n, float* y);
  * tta_learn.hip). */
 int tta_num_bond_features(void);
 int tta_bond_features(tta_ctx* ctx, int count, tta_tt* const* in, double eps, float* feats);
-/* Learned-rank rounding: target rank of each bond = ceil(prediction * r_k) +
- * oversample (clipped to [1, r_k]), randomized rounding with those ranks, plain
- * fallback when the discarded norm exceeds eps.  ranks_used (optional) gets
- * the d_i + 1 target ranks per TT. */
+/* Learned-rank rounding: target rank of each bond =
+ * ceil(prediction * r_k * (1 + slack)) + oversample (clipped to [1, r_k]),
+ * randomized rounding with those ranks, plain fallback when the discarded norm
+ * exceeds eps.  The relative slack covers the sketch's error factor over the
+ * optimal truncation, which grows as the spectrum decays more slowly.
+ * ranks_used (optional) gets the d_i + 1 target ranks per TT. */
 int tta_round_learned_batched(tta_ctx* ctx, tta_mlp* m, int count, tta_tt* const* in, double eps, int oversample,
-                              unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
+                              double slack, unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
                               int* ranks_used);
 
 /* ---- dense kernels of the path (host buffers, row-major) --------------- */

@@ -648,9 +648,10 @@ def bond_features(tts, eps):
     return out[:nb]
 
 
-def round_learned_batched(mlp, tts, eps, oversample=6, seed=1):
-    """Learned-rank randomized rounding with plain fallback; returns
-    (outputs, rel_err, fell_back, target ranks per TT)."""
+def round_learned_batched(mlp, tts, eps, oversample=6, seed=1, slack=0.1):
+    """Learned-rank randomized rounding with plain fallback; target rank per
+    bond = ceil(pred * r_k * (1 + slack)) + oversample.  Returns (outputs,
+    rel_err, fell_back, target ranks per TT)."""
     ctx = tts[0]._ctx
     n = len(tts)
     arr_in = (ctypes.c_void_p * n)(*[t._h.value for t in tts])
@@ -659,7 +660,7 @@ def round_learned_batched(mlp, tts, eps, oversample=6, seed=1):
     fb = np.zeros(n, np.int32)
     rk = np.zeros(sum(t.d + 1 for t in tts), np.int32)
     ctx.check(ctx.lib.tta_round_learned_batched(ctx.h, mlp._h, n, arr_in, ctypes.c_double(eps), oversample,
-                                                ctypes.c_ulonglong(seed), arr_out,
+                                                ctypes.c_double(slack), ctypes.c_ulonglong(seed), arr_out,
                                                 err.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                 rk.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))

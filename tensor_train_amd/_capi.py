@@ -64,7 +64,7 @@ SIGNATURES = {
     "tta_num_bond_features": (ctypes.c_int, []),
     "tta_bond_features": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double, _f32p]),
     "tta_round_learned_batched": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
-                                                 ctypes.c_double, ctypes.c_int, ctypes.c_ulonglong,
+                                                 ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
                                                  ctypes.POINTER(ctypes.c_void_p), _f64p, _i32p, _i32p]),
     "tta_qr_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "tta_svd_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]),

@@ -140,9 +140,20 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     qb.max_m = max_m;
     qb.max_n = max_n;
     qb.form_q = true;
-    launch_qr_blocked(qb, c->stream);
-    launch_extract_upper_batched(S.upload(ex), nb, max_ex, c->stream);
-    launch_gemm_grouped(S.upload(gm), nb, max_gm, max_gn, c->stream);
+    {
+      const credit w = qr_credit(qr.data(), qr.size());
+      prof_scope ps(c, TTA_K_QR, w.flops, w.bytes);
+      launch_qr_blocked(qb, c->stream);
+    }
+    {
+      prof_scope ps(c, TTA_K_OTHER);
+      launch_extract_upper_batched(S.upload(ex), nb, max_ex, c->stream);
+    }
+    {
+      const credit w = gemm_credit(gm.data(), gm.size());
+      prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
+      launch_gemm_grouped(S.upload(gm), nb, max_gm, max_gn, c->stream);
+    }
     for (int a = 0; a < nb; ++a)
     {
       tta_tt* t = ts[idx[a]];
@@ -305,21 +316,35 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     const int nb = (int)idx.size();
     if (nb == 0) continue;
     if (max_q > 640) fail(TTA_ERR_UNSUPPORTED, "round_batched: bond rank %d above 640", max_q);
-    if (!tr.empty()) launch_transpose_batched(S.upload(tr), (int)tr.size(), max_tr_r, max_tr_c, c->stream);
-    if (!cp.empty()) launch_copy_batched(S.upload(cp), (int)cp.size(), max_cp, c->stream);
+    {
+      prof_scope ps(c, TTA_K_OTHER);
+      if (!tr.empty()) launch_transpose_batched(S.upload(tr), (int)tr.size(), max_tr_r, max_tr_c, c->stream);
+      if (!cp.empty()) launch_copy_batched(S.upload(cp), (int)cp.size(), max_cp, c->stream);
+    }
     qr_batch qb;
     qb.descs = S.upload(qr);
     qb.batch = nb;
     qb.max_m = max_m;
     qb.max_n = max_n;
     qb.form_q = true;
-    launch_qr_blocked(qb, c->stream);
-    launch_extract_upper_batched(S.upload(ex), nb, max_ex, c->stream);
+    {
+      const credit w = qr_credit(qr.data(), qr.size());
+      prof_scope ps(c, TTA_K_QR, w.flops, w.bytes);
+      launch_qr_blocked(qb, c->stream);
+    }
+    {
+      prof_scope ps(c, TTA_K_OTHER);
+      launch_extract_upper_batched(S.upload(ex), nb, max_ex, c->stream);
+    }
     jac_batch jb;
     jb.descs = S.upload(jp);
     jb.batch = nb;
     int* done = S.alloc<int>(1);
-    launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c->stream);
+    {
+      const credit w = jacobi_credit(jp.data(), jp.size());
+      prof_scope ps(c, TTA_K_JACOBI, w.flops, w.bytes);
+      launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c->stream);
+    }
     std::vector<int> hr(count);
     hip_check(hipMemcpyAsync(hr.data(), rk, count * sizeof(int), hipMemcpyDeviceToHost, c->stream), "ranks");
     sync(c);
@@ -410,10 +435,20 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       m2_m = std::max(m2_m, r_new);
       m2_n = std::max(m2_n, ncols_n);
     }
-    launch_gather_cols_batched(S.upload(g1), nb, max_g1, c->stream);
-    launch_gather_cols_batched(S.upload(g2), nb, max_g2, c->stream);
-    launch_gemm_grouped(S.upload(m1), nb, m1_m, m1_n, c->stream);
-    launch_gemm_grouped(S.upload(m2), nb, m2_m, m2_n, c->stream);
+    {
+      prof_scope ps(c, TTA_K_OTHER);
+      launch_gather_cols_batched(S.upload(g1), nb, max_g1, c->stream);
+      launch_gather_cols_batched(S.upload(g2), nb, max_g2, c->stream);
+    }
+    {
+      credit w = gemm_credit(m1.data(), m1.size());
+      const credit w2 = gemm_credit(m2.data(), m2.size());
+      w.flops += w2.flops;
+      w.bytes += w2.bytes;
+      prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
+      launch_gemm_grouped(S.upload(m1), nb, m1_m, m1_n, c->stream);
+      launch_gemm_grouped(S.upload(m2), nb, m2_m, m2_n, c->stream);
+    }
     for (int a = 0; a < nb; ++a)
     {
       tta_tt* t = ts[idx[a]];

@@ -52,6 +52,21 @@ double* ctx_ws(tta_ctx* c, int slot, size_t n)
   return w.p;
 }
 
+
+// Event for the timing of one launch on the context stream (recycled).
+hipEvent_t prof_event(tta_ctx* c)
+{
+  if (!c->prof_free.empty())
+  {
+    hipEvent_t e = c->prof_free.back();
+    c->prof_free.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  hip_check(hipEventCreate(&e), "hipEventCreate");
+  return e;
+}
+
 }  // namespace tta
 
 // ----------------------------------------------------------------------------
@@ -113,59 +128,6 @@ void replace_core(tta_ctx* c, tta_tt* t, int k, double* p)
 }
 
 double* alloc_core(tta_ctx* c, size_t n) { return static_cast<double*>(c->mem.get(std::max<size_t>(n, 1) * 8)); }
-
-// Event pair around one launch on the context stream while timing is on.
-hipEvent_t prof_event(tta_ctx* c)
-{
-  if (!c->prof_free.empty())
-  {
-    hipEvent_t e = c->prof_free.back();
-    c->prof_free.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  hip_check(hipEventCreate(&e), "hipEventCreate");
-  return e;
-}
-
-// Flop models (Golub & Van Loan counts).  Householder QR of m x n with
-// k = min(m, n) reflectors plus the thin Q: 4 m k^2 - 4/3 k^3 for m >= n,
-// 2 n m^2 + 2/3 m^3 for m < n.  Thin SVD with U and V of p x q (p >= q):
-// 14 p q^2 + 8 q^3 (22 q^3 for a square matrix).
-double qr_flops(double m, double n)
-{
-  return m >= n ? 4.0 * m * n * n - 4.0 / 3.0 * n * n * n : 2.0 * n * m * m + 2.0 / 3.0 * m * m * m;
-}
-double svd_flops(double p, double q)
-{
-  if (p < q) std::swap(p, q);
-  return 14.0 * p * q * q + 8.0 * q * q * q;
-}
-
-// Algorithmic work credited to one launch (not hardware counters):
-// flops of the textbook algorithm at that shape and the minimum HBM bytes
-// (each operand read once, each result written once).
-struct prof_scope
-{
-  tta_ctx* c;
-  int cls;
-  double flops, bytes;
-  hipEvent_t a = nullptr;
-  prof_scope(tta_ctx* c_, int cls_, double f = 0.0, double by = 0.0) : c(c_), cls(cls_), flops(f), bytes(by)
-  {
-    if (c->prof)
-    {
-      a = prof_event(c);
-      hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
-    }
-  }
-  ~prof_scope()
-  {
-    if (!a) return;
-    hipEvent_t b = prof_event(c);
-    if (hipEventRecord(b, c->stream) == hipSuccess) c->prof_pending.push_back({cls, a, b, flops, bytes});
-  }
-};
 
 void prof_collect(tta_ctx* c)
 {

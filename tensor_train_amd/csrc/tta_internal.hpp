@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <map>
 #include <set>
 #include <stdexcept>
@@ -112,3 +113,35 @@ namespace tta
 {
 double* ctx_ws(tta_ctx* c, int slot, size_t n);
 }
+
+namespace tta
+{
+hipEvent_t prof_event(tta_ctx* c);
+void hip_check(hipError_t e, const char* what);
+
+// Algorithmic work credited to one launch (not hardware counters):
+// flops of the textbook algorithm at that shape and the minimum HBM bytes
+// (each operand read once, each result written once).
+struct prof_scope
+{
+  tta_ctx* c;
+  int cls;
+  double flops, bytes;
+  hipEvent_t a = nullptr;
+  prof_scope(tta_ctx* c_, int cls_, double f = 0.0, double by = 0.0) : c(c_), cls(cls_), flops(f), bytes(by)
+  {
+    if (c->prof)
+    {
+      a = prof_event(c);
+      hip_check(hipEventRecord(a, c->stream), "hipEventRecord");
+    }
+  }
+  ~prof_scope()
+  {
+    if (!a) return;
+    hipEvent_t b = prof_event(c);
+    if (hipEventRecord(b, c->stream) == hipSuccess) c->prof_pending.push_back({cls, a, b, flops, bytes});
+  }
+};
+
+}  // namespace tta

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 namespace tta
 {
 
@@ -207,5 +209,59 @@ void launch_frob_batched(const sq_desc* ds, int count, hipStream_t st);
 void launch_col_norms_batched(const cn_desc* ds, int count, int max_cols, hipStream_t st);
 void launch_geo_mean_batched(const gm_desc* ds, int count, hipStream_t st);
 void launch_balance_batched(const bal_desc* ds, int count, long max_elems, hipStream_t st);
+
+// Flop models (Golub & Van Loan counts).  Householder QR of m x n with
+// k = min(m, n) reflectors plus the thin Q: 4 m k^2 - 4/3 k^3 for m >= n,
+// 2 n m^2 + 2/3 m^3 for m < n.  Thin SVD with U and V of p x q (p >= q):
+// 14 p q^2 + 8 q^3 (22 q^3 for a square matrix).
+inline double qr_flops(double m, double n)
+{
+  return m >= n ? 4.0 * m * n * n - 4.0 / 3.0 * n * n * n : 2.0 * n * m * m + 2.0 / 3.0 * m * m * m;
+}
+inline double svd_flops(double p, double q)
+{
+  if (p < q) std::swap(p, q);
+  return 14.0 * p * q * q + 8.0 * q * q * q;
+}
+
+// Algorithmic work of a batch of launches (credited to timed launches):
+// textbook flops at each problem's shape and each operand read / result
+// written once.
+struct credit
+{
+  double flops = 0.0, bytes = 0.0;
+};
+inline credit gemm_credit(const gemm_desc* g, size_t n)
+{
+  credit c;
+  for (size_t i = 0; i < n; ++i)
+  {
+    c.flops += 2.0 * g[i].M * g[i].N * (double)g[i].K;
+    c.bytes += 8.0 * ((double)g[i].M * g[i].K + (double)g[i].K * g[i].N + (double)g[i].M * g[i].N);
+  }
+  return c;
+}
+inline credit qr_credit(const qrp* q, size_t n)
+{
+  credit c;
+  for (size_t i = 0; i < n; ++i)
+  {
+    const double m = q[i].m, nn = q[i].n, k = nn < m ? nn : m;
+    c.flops += qr_flops(m, nn);
+    c.bytes += 8.0 * (m * nn + m * k);
+  }
+  return c;
+}
+inline credit jacobi_credit(const jac_prob* j, size_t n)
+{
+  credit c;
+  for (size_t i = 0; i < n; ++i)
+  {
+    const double q = j[i].q;
+    c.flops += svd_flops(q, q);
+    c.bytes += 32.0 * q * q;
+  }
+  return c;
+}
 
 }  // namespace tta

@@ -10,7 +10,8 @@
 //     -log10(eps / sqrt(d-1))/10,
 //     log10(c_j / c_0)/10 for j = 1, 2, 4, 8, 16, 32, 64 (j clamped to r_k-1),
 //     log10(c_{r_k-1} / c_0)/10 ]                                (13 values)
-// The label is the plain rounding's new rank at that bond over r_k.
+// The label is the plain rounding's new rank at that bond over r_k.  The
+// target rank is ceil(pred r_k (1 + slack)) + oversample, clipped to [1, r_k].
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -114,9 +115,11 @@ int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const* in, const
                                  int* fell_back);
 
 int tta_round_learned_batched(tta_ctx* c, tta_mlp* m, int count, tta_tt* const* in, double eps, int oversample,
-                              unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back, int* ranks_used)
+                              double slack, unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
+                              int* ranks_used)
 {
-  if (!c || !m || count < 0 || (count > 0 && (!in || !out)) || !(eps > 0.0) || oversample < 0) return TTA_ERR_ARG;
+  if (!c || !m || count < 0 || (count > 0 && (!in || !out)) || !(eps > 0.0) || oversample < 0 || !(slack >= 0.0))
+    return TTA_ERR_ARG;
   try
   {
     std::vector<float> F;
@@ -138,7 +141,7 @@ int tta_round_learned_batched(tta_ctx* c, tta_mlp* m, int count, tta_tt* const* 
       {
         const int r = in[i]->ranks[k];
         const double frac = std::min(1.0, std::max(0.0, (double)pred[b]));
-        int l = (int)std::ceil(frac * r) + oversample;
+        int l = (int)std::ceil(frac * r * (1.0 + slack)) + oversample;
         l = std::max(1, std::min(l, r));
         ranks.push_back(l);
       }

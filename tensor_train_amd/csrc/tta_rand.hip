@@ -96,6 +96,8 @@ void run_grouped(tta_ctx* c, owner& O, const std::vector<gemm_desc>& gs)
     mm = std::max(mm, g.M);
     nn = std::max(nn, g.N);
   }
+  const credit w = gemm_credit(gs.data(), gs.size());
+  prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
   launch_gemm_grouped(O.upload(gs), (int)gs.size(), mm, nn, c->stream);
 }
 }  // namespace
@@ -134,6 +136,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
         sd.push_back(seed + 0x9E3779B97F4A7C15ULL * (unsigned long long)(i + 1));
       }
     }
+    prof_scope ps(c, TTA_K_OTHER);
     launch_gauss_batched(dst.data(), m.data(), nc.data(), sd.data(), kk.data(), (int)dst.size(), c);
   }
 
@@ -322,8 +325,15 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     qb.max_m = max_m;
     qb.max_n = max_n;
     qb.form_q = true;
-    launch_qr_blocked(qb, c->stream);
-    launch_transpose_batched(S.upload(tr), (int)tr.size(), max_tr_r, max_tr_c, c->stream);
+    {
+      const credit w = qr_credit(qr.data(), qr.size());
+      prof_scope ps(c, TTA_K_QR, w.flops, w.bytes);
+      launch_qr_blocked(qb, c->stream);
+    }
+    {
+      prof_scope ps(c, TTA_K_OTHER);
+      launch_transpose_batched(S.upload(tr), (int)tr.size(), max_tr_r, max_tr_c, c->stream);
+    }
     run_grouped(c, S, gM);
     run_grouped(c, S, gZ);
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
