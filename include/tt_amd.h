@@ -236,6 +236,9 @@ int tta_prof_enable(tta_ctx* ctx, int on);
  * each launch's shape, not hardware counters) for one kernel class. */
 int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* total_ms, double* flops,
                   double* bytes);
+/* Block-Jacobi SVD calls (matrices above 64 columns) since the last
+ * tta_prof_enable and the sweeps they ran. */
+int tta_prof_jacobi_sweeps(tta_ctx* ctx, long long* calls, long long* sweeps);
 
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks

@@ -207,12 +207,18 @@ static void jacobi_columns(std::vector<double>& X, int p, int q, std::vector<dou
           b += xj[t] * xj[t];
           g += xi[t] * xj[t];
         }
-        if (g == 0.0 || std::fabs(g) <= tol * std::sqrt(a * b)) continue;
-        rotated = true;
+        // No rotation when the pair is orthogonal to tol, a squared norm
+        // underflowed to zero, or the angle rounds to zero (|zeta| > 1e150
+        // uses the asymptotic tangent 1 / (2 zeta)).
+        if (a == 0.0 || b == 0.0 || !(std::fabs(g) > tol * std::sqrt(a) * std::sqrt(b))) continue;
         const double zeta = (b - a) / (2.0 * g);
-        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1.0 + zeta * zeta));
+        const double az = std::fabs(zeta);
+        const double t =
+            (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + std::sqrt(1.0 + zeta * zeta));
         const double cs = 1.0 / std::sqrt(1.0 + t * t);
         const double sn = cs * t;
+        if (sn == 0.0) continue;
+        rotated = true;
         for (int u = 0; u < p; ++u)
         {
           const double x1 = xi[u], x2 = xj[u];

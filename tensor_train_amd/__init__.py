@@ -82,6 +82,12 @@ class Context:
                                           ctypes.byref(ms), ctypes.byref(fl), ctypes.byref(by)))
         return {"launches": n.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
 
+    def jacobi_sweeps(self):
+        """(calls, sweeps) of the block-Jacobi SVD since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_jacobi_sweeps(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def mem_stats(self):
         """{live_bytes, cached_bytes, device_allocs, trims} of the context's
         caching device allocator."""

@@ -82,6 +82,7 @@ SIGNATURES = {
     "tta_scale_core": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "tta_prof_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),
+    "tta_prof_jacobi_sweeps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tta_mem_stats": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_ulonglong)] * 4),
     "tta_mem_trim": (ctypes.c_int, [_vp]),
 }

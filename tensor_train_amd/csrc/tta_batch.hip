@@ -231,7 +231,7 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     std::vector<qrp> qr;
     std::vector<ex_desc> ex;
     std::vector<jac_prob> jp;
-    std::vector<double*> Qs, Xs, Js;
+    std::vector<double*> Qs, Xs, Js, Ss;
     std::vector<double*> Fs;
     std::vector<int*> perms;
     int max_m = 0, max_n = 0, max_q = 0, max_tr_r = 0, max_tr_c = 0;
@@ -289,9 +289,9 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       e.lda = m;
       e.kr = q;
       e.nc = q;
-      e.dst = X;
-      e.drs = is_tall ? q : 1;
-      e.dcs = is_tall ? 1 : q;
+      e.dst = X;  // X = R^T in both orientations (see svd_left)
+      e.drs = q;
+      e.dcs = 1;
       ex.push_back(e);
       max_ex = std::max(max_ex, (long)q * q);
       Xs.push_back(X);
@@ -302,6 +302,7 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       j.J = S.alloc<double>((size_t)q * q);
       j.ldj = q;
       j.s = S.alloc<double>((size_t)q);
+      Ss.push_back(j.s);
       j.perm = S.alloc<int>((size_t)q);
       j.rank_out = rk + i;
       j.delta_ptr = deltas + i;
@@ -343,7 +344,12 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     {
       const credit w = jacobi_credit(jp.data(), jp.size());
       prof_scope ps(c, TTA_K_JACOBI, w.flops, w.bytes);
-      launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c->stream);
+      const int sw = launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c->stream);
+      if (sw > 0)
+      {
+        c->jac_calls += 1;
+        c->jac_sweeps += sw;
+      }
     }
     std::vector<int> hr(count);
     hip_check(hipMemcpyAsync(hr.data(), rk, count * sizeof(int), hipMemcpyDeviceToHost, c->stream), "ranks");
@@ -392,10 +398,11 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       }
       else
       {
-        ga_desc d1{Js[a], q, q, perms[a], r_new, U, r_new, 1};
+        // M = W (Q J)^T: U = W diag(s)^-1, F = diag(s) (Q J)^T.
+        ga_desc d1{Xs[a], q, q, perms[a], r_new, U, r_new, 1, Ss[a], GATHER_DIV};
         g1.push_back(d1);
         double* Wp = S.alloc<double>((size_t)q * r_new);
-        ga_desc d2{Xs[a], q, q, perms[a], r_new, Wp, 1, q};
+        ga_desc d2{Js[a], q, q, perms[a], r_new, Wp, 1, q, Ss[a], GATHER_MUL};
         g2.push_back(d2);
         gemm_desc g;
         g.M = r_new;

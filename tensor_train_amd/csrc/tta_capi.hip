@@ -322,7 +322,12 @@ int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_ra
   b.one.sweeps = ints + 5;
   {
     prof_scope ps(c, TTA_K_JACOBI, svd_flops(q, q), 8.0 * 4.0 * (double)q * q);
-    launch_jacobi_svd(b, q, ints + 4, c->h_int + 2, c->stream);
+    const int sw = launch_jacobi_svd(b, q, ints + 4, c->h_int + 2, c->stream);
+    if (sw > 0)
+    {
+      c->jac_calls += 1;
+      c->jac_sweeps += sw;
+    }
   }
   *s_out = sv;
   *perm_out = perm;
@@ -333,9 +338,17 @@ int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_ra
 //   U_head (rows x r_new) row-major with orthonormal columns (products of
 //          orthogonal factors, exact up to rounding), in a fresh pool block;
 //   F      (r_new x cols) row-major = diag(s_head) Vt_head, in scratch.
-// The matrix is QR-condensed to its square triangular factor first:
-//   rows >= cols: M = Q R, Jacobi on R^T: R^T J = W -> M = (Q J) W^T;
-//   rows <  cols: M^T = Q R, Jacobi on R:  R J = W  -> M = J (Q W)^T.
+// The matrix is QR-condensed to its square triangular factor first, and the
+// Jacobi iteration always runs on the columns of R^T (rows of R): on graded
+// factors that orientation converges in a handful of sweeps where the
+// columns of R can take dozens.
+//   rows >= cols: M = Q R,   R^T J = W -> M = (Q J) W^T: U = Q J, F = W^T;
+//   rows <  cols: M^T = Q R, R^T J = W -> M = W (Q J)^T: U = W diag(s)^-1
+//                 (orthonormal to the Jacobi tolerance, which the last sweep
+//                 verifies pair by pair), F = diag(s) (Q J)^T.
+// keep_all (every singular triplet, zero ones included) keeps, for
+// rows < cols, the exact-rotation form R J = W -> M = J (Q W)^T, whose U is
+// orthonormal however many singular values vanish.
 int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int max_rank, bool keep_all,
              double** U_out, double** F_out, double** s_out = nullptr)
 {
@@ -384,13 +397,24 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   double* Q = ctx_ws(c, WS_Q, (size_t)cols * q);
   qr_factor(c, A, cols, cols, rows, tau, Q, cols);
   double* Y = ctx_ws(c, WS_X, (size_t)q * q);
-  launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);
+  if (keep_all)
+    launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);  // Y = R
+  else
+    launch_extract_upper(A, cols, q, q, Y, q, 1, c->stream);  // Y = R^T
   double* J = ctx_ws(c, WS_J, (size_t)q * q);
   const int r_new = square_svd(c, Y, q, J, delta, max_rank, keep_all, &sv, &perm);
   double* U = alloc_core(c, (size_t)rows * r_new);
-  launch_gather_cols(J, q, q, perm, r_new, U, r_new, 1, c->stream);
-  double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);
-  launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
+  double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);  // left factor of F^T
+  if (keep_all)
+  {
+    launch_gather_cols(J, q, q, perm, r_new, U, r_new, 1, c->stream);
+    launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
+  }
+  else
+  {
+    launch_gather_cols(Y, q, q, perm, r_new, U, r_new, 1, c->stream, sv, GATHER_DIV);
+    launch_gather_cols(J, q, q, perm, r_new, Wp, 1, q, c->stream, sv, GATHER_MUL);
+  }
   double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
   gemm_desc g;
   g.M = r_new;
@@ -428,10 +452,11 @@ void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
 }
 
 // svd_step_right (truncate.hpp:114-166): SVD of right_unfold(core k)
-// (r x n*rp) with V exactly orthonormal (the new core's rows) and
-// U diag(s) absorbed into core k-1.
-//   cols >= rows: M^T = Q R, Jacobi on R^T: R^T J = W -> M = W (Q J)^T;
-//   cols <  rows: M = Q R,   Jacobi on R:   R J = W   -> M = (Q W) J^T.
+// (r x n*rp) with V orthonormal (the new core's rows) and U diag(s)
+// absorbed into core k-1.  Jacobi runs on R^T as in svd_left:
+//   cols >= rows: M^T = Q R, R^T J = W -> M = W (Q J)^T: V = Q J, U s = W;
+//   cols <  rows: M = Q R,   R^T J = W -> M = (Q J) W^T: V = W diag(s)^-1,
+//                 U s = Q J diag(s).
 void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
 {
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
@@ -481,13 +506,13 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
     qr_factor(c, A, rows, rows, cols, tau, Q, rows);
     double* Y = ctx_ws(c, WS_X, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, Y, 1, q, c->stream);
+    launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream);  // Y = R^T
     double* J = ctx_ws(c, WS_J, (size_t)q * q);
     r_new = square_svd(c, Y, q, J, delta, max_rank, false, &sv, &perm);
     Cn = alloc_core(c, (size_t)r_new * cols);
-    launch_gather_cols(J, q, q, perm, r_new, Cn, 1, q, c->stream);
+    launch_gather_cols(Y, q, q, perm, r_new, Cn, 1, q, c->stream, sv, GATHER_DIV);
     double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);
-    launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
+    launch_gather_cols(J, q, q, perm, r_new, Wp, 1, q, c->stream, sv, GATHER_MUL);
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
     gemm_desc g;
     g.M = rows;
@@ -1334,7 +1359,17 @@ int tta_prof_enable(tta_ctx* c, int on)
     c->prof_flops[i] = 0.0;
     c->prof_bytes[i] = 0.0;
   }
+  c->jac_calls = 0;
+  c->jac_sweeps = 0;
   TTA_GUARD_END(c)
+}
+
+int tta_prof_jacobi_sweeps(tta_ctx* c, long long* calls, long long* sweeps)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (calls) *calls = c->jac_calls;
+  if (sweeps) *sweeps = c->jac_sweeps;
+  return TTA_OK;
 }
 
 int tta_prof_read(tta_ctx* c, int cls, long long* launches, double* total_ms, double* flops, double* bytes)

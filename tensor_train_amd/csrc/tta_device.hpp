@@ -181,6 +181,24 @@ __device__ void org2r(const double* __restrict__ A, int lda, int m, int k, const
   }
 }
 
+// Jacobi rotation that orthogonalises columns x_i, x_j from a = |x_i|^2,
+// b = |x_j|^2, g = x_i . x_j (Rutishauser's formulas).  Returns false (no
+// rotation) when |g| <= tol |x_i| |x_j|, when a squared norm underflowed to
+// zero (the column is negligible and its products would never settle), when
+// g is not finite, or when the angle rounds to zero; so every rotation
+// reported changes the columns.  For |zeta| > 1e150 the tangent takes its
+// asymptotic form 1 / (2 zeta), where zeta^2 would overflow.
+__device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn)
+{
+  if (a == 0.0 || b == 0.0 || !(fabs(g) > tol * sqrt(a) * sqrt(b))) return false;
+  const double zeta = (b - a) / (2.0 * g);
+  const double az = fabs(zeta);
+  const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + sqrt(1.0 + zeta * zeta));
+  cs = 1.0 / sqrt(1.0 + t * t);
+  sn = cs * t;
+  return sn != 0.0;
+}
+
 // Circle-method round-robin pairing of qq (even) players: round s in
 // [0, qq-1), pair p in [0, qq/2).  Every unordered pair meets exactly once
 // per qq-1 rounds and the pairs of one round are disjoint.
@@ -253,12 +271,9 @@ __device__ int jacobi_sweeps(double* __restrict__ X, int ldx, int p, int q, doub
         a = wave_sum(a);
         b = wave_sum(b);
         g = wave_sum(g);
-        if (g == 0.0 || fabs(g) <= tol * sqrt(a * b)) continue;
+        double cs, sn;
+        if (!jacobi_rotation(a, b, g, tol, cs, sn)) continue;
         rotated = 1;
-        const double zeta = (b - a) / (2.0 * g);
-        const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / sqrt(1.0 + tt * tt);
-        const double sn = cs * tt;
         for (int t = lane; t < p; t += kWave)
         {
           const double u = xi[t], v = xj[t];

@@ -95,6 +95,9 @@ struct tta_ctx
   long long prof_n[8] = {0};
   double prof_flops[8] = {0};
   double prof_bytes[8] = {0};
+  // Block-Jacobi launches (q > 64) since the last tta_prof_enable and the
+  // block sweeps they ran (host-driven, so counted whether or not timing is on).
+  long long jac_calls = 0, jac_sweeps = 0;
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])

@@ -15,6 +15,14 @@
 // product; below it a "rotation" only chases rounding noise and a sweep
 // would never come out clean.  J stays orthogonal to working precision
 // whatever tol is, so the singular vectors built from it do too.
+// Negligible columns: a column with |x|^2 <= negl2 takes part in no
+// rotation, where negl2 = min(delta^2 / (4 q), (q u)^2 ||X||_F^2) with delta
+// the truncation threshold.  Leaving such columns alone perturbs X by at
+// most the backward error of the QR that produced it (q u ||X||), and all of
+// them together hold less than delta^2 / 4, so the truncation rule drops
+// every one of them; rotating them only stirs rounding noise between
+// columns that are discarded, which costs sweeps.  delta = 0 (no truncation,
+// full SVD) disables the rule.
 #include "tta_device.hpp"
 #include "tta_kernels.hpp"
 
@@ -31,6 +39,14 @@ __device__ __forceinline__ const jac_prob& jpick(const jac_batch& b, int idx)
 }
 
 __device__ __forceinline__ double jac_tol(int q) { return fmax(1.0e-15, (double)q * 2.220446049250313e-16); }
+
+// Squared-norm bound of negligible columns from ||X||_F^2 (see above).
+__device__ __forceinline__ double jac_negl2(const jac_prob& P, double xnorm2)
+{
+  const double delta = P.delta_ptr ? *P.delta_ptr : P.delta;
+  const double qu = (double)P.q * 2.220446049250313e-16;
+  return fmin(delta * delta / (4.0 * P.q), qu * qu * xnorm2);
+}
 
 // Sums of three values over the wave, DPP stages interleaved so the three
 // chains overlap.
@@ -59,7 +75,7 @@ __device__ __forceinline__ void wave_sum3(double& a, double& b, double& g)
 // must be <= 64 * PL).
 template <int PL>
 __device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, double* jj, int p, int q, double tol,
-                                           int lane)
+                                           double negl2, int lane)
 {
   double u[PL], v[PL];
   double a = 0.0, b = 0.0, g = 0.0;
@@ -74,11 +90,8 @@ __device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, d
     g = fma(u[e], v[e], g);
   }
   wave_sum3(a, b, g);
-  if (g == 0.0 || fabs(g) <= tol * sqrt(a * b)) return 0;
-  const double zeta = (b - a) / (2.0 * g);
-  const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-  const double cs = 1.0 / sqrt(1.0 + tt * tt);
-  const double sn = cs * tt;
+  double cs, sn;
+  if (a <= negl2 || b <= negl2 || !jacobi_rotation(a, b, g, tol, cs, sn)) return 0;
 #pragma unroll
   for (int e = 0; e < PL; ++e)
   {
@@ -121,6 +134,10 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
     Js[e] = (r == c) ? 1.0 : 0.0;
   }
   __syncthreads();
+  __shared__ double red[NT / 64];
+  double part = 0.0;
+  for (int e = tid; e < q * q; e += NT) part = fma(Xs[e], Xs[e], part);
+  const double negl2 = jac_negl2(P, block_sum<NT>(part, red));
   const double tol = jac_tol(q);
   const int qq = (q + 1) & ~1;
   int sweep = 0;
@@ -142,7 +159,7 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
           ci = cj;
           cj = t;
         }
-        rot |= rotate_pair<1>(Xs + ci * q, Xs + cj * q, Js + ci * q, Js + cj * q, q, q, tol, lane);
+        rot |= rotate_pair<1>(Xs + ci * q, Xs + cj * q, Js + ci * q, Js + cj * q, q, q, tol, negl2, lane);
       }
       __syncthreads();
     }
@@ -201,6 +218,7 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   }
   __syncthreads();
   const double tol = jac_tol(q);
+  const double negl2 = jac_negl2(P, P.s[0]);  // ||X||_F^2 from jac_init_kernel
   int rot = 0;
   for (int it = 0; it < inner_sweeps; ++it)
   {
@@ -217,7 +235,7 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
           lj = t;
         }
         if ((li % W) >= ncol[li / W] || (lj % W) >= ncol[lj / W]) continue;
-        rot |= rotate_pair<PL>(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, lane);
+        rot |= rotate_pair<PL>(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, negl2, lane);
       }
       __syncthreads();
     }
@@ -258,6 +276,24 @@ __global__ void jac_init_kernel(jac_batch bt)
   }
 }
 
+// s[0] = ||X||_F^2 (one workgroup per problem); s is not otherwise used
+// before jac_finish_kernel writes the singular values.
+template <int NT>
+__global__ __launch_bounds__(NT) void jac_norm_kernel(jac_batch bt)
+{
+  const jac_prob& P = jpick(bt, blockIdx.x);
+  __shared__ double red[NT / 64];
+  const int q = P.q;
+  double part = 0.0;
+  for (long e = threadIdx.x; e < (long)q * q; e += NT)
+  {
+    const double v = P.X[(size_t)(e / q) * P.ldx + (e % q)];
+    part = fma(v, v, part);
+  }
+  const double t = block_sum<NT>(part, red);
+  if (threadIdx.x == 0) P.s[0] = t;
+}
+
 // After a sweep: flag[1] = (no rotation in the sweep); flag[0] reset;
 // counts converged problems into *done.
 __global__ void jac_sweep_end_kernel(jac_batch bt, int batch, int* done)
@@ -290,10 +326,11 @@ __global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
 
 int jacobi_block_threshold() { return 64; }
 
-void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st)
+int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st)
 {
   const int batch = b.descs ? b.batch : 1;
-  if (batch <= 0 || max_q <= 0) return;
+  if (batch <= 0 || max_q <= 0) return 0;
+  int sweeps_run = -1;
   if (max_q <= 64)
   {
     constexpr int NT = 512;
@@ -334,7 +371,7 @@ void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_h
         jd.max_rank = b.one.max_rank;
         launch_jacobi(jd, 1, st);
       }
-      return;
+      return -1;
     }
     const int W = (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
     const int nb = (max_q + W - 1) / W;
@@ -344,9 +381,11 @@ void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_h
       long g = ((long)max_q * max_q + 255) / 256;
       if (g > 1024) g = 1024;
       hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+      hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
     }
     for (int sweep = 0; sweep < 30; ++sweep)
     {
+      sweeps_run = sweep + 1;
       for (int s = 0; s < nbb - 1; ++s)
       {
         const dim3 grid(nbb / 2, batch);
@@ -371,6 +410,7 @@ void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_h
   }
   constexpr int NF = 256;
   hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);
+  return sweeps_run;
 }
 
 }  // namespace tta

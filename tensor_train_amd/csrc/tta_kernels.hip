@@ -167,14 +167,26 @@ __global__ void extract_upper_kernel(const double* A, int lda, int kr, int nc, d
 }
 
 // dst[t*drs + o*dcs] = src[t + perm[o]*lds] for t < rows, o < ncols.
+// Column o of the result is column perm[o] of src, times scale[o]
+// (mode GATHER_MUL) or divided by it (GATHER_DIV: unit vector e_o when
+// scale[o] == 0, the normalisation of a zero column).
+__device__ __forceinline__ double gathered(const double* src, int lds_, int t, int o, const int* perm,
+                                           const double* scale, int mode)
+{
+  const double v = src[t + (size_t)perm[o] * lds_];
+  if (mode == GATHER_MUL) return v * scale[o];
+  if (mode == GATHER_DIV) return scale[o] != 0.0 ? v / scale[o] : (t == o ? 1.0 : 0.0);
+  return v;
+}
+
 __global__ void gather_cols_kernel(const double* src, int lds_, int rows, const int* perm, int ncols, double* dst,
-                                   long drs, long dcs)
+                                   long drs, long dcs, const double* scale, int mode)
 {
   const long total = (long)rows * ncols;
   for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
   {
     const int o = (int)(x / rows), t = (int)(x % rows);
-    dst[t * drs + o * dcs] = src[t + (size_t)perm[o] * lds_];
+    dst[t * drs + o * dcs] = gathered(src, lds_, t, o, perm, scale, mode);
   }
 }
 
@@ -411,7 +423,7 @@ __global__ void gather_cols_batched_kernel(const ga_desc* ds)
   for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
   {
     const int o = (int)(x / d.rows), t = (int)(x % d.rows);
-    d.dst[t * d.drs + o * d.dcs] = d.src[t + (size_t)d.perm[o] * d.lds];
+    d.dst[t * d.drs + o * d.dcs] = gathered(d.src, d.lds, t, o, d.perm, d.scale, d.mode);
   }
 }
 
@@ -522,10 +534,10 @@ void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst,
 }
 
 void launch_gather_cols(const double* src, int lds_, int rows, const int* perm, int ncols, double* dst, long drs,
-                        long dcs, hipStream_t st)
+                        long dcs, hipStream_t st, const double* scale, int mode)
 {
   hipLaunchKernelGGL(gather_cols_kernel, dim3(grid_for((long)rows * ncols)), dim3(256), 0, st, src, lds_, rows, perm,
-                     ncols, dst, drs, dcs);
+                     ncols, dst, drs, dcs, scale, mode);
 }
 
 void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st)

@@ -121,7 +121,15 @@ struct ex_desc  // dst[i*drs + c*dcs] = (i <= c) ? A[c*lda + i] : 0
   double* dst = nullptr;
   long drs = 0, dcs = 0;
 };
-struct ga_desc  // dst[t*drs + o*dcs] = src[t + perm[o]*lds]
+// Column gathers: dst[t*drs + o*dcs] = src[t + perm[o]*lds], optionally
+// multiplied by (GATHER_MUL) or divided by (GATHER_DIV) scale[o].
+enum
+{
+  GATHER_COPY = 0,
+  GATHER_MUL = 1,
+  GATHER_DIV = 2
+};
+struct ga_desc
 {
   const double* src = nullptr;
   int lds = 0, rows = 0;
@@ -129,6 +137,8 @@ struct ga_desc  // dst[t*drs + o*dcs] = src[t + perm[o]*lds]
   int ncols = 0;
   double* dst = nullptr;
   long drs = 0, dcs = 0;
+  const double* scale = nullptr;
+  int mode = GATHER_COPY;
 };
 struct cp_desc
 {
@@ -179,12 +189,15 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
-void launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st);
+// Returns the number of block sweeps the host ran (q > 64), or -1 when the
+// sweeps ran inside one kernel.
+int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st);
 void launch_qr(const qr_desc& q, int batch, hipStream_t st);
 void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
 void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
 void launch_gather_cols(const double* src, int lds, int rows, const int* perm, int ncols, double* dst, long drs,
-                        long dcs, hipStream_t st);
+                        long dcs, hipStream_t st,
+                        const double* scale = nullptr, int mode = GATHER_COPY);
 void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st);
 void launch_permute(const permute_desc& p, hipStream_t st);
 void launch_matvec_core(const double* A, const double* x, double* y, int r, int m, int n, int rp, int s, int sp,

@@ -83,8 +83,8 @@ def test_wave_sum_dpp_matches_bpermute():
 
 @pytest.mark.parametrize("decades", [4, 12, 20])
 def test_svd_graded_large_dynamic_range(decades):
-    # Block Jacobi with Gram inner solves on spectra spanning many decades,
-    # plus an exactly rank-deficient block: backward-stable accuracy
+    # Block Jacobi on spectra spanning many decades, plus an exactly
+    # rank-deficient block: backward-stable accuracy
     # (absolute errors ~ u * s_max, as LAPACK and Eigen's BDCSVD give).
     rng = np.random.default_rng(decades)
     q = 256
@@ -99,3 +99,26 @@ def test_svd_graded_large_dynamic_range(decades):
     assert np.abs(U.T @ U - np.eye(q)).max() <= 1e-12
     keep = s > 1e-12
     assert np.abs((U[:, keep] * s[keep]) @ Vt[keep] - M).max() <= 1e-13
+
+
+@pytest.mark.parametrize("q", [48, 100])
+def test_svd_negligible_columns_converge(q):
+    # Columns whose squared norms underflow to zero (entries ~1e-170) are
+    # negligible: the Jacobi sweep must stop rotating them instead of
+    # applying angle-zero rotations forever, and the factorization stays
+    # backward stable.
+    rng = np.random.default_rng(q)
+    M = rng.standard_normal((2 * q, q))
+    M[:, 7] *= 1e-170
+    M[:, q // 2] *= 1e-200
+    ctx = T.Context.default()
+    ctx.prof_enable(True)
+    U, s, Vt = T.svd_thin(M)
+    calls, sweeps = ctx.jacobi_sweeps()
+    ctx.prof_enable(False)
+    if q > 64:
+        assert calls == 1 and sweeps <= 24  # the sweep cap is 30
+    s_ref = np.linalg.svd(M, compute_uv=False)
+    assert np.abs(s - s_ref).max() <= 1e-13 * s_ref[0]
+    assert np.abs(U[:, :q - 2].T @ U[:, :q - 2] - np.eye(q - 2)).max() <= 1e-12
+    assert np.abs((U * s) @ Vt - M).max() <= 1e-13 * s_ref[0]
