@@ -28,36 +28,11 @@ namespace tta
 namespace
 {
 
-// Device allocations owned by one batched call, returned to the pool at the end.
-struct scratch
-{
-  tta_ctx* c;
-  std::vector<void*> blocks;
-  explicit scratch(tta_ctx* c_) : c(c_) {}
-  ~scratch()
-  {
-    for (void* p : blocks) c->mem.put(p);
-  }
-  template <class T>
-  T* alloc(size_t n)
-  {
-    void* p = c->mem.get(std::max<size_t>(n, 1) * sizeof(T));
-    blocks.push_back(p);
-    return static_cast<T*>(p);
-  }
-  template <class T>
-  T* upload(const std::vector<T>& v)
-  {
-    T* d = alloc<T>(v.size());
-    if (!v.empty())
-      hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), "descs");
-    return d;
-  }
-};
+using scratch = arena;  // per-step device buffers
 
 double* new_block(tta_ctx* c, size_t n) { return static_cast<double*>(c->mem.get(std::max<size_t>(n, 1) * 8)); }
 
-void sync(tta_ctx* c) { hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize"); }
+void sync(tta_ctx* c) { ctx_sync(c); }
 
 }  // namespace
 
@@ -164,7 +139,7 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       t->cores[k - 1] = newP[a];
       t->ranks[k] = std::min(t->modes[k] * t->ranks[k + 1], t->ranks[k]);
     }
-    sync(c);  // descriptors and scratch of this step are released next
+    // The step's scratch returns to the stream-ordered pool: no wait needed.
   }
 
   // ------------------------------------------------------------ norms, delta
@@ -465,7 +440,6 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
       t->cores[k + 1] = Nn[a];
       t->ranks[k + 1] = hr[idx[a]];
     }
-    sync(c);
   }
 
   // ------------------------------------------------------------ balancing

@@ -988,6 +988,7 @@ int tta_ctx_destroy(tta_ctx* c)
   (void)hipStreamSynchronize(c->stream);
   for (auto& w : c->ws)
     if (w.p) c->mem.put(w.p);
+  free_stage(c);
   (void)hipHostFree(c->h_int);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1000,7 +1001,7 @@ int tta_synchronize(tta_ctx* c)
 {
   if (!c) return TTA_ERR_ARG;
   TTA_GUARD_BEGIN
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  ctx_sync(c);
   TTA_GUARD_END(c)
 }
 

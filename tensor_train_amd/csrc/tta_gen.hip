@@ -159,7 +159,7 @@ T* upload(tta_ctx* c, const std::vector<T>& v, std::vector<void*>& own)
 {
   void* p = c->mem.get(std::max<size_t>(v.size(), 1) * sizeof(T));
   own.push_back(p);
-  if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), "descs");
+  upload_async(c, p, v.data(), v.size() * sizeof(T));
   return static_cast<T*>(p);
 }
 
@@ -216,8 +216,7 @@ void launch_gauss_batched(const double* const* dst, const int* m, const int* nc,
     const int chunk = (int)std::min<size_t>(65535, gd.size() - off);
     hipLaunchKernelGGL(gauss_kernel, dim3(g < 1 ? 1 : g, chunk), dim3(256), 0, c->stream, dd + off);
   }
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-  for (void* p : own) c->mem.put(p);
+  for (void* p : own) c->mem.put(p);  // stream-ordered reuse
 }
 
 }  // namespace tta

@@ -79,6 +79,12 @@ struct tta_ctx
   tta::ws_slot ws[16];
   int* h_int = nullptr;     // pinned host scalars
   double* h_dbl = nullptr;
+  // Pinned staging for host-to-device descriptor uploads: a copy from
+  // pageable memory would wait for the stream, so uploads go through this
+  // buffer and stay asynchronous.  Offsets restart after each stream
+  // synchronisation (tta::ctx_sync).
+  char* stage = nullptr;
+  size_t stage_cap = 0, stage_off = 0;
   std::string last_error;
   // Per-launch event timing (tta_prof_enable): recorded pairs awaiting
   // tta_prof_read, recycled events, totals per kernel class.
@@ -119,6 +125,14 @@ double* ctx_ws(tta_ctx* c, int slot, size_t n);
 
 namespace tta
 {
+// Asynchronous upload of `bytes` host bytes to device memory `dev` on the
+// context stream, through the pinned staging buffer (the source may be
+// reused or freed on return).
+void upload_async(tta_ctx* c, void* dev, const void* src, size_t bytes);
+// Stream synchronisation; also recycles the staging buffer.
+void ctx_sync(tta_ctx* c);
+void free_stage(tta_ctx* c);
+
 hipEvent_t prof_event(tta_ctx* c);
 void hip_check(hipError_t e, const char* what);
 
@@ -145,6 +159,52 @@ struct prof_scope
     hipEvent_t b = prof_event(c);
     if (hipEventRecord(b, c->stream) == hipSuccess) c->prof_pending.push_back({cls, a, b, flops, bytes});
   }
+};
+
+// Bump allocator over large pool blocks for the many short-lived buffers
+// of a batched step (one pool call per chunk instead of one per buffer).
+// Buffers are 256-byte aligned and all return to the pool when the arena
+// is destroyed; stream ordering makes that safe while kernels still queue.
+class arena
+{
+ public:
+  explicit arena(tta_ctx* c, size_t chunk_bytes = size_t(64) << 20) : c_(c), chunk_(chunk_bytes) {}
+  arena(const arena&) = delete;
+  arena& operator=(const arena&) = delete;
+  ~arena()
+  {
+    for (void* p : chunks_) c_->mem.put(p);
+  }
+  template <class T>
+  T* alloc(size_t n)
+  {
+    const size_t b = (std::max<size_t>(n, 1) * sizeof(T) + 255) / 256 * 256;
+    if (b > left_)
+    {
+      const size_t sz = std::max(chunk_, b);
+      cur_ = static_cast<char*>(c_->mem.get(sz));
+      chunks_.push_back(cur_);
+      left_ = sz;
+    }
+    T* r = reinterpret_cast<T*>(cur_);
+    cur_ += b;
+    left_ -= b;
+    return r;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v)
+  {
+    T* d = alloc<T>(v.size());
+    upload_async(c_, d, v.data(), v.size() * sizeof(T));
+    return d;
+  }
+
+ private:
+  tta_ctx* c_;
+  size_t chunk_;
+  std::vector<void*> chunks_;
+  char* cur_ = nullptr;
+  size_t left_ = 0;
 };
 
 }  // namespace tta

@@ -54,13 +54,13 @@ void bond_features(tta_ctx* c, int count, tta_tt* const* in, double eps, std::ve
     for (int k = 1; k < in[i]->d(); ++k, ++b)
       cn.push_back(cn_desc{in[i]->cores[k - 1], in[i]->ranks[k - 1] * in[i]->modes[k - 1], in[i]->ranks[k], nrm + off[b]});
   void* dd = c->mem.get(cn.size() * sizeof(cn_desc));
-  hip_check(hipMemcpyAsync(dd, cn.data(), cn.size() * sizeof(cn_desc), hipMemcpyHostToDevice, c->stream), "descs");
+  upload_async(c, dd, cn.data(), cn.size() * sizeof(cn_desc));
   for (size_t o = 0; o < cn.size(); o += 65535)
     launch_col_norms_batched(static_cast<cn_desc*>(dd) + o, (int)std::min<size_t>(65535, cn.size() - o), max_cols,
                              c->stream);
   std::vector<double> h(total);
   hip_check(hipMemcpyAsync(h.data(), nrm, total * 8, hipMemcpyDeviceToHost, c->stream), "norms");
-  hip_check(hipStreamSynchronize(c->stream), "sync");
+  ctx_sync(c);
   c->mem.put(dd);
   c->mem.put(nrm);
   b = 0;

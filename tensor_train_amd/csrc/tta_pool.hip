@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "../../include/tt_amd.h"
 #include "tta_internal.hpp"
@@ -128,6 +129,43 @@ void pool::trim()
       keep.push_back(s);
   }
   segments_.swap(keep);
+}
+
+void upload_async(tta_ctx* c, void* dev, const void* src, size_t bytes)
+{
+  if (bytes == 0) return;
+  const size_t need = round_up(bytes, 256);
+  if (c->stage_off + need > c->stage_cap)
+  {
+    // Every staged copy so far must finish before the buffer is reused.
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    c->stage_off = 0;
+    if (need > c->stage_cap)
+    {
+      if (c->stage) (void)hipHostFree(c->stage);
+      c->stage = nullptr;
+      const size_t cap = std::max(need, std::max<size_t>(2 * c->stage_cap, size_t(8) << 20));
+      hip_check(hipHostMalloc(reinterpret_cast<void**>(&c->stage), cap, hipHostMallocDefault), "hipHostMalloc");
+      c->stage_cap = cap;
+    }
+  }
+  char* p = c->stage + c->stage_off;
+  std::memcpy(p, src, bytes);
+  c->stage_off += need;
+  hip_check(hipMemcpyAsync(dev, p, bytes, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+}
+
+void ctx_sync(tta_ctx* c)
+{
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  c->stage_off = 0;
+}
+
+void free_stage(tta_ctx* c)
+{
+  if (c->stage) (void)hipHostFree(c->stage);
+  c->stage = nullptr;
+  c->stage_cap = c->stage_off = 0;
 }
 
 pool::~pool()

@@ -20,6 +20,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <memory>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/tt_amd.h"
@@ -34,41 +36,6 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
 
 namespace
 {
-struct owner
-{
-  tta_ctx* c;
-  std::vector<void*> blocks;
-  explicit owner(tta_ctx* c_) : c(c_) {}
-  ~owner()
-  {
-    for (void* p : blocks) c->mem.put(p);
-  }
-  template <class T>
-  T* alloc(size_t n)
-  {
-    void* p = c->mem.get(std::max<size_t>(n, 1) * sizeof(T));
-    blocks.push_back(p);
-    return static_cast<T*>(p);
-  }
-  template <class T>
-  T* upload(const std::vector<T>& v)
-  {
-    T* d = alloc<T>(v.size());
-    if (!v.empty())
-      hip_check(hipMemcpyAsync(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), "descs");
-    return d;
-  }
-  void release(void* p)
-  {
-    auto it = std::find(blocks.begin(), blocks.end(), p);
-    if (it != blocks.end())
-    {
-      c->mem.put(p);
-      blocks.erase(it);
-    }
-  }
-};
-
 gemm_desc rm_gemm(int M, int N, int K, const double* A, long lda, const double* B, long ldb, double* C, long ldc)
 {
   gemm_desc g;
@@ -87,7 +54,7 @@ gemm_desc rm_gemm(int M, int N, int K, const double* A, long lda, const double* 
   return g;
 }
 
-void run_grouped(tta_ctx* c, owner& O, const std::vector<gemm_desc>& gs)
+void run_grouped(tta_ctx* c, arena& O, const std::vector<gemm_desc>& gs)
 {
   if (gs.empty()) return;
   int mm = 0, nn = 0;
@@ -103,17 +70,26 @@ void run_grouped(tta_ctx* c, owner& O, const std::vector<gemm_desc>& gs)
 }  // namespace
 
 // Randomized rounding of the working TTs `ts` (replaced in place by yhat).
-// ell[i] holds d_i + 1 target ranks (ell[i][0] = ell[i][d] = 1).  On return
+// ell[i] holds d_i + 1 target ranks (ell[i][0] = ell[i][d] = 1).  Cores in
+// `borrowed` belong to the caller's inputs: they are only read, and every
+// core of a TT with d >= 2 is replaced by a new one.  On return
 // rel_err[i] = ||y - yhat|| / ||y|| (from the norm identity; 0 for a zero y).
 void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<std::vector<int>>& ell,
-                              unsigned long long seed, std::vector<double>& rel_err)
+                              unsigned long long seed, const std::unordered_set<const double*>& borrowed,
+                              std::vector<double>& rel_err)
 {
+  auto release_core = [&](double* p) {
+    if (!borrowed.count(p)) c->mem.put(p);
+  };
   const int count = (int)ts.size();
   rel_err.assign(count, 0.0);
   int maxd = 0;
   for (auto* t : ts) maxd = std::max(maxd, t->d());
   if (count == 0) return;
-  owner G(c);
+  // Device buffers by lifetime: sketch cores (phases 1-2), W_k (phases 2-4),
+  // Gram environments (two live steps at a time), per-step scratch.
+  auto Rar = std::make_unique<arena>(c);
+  arena War(c);
 
   // 1. Sketch cores (l_{k-1} n_k x l_k row-major).
   std::vector<std::vector<double*>> R(count);
@@ -127,7 +103,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       for (int k = 0; k < t->d(); ++k)
       {
         const int rows = ell[i][k] * t->modes[k], cols = ell[i][k + 1];
-        double* p = G.alloc<double>((size_t)rows * cols);
+        double* p = Rar->alloc<double>((size_t)rows * cols);
         R[i].push_back(p);
         dst.push_back(p);
         m.push_back(rows);
@@ -145,7 +121,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
   for (int i = 0; i < count; ++i) W[i].assign(ts[i]->d() + 1, nullptr);
   for (int j = 0; j <= maxd - 1; ++j)
   {
-    owner S(c);
+    arena S(c);
     std::vector<gemm_desc> g1, g2;
     for (int i = 0; i < count; ++i)
     {
@@ -161,7 +137,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
         g1.push_back(rm_gemm(r * n, lp, rp, t->cores[k], rp, W[i][k + 1], lp, tmp, lp));
         T1 = tmp;
       }
-      double* Wk = G.alloc<double>((size_t)r * l);
+      double* Wk = War.alloc<double>((size_t)r * l);
       W[i][k] = Wk;
       gemm_desc g;
       g.M = r;
@@ -180,21 +156,22 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     }
     run_grouped(c, S, g1);
     run_grouped(c, S, g2);
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   }
+  Rar.reset();  // sketch cores are consumed (stream-ordered release)
 
   // 3. ||y||^2 by the left-to-right Gram environment (two GEMMs per core).
+  // E starts as [1] for every TT; each step's environments live in their own
+  // arena, released once the next step has been queued.
   std::vector<double*> E(count, nullptr);
-  for (int i = 0; i < count; ++i)
+  auto Ecur = std::make_unique<arena>(c, size_t(16) << 20);
   {
-    E[i] = G.alloc<double>(1);
-    const double one = 1.0;
-    hip_check(hipMemcpyAsync(E[i], &one, 8, hipMemcpyHostToDevice, c->stream), "one");
+    double* e0 = Ecur->upload(std::vector<double>((size_t)count, 1.0));
+    for (int i = 0; i < count; ++i) E[i] = e0 + i;
   }
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   for (int k = 0; k < maxd; ++k)
   {
-    owner S(c);
+    arena S(c);
+    auto Enext = std::make_unique<arena>(c);
     std::vector<gemm_desc> g1, g2;
     std::vector<std::pair<int, double*>> next;
     for (int i = 0; i < count; ++i)
@@ -217,7 +194,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       a.scr = (long)n * rp;
       a.scc = 1;
       g1.push_back(a);
-      double* En = G.alloc<double>((size_t)rp * rp);
+      double* En = Enext->alloc<double>((size_t)rp * rp);
       gemm_desc b;  // E' (rp x rp) = tmp_as^T (rp x r n) * left_unfold(Y_k)
       b.M = rp;
       b.N = rp;
@@ -236,25 +213,44 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     }
     run_grouped(c, S, g1);
     run_grouped(c, S, g2);
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    for (auto& pr : next)
-    {
-      G.release(E[pr.first]);
-      E[pr.first] = pr.second;
-    }
+    // TTs shorter than k keep their final environment: carry it over.
+    std::vector<cp_desc> carry;
+    std::vector<char> moved(count, 0);
+    for (auto& pr : next) moved[pr.first] = 1;
+    for (int i = 0; i < count; ++i)
+      if (!moved[i])
+      {
+        double* keep = Enext->alloc<double>(1);
+        carry.push_back(cp_desc{E[i], keep, 1});
+        E[i] = keep;
+      }
+    if (!carry.empty()) launch_copy_batched(S.upload(carry), (int)carry.size(), 1, c->stream);
+    for (auto& pr : next) E[pr.first] = pr.second;
+    Ecur = std::move(Enext);
   }
+  // ||y||^2 = the final 1 x 1 environment of each TT, gathered in one launch.
   std::vector<double> ny2(count);
-  for (int i = 0; i < count; ++i)
-    hip_check(hipMemcpyAsync(&ny2[i], E[i], 8, hipMemcpyDeviceToHost, c->stream), "norm");
+  arena G(c, size_t(1) << 20);
+  {
+    double* n2 = G.alloc<double>((size_t)count);
+    std::vector<cp_desc> cp;
+    for (int i = 0; i < count; ++i) cp.push_back(cp_desc{E[i], n2 + i, 1});
+    launch_copy_batched(G.upload(cp), count, 1, c->stream);
+    hip_check(hipMemcpyAsync(ny2.data(), n2, (size_t)count * 8, hipMemcpyDeviceToHost, c->stream), "norms");
+  }
+  Ecur.reset();
 
   // 4. Left-to-right QR sweep of the sketched cores.
   std::vector<double*> Z(count, nullptr);
   std::vector<int> zl(count, 1);  // left rank of Z
   for (int i = 0; i < count; ++i) Z[i] = ts[i]->cores[0];
-  std::vector<char> z_owned(count, 0);
+  // Z_{k+1} lives in the next step's arena, except at a TT's last step, where
+  // it becomes the TT's last core (a pool block of its own).
+  auto Zcur = std::make_unique<arena>(c);
   for (int k = 0; k <= maxd - 2; ++k)
   {
-    owner S(c);
+    arena S(c);
+    auto Znext = std::make_unique<arena>(c);
     std::vector<int> idx;
     std::vector<gemm_desc> gS, gM, gZ;
     std::vector<qrp> qr;
@@ -312,7 +308,8 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       gM.push_back(gm);
       // Z_{k+1} (l2 x n' r'') = M * right_unfold(Y_{k+1}).
       const int nc2 = t->modes[k + 1] * t->ranks[k + 2];
-      double* Zn = static_cast<double*>(c->mem.get((size_t)std::max(l2 * nc2, 1) * 8));
+      double* Zn = k == t->d() - 2 ? static_cast<double*>(c->mem.get((size_t)std::max(l2 * nc2, 1) * 8))
+                                   : Znext->alloc<double>((size_t)l2 * nc2);
       newZ.push_back(Zn);
       gZ.push_back(rm_gemm(l2, nc2, rp, Mm, rp, t->cores[k + 1], nc2, Zn, nc2));
       newl.push_back(l2);
@@ -336,23 +333,23 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     }
     run_grouped(c, S, gM);
     run_grouped(c, S, gZ);
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    // stream-ordered: the next step may reuse this step's scratch
     for (size_t a = 0; a < idx.size(); ++a)
     {
       const int i = idx[a];
       tta_tt* t = ts[i];
-      if (z_owned[i]) c->mem.put(Z[i]);
-      c->mem.put(t->cores[k]);
+      release_core(t->cores[k]);
       t->cores[k] = newcore[a];
       t->ranks[k + 1] = newl[a];
       Z[i] = newZ[a];
-      z_owned[i] = 1;
       zl[i] = newl[a];
     }
+    Zcur = std::move(Znext);
   }
+  Zcur.reset();
   // Last core <- Z; ||yhat|| = ||Z||.
   {
-    owner S(c);
+    arena S(c, size_t(1) << 20);
     std::vector<sq_desc> sq;
     double* nz = S.alloc<double>((size_t)count);
     for (int i = 0; i < count; ++i)
@@ -361,7 +358,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       const int k = t->d() - 1;
       if (t->d() >= 2)
       {
-        c->mem.put(t->cores[k]);
+        release_core(t->cores[k]);
         t->cores[k] = Z[i];
         t->ranks[k] = zl[i];
       }
@@ -370,7 +367,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     launch_frob_batched(S.upload(sq), count, c->stream);
     std::vector<double> h(count);
     hip_check(hipMemcpyAsync(h.data(), nz, (size_t)count * 8, hipMemcpyDeviceToHost, c->stream), "norms");
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    ctx_sync(c);
     for (int i = 0; i < count; ++i)
     {
       const double y2 = ny2[i], yh2 = h[i] * h[i];
@@ -394,6 +391,9 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
 {
   if (!c || count < 0 || (count > 0 && (!in || !ranks || !out))) return TTA_ERR_ARG;
   std::vector<tta_tt*> ts;
+  // TTs with d >= 2 start from the caller's cores (read only; every core is
+  // replaced); shorter ones pass through as copies.
+  std::unordered_set<const double*> borrowed;
   try
   {
     std::vector<std::vector<int>> ell(count);
@@ -417,6 +417,12 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
       t->modes = in[i]->modes;
       t->ranks = in[i]->ranks;
       ts.push_back(t);
+      if (d >= 2)
+      {
+        t->cores = in[i]->cores;
+        borrowed.insert(in[i]->cores.begin(), in[i]->cores.end());
+        continue;
+      }
       for (int k = 0; k < d; ++k)
       {
         const long n = (long)in[i]->ranks[k] * in[i]->modes[k] * in[i]->ranks[k + 1];
@@ -429,13 +435,12 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
     if (!cp.empty())
     {
       void* d = c->mem.get(cp.size() * sizeof(cp_desc));
-      hip_check(hipMemcpyAsync(d, cp.data(), cp.size() * sizeof(cp_desc), hipMemcpyHostToDevice, c->stream), "descs");
+      upload_async(c, d, cp.data(), cp.size() * sizeof(cp_desc));
       launch_copy_batched(static_cast<cp_desc*>(d), (int)cp.size(), max_el, c->stream);
-      hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
       c->mem.put(d);
     }
     std::vector<double> err;
-    round_batched_randomized(c, ts, ell, seed, err);
+    round_batched_randomized(c, ts, ell, seed, borrowed, err);
     std::vector<int> redo;
     for (int i = 0; i < count; ++i)
     {
@@ -466,9 +471,8 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
         again.push_back(ts[i]);
       }
       void* d = c->mem.get(cp2.size() * sizeof(cp_desc));
-      hip_check(hipMemcpyAsync(d, cp2.data(), cp2.size() * sizeof(cp_desc), hipMemcpyHostToDevice, c->stream), "descs");
+      upload_async(c, d, cp2.data(), cp2.size() * sizeof(cp_desc));
       launch_copy_batched(static_cast<cp_desc*>(d), (int)cp2.size(), me, c->stream);
-      hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
       c->mem.put(d);
       round_batched_plain(c, again, eps, 0);
     }
@@ -478,7 +482,8 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
   {
     for (auto* t : ts)
     {
-      for (double* p : t->cores) c->mem.put(p);
+      for (double* p : t->cores)
+        if (!borrowed.count(p)) c->mem.put(p);
       delete t;
     }
     c->last_error = e.what();

@@ -193,6 +193,30 @@ def test_config1_closed_form(gauge):
     assert np.linalg.norm(r.to_dense() - dense) / np.linalg.norm(dense) <= 1e-10
 
 
+def test_round_right_canonical_properties():
+    # round_right_canonical (tt_round.hpp:151-191): cores 1..d-1 are
+    # right-orthogonal, the error bound holds, and the dense result equals
+    # the gauge-none rounding (the truncated projection is unique when the
+    # singular-value gaps at the cuts are clear).
+    a = O.OTT.random([3, 4, 2, 5, 3], 6, 11)
+    b = O.OTT.random([3, 4, 2, 5, 3], 6, 12)
+    c = O.add(O.add(a, b), a)
+    r1 = O.round(c, 1e-9, 0, 1)
+    r0 = O.round(c, 1e-9, 0, 0)
+    assert r1.ranks == r0.ranks
+    for core in r1.cores()[1:]:
+        M = core.reshape(core.shape[0], -1)
+        assert np.abs(M @ M.T - np.eye(M.shape[0])).max() <= 1e-14 * 10
+    dense = c.to_dense()
+    assert np.linalg.norm(r1.to_dense() - dense) <= 1e-9 * np.linalg.norm(dense)
+    assert np.linalg.norm(r1.to_dense() - r0.to_dense()) <= 1e-12 * np.linalg.norm(dense)
+    # A real cut: the first core absorbs the norm, the rest stay orthogonal.
+    x = O.OTT.random([2, 3, 4, 3, 2], 5, 21)
+    rx = O.round(x, 0.05, 0, 1)
+    assert np.linalg.norm(rx.to_dense() - x.to_dense()) <= 0.05 * np.linalg.norm(x.to_dense())
+    assert abs(np.linalg.norm(rx.cores()[0]) - np.linalg.norm(rx.to_dense())) <= 1e-12 * np.linalg.norm(x.to_dense())
+
+
 def _tt_svd(dense, shape, eps=1e-14):
     """Plain TT-SVD (numpy) of a dense tensor: the input builder of
     tests/core/test_tt_round.cpp:20-50."""
