@@ -227,7 +227,11 @@ enum tta_kernel_class
   TTA_K_GEMM = 2,   /* FP64 contractions (absorb factors, U = Q J, inner) */
   TTA_K_APPLY = 3,  /* MPO core contraction (matvec / matmat cores) */
   TTA_K_OTHER = 4,  /* transposes, gathers, norms, balancing */
-  TTA_K_COUNT = 5
+  /* Each launch of the block-Jacobi round kernel (q > 64), credited an equal
+   * share of its SVD call's textbook flops and bytes.  These launches are
+   * also inside TTA_K_JACOBI, which times whole SVD calls. */
+  TTA_K_JACOBI_ROUND = 5,
+  TTA_K_COUNT = 6
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

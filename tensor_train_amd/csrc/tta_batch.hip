@@ -319,7 +319,7 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     {
       const credit w = jacobi_credit(jp.data(), jp.size());
       prof_scope ps(c, TTA_K_JACOBI, w.flops, w.bytes);
-      const int sw = launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c->stream);
+      const int sw = launch_jacobi_svd(jb, max_q, done, c->h_int + 2, c, w.flops, w.bytes);
       if (sw > 0)
       {
         c->jac_calls += 1;

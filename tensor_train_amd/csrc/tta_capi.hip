@@ -322,7 +322,7 @@ int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_ra
   b.one.sweeps = ints + 5;
   {
     prof_scope ps(c, TTA_K_JACOBI, svd_flops(q, q), 8.0 * 4.0 * (double)q * q);
-    const int sw = launch_jacobi_svd(b, q, ints + 4, c->h_int + 2, c->stream);
+    const int sw = launch_jacobi_svd(b, q, ints + 4, c->h_int + 2, c, svd_flops(q, q), 8.0 * 4.0 * (double)q * q);
     if (sw > 0)
     {
       c->jac_calls += 1;

@@ -23,7 +23,9 @@
 // every one of them; rotating them only stirs rounding noise between
 // columns that are discarded, which costs sweeps.  delta = 0 (no truncation,
 // full SVD) disables the rule.
+#include "../../include/tt_amd.h"
 #include "tta_device.hpp"
+#include "tta_internal.hpp"
 #include "tta_kernels.hpp"
 
 namespace tta
@@ -326,8 +328,10 @@ __global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
 
 int jacobi_block_threshold() { return 64; }
 
-int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st)
+int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, tta_ctx* c, double credit_flops,
+                      double credit_bytes)
 {
+  hipStream_t st = c->stream;
   const int batch = b.descs ? b.batch : 1;
   if (batch <= 0 || max_q <= 0) return 0;
   int sweeps_run = -1;
@@ -383,12 +387,14 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
       hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
     }
+    const size_t rec0 = c->prof_pending.size();
     for (int sweep = 0; sweep < 30; ++sweep)
     {
       sweeps_run = sweep + 1;
       for (int s = 0; s < nbb - 1; ++s)
       {
         const dim3 grid(nbb / 2, batch);
+        prof_scope ps(c, TTA_K_JACOBI_ROUND);
         if (W == 16 && max_q <= 128)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
         else if (W == 16 && max_q <= 256)
@@ -407,6 +413,15 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
         if (*done_host >= batch) break;
       }
     }
+    // Equal share of the call's credit for every round launch recorded.
+    size_t nrec = 0;
+    for (size_t r = rec0; r < c->prof_pending.size(); ++r) nrec += c->prof_pending[r].cls == TTA_K_JACOBI_ROUND;
+    for (size_t r = rec0; r < c->prof_pending.size(); ++r)
+      if (c->prof_pending[r].cls == TTA_K_JACOBI_ROUND)
+      {
+        c->prof_pending[r].flops = credit_flops / (double)nrec;
+        c->prof_pending[r].bytes = credit_bytes / (double)nrec;
+      }
   }
   constexpr int NF = 256;
   hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);

@@ -5,6 +5,8 @@
 
 #include <utility>
 
+struct tta_ctx;
+
 namespace tta
 {
 
@@ -191,7 +193,10 @@ int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
 // Returns the number of block sweeps the host ran (q > 64), or -1 when the
 // sweeps ran inside one kernel.
-int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, hipStream_t st);
+// With timing on, each round launch is recorded under TTA_K_JACOBI_ROUND and
+// credited credit_flops / credit_bytes divided by the number of rounds run.
+int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, tta_ctx* c, double credit_flops,
+                      double credit_bytes);
 void launch_qr(const qr_desc& q, int batch, hipStream_t st);
 void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
 void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
