@@ -256,6 +256,136 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
+// One round of block Jacobi with 16-lane groups: each 16-lane row of a wave
+// owns one column pair of the inner round (4 pairs per wave, W / 4 waves),
+// holds PL = ceil(q / 16) elements of each column in registers and reduces
+// its three dot products within the row (4 DPP stages, no cross-row step).
+// The reduction and the rotation formulas are thus shared by the four pairs
+// of a wave instead of being paid per pair, which cuts the VALU work per
+// rotation about threefold against one pair per wave.  Same staging, pairing,
+// rotation rule and convergence flag as bj_round_kernel.
+template <int W, int PL>
+__global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, int nbb)
+{
+  constexpr int NT = W * 16;
+  const jac_prob& P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  const int nb = (q + W - 1) / W;
+  if (P.flag && P.flag[1]) return;
+  int bi, bj;
+  rr_pair(s, blockIdx.x, nbb, bi, bj);
+  if (bi >= nb && bj >= nb) return;
+  extern __shared__ double lds[];
+  double* Xs = lds;              // [2W][q]
+  double* Js = lds + 2 * W * q;  // [2W][q]
+  const int tid = threadIdx.x;
+  int ncol[2], cbeg[2];
+  cbeg[0] = bi * W;
+  cbeg[1] = bj * W;
+  ncol[0] = bi < nb ? min(W, q - cbeg[0]) : 0;
+  ncol[1] = bj < nb ? min(W, q - cbeg[1]) : 0;
+  for (int e = tid; e < 2 * W * q; e += NT)
+  {
+    const int l = e / q, r = e % q;
+    const int h = l / W, cl = l % W;
+    double xv = 0.0, jv = 0.0;
+    if (cl < ncol[h])
+    {
+      const int g = cbeg[h] + cl;
+      xv = P.X[(size_t)g * P.ldx + r];
+      jv = P.J[(size_t)g * P.ldj + r];
+    }
+    Xs[e] = xv;
+    Js[e] = jv;
+  }
+  __syncthreads();
+  const double tol = jac_tol(q);
+  const double negl2 = jac_negl2(P, P.s[0]);
+  const int pr = tid >> 4;  // pair slot of this 16-lane group
+  const int li16 = tid & 15;
+  int rot = 0;
+  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+  {
+    int li, lj;
+    rr_pair(r2, pr, 2 * W, li, lj);
+    if (li > lj)
+    {
+      const int t = li;
+      li = lj;
+      lj = t;
+    }
+    const bool real = (li % W) < ncol[li / W] && (lj % W) < ncol[lj / W];
+    double* xi = Xs + li * q;
+    double* xj = Xs + lj * q;
+    double u[PL], v[PL];
+    double a = 0.0, b = 0.0, g = 0.0;
+#pragma unroll
+    for (int e = 0; e < PL; ++e)
+    {
+      const int t = li16 + 16 * e;
+      u[e] = t < q ? xi[t] : 0.0;
+      v[e] = t < q ? xj[t] : 0.0;
+      a = fma(u[e], u[e], a);
+      b = fma(v[e], v[e], b);
+      g = fma(u[e], v[e], g);
+    }
+    // Row sums: every lane of the 16-lane row ends with the pair's totals.
+    a += dpp_mov<0xB1>(a);
+    b += dpp_mov<0xB1>(b);
+    g += dpp_mov<0xB1>(g);
+    a += dpp_mov<0x4E>(a);
+    b += dpp_mov<0x4E>(b);
+    g += dpp_mov<0x4E>(g);
+    a += dpp_mov<0x124>(a);
+    b += dpp_mov<0x124>(b);
+    g += dpp_mov<0x124>(g);
+    a += dpp_mov<0x128>(a);
+    b += dpp_mov<0x128>(b);
+    g += dpp_mov<0x128>(g);
+    double cs, sn;
+    if (real && !(a <= negl2 || b <= negl2) && jacobi_rotation(a, b, g, tol, cs, sn))
+    {
+      rot = 1;
+#pragma unroll
+      for (int e = 0; e < PL; ++e)
+      {
+        const int t = li16 + 16 * e;
+        if (t < q)
+        {
+          xi[t] = cs * u[e] - sn * v[e];
+          xj[t] = sn * u[e] + cs * v[e];
+        }
+      }
+      double* ji = Js + li * q;
+      double* jj = Js + lj * q;
+#pragma unroll
+      for (int e = 0; e < PL; ++e)
+      {
+        const int t = li16 + 16 * e;
+        if (t < q)
+        {
+          const double y1 = ji[t], y2 = jj[t];
+          ji[t] = cs * y1 - sn * y2;
+          jj[t] = sn * y1 + cs * y2;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < 2 * W * q; e += NT)
+  {
+    const int l = e / q, r = e % q;
+    const int h = l / W, cl = l % W;
+    if (cl < ncol[h])
+    {
+      const int g = cbeg[h] + cl;
+      P.X[(size_t)g * P.ldx + r] = Xs[e];
+      P.J[(size_t)g * P.ldj + r] = Js[e];
+    }
+  }
+  if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
+}
+
 // J = I for every problem; flags cleared.
 __global__ void jac_init_kernel(jac_batch bt)
 {
@@ -395,7 +525,13 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       {
         const dim3 grid(nbb / 2, batch);
         prof_scope ps(c, TTA_K_JACOBI_ROUND);
-        if (W == 16 && max_q <= 128)
+        // Batches of q <= 128 fill the chip with 256-thread workgroups (two
+        // per CU by LDS), where the grouped kernel's lower VALU count wins;
+        // few or larger problems need the 16-wave workgroups of
+        // bj_round_kernel to hide latency.
+        if (W == 16 && max_q <= 128 && batch >= 32)
+          hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
+        else if (W == 16 && max_q <= 128)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
         else if (W == 16 && max_q <= 256)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1);
