@@ -10,94 +10,130 @@ namespace tta
 using namespace dev;
 
 // ---------------------------------------------------------------------------
-// FP64 GEMM with arbitrary element strides:
+// FP64 GEMM with arbitrary element strides on the matrix cores:
 //   C(i,j) = alpha * sum_k A(i,k) B(k,j) + beta * C(i,j)
 // A(i,k) = A[i*sar + k*sac], B(k,j) = B[k*sbr + j*sbc], C(i,j) = C[i*scr + j*scc].
-// 64x64 output tile per 256-thread workgroup, K staged through LDS in 16-deep
-// slabs; each thread owns a 4x4 micro-tile strided by 16.  blockIdx.z walks a
-// batch with pointer strides bsa/bsb/bsc.
+// 64x64 output tile per 256-thread workgroup; K staged through LDS in
+// 32-deep slabs, the next slab prefetched into registers while the current
+// one is multiplied; each wave owns a 32x32 quarter as 2x2 blocks of
+// v_mfma_f64_16x16x4f64.  Operand lanes: A(i,k) in lane i + 16k, B(k,j) in
+// lane j + 16k; accumulator element r of lane l is C(l/16 + 4r, l%16).
+// blockIdx.z walks a batch with pointer strides bsa/bsb/bsc.
 // ---------------------------------------------------------------------------
 namespace
 {
-constexpr int GT = 64, GK = 16;
+constexpr int GT = 64, GK = 32;
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// Element e (0..7) of this thread's share of a K slab: A(i0 + ii, k0 + kk)
+// and B(k0 + kk, j0 + jj), with the thread order matched to the operand's
+// contiguous direction so that loads coalesce.
+__device__ __forceinline__ void slab_coords_a(bool kc, int idx, int& ii, int& kk)
+{
+  if (kc)
+  {
+    ii = idx >> 5;
+    kk = idx & 31;
+  }
+  else
+  {
+    ii = idx & 63;
+    kk = idx >> 6;
+  }
+}
+__device__ __forceinline__ void slab_coords_b(bool jc, int idx, int& kk, int& jj)
+{
+  if (jc)
+  {
+    kk = idx >> 6;
+    jj = idx & 63;
+  }
+  else
+  {
+    jj = idx >> 5;
+    kk = idx & 31;
+  }
+}
 
 __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, const double* B, double* C)
 {
   __shared__ double As[GK][GT + 1];
   __shared__ double Bs[GK][GT + 1];
   const int tid = threadIdx.x;
-  const int tx = tid & 15, ty = tid >> 4;
+  const int lane = tid & 63, wid = tid >> 6;
+  const int wr = wid >> 1, wc = wid & 1;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
   const bool a_kc = (g.sac == 1);  // A contiguous along k
   const bool b_jc = (g.sbc == 1);  // B contiguous along j
-  double acc[4][4];
+  dbl4 acc[2][2];
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+  const int fl = lane & 15, fk = lane >> 4;
+  // Registers holding the next slab while the current one is multiplied.
+  double pa[8], pb[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+    {
+      const int idx = tid + 256 * l;
+      int ii, kk, jj, kb;
+      slab_coords_a(a_kc, idx, ii, kk);
+      const int gi = i0 + ii, gk = k0 + kk;
+      pa[l] = (gi < g.M && gk < g.K) ? A[(size_t)gi * g.sar + (size_t)gk * g.sac] : 0.0;
+      slab_coords_b(b_jc, idx, kb, jj);
+      const int gj = j0 + jj, gk2 = k0 + kb;
+      pb[l] = (gj < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sbr + (size_t)gj * g.sbc] : 0.0;
+    }
+  };
+  fetch(0);
   for (int k0 = 0; k0 < g.K; k0 += GK)
   {
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
+    for (int l = 0; l < 8; ++l)
     {
       const int idx = tid + 256 * l;
-      int ii, kk;
-      if (a_kc)
-      {
-        ii = idx >> 4;
-        kk = idx & 15;
-      }
-      else
-      {
-        ii = idx & 63;
-        kk = idx >> 6;
-      }
-      const int gi = i0 + ii, gk = k0 + kk;
-      As[kk][ii] = (gi < g.M && gk < g.K) ? A[(size_t)gi * g.sar + (size_t)gk * g.sac] : 0.0;
-      int jj;
-      if (b_jc)
-      {
-        kk = idx >> 6;
-        jj = idx & 63;
-      }
-      else
-      {
-        jj = idx >> 4;
-        kk = idx & 15;
-      }
-      const int gj = j0 + jj, gk2 = k0 + kk;
-      Bs[kk][jj] = (gj < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sbr + (size_t)gj * g.sbc] : 0.0;
+      int ii, kk, jj, kb;
+      slab_coords_a(a_kc, idx, ii, kk);
+      As[kk][ii] = pa[l];
+      slab_coords_b(b_jc, idx, kb, jj);
+      Bs[kb][jj] = pb[l];
     }
     __syncthreads();
+    if (k0 + GK < g.K) fetch(k0 + GK);
 #pragma unroll
-    for (int kk = 0; kk < GK; ++kk)
+    for (int ks = 0; ks < GK; ks += 4)
     {
-      double a[4], b[4];
+      double a[2], b[2];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] = As[kk][ty + 16 * u];
+      for (int u = 0; u < 2; ++u) a[u] = As[ks + fk][wr * 32 + u * 16 + fl];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) b[v] = Bs[kk][tx + 16 * v];
+      for (int v = 0; v < 2; ++v) b[v] = Bs[ks + fk][wc * 32 + v * 16 + fl];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = fma(a[u], b[v], acc[u][v]);
+        for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u][v], 0, 0, 0);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u)
-  {
-    const int gi = i0 + ty + 16 * u;
-    if (gi >= g.M) continue;
+  for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 4; ++v)
+    for (int v = 0; v < 2; ++v)
     {
-      const int gj = j0 + tx + 16 * v;
+      const int gj = j0 + wc * 32 + v * 16 + fl;
       if (gj >= g.N) continue;
-      double* c = C + (size_t)gi * g.scr + (size_t)gj * g.scc;
-      *c = (g.beta == 0.0) ? g.alpha * acc[u][v] : g.alpha * acc[u][v] + g.beta * (*c);
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const int gi = i0 + wr * 32 + u * 16 + fk + 4 * r;
+        if (gi >= g.M) continue;
+        double* c = C + (size_t)gi * g.scr + (size_t)gj * g.scc;
+        const double x = acc[u][v][r];
+        *c = (g.beta == 0.0) ? g.alpha * x : g.alpha * x + g.beta * (*c);
+      }
     }
-  }
 }
 
 __global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
