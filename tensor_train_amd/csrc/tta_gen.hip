@@ -115,13 +115,15 @@ __global__ void sum_build_kernel(const sum_desc* ds)
   }
 }
 
-// Column-major Gaussian (m x nc) for (seed, core).
+// Column-major Gaussian (m x nc) for (seed, core), FP64 (dst) or FP32
+// (dst32, when set).
 struct gauss_desc
 {
   double* dst;
   int m, nc;
   unsigned long long seed;
   int k;
+  float* dst32;
 };
 
 __global__ void gauss_kernel(const gauss_desc* ds)
@@ -129,7 +131,13 @@ __global__ void gauss_kernel(const gauss_desc* ds)
   const gauss_desc& D = ds[blockIdx.y];
   const long total = (long)D.m * D.nc;
   for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
-    D.dst[e] = philox_normal(D.seed, (unsigned)D.k, (unsigned long long)e);
+  {
+    const double v = philox_normal(D.seed, (unsigned)D.k, (unsigned long long)e);
+    if (D.dst32)
+      D.dst32[e] = (float)v;
+    else
+      D.dst[e] = v;
+  }
 }
 
 // core (m x nc row-major) = Q (m x nc column-major) * diag(10^(-j s)); for
@@ -198,14 +206,15 @@ tta_tt* alloc_tt(tta_ctx* c, const std::vector<int>& modes, const std::vector<in
 }  // namespace
 
 // Row-major N(0,1) blocks (m x nc) for (seed, stream k), one per descriptor.
-void launch_gauss_batched(const double* const* dst, const int* m, const int* nc, const unsigned long long* seed,
-                          const int* k, int count, tta_ctx* c)
+void launch_gauss_batched(void* const* dst, const int* m, const int* nc, const unsigned long long* seed, const int* k,
+                          int count, bool f32, tta_ctx* c)
 {
   std::vector<gauss_desc> gd;
   long max_el = 0;
   for (int i = 0; i < count; ++i)
   {
-    gd.push_back(gauss_desc{const_cast<double*>(dst[i]), m[i], nc[i], seed[i], k[i]});
+    gd.push_back(f32 ? gauss_desc{nullptr, m[i], nc[i], seed[i], k[i], static_cast<float*>(dst[i])}
+                     : gauss_desc{static_cast<double*>(dst[i]), m[i], nc[i], seed[i], k[i], nullptr});
     max_el = std::max(max_el, (long)m[i] * nc[i]);
   }
   std::vector<void*> own;

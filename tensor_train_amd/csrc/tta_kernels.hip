@@ -55,24 +55,51 @@ __device__ __forceinline__ void slab_coords_b(bool jc, int idx, int& kk, int& jj
   }
 }
 
-__device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, const double* B, double* C)
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+// One 16x16x4 matrix-core step in the compute type CT.  FP64: accumulator
+// element r of lane l is C(l/16 + 4r, l%16); FP32: C(4(l/16) + r, l%16).
+template <class CT>
+struct mfma_step;
+template <>
+struct mfma_step<double>
 {
-  __shared__ double As[GK][GT + 1];
-  __shared__ double Bs[GK][GT + 1];
+  using acc_t = dbl4;
+  __device__ static acc_t run(double a, double b, acc_t c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int fk, int r) { return fk + 4 * r; }
+};
+template <>
+struct mfma_step<float>
+{
+  using acc_t = flt4;
+  __device__ static acc_t run(float a, float b, acc_t c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int fk, int r) { return 4 * fk + r; }
+};
+
+// Tile of C = alpha A B + beta C with A of type TA, B of type TB, C of type
+// TC, multiplied on the matrix cores in CT (operands converted on the way
+// into LDS).
+template <class TA, class TB, class TC, class CT>
+__device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, long sar, long sac, const TB* B, long sbr,
+                                            long sbc, TC* C, long scr, long scc, double alpha, double beta)
+{
+  using S = mfma_step<CT>;
+  __shared__ CT As[GK][GT + 1];
+  __shared__ CT Bs[GK][GT + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wr = wid >> 1, wc = wid & 1;
   const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
-  const bool a_kc = (g.sac == 1);  // A contiguous along k
-  const bool b_jc = (g.sbc == 1);  // B contiguous along j
-  dbl4 acc[2][2];
+  const bool a_kc = (sac == 1);  // A contiguous along k
+  const bool b_jc = (sbc == 1);  // B contiguous along j
+  typename S::acc_t acc[2][2];
 #pragma unroll
   for (int u = 0; u < 2; ++u)
 #pragma unroll
-    for (int v = 0; v < 2; ++v) acc[u][v] = dbl4{0.0, 0.0, 0.0, 0.0};
+    for (int v = 0; v < 2; ++v) acc[u][v] = typename S::acc_t{0, 0, 0, 0};
   const int fl = lane & 15, fk = lane >> 4;
   // Registers holding the next slab while the current one is multiplied.
-  double pa[8], pb[8];
+  CT pa[8], pb[8];
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int l = 0; l < 8; ++l)
@@ -81,14 +108,14 @@ __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, c
       int ii, kk, jj, kb;
       slab_coords_a(a_kc, idx, ii, kk);
       const int gi = i0 + ii, gk = k0 + kk;
-      pa[l] = (gi < g.M && gk < g.K) ? A[(size_t)gi * g.sar + (size_t)gk * g.sac] : 0.0;
+      pa[l] = (gi < M && gk < K) ? (CT)A[(size_t)gi * sar + (size_t)gk * sac] : (CT)0;
       slab_coords_b(b_jc, idx, kb, jj);
       const int gj = j0 + jj, gk2 = k0 + kb;
-      pb[l] = (gj < g.N && gk2 < g.K) ? B[(size_t)gk2 * g.sbr + (size_t)gj * g.sbc] : 0.0;
+      pb[l] = (gj < N && gk2 < K) ? (CT)B[(size_t)gk2 * sbr + (size_t)gj * sbc] : (CT)0;
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 < g.K; k0 += GK)
+  for (int k0 = 0; k0 < K; k0 += GK)
   {
 #pragma unroll
     for (int l = 0; l < 8; ++l)
@@ -101,11 +128,11 @@ __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, c
       Bs[kb][jj] = pb[l];
     }
     __syncthreads();
-    if (k0 + GK < g.K) fetch(k0 + GK);
+    if (k0 + GK < K) fetch(k0 + GK);
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 4)
     {
-      double a[2], b[2];
+      CT a[2], b[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) a[u] = As[ks + fk][wr * 32 + u * 16 + fl];
 #pragma unroll
@@ -113,7 +140,7 @@ __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, c
 #pragma unroll
       for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u], b[v], acc[u][v], 0, 0, 0);
+        for (int v = 0; v < 2; ++v) acc[u][v] = S::run(a[u], b[v], acc[u][v]);
     }
     __syncthreads();
   }
@@ -123,17 +150,23 @@ __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, c
     for (int v = 0; v < 2; ++v)
     {
       const int gj = j0 + wc * 32 + v * 16 + fl;
-      if (gj >= g.N) continue;
+      if (gj >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
-        const int gi = i0 + wr * 32 + u * 16 + fk + 4 * r;
-        if (gi >= g.M) continue;
-        double* c = C + (size_t)gi * g.scr + (size_t)gj * g.scc;
-        const double x = acc[u][v][r];
-        *c = (g.beta == 0.0) ? g.alpha * x : g.alpha * x + g.beta * (*c);
+        const int gi = i0 + wr * 32 + u * 16 + S::row(fk, r);
+        if (gi >= M) continue;
+        TC* c = C + (size_t)gi * scr + (size_t)gj * scc;
+        const double x = (double)acc[u][v][r];
+        *c = (TC)((beta == 0.0) ? alpha * x : alpha * x + beta * (double)(*c));
       }
     }
+}
+
+__device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, const double* B, double* C)
+{
+  gemm_tile_t<double, double, double, double>(g.M, g.N, g.K, A, g.sar, g.sac, B, g.sbr, g.sbc, C, g.scr, g.scc, g.alpha,
+                                              g.beta);
 }
 
 __global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
@@ -148,6 +181,18 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* 
   const gemm_desc& g = gs[blockIdx.z];
   if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
   gemm_tile(g, g.A, g.B, g.C);
+}
+
+// Grouped GEMM with per-operand precision (xgemm_desc.types): A, B and C
+// each FP64 or FP32, multiplied in FP32 on the matrix cores when
+// types.compute is FP32, in FP64 otherwise.
+template <class TA, class TB, class TC, class CT>
+__global__ __launch_bounds__(256) void xgemm_grouped_kernel(const xgemm_desc* gs)
+{
+  const xgemm_desc& g = gs[blockIdx.z];
+  if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
+  gemm_tile_t<TA, TB, TC, CT>(g.M, g.N, g.K, static_cast<const TA*>(g.A), g.sar, g.sac, static_cast<const TB*>(g.B),
+                              g.sbr, g.sbc, static_cast<TC*>(g.C), g.scr, g.scc, 1.0, 0.0);
 }
 
 // Householder QR (+ optional thin Q) of one column-major matrix per block.
@@ -646,6 +691,21 @@ void launch_scale(double* x, long n, double alpha, hipStream_t st)
   hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(256), 0, st, x, n, alpha);
 }
 
+
+void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
+                          hipStream_t st)
+{
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, count);
+  if (t == XG_D_S_S_S)
+    hipLaunchKernelGGL((xgemm_grouped_kernel<double, float, float, float>), grid, dim3(256), 0, st, descs_dev);
+  else if (t == XG_S_S_S_S)
+    hipLaunchKernelGGL((xgemm_grouped_kernel<float, float, float, float>), grid, dim3(256), 0, st, descs_dev);
+  else if (t == XG_D_S_D_D)
+    hipLaunchKernelGGL((xgemm_grouped_kernel<double, float, double, double>), grid, dim3(256), 0, st, descs_dev);
+  else
+    hipLaunchKernelGGL((xgemm_grouped_kernel<double, double, double, double>), grid, dim3(256), 0, st, descs_dev);
+}
 
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st)
 {

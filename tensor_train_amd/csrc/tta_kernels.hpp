@@ -25,6 +25,27 @@ struct gemm_desc
   long bsa = 0, bsb = 0, bsc = 0;
 };
 
+// GEMM descriptor with untyped operands; the element types of one grouped
+// launch are given by an xgemm_types code (operands A, B, C, then the
+// compute type; D = FP64, S = FP32).
+struct xgemm_desc
+{
+  int M = 0, N = 0, K = 0;
+  const void* A = nullptr;
+  long sar = 0, sac = 0;
+  const void* B = nullptr;
+  long sbr = 0, sbc = 0;
+  void* C = nullptr;
+  long scr = 0, scc = 0;
+};
+enum xgemm_types
+{
+  XG_D_D_D_D = 0,
+  XG_D_S_S_S = 1,  // FP64 A, FP32 B -> FP32 C, FP32 matrix cores
+  XG_S_S_S_S = 2,
+  XG_D_S_D_D = 3   // FP64 A, FP32 B -> FP64 C, FP64 matrix cores
+};
+
 // Householder QR of `batch` column-major (m x n) matrices; thin Q optional.
 struct qr_desc
 {
@@ -188,6 +209,8 @@ struct permute_desc
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st);
+void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
+                          hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.

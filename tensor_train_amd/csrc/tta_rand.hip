@@ -30,8 +30,8 @@
 
 namespace tta
 {
-void launch_gauss_batched(const double* const* dst, const int* m, const int* nc, const unsigned long long* seed,
-                          const int* k, int count, tta_ctx* c);
+void launch_gauss_batched(void* const* dst, const int* m, const int* nc, const unsigned long long* seed, const int* k,
+                          int count, bool f32, tta_ctx* c);
 void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int max_rank);
 
 namespace
@@ -52,6 +52,22 @@ gemm_desc rm_gemm(int M, int N, int K, const double* A, long lda, const double* 
   g.scr = ldc;
   g.scc = 1;
   return g;
+}
+
+void run_xgrouped(tta_ctx* c, arena& O, const std::vector<xgemm_desc>& gs, xgemm_types t)
+{
+  if (gs.empty()) return;
+  int mm = 0, nn = 0;
+  credit w;
+  for (auto& g : gs)
+  {
+    mm = std::max(mm, g.M);
+    nn = std::max(nn, g.N);
+    w.flops += 2.0 * g.M * g.N * (double)g.K;
+    w.bytes += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
+  }
+  prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
+  launch_xgemm_grouped(O.upload(gs), (int)gs.size(), mm, nn, t, c->stream);
 }
 
 void run_grouped(tta_ctx* c, arena& O, const std::vector<gemm_desc>& gs)
@@ -91,10 +107,13 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
   auto Rar = std::make_unique<arena>(c);
   arena War(c);
 
-  // 1. Sketch cores (l_{k-1} n_k x l_k row-major).
-  std::vector<std::vector<double*>> R(count);
+  // 1. Sketch cores (l_{k-1} n_k x l_k row-major), FP32: the sketch only
+  // selects a subspace, and the projection onto it (phases 3-4) is FP64, so
+  // sketch rounding perturbs which near-optimal subspace is found, never the
+  // accuracy of the result or of its discarded-norm check.
+  std::vector<std::vector<float*>> R(count);
   {
-    std::vector<const double*> dst;
+    std::vector<void*> dst;
     std::vector<int> m, nc, kk;
     std::vector<unsigned long long> sd;
     for (int i = 0; i < count; ++i)
@@ -103,7 +122,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       for (int k = 0; k < t->d(); ++k)
       {
         const int rows = ell[i][k] * t->modes[k], cols = ell[i][k + 1];
-        double* p = Rar->alloc<double>((size_t)rows * cols);
+        float* p = Rar->alloc<float>((size_t)rows * cols);
         R[i].push_back(p);
         dst.push_back(p);
         m.push_back(rows);
@@ -113,16 +132,17 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       }
     }
     prof_scope ps(c, TTA_K_OTHER);
-    launch_gauss_batched(dst.data(), m.data(), nc.data(), sd.data(), kk.data(), (int)dst.size(), c);
+    launch_gauss_batched(dst.data(), m.data(), nc.data(), sd.data(), kk.data(), (int)dst.size(), true, c);
   }
 
-  // 2. W_k (r_k x l_k) from the right; W_d = [1] implicit.
-  std::vector<std::vector<double*>> W(count);
+  // 2. W_k (r_k x l_k) from the right, FP32 on the matrix cores (the input
+  // cores are read in FP64 and rounded on the way in); W_d = [1] implicit.
+  std::vector<std::vector<float*>> W(count);
   for (int i = 0; i < count; ++i) W[i].assign(ts[i]->d() + 1, nullptr);
   for (int j = 0; j <= maxd - 1; ++j)
   {
     arena S(c);
-    std::vector<gemm_desc> g1, g2;
+    std::vector<xgemm_desc> g1, g2d, g2s;  // T1 = Y W'; W = T1 R^T (T1 = Y or FP32)
     for (int i = 0; i < count; ++i)
     {
       tta_tt* t = ts[i];
@@ -130,32 +150,22 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       if (k < 1) continue;
       const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
       const int lp = ell[i][k + 1], l = ell[i][k];
-      const double* T1 = t->cores[k];  // k = d-1: Y_{d-1} W_d = Y_{d-1}
-      if (k < t->d() - 1)
+      const void* T1 = t->cores[k];  // k = d-1: Y_{d-1} W_d = Y_{d-1}
+      const bool t1_f32 = k < t->d() - 1;
+      if (t1_f32)
       {
-        double* tmp = S.alloc<double>((size_t)r * n * lp);
-        g1.push_back(rm_gemm(r * n, lp, rp, t->cores[k], rp, W[i][k + 1], lp, tmp, lp));
+        float* tmp = S.alloc<float>((size_t)r * n * lp);
+        g1.push_back(xgemm_desc{r * n, lp, rp, t->cores[k], rp, 1, W[i][k + 1], lp, 1, tmp, lp, 1});
         T1 = tmp;
       }
-      double* Wk = War.alloc<double>((size_t)r * l);
+      float* Wk = War.alloc<float>((size_t)r * l);
       W[i][k] = Wk;
-      gemm_desc g;
-      g.M = r;
-      g.N = l;
-      g.K = n * lp;
-      g.A = T1;
-      g.sar = (long)n * lp;
-      g.sac = 1;
-      g.B = R[i][k];
-      g.sbr = 1;
-      g.sbc = (long)n * lp;
-      g.C = Wk;
-      g.scr = l;
-      g.scc = 1;
-      g2.push_back(g);
+      const xgemm_desc g{r, l, n * lp, T1, (long)n * lp, 1, R[i][k], 1, (long)n * lp, Wk, l, 1};
+      (t1_f32 ? g2s : g2d).push_back(g);
     }
-    run_grouped(c, S, g1);
-    run_grouped(c, S, g2);
+    run_xgrouped(c, S, g1, XG_D_S_S_S);
+    run_xgrouped(c, S, g2d, XG_D_S_S_S);
+    run_xgrouped(c, S, g2s, XG_S_S_S_S);
   }
   Rar.reset();  // sketch cores are consumed (stream-ordered release)
 
@@ -252,7 +262,8 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
     arena S(c);
     auto Znext = std::make_unique<arena>(c);
     std::vector<int> idx;
-    std::vector<gemm_desc> gS, gM, gZ;
+    std::vector<xgemm_desc> gS;
+    std::vector<gemm_desc> gM, gZ;
     std::vector<qrp> qr;
     std::vector<tr_desc> tr;
     std::vector<double*> newcore, newZ;
@@ -266,12 +277,9 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       const int m = lq * n, lt = ell[i][k + 1];
       const int l2 = std::min(m, lt);
       idx.push_back(i);
-      // S (m x lt) column-major = Z (m x rp) * W_{k+1} (rp x lt).
+      // S (m x lt) column-major = Z (m x rp, FP64) * W_{k+1} (rp x lt, FP32), in FP64.
       double* Sm = S.alloc<double>((size_t)m * lt);
-      gemm_desc g = rm_gemm(m, lt, rp, Z[i], rp, W[i][k + 1], lt, Sm, 0);
-      g.scr = 1;
-      g.scc = m;
-      gS.push_back(g);
+      gS.push_back(xgemm_desc{m, lt, rp, Z[i], rp, 1, W[i][k + 1], lt, 1, Sm, 1, m});
       qrp p;
       p.A = Sm;
       p.lda = m;
@@ -315,7 +323,7 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
       newl.push_back(l2);
     }
     if (idx.empty()) continue;
-    run_grouped(c, S, gS);
+    run_xgrouped(c, S, gS, XG_D_S_D_D);
     qr_batch qb;
     qb.descs = S.upload(qr);
     qb.batch = (int)qr.size();
