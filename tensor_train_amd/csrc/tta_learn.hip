@@ -29,61 +29,112 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
 
 using namespace tta;
 
+namespace tta
+{
+void mlp_predict_dev(tta_ctx* c, tta_mlp* m, const float* X_dev, int n, float* y_host);
+}
+
 namespace
 {
 constexpr int NF = 13;
+constexpr int FMAX = 4096;  // largest bond rank handled (sort in LDS)
 
-void bond_features(tta_ctx* c, int count, tta_tt* const* in, double eps, std::vector<float>& F)
+struct bf_desc
 {
-  F.clear();
-  std::vector<cn_desc> cn;
-  std::vector<size_t> off;
-  size_t total = 0;
-  int max_cols = 0;
+  const double* M;  // left unfolding of core k-1: rows x cols row-major
+  int rows, cols;   // r_{k-1} n_{k-1}, r_k
+  int n, k, d;      // n_{k-1}, bond index, TT length
+};
+
+__device__ __forceinline__ float feat_log(double x, double c0)
+{
+  return (float)(log10(fmax(x / c0, 1e-300)) / 10.0);
+}
+
+// One workgroup per bond: column norms (thread per column, rows walked in
+// order, so the wave's loads are contiguous and the sums deterministic),
+// bitonic sort descending in LDS, then the 13 features.
+__global__ __launch_bounds__(256) void bond_feat_kernel(const bf_desc* ds, double eps, float* F)
+{
+  const bf_desc& D = ds[blockIdx.x];
+  __shared__ double v[FMAX];
+  const int tid = threadIdx.x;
+  int P = 1;
+  while (P < D.cols) P <<= 1;
+  for (int c = tid; c < P; c += 256)
+  {
+    double acc = -1.0;  // padding sorts last
+    if (c < D.cols)
+    {
+      acc = 0.0;
+      for (int i = 0; i < D.rows; ++i)
+      {
+        const double x = D.M[(size_t)i * D.cols + c];
+        acc = fma(x, x, acc);
+      }
+      acc = sqrt(acc);
+    }
+    v[c] = acc;
+  }
+  __syncthreads();
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+    {
+      for (int i = tid; i < P; i += 256)
+      {
+        const int j = i ^ stride;
+        if (j > i)
+        {
+          const bool desc = (i & size) == 0;
+          const double a = v[i], b = v[j];
+          if (desc ? (a < b) : (a > b))
+          {
+            v[i] = b;
+            v[j] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  if (tid == 0)
+  {
+    const int r = D.cols;
+    const double c0 = v[0] > 0.0 ? v[0] : 1.0;
+    float* f = F + (size_t)blockIdx.x * NF;
+    f[0] = (float)(log2((double)r) / 8.0);
+    f[1] = (float)(log2((double)D.n) / 3.0);
+    f[2] = (float)D.k / (float)D.d;
+    f[3] = (float)(-log10(eps) / 10.0);
+    f[4] = (float)(-log10(eps / sqrt((double)max(D.d - 1, 1))) / 10.0);
+    const int js[7] = {1, 2, 4, 8, 16, 32, 64};
+    for (int q = 0; q < 7; ++q) f[5 + q] = feat_log(v[min(js[q], r - 1)], c0);
+    f[12] = feat_log(v[r - 1], c0);
+  }
+}
+
+// Features of every bond k = 1..d-1 of every TT, into device memory
+// (bonds in TT order, then bond order); returns the bond count.
+int bond_features_dev(tta_ctx* c, int count, tta_tt* const* in, double eps, arena& A, float** F_out)
+{
+  std::vector<bf_desc> ds;
   for (int i = 0; i < count; ++i)
     for (int k = 1; k < in[i]->d(); ++k)
     {
-      off.push_back(total);
-      total += (size_t)in[i]->ranks[k];
-      max_cols = std::max(max_cols, in[i]->ranks[k]);
+      const tta_tt* t = in[i];
+      if (t->ranks[k] > FMAX) fail(TTA_ERR_UNSUPPORTED, "bond features: rank %d above %d", t->ranks[k], FMAX);
+      ds.push_back(bf_desc{t->cores[k - 1], t->ranks[k - 1] * t->modes[k - 1], t->ranks[k], t->modes[k - 1], k, t->d()});
     }
-  if (total == 0) return;
-  double* nrm = static_cast<double*>(c->mem.get(total * 8));
-  size_t b = 0;
-  for (int i = 0; i < count; ++i)
-    for (int k = 1; k < in[i]->d(); ++k, ++b)
-      cn.push_back(cn_desc{in[i]->cores[k - 1], in[i]->ranks[k - 1] * in[i]->modes[k - 1], in[i]->ranks[k], nrm + off[b]});
-  void* dd = c->mem.get(cn.size() * sizeof(cn_desc));
-  upload_async(c, dd, cn.data(), cn.size() * sizeof(cn_desc));
-  for (size_t o = 0; o < cn.size(); o += 65535)
-    launch_col_norms_batched(static_cast<cn_desc*>(dd) + o, (int)std::min<size_t>(65535, cn.size() - o), max_cols,
-                             c->stream);
-  std::vector<double> h(total);
-  hip_check(hipMemcpyAsync(h.data(), nrm, total * 8, hipMemcpyDeviceToHost, c->stream), "norms");
-  ctx_sync(c);
-  c->mem.put(dd);
-  c->mem.put(nrm);
-  b = 0;
-  const int js[7] = {1, 2, 4, 8, 16, 32, 64};
-  for (int i = 0; i < count; ++i)
+  const int nb = (int)ds.size();
+  float* F = A.alloc<float>((size_t)std::max(nb, 1) * NF);
+  if (nb > 0)
   {
-    const int d = in[i]->d();
-    for (int k = 1; k < d; ++k, ++b)
-    {
-      const int r = in[i]->ranks[k];
-      std::vector<double> v(h.begin() + (long)off[b], h.begin() + (long)off[b] + r);
-      std::sort(v.begin(), v.end(), [](double x, double y) { return x > y; });
-      const double c0 = v[0] > 0 ? v[0] : 1.0;
-      auto lg = [&](double x) { return (float)(std::log10(std::max(x / c0, 1e-300)) / 10.0); };
-      F.push_back((float)(std::log2((double)r) / 8.0));
-      F.push_back((float)(std::log2((double)in[i]->modes[k - 1]) / 3.0));
-      F.push_back((float)k / (float)d);
-      F.push_back((float)(-std::log10(eps) / 10.0));
-      F.push_back((float)(-std::log10(eps / std::sqrt((double)std::max(d - 1, 1))) / 10.0));
-      for (int j : js) F.push_back(lg(v[std::min(j, r - 1)]));
-      F.push_back(lg(v[r - 1]));
-    }
+    const bf_desc* dd = A.upload(ds);
+    for (int o = 0; o < nb; o += 65535)
+      hipLaunchKernelGGL(bond_feat_kernel, dim3(std::min(65535, nb - o)), dim3(256), 0, c->stream, dd + o, eps,
+                         F + (size_t)o * NF);
   }
+  *F_out = F;
+  return nb;
 }
 }  // namespace
 
@@ -97,9 +148,12 @@ int tta_bond_features(tta_ctx* c, int count, tta_tt* const* in, double eps, floa
   if (!c || count < 0 || (count > 0 && (!in || !feats)) || !(eps > 0.0)) return TTA_ERR_ARG;
   try
   {
-    std::vector<float> F;
-    bond_features(c, count, in, eps, F);
-    std::copy(F.begin(), F.end(), feats);
+    arena A(c, size_t(4) << 20);
+    float* F = nullptr;
+    const int nb = bond_features_dev(c, count, in, eps, A, &F);
+    if (nb > 0)
+      hip_check(hipMemcpyAsync(feats, F, (size_t)nb * NF * sizeof(float), hipMemcpyDeviceToHost, c->stream), "features");
+    ctx_sync(c);
   }
   catch (const tta::error& e)
   {
@@ -122,14 +176,13 @@ int tta_round_learned_batched(tta_ctx* c, tta_mlp* m, int count, tta_tt* const* 
     return TTA_ERR_ARG;
   try
   {
-    std::vector<float> F;
-    bond_features(c, count, in, eps, F);
-    const int nb = (int)(F.size() / NF);
-    std::vector<float> pred((size_t)std::max(nb, 1));
-    if (nb > 0)
+    std::vector<float> pred;
     {
-      const int st = tta_mlp_predict(c, m, F.data(), nb, pred.data());
-      if (st != TTA_OK) return st;
+      arena A(c, size_t(4) << 20);
+      float* F = nullptr;
+      const int nb = bond_features_dev(c, count, in, eps, A, &F);
+      pred.assign((size_t)std::max(nb, 1), 0.f);
+      if (nb > 0) mlp_predict_dev(c, m, F, nb, pred.data());
     }
     std::vector<int> ranks;
     size_t b = 0;

@@ -345,6 +345,28 @@ int tta_mlp_train(tta_ctx* c, tta_mlp* m, const float* X, const float* y, int n,
   return TTA_OK;
 }
 
+}  // extern "C"
+
+namespace tta
+{
+// Forward pass on n feature rows already in device memory; predictions to
+// the host (throws tta::error).
+void mlp_predict_dev(tta_ctx* c, tta_mlp* m, const float* X_dev, int n, float* y_host)
+{
+  if (n <= 0) return;
+  ws_owner O(c);
+  mlp_ws w;
+  make_ws(c, m, n, O, w, false);
+  w.act[0] = const_cast<float*>(X_dev);
+  forward(c, m, n, w);
+  hip_check(hipMemcpyAsync(y_host, w.act.back(), (size_t)n * 4, hipMemcpyDeviceToHost, c->stream), "y");
+  ctx_sync(c);
+}
+}  // namespace tta
+
+extern "C"
+{
+
 int tta_mlp_predict(tta_ctx* c, tta_mlp* m, const float* X, int n, float* y_out)
 {
   if (!c || !m || !X || !y_out || n < 0) return TTA_ERR_ARG;
