@@ -118,6 +118,33 @@ __device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, d
   return 1;
 }
 
+// Deferred J update of one block-pair inner sweep: jr holds one row of the
+// pair's 2W columns of J; the sweep's rotations (round-robin order, rounds
+// applied in sequence) come from rcs / rsn[r2 * W + pair].  Fully unrolled so
+// the column indices are constants and jr stays in registers.
+template <int W>
+__device__ __forceinline__ void apply_rotations(double (&jr)[2 * W], const double* rcs, const double* rsn)
+{
+#pragma unroll
+  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+#pragma unroll
+    for (int pr = 0; pr < W; ++pr)
+    {
+      int li, lj;
+      rr_pair(r2, pr, 2 * W, li, lj);
+      if (li > lj)
+      {
+        const int t = li;
+        li = lj;
+        lj = t;
+      }
+      const double c = rcs[r2 * W + pr], sv = rsn[r2 * W + pr];
+      const double y1 = jr[li], y2 = jr[lj];
+      jr[li] = c * y1 - sv * y2;
+      jr[lj] = sv * y1 + c * y2;
+    }
+}
+
 // Whole Jacobi in one workgroup with X, J in LDS (q <= 64).
 template <int NT>
 __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_sweeps)
@@ -180,8 +207,73 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
   if (tid == 0 && P.sweeps) *P.sweeps = sweep;
 }
 
+// Block pair of bj_group_kernel: block pair (bi, bj) of round s
+// of the round-robin over nbb (even) blocks of W columns; a block paired with
+// a padding block (odd block count) is still orthogonalised internally.
+// LDS layout: X columns [2W][q], then the inner sweep's rotations
+// rcs / rsn [(2W - 1) * W].  J is not staged: thread t < q keeps row t of the
+// pair's 2W columns of J in registers and applies the recorded rotations
+// once the inner sweep is done, which halves the LDS traffic of every inner
+// round and takes J off its critical path.
+template <int W>
+struct bj_pair
+{
+  int ncol0, ncol1, cbeg0, cbeg1;  // scalars: a local array would live in scratch
+  __device__ bool setup(const jac_prob& P, int s, int nbb)
+  {
+    const int q = P.q;
+    const int nb = (q + W - 1) / W;
+    int bi, bj;
+    rr_pair(s, blockIdx.x, nbb, bi, bj);
+    if (bi >= nb && bj >= nb) return false;
+    cbeg0 = bi * W;
+    cbeg1 = bj * W;
+    ncol0 = bi < nb ? min(W, q - cbeg0) : 0;
+    ncol1 = bj < nb ? min(W, q - cbeg1) : 0;
+    return true;
+  }
+  __device__ bool real(int l) const { return (l % W) < (l < W ? ncol0 : ncol1); }
+  __device__ int global_col(int l) const { return (l < W ? cbeg0 : cbeg1) + l % W; }
+  template <int NT>
+  __device__ void load(const jac_prob& P, double* Xs, double (&jr)[2 * W]) const
+  {
+    const int q = P.q;
+    for (int e = threadIdx.x; e < 2 * W * q; e += NT)
+    {
+      const int l = e / q, r = e % q;
+      Xs[e] = real(l) ? P.X[(size_t)global_col(l) * P.ldx + r] : 0.0;
+    }
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int l = 0; l < 2 * W; ++l) jr[l] = (t < q && real(l)) ? P.J[(size_t)global_col(l) * P.ldj + t] : 0.0;
+  }
+  template <int NT>
+  __device__ void store(const jac_prob& P, const double* Xs, const double (&jr)[2 * W]) const
+  {
+    const int q = P.q;
+    for (int e = threadIdx.x; e < 2 * W * q; e += NT)
+    {
+      const int l = e / q, r = e % q;
+      if (real(l)) P.X[(size_t)global_col(l) * P.ldx + r] = Xs[e];
+    }
+    const int t = threadIdx.x;
+    if (t < q)
+    {
+#pragma unroll
+      for (int l = 0; l < 2 * W; ++l)
+        if (real(l)) P.J[(size_t)global_col(l) * P.ldj + t] = jr[l];
+    }
+  }
+};
+
+// LDS bytes of a block-Jacobi round kernel for problems of up to q columns.
+constexpr size_t bj_lds_bytes(int W, int q) { return sizeof(double) * (2 * (size_t)W * q + 2 * (2 * (size_t)W - 1) * W); }
+
 // One round of block Jacobi: workgroup = block pair (bi, bj) of the
-// round-robin over nbb (even) blocks of W columns.
+// round-robin over nbb (even) blocks of W columns, one wave per column pair,
+// X and J of the pair staged in LDS ([2W][q] each) and rotated together.
+// With few problems (or q > 128) its 16 waves per workgroup hide the latency
+// of each rotation chain better than deferring J would save.
 template <int NT, int W, int PL>
 __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int nbb, int inner_sweeps)
 {
@@ -262,43 +354,26 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
 // its three dot products within the row (4 DPP stages, no cross-row step).
 // The reduction and the rotation formulas are thus shared by the four pairs
 // of a wave instead of being paid per pair, which cuts the VALU work per
-// rotation about threefold against one pair per wave.  Same staging, pairing,
-// rotation rule and convergence flag as bj_round_kernel.
+// rotation about threefold against one pair per wave.  Same pairing,
+// rotation rule and convergence flag as bj_round_kernel; J deferred
+// (bj_pair).
 template <int W, int PL>
 __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, int nbb)
 {
   constexpr int NT = W * 16;
   const jac_prob& P = jpick(bt, blockIdx.y);
   const int q = P.q;
-  const int nb = (q + W - 1) / W;
   if (P.flag && P.flag[1]) return;
-  int bi, bj;
-  rr_pair(s, blockIdx.x, nbb, bi, bj);
-  if (bi >= nb && bj >= nb) return;
+  bj_pair<W> bp;
+  if (!bp.setup(P, s, nbb)) return;
   extern __shared__ double lds[];
-  double* Xs = lds;              // [2W][q]
-  double* Js = lds + 2 * W * q;  // [2W][q]
-  const int tid = threadIdx.x;
-  int ncol[2], cbeg[2];
-  cbeg[0] = bi * W;
-  cbeg[1] = bj * W;
-  ncol[0] = bi < nb ? min(W, q - cbeg[0]) : 0;
-  ncol[1] = bj < nb ? min(W, q - cbeg[1]) : 0;
-  for (int e = tid; e < 2 * W * q; e += NT)
-  {
-    const int l = e / q, r = e % q;
-    const int h = l / W, cl = l % W;
-    double xv = 0.0, jv = 0.0;
-    if (cl < ncol[h])
-    {
-      const int g = cbeg[h] + cl;
-      xv = P.X[(size_t)g * P.ldx + r];
-      jv = P.J[(size_t)g * P.ldj + r];
-    }
-    Xs[e] = xv;
-    Js[e] = jv;
-  }
+  double* Xs = lds;
+  double* rcs = lds + 2 * W * q;
+  double* rsn = rcs + (2 * W - 1) * W;
+  double jr[2 * W];
+  bp.template load<NT>(P, Xs, jr);
   __syncthreads();
+  const int tid = threadIdx.x;
   const double tol = jac_tol(q);
   const double negl2 = jac_negl2(P, P.s[0]);
   const int pr = tid >> 4;  // pair slot of this 16-lane group
@@ -314,7 +389,7 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
       li = lj;
       lj = t;
     }
-    const bool real = (li % W) < ncol[li / W] && (lj % W) < ncol[lj / W];
+    const bool real = bp.real(li) && bp.real(lj);
     double* xi = Xs + li * q;
     double* xj = Xs + lj * q;
     double u[PL], v[PL];
@@ -342,7 +417,7 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
     a += dpp_mov<0x128>(a);
     b += dpp_mov<0x128>(b);
     g += dpp_mov<0x128>(g);
-    double cs, sn;
+    double cs = 1.0, sn = 0.0;
     if (real && !(a <= negl2 || b <= negl2) && jacobi_rotation(a, b, g, tol, cs, sn))
     {
       rot = 1;
@@ -356,33 +431,21 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
           xj[t] = sn * u[e] + cs * v[e];
         }
       }
-      double* ji = Js + li * q;
-      double* jj = Js + lj * q;
-#pragma unroll
-      for (int e = 0; e < PL; ++e)
-      {
-        const int t = li16 + 16 * e;
-        if (t < q)
-        {
-          const double y1 = ji[t], y2 = jj[t];
-          ji[t] = cs * y1 - sn * y2;
-          jj[t] = sn * y1 + cs * y2;
-        }
-      }
+    }
+    else
+    {
+      cs = 1.0;
+      sn = 0.0;
+    }
+    if (li16 == 0)
+    {
+      rcs[r2 * W + pr] = cs;
+      rsn[r2 * W + pr] = sn;
     }
     __syncthreads();
   }
-  for (int e = tid; e < 2 * W * q; e += NT)
-  {
-    const int l = e / q, r = e % q;
-    const int h = l / W, cl = l % W;
-    if (cl < ncol[h])
-    {
-      const int g = cbeg[h] + cl;
-      P.X[(size_t)g * P.ldx + r] = Xs[e];
-      P.J[(size_t)g * P.ldj + r] = Js[e];
-    }
-  }
+  if (tid < q) apply_rotations<W>(jr, rcs, rsn);
+  bp.template store<NT>(P, Xs, jr);
   if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
@@ -474,7 +537,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
   else
   {
     // Block width: the staged 2W columns of X and J (32 W q bytes) must fit
-    // in the workgroup's LDS.
+    // in the workgroup's LDS, and q <= 640 bounds the per-lane column slices.
     constexpr int NT = 1024;
     static int max_lds = 0;
     if (max_lds == 0)
@@ -507,10 +570,14 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       }
       return -1;
     }
-    const int W = (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
+    // Batches of q <= 128 fill the chip with 256-thread workgroups, where the
+    // grouped kernel's lower VALU count and deferred J win; few or larger
+    // problems use the 16-wave workgroups of bj_round_kernel.
+    const bool grouped = max_q <= 128 && batch >= 32 && bj_lds_bytes(16, max_q) <= (size_t)max_lds;
+    const int W = grouped || (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
-    const size_t lds = 4 * sizeof(double) * (size_t)W * max_q;
+    const size_t lds = grouped ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
     {
       long g = ((long)max_q * max_q + 255) / 256;
       if (g > 1024) g = 1024;
@@ -525,11 +592,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       {
         const dim3 grid(nbb / 2, batch);
         prof_scope ps(c, TTA_K_JACOBI_ROUND);
-        // Batches of q <= 128 fill the chip with 256-thread workgroups (two
-        // per CU by LDS), where the grouped kernel's lower VALU count wins;
-        // few or larger problems need the 16-wave workgroups of
-        // bj_round_kernel to hide latency.
-        if (W == 16 && max_q <= 128 && batch >= 32)
+        if (grouped)
           hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
         else if (W == 16 && max_q <= 128)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
