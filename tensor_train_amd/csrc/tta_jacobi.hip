@@ -570,10 +570,10 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       }
       return -1;
     }
-    // Batches of q <= 128 fill the chip with 256-thread workgroups, where the
+    // Batches of q <= 256 fill the chip with 256-thread workgroups, where the
     // grouped kernel's lower VALU count and deferred J win; few or larger
     // problems use the 16-wave workgroups of bj_round_kernel.
-    const bool grouped = max_q <= 128 && batch >= 32 && bj_lds_bytes(16, max_q) <= (size_t)max_lds;
+    const bool grouped = max_q <= 256 && batch >= 32 && bj_lds_bytes(16, max_q) <= (size_t)max_lds;
     const int W = grouped || (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
@@ -592,8 +592,10 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       {
         const dim3 grid(nbb / 2, batch);
         prof_scope ps(c, TTA_K_JACOBI_ROUND);
-        if (grouped)
+        if (grouped && max_q <= 128)
           hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
+        else if (grouped)
+          hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
         else if (W == 16 && max_q <= 128)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
         else if (W == 16 && max_q <= 256)
