@@ -345,6 +345,116 @@ __global__ __launch_bounds__(256) void qr_update_kernel(qr_batch bt, int j0, int
   }
 }
 
+// Matrix-core form of qr_update_kernel for 16 columns per workgroup:
+//   W  = V^T C        (16 x 16; each wave accumulates a quarter of the rows
+//                      with v_mfma_f64_16x16x4f64, summed through LDS),
+//   W2 = op(T) W      (16 x 16, one element per thread),
+//   C -= V W2         (computed as (W2^T V^T) so that the accumulator of a
+//                      lane runs down a column of C: coalesced updates).
+// V and C pass through LDS in 64-row chunks.  Same modes as qr_update_kernel.
+typedef double qdbl4 __attribute__((ext_vector_type(4)));
+constexpr int UCT = 16;   // columns per workgroup
+constexpr int URC = 64;   // rows per LDS chunk (16 per wave)
+
+__global__ __launch_bounds__(256) void qr_update_mfma_kernel(qr_batch bt, int j0, int mode)
+{
+  const qrp& P = pick(bt);
+  const int m = P.m, n = P.n;
+  const int k = m < n ? m : n;
+  if (j0 >= k) return;
+  const int bw = (k - j0) < QB ? (k - j0) : QB;
+  const int len = m - j0;
+  const int cbeg = mode == 0 ? j0 + bw : j0;
+  const int cend = mode == 0 ? n : k;
+  const int c0 = cbeg + blockIdx.x * UCT;
+  if (c0 >= cend) return;
+  const int nc = (cend - c0) < UCT ? (cend - c0) : UCT;
+  const int lda = P.lda;
+  const double* A = P.A;
+  double* C = mode == 0 ? P.A : P.Q;
+  const int ldc = mode == 0 ? lda : P.ldq;
+  __shared__ double Vs[URC][QB + 1];
+  __shared__ double Cs[URC][UCT + 1];
+  __shared__ double Wp[4][QB][UCT + 1];
+  __shared__ double W2[QB][UCT + 1];
+  __shared__ double Ts[QB][QB];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const int fl = lane & 15, fk = lane >> 4;
+  const double* Tg = P.T + (size_t)(j0 / QB) * QB * QB;
+  Ts[tid / QB][tid % QB] = Tg[tid];
+  // V(r, jj): unit diagonal, zero above, stored below the diagonal of A.
+  auto vload = [&](int r, int jj) -> double {
+    if (jj >= bw || r >= len) return 0.0;
+    return r == jj ? 1.0 : (r > jj ? A[(size_t)(j0 + jj) * lda + j0 + r] : 0.0);
+  };
+  qdbl4 acc = {0.0, 0.0, 0.0, 0.0};
+  for (int r0 = 0; r0 < len; r0 += URC)
+  {
+    __syncthreads();
+    for (int e = tid; e < URC * QB; e += 256)
+    {
+      const int rr = e % URC, cc = e / URC;
+      Vs[rr][cc] = vload(r0 + rr, cc);
+      Cs[rr][cc] = (cc < nc && r0 + rr < len) ? C[(size_t)(c0 + cc) * ldc + j0 + r0 + rr] : 0.0;
+    }
+    __syncthreads();
+    // Rows 16 wid .. 16 wid + 15 of the chunk, 4 per step: A = V^T, B = C.
+#pragma unroll
+    for (int ks = 0; ks < 16; ks += 4)
+    {
+      const int rr = wid * 16 + ks + fk;
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Vs[rr][fl], Cs[rr][fl], acc, 0, 0, 0);
+    }
+  }
+  // acc[r] = W(fk + 4r, fl) over this wave's rows.
+#pragma unroll
+  for (int r = 0; r < 4; ++r) Wp[wid][fk + 4 * r][fl] = acc[r];
+  __syncthreads();
+  {
+    const int jj = tid / UCT, cc = tid % UCT;
+    double s = 0.0;
+    if (mode == 0)
+    {
+#pragma unroll
+      for (int a = 0; a < QB; ++a) s += Ts[a][jj] * (Wp[0][a][cc] + Wp[1][a][cc] + Wp[2][a][cc] + Wp[3][a][cc]);
+    }
+    else
+    {
+#pragma unroll
+      for (int a = 0; a < QB; ++a) s += Ts[jj][a] * (Wp[0][a][cc] + Wp[1][a][cc] + Wp[2][a][cc] + Wp[3][a][cc]);
+    }
+    W2[jj][cc] = s;
+  }
+  for (int r0 = 0; r0 < len; r0 += URC)
+  {
+    __syncthreads();
+    for (int e = tid; e < URC * QB; e += 256)
+    {
+      const int rr = e % URC, cc = e / URC;
+      Vs[rr][cc] = vload(r0 + rr, cc);
+    }
+    __syncthreads();
+    // Z(col, row) = sum_jj W2(jj, col) V(row, jj) for the wave's 16 rows:
+    // A(i = col, k = jj) = W2(jj, col), B(k = jj, j = row) = V(row, jj).
+    qdbl4 z = {0.0, 0.0, 0.0, 0.0};
+    const int rb = wid * 16;
+#pragma unroll
+    for (int ks = 0; ks < QB; ks += 4)
+      z = __builtin_amdgcn_mfma_f64_16x16x4f64(W2[ks + fk][fl], Vs[rb + fl][ks + fk], z, 0, 0, 0);
+    // z[r] = Z(col = fk + 4r, row = rb + fl).
+    const int row = r0 + rb + fl;
+    if (row < len)
+    {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const int cc = fk + 4 * r;
+        if (cc < nc) C[(size_t)(c0 + cc) * ldc + j0 + row] -= z[r];
+      }
+    }
+  }
+}
+
 // Q = I[:, :k] (m x k, leading dimension ldq).
 __global__ void qr_eye_kernel(qr_batch bt)
 {
@@ -367,6 +477,10 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
   if (batch <= 0) return;
   const int kmax = b.max_m < b.max_n ? b.max_m : b.max_n;
   constexpr int NT = 512;
+  // Batches: 16-column matrix-core updates (enough workgroups anyway);
+  // a single matrix keeps the 8-column VALU update, whose twice as many
+  // workgroups matter more there than the per-column cost.
+  const bool mfma_update = batch >= 16;
   for (int j0 = 0; j0 < kmax; j0 += QB)
   {
     if (b.max_m <= NT)
@@ -378,8 +492,10 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
     const int ncols = b.max_n - j0 - 1;
     if (ncols > 0)
     {
-      dim3 grid((ncols + QCT - 1) / QCT, batch);
-      hipLaunchKernelGGL(qr_update_kernel, grid, dim3(256), 0, st, b, j0, 0);
+      if (mfma_update)
+        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((ncols + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 0);
+      else
+        hipLaunchKernelGGL(qr_update_kernel, dim3((ncols + QCT - 1) / QCT, batch), dim3(256), 0, st, b, j0, 0);
     }
   }
   if (b.descs ? b.form_q : b.one.Q != nullptr)
@@ -391,8 +507,10 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
     const int last = ((kmax - 1) / QB) * QB;
     for (int j0 = last; j0 >= 0; j0 -= QB)
     {
-      dim3 grid((kmax - j0 + QCT - 1) / QCT, batch);
-      hipLaunchKernelGGL(qr_update_kernel, grid, dim3(256), 0, st, b, j0, 1);
+      if (mfma_update)
+        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((kmax - j0 + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 1);
+      else
+        hipLaunchKernelGGL(qr_update_kernel, dim3((kmax - j0 + QCT - 1) / QCT, batch), dim3(256), 0, st, b, j0, 1);
     }
   }
 }
