@@ -574,7 +574,11 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     // grouped kernel's lower VALU count and deferred J win; few or larger
     // problems use the 16-wave workgroups of bj_round_kernel.
     const bool grouped = max_q <= 256 && batch >= 32 && bj_lds_bytes(16, max_q) <= (size_t)max_lds;
-    const int W = grouped || (size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8;
+    // Few problems of q <= 256: blocks of 8 columns, so that a round spreads
+    // over twice as many workgroups (CUs); each CU is bound by its LDS and
+    // VALU throughput on the rotations, not by the round count.
+    const bool narrow = !grouped && batch < 32 && max_q > 128 && max_q <= 256;
+    const int W = grouped ? 16 : (narrow ? 8 : ((size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8));
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
     const size_t lds = grouped ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
@@ -596,6 +600,8 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
           hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
         else if (grouped)
           hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
+        else if (narrow)
+          hipLaunchKernelGGL((bj_round_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb, 1);
         else if (W == 16 && max_q <= 128)
           hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
         else if (W == 16 && max_q <= 256)
