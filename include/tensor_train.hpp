@@ -15,7 +15,8 @@
 //     inner / norm (algebra/tt_inner.hpp:33, :55), frob_inner / frob_norm
 //     (algebra/tt_matrix_inner.hpp:31, :44), scale / axpby / add / sub
 //     (algebra/tt_ops.hpp:35-122), random / zeros / ones
-//     (factory/tt_factory.hpp:31-207, tt_matrix_factory.hpp:133-191);
+//     (factory/tt_factory.hpp:31-207, tt_matrix_factory.hpp:133-191),
+//     from_dense and its options (factory/from_dense.hpp:27-278);
 //   * errors: fprintf(stderr) + std::abort(), as the reference does.
 // Each call copies its host operands to the GPU, runs the device path and
 // copies the result back; callers that want to keep tensors resident use the
@@ -653,6 +654,66 @@ inline tt ones(const std::vector<int>& shape)
     cores.push_back(std::move(c));
   }
   return tt(std::move(cores));
+}
+
+// ---------------------------------------------------------------------------
+// Dense to TT (include/core/factory/from_dense.hpp:27-40, 215-278).
+// ---------------------------------------------------------------------------
+enum class from_dense_method
+{
+  svd,  // TT-SVD (Oseledets' algorithm)
+  qtt   // quantized TT: modes base^b split into b modes of size base
+};
+
+struct from_dense_options
+{
+  double eps = 1.0e-10;
+  int max_rank = 0;
+  from_dense_method method = from_dense_method::svd;
+  int qtt_base = 2;
+  bool validate = true;  // the QTT shape is always checked here
+};
+
+inline tt from_dense(const double* data, const std::vector<int>& shape, const from_dense_options& opts)
+{
+  tta_ctx* c = detail::device_ctx();
+  tta_tt* t = nullptr;
+  detail::check(tta_from_dense(c, data, (int)shape.size(), shape.data(), opts.eps, opts.max_rank,
+                               opts.method == from_dense_method::qtt ? 1 : 0, opts.qtt_base, &t),
+                "from_dense");
+  tt out = detail::download_tt(t);
+  tta_tt_free(c, t);
+  return out;
+}
+
+inline tt from_dense(const double* data, const std::vector<int>& shape, double eps)
+{
+  from_dense_options o;
+  o.eps = eps;
+  return from_dense(data, shape, o);
+}
+
+inline tt_matrix from_dense(const double* data, const std::vector<int>& row_shape, const std::vector<int>& col_shape,
+                            const from_dense_options& opts)
+{
+  if (opts.method == from_dense_method::qtt) tta_die("from_dense(tt_matrix, qtt)", "not implemented");
+  if (row_shape.size() != col_shape.size()) tta_die("matrix_from_dense", "row/col shape length mismatch");
+  tta_ctx* c = detail::device_ctx();
+  tta_tt* t = nullptr;
+  detail::check(tta_matrix_from_dense(c, data, (int)row_shape.size(), row_shape.data(), col_shape.data(), opts.eps,
+                                      opts.max_rank, &t),
+                "matrix_from_dense");
+  tt_matrix out = detail::download_ttm(t);
+  tta_tt_free(c, t);
+  return out;
+}
+
+inline tt_matrix from_dense(const double* data, const std::vector<int>& row_shape, const std::vector<int>& col_shape,
+                            double eps)
+{
+  from_dense_options o;
+  o.eps = eps;
+  return from_dense(data, row_shape, col_shape, o);
 }
 
 }  // namespace tensor_train

@@ -189,6 +189,21 @@ int tta_qr_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* Q, do
  * Vt (k x cols). */
 int tta_svd_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* U, double* s, double* Vt);
 
+/* ---- dense to TT (the step before the path) ------------------------------- */
+/* from_dense(data, shape, {eps, max_rank, method, qtt_base})
+ * (include/core/factory/from_dense.hpp:215-225): method 0 = TT-SVD
+ * (tt_svd.hpp:46-160, Oseledets' algorithm, ||T - TT|| <= eps ||T||),
+ * method 1 = QTT (from_dense.hpp:109-199: modes base^b split into b modes of
+ * size base; base a power of 2).  dense: prod(shape) doubles, row-major (at
+ * most INT_MAX).  A zero tensor gives rank-1 zero cores; non-finite input is
+ * TTA_ERR_ARG. */
+int tta_from_dense(tta_ctx* ctx, const double* dense, int d, const int* shape, double eps, int max_rank, int method,
+                   int qtt_base, tta_tt** out);
+/* matrix_from_dense (include/core/factory/tt_matrix_factory.hpp:205-290):
+ * dense (prod m) x (prod n) row-major -> TT-matrix. */
+int tta_matrix_from_dense(tta_ctx* ctx, const double* dense, int d, const int* row_shape, const int* col_shape,
+                          double eps, int max_rank, tta_tt** out);
+
 /* ---- seeded generators (inputs of the benchmark configurations) --------- */
 /* random(shape, max_rank, seed) (include/core/factory/tt_factory.hpp:141-207),
  * on the host with std::mt19937_64 + std::normal_distribution<double>, drawn

@@ -6,7 +6,7 @@ algorithm of bdaszuta/tensor_train (``include/core/round/tt_round.hpp``,
 ``include/core/detail/{orthogonalize,truncate,contract,streaming_apply,
 fused_apply_core}.hpp``, ``include/core/apply/tt_matrix_apply.hpp``,
 ``include/core/algebra/{tt_inner,tt_ops}.hpp``,
-``include/core/factory/{tt_factory,tt_matrix_factory}.hpp``).
+``include/core/factory/{tt_factory,tt_matrix_factory,tt_svd,from_dense}.hpp``).
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
 ``bench.py`` may import this module, and only as the checker.
@@ -80,6 +80,12 @@ def lib():
         L.ora_qr_thin.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
         L.ora_svd_thin.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]
         L.ora_truncate_eps_rank.argtypes = [_f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.ora_from_dense.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int]
+        L.ora_from_dense.restype = _vp
+        L.ora_from_dense_qtt.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.ora_from_dense_qtt.restype = _vp
+        L.ora_matrix_from_dense.argtypes = [_f64p, ctypes.c_int, _i32p, _i32p, ctypes.c_double, ctypes.c_int]
+        L.ora_matrix_from_dense.restype = _vp
         _lib = L
     return _lib
 
@@ -235,6 +241,25 @@ def svd_thin(M):
     Vt = np.zeros((k, cols))
     lib().ora_svd_thin(_dp(M), rows, cols, _dp(U), _dp(s), _dp(Vt))
     return U, s, Vt
+
+
+def from_dense(dense, shape, eps=1e-10, max_rank=0, method="svd", qtt_base=2):
+    """TT-SVD of a row-major dense tensor (include/core/factory/tt_svd.hpp,
+    from_dense.hpp)."""
+    dense = np.ascontiguousarray(dense, dtype=np.float64).ravel()
+    sh = np.array(shape, np.int32)
+    if method == "qtt":
+        return OTT(lib().ora_from_dense_qtt(_dp(dense), len(sh), _ip(sh), qtt_base, eps, max_rank))
+    return OTT(lib().ora_from_dense(_dp(dense), len(sh), _ip(sh), eps, max_rank))
+
+
+def matrix_from_dense(dense, row_shape, col_shape, eps=1e-10, max_rank=0):
+    """TT-matrix from a row-major (prod m) x (prod n) matrix
+    (include/core/factory/tt_matrix_factory.hpp:205-290)."""
+    dense = np.ascontiguousarray(dense, dtype=np.float64).ravel()
+    ms = np.array(row_shape, np.int32)
+    ns = np.array(col_shape, np.int32)
+    return OTT(lib().ora_matrix_from_dense(_dp(dense), len(ms), _ip(ms), _ip(ns), eps, max_rank), row_shape, col_shape)
 
 
 def truncate_eps_rank(s, delta, max_rank=0):

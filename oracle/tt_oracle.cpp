@@ -963,6 +963,152 @@ static std::vector<double> to_dense(const tt& a)
 // ===========================================================================
 using ora::tt;
 
+namespace ora
+{
+// detail::svd (include/core/factory/tt_svd.hpp:46-160), Oseledets' TT-SVD:
+// norm of the dense buffer, delta = eps ||T|| / sqrt(max(d-1, 1)); for
+// k = 0..d-2 the thin SVD of C viewed as (r_prev n_k) x rest / n_k, core k =
+// U[:, :r_new], C <- diag(s) Vt[:r_new]; the remainder is the last core.
+// A zero tensor gives rank-1 zero cores.
+static tt tt_svd(const double* dense, const std::vector<int>& shape, double eps, int max_rank)
+{
+  const int d = (int)shape.size();
+  tt out;
+  if (d == 0) return out;
+  size_t total = 1;
+  for (int n : shape)
+  {
+    if (n <= 0) die("tt_svd: non-positive mode size");
+    total *= (size_t)n;
+  }
+  const double norm_T = frob_norm_buffer(dense, total);
+  if (!std::isfinite(norm_T)) die("tt_svd: non-finite input");
+  if (norm_T == 0.0)
+  {
+    for (int k = 0; k < d; ++k) out.c.emplace_back(1, shape[k], 1);
+    return out;
+  }
+  const double delta = eps * norm_T / std::sqrt((double)std::max(d - 1, 1));
+  std::vector<double> C(dense, dense + total);
+  int r_prev = 1;
+  size_t rest = total;
+  for (int k = 0; k < d - 1; ++k)
+  {
+    const int n_k = shape[k];
+    const int rows = r_prev * n_k;
+    const int cols = (int)(rest / n_k);
+    mat U, Vt;
+    std::vector<double> s;
+    svd_thin(C.data(), rows, cols, U, s, Vt);
+    const int r_new = truncate_eps_rank(s.data(), (int)s.size(), delta, max_rank);
+    core ck(r_prev, n_k, r_new);
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < r_new; ++j) ck.v[(size_t)i * r_new + j] = U(i, j);
+    out.c.push_back(std::move(ck));
+    C.assign((size_t)r_new * cols, 0.0);
+    for (int i = 0; i < r_new; ++i)
+      for (int j = 0; j < cols; ++j) C[(size_t)i * cols + j] = s[i] * Vt(i, j);
+    r_prev = r_new;
+    rest = (size_t)cols;
+  }
+  core last(r_prev, shape[d - 1], 1);
+  std::memcpy(last.data(), C.data(), sizeof(double) * last.size());
+  out.c.push_back(std::move(last));
+  return out;
+}
+
+// detail::qtt_svd (include/core/factory/from_dense.hpp:109-199): each mode
+// n_k = base^{b_k} splits into b_k modes of size base, the dense data is
+// reordered to the QTT multi-index (lexicographic, last fastest; digits of
+// an original index most significant first), then TT-SVD.
+static tt qtt_svd(const double* data, const std::vector<int>& shape, int base, double eps, int max_rank)
+{
+  const int d0 = (int)shape.size();
+  std::vector<int> bits(d0, 0), qshape;
+  for (int k = 0; k < d0; ++k)
+  {
+    int n = shape[k], b = 0;
+    if (n <= 0) die("qtt_svd: non-positive mode size");
+    while (n > 1)
+    {
+      if (n % base) die("qtt_svd: mode size not a power of the base");
+      n /= base;
+      ++b;
+    }
+    bits[k] = b;
+    for (int i = 0; i < b; ++i) qshape.push_back(base);
+  }
+  if (qshape.empty())
+  {
+    tt out;
+    core c(1, 1, 1);
+    c.v[0] = data[0];
+    out.c.push_back(c);
+    return out;
+  }
+  std::vector<long long> stride(d0, 1);
+  for (int k = d0 - 2; k >= 0; --k) stride[k] = stride[k + 1] * shape[k + 1];
+  size_t total = 1;
+  for (int s : qshape) total *= (size_t)s;
+  std::vector<double> q(total);
+  std::vector<int> idx(qshape.size(), 0);
+  for (size_t pos = 0; pos < total; ++pos)
+  {
+    int bp = 0;
+    long long orig = 0;
+    for (int k = 0; k < d0; ++k)
+    {
+      long long v = 0;
+      for (int b = 0; b < bits[k]; ++b, ++bp) v = v * base + idx[bp];
+      orig += v * stride[k];
+    }
+    q[pos] = data[orig];
+    for (int i = (int)qshape.size() - 1; i >= 0; --i)
+    {
+      if (++idx[i] < base) break;
+      idx[i] = 0;
+    }
+  }
+  return tt_svd(q.data(), qshape, eps, max_rank);
+}
+
+// matrix_from_dense (include/core/factory/tt_matrix_factory.hpp:205-290):
+// the dense (prod m) x (prod n) matrix is permuted to per-axis (m_k, n_k)
+// pairs, axis-major, then TT-SVD over the modes m_k n_k.
+static tt matrix_from_dense(const double* dense, const std::vector<int>& ms, const std::vector<int>& ns, double eps,
+                            int max_rank)
+{
+  const int d = (int)ms.size();
+  std::vector<int> mn(d);
+  size_t M = 1, N = 1;
+  for (int k = 0; k < d; ++k)
+  {
+    mn[k] = ms[k] * ns[k];
+    M *= (size_t)ms[k];
+    N *= (size_t)ns[k];
+  }
+  std::vector<double> perm(M * N);
+  for (size_t I = 0; I < M; ++I)
+    for (size_t J = 0; J < N; ++J)
+    {
+      size_t i = I, j = J, flat = 0, kstride = 1;
+      for (int k = d - 1; k >= 0; --k)
+      {
+        const size_t ik = i % ms[k], jk = j % ns[k];
+        i /= ms[k];
+        j /= ns[k];
+        flat += (ik * ns[k] + jk) * kstride;
+        kstride *= (size_t)mn[k];
+      }
+      perm[flat] = dense[I * N + J];
+    }
+  tt out = tt_svd(perm.data(), mn, eps, max_rank);
+  out.m = ms;
+  out.nn = ns;
+  return out;
+}
+}  // namespace ora
+
 extern "C"
 {
   void* ora_new(int d, const int* n, const int* m, const int* nn, const int* ranks, const double* data)
@@ -1107,5 +1253,17 @@ extern "C"
   int ora_truncate_eps_rank(const double* s, int n, double delta, int max_rank)
   {
     return ora::truncate_eps_rank(s, n, delta, max_rank);
+  }
+  void* ora_from_dense(const double* dense, int d, const int* shape, double eps, int max_rank)
+  {
+    return new tt(ora::tt_svd(dense, std::vector<int>(shape, shape + d), eps, max_rank));
+  }
+  void* ora_from_dense_qtt(const double* dense, int d, const int* shape, int base, double eps, int max_rank)
+  {
+    return new tt(ora::qtt_svd(dense, std::vector<int>(shape, shape + d), base, eps, max_rank));
+  }
+  void* ora_matrix_from_dense(const double* dense, int d, const int* ms, const int* ns, double eps, int max_rank)
+  {
+    return new tt(ora::matrix_from_dense(dense, std::vector<int>(ms, ms + d), std::vector<int>(ns, ns + d), eps, max_rank));
   }
 }

@@ -21,7 +21,7 @@ __all__ = [
     "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
     "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round", "inner",
     "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
-    "zeros", "ones", "random_host",
+    "zeros", "ones", "random_host", "from_dense", "matrix_from_dense",
 ]
 
 # round_method / round_gauge (include/core/round/round.hpp:33-53).
@@ -486,6 +486,37 @@ def zeros(shape, ctx: Context | None = None) -> TensorTrain:
 
 def ones(shape, ctx: Context | None = None) -> TensorTrain:
     return _constant(shape, 1.0, ctx)
+
+
+def from_dense(dense, shape=None, eps=1e-10, max_rank=0, method="svd", qtt_base=2,
+               ctx: Context | None = None) -> TensorTrain:
+    """TT of a dense tensor on the GPU: from_dense with from_dense_options
+    {eps, max_rank, method ("svd" or "qtt"), qtt_base}
+    (include/core/factory/from_dense.hpp:215-225).  ``shape`` defaults to
+    dense.shape; the data is read row-major."""
+    ctx = ctx or Context.default()
+    arr = np.ascontiguousarray(dense, np.float64)
+    sh = np.ascontiguousarray(arr.shape if shape is None else shape, np.int32)
+    if int(np.prod(sh)) != arr.size:
+        raise ValueError(f"from_dense: shape {list(sh)} does not match {arr.size} values")
+    m = {"svd": 0, "qtt": 1}[method]
+    return _out(ctx, ctx.lib.tta_from_dense, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(sh),
+                sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.c_double(eps), max_rank, m, qtt_base)
+
+
+def matrix_from_dense(dense, row_shape, col_shape, eps=1e-10, max_rank=0,
+                      ctx: Context | None = None) -> TensorTrainMatrix:
+    """TT-matrix of a dense (prod m) x (prod n) matrix on the GPU
+    (matrix_from_dense, include/core/factory/tt_matrix_factory.hpp:205-290)."""
+    ctx = ctx or Context.default()
+    arr = np.ascontiguousarray(dense, np.float64)
+    rm = np.ascontiguousarray(row_shape, np.int32)
+    cm = np.ascontiguousarray(col_shape, np.int32)
+    if int(np.prod(rm)) * int(np.prod(cm)) != arr.size:
+        raise ValueError("matrix_from_dense: shapes do not match the data")
+    ip = ctypes.POINTER(ctypes.c_int)
+    return _out(ctx, ctx.lib.tta_matrix_from_dense, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(rm),
+                rm.ctypes.data_as(ip), cm.ctypes.data_as(ip), ctypes.c_double(eps), max_rank, cls=TensorTrainMatrix)
 
 
 def scale_core(t, k: int, alpha: float) -> None:

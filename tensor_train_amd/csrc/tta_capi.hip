@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <stdexcept>
 #include <string>
@@ -434,6 +435,73 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   *F_out = F;
   if (s_out) *s_out = sv;
   return r_new;
+}
+
+void make_rank1_zero(tta_ctx* c, tta_tt* t);
+
+// detail::svd (include/core/factory/tt_svd.hpp:46-160) on the device: the
+// dense tensor (row-major, uploaded once) is swept left to right, each
+// unfolding (r_prev n_k) x rest/n_k through svd_left (QR-condensed Jacobi SVD
+// with the truncation rank decided on the device); core k = U, the next
+// unfolding = diag(s) Vt.  Zero tensor: rank-1 zero cores.
+tta_tt* from_dense_impl(tta_ctx* c, const double* host, const std::vector<int>& shape, double eps, int max_rank)
+{
+  const int d = (int)shape.size();
+  long long total = 1;
+  for (int k = 0; k < d; ++k)
+  {
+    if (shape[k] <= 0) fail(TTA_ERR_SHAPE, "from_dense: mode %d has size %d", k, shape[k]);
+    total *= shape[k];
+    if (total > (long long)std::numeric_limits<int>::max())
+      fail(TTA_ERR_SHAPE, "from_dense: dense tensor above INT_MAX entries");
+  }
+  std::vector<int> ranks(d + 1, 1);
+  tta_tt* t = new_tt(c, shape, ranks, false);
+  if (d == 0) return t;
+  try
+  {
+    double* C = alloc_core(c, (size_t)total);
+    hip_check(hipMemcpyAsync(C, host, (size_t)total * 8, hipMemcpyHostToDevice, c->stream), "upload");
+    const double norm = frob_norm_dev(c, C, (size_t)total, 0.0, 0.0);
+    if (!std::isfinite(norm))
+    {
+      c->mem.put(C);
+      fail(TTA_ERR_ARG, "from_dense: input contains non-finite values (norm %g)", norm);
+    }
+    if (norm == 0.0)
+    {
+      c->mem.put(C);
+      make_rank1_zero(c, t);
+      return t;
+    }
+    const double delta = eps * norm / std::sqrt((double)std::max(d - 1, 1));
+    int r_prev = 1;
+    long long rest = total;
+    for (int k = 0; k < d - 1; ++k)
+    {
+      const int rows = r_prev * shape[k];
+      const int cols = (int)(rest / shape[k]);
+      double *U = nullptr, *F = nullptr;
+      const int r_new = svd_left(c, C, rows, cols, delta, max_rank, false, &U, &F);
+      replace_core(c, t, k, U);
+      // F lives in a workspace the next step reuses: copy it out.
+      double* Cn = alloc_core(c, (size_t)r_new * cols);
+      hip_check(hipMemcpyAsync(Cn, F, (size_t)r_new * cols * 8, hipMemcpyDeviceToDevice, c->stream), "copy");
+      c->mem.put(C);
+      C = Cn;
+      t->ranks[k + 1] = r_new;
+      r_prev = r_new;
+      rest = cols;
+    }
+    replace_core(c, t, d - 1, C);  // (r_prev, n_{d-1}, 1)
+  }
+  catch (...)
+  {
+    for (double* p : t->cores) c->mem.put(p);
+    delete t;
+    throw;
+  }
+  return t;
 }
 
 // svd_step (include/core/detail/truncate.hpp:61-99).
@@ -1452,6 +1520,116 @@ int tta_svd_thin(tta_ctx* c, const double* M, int rows, int cols, double* U, dou
     for (int j = 0; j < cols; ++j) Vt[(size_t)i * cols + j] = s[i] > 0.0 ? F[(size_t)i * cols + j] / s[i] : 0.0;
   c->mem.put(Md);
   c->mem.put(Ud);
+  TTA_GUARD_END(c)
+}
+
+// from_dense (include/core/factory/from_dense.hpp:215-225): method 0 = TT-SVD
+// (tt_svd.hpp:46-160); method 1 = QTT (from_dense.hpp:109-199: each mode
+// n = base^b split into b modes of size base, data reordered on the host,
+// then TT-SVD).
+int tta_from_dense(tta_ctx* c, const double* dense, int d, const int* shape, double eps, int max_rank, int method,
+                   int qtt_base, tta_tt** out)
+{
+  if (!c || !out || d < 0 || (d > 0 && (!dense || !shape)) || !(eps >= 0.0) || max_rank < 0) return TTA_ERR_ARG;
+  if (method != 0 && method != 1) return TTA_ERR_METHOD;
+  TTA_GUARD_BEGIN
+  std::vector<int> sh(shape, shape + d);
+  if (method == 0)
+  {
+    *out = from_dense_impl(c, dense, sh, eps, max_rank);
+  }
+  else
+  {
+    if (qtt_base <= 1 || (qtt_base & (qtt_base - 1)) != 0)
+      fail(TTA_ERR_ARG, "from_dense(qtt): base %d must be a power of 2 above 1", qtt_base);
+    std::vector<int> bits(d, 0), qshape;
+    for (int k = 0; k < d; ++k)
+    {
+      int n = sh[k], b = 0;
+      if (n <= 0) fail(TTA_ERR_SHAPE, "from_dense(qtt): mode %d has size %d", k, n);
+      while (n > 1)
+      {
+        if (n % qtt_base) fail(TTA_ERR_SHAPE, "from_dense(qtt): mode %d size %d is not a power of %d", k, sh[k], qtt_base);
+        n /= qtt_base;
+        ++b;
+      }
+      bits[k] = b;
+      for (int i = 0; i < b; ++i) qshape.push_back(qtt_base);
+    }
+    if (qshape.empty())
+    {
+      // Every mode has size 1: a single scalar, one (1, 1, 1) core.
+      *out = from_dense_impl(c, dense, std::vector<int>{1}, 0.0, 0);
+    }
+    else
+    {
+      std::vector<long long> stride(d, 1);
+      for (int k = d - 2; k >= 0; --k) stride[k] = stride[k + 1] * sh[k + 1];
+      size_t total = 1;
+      for (int q : qshape) total *= (size_t)q;
+      std::vector<double> qd(total);
+      std::vector<int> idx(qshape.size(), 0);
+      for (size_t pos = 0; pos < total; ++pos)
+      {
+        int bp = 0;
+        long long orig = 0;
+        for (int k = 0; k < d; ++k)
+        {
+          long long v = 0;
+          for (int b = 0; b < bits[k]; ++b, ++bp) v = v * qtt_base + idx[bp];
+          orig += v * stride[k];
+        }
+        qd[pos] = dense[orig];
+        for (int i = (int)qshape.size() - 1; i >= 0; --i)
+        {
+          if (++idx[i] < qtt_base) break;
+          idx[i] = 0;
+        }
+      }
+      *out = from_dense_impl(c, qd.data(), qshape, eps, max_rank);
+    }
+  }
+  TTA_GUARD_END(c)
+}
+
+// matrix_from_dense (include/core/factory/tt_matrix_factory.hpp:205-290): the
+// (prod m) x (prod n) row-major matrix is permuted on the host to per-axis
+// (m_k, n_k) pairs, then TT-SVD over the modes m_k n_k.
+int tta_matrix_from_dense(tta_ctx* c, const double* dense, int d, const int* row_shape, const int* col_shape,
+                          double eps, int max_rank, tta_tt** out)
+{
+  if (!c || !out || d < 1 || !dense || !row_shape || !col_shape || !(eps >= 0.0) || max_rank < 0) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  std::vector<int> ms(row_shape, row_shape + d), ns(col_shape, col_shape + d), mn(d);
+  size_t M = 1, N = 1;
+  for (int k = 0; k < d; ++k)
+  {
+    if (ms[k] <= 0 || ns[k] <= 0) fail(TTA_ERR_SHAPE, "matrix_from_dense: axis %d has size %d x %d", k, ms[k], ns[k]);
+    mn[k] = ms[k] * ns[k];
+    M *= (size_t)ms[k];
+    N *= (size_t)ns[k];
+  }
+  if (M * N > (size_t)std::numeric_limits<int>::max()) fail(TTA_ERR_SHAPE, "matrix_from_dense: above INT_MAX entries");
+  std::vector<double> perm(M * N);
+  for (size_t I = 0; I < M; ++I)
+    for (size_t J = 0; J < N; ++J)
+    {
+      size_t i = I, j = J, flat = 0, kstride = 1;
+      for (int k = d - 1; k >= 0; --k)
+      {
+        const size_t ik = i % (size_t)ms[k], jk = j % (size_t)ns[k];
+        i /= (size_t)ms[k];
+        j /= (size_t)ns[k];
+        flat += (ik * (size_t)ns[k] + jk) * kstride;
+        kstride *= (size_t)mn[k];
+      }
+      perm[flat] = dense[I * N + J];
+    }
+  tta_tt* t = from_dense_impl(c, perm.data(), mn, eps, max_rank);
+  t->row_modes = ms;
+  t->col_modes = ns;
+  t->matrix = true;
+  *out = t;
   TTA_GUARD_END(c)
 }
 

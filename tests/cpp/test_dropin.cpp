@@ -172,6 +172,17 @@ int main()
     tt::tt e;
     CHECK(e.ranks() == std::vector<int>({1}) && e.to_dense() == std::vector<double>({1.0}), "d=0 conventions");
   }
+  // test_tt_svd.cpp: from_dense recovers a rank-1 outer product with ranks 1.
+  {
+    const double u[3] = {1.0, 2.0, 3.0}, v[4] = {1.0, -1.0, 0.5, 0.25}, w[5] = {2.0, 0.5, -1.0, 1.0, 3.0};
+    std::vector<double> T(60);
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 4; ++j)
+        for (int k = 0; k < 5; ++k) T[(i * 4 + j) * 5 + k] = u[i] * v[j] * w[k];
+    auto t = tt::from_dense(T.data(), {3, 4, 5}, 1e-12);
+    CHECK(t.ranks() == std::vector<int>({1, 1, 1, 1}), "from_dense: rank-1 outer product, ranks 1");
+    CHECK(max_abs_diff(t.to_dense(), T) <= 1e-12 * frob(T), "from_dense: reconstruction");
+  }
   std::printf("%d failure(s)\n", failures);
   return failures;
 }

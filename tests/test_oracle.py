@@ -345,3 +345,65 @@ def test_rel_err_resolves_tiny_differences():
     # An exact copy leaves only the rounding of the block QR of [a, -a].
     assert rel_err(a.cores(), a) <= 2e-15
     assert np.allclose(dense_from_cores(a.cores()), a.to_dense())
+
+
+def _np_tt_svd(T, shape, eps, max_rank=0):
+    """Independent TT-SVD with numpy's LAPACK SVD and the reference's rank
+    rule (for pinning the oracle's from_dense)."""
+    d = len(shape)
+    flat = np.asarray(T, np.float64).ravel()
+    nrm = np.linalg.norm(flat)
+    delta = eps * nrm / np.sqrt(max(d - 1, 1))
+    cores, r, C = [], 1, flat
+    for k in range(d - 1):
+        M = C.reshape(r * shape[k], -1)
+        U, s, Vt = np.linalg.svd(M, full_matrices=False)
+        rk = O.truncate_eps_rank(s, delta, max_rank)
+        cores.append(U[:, :rk].reshape(r, shape[k], rk))
+        C = (s[:rk, None] * Vt[:rk]).ravel()
+        r = rk
+    cores.append(C.reshape(r, shape[-1], 1))
+    return cores
+
+
+def test_from_dense_known_answers():
+    # tests/core/test_tt_svd.cpp:24-104: rank-1 outer product recovered with
+    # ranks 1; random tensor within 10 eps; zero tensor -> rank-1 zeros.
+    u, v, w = np.array([1.0, 2, 3]), np.array([1.0, -1, 0.5, 0.25]), np.array([2.0, 0.5, -1, 1, 3])
+    T = np.einsum("i,j,k->ijk", u, v, w)
+    t = O.from_dense(T, [3, 4, 5], 1e-12)
+    assert t.ranks == [1, 1, 1, 1]
+    assert np.linalg.norm(t.to_dense().ravel() - T.ravel()) <= 1e-10 * np.linalg.norm(T)
+    R = np.random.default_rng(42).standard_normal((4, 5, 6))
+    t = O.from_dense(R, [4, 5, 6], 1e-8)
+    assert np.linalg.norm(t.to_dense().ravel() - R.ravel()) <= 10 * 1e-8 * np.linalg.norm(R)
+    z = O.from_dense(np.zeros(24), [2, 3, 4], 1e-10)
+    assert z.ranks == [1, 1, 1, 1] and not np.any(z.to_dense())
+
+
+@pytest.mark.parametrize("shape,rank,eps", [([3, 4, 2, 5], 4, 1e-12), ([4, 4, 4, 4, 4], 6, 1e-6), ([7, 2, 9], 3, 0.0)])
+def test_from_dense_matches_numpy_tt_svd(shape, rank, eps):
+    a = O.OTT.random(shape, rank, 17)
+    D = a.to_dense().ravel()
+    D = D + 1e-4 * np.random.default_rng(3).standard_normal(D.size)  # full rank, decaying
+    t = O.from_dense(D, shape, eps)
+    ref = _np_tt_svd(D, shape, eps)
+    assert t.ranks == [1] + [c.shape[2] for c in ref]
+    Dt = t.to_dense().ravel()
+    Dr = dense_from_cores(ref).ravel()
+    assert np.linalg.norm(Dt - Dr) <= 1e-12 * np.linalg.norm(D)
+    assert np.linalg.norm(Dt - D) <= max(eps, 1e-13) * np.linalg.norm(D) * 1.0000001  # eps = 0: rounding only
+
+
+def test_from_dense_qtt_and_matrix():
+    # QTT of a linear function of the index has ranks 2 (from_dense.hpp:109-199).
+    x = np.arange(64.0).reshape(8, 8)
+    q = O.from_dense(x, [8, 8], 1e-12, method="qtt")
+    assert q.ranks == [1, 2, 2, 2, 2, 2, 1]
+    # The QTT index order is the binary digits of (i, j), most significant first.
+    assert np.allclose(q.to_dense().ravel(), x.ravel(), rtol=0, atol=1e-12)
+    M = np.random.default_rng(5).standard_normal((12, 20))
+    tm = O.matrix_from_dense(M, [3, 4], [4, 5], 1e-13)
+    # to_dense of a TT-matrix is (m0, n0, m1, n1); back to (m0 m1) x (n0 n1).
+    Dm = tm.to_dense().reshape(3, 4, 4, 5).transpose(0, 2, 1, 3).reshape(12, 20)
+    assert np.linalg.norm(Dm - M) <= 1e-12 * np.linalg.norm(M)
