@@ -189,6 +189,11 @@ int tta_qr_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* Q, do
  * Vt (k x cols). */
 int tta_svd_thin(tta_ctx* ctx, const double* M, int rows, int cols, double* U, double* s, double* Vt);
 
+/* soft_threshold(a, tau) (include/core/round/tt_soft_threshold.hpp:42-176):
+ * the proximal step of the nuclear norm at every bond, singular values shrunk
+ * to max(s - tau, 0); a copy when d <= 1 or tau <= 0. */
+int tta_soft_threshold(tta_ctx* ctx, const tta_tt* a, double tau, tta_tt** out);
+
 /* ---- dense to TT (the step before the path) ------------------------------- */
 /* from_dense(data, shape, {eps, max_rank, method, qtt_base})
  * (include/core/factory/from_dense.hpp:215-225): method 0 = TT-SVD

@@ -2,7 +2,7 @@
 
 ctypes wrapper over ``oracle/_build/libtt_oracle.so`` (built from
 ``oracle/tt_oracle.cpp`` by ``oracle/Makefile``).  It restates the reference
-algorithm of bdaszuta/tensor_train (``include/core/round/tt_round.hpp``,
+algorithm of bdaszuta/tensor_train (``include/core/round/{tt_round,tt_soft_threshold}.hpp``,
 ``include/core/detail/{orthogonalize,truncate,contract,streaming_apply,
 fused_apply_core}.hpp``, ``include/core/apply/tt_matrix_apply.hpp``,
 ``include/core/algebra/{tt_inner,tt_ops}.hpp``,
@@ -80,6 +80,8 @@ def lib():
         L.ora_qr_thin.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]
         L.ora_svd_thin.argtypes = [_f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]
         L.ora_truncate_eps_rank.argtypes = [_f64p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.ora_soft_threshold.argtypes = [_vp, ctypes.c_double]
+        L.ora_soft_threshold.restype = _vp
         L.ora_from_dense.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int]
         L.ora_from_dense.restype = _vp
         L.ora_from_dense_qtt.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
@@ -241,6 +243,11 @@ def svd_thin(M):
     Vt = np.zeros((k, cols))
     lib().ora_svd_thin(_dp(M), rows, cols, _dp(U), _dp(s), _dp(Vt))
     return U, s, Vt
+
+
+def soft_threshold(a, tau=0.0):
+    """Singular-value soft thresholding (include/core/round/tt_soft_threshold.hpp)."""
+    return OTT(lib().ora_soft_threshold(a.h, tau))
 
 
 def from_dense(dense, shape, eps=1e-10, max_rank=0, method="svd", qtt_base=2):

@@ -965,6 +965,54 @@ using ora::tt;
 
 namespace ora
 {
+// soft_threshold (include/core/round/tt_soft_threshold.hpp:42-176): d <= 1
+// or tau <= 0 returns a copy; otherwise right-orthogonalize, then for
+// k = d-1..1 the thin SVD of left_unfold(c_{k-1}) right_unfold(c_k), the
+// singular values shrunk to max(s - tau, 0), the rank cut after the last
+// positive one; c_{k-1} <- U diag(s_shrunk), c_k <- Vt.  When every value is
+// zeroed both cores become rank-1 zeros at that bond.
+static tt soft_threshold(tt acc, double tau)
+{
+  const int d = acc.d();
+  if (d <= 1 || tau <= 0.0) return acc;
+  right_orthogonalize(acc);
+  for (int k = d - 1; k >= 1; --k)
+  {
+    core& cp = acc.c[k - 1];
+    core& cc = acc.c[k];
+    const int rL = cp.r, np = cp.n, r = cp.rp, nc = cc.n, rR = cc.rp;
+    const int rows = rL * np, cols = nc * rR;
+    std::vector<double> M((size_t)rows * cols, 0.0);
+    gemm(rows, cols, r, cp.data(), r, cc.data(), cols, M.data(), cols);
+    mat U, Vt;
+    std::vector<double> s;
+    svd_thin(M.data(), rows, cols, U, s, Vt);
+    const int nsv = (int)s.size();
+    int r_new = 0;
+    std::vector<double> sh(nsv, 0.0);
+    for (int j = 0; j < nsv; ++j)
+      if (s[j] > tau)
+      {
+        sh[j] = s[j] - tau;
+        r_new = j + 1;
+      }
+    if (nsv == 0 || sh[0] <= 0.0)
+    {
+      cp = core(rL, np, 1);
+      cc = core(1, nc, rR);
+      continue;
+    }
+    core np_(rL, np, r_new), nc_(r_new, nc, rR);
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < r_new; ++j) np_.v[(size_t)i * r_new + j] = U(i, j) * (sh[j] > 0.0 ? sh[j] : 1.0);
+    for (int j = 0; j < r_new; ++j)
+      for (int t = 0; t < cols; ++t) nc_.v[(size_t)j * cols + t] = Vt(j, t);
+    cp = std::move(np_);
+    cc = std::move(nc_);
+  }
+  return acc;
+}
+
 // detail::svd (include/core/factory/tt_svd.hpp:46-160), Oseledets' TT-SVD:
 // norm of the dense buffer, delta = eps ||T|| / sqrt(max(d-1, 1)); for
 // k = 0..d-2 the thin SVD of C viewed as (r_prev n_k) x rest / n_k, core k =
@@ -1254,6 +1302,7 @@ extern "C"
   {
     return ora::truncate_eps_rank(s, n, delta, max_rank);
   }
+  void* ora_soft_threshold(void* a, double tau) { return new tt(ora::soft_threshold(*static_cast<tt*>(a), tau)); }
   void* ora_from_dense(const double* dense, int d, const int* shape, double eps, int max_rank)
   {
     return new tt(ora::tt_svd(dense, std::vector<int>(shape, shape + d), eps, max_rank));

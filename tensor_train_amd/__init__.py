@@ -21,7 +21,7 @@ __all__ = [
     "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
     "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round", "inner",
     "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
-    "zeros", "ones", "random_host", "from_dense", "matrix_from_dense",
+    "zeros", "ones", "random_host", "from_dense", "matrix_from_dense", "soft_threshold",
 ]
 
 # round_method / round_gauge (include/core/round/round.hpp:33-53).
@@ -486,6 +486,11 @@ def zeros(shape, ctx: Context | None = None) -> TensorTrain:
 
 def ones(shape, ctx: Context | None = None) -> TensorTrain:
     return _constant(shape, 1.0, ctx)
+
+
+def soft_threshold(a, tau=0.0):
+    """soft_threshold(a, tau) (include/core/round/tt_soft_threshold.hpp:42-176)."""
+    return _out(a._ctx, a._ctx.lib.tta_soft_threshold, a._h, ctypes.c_double(tau))
 
 
 def from_dense(dense, shape=None, eps=1e-10, max_rank=0, method="svd", qtt_base=2,

@@ -1523,6 +1523,82 @@ int tta_svd_thin(tta_ctx* c, const double* M, int rows, int cols, double* U, dou
   TTA_GUARD_END(c)
 }
 
+// soft_threshold (include/core/round/tt_soft_threshold.hpp:42-176): a copy of
+// a when d <= 1 or tau <= 0; otherwise right-orthogonalize, then for
+// k = d-1..1 the thin SVD of left_unfold(c_{k-1}) right_unfold(c_k) (device
+// GEMM + QR-condensed Jacobi), singular values shrunk to max(s - tau, 0) and
+// cut after the last positive one: c_{k-1} <- U diag(s - tau), c_k <- Vt.
+// All values zeroed: rank-1 zero cores at that bond.
+int tta_soft_threshold(tta_ctx* c, const tta_tt* a, double tau, tta_tt** out)
+{
+  if (!c || !a || !out) return TTA_ERR_ARG;
+  if (a->is_matrix()) return TTA_ERR_SHAPE;
+  TTA_GUARD_BEGIN
+  tta_tt* t = copy_tt(c, a);
+  try
+  {
+    const int d = t->d();
+    if (d > 1 && tau > 0.0)
+    {
+      right_orthogonalize_impl(c, t);
+      for (int k = d - 1; k >= 1; --k)
+      {
+        const int rL = t->ranks[k - 1], np = t->modes[k - 1], r = t->ranks[k], nc = t->modes[k], rR = t->ranks[k + 1];
+        const int rows = rL * np, cols = nc * rR, kk = std::min(rows, cols);
+        double* M = alloc_core(c, (size_t)rows * cols);
+        gemm_rm(c, rows, cols, r, t->cores[k - 1], r, t->cores[k], cols, M, cols);
+        double *U = nullptr, *F = nullptr, *sd = nullptr;
+        svd_left(c, M, rows, cols, 0.0, 0, true, &U, &F, &sd);
+        c->mem.put(M);
+        std::vector<double> s((size_t)kk);
+        hip_check(hipMemcpyAsync(s.data(), sd, (size_t)kk * 8, hipMemcpyDeviceToHost, c->stream), "s");
+        ctx_sync(c);
+        int r_new = 0;
+        std::vector<double> sh((size_t)kk, 0.0), inv((size_t)kk, 0.0);
+        for (int j = 0; j < kk; ++j)
+          if (s[j] > tau)
+          {
+            sh[j] = s[j] - tau;
+            inv[j] = 1.0 / s[j];
+            r_new = j + 1;
+          }
+        if (r_new == 0)
+        {
+          c->mem.put(U);
+          double* zp = alloc_core(c, (size_t)rows);
+          double* zc = alloc_core(c, (size_t)cols);
+          hip_check(hipMemsetAsync(zp, 0, (size_t)rows * 8, c->stream), "zero");
+          hip_check(hipMemsetAsync(zc, 0, (size_t)cols * 8, c->stream), "zero");
+          replace_core(c, t, k - 1, zp);
+          replace_core(c, t, k, zc);
+          t->ranks[k] = 1;
+          continue;
+        }
+        double* f = ctx_ws(c, WS_JP, 2 * (size_t)r_new);
+        std::vector<double> fh(sh.begin(), sh.begin() + r_new);
+        fh.insert(fh.end(), inv.begin(), inv.begin() + r_new);
+        upload_async(c, f, fh.data(), fh.size() * 8);
+        double* Pn = alloc_core(c, (size_t)rows * r_new);
+        launch_scale_copy(U, rows, r_new, kk, nullptr, f, Pn, c->stream);
+        double* Cn = alloc_core(c, (size_t)r_new * cols);
+        launch_scale_copy(F, r_new, cols, cols, f + r_new, nullptr, Cn, c->stream);
+        c->mem.put(U);
+        replace_core(c, t, k - 1, Pn);
+        replace_core(c, t, k, Cn);
+        t->ranks[k] = r_new;
+      }
+    }
+  }
+  catch (...)
+  {
+    for (double* p : t->cores) c->mem.put(p);
+    delete t;
+    throw;
+  }
+  *out = t;
+  TTA_GUARD_END(c)
+}
+
 // from_dense (include/core/factory/from_dense.hpp:215-225): method 0 = TT-SVD
 // (tt_svd.hpp:46-160); method 1 = QTT (from_dense.hpp:109-199: each mode
 // n = base^b split into b modes of size base, data reordered on the host,

@@ -614,6 +614,28 @@ void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst,
                      dcs);
 }
 
+__global__ void scale_copy_kernel(const double* src, int rows, int cols, long ld, const double* rowf, const double* colf,
+                                  double* dst)
+{
+  const long total = (long)rows * cols;
+  for (long x = blockIdx.x * (long)blockDim.x + threadIdx.x; x < total; x += (long)gridDim.x * blockDim.x)
+  {
+    const int i = (int)(x / cols), j = (int)(x % cols);
+    double v = src[(size_t)i * ld + j];
+    if (rowf) v *= rowf[i];
+    if (colf) v *= colf[j];
+    dst[x] = v;
+  }
+}
+
+void launch_scale_copy(const double* src, int rows, int cols, long ld, const double* rowf, const double* colf,
+                       double* dst, hipStream_t st)
+{
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(scale_copy_kernel, dim3(grid_for((long)rows * cols)), dim3(256), 0, st, src, rows, cols, ld, rowf,
+                     colf, dst);
+}
+
 void launch_gather_cols(const double* src, int lds_, int rows, const int* perm, int ncols, double* dst, long drs,
                         long dcs, hipStream_t st, const double* scale, int mode)
 {

@@ -223,6 +223,10 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
 void launch_qr(const qr_desc& q, int batch, hipStream_t st);
 void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
 void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
+// dst (rows x cols, packed row-major) = src (row-major, leading dimension ld)
+// with row i scaled by rowf[i] and column j by colf[j] (either may be null).
+void launch_scale_copy(const double* src, int rows, int cols, long ld, const double* rowf, const double* colf,
+                       double* dst, hipStream_t st);
 void launch_gather_cols(const double* src, int lds, int rows, const int* perm, int ncols, double* dst, long drs,
                         long dcs, hipStream_t st,
                         const double* scale = nullptr, int mode = GATHER_COPY);

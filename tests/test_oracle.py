@@ -407,3 +407,23 @@ def test_from_dense_qtt_and_matrix():
     # to_dense of a TT-matrix is (m0, n0, m1, n1); back to (m0 m1) x (n0 n1).
     Dm = tm.to_dense().reshape(3, 4, 4, 5).transpose(0, 2, 1, 3).reshape(12, 20)
     assert np.linalg.norm(Dm - M) <= 1e-12 * np.linalg.norm(M)
+
+
+def test_soft_threshold_closed_form():
+    # d = 2: the TT is one matrix, and soft thresholding is U max(s - tau, 0) V^T
+    # of its LAPACK SVD; tau <= 0 and d = 1 return copies
+    # (include/core/round/tt_soft_threshold.hpp:42-52).
+    a = O.OTT.random([6, 7], 5, 3)
+    D = a.to_dense().reshape(6, 7)
+    U, s, Vt = np.linalg.svd(D, full_matrices=False)
+    for tau in (0.4 * s[2], 0.9 * s[0], 1.1 * s[0]):
+        r = O.soft_threshold(a, tau)
+        ref = (U * np.maximum(s - tau, 0.0)) @ Vt
+        assert r.ranks[1] == max(1, int(np.sum(s > tau)))
+        assert np.abs(r.to_dense().reshape(6, 7) - ref).max() <= 1e-13 * max(1.0, np.abs(D).max())
+    assert np.array_equal(O.soft_threshold(a, 0.0).to_dense(), a.to_dense())
+    one = O.OTT.random([5], 1, 2)
+    assert np.array_equal(O.soft_threshold(one, 3.0).to_dense(), one.to_dense())
+    # Every step shrinks: the norm never grows.
+    b = O.OTT.random([3, 4, 3, 2], 6, 9)
+    assert O.norm(O.soft_threshold(b, 0.1)) <= O.norm(b) * (1 + 1e-14)
