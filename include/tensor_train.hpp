@@ -21,7 +21,8 @@
 // Each call copies its host operands to the GPU, runs the device path and
 // copies the result back; callers that want to keep tensors resident use the
 // C ABI (tt_amd.h) directly.  One device context per thread (device 0, or
-// TT_AMD_DEVICE).  ALS / DMRG methods are not provided (abort).
+// TT_AMD_DEVICE).  round_method::als runs the ALS engine on the device
+// (round/tt_als.hpp); round_method::dmrg is not provided (abort).
 #pragma once
 
 #include <array>
@@ -261,6 +262,71 @@ class tt_matrix
     for (auto& c : cores_) s += c.size();
     return s;
   }
+  std::size_t total_rows() const
+  {
+    std::size_t s = 1;
+    for (auto& c : cores_) s *= (std::size_t)c.m_phys();
+    return s;
+  }
+  std::size_t total_cols() const
+  {
+    std::size_t s = 1;
+    for (auto& c : cores_) s *= (std::size_t)c.n_phys();
+    return s;
+  }
+  // Dense row-major (total_rows x total_cols) matrix, row index (i_1..i_d),
+  // column index (j_1..j_d), both with the last site fastest
+  // (include/core/types/tt_matrix.hpp:415-500); d = 0 is [1].
+  std::vector<double> to_dense() const
+  {
+    const int d = (int)cores_.size();
+    if (d == 0) return {1.0};
+    // T holds prod_k (m_k n_k) rows over the interleaved (i_1 j_1 .. i_k j_k)
+    // index times the current right rank.
+    std::vector<double> T(cores_[0].size());
+    for (std::size_t t = 0; t < T.size(); ++t) T[t] = cores_[0].data()[t];
+    std::size_t rows = (std::size_t)cores_[0].m_phys() * cores_[0].n_phys();
+    for (int k = 1; k < d; ++k)
+    {
+      const tt_matrix_core& c = cores_[k];
+      const std::size_t r = c.r_left(), w = (std::size_t)c.m_phys() * c.n_phys() * c.r_right();
+      std::vector<double> Tn(rows * w, 0.0);
+      for (std::size_t i = 0; i < rows; ++i)
+        for (std::size_t a = 0; a < r; ++a)
+        {
+          const double x = T[i * r + a];
+          if (x == 0.0) continue;
+          const double* src = c.data() + a * w;
+          double* dst = Tn.data() + i * w;
+          for (std::size_t t = 0; t < w; ++t) dst[t] += x * src[t];
+        }
+      T.swap(Tn);
+      rows *= (std::size_t)c.m_phys() * c.n_phys();
+    }
+    const std::size_t M = total_rows(), N = total_cols();
+    std::vector<double> out(M * N);
+    std::vector<int> ii(d, 0), jj(d, 0);
+    for (std::size_t flat = 0; flat < rows; ++flat)
+    {
+      std::size_t rem = flat;
+      for (int k = d - 1; k >= 0; --k)
+      {
+        const int m = cores_[k].m_phys(), n = cores_[k].n_phys();
+        const std::size_t ij = rem % ((std::size_t)m * n);
+        rem /= (std::size_t)m * n;
+        ii[k] = (int)(ij / n);
+        jj[k] = (int)(ij % n);
+      }
+      std::size_t I = 0, J = 0;
+      for (int k = 0; k < d; ++k)
+      {
+        I = I * cores_[k].m_phys() + ii[k];
+        J = J * cores_[k].n_phys() + jj[k];
+      }
+      out[I * N + J] = T[flat];
+    }
+    return out;
+  }
   const tt_matrix_core& core(int k) const { return cores_.at((size_t)k); }
   tt_matrix_core& core(int k) { return cores_.at((size_t)k); }
   const std::vector<tt_matrix_core>& cores() const { return cores_; }
@@ -394,18 +460,21 @@ inline tt_matrix download_ttm(const tta_tt* t)
 
 inline int method_code(round_method m) { return static_cast<int>(m); }
 
-inline void svd_result_default(round_result* r)
+inline bool iterative(round_method m) { return m == round_method::als || m == round_method::dmrg; }
+
+// The warm start travels only to the iterative engines (the SVD paths ignore it).
+template <class T>
+inline tta_tt* upload_warm(round_method m, const T* w)
 {
-  if (!r) return;
-  r->iters_run = 1;
-  r->final_resid = 0.0;
-  r->converged = true;
+  return (w && iterative(m)) ? upload(*w) : nullptr;
 }
 
-inline void reject_iterative(round_method m, const char* what)
+inline void write_result(round_result* r, int iters, double resid, int conv)
 {
-  if (m == round_method::als || m == round_method::dmrg)
-    tta_die(what, "ALS / DMRG engines are not provided by this library");
+  if (!r) return;
+  r->iters_run = iters;
+  r->final_resid = resid;
+  r->converged = conv != 0;
 }
 }  // namespace detail
 
@@ -417,14 +486,17 @@ inline tt round(const tt& a, const round_options& opts, round_result* info = nul
   if (opts.method == round_method::svd_streaming || opts.method == round_method::svd_naive)
     tta_die("round(tt)",
             "round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round respectively.");
-  detail::reject_iterative(opts.method, "round(tt)");
   detail::dev_handle in(detail::upload(a));
+  detail::dev_handle w(detail::upload_warm(opts.method, opts.warm_start_tt));
   tta_tt* out = nullptr;
-  detail::check(tta_round(detail::device_ctx(), in.t, opts.eps, opts.max_rank, detail::method_code(opts.method),
-                          static_cast<int>(opts.gauge), &out),
+  int it = 0, cv = 0;
+  double ch = 0.0;
+  detail::check(tta_round_iterative(detail::device_ctx(), in.t, opts.eps, opts.max_rank,
+                                    detail::method_code(opts.method), static_cast<int>(opts.gauge), opts.max_iters,
+                                    opts.tol, opts.seed, w.t, &out, &it, &ch, &cv),
                 "round(tt)");
   detail::dev_handle o(out);
-  detail::svd_result_default(info);
+  detail::write_result(info, it, ch, cv);
   return detail::download_tt(out);
 }
 
@@ -439,14 +511,17 @@ inline tt_matrix round(const tt_matrix& a, const round_options& opts, round_resu
   if (opts.method == round_method::svd_streaming || opts.method == round_method::svd_naive)
     tta_die("round(tt_matrix)",
             "round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round respectively.");
-  detail::reject_iterative(opts.method, "round(tt_matrix)");
   detail::dev_handle in(detail::upload(a));
+  detail::dev_handle w(detail::upload_warm(opts.method, opts.warm_start_tt_matrix));
   tta_tt* out = nullptr;
-  detail::check(tta_round(detail::device_ctx(), in.t, opts.eps, opts.max_rank, detail::method_code(opts.method),
-                          static_cast<int>(opts.gauge), &out),
+  int it = 0, cv = 0;
+  double ch = 0.0;
+  detail::check(tta_round_iterative(detail::device_ctx(), in.t, opts.eps, opts.max_rank,
+                                    detail::method_code(opts.method), static_cast<int>(opts.gauge), opts.max_iters,
+                                    opts.tol, opts.seed, w.t, &out, &it, &ch, &cv),
                 "round(tt_matrix)");
   detail::dev_handle o(out);
-  detail::svd_result_default(info);
+  detail::write_result(info, it, ch, cv);
   return detail::download_ttm(out);
 }
 
@@ -506,14 +581,17 @@ inline tt_matrix matmat(const tt_matrix& A, const tt_matrix& B)
 inline tt matvec_round(const tt_matrix& A, const tt& x, const round_options& opts, round_result* info = nullptr)
 {
   if (opts.method == round_method::svd_naive) tta_die("matvec_round", "round_method::svd_naive is matmat-only");
-  detail::reject_iterative(opts.method, "matvec_round");
   detail::dev_handle a(detail::upload(A)), b(detail::upload(x));
+  detail::dev_handle w(detail::upload_warm(opts.method, opts.warm_start_tt));
   tta_tt* out = nullptr;
-  detail::check(tta_matvec_round(detail::device_ctx(), a.t, b.t, opts.eps, opts.max_rank,
-                                 detail::method_code(opts.method), &out),
+  int it = 0, cv = 0;
+  double ch = 0.0;
+  detail::check(tta_matvec_round_iterative(detail::device_ctx(), a.t, b.t, opts.eps, opts.max_rank,
+                                           detail::method_code(opts.method), opts.max_iters, opts.tol, opts.seed,
+                                           w.t, &out, &it, &ch, &cv),
                 "matvec_round");
   detail::dev_handle o(out);
-  detail::svd_result_default(info);
+  detail::write_result(info, it, ch, cv);
   return detail::download_tt(out);
 }
 
@@ -522,14 +600,17 @@ inline tt_matrix matmat_round(const tt_matrix& A, const tt_matrix& B, const roun
 {
   if (opts.method == round_method::svd_streaming)
     tta_die("matmat_round", "round_method::svd_streaming is matvec-only");
-  detail::reject_iterative(opts.method, "matmat_round");
   detail::dev_handle a(detail::upload(A)), b(detail::upload(B));
+  detail::dev_handle w(detail::upload_warm(opts.method, opts.warm_start_tt_matrix));
   tta_tt* out = nullptr;
-  detail::check(tta_matmat_round(detail::device_ctx(), a.t, b.t, opts.eps, opts.max_rank,
-                                 detail::method_code(opts.method), &out),
+  int it = 0, cv = 0;
+  double ch = 0.0;
+  detail::check(tta_matmat_round_iterative(detail::device_ctx(), a.t, b.t, opts.eps, opts.max_rank,
+                                           detail::method_code(opts.method), opts.max_iters, opts.tol, opts.seed,
+                                           w.t, &out, &it, &ch, &cv),
                 "matmat_round");
   detail::dev_handle o(out);
-  detail::svd_result_default(info);
+  detail::write_result(info, it, ch, cv);
   return detail::download_ttm(out);
 }
 

@@ -91,9 +91,22 @@ int tta_tt_copy(tta_ctx* ctx, const tta_tt* t, tta_tt** out);
 /* round(const tt&, round_options{eps, max_rank, method, gauge})
  * (include/core/round/round.hpp:230-267) and round(const tt_matrix&, ...)
  * (round.hpp:329-370, via round_matrix, tt_matrix_round.hpp:27-71).
- * method must be TTA_METHOD_SVD (svd_streaming / svd_naive -> TTA_ERR_METHOD,
- * round.hpp:124-134; ALS / DMRG -> TTA_ERR_UNSUPPORTED). */
+ * method svd or als (svd_streaming / svd_naive -> TTA_ERR_METHOD,
+ * round.hpp:124-134; DMRG -> TTA_ERR_UNSUPPORTED). */
 int tta_round(tta_ctx* ctx, const tta_tt* a, double eps, int max_rank, int method, int gauge, tta_tt** out);
+/* round(a, round_options, round_result*) with every option
+ * (include/core/round/round.hpp:46-110, 230-420): method svd / als (dmrg ->
+ * TTA_ERR_UNSUPPORTED); gauge applies to svd; max_iters, tol and warm
+ * (warm_start_tt, or warm_start_tt_matrix when a is a TT-matrix) to als,
+ * whose cold start is round_right_canonical(a, 0, max_rank)
+ * (tt_als.hpp:226-300, tt_matrix_als_round.hpp:40-72).  seed is unused by
+ * round (as in the reference).  The round_result fields iters_run,
+ * final_resid, converged go to the three outputs (each may be null); the svd
+ * method reports {1, 0, 1}.  tta_round is this call with the reference's
+ * defaults max_iters = 20, tol = 1e-12, seed = 0, no warm start. */
+int tta_round_iterative(tta_ctx* ctx, const tta_tt* a, double eps, int max_rank, int method, int gauge,
+                        int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** out,
+                        int* iters_run, double* final_resid, int* converged);
 /* right_orthogonalize(tt&) (include/core/gauge/tt_orthogonalize.hpp:26-36) and
  * right_orthogonalize(tt_matrix&) (tt_matrix_orthogonalize.hpp:34-85): in place;
  * ranks may shrink to r_eff = min(n*r_right, r_left). */
@@ -109,11 +122,26 @@ int tta_matmat(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, tta_tt** C);
  * fused apply + round (streaming_apply.hpp:39-110); svd_naive -> TTA_ERR_METHOD. */
 int tta_matvec_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
                      tta_tt** y);
+/* matvec_round(A, x, round_options, round_result*) with every option
+ * (matvec_round.hpp:42-91): method als = matvec_als_round
+ * (tt_matvec_als_round.hpp:30-113): the product A x is formed and rounded by
+ * ALS from warm, from the SVD round at eps when max_rank <= 0, else from
+ * random(modes, max_rank, seed).  Outputs as tta_round_iterative. */
+int tta_matvec_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
+                               int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** y,
+                               int* iters_run, double* final_resid, int* converged);
 /* matmat_round (include/core/round/matmat_round.hpp:41-94): method svd =
  * streaming (streaming_apply.hpp:115-202); svd_naive = round_matrix(matmat);
  * svd_streaming -> TTA_ERR_METHOD. */
 int tta_matmat_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
                      tta_tt** C);
+/* matmat_round(A, B, round_options, round_result*) with every option
+ * (matmat_round.hpp:41-94): method als = matmat_als_round
+ * (tt_matmat_als_round.hpp:39-141) on the packed product, starts as for
+ * tta_matvec_round_iterative (warm is a TT-matrix). */
+int tta_matmat_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
+                               int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** C,
+                               int* iters_run, double* final_resid, int* converged);
 /* inner(a, b) (include/core/algebra/tt_inner.hpp:33-48); frob_inner for
  * TT-matrices (tt_matrix_inner.hpp:31-36). */
 int tta_inner(tta_ctx* ctx, const tta_tt* a, const tta_tt* b, double* out);

@@ -965,6 +965,200 @@ using ora::tt;
 
 namespace ora
 {
+// contract_inner_step_right (include/core/detail/contract.hpp:90-139):
+// F_out (ca.r x cb.r) = (left_unfold(ca) F as (ca.r x n F.cols))
+// right_unfold(cb)^T.
+static mat env_right(const core& ca, const core& cb, const mat& F)
+{
+  const int ral = ca.r, n = ca.n, rar = ca.rp, rbl = cb.r, rbr = cb.rp;
+  if (F.rows != rar || F.cols != rbr || ca.n != cb.n) die("env_right: shape mismatch");
+  std::vector<double> T1((size_t)ral * n * rbr, 0.0);
+  gemm(ral * n, rbr, rar, ca.data(), rar, F.v.data(), rbr, T1.data(), rbr);
+  mat out(ral, rbl);
+  for (int a = 0; a < ral; ++a)
+    for (int b = 0; b < rbl; ++b)
+    {
+      double s = 0.0;
+      for (int t = 0; t < n * rbr; ++t) s += T1[(size_t)a * n * rbr + t] * cb.v[(size_t)b * n * rbr + t];
+      out(a, b) = s;
+    }
+  return out;
+}
+
+// contract_inner_step (contract.hpp:141-205): E' (ca.rp x cb.rp) from
+// E (ca.r x cb.r): tmp = E^T right_unfold(ca) as (cb.r n x ca.rp), then
+// tmp^T left_unfold(cb).
+static mat env_left(const mat& E, const core& ca, const core& cb)
+{
+  const int ra = ca.r, n = ca.n, rap = ca.rp, rb = cb.r, rbp = cb.rp;
+  if (E.rows != ra || E.cols != rb || ca.n != cb.n) die("env_left: shape mismatch");
+  std::vector<double> tmp((size_t)rb * n * rap, 0.0);
+  for (int b = 0; b < rb; ++b)
+    for (int a = 0; a < ra; ++a)
+    {
+      const double e = E(a, b);
+      for (int t = 0; t < n * rap; ++t) tmp[(size_t)b * n * rap + t] += e * ca.v[(size_t)a * n * rap + t];
+    }
+  mat out(rap, rbp);
+  for (int bj = 0; bj < rb * n; ++bj)
+    for (int p = 0; p < rap; ++p)
+    {
+      const double x = tmp[(size_t)bj * rap + p];
+      for (int q = 0; q < rbp; ++q) out(p, q) += x * cb.v[(size_t)bj * rbp + q];
+    }
+  return out;
+}
+
+// als_local_update (include/core/detail/als_local_solve.hpp:24-52):
+// left_unfold(X_k) = (L right_unfold(Y_k) as (L.rows n x Y.rp)) R.
+static core als_local(const mat& L, const core& Y, const mat& R)
+{
+  const int rxl = L.rows, n = Y.n, ryr = Y.rp, rxr = R.cols;
+  if (L.cols != Y.r || R.rows != ryr) die("als_local: shape mismatch");
+  std::vector<double> T1((size_t)rxl * n * ryr, 0.0);
+  gemm(rxl, n * ryr, Y.r, L.v.data(), L.cols, Y.data(), n * ryr, T1.data(), n * ryr);
+  core X(rxl, n, rxr);
+  gemm(rxl * n, rxr, ryr, T1.data(), ryr, R.v.data(), rxr, X.data(), rxr);
+  return X;
+}
+
+// detail::als_engine (include/core/round/tt_als.hpp:54-206): X right-
+// orthogonalized unless already right-canonical; right environments; per
+// iteration one L->R sweep (local update, left QR, left environment) and one
+// R->L sweep (local update except at d-1, right QR, right environment); the
+// change is the largest relative change of the singular values of every
+// left_unfold(X_k), k < d-1 (1.0 with no baseline or a shape change); stop
+// when change <= tol.
+static tt als_engine(const tt& Y, tt X, int max_iters, double tol, bool right_canonical, int* iters_out,
+                     double* change_out, int* conv_out)
+{
+  const int d = Y.d();
+  if (!right_canonical) right_orthogonalize(X);
+  std::vector<mat> R(d + 1), L(d + 1);
+  R[d] = mat(1, 1);
+  R[d](0, 0) = 1.0;
+  for (int k = d - 1; k >= 1; --k) R[k] = env_right(Y.c[k], X.c[k], R[k + 1]);
+  L[0] = mat(1, 1);
+  L[0](0, 0) = 1.0;
+  std::vector<std::vector<double>> s_prev(d - 1);
+  bool have_prev = false, converged = false;
+  double change = 0.0;
+  int iters = 0;
+  for (int it = 0; it < max_iters; ++it)
+  {
+    for (int k = 0; k < d; ++k)
+    {
+      X.c[k] = als_local(L[k], Y.c[k], R[k + 1]);
+      if (k < d - 1)
+      {
+        left_qr_step(X.c[k], X.c[k + 1]);
+        L[k + 1] = env_left(L[k], X.c[k], Y.c[k]);
+      }
+    }
+    for (int k = d - 1; k >= 0; --k)
+    {
+      if (k < d - 1) X.c[k] = als_local(L[k], Y.c[k], R[k + 1]);
+      if (k > 0)
+      {
+        right_qr_step(X.c[k], X.c[k - 1]);
+        R[k] = env_right(Y.c[k], X.c[k], R[k + 1]);
+      }
+    }
+    ++iters;
+    std::vector<std::vector<double>> s_now(d - 1);
+    for (int k = 0; k < d - 1; ++k)
+    {
+      mat U, Vt;
+      svd_thin(X.c[k].data(), X.c[k].r * X.c[k].n, X.c[k].rp, U, s_now[k], Vt);
+    }
+    if (have_prev)
+    {
+      double mx = 0.0;
+      for (int k = 0; k < d - 1; ++k)
+      {
+        double nn = 0.0, df = 0.0;
+        for (double v : s_now[k]) nn += v * v;
+        nn = std::sqrt(nn);
+        if (s_prev[k].size() == s_now[k].size())
+        {
+          for (size_t i = 0; i < s_now[k].size(); ++i) df += (s_now[k][i] - s_prev[k][i]) * (s_now[k][i] - s_prev[k][i]);
+          mx = std::max(mx, std::sqrt(df) / (nn + 1.0e-300));
+        }
+        else
+          mx = 1.0;
+      }
+      change = mx;
+    }
+    else
+      change = 1.0;
+    s_prev = std::move(s_now);
+    have_prev = true;
+    if (change <= tol)
+    {
+      converged = true;
+      break;
+    }
+  }
+  *iters_out = iters;
+  *change_out = change;
+  *conv_out = converged ? 1 : 0;
+  return X;
+}
+
+// detail::als_round (tt_als.hpp:208-290): d <= 1 -> SVD round with
+// {0, 0, true}; without max_rank the target is the largest rank of the SVD
+// round at eps, whose result is also the warm start unless one is given; a
+// warm start must match d and the modes (else it is ignored) and stay within
+// max_rank; without one the start is round_right_canonical(Y, 0, max_rank).
+// cold_random selects the start of the apply-then-ALS paths instead:
+// matvec_als_round (tt_matvec_als_round.hpp:30-113) and matmat_als_round
+// (tt_matmat_als_round.hpp:39-141) start from random(shape, max_rank, seed)
+// and right-orthogonalize every start, the SVD reference included.  For a
+// TT-matrix warm start the per-site (m, n) must match
+// (warm_start.hpp:44-60); Y and warm carry m in .m when they are matrices.
+static tt als_round(const tt& Y, double eps, int max_rank, int max_iters, double tol, const tt* warm, bool cold_random,
+                    uint64_t seed, int* iters, double* change, int* conv)
+{
+  const int d = Y.d();
+  if (std::isnan(eps)) die("als_round: eps is NaN");
+  if (d <= 1)
+  {
+    *iters = 0;
+    *change = 0.0;
+    *conv = 1;
+    return round_tt(Y, eps, max_rank);
+  }
+  tt svd_ref;
+  if (max_rank <= 0)
+  {
+    if (eps <= 0.0) die("als_round: max_rank <= 0 and eps <= 0");
+    svd_ref = round_tt(Y, eps, 0);
+    for (auto& c : svd_ref.c) max_rank = std::max(max_rank, std::max(c.r, c.rp));
+    if (!warm) warm = &svd_ref;
+  }
+  bool use_warm = warm && warm->d() == d;
+  if (use_warm)
+    for (int k = 0; k < d; ++k) use_warm = use_warm && warm->c[k].n == Y.c[k].n;
+  if (use_warm && !Y.m.empty() && warm != &svd_ref) use_warm = warm->m == Y.m;
+  if (use_warm)
+    for (auto& c : warm->c)
+      if (c.r > max_rank || c.rp > max_rank) die("als_round: warm start rank above max_rank");
+  tt X0;
+  if (use_warm)
+    X0 = *warm;
+  else if (cold_random)
+  {
+    std::vector<int> shape(d);
+    for (int k = 0; k < d; ++k) shape[k] = Y.c[k].n;
+    X0 = random_impl(shape, max_rank, seed);
+  }
+  else
+    X0 = round_right_canonical(Y, 0.0, max_rank);
+  X0.m = Y.m;
+  X0.nn = Y.nn;
+  return als_engine(Y, std::move(X0), max_iters, tol, !use_warm && !cold_random, iters, change, conv);
+}
+
 // soft_threshold (include/core/round/tt_soft_threshold.hpp:42-176): d <= 1
 // or tau <= 0 returns a copy; otherwise right-orthogonalize, then for
 // k = d-1..1 the thin SVD of left_unfold(c_{k-1}) right_unfold(c_k), the
@@ -1301,6 +1495,12 @@ extern "C"
   int ora_truncate_eps_rank(const double* s, int n, double delta, int max_rank)
   {
     return ora::truncate_eps_rank(s, n, delta, max_rank);
+  }
+  void* ora_round_als(void* a, double eps, int max_rank, int max_iters, double tol, void* warm, int cold_random,
+                      unsigned long long seed, int* info_iters, double* info_change, int* info_conv)
+  {
+    return new tt(ora::als_round(*static_cast<tt*>(a), eps, max_rank, max_iters, tol, static_cast<tt*>(warm),
+                                 cold_random != 0, seed, info_iters, info_change, info_conv));
   }
   void* ora_soft_threshold(void* a, double tau) { return new tt(ora::soft_threshold(*static_cast<tt*>(a), tau)); }
   void* ora_from_dense(const double* dense, int d, const int* shape, double eps, int max_rank)

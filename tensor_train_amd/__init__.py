@@ -19,7 +19,8 @@ from ._capi import TTAError
 
 __all__ = [
     "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
-    "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round", "inner",
+    "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round",
+    "matvec_round_ex", "matmat_round_ex", "inner",
     "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
     "zeros", "ones", "random_host", "from_dense", "matrix_from_dense", "soft_threshold",
 ]
@@ -105,8 +106,10 @@ class RoundOptions:
 
     Same fields and defaults as ``round_options`` (round.hpp:55-105):
     eps=0.0 (no truncation), max_rank=0 (no cap), method="svd",
-    gauge="none".  ALS / DMRG knobs are accepted but those engines are not
-    part of this library.
+    gauge="none", max_iters=20, tol=1e-12, seed=0, warm_start=None (a
+    TensorTrain, or a TensorTrainMatrix for matrix rounds: warm_start_tt /
+    warm_start_tt_matrix).  method="als" runs the ALS engine; "dmrg" is
+    rejected as unsupported.
     """
 
     def __init__(self, eps=0.0, max_rank=0, method="svd", gauge="none", max_iters=20, tol=1e-12, seed=0,
@@ -141,8 +144,9 @@ class RoundOptions:
 
 
 class RoundResult:
-    """Diagnostics of the svd path: {iters_run=1, final_resid=0, converged=True}
-    (round.hpp:114-119, :196-203)."""
+    """round_result (round.hpp:112-119): iters_run, final_resid (the ALS
+    final singular-value change), converged; the svd path reports {1, 0,
+    True}."""
 
     def __init__(self, iters_run=1, final_resid=0.0, converged=True):
         self.iters_run = iters_run
@@ -360,15 +364,27 @@ def _out(ctx, fn, *args, cls=TensorTrain):
     return cls(h, ctx)
 
 
-def round(a, opts=None, **kwargs):  # noqa: A001 - the reference's name
-    """round(a, round_options) (include/core/round/round.hpp:230-425)."""
-    o = RoundOptions.merge(opts, **kwargs)
-    cls = type(a)
-    return _out(a._ctx, a._ctx.lib.tta_round, a._h, o.eps, o.max_rank, o.method, o.gauge, cls=cls)
+def _iterative(ctx, fn, args, o, cls):
+    h = ctypes.c_void_p()
+    it, cv = ctypes.c_int(), ctypes.c_int()
+    ch = ctypes.c_double()
+    warm = o.warm_start._h if o.warm_start is not None else None
+    ctx.check(fn(ctx.h, *args, ctypes.c_int(o.max_iters), ctypes.c_double(o.tol), ctypes.c_ulonglong(o.seed), warm,
+                 ctypes.byref(h), ctypes.byref(it), ctypes.byref(ch), ctypes.byref(cv)))
+    return cls(h, ctx), RoundResult(it.value, ch.value, bool(cv.value))
 
 
 def round_ex(a, opts=None, **kwargs):
-    return round(a, opts, **kwargs), RoundResult()
+    """round(a, round_options, round_result*) (include/core/round/round.hpp:230-425):
+    returns (rounded, RoundResult)."""
+    o = RoundOptions.merge(opts, **kwargs)
+    return _iterative(a._ctx, a._ctx.lib.tta_round_iterative,
+                      (a._h, ctypes.c_double(o.eps), o.max_rank, o.method, o.gauge), o, type(a))
+
+
+def round(a, opts=None, **kwargs):  # noqa: A001 - the reference's name
+    """round(a, round_options) (include/core/round/round.hpp:230-425)."""
+    return round_ex(a, opts, **kwargs)[0]
 
 
 def right_orthogonalize(a) -> None:
@@ -388,17 +404,28 @@ def matmat(A, B):
     return _out(A._ctx, A._ctx.lib.tta_matmat, A._h, B._h, cls=TensorTrainMatrix)
 
 
+def matvec_round_ex(A, x, opts=None, **kwargs):
+    """matvec_round(A, x, round_options, round_result*) (include/core/round/matvec_round.hpp:42-91)."""
+    o = RoundOptions.merge(opts, **kwargs)
+    return _iterative(A._ctx, A._ctx.lib.tta_matvec_round_iterative,
+                      (A._h, x._h, ctypes.c_double(o.eps), o.max_rank, o.method), o, TensorTrain)
+
+
 def matvec_round(A, x, opts=None, **kwargs):
     """matvec_round (include/core/round/matvec_round.hpp:42-91)."""
+    return matvec_round_ex(A, x, opts, **kwargs)[0]
+
+
+def matmat_round_ex(A, B, opts=None, **kwargs):
+    """matmat_round(A, B, round_options, round_result*) (include/core/round/matmat_round.hpp:41-94)."""
     o = RoundOptions.merge(opts, **kwargs)
-    return _out(A._ctx, A._ctx.lib.tta_matvec_round, A._h, x._h, o.eps, o.max_rank, o.method)
+    return _iterative(A._ctx, A._ctx.lib.tta_matmat_round_iterative,
+                      (A._h, B._h, ctypes.c_double(o.eps), o.max_rank, o.method), o, TensorTrainMatrix)
 
 
 def matmat_round(A, B, opts=None, **kwargs):
     """matmat_round (include/core/round/matmat_round.hpp:41-94)."""
-    o = RoundOptions.merge(opts, **kwargs)
-    return _out(A._ctx, A._ctx.lib.tta_matmat_round, A._h, B._h, o.eps, o.max_rank, o.method,
-                cls=TensorTrainMatrix)
+    return matmat_round_ex(A, B, opts, **kwargs)[0]
 
 
 def inner(a, b) -> float:

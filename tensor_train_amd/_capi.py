@@ -44,6 +44,15 @@ SIGNATURES = {
     "tta_matmat": (ctypes.c_int, [_vp, _vp, _vp, _vpp]),
     "tta_matvec_round": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vpp]),
     "tta_matmat_round": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, _vpp]),
+    "tta_round_iterative": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _vp, _vpp, _i32p,
+                                           _f64p, _i32p]),
+    "tta_matvec_round_iterative": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _vp, _vpp,
+                                                  _i32p, _f64p, _i32p]),
+    "tta_matmat_round_iterative": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                                  ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong, _vp, _vpp,
+                                                  _i32p, _f64p, _i32p]),
     "tta_inner": (ctypes.c_int, [_vp, _vp, _vp, _f64p]),
     "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
     "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),

@@ -254,9 +254,11 @@ double frob_norm_dev(tta_ctx* c, const double* x, size_t n, double eps, double i
 // ---------------------------------------------------------------------------
 // right_qr_step (include/core/detail/orthogonalize.hpp:63-100): QR of the
 // column-major (n*rp x r) view of core k (its own bytes, factored in place),
-// thin Q -> new core (r_eff, n, rp), L = R^T absorbed into core k-1.
+// thin Q -> new core (r_eff, n, rp), L = R^T absorbed into core k-1.  With
+// absorb false core k-1 is left as it was (its right rank then no longer
+// matches ranks[k]): for callers that overwrite it next.
 // ---------------------------------------------------------------------------
-void right_qr_step(tta_ctx* c, tta_tt* t, int k)
+void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
 {
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int m = n * rp;
@@ -265,6 +267,12 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k)
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
   qr_factor(c, A, m, m, r, tau, Qn, m);
+  if (!absorb)
+  {
+    replace_core(c, t, k, Qn);
+    t->ranks[k] = r_eff;
+    return;
+  }
   double* L = ctx_ws(c, WS_X, (size_t)r * r_eff);
   launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream);
   const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
@@ -276,8 +284,8 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k)
 }
 
 // left_qr_step (orthogonalize.hpp:35-60): QR of left_unfold(core k), R
-// absorbed into core k+1.
-void left_qr_step(tta_ctx* c, tta_tt* t, int k)
+// absorbed into core k+1 (skipped with absorb false, as above).
+void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
 {
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int m = r * n;
@@ -289,6 +297,12 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k)
   qr_factor(c, A, m, m, rp, tau, Q, m);
   double* Cn = alloc_core(c, (size_t)m * kk);
   launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
+  if (!absorb)
+  {
+    replace_core(c, t, k, Cn);
+    t->ranks[k + 1] = kk;
+    return;
+  }
   double* R = ctx_ws(c, WS_X, (size_t)kk * rp);
   launch_extract_upper(A, m, kk, rp, R, rp, 1, c->stream);
   const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
@@ -947,6 +961,373 @@ double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
   return read_double(c, E0);
 }
 
+// ---------------------------------------------------------------------------
+// ALS rounding (include/core/round/tt_als.hpp:54-290; the apply-then-ALS
+// paths tt_matvec_als_round.hpp:30-113 and tt_matmat_als_round.hpp:39-141).
+// Environments are small row-major pool blocks: L_k (X.r_k x Y.r_k) and
+// R_k (Y.r_k x X.r_k).  The reference absorbs the triangular factor of every
+// QR step into the neighbouring core, which the next local update overwrites
+// before anything reads it; here the QR steps keep only Q.
+// ---------------------------------------------------------------------------
+struct iter_info
+{
+  int iters = 0;
+  double change = 0.0;
+  int converged = 0;
+};
+
+struct env_chain
+{
+  tta_ctx* c;
+  std::vector<double*> L, R;
+  double* one = nullptr;  // the 1x1 boundary environment [[1]]
+  env_chain(tta_ctx* cc, int d) : c(cc), L(d + 1, nullptr), R(d + 1, nullptr)
+  {
+    one = alloc_core(c, 1);
+    upload_async(c, one, &kOne, sizeof(double));
+    L[0] = one;
+    R[d] = one;
+  }
+  ~env_chain()
+  {
+    for (double* p : L)
+      if (p && p != one) c->mem.put(p);
+    for (double* p : R)
+      if (p && p != one) c->mem.put(p);
+    if (one) c->mem.put(one);
+  }
+  static void put(tta_ctx* c, std::vector<double*>& E, int k, double* p, double* one)
+  {
+    if (E[k] && E[k] != one) c->mem.put(E[k]);
+    E[k] = p;
+  }
+  void set_L(int k, double* p) { put(c, L, k, p, one); }
+  void set_R(int k, double* p) { put(c, R, k, p, one); }
+  static constexpr double kOne = 1.0;
+};
+
+// als_local_update (include/core/detail/als_local_solve.hpp:24-52):
+// left_unfold(X_k) = (L right_unfold(Y_k) as (rx n x Y.r_{k+1})) R.
+double* als_local_dev(tta_ctx* c, const double* L, int rx, const tta_tt* Y, int k, const double* R, int rxp)
+{
+  const int ry = Y->ranks[k], n = Y->modes[k], ryp = Y->ranks[k + 1];
+  double* T1 = ctx_ws(c, WS_M1, (size_t)rx * n * ryp);
+  gemm_rm(c, rx, n * ryp, ry, L, ry, Y->cores[k], (long)n * ryp, T1, (long)n * ryp);
+  double* Xn = alloc_core(c, (size_t)rx * n * rxp);
+  gemm_rm(c, rx * n, rxp, ryp, T1, ryp, R, rxp, Xn, rxp);
+  return Xn;
+}
+
+// contract_inner_step (contract.hpp:141-205) with ca = X_k, cb = Y_k:
+// L_{k+1} = (E^T right_unfold(X_k))_as^T left_unfold(Y_k).
+double* env_left_dev(tta_ctx* c, const double* E, const tta_tt* X, int k, const tta_tt* Y)
+{
+  const int rx = X->ranks[k], n = X->modes[k], rxp = X->ranks[k + 1];
+  const int ry = Y->ranks[k], ryp = Y->ranks[k + 1];
+  double* T = ctx_ws(c, WS_M1, (size_t)ry * n * rxp);
+  gemm_desc g;
+  g.M = ry;
+  g.N = n * rxp;
+  g.K = rx;
+  g.A = E;
+  g.sar = 1;
+  g.sac = ry;
+  g.B = X->cores[k];
+  g.sbr = (long)n * rxp;
+  g.sbc = 1;
+  g.C = T;
+  g.scr = (long)n * rxp;
+  g.scc = 1;
+  run_gemm(c, g);
+  double* out = alloc_core(c, (size_t)rxp * ryp);
+  gemm_desc h;
+  h.M = rxp;
+  h.N = ryp;
+  h.K = ry * n;
+  h.A = T;
+  h.sar = 1;
+  h.sac = rxp;
+  h.B = Y->cores[k];
+  h.sbr = ryp;
+  h.sbc = 1;
+  h.C = out;
+  h.scr = ryp;
+  h.scc = 1;
+  run_gemm(c, h);
+  return out;
+}
+
+// contract_inner_step_right (contract.hpp:90-139) with ca = Y_k, cb = X_k:
+// R_k = (left_unfold(Y_k) F as (Y.r_k x n X.r_{k+1})) right_unfold(X_k)^T.
+double* env_right_dev(tta_ctx* c, const tta_tt* Y, int k, const tta_tt* X, const double* F)
+{
+  const int ry = Y->ranks[k], n = Y->modes[k], ryp = Y->ranks[k + 1];
+  const int rx = X->ranks[k], rxp = X->ranks[k + 1];
+  double* T1 = ctx_ws(c, WS_M1, (size_t)ry * n * rxp);
+  gemm_rm(c, ry * n, rxp, ryp, Y->cores[k], ryp, F, rxp, T1, rxp);
+  double* out = alloc_core(c, (size_t)ry * rx);
+  gemm_desc g;
+  g.M = ry;
+  g.N = rx;
+  g.K = n * rxp;
+  g.A = T1;
+  g.sar = (long)n * rxp;
+  g.sac = 1;
+  g.B = X->cores[k];
+  g.sbr = 1;
+  g.sbc = (long)n * rxp;
+  g.C = out;
+  g.scr = rx;
+  g.scc = 1;
+  run_gemm(c, g);
+  return out;
+}
+
+// Singular values (descending) of the row-major rows x cols matrix M into
+// dst[0, min(rows, cols)): QR condensation, then one-sided Jacobi on R^T run
+// to convergence (delta 0, every value kept).
+int singular_values_dev(tta_ctx* c, const double* M, int rows, int cols, double* dst)
+{
+  const int q = std::min(rows, cols);
+  double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+  double* tau = ctx_ws(c, WS_F, (size_t)q);
+  double* X = ctx_ws(c, WS_X, (size_t)q * q);
+  if (rows >= cols)
+  {
+    launch_transpose(M, cols, rows, cols, A, rows, c->stream);
+    qr_factor(c, A, rows, rows, cols, tau, nullptr, 0);
+    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+  }
+  else
+  {
+    hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    qr_factor(c, A, cols, cols, rows, tau, nullptr, 0);
+    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
+  }
+  double* J = ctx_ws(c, WS_J, (size_t)q * q);
+  double* sv = nullptr;
+  int* perm = nullptr;
+  square_svd(c, X, q, J, 0.0, 0, true, &sv, &perm);
+  hip_check(hipMemcpyAsync(dst, sv, (size_t)q * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+  return q;
+}
+
+// Largest relative change of the per-core singular values between two
+// sweeps (tt_als.hpp:160-205): ||s_now - s_prev|| / (||s_now|| + 1e-300) per
+// core, 1.0 for a core whose count changed.
+double sv_change(const std::vector<std::vector<double>>& now, const std::vector<std::vector<double>>& prev)
+{
+  double mx = 0.0;
+  for (size_t k = 0; k < now.size(); ++k)
+  {
+    double nn = 0.0;
+    for (double v : now[k]) nn += v * v;
+    nn = std::sqrt(nn);
+    if (now[k].size() == prev[k].size())
+    {
+      double df = 0.0;
+      for (size_t i = 0; i < now[k].size(); ++i) df += (now[k][i] - prev[k][i]) * (now[k][i] - prev[k][i]);
+      mx = std::max(mx, std::sqrt(df) / (nn + 1.0e-300));
+    }
+    else
+      mx = 1.0;
+  }
+  return mx;
+}
+
+// Singular values of left_unfold(X_k), k = 0 .. d-2, on the host.
+std::vector<std::vector<double>> core_singular_values(tta_ctx* c, const tta_tt* X)
+{
+  const int d = X->d();
+  std::vector<int> q(d - 1);
+  size_t total = 0;
+  for (int k = 0; k < d - 1; ++k)
+  {
+    q[k] = std::min(X->ranks[k] * X->modes[k], X->ranks[k + 1]);
+    total += (size_t)q[k];
+  }
+  double* sv = alloc_core(c, total);
+  std::vector<double> host(total);
+  try
+  {
+    size_t off = 0;
+    for (int k = 0; k < d - 1; ++k)
+    {
+      singular_values_dev(c, X->cores[k], X->ranks[k] * X->modes[k], X->ranks[k + 1], sv + off);
+      off += (size_t)q[k];
+    }
+    hip_check(hipMemcpyAsync(host.data(), sv, total * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  catch (...)
+  {
+    c->mem.put(sv);
+    throw;
+  }
+  c->mem.put(sv);
+  std::vector<std::vector<double>> out(d - 1);
+  size_t off = 0;
+  for (int k = 0; k < d - 1; ++k)
+  {
+    out[k].assign(host.begin() + off, host.begin() + off + q[k]);
+    off += (size_t)q[k];
+  }
+  return out;
+}
+
+// detail::als_engine (tt_als.hpp:54-224) on X in place.
+void als_engine_dev(tta_ctx* c, const tta_tt* Y, tta_tt* X, int max_iters, double tol, bool right_canonical,
+                    iter_info* info)
+{
+  const int d = Y->d();
+  if (!right_canonical) right_orthogonalize_impl(c, X);
+  env_chain E(c, d);
+  for (int k = d - 1; k >= 1; --k) E.set_R(k, env_right_dev(c, Y, k, X, E.R[k + 1]));
+  std::vector<std::vector<double>> s_prev;
+  bool have_prev = false, converged = false;
+  double change = 0.0;
+  int iters = 0;
+  for (int it = 0; it < max_iters; ++it)
+  {
+    for (int k = 0; k < d; ++k)
+    {
+      replace_core(c, X, k, als_local_dev(c, E.L[k], X->ranks[k], Y, k, E.R[k + 1], X->ranks[k + 1]));
+      if (k < d - 1)
+      {
+        left_qr_step(c, X, k, false);
+        E.set_L(k + 1, env_left_dev(c, E.L[k], X, k, Y));
+      }
+    }
+    for (int k = d - 1; k >= 0; --k)
+    {
+      if (k < d - 1)
+        replace_core(c, X, k, als_local_dev(c, E.L[k], X->ranks[k], Y, k, E.R[k + 1], X->ranks[k + 1]));
+      if (k > 0)
+      {
+        right_qr_step(c, X, k, false);
+        E.set_R(k, env_right_dev(c, Y, k, X, E.R[k + 1]));
+      }
+    }
+    ++iters;
+    std::vector<std::vector<double>> s_now = core_singular_values(c, X);
+    change = have_prev ? sv_change(s_now, s_prev) : 1.0;
+    s_prev = std::move(s_now);
+    have_prev = true;
+    if (change <= tol)
+    {
+      converged = true;
+      break;
+    }
+  }
+  info->iters = iters;
+  info->change = change;
+  info->converged = converged ? 1 : 0;
+}
+
+// A warm start is used when it has the core count and modes of Y, per-site
+// (m, n) for TT-matrices (warm_start.hpp:22-60).
+bool warm_compatible(const tta_tt* w, const tta_tt* Y)
+{
+  if (!w || w->d() != Y->d() || w->is_matrix() != Y->is_matrix()) return false;
+  for (int k = 0; k < Y->d(); ++k)
+    if (w->modes[k] != Y->modes[k]) return false;
+  if (Y->is_matrix())
+    for (int k = 0; k < Y->d(); ++k)
+      if (w->row_modes[k] != Y->row_modes[k]) return false;
+  return true;
+}
+
+// random(shape, max_rank, seed) (tt_factory.hpp:141-207) over Y's (packed)
+// modes, uploaded.
+tta_tt* random_like(tta_ctx* c, const tta_tt* Y, int max_rank, unsigned long long seed)
+{
+  const int d = Y->d();
+  std::vector<int> ranks(d + 1);
+  long long n = 0;
+  int st = tta_random_host(d, Y->modes.data(), max_rank, seed, ranks.data(), nullptr, &n);
+  if (st != TTA_OK) fail(st, "random: invalid shape or rank cap");
+  std::vector<double> host((size_t)n);
+  tta_random_host(d, Y->modes.data(), max_rank, seed, nullptr, host.data(), nullptr);
+  tta_tt* t = new_tt(c, Y->modes, ranks, false);
+  size_t off = 0;
+  for (int k = 0; k < d; ++k)
+  {
+    upload_async(c, t->cores[k], host.data() + off, core_numel(t, k) * 8);
+    off += core_numel(t, k);
+  }
+  return t;
+}
+
+// detail::als_round (tt_als.hpp:226-300), and with cold_random the start of
+// matvec_als_round / matmat_als_round (random(shape, max_rank, seed), every
+// start right-orthogonalized by the engine).
+tta_tt* als_round_dev(tta_ctx* c, const tta_tt* Y, double eps, int max_rank, int max_iters, double tol,
+                      const tta_tt* warm, bool cold_random, unsigned long long seed, iter_info* info, const char* what)
+{
+  if (std::isnan(eps)) fail(TTA_ERR_ARG, "%s: eps is NaN; must be a valid number", what);
+  const int d = Y->d();
+  if (d <= 1)
+  {
+    tta_tt* t = copy_tt(c, Y);
+    try
+    {
+      round_inplace(c, t, eps, max_rank, TTA_GAUGE_NONE);
+    }
+    catch (...)
+    {
+      free_tt(c, t);
+      throw;
+    }
+    *info = iter_info{0, 0.0, 1};
+    return t;
+  }
+  tta_tt* svd_ref = nullptr;
+  tta_tt* X = nullptr;
+  try
+  {
+    if (max_rank <= 0)
+    {
+      if (eps <= 0.0)
+        fail(TTA_ERR_ARG,
+             "%s: max_rank <= 0 and eps <= 0; ALS requires either max_rank > 0 or eps > 0 to determine "
+             "compression target",
+             what);
+      svd_ref = copy_tt(c, Y);
+      round_inplace(c, svd_ref, eps, 0, TTA_GAUGE_NONE);
+      max_rank = *std::max_element(svd_ref->ranks.begin(), svd_ref->ranks.end());
+      if (!warm) warm = svd_ref;
+    }
+    const bool use_warm = warm_compatible(warm, Y);
+    if (use_warm)
+      for (int r : warm->ranks)
+        if (r > max_rank) fail(TTA_ERR_ARG, "%s: warm_start max rank %d exceeds max_rank %d", what, r, max_rank);
+    if (use_warm)
+      X = copy_tt(c, warm);
+    else if (cold_random)
+      X = random_like(c, Y, max_rank, seed);
+    else
+    {
+      X = copy_tt(c, Y);
+      round_inplace(c, X, 0.0, max_rank, TTA_GAUGE_RIGHT_CANONICAL);
+    }
+    X->row_modes = Y->row_modes;
+    X->col_modes = Y->col_modes;
+    if (svd_ref)
+    {
+      free_tt(c, svd_ref);
+      svd_ref = nullptr;
+    }
+    als_engine_dev(c, Y, X, max_iters, tol, !use_warm && !cold_random, info);
+  }
+  catch (...)
+  {
+    free_tt(c, svd_ref);
+    free_tt(c, X);
+    throw;
+  }
+  return X;
+}
+
 void check_tt(const tta_tt* t, const char* what)
 {
   if (!t) fail(TTA_ERR_ARG, "%s: null tensor", what);
@@ -983,6 +1364,148 @@ void check_matmat(const tta_tt* A, const tta_tt* B, const char* what)
   for (int k = 0; k < A->d(); ++k)
     if (A->col_modes[k] != B->row_modes[k])
       fail(TTA_ERR_SHAPE, "%s: mode mismatch at core %d (%d vs %d)", what, k, A->col_modes[k], B->row_modes[k]);
+}
+
+// round(tt / tt_matrix, round_options, round_result*) (round.hpp:230-420)
+// for every method; iteration diagnostics in info (svd_result_default:
+// {1, 0, true}).
+tta_tt* round_dispatch(tta_ctx* c, const tta_tt* a, double eps, int max_rank, int method, int gauge, int max_iters,
+                       double tol, unsigned long long seed, const tta_tt* warm, iter_info* info)
+{
+  if (method == TTA_METHOD_SVD_STREAMING || method == TTA_METHOD_SVD_NAIVE)
+    fail(TTA_ERR_METHOD,
+         "round: round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round "
+         "respectively.");
+  if (method == TTA_METHOD_DMRG) fail(TTA_ERR_UNSUPPORTED, "round: the DMRG engine is not part of this library");
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_ALS) fail(TTA_ERR_METHOD, "round: unknown method %d", method);
+  if (std::isnan(eps)) fail(TTA_ERR_ARG, "round: eps is NaN");
+  if (method == TTA_METHOD_ALS)
+  {
+    if (max_iters < 0) fail(TTA_ERR_ARG, "round: max_iters is negative");
+    tta_tt* t = als_round_dev(c, a, eps, max_rank, max_iters, tol, warm, false, seed, info,
+                              a->is_matrix() ? "als_round_matrix" : "als_round");
+    t->matrix = a->matrix;
+    return t;
+  }
+  tta_tt* t = copy_tt(c, a);
+  t->matrix = a->matrix;
+  try
+  {
+    round_inplace(c, t, eps, max_rank, gauge);
+  }
+  catch (...)
+  {
+    free_tt(c, t);
+    throw;
+  }
+  *info = iter_info{1, 0.0, 1};
+  return t;
+}
+
+// matvec_round (matvec_round.hpp:42-91) for every method.
+tta_tt* matvec_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
+                              int max_iters, double tol, unsigned long long seed, const tta_tt* warm, iter_info* info)
+{
+  if (method == TTA_METHOD_SVD_NAIVE) fail(TTA_ERR_METHOD, "matvec_round: round_method::svd_naive is matmat-only");
+  if (method == TTA_METHOD_DMRG)
+    fail(TTA_ERR_UNSUPPORTED, "matvec_round: the DMRG engine is not part of this library");
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_STREAMING && method != TTA_METHOD_ALS)
+    fail(TTA_ERR_METHOD, "matvec_round: unknown round_method %d", method);
+  check_matvec(A, x, "matvec_round");
+  if (method == TTA_METHOD_SVD_STREAMING && A->d() > 1)
+  {
+    *info = iter_info{1, 0.0, 1};
+    return streaming_impl(c, A, x, false, eps, max_rank);
+  }
+  if (method == TTA_METHOD_ALS && std::isnan(eps))
+    fail(TTA_ERR_ARG, "matvec_als_round: eps is NaN; must be a valid number");
+  if (method == TTA_METHOD_ALS && max_iters < 0) fail(TTA_ERR_ARG, "matvec_round: max_iters is negative");
+  tta_tt* Y = matvec_impl(c, A, x);
+  if (method == TTA_METHOD_ALS)
+  {
+    tta_tt* r = nullptr;
+    try
+    {
+      r = als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matvec_als_round");
+    }
+    catch (...)
+    {
+      free_tt(c, Y);
+      throw;
+    }
+    free_tt(c, Y);
+    return r;
+  }
+  try
+  {
+    round_inplace(c, Y, eps, max_rank, TTA_GAUGE_NONE);
+  }
+  catch (...)
+  {
+    free_tt(c, Y);
+    throw;
+  }
+  *info = iter_info{1, 0.0, 1};
+  return Y;
+}
+
+// matmat_round (matmat_round.hpp:41-94) for every method.
+tta_tt* matmat_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
+                              int max_iters, double tol, unsigned long long seed, const tta_tt* warm, iter_info* info)
+{
+  if (method == TTA_METHOD_SVD_STREAMING)
+    fail(TTA_ERR_METHOD, "matmat_round: round_method::svd_streaming is matvec-only");
+  if (method == TTA_METHOD_DMRG)
+    fail(TTA_ERR_UNSUPPORTED, "matmat_round: the DMRG engine is not part of this library");
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_NAIVE && method != TTA_METHOD_ALS)
+    fail(TTA_ERR_METHOD, "matmat_round: unknown round_method %d", method);
+  check_matmat(A, B, "matmat_round");
+  tta_tt* r = nullptr;
+  if (method == TTA_METHOD_SVD && A->d() > 1)
+  {
+    r = streaming_impl(c, A, B, true, eps, max_rank);
+    r->matrix = true;
+    *info = iter_info{1, 0.0, 1};
+    return r;
+  }
+  if (method == TTA_METHOD_ALS && std::isnan(eps))
+    fail(TTA_ERR_ARG, "matmat_als_round: eps is NaN; must be a valid number");
+  if (method == TTA_METHOD_ALS && max_iters < 0) fail(TTA_ERR_ARG, "matmat_round: max_iters is negative");
+  tta_tt* Y = matmat_impl(c, A, B);
+  Y->matrix = true;
+  if (method == TTA_METHOD_ALS)
+  {
+    try
+    {
+      r = als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matmat_als_round");
+    }
+    catch (...)
+    {
+      free_tt(c, Y);
+      throw;
+    }
+    free_tt(c, Y);
+    r->matrix = true;
+    return r;
+  }
+  try
+  {
+    round_inplace(c, Y, eps, max_rank, TTA_GAUGE_NONE);
+  }
+  catch (...)
+  {
+    free_tt(c, Y);
+    throw;
+  }
+  *info = iter_info{1, 0.0, 1};
+  return Y;
+}
+
+void write_info(const iter_info& in, int* iters_run, double* final_resid, int* converged)
+{
+  if (iters_run) *iters_run = in.iters;
+  if (final_resid) *final_resid = in.change;
+  if (converged) *converged = in.converged;
 }
 
 void validate_ranks(int d, const int* modes, const int* ranks)
@@ -1194,21 +1717,20 @@ int tta_tt_copy(tta_ctx* c, const tta_tt* t, tta_tt** out)
 
 int tta_round(tta_ctx* c, const tta_tt* a, double eps, int max_rank, int method, int gauge, tta_tt** out)
 {
+  return tta_round_iterative(c, a, eps, max_rank, method, gauge, 20, 1.0e-12, 0, nullptr, out, nullptr, nullptr,
+                             nullptr);
+}
+
+int tta_round_iterative(tta_ctx* c, const tta_tt* a, double eps, int max_rank, int method, int gauge, int max_iters,
+                        double tol, unsigned long long seed, const tta_tt* warm, tta_tt** out, int* iters_run,
+                        double* final_resid, int* converged)
+{
   if (!c || !a || !out) return TTA_ERR_ARG;
   TTA_GUARD_BEGIN
-  if (method == TTA_METHOD_SVD_STREAMING || method == TTA_METHOD_SVD_NAIVE)
-    fail(TTA_ERR_METHOD,
-         "round: round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round "
-         "respectively.");
-  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
-    fail(TTA_ERR_UNSUPPORTED, "round: ALS / DMRG engines are not part of this library");
-  if (method != TTA_METHOD_SVD) fail(TTA_ERR_METHOD, "round: unknown method %d", method);
-  if (std::isnan(eps)) fail(TTA_ERR_ARG, "round: eps is NaN");
-  tta_tt* t = copy_tt(c, a);
-  t->matrix = a->matrix;
+  iter_info info;
+  tta_tt* t = round_dispatch(c, a, eps, max_rank, method, gauge, max_iters, tol, seed, warm, &info);
   try
   {
-    round_inplace(c, t, eps, max_rank, gauge);
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   }
   catch (...)
@@ -1216,6 +1738,7 @@ int tta_round(tta_ctx* c, const tta_tt* a, double eps, int max_rank, int method,
     free_tt(c, t);
     throw;
   }
+  write_info(info, iters_run, final_resid, converged);
   *out = t;
   TTA_GUARD_END(c)
 }
@@ -1262,69 +1785,57 @@ int tta_matmat(tta_ctx* c, const tta_tt* A, const tta_tt* B, tta_tt** C)
 
 int tta_matvec_round(tta_ctx* c, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method, tta_tt** y)
 {
+  return tta_matvec_round_iterative(c, A, x, eps, max_rank, method, 20, 1.0e-12, 0, nullptr, y, nullptr, nullptr,
+                                    nullptr);
+}
+
+int tta_matvec_round_iterative(tta_ctx* c, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
+                               int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** y,
+                               int* iters_run, double* final_resid, int* converged)
+{
   if (!c || !y) return TTA_ERR_ARG;
   TTA_GUARD_BEGIN
-  if (method == TTA_METHOD_SVD_NAIVE) fail(TTA_ERR_METHOD, "matvec_round: round_method::svd_naive is matmat-only");
-  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
-    fail(TTA_ERR_UNSUPPORTED, "matvec_round: ALS / DMRG engines are not part of this library");
-  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_STREAMING)
-    fail(TTA_ERR_METHOD, "matvec_round: unknown round_method %d", method);
-  check_matvec(A, x, "matvec_round");
-  tta_tt* r = nullptr;
-  if (method == TTA_METHOD_SVD_STREAMING && A->d() > 1)
+  iter_info info;
+  tta_tt* r = matvec_round_dispatch(c, A, x, eps, max_rank, method, max_iters, tol, seed, warm, &info);
+  try
   {
-    if (A->d() == 0) fail(TTA_ERR_ARG, "build_streaming_matvec: d=0 not supported");
-    r = streaming_impl(c, A, x, false, eps, max_rank);
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   }
-  else
+  catch (...)
   {
-    r = matvec_impl(c, A, x);
-    try
-    {
-      round_inplace(c, r, eps, max_rank, TTA_GAUGE_NONE);
-    }
-    catch (...)
-    {
-      free_tt(c, r);
-      throw;
-    }
+    free_tt(c, r);
+    throw;
   }
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  write_info(info, iters_run, final_resid, converged);
   *y = r;
   TTA_GUARD_END(c)
 }
 
 int tta_matmat_round(tta_ctx* c, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method, tta_tt** C)
 {
+  return tta_matmat_round_iterative(c, A, B, eps, max_rank, method, 20, 1.0e-12, 0, nullptr, C, nullptr, nullptr,
+                                    nullptr);
+}
+
+int tta_matmat_round_iterative(tta_ctx* c, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
+                               int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** C,
+                               int* iters_run, double* final_resid, int* converged)
+{
   if (!c || !C) return TTA_ERR_ARG;
   TTA_GUARD_BEGIN
-  if (method == TTA_METHOD_SVD_STREAMING)
-    fail(TTA_ERR_METHOD, "matmat_round: round_method::svd_streaming is matvec-only");
-  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
-    fail(TTA_ERR_UNSUPPORTED, "matmat_round: ALS / DMRG engines are not part of this library");
-  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_NAIVE)
-    fail(TTA_ERR_METHOD, "matmat_round: unknown round_method %d", method);
-  check_matmat(A, B, "matmat_round");
-  tta_tt* r = nullptr;
-  if (method == TTA_METHOD_SVD && A->d() > 1)
-  {
-    r = streaming_impl(c, A, B, true, eps, max_rank);
-  }
-  else
-  {
-    r = matmat_impl(c, A, B);
-    try
-    {
-      round_inplace(c, r, eps, max_rank, TTA_GAUGE_NONE);
-    }
-    catch (...)
-    {
-      free_tt(c, r);
-      throw;
-    }
-  }
+  iter_info info;
+  tta_tt* r = matmat_round_dispatch(c, A, B, eps, max_rank, method, max_iters, tol, seed, warm, &info);
   r->matrix = true;
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  try
+  {
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  }
+  catch (...)
+  {
+    free_tt(c, r);
+    throw;
+  }
+  write_info(info, iters_run, final_resid, converged);
   *C = r;
   TTA_GUARD_END(c)
 }

@@ -183,6 +183,82 @@ int main()
     CHECK(t.ranks() == std::vector<int>({1, 1, 1, 1}), "from_dense: rank-1 outer product, ranks 1");
     CHECK(max_abs_diff(t.to_dense(), T) <= 1e-12 * frob(T), "from_dense: reconstruction");
   }
+  // test_tt_round.cpp:117-226: ALS on a zero-padded exact TT (cold, warm,
+  // against svd, warm with a perturbation).
+  {
+    std::mt19937_64 rng(13);
+    std::normal_distribution<double> N(0.0, 1.0);
+    std::vector<double> T(4 * 5 * 6 * 4);
+    for (auto& v : T) v = N(rng);
+    auto A = tt::from_dense(T.data(), {4, 5, 6, 4}, 1e-12);
+    const int cap = A.max_rank();
+    std::vector<tt::tt_core> cs;
+    for (int k = 0; k < A.d(); ++k)
+    {
+      const auto& s = A.core(k);
+      const int rl = s.r_left() + (k > 0 ? 5 : 0), rr = s.r_right() + (k < A.d() - 1 ? 5 : 0);
+      tt::tt_core d(rl, s.n_phys(), rr);
+      for (int a = 0; a < rl; ++a)
+        for (int i = 0; i < s.n_phys(); ++i)
+          for (int b = 0; b < rr; ++b) d(a, i, b) = (a < s.r_left() && b < s.r_right()) ? s(a, i, b) : 0.0;
+      cs.push_back(std::move(d));
+    }
+    tt::tt B(std::move(cs));
+    const auto dA = A.to_dense();
+    auto rel_to = [&](const std::vector<double>& x, const std::vector<double>& ref)
+    {
+      std::vector<double> df(ref.size());
+      for (size_t i = 0; i < ref.size(); ++i) df[i] = x[i] - ref[i];
+      return frob(df) / frob(ref);
+    };
+    tt::round_options o;
+    o.method = tt::round_method::als;
+    o.max_rank = cap;
+    o.max_iters = 3;
+    o.tol = 1e-12;
+    tt::round_result info;
+    auto C = tt::round(B, o, &info);
+    CHECK(rel_to(C.to_dense(), dA) <= 1e-10 && C.max_rank() <= cap && info.iters_run <= 3,
+          "als cold: recovers the padded TT within the cap");
+    o.warm_start_tt = &A;
+    auto W = tt::round(B, o, &info);
+    CHECK(rel_to(W.to_dense(), dA) <= 1e-10, "als warm: recovers the padded TT");
+    o.warm_start_tt = nullptr;
+    o.tol = 1e-13;
+    auto Cs = tt::round(B, opts_svd(1e-10, cap));
+    auto Ca = tt::round(B, o);
+    CHECK(rel_to(Ca.to_dense(), dA) <= 10.0 * std::fmax(rel_to(Cs.to_dense(), dA), 1e-13), "als vs svd");
+    auto Bp = tt::axpby(1.0, A, 1.0e-3, tt::random({4, 5, 6, 4}, 4, 99));
+    o.tol = 1e-12;
+    o.warm_start_tt = &A;
+    auto P = tt::round(Bp, o, &info);
+    CHECK(info.iters_run <= 5 && rel_to(P.to_dense(), Bp.to_dense()) <= 1e-2, "als warm-pert");
+  }
+  // test_tt_matvec_round.cpp / test_tt_matrix_round.cpp: ALS apply-round
+  // recovers the product at its exact rank from a random cold start.
+  {
+    auto A = tt::random({2, 3, 2, 3}, {3, 2, 2, 2}, 2, 7);
+    auto x = tt::random({3, 2, 2, 2}, 2, 8);
+    auto y = tt::matvec(A, x);
+    tt::round_options o;
+    o.method = tt::round_method::als;
+    o.max_rank = tt::round(y, opts_svd(1e-13)).max_rank();
+    o.seed = 0x3030;
+    tt::round_result info;
+    auto ya = tt::matvec_round(A, x, o, &info);
+    const auto dy = y.to_dense(), da = ya.to_dense();
+    std::vector<double> df(dy.size());
+    for (size_t i = 0; i < dy.size(); ++i) df[i] = da[i] - dy[i];
+    CHECK(frob(df) <= 1e-10 * frob(dy) && info.converged, "matvec_round(als) recovers A x");
+    auto B = tt::random({3, 2, 2, 2}, {2, 2, 3, 2}, 2, 9);
+    auto Cn = tt::matmat(A, B);
+    o.max_rank = tt::round(Cn, opts_svd(1e-13)).max_rank();
+    auto Ca = tt::matmat_round(A, B, o, &info);
+    const auto dc = Cn.to_dense(), dca = Ca.to_dense();
+    df.assign(dc.size(), 0.0);
+    for (size_t i = 0; i < dc.size(); ++i) df[i] = dca[i] - dc[i];
+    CHECK(frob(df) <= 1e-10 * frob(dc) && info.converged, "matmat_round(als) recovers A B");
+  }
   std::printf("%d failure(s)\n", failures);
   return failures;
 }

@@ -16,6 +16,15 @@ def test_dropin_compiles():
     assert os.path.exists(BIN)
 
 
+def test_host_value_types():
+    # Host-only parts of the header (to_dense of tt_matrix): no device call.
+    host = os.path.join(CPP, "_build", "test_host_types")
+    if not os.path.exists(host):
+        subprocess.run(["make", "-s", "-C", CPP], check=True)
+    r = subprocess.run([host], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 @pytest.mark.gpu
 def test_dropin_runs_on_gpu():
     if not os.path.exists(BIN):
