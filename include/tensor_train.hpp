@@ -21,8 +21,8 @@
 // Each call copies its host operands to the GPU, runs the device path and
 // copies the result back; callers that want to keep tensors resident use the
 // C ABI (tt_amd.h) directly.  One device context per thread (device 0, or
-// TT_AMD_DEVICE).  round_method::als runs the ALS engine on the device
-// (round/tt_als.hpp); round_method::dmrg is not provided (abort).
+// TT_AMD_DEVICE).  round_method::als / dmrg run the ALS and two-site DMRG
+// engines on the device (round/tt_als.hpp, round/tt_dmrg.hpp).
 #pragma once
 
 #include <array>

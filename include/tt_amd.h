@@ -91,16 +91,18 @@ int tta_tt_copy(tta_ctx* ctx, const tta_tt* t, tta_tt** out);
 /* round(const tt&, round_options{eps, max_rank, method, gauge})
  * (include/core/round/round.hpp:230-267) and round(const tt_matrix&, ...)
  * (round.hpp:329-370, via round_matrix, tt_matrix_round.hpp:27-71).
- * method svd or als (svd_streaming / svd_naive -> TTA_ERR_METHOD,
- * round.hpp:124-134; DMRG -> TTA_ERR_UNSUPPORTED). */
+ * method svd, als or dmrg (svd_streaming / svd_naive -> TTA_ERR_METHOD,
+ * round.hpp:124-134). */
 int tta_round(tta_ctx* ctx, const tta_tt* a, double eps, int max_rank, int method, int gauge, tta_tt** out);
 /* round(a, round_options, round_result*) with every option
- * (include/core/round/round.hpp:46-110, 230-420): method svd / als (dmrg ->
- * TTA_ERR_UNSUPPORTED); gauge applies to svd; max_iters, tol and warm
- * (warm_start_tt, or warm_start_tt_matrix when a is a TT-matrix) to als,
- * whose cold start is round_right_canonical(a, 0, max_rank)
- * (tt_als.hpp:226-300, tt_matrix_als_round.hpp:40-72).  seed is unused by
- * round (as in the reference).  The round_result fields iters_run,
+ * (include/core/round/round.hpp:46-110, 230-420): method svd / als / dmrg;
+ * gauge applies to svd; max_iters, tol and warm (warm_start_tt, or
+ * warm_start_tt_matrix when a is a TT-matrix) to als and dmrg.  als starts
+ * cold from round_right_canonical(a, 0, max_rank) (tt_als.hpp:226-300,
+ * tt_matrix_als_round.hpp:40-72); dmrg needs eps > 0 (else TTA_ERR_ARG,
+ * round.hpp:155-176) and starts cold from round_right_canonical(a, eps,
+ * max_rank) (tt_dmrg.hpp:235-310, tt_matrix_dmrg_round.hpp:30-55).  seed is
+ * unused by round (as in the reference).  The round_result fields iters_run,
  * final_resid, converged go to the three outputs (each may be null); the svd
  * method reports {1, 0, 1}.  tta_round is this call with the reference's
  * defaults max_iters = 20, tol = 1e-12, seed = 0, no warm start. */
@@ -126,7 +128,10 @@ int tta_matvec_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, double eps,
  * (matvec_round.hpp:42-91): method als = matvec_als_round
  * (tt_matvec_als_round.hpp:30-113): the product A x is formed and rounded by
  * ALS from warm, from the SVD round at eps when max_rank <= 0, else from
- * random(modes, max_rank, seed).  Outputs as tta_round_iterative. */
+ * random(modes, max_rank, seed); method dmrg = matvec_dmrg_round
+ * (tt_matvec_dmrg_round.hpp:29-105): from warm, else random(modes, max_rank,
+ * seed) when max_rank > 0, else round_right_canonical(A x, eps, 0).
+ * Outputs as tta_round_iterative. */
 int tta_matvec_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* x, double eps, int max_rank, int method,
                                int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** y,
                                int* iters_run, double* final_resid, int* converged);
@@ -137,7 +142,8 @@ int tta_matmat_round(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, double eps,
                      tta_tt** C);
 /* matmat_round(A, B, round_options, round_result*) with every option
  * (matmat_round.hpp:41-94): method als = matmat_als_round
- * (tt_matmat_als_round.hpp:39-141) on the packed product, starts as for
+ * (tt_matmat_als_round.hpp:39-141) and method dmrg = matmat_dmrg_round
+ * (tt_matmat_dmrg_round.hpp:38-130) on the packed product, starts as for
  * tta_matvec_round_iterative (warm is a TT-matrix). */
 int tta_matmat_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, double eps, int max_rank, int method,
                                int max_iters, double tol, unsigned long long seed, const tta_tt* warm, tta_tt** C,

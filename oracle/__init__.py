@@ -2,7 +2,7 @@
 
 ctypes wrapper over ``oracle/_build/libtt_oracle.so`` (built from
 ``oracle/tt_oracle.cpp`` by ``oracle/Makefile``).  It restates the reference
-algorithm of bdaszuta/tensor_train (``include/core/round/{tt_round,tt_soft_threshold,tt_als}.hpp``,
+algorithm of bdaszuta/tensor_train (``include/core/round/{tt_round,tt_soft_threshold,tt_als,tt_dmrg}.hpp``,
 ``include/core/detail/{orthogonalize,truncate,contract,streaming_apply,
 fused_apply_core}.hpp``, ``include/core/apply/tt_matrix_apply.hpp``,
 ``include/core/algebra/{tt_inner,tt_ops}.hpp``,
@@ -83,6 +83,9 @@ def lib():
         L.ora_round_als.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
                                     ctypes.c_int, ctypes.c_ulonglong, _i32p, _f64p, _i32p]
         L.ora_round_als.restype = _vp
+        L.ora_round_dmrg.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
+                                     ctypes.c_int, ctypes.c_ulonglong, _i32p, _f64p, _i32p]
+        L.ora_round_dmrg.restype = _vp
         L.ora_soft_threshold.argtypes = [_vp, ctypes.c_double]
         L.ora_soft_threshold.restype = _vp
         L.ora_from_dense.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int]
@@ -267,6 +270,27 @@ def matvec_round_als(A, x, eps=0.0, max_rank=0, max_iters=20, tol=1e-12, warm=No
 def matmat_round_als(A, B, eps=0.0, max_rank=0, max_iters=20, tol=1e-12, warm=None, seed=0):
     """matmat_als_round (include/core/round/tt_matmat_als_round.hpp:39-141)."""
     return round_als(matmat(A, B), eps, max_rank, max_iters, tol, warm, True, seed)
+
+
+def round_dmrg(a, eps, max_rank=0, max_iters=20, tol=1e-12, warm=None, apply_path=False, seed=0):
+    """Two-site DMRG rounding (include/core/round/tt_dmrg.hpp); returns (tt,
+    iters_run, final_change, converged).  apply_path selects the starts of
+    the apply-then-DMRG paths."""
+    it, cv = ctypes.c_int(), ctypes.c_int()
+    ch = ctypes.c_double()
+    h = lib().ora_round_dmrg(a.h, eps, max_rank, max_iters, tol, warm.h if warm is not None else None,
+                             1 if apply_path else 0, seed, ctypes.byref(it), ctypes.byref(ch), ctypes.byref(cv))
+    return OTT(h, a.m, a.n), it.value, ch.value, bool(cv.value)
+
+
+def matvec_round_dmrg(A, x, eps, max_rank=0, max_iters=20, tol=1e-12, warm=None, seed=0):
+    """matvec_dmrg_round (include/core/round/tt_matvec_dmrg_round.hpp:29-105)."""
+    return round_dmrg(matvec(A, x), eps, max_rank, max_iters, tol, warm, True, seed)
+
+
+def matmat_round_dmrg(A, B, eps, max_rank=0, max_iters=20, tol=1e-12, warm=None, seed=0):
+    """matmat_dmrg_round (include/core/round/tt_matmat_dmrg_round.hpp:38-130)."""
+    return round_dmrg(matmat(A, B), eps, max_rank, max_iters, tol, warm, True, seed)
 
 
 def soft_threshold(a, tau=0.0):

@@ -1159,6 +1159,162 @@ static tt als_round(const tt& Y, double eps, int max_rank, int max_iters, double
   return als_engine(Y, std::move(X0), max_iters, tol, !use_warm && !cold_random, iters, change, conv);
 }
 
+// merge_y_block (include/core/detail/dmrg_local_solve.hpp:34-75): the
+// two-site target Theta (rXl n_k x n_{k+1} rXr) = (L right_unfold(Y_k)) as
+// (rXl n_k x rYm) times (left_unfold(Y_{k+1}) R) as (rYm x n_{k+1} rXr).
+static mat merge_block(const mat& L, const core& Yk, const core& Yk1, const mat& R)
+{
+  const int rXl = L.rows, nk = Yk.n, rYm = Yk.rp, nk1 = Yk1.n, rXr = R.cols;
+  if (L.cols != Yk.r || R.rows != Yk1.rp || Yk1.r != rYm) die("merge_block: shape mismatch");
+  std::vector<double> T1((size_t)rXl * nk * rYm, 0.0), T2((size_t)rYm * nk1 * rXr, 0.0);
+  gemm(rXl, nk * rYm, Yk.r, L.v.data(), L.cols, Yk.data(), nk * rYm, T1.data(), nk * rYm);
+  gemm(rYm * nk1, rXr, Yk1.rp, Yk1.data(), Yk1.rp, R.v.data(), rXr, T2.data(), rXr);
+  mat Th(rXl * nk, nk1 * rXr);
+  gemm(rXl * nk, nk1 * rXr, rYm, T1.data(), rYm, T2.data(), nk1 * rXr, Th.v.data(), nk1 * rXr);
+  return Th;
+}
+
+// split_two_site_lr / _rl (dmrg_local_solve.hpp:83-177): truncated SVD of
+// Theta; L->R keeps U (left-orthogonal X_k) and diag(s) Vt (X_{k+1}), R->L
+// keeps U diag(s) (X_k) and Vt (right-orthogonal X_{k+1}).
+static void split_two_site(const mat& Th, int rXl, int nk, int nk1, int rXr, double delta, int max_rank, bool lr,
+                           core& Xk, core& Xk1, std::vector<double>* s_head)
+{
+  mat U, Vt;
+  std::vector<double> s;
+  svd_thin(Th.v.data(), Th.rows, Th.cols, U, s, Vt);
+  const int r = truncate_eps_rank(s.data(), (int)s.size(), delta, max_rank);
+  Xk = core(rXl, nk, r);
+  Xk1 = core(r, nk1, rXr);
+  for (int i = 0; i < rXl * nk; ++i)
+    for (int j = 0; j < r; ++j) Xk.v[(size_t)i * r + j] = U(i, j) * (lr ? 1.0 : s[j]);
+  for (int j = 0; j < r; ++j)
+    for (int t = 0; t < nk1 * rXr; ++t) Xk1.v[(size_t)j * nk1 * rXr + t] = Vt(j, t) * (lr ? s[j] : 1.0);
+  if (s_head) s_head->assign(s.begin(), s.begin() + r);
+}
+
+// detail::dmrg_engine (include/core/round/tt_dmrg.hpp:53-230): per-bond
+// delta = eps ||Y|| / sqrt(d-1); right environments of the (right-
+// orthogonalized unless right-canonical) start; each iteration one L->R and
+// one R->L two-site sweep; the change is the largest relative change of the
+// kept singular values of the L->R sweep (1.0 with no baseline or a changed
+// count), converged when change <= tol.
+static tt dmrg_engine(const tt& Y, tt X, double eps, int max_rank, int max_iters, double tol, bool right_canonical,
+                      int* iters_out, double* change_out, int* conv_out)
+{
+  const int d = Y.d();
+  if (!right_canonical) right_orthogonalize(X);
+  const double y_norm = std::sqrt(std::max(inner(Y, Y), 0.0));
+  const double delta = eps * y_norm / std::sqrt((double)std::max(d - 1, 1));
+  std::vector<mat> R(d + 1), L(d + 1);
+  R[d] = mat(1, 1);
+  R[d](0, 0) = 1.0;
+  for (int k = d - 1; k >= 1; --k) R[k] = env_right(Y.c[k], X.c[k], R[k + 1]);
+  L[0] = mat(1, 1);
+  L[0](0, 0) = 1.0;
+  std::vector<std::vector<double>> s_prev(d - 1), s_now(d - 1);
+  bool have_prev = false, converged = false;
+  double change = 0.0;
+  int iters = 0;
+  for (int it = 0; it < max_iters; ++it)
+  {
+    for (int k = 0; k < d - 1; ++k)
+    {
+      const mat Th = merge_block(L[k], Y.c[k], Y.c[k + 1], R[k + 2]);
+      split_two_site(Th, L[k].rows, Y.c[k].n, Y.c[k + 1].n, R[k + 2].cols, delta, max_rank, true, X.c[k], X.c[k + 1],
+                     &s_now[k]);
+      L[k + 1] = env_left(L[k], X.c[k], Y.c[k]);
+    }
+    for (int k = d - 2; k >= 0; --k)
+    {
+      const mat Th = merge_block(L[k], Y.c[k], Y.c[k + 1], R[k + 2]);
+      split_two_site(Th, L[k].rows, Y.c[k].n, Y.c[k + 1].n, R[k + 2].cols, delta, max_rank, false, X.c[k],
+                     X.c[k + 1], nullptr);
+      R[k + 1] = env_right(Y.c[k + 1], X.c[k + 1], R[k + 2]);
+    }
+    ++iters;
+    if (have_prev)
+    {
+      double mx = 0.0;
+      for (int k = 0; k < d - 1; ++k)
+      {
+        double nn = 0.0;
+        for (double v : s_now[k]) nn += v * v;
+        nn = std::sqrt(nn);
+        if (s_prev[k].size() == s_now[k].size())
+        {
+          double df = 0.0;
+          for (size_t i = 0; i < s_now[k].size(); ++i) df += (s_now[k][i] - s_prev[k][i]) * (s_now[k][i] - s_prev[k][i]);
+          mx = std::max(mx, std::sqrt(df) / (nn + 1.0e-300));
+        }
+        else
+          mx = 1.0;
+      }
+      change = mx;
+    }
+    else
+      change = 1.0;
+    s_prev = s_now;
+    have_prev = true;
+    if (change <= tol)
+    {
+      converged = true;
+      break;
+    }
+  }
+  *iters_out = iters;
+  *change_out = change;
+  *conv_out = converged ? 1 : 0;
+  return X;
+}
+
+// detail::dmrg_round (tt_dmrg.hpp:235-310) and, with apply_path, the starts
+// of matvec_dmrg_round (tt_matvec_dmrg_round.hpp:29-105) and
+// matmat_dmrg_round (tt_matmat_dmrg_round.hpp:38-130): eps must be > 0;
+// d <= 1 -> SVD round with {0, 0, true}; a compatible warm start (ranks
+// checked against max_rank when one is set) is right-orthogonalized; cold
+// start round_right_canonical(Y, eps, max_rank), or for the apply paths
+// random(shape, max_rank, seed) when max_rank > 0, else
+// round_right_canonical(Y, eps, 0).
+static tt dmrg_round(const tt& Y, double eps, int max_rank, int max_iters, double tol, const tt* warm,
+                     bool apply_path, uint64_t seed, int* iters, double* change, int* conv)
+{
+  if (std::isnan(eps) || eps <= 0.0) die("dmrg_round: eps must be > 0");
+  const int d = Y.d();
+  if (d <= 1)
+  {
+    *iters = 0;
+    *change = 0.0;
+    *conv = 1;
+    return round_tt(Y, eps, max_rank);
+  }
+  bool use_warm = warm && warm->d() == d;
+  if (use_warm)
+    for (int k = 0; k < d; ++k) use_warm = use_warm && warm->c[k].n == Y.c[k].n;
+  if (use_warm && !Y.m.empty()) use_warm = warm->m == Y.m;
+  if (use_warm && max_rank > 0)
+    for (auto& c : warm->c)
+      if (c.r > max_rank || c.rp > max_rank) die("dmrg_round: warm start rank above max_rank");
+  tt X0;
+  bool rc = false;
+  if (use_warm)
+    X0 = *warm;
+  else if (apply_path && max_rank > 0)
+  {
+    std::vector<int> shape(d);
+    for (int k = 0; k < d; ++k) shape[k] = Y.c[k].n;
+    X0 = random_impl(shape, max_rank, seed);
+  }
+  else
+  {
+    X0 = round_right_canonical(Y, eps, apply_path ? 0 : max_rank);
+    rc = true;
+  }
+  X0.m = Y.m;
+  X0.nn = Y.nn;
+  return dmrg_engine(Y, std::move(X0), eps, max_rank, max_iters, tol, rc, iters, change, conv);
+}
+
 // soft_threshold (include/core/round/tt_soft_threshold.hpp:42-176): d <= 1
 // or tau <= 0 returns a copy; otherwise right-orthogonalize, then for
 // k = d-1..1 the thin SVD of left_unfold(c_{k-1}) right_unfold(c_k), the
@@ -1501,6 +1657,12 @@ extern "C"
   {
     return new tt(ora::als_round(*static_cast<tt*>(a), eps, max_rank, max_iters, tol, static_cast<tt*>(warm),
                                  cold_random != 0, seed, info_iters, info_change, info_conv));
+  }
+  void* ora_round_dmrg(void* a, double eps, int max_rank, int max_iters, double tol, void* warm, int apply_path,
+                       unsigned long long seed, int* info_iters, double* info_change, int* info_conv)
+  {
+    return new tt(ora::dmrg_round(*static_cast<tt*>(a), eps, max_rank, max_iters, tol, static_cast<tt*>(warm),
+                                  apply_path != 0, seed, info_iters, info_change, info_conv));
   }
   void* ora_soft_threshold(void* a, double tau) { return new tt(ora::soft_threshold(*static_cast<tt*>(a), tau)); }
   void* ora_from_dense(const double* dense, int d, const int* shape, double eps, int max_rank)

@@ -108,8 +108,8 @@ class RoundOptions:
     eps=0.0 (no truncation), max_rank=0 (no cap), method="svd",
     gauge="none", max_iters=20, tol=1e-12, seed=0, warm_start=None (a
     TensorTrain, or a TensorTrainMatrix for matrix rounds: warm_start_tt /
-    warm_start_tt_matrix).  method="als" runs the ALS engine; "dmrg" is
-    rejected as unsupported.
+    warm_start_tt_matrix).  method="als" and "dmrg" run the iterative
+    engines.
     """
 
     def __init__(self, eps=0.0, max_rank=0, method="svd", gauge="none", max_iters=20, tol=1e-12, seed=0,

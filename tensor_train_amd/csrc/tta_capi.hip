@@ -1328,6 +1328,210 @@ tta_tt* als_round_dev(tta_ctx* c, const tta_tt* Y, double eps, int max_rank, int
   return X;
 }
 
+// ---------------------------------------------------------------------------
+// Two-site DMRG rounding (include/core/round/tt_dmrg.hpp:53-310; the apply
+// paths tt_matvec_dmrg_round.hpp:29-105, tt_matmat_dmrg_round.hpp:38-130).
+// The L->R split keeps U (X_k) and the kept singular values; the R->L split
+// factors Theta^T, whose left factor V is the right-orthogonal X_{k+1}.  The
+// factors the reference forms and the next bond overwrites unread
+// (diag(s) V^T going right, U diag(s) going left except at bond 0) and the
+// environments no bond reads (L_{d-1}, R_1) are not formed.
+// ---------------------------------------------------------------------------
+
+// merge_y_block (include/core/detail/dmrg_local_solve.hpp:34-75): Theta
+// (rXl n_k x n_{k+1} rXr) row-major into Th, or its transpose.
+void merge_block_dev(tta_ctx* c, const double* L, int rXl, const tta_tt* Y, int k, const double* R, int rXr,
+                     bool transposed, double* Th)
+{
+  const int rYl = Y->ranks[k], nk = Y->modes[k], rYm = Y->ranks[k + 1], nk1 = Y->modes[k + 1],
+            rYr = Y->ranks[k + 2];
+  const int rows = rXl * nk, cols = nk1 * rXr;
+  double* T1 = alloc_core(c, (size_t)rXl * nk * rYm);
+  double* T2 = alloc_core(c, (size_t)rYm * nk1 * rXr);
+  gemm_rm(c, rXl, nk * rYm, rYl, L, rYl, Y->cores[k], (long)nk * rYm, T1, (long)nk * rYm);
+  gemm_rm(c, rYm * nk1, rXr, rYr, Y->cores[k + 1], rYr, R, rXr, T2, rXr);
+  gemm_desc g;
+  g.M = rows;
+  g.N = cols;
+  g.K = rYm;
+  g.A = T1;
+  g.sar = rYm;
+  g.sac = 1;
+  g.B = T2;
+  g.sbr = cols;
+  g.sbc = 1;
+  g.C = Th;
+  g.scr = transposed ? 1 : cols;
+  g.scc = transposed ? rows : 1;
+  run_gemm(c, g);
+  c->mem.put(T1);
+  c->mem.put(T2);
+}
+
+// detail::dmrg_engine (tt_dmrg.hpp:53-230) on X in place.
+void dmrg_engine_dev(tta_ctx* c, const tta_tt* Y, tta_tt* X, double eps, int max_rank, int max_iters, double tol,
+                     bool right_canonical, iter_info* info)
+{
+  const int d = Y->d();
+  if (!right_canonical) right_orthogonalize_impl(c, X);
+  const double y_norm = std::sqrt(std::max(inner_impl(c, Y, Y), 0.0));
+  const double delta = eps * y_norm / std::sqrt((double)std::max(d - 1, 1));
+  if (y_norm == 0.0 && max_iters > 0)
+  {
+    // Every bond target is zero: each split keeps one zero singular value,
+    // so the result is the rank-1 zero tensor and the change is exactly 0
+    // from the second sweep on.
+    make_rank1_zero(c, X);
+    const bool conv = max_iters >= 2 && 0.0 <= tol;
+    const int it = conv ? 2 : max_iters;
+    *info = iter_info{it, it >= 2 ? 0.0 : 1.0, conv ? 1 : 0};
+    return;
+  }
+  env_chain E(c, d);
+  for (int k = d - 1; k >= 2; --k) E.set_R(k, env_right_dev(c, Y, k, X, E.R[k + 1]));
+  std::vector<std::vector<double>> s_prev;
+  bool have_prev = false, converged = false;
+  double change = 0.0;
+  int iters = 0;
+  std::vector<double*> sblk(d - 1, nullptr);
+  std::vector<int> skept(d - 1, 0);
+  auto release = [&]()
+  {
+    for (double*& p : sblk)
+      if (p)
+      {
+        c->mem.put(p);
+        p = nullptr;
+      }
+  };
+  try
+  {
+    for (int it = 0; it < max_iters; ++it)
+    {
+      for (int k = 0; k < d - 1; ++k)
+      {
+        const int rXl = X->ranks[k], rXr = X->ranks[k + 2];
+        const int rows = rXl * Y->modes[k], cols = Y->modes[k + 1] * rXr;
+        double* Th = ctx_ws(c, WS_M1, (size_t)rows * cols);
+        merge_block_dev(c, E.L[k], rXl, Y, k, E.R[k + 2], rXr, false, Th);
+        double *U = nullptr, *F = nullptr, *s = nullptr;
+        const int r = svd_left(c, Th, rows, cols, delta, max_rank, false, &U, &F, &s);
+        replace_core(c, X, k, U);
+        X->ranks[k + 1] = r;
+        if (sblk[k]) c->mem.put(sblk[k]);
+        sblk[k] = alloc_core(c, (size_t)r);
+        skept[k] = r;
+        hip_check(hipMemcpyAsync(sblk[k], s, (size_t)r * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+        if (k < d - 2) E.set_L(k + 1, env_left_dev(c, E.L[k], X, k, Y));
+      }
+      for (int k = d - 2; k >= 0; --k)
+      {
+        const int rXl = X->ranks[k], rXr = X->ranks[k + 2];
+        const int rows = rXl * Y->modes[k], cols = Y->modes[k + 1] * rXr;
+        double* ThT = ctx_ws(c, WS_M1, (size_t)rows * cols);
+        merge_block_dev(c, E.L[k], rXl, Y, k, E.R[k + 2], rXr, true, ThT);
+        double *V = nullptr, *G = nullptr;
+        const int r = svd_left(c, ThT, cols, rows, delta, max_rank, false, &V, &G);
+        double* Xn1 = alloc_core(c, (size_t)r * cols);
+        launch_transpose(V, r, cols, r, Xn1, cols, c->stream);
+        c->mem.put(V);
+        replace_core(c, X, k + 1, Xn1);
+        X->ranks[k + 1] = r;
+        if (k == 0)
+        {
+          double* Xn0 = alloc_core(c, (size_t)rows * r);
+          launch_transpose(G, rows, r, rows, Xn0, r, c->stream);
+          replace_core(c, X, 0, Xn0);
+        }
+        else
+          E.set_R(k + 1, env_right_dev(c, Y, k + 1, X, E.R[k + 2]));
+      }
+      ++iters;
+      std::vector<std::vector<double>> s_now(d - 1);
+      for (int k = 0; k < d - 1; ++k)
+      {
+        s_now[k].resize((size_t)skept[k]);
+        hip_check(hipMemcpyAsync(s_now[k].data(), sblk[k], (size_t)skept[k] * 8, hipMemcpyDeviceToHost, c->stream),
+                  "hipMemcpyAsync");
+      }
+      hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+      change = have_prev ? sv_change(s_now, s_prev) : 1.0;
+      s_prev = std::move(s_now);
+      have_prev = true;
+      if (change <= tol)
+      {
+        converged = true;
+        break;
+      }
+    }
+  }
+  catch (...)
+  {
+    release();
+    throw;
+  }
+  release();
+  info->iters = iters;
+  info->change = change;
+  info->converged = converged ? 1 : 0;
+}
+
+// detail::dmrg_round (tt_dmrg.hpp:235-310), and with apply_path the starts
+// of matvec_dmrg_round / matmat_dmrg_round: random(modes, max_rank, seed)
+// when max_rank > 0, else round_right_canonical(Y, eps, 0).
+tta_tt* dmrg_round_dev(tta_ctx* c, const tta_tt* Y, double eps, int max_rank, int max_iters, double tol,
+                       const tta_tt* warm, bool apply_path, unsigned long long seed, iter_info* info, const char* what)
+{
+  if (std::isnan(eps) || eps <= 0.0)
+    fail(TTA_ERR_ARG,
+         "%s: eps must be > 0 for DMRG compression (got %.15e); eps=0 means 'no truncation' only for the SVD path",
+         what, eps);
+  const int d = Y->d();
+  if (d <= 1)
+  {
+    tta_tt* t = copy_tt(c, Y);
+    try
+    {
+      round_inplace(c, t, eps, max_rank, TTA_GAUGE_NONE);
+    }
+    catch (...)
+    {
+      free_tt(c, t);
+      throw;
+    }
+    *info = iter_info{0, 0.0, 1};
+    return t;
+  }
+  const bool use_warm = warm_compatible(warm, Y);
+  if (use_warm && max_rank > 0)
+    for (int r : warm->ranks)
+      if (r > max_rank) fail(TTA_ERR_ARG, "%s: warm_start max rank %d exceeds max_rank %d", what, r, max_rank);
+  tta_tt* X = nullptr;
+  bool rc = false;
+  try
+  {
+    if (use_warm)
+      X = copy_tt(c, warm);
+    else if (apply_path && max_rank > 0)
+      X = random_like(c, Y, max_rank, seed);
+    else
+    {
+      X = copy_tt(c, Y);
+      round_inplace(c, X, eps, apply_path ? 0 : max_rank, TTA_GAUGE_RIGHT_CANONICAL);
+      rc = true;
+    }
+    X->row_modes = Y->row_modes;
+    X->col_modes = Y->col_modes;
+    dmrg_engine_dev(c, Y, X, eps, max_rank, max_iters, tol, rc, info);
+  }
+  catch (...)
+  {
+    free_tt(c, X);
+    throw;
+  }
+  return X;
+}
+
 void check_tt(const tta_tt* t, const char* what)
 {
   if (!t) fail(TTA_ERR_ARG, "%s: null tensor", what);
@@ -1376,8 +1580,18 @@ tta_tt* round_dispatch(tta_ctx* c, const tta_tt* a, double eps, int max_rank, in
     fail(TTA_ERR_METHOD,
          "round: round_method::svd_streaming / svd_naive are only valid on matvec_round / matmat_round "
          "respectively.");
-  if (method == TTA_METHOD_DMRG) fail(TTA_ERR_UNSUPPORTED, "round: the DMRG engine is not part of this library");
-  if (method != TTA_METHOD_SVD && method != TTA_METHOD_ALS) fail(TTA_ERR_METHOD, "round: unknown method %d", method);
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_ALS && method != TTA_METHOD_DMRG)
+    fail(TTA_ERR_METHOD, "round: unknown method %d", method);
+  if (method == TTA_METHOD_DMRG)
+  {
+    // round_options_to_dmrg (round.hpp:155-176).
+    if (std::isnan(eps) || eps <= 0.0)
+      fail(TTA_ERR_ARG, "round: eps must be > 0 for method=dmrg (eps=0 means 'no truncation' only for the SVD path)");
+    if (max_iters < 0) fail(TTA_ERR_ARG, "round: max_iters is negative");
+    tta_tt* t = dmrg_round_dev(c, a, eps, max_rank, max_iters, tol, warm, false, seed, info, "dmrg_round");
+    t->matrix = a->matrix;
+    return t;
+  }
   if (std::isnan(eps)) fail(TTA_ERR_ARG, "round: eps is NaN");
   if (method == TTA_METHOD_ALS)
   {
@@ -1407,9 +1621,8 @@ tta_tt* matvec_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* x, doub
                               int max_iters, double tol, unsigned long long seed, const tta_tt* warm, iter_info* info)
 {
   if (method == TTA_METHOD_SVD_NAIVE) fail(TTA_ERR_METHOD, "matvec_round: round_method::svd_naive is matmat-only");
-  if (method == TTA_METHOD_DMRG)
-    fail(TTA_ERR_UNSUPPORTED, "matvec_round: the DMRG engine is not part of this library");
-  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_STREAMING && method != TTA_METHOD_ALS)
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_STREAMING && method != TTA_METHOD_ALS &&
+      method != TTA_METHOD_DMRG)
     fail(TTA_ERR_METHOD, "matvec_round: unknown round_method %d", method);
   check_matvec(A, x, "matvec_round");
   if (method == TTA_METHOD_SVD_STREAMING && A->d() > 1)
@@ -1419,14 +1632,22 @@ tta_tt* matvec_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* x, doub
   }
   if (method == TTA_METHOD_ALS && std::isnan(eps))
     fail(TTA_ERR_ARG, "matvec_als_round: eps is NaN; must be a valid number");
-  if (method == TTA_METHOD_ALS && max_iters < 0) fail(TTA_ERR_ARG, "matvec_round: max_iters is negative");
+  if (method == TTA_METHOD_DMRG && (std::isnan(eps) || eps <= 0.0))
+    fail(TTA_ERR_ARG,
+         "matvec_dmrg_round: eps must be > 0 for DMRG compression (got %.15e); eps=0 means 'no truncation' only "
+         "for the SVD path",
+         eps);
+  if ((method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG) && max_iters < 0)
+    fail(TTA_ERR_ARG, "matvec_round: max_iters is negative");
   tta_tt* Y = matvec_impl(c, A, x);
-  if (method == TTA_METHOD_ALS)
+  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
   {
     tta_tt* r = nullptr;
     try
     {
-      r = als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matvec_als_round");
+      r = method == TTA_METHOD_ALS
+            ? als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matvec_als_round")
+            : dmrg_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matvec_dmrg_round");
     }
     catch (...)
     {
@@ -1455,9 +1676,8 @@ tta_tt* matmat_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* B, doub
 {
   if (method == TTA_METHOD_SVD_STREAMING)
     fail(TTA_ERR_METHOD, "matmat_round: round_method::svd_streaming is matvec-only");
-  if (method == TTA_METHOD_DMRG)
-    fail(TTA_ERR_UNSUPPORTED, "matmat_round: the DMRG engine is not part of this library");
-  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_NAIVE && method != TTA_METHOD_ALS)
+  if (method != TTA_METHOD_SVD && method != TTA_METHOD_SVD_NAIVE && method != TTA_METHOD_ALS &&
+      method != TTA_METHOD_DMRG)
     fail(TTA_ERR_METHOD, "matmat_round: unknown round_method %d", method);
   check_matmat(A, B, "matmat_round");
   tta_tt* r = nullptr;
@@ -1470,14 +1690,22 @@ tta_tt* matmat_round_dispatch(tta_ctx* c, const tta_tt* A, const tta_tt* B, doub
   }
   if (method == TTA_METHOD_ALS && std::isnan(eps))
     fail(TTA_ERR_ARG, "matmat_als_round: eps is NaN; must be a valid number");
-  if (method == TTA_METHOD_ALS && max_iters < 0) fail(TTA_ERR_ARG, "matmat_round: max_iters is negative");
+  if (method == TTA_METHOD_DMRG && (std::isnan(eps) || eps <= 0.0))
+    fail(TTA_ERR_ARG,
+         "matmat_dmrg_round: eps must be > 0 for DMRG compression (got %.15e); eps=0 means 'no truncation' only "
+         "for the SVD path",
+         eps);
+  if ((method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG) && max_iters < 0)
+    fail(TTA_ERR_ARG, "matmat_round: max_iters is negative");
   tta_tt* Y = matmat_impl(c, A, B);
   Y->matrix = true;
-  if (method == TTA_METHOD_ALS)
+  if (method == TTA_METHOD_ALS || method == TTA_METHOD_DMRG)
   {
     try
     {
-      r = als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matmat_als_round");
+      r = method == TTA_METHOD_ALS
+            ? als_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matmat_als_round")
+            : dmrg_round_dev(c, Y, eps, max_rank, max_iters, tol, warm, true, seed, info, "matmat_dmrg_round");
     }
     catch (...)
     {

@@ -233,6 +233,29 @@ int main()
     o.warm_start_tt = &A;
     auto P = tt::round(Bp, o, &info);
     CHECK(info.iters_run <= 5 && rel_to(P.to_dense(), Bp.to_dense()) <= 1e-2, "als warm-pert");
+    // test_tt_round.cpp:227-384: the same cases for dmrg, plus eps-only.
+    tt::round_options g;
+    g.method = tt::round_method::dmrg;
+    g.max_rank = cap;
+    g.eps = 1e-12;
+    g.max_iters = 3;
+    g.tol = 1e-12;
+    auto D = tt::round(B, g, &info);
+    CHECK(rel_to(D.to_dense(), dA) <= 1e-10 && D.max_rank() <= cap, "dmrg cold");
+    g.warm_start_tt = &A;
+    CHECK(rel_to(tt::round(B, g).to_dense(), dA) <= 1e-10, "dmrg warm");
+    g.warm_start_tt = nullptr;
+    g.tol = 1e-13;
+    CHECK(rel_to(tt::round(B, g).to_dense(), dA) <= 1.1 * rel_to(Cs.to_dense(), dA) + 1e-12, "dmrg vs svd");
+    g.tol = 1e-12;
+    g.warm_start_tt = &A;
+    auto Pd = tt::round(Bp, g, &info);
+    CHECK(info.iters_run <= 5 && rel_to(Pd.to_dense(), Bp.to_dense()) <= 1e-2, "dmrg warm-pert");
+    g.warm_start_tt = nullptr;
+    g.max_rank = 0;
+    g.eps = 1e-10;
+    auto De = tt::round(B, g, &info);
+    CHECK(rel_to(De.to_dense(), dA) <= 1e-8 && De.max_rank() <= B.max_rank(), "dmrg eps-only");
   }
   // test_tt_matvec_round.cpp / test_tt_matrix_round.cpp: ALS apply-round
   // recovers the product at its exact rank from a random cold start.

@@ -176,8 +176,6 @@ def test_als_info_and_errors():
         T.round(Yd, method="als", eps=float("nan"), max_rank=2)
     with pytest.raises(T.TTAError, match="warm_start max rank"):
         T.round(Yd, method="als", max_rank=2, warm_start=Yd)
-    # The svd method reports {1, 0, true}; dmrg is not provided.
+    # The svd method reports {1, 0, true}.
     _, info = T.round_ex(Yd, eps=1e-8)
     assert (info.iters_run, info.final_resid, info.converged) == (1, 0.0, True)
-    with pytest.raises(T.TTAError):
-        T.round(Yd, method="dmrg", eps=1e-3)
