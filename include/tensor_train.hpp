@@ -644,6 +644,26 @@ inline double frob_norm(const tt_matrix& a)
   return std::sqrt(std::max(s, 0.0));
 }
 
+// ||A x||_F and ||A B||_F without the product tensor
+// (algebra/tt_matrix_frob_norm_apply.hpp:36-94); d = 0 gives 0.
+inline double frob_norm_apply(const tt_matrix& a, const tt& x)
+{
+  if (a.d() == 0) return 0.0;
+  detail::dev_handle p(detail::upload(a)), q(detail::upload(x));
+  double v = 0.0;
+  detail::check(tta_frob_norm_apply(detail::device_ctx(), p.t, q.t, &v), "frob_norm_apply(matvec)");
+  return v;
+}
+
+inline double frob_norm_apply(const tt_matrix& a, const tt_matrix& b)
+{
+  if (a.d() == 0) return 0.0;
+  detail::dev_handle p(detail::upload(a)), q(detail::upload(b));
+  double v = 0.0;
+  detail::check(tta_frob_norm_apply(detail::device_ctx(), p.t, q.t, &v), "frob_norm_apply(matmat)");
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Input algebra and factories (round inputs).
 // ---------------------------------------------------------------------------

@@ -153,6 +153,11 @@ int tta_matmat_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, d
 int tta_inner(tta_ctx* ctx, const tta_tt* a, const tta_tt* b, double* out);
 /* norm(a) = sqrt(max(inner(a, a), 0)) (tt_inner.hpp:55-59; tt_matrix_inner.hpp:44-48). */
 int tta_norm(tta_ctx* ctx, const tta_tt* a, double* out);
+/* frob_norm_apply(A, x) / frob_norm_apply(A, B)
+ * (include/core/algebra/tt_matrix_frob_norm_apply.hpp:36-94): ||A x||_F or
+ * ||A B||_F by one environment sweep over the product cores, without the
+ * product tensor; X is a TT (matvec) or a TT-matrix (matmat). */
+int tta_frob_norm_apply(tta_ctx* ctx, const tta_tt* A, const tta_tt* X, double* out);
 /* axpby(alpha, a, beta, b) (include/core/algebra/tt_ops.hpp:72-110). */
 int tta_axpby(tta_ctx* ctx, double alpha, const tta_tt* a, double beta, const tta_tt* b, tta_tt** out);
 /* scale(a, alpha) (tt_ops.hpp:35-50). */

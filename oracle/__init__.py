@@ -251,6 +251,16 @@ def svd_thin(M):
     return U, s, Vt
 
 
+def frob_norm_apply(A, X):
+    """frob_norm_apply (include/core/algebra/tt_matrix_frob_norm_apply.hpp:36-94):
+    per core the product core (matvec_core / matmat_core) folded into the
+    inner-product environment, i.e. norm(matvec(A, x)) / norm(matmat(A, B))
+    with the same operation order; d = 0 gives 0."""
+    if A.d == 0:
+        return 0.0
+    return norm(matmat(A, X) if X.m is not None else matvec(A, X))
+
+
 def round_als(a, eps=0.0, max_rank=0, max_iters=20, tol=1e-12, warm=None, cold_random=False, seed=0):
     """ALS rounding (include/core/round/tt_als.hpp); returns (tt, iters_run,
     final_change, converged).  cold_random starts from random(shape,

@@ -21,7 +21,7 @@ __all__ = [
     "Context", "TensorTrain", "TensorTrainMatrix", "RoundOptions", "RoundResult", "round", "round_ex",
     "right_orthogonalize", "left_orthogonalize", "matvec", "matmat", "matvec_round", "matmat_round",
     "matvec_round_ex", "matmat_round_ex", "inner",
-    "norm", "frob_inner", "frob_norm", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
+    "norm", "frob_inner", "frob_norm", "frob_norm_apply", "axpby", "add", "sub", "scale", "TTAError", "random", "random_matrix",
     "zeros", "ones", "random_host", "from_dense", "matrix_from_dense", "soft_threshold",
 ]
 
@@ -442,6 +442,14 @@ def norm(a) -> float:
 
 frob_inner = inner
 frob_norm = norm
+
+
+def frob_norm_apply(A, X) -> float:
+    """||A x||_F or ||A B||_F without the product tensor
+    (include/core/algebra/tt_matrix_frob_norm_apply.hpp:36-94)."""
+    out = ctypes.c_double()
+    A._ctx.check(A._ctx.lib.tta_frob_norm_apply(A._ctx.h, A._h, X._h, ctypes.byref(out)))
+    return out.value
 
 
 def axpby(alpha, a, beta, b):

@@ -901,6 +901,42 @@ tta_tt* streaming_impl(tta_ctx* c, const tta_tt* A_in, const tta_tt* X_in, bool 
   return out;
 }
 
+// One contract_inner_step (contract.hpp:155-205): E1 (rap x rbp) =
+// (E0^T right_unfold(ca))_as^T left_unfold(cb), with E0 (ra x rb) and the
+// (rb x n rap) intermediate in T.
+void inner_step_dev(tta_ctx* c, const double* E0, double* E1, double* T, const double* ca, int ra, int n, int rap,
+                    const double* cb, int rb, int rbp)
+{
+  gemm_desc g;
+  g.M = rb;
+  g.N = n * rap;
+  g.K = ra;
+  g.A = E0;
+  g.sar = 1;
+  g.sac = rb;
+  g.B = ca;
+  g.sbr = (long)n * rap;
+  g.sbc = 1;
+  g.C = T;
+  g.scr = (long)n * rap;
+  g.scc = 1;
+  run_gemm(c, g);
+  gemm_desc h;
+  h.M = rap;
+  h.N = rbp;
+  h.K = rb * n;
+  h.A = T;
+  h.sar = 1;
+  h.sac = rap;
+  h.B = cb;
+  h.sbr = rbp;
+  h.sbc = 1;
+  h.C = E1;
+  h.scr = rbp;
+  h.scc = 1;
+  run_gemm(c, h);
+}
+
 // inner (tt_inner.hpp:33-48) via contract_inner_step (contract.hpp:155-205).
 double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
 {
@@ -917,48 +953,56 @@ double inner_impl(tta_ctx* c, const tta_tt* a, const tta_tt* b)
   hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   for (int k = 0; k < d; ++k)
   {
-    const int n = a->modes[k], ra = a->ranks[k], rap = a->ranks[k + 1], rb = b->ranks[k], rbp = b->ranks[k + 1];
-    // tmp (rb x n*rap) = E^T (rb x ra) * right_unfold(ca) (ra x n*rap)
-    gemm_desc g;
-    g.M = rb;
-    g.N = n * rap;
-    g.K = ra;
-    g.A = E0;
-    g.sar = 1;
-    g.sac = rb;
-    g.B = a->cores[k];
-    g.sbr = (long)n * rap;
-    g.sbc = 1;
-    g.C = T;
-    g.scr = (long)n * rap;
-    g.scc = 1;
-    {
-      prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K,
-                    8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
-      launch_gemm(g, 1, c->stream);
-    }
-    // E' (rap x rbp) = tmp_as^T (rap x rb*n) * left_unfold(cb) (rb*n x rbp)
-    gemm_desc h;
-    h.M = rap;
-    h.N = rbp;
-    h.K = rb * n;
-    h.A = T;
-    h.sar = 1;
-    h.sac = rap;
-    h.B = b->cores[k];
-    h.sbr = rbp;
-    h.sbc = 1;
-    h.C = E1;
-    h.scr = rbp;
-    h.scc = 1;
-    {
-      prof_scope ps(c, TTA_K_GEMM, 2.0 * h.M * h.N * h.K,
-                    8.0 * ((double)h.M * h.K + (double)h.K * h.N + (double)h.M * h.N));
-      launch_gemm(h, 1, c->stream);
-    }
+    inner_step_dev(c, E0, E1, T, a->cores[k], a->ranks[k], a->modes[k], a->ranks[k + 1], b->cores[k], b->ranks[k],
+                   b->ranks[k + 1]);
     std::swap(E0, E1);
   }
   return read_double(c, E0);
+}
+
+// frob_norm_apply (include/core/algebra/tt_matrix_frob_norm_apply.hpp:36-94):
+// ||A x|| (or ||A B||) by one environment sweep over the product cores, each
+// formed in a reused scratch core and consumed at once; the rank-multiplied
+// product never exists as a whole.
+double frob_norm_apply_impl(tta_ctx* c, const tta_tt* A, const tta_tt* X, bool is_matmat)
+{
+  const int d = A->d();
+  if (d == 0) return 0.0;
+  size_t emax = 1, tmax = 1, ymax = 1;
+  for (int k = 0; k <= d; ++k)
+    emax = std::max(emax, (size_t)A->ranks[k] * X->ranks[k] * A->ranks[k] * X->ranks[k]);
+  std::vector<int> nk(d);
+  for (int k = 0; k < d; ++k)
+  {
+    nk[k] = A->row_modes[k] * (is_matmat ? X->col_modes[k] : 1);
+    const size_t r = (size_t)A->ranks[k] * X->ranks[k], rp = (size_t)A->ranks[k + 1] * X->ranks[k + 1];
+    ymax = std::max(ymax, r * nk[k] * rp);
+    tmax = std::max(tmax, r * nk[k] * rp);
+  }
+  double* E0 = ctx_ws(c, WS_X, emax);
+  double* E1 = ctx_ws(c, WS_J, emax);
+  double* T = ctx_ws(c, WS_M1, tmax);
+  double* Yk = ctx_ws(c, WS_A, ymax);
+  const double one = 1.0;
+  hip_check(hipMemcpyAsync(E0, &one, 8, hipMemcpyHostToDevice, c->stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  for (int k = 0; k < d; ++k)
+  {
+    const int r = A->ranks[k] * X->ranks[k], rp = A->ranks[k + 1] * X->ranks[k + 1];
+    {
+      prof_scope ps(c, TTA_K_APPLY, 2.0 * r * nk[k] * rp * A->col_modes[k],
+                    8.0 * ((double)core_numel(A, k) + (double)core_numel(X, k) + (double)r * nk[k] * rp));
+      if (is_matmat)
+        launch_matmat_core(A->cores[k], X->cores[k], Yk, A->ranks[k], A->row_modes[k], A->col_modes[k],
+                           A->ranks[k + 1], X->ranks[k], X->col_modes[k], X->ranks[k + 1], c->stream);
+      else
+        launch_matvec_core(A->cores[k], X->cores[k], Yk, A->ranks[k], A->row_modes[k], A->col_modes[k],
+                           A->ranks[k + 1], X->ranks[k], X->ranks[k + 1], c->stream);
+    }
+    inner_step_dev(c, E0, E1, T, Yk, r, nk[k], rp, Yk, r, rp);
+    std::swap(E0, E1);
+  }
+  return std::sqrt(std::max(read_double(c, E0), 0.0));
 }
 
 // ---------------------------------------------------------------------------
@@ -2074,6 +2118,20 @@ int tta_inner(tta_ctx* c, const tta_tt* a, const tta_tt* b, double* out)
   TTA_GUARD_BEGIN
   check_same_shape(a, b, "inner");
   *out = inner_impl(c, a, b);
+  TTA_GUARD_END(c)
+}
+
+int tta_frob_norm_apply(tta_ctx* c, const tta_tt* A, const tta_tt* X, double* out)
+{
+  if (!c || !out) return TTA_ERR_ARG;
+  TTA_GUARD_BEGIN
+  check_tt(X, "frob_norm_apply");
+  const bool mm = X->is_matrix();
+  if (mm)
+    check_matmat(A, X, "frob_norm_apply(matmat)");
+  else
+    check_matvec(A, X, "frob_norm_apply(matvec)");
+  *out = frob_norm_apply_impl(c, A, X, mm);
   TTA_GUARD_END(c)
 }
 

@@ -282,6 +282,15 @@ int main()
     for (size_t i = 0; i < dc.size(); ++i) df[i] = dca[i] - dc[i];
     CHECK(frob(df) <= 1e-10 * frob(dc) && info.converged, "matmat_round(als) recovers A B");
   }
+  // tt_matrix_frob_norm_apply.hpp: ||A x|| and ||A B|| without the product.
+  {
+    auto A = tt::random({2, 3, 2, 3}, {3, 2, 2, 2}, 3, 21);
+    auto x = tt::random({3, 2, 2, 2}, 4, 22);
+    auto B = tt::random({3, 2, 2, 2}, {2, 2, 3, 2}, 2, 23);
+    const double nx = tt::norm(tt::matvec(A, x)), nb = tt::frob_norm(tt::matmat(A, B));
+    CHECK(std::fabs(tt::frob_norm_apply(A, x) - nx) <= 1e-12 * nx, "frob_norm_apply(A, x) = norm(matvec(A, x))");
+    CHECK(std::fabs(tt::frob_norm_apply(A, B) - nb) <= 1e-12 * nb, "frob_norm_apply(A, B) = frob_norm(matmat(A, B))");
+  }
   std::printf("%d failure(s)\n", failures);
   return failures;
 }

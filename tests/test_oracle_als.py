@@ -229,3 +229,23 @@ def test_als_gauge_invariant_warm_start():
     Xa = O.round_als(Yt, 0.0, 3, 4, 0.0, warm=W)[0]
     Xb = O.round_als(Yt, 0.0, 3, 4, 0.0, warm=W2)[0]
     assert _rel(Xa.to_dense(), Xb.to_dense()) <= 1e-11
+
+
+def test_frob_norm_apply_vs_dense():
+    # include/core/algebra/tt_matrix_frob_norm_apply.hpp: ||A x|| and ||A B||
+    # against the dense products.
+    A = O.OTT.random_matrix([2, 3, 2], [3, 2, 2], 3, 17)
+    x = O.OTT.random([3, 2, 2], 4, 18)
+    B = O.OTT.random_matrix([3, 2, 2], [2, 2, 3], 2, 19)
+
+    def dense_mat(M):
+        rows, cols = int(np.prod(M.m)), int(np.prod(M.n))
+        T = M.to_dense().reshape([v for pair in zip(M.m, M.n) for v in pair])
+        d = len(M.m)
+        return T.transpose(list(range(0, 2 * d, 2)) + list(range(1, 2 * d, 2))).reshape(rows, cols)
+
+    Ad = dense_mat(A)
+    want = np.linalg.norm(Ad @ x.to_dense())
+    assert abs(O.frob_norm_apply(A, x) - want) <= 1e-13 * want
+    want = np.linalg.norm(Ad @ dense_mat(B))
+    assert abs(O.frob_norm_apply(A, B) - want) <= 1e-13 * want
