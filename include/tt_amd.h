@@ -303,6 +303,19 @@ int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* t
  * tta_prof_enable and the sweeps they ran. */
 int tta_prof_jacobi_sweeps(tta_ctx* ctx, long long* calls, long long* sweeps);
 
+/* ---- full-rank certificate of the SVD steps (no reference counterpart) ---- */
+/* Before the Jacobi SVD of an SVD step of round / round_right_canonical, the
+ * step's triangular factor R is inverted on the device; when
+ * 1/||R^-1||_F (a lower bound on sigma_min) exceeds delta with margin and
+ * ||R|| ||R^-1|| <= 1e8, the rank rule provably keeps every singular value,
+ * and the step keeps the tensor as Q R without the SVD (same ranks, same
+ * tensor, another orthogonal gauge of the cores).  On by default;
+ * on = 0 always runs the SVD. */
+int tta_set_rank_certificate(tta_ctx* ctx, int on);
+/* Steps where the certificate was tried and where it held, since the last
+ * tta_prof_enable. */
+int tta_prof_rank_certificates(tta_ctx* ctx, long long* tried, long long* held);
+
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks
  * (tensor cores and workspaces), bytes cached for reuse, device allocations

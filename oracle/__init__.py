@@ -86,6 +86,8 @@ def lib():
         L.ora_round_dmrg.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_double, _vp,
                                      ctypes.c_int, ctypes.c_ulonglong, _i32p, _f64p, _i32p]
         L.ora_round_dmrg.restype = _vp
+        L.ora_set_svd_extended.argtypes = [ctypes.c_int]
+        L.ora_set_svd_extended.restype = None
         L.ora_soft_threshold.argtypes = [_vp, ctypes.c_double]
         L.ora_soft_threshold.restype = _vp
         L.ora_from_dense.argtypes = [_f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int]
@@ -259,6 +261,13 @@ def frob_norm_apply(A, X):
     if A.d == 0:
         return 0.0
     return norm(matmat(A, X) if X.m is not None else matvec(A, X))
+
+
+def set_svd_extended(on: bool) -> None:
+    """Accumulate the Jacobi SVD in long double (default) or in FP64.  The
+    FP64 form is the reference's arithmetic, for timing; the extended form
+    keeps the oracle's own rounding far below what it checks."""
+    lib().ora_set_svd_extended(1 if on else 0)
 
 
 def round_als(a, eps=0.0, max_rank=0, max_iters=20, tol=1e-12, warm=None, cold_random=False, seed=0):

@@ -185,22 +185,27 @@ static void qr_thin(const double* M, int rows, int cols, mat& Q, mat& R)
 // One-sided Jacobi on the columns of X (p x q, column-major, columns
 // contiguous), accumulating the rotations in J (q x q, column-major).
 // On return the columns of X are mutually orthogonal and X_in * J = X_out.
-static void jacobi_columns(std::vector<double>& X, int p, int q, std::vector<double>& J)
+// T is the accumulation type: the rotations are applied to X and J in T, so
+// with long double (x87 extended, u = 2^-64) the product X_out J^T returns
+// X_in to far below FP64 rounding however many rotations ran; the stopping
+// rule is the FP64 one in either case.
+template <typename T>
+static void jacobi_columns(std::vector<T>& X, int p, int q, std::vector<T>& J)
 {
-  J.assign((size_t)q * q, 0.0);
-  for (int i = 0; i < q; ++i) J[(size_t)i * q + i] = 1.0;
+  J.assign((size_t)q * q, T(0));
+  for (int i = 0; i < q; ++i) J[(size_t)i * q + i] = T(1);
   // Rotate while |x_i . x_j| > tol * ||x_i|| ||x_j||, tol = max(1e-15, sqrt(p) u)
   // (the dot product itself carries ~sqrt(p) u relative rounding).
-  const double tol = std::max(1.0e-15, std::sqrt((double)p) * 2.220446049250313e-16);
+  const T tol = (T)std::max(1.0e-15, std::sqrt((double)p) * 2.220446049250313e-16);
   for (int sweep = 0; sweep < 80; ++sweep)
   {
     bool rotated = false;
     for (int i = 0; i < q - 1; ++i)
       for (int j = i + 1; j < q; ++j)
       {
-        double* xi = &X[(size_t)i * p];
-        double* xj = &X[(size_t)j * p];
-        double a = 0.0, b = 0.0, g = 0.0;
+        T* xi = &X[(size_t)i * p];
+        T* xj = &X[(size_t)j * p];
+        T a = 0, b = 0, g = 0;
         for (int t = 0; t < p; ++t)
         {
           a += xi[t] * xi[t];
@@ -210,31 +215,93 @@ static void jacobi_columns(std::vector<double>& X, int p, int q, std::vector<dou
         // No rotation when the pair is orthogonal to tol, a squared norm
         // underflowed to zero, or the angle rounds to zero (|zeta| > 1e150
         // uses the asymptotic tangent 1 / (2 zeta)).
-        if (a == 0.0 || b == 0.0 || !(std::fabs(g) > tol * std::sqrt(a) * std::sqrt(b))) continue;
-        const double zeta = (b - a) / (2.0 * g);
-        const double az = std::fabs(zeta);
-        const double t =
-            (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + std::sqrt(1.0 + zeta * zeta));
-        const double cs = 1.0 / std::sqrt(1.0 + t * t);
-        const double sn = cs * t;
-        if (sn == 0.0) continue;
+        using std::fabs;
+        using std::sqrt;
+        if (a == T(0) || b == T(0) || !(fabs(g) > tol * sqrt(a) * sqrt(b))) continue;
+        const T zeta = (b - a) / (T(2) * g);
+        const T az = fabs(zeta);
+        const T t = (zeta >= T(0) ? T(1) : T(-1)) / (az > T(1.0e150) ? T(2) * az : az + sqrt(T(1) + zeta * zeta));
+        const T cs = T(1) / sqrt(T(1) + t * t);
+        const T sn = cs * t;
+        if (sn == T(0)) continue;
         rotated = true;
         for (int u = 0; u < p; ++u)
         {
-          const double x1 = xi[u], x2 = xj[u];
+          const T x1 = xi[u], x2 = xj[u];
           xi[u] = cs * x1 - sn * x2;
           xj[u] = sn * x1 + cs * x2;
         }
-        double* ji = &J[(size_t)i * q];
-        double* jj = &J[(size_t)j * q];
+        T* ji = &J[(size_t)i * q];
+        T* jj = &J[(size_t)j * q];
         for (int u = 0; u < q; ++u)
         {
-          const double y1 = ji[u], y2 = jj[u];
+          const T y1 = ji[u], y2 = jj[u];
           ji[u] = cs * y1 - sn * y2;
           jj[u] = sn * y1 + cs * y2;
         }
       }
     if (!rotated) break;
+  }
+}
+
+// Extended-precision accumulation of the SVD (on by default; the FP64 form
+// is what a timing of the reference's arithmetic runs).
+static bool g_svd_extended = true;
+
+template <typename T>
+static void svd_thin_t(const double* M, int rows, int cols, mat& U, std::vector<double>& s, mat& Vt)
+{
+  const int k = std::min(rows, cols);
+  std::vector<T> X, J;
+  std::vector<T> norms(k);
+  U = mat(rows, k);
+  Vt = mat(k, cols);
+  s.assign(k, 0.0);
+  std::vector<int> perm(k);
+  std::iota(perm.begin(), perm.end(), 0);
+  if (rows <= cols)
+  {
+    // X = M^T (cols x rows) column-major == M row-major: columns = rows of M.
+    X.assign(M, M + (size_t)rows * cols);
+    jacobi_columns(X, cols, rows, J);
+    for (int i = 0; i < k; ++i)
+    {
+      T nn = 0;
+      for (int t = 0; t < cols; ++t) nn += X[(size_t)i * cols + t] * X[(size_t)i * cols + t];
+      norms[i] = std::sqrt(nn);
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return norms[a] > norms[b]; });
+    for (int o = 0; o < k; ++o)
+    {
+      const int i = perm[o];
+      s[o] = (double)norms[i];
+      for (int t = 0; t < rows; ++t) U(t, o) = (double)J[(size_t)i * rows + t];
+      const T inv = norms[i] > T(0) ? T(1) / norms[i] : T(0);
+      for (int t = 0; t < cols; ++t) Vt(o, t) = (double)(X[(size_t)i * cols + t] * inv);
+    }
+  }
+  else
+  {
+    // X = M (rows x cols) column-major: columns = columns of M.
+    X.assign((size_t)rows * cols, T(0));
+    for (int i = 0; i < rows; ++i)
+      for (int j = 0; j < cols; ++j) X[(size_t)j * rows + i] = M[(size_t)i * cols + j];
+    jacobi_columns(X, rows, cols, J);
+    for (int i = 0; i < k; ++i)
+    {
+      T nn = 0;
+      for (int t = 0; t < rows; ++t) nn += X[(size_t)i * rows + t] * X[(size_t)i * rows + t];
+      norms[i] = std::sqrt(nn);
+    }
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return norms[a] > norms[b]; });
+    for (int o = 0; o < k; ++o)
+    {
+      const int i = perm[o];
+      s[o] = (double)norms[i];
+      const T inv = norms[i] > T(0) ? T(1) / norms[i] : T(0);
+      for (int t = 0; t < rows; ++t) U(t, o) = (double)(X[(size_t)i * rows + t] * inv);
+      for (int t = 0; t < cols; ++t) Vt(o, t) = (double)J[(size_t)i * cols + t];
+    }
   }
 }
 
@@ -252,58 +319,10 @@ static void svd_thin(const double* M, int rows, int cols, mat& U, std::vector<do
     Vt = mat();
     return;
   }
-  const int k = std::min(rows, cols);
-  std::vector<double> X, J;
-  std::vector<double> norms(k);
-  U = mat(rows, k);
-  Vt = mat(k, cols);
-  s.assign(k, 0.0);
-  std::vector<int> perm(k);
-  std::iota(perm.begin(), perm.end(), 0);
-  if (rows <= cols)
-  {
-    // X = M^T (cols x rows) column-major == M row-major: columns = rows of M.
-    X.assign(M, M + (size_t)rows * cols);
-    jacobi_columns(X, cols, rows, J);
-    for (int i = 0; i < k; ++i)
-    {
-      double nn = 0.0;
-      for (int t = 0; t < cols; ++t) nn += X[(size_t)i * cols + t] * X[(size_t)i * cols + t];
-      norms[i] = std::sqrt(nn);
-    }
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return norms[a] > norms[b]; });
-    for (int o = 0; o < k; ++o)
-    {
-      const int i = perm[o];
-      s[o] = norms[i];
-      for (int t = 0; t < rows; ++t) U(t, o) = J[(size_t)i * rows + t];
-      const double inv = norms[i] > 0.0 ? 1.0 / norms[i] : 0.0;
-      for (int t = 0; t < cols; ++t) Vt(o, t) = X[(size_t)i * cols + t] * inv;
-    }
-  }
+  if (g_svd_extended)
+    svd_thin_t<long double>(M, rows, cols, U, s, Vt);
   else
-  {
-    // X = M (rows x cols) column-major: columns = columns of M.
-    X.assign((size_t)rows * cols, 0.0);
-    for (int i = 0; i < rows; ++i)
-      for (int j = 0; j < cols; ++j) X[(size_t)j * rows + i] = M[(size_t)i * cols + j];
-    jacobi_columns(X, rows, cols, J);
-    for (int i = 0; i < k; ++i)
-    {
-      double nn = 0.0;
-      for (int t = 0; t < rows; ++t) nn += X[(size_t)i * rows + t] * X[(size_t)i * rows + t];
-      norms[i] = std::sqrt(nn);
-    }
-    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return norms[a] > norms[b]; });
-    for (int o = 0; o < k; ++o)
-    {
-      const int i = perm[o];
-      s[o] = norms[i];
-      const double inv = norms[i] > 0.0 ? 1.0 / norms[i] : 0.0;
-      for (int t = 0; t < rows; ++t) U(t, o) = X[(size_t)i * rows + t] * inv;
-      for (int t = 0; t < cols; ++t) Vt(o, t) = J[(size_t)i * cols + t];
-    }
-  }
+    svd_thin_t<double>(M, rows, cols, U, s, Vt);
 }
 
 // Restates detail::truncate_eps_rank (tt_eigen_bridge.hpp:172-197): walk
@@ -1639,6 +1658,7 @@ extern "C"
     std::memcpy(Q, q.v.data(), sizeof(double) * q.v.size());
     std::memcpy(R, r.v.data(), sizeof(double) * r.v.size());
   }
+  void ora_set_svd_extended(int on) { ora::g_svd_extended = on != 0; }
   void ora_svd_thin(const double* M, int rows, int cols, double* U, double* s, double* Vt)
   {
     ora::mat u, vt;

@@ -89,6 +89,17 @@ class Context:
         self.check(self.lib.tta_prof_jacobi_sweeps(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_rank_certificate(self, on: bool) -> None:
+        """Full-rank certificate of the SVD steps (on by default): a step whose
+        triangular factor provably keeps every singular value skips its SVD."""
+        self.check(self.lib.tta_set_rank_certificate(self.h, 1 if on else 0))
+
+    def rank_certificates(self):
+        """(tried, held) certificate counts since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_rank_certificates(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def mem_stats(self):
         """{live_bytes, cached_bytes, device_allocs, trims} of the context's
         caching device allocator."""

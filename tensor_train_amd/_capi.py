@@ -92,6 +92,9 @@ SIGNATURES = {
     "tta_scale_core": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.c_double]),
     "tta_prof_enable": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),
+    "tta_set_rank_certificate": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tta_prof_rank_certificates": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                                  ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_jacobi_sweeps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tta_soft_threshold": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
     "tta_from_dense": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int, ctypes.c_int,

@@ -181,7 +181,8 @@ enum
   WS_M1P,
   WS_PART,
   WS_SMALL,
-  WS_T
+  WS_T,
+  WS_CERT
 };
 
 // Householder QR of a column-major (m x n) matrix in place, thin Q (m x k)
@@ -349,6 +350,80 @@ int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_ra
   return keep_all ? q : read_int(c, ints);
 }
 
+// C (M x N) = alpha A B + beta C with element strides: A(i,k) =
+// A[i sar + k sac], B(k,j) = B[k sbr + j sbc], C(i,j) = C[i scr + j scc].
+void gemm_s(tta_ctx* c, int M, int N, int K, const double* A, long sar, long sac, const double* B, long sbr, long sbc,
+            double* C, long scr, long scc, double alpha = 1.0, double beta = 0.0)
+{
+  gemm_desc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sar = sar;
+  g.sac = sac;
+  g.B = B;
+  g.sbr = sbr;
+  g.sbc = sbc;
+  g.C = C;
+  g.scr = scr;
+  g.scc = scc;
+  g.alpha = alpha;
+  g.beta = beta;
+  run_gemm(c, g);
+}
+
+// One Newton-Schulz step toward the nearest orthogonal matrix, J (q x q,
+// column-major) <- J (3 I - J^T J) / 2.  The accumulated Jacobi rotations
+// leave J orthogonal only to ~1e-13 after thousands of them; the step
+// squares that departure, down to the two GEMMs' own rounding.
+void polar_correct(tta_ctx* c, double* J, int q)
+{
+  double* G = alloc_core(c, (size_t)q * q);
+  double* J0 = alloc_core(c, (size_t)q * q);
+  hip_check(hipMemcpyAsync(J0, J, (size_t)q * q * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+  gemm_s(c, q, q, q, J0, q, 1, J0, 1, q, G, q, 1);               // G = J^T J
+  gemm_s(c, q, q, q, J0, 1, q, G, q, 1, J, 1, q, -0.5, 1.5);     // J = 1.5 J - 0.5 J G
+  c->mem.put(G);
+  c->mem.put(J0);
+}
+
+// Certificate that the rank rule keeps all q singular values of the
+// upper-triangular R (row-major in `R`): with X = R^-1 formed on the device,
+// sigma_min(R) >= 1 / ||X||_F, and the rule (truncate_eps_rank,
+// tt_eigen_bridge.hpp:172-197) keeps every value once sigma_min > delta.
+// Margins: the condition number ||R||_F ||X||_F is capped at 1e8, so the
+// inverse's relative rounding error (<= q u kappa) and the difference between
+// this R and the one of any backward-stable factorization (<= q u ||M||,
+// i.e. <= 1e-5 sigma_min) stay far below the 1e-3 slack taken on each side.
+// Non-finite norms (a zero pivot) fail the certificate.
+bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta)
+{
+  if (!c->rank_certificate || q < 1 || q > kRinvCertMaxQ) return false;
+  const int nb = rinv_cert_blocks(q);
+  double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
+  c->cert_tried += 1;
+  {
+    prof_scope ps(c, TTA_K_OTHER, (double)q * q * q / 3.0, 8.0 * 1.5 * (double)q * q);
+    launch_rinv_cert(R, q, part, c->stream);
+  }
+  std::vector<double> h(2 * (size_t)nb);
+  hip_check(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+  double xf2 = 0.0, rf2 = 0.0;
+  for (int b = 0; b < nb; ++b)
+  {
+    xf2 += h[2 * (size_t)b];
+    rf2 += h[2 * (size_t)b + 1];
+  }
+  if (!std::isfinite(xf2) || !std::isfinite(rf2) || !(xf2 > 0.0) || !(rf2 > 0.0)) return false;
+  const double xf = std::sqrt(xf2), rf = std::sqrt(rf2);
+  if (rf * xf > 1.0e8) return false;
+  const bool held = 1.0 / (xf * (1.0 + 1.0e-3)) > delta * (1.0 + 1.0e-3);
+  c->cert_held += held ? 1 : 0;
+  return held;
+}
+
 // Thin SVD of a row-major (rows x cols) matrix M, truncated on the device:
 //   U_head (rows x r_new) row-major with orthonormal columns (products of
 //          orthogonal factors, exact up to rounding), in a fresh pool block;
@@ -364,11 +439,21 @@ int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_ra
 // keep_all (every singular triplet, zero ones included) keeps, for
 // rows < cols, the exact-rotation form R J = W -> M = J (Q W)^T, whose U is
 // orthonormal however many singular values vanish.
+// With certify (and no singular values asked for, no binding max_rank), a
+// step whose full rank is certified (certify_full_rank) keeps the same
+// tensor without the Jacobi SVD: U = Q, F = R for rows >= cols, and
+// U = I, F = M for rows < cols.
+// F is formed from the exactly orthonormal factor by one GEMM, F = J'^T R
+// (rows >= cols, J' the polar-corrected rotations) or F = U^T M
+// (rows < cols, U = W diag(s)^-1 orthonormal to the Jacobi tolerance), so
+// that U F reproduces M (or its projection on the kept vectors) to one
+// GEMM's rounding however many rotations the Jacobi applied.
 int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int max_rank, bool keep_all,
-             double** U_out, double** F_out, double** s_out = nullptr)
+             double** U_out, double** F_out, double** s_out = nullptr, bool certify = false)
 {
   double* sv = nullptr;
   int* perm = nullptr;
+  const bool try_cert = certify && !keep_all && !s_out && (max_rank <= 0 || max_rank >= std::min(rows, cols));
   if (rows >= cols)
   {
     const int q = cols;
@@ -379,27 +464,26 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     qr_factor(c, A, rows, rows, cols, tau, Q, rows);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+    if (try_cert && certify_full_rank(c, X, q, delta))
+    {
+      double* U = alloc_core(c, (size_t)rows * q);
+      launch_transpose(Q, rows, q, rows, U, q, c->stream);
+      *U_out = U;
+      *F_out = X;
+      return q;
+    }
     double* J = ctx_ws(c, WS_J, (size_t)q * q);
     const int r_new = square_svd(c, X, q, J, delta, max_rank, keep_all, &sv, &perm);
+    polar_correct(c, J, q);
     double* Jp = ctx_ws(c, WS_JP, (size_t)q * r_new);
     launch_gather_cols(J, q, q, perm, r_new, Jp, 1, q, c->stream);
     double* U = alloc_core(c, (size_t)rows * r_new);
-    gemm_desc g;
-    g.M = rows;
-    g.N = r_new;
-    g.K = q;
-    g.A = Q;
-    g.sar = 1;
-    g.sac = rows;
-    g.B = Jp;
-    g.sbr = 1;
-    g.sbc = q;
-    g.C = U;
-    g.scr = r_new;
-    g.scc = 1;
-    run_gemm(c, g);
+    gemm_s(c, rows, r_new, q, Q, 1, rows, Jp, 1, q, U, r_new, 1);  // U = Q J'_k
+    double* Rb = alloc_core(c, (size_t)q * q);
+    launch_extract_upper(A, rows, q, q, Rb, q, 1, c->stream);      // R row-major
     double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
-    launch_gather_cols(X, q, q, perm, r_new, F, 1, q, c->stream);
+    gemm_s(c, r_new, q, q, Jp, q, 1, Rb, q, 1, F, cols, 1);         // F = J'_k^T R
+    c->mem.put(Rb);
     *U_out = U;
     *F_out = F;
     if (s_out) *s_out = sv;
@@ -416,35 +500,28 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);  // Y = R
   else
     launch_extract_upper(A, cols, q, q, Y, q, 1, c->stream);  // Y = R^T
+  if (try_cert && certify_full_rank(c, Y, q, delta))
+  {
+    double* U = alloc_core(c, (size_t)q * q);
+    launch_eye(U, q, q, c->stream);
+    double* F = ctx_ws(c, WS_M1P, (size_t)rows * cols);
+    hip_check(hipMemcpyAsync(F, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    *U_out = U;
+    *F_out = F;
+    return q;
+  }
   double* J = ctx_ws(c, WS_J, (size_t)q * q);
   const int r_new = square_svd(c, Y, q, J, delta, max_rank, keep_all, &sv, &perm);
   double* U = alloc_core(c, (size_t)rows * r_new);
-  double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);  // left factor of F^T
   if (keep_all)
   {
+    polar_correct(c, J, q);
     launch_gather_cols(J, q, q, perm, r_new, U, r_new, 1, c->stream);
-    launch_gather_cols(Y, q, q, perm, r_new, Wp, 1, q, c->stream);
   }
   else
-  {
     launch_gather_cols(Y, q, q, perm, r_new, U, r_new, 1, c->stream, sv, GATHER_DIV);
-    launch_gather_cols(J, q, q, perm, r_new, Wp, 1, q, c->stream, sv, GATHER_MUL);
-  }
   double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
-  gemm_desc g;
-  g.M = r_new;
-  g.N = cols;
-  g.K = q;
-  g.A = Wp;
-  g.sar = q;
-  g.sac = 1;
-  g.B = Q;
-  g.sbr = cols;
-  g.sbc = 1;
-  g.C = F;
-  g.scr = cols;
-  g.scc = 1;
-  run_gemm(c, g);
+  gemm_s(c, r_new, cols, q, U, 1, r_new, M, cols, 1, F, cols, 1);  // F = U^T M
   *U_out = U;
   *F_out = F;
   if (s_out) *s_out = sv;
@@ -524,7 +601,7 @@ void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int rows = r * n, cols = rp;
   double *U = nullptr, *F = nullptr;
-  const int r_new = svd_left(c, t->cores[k], rows, cols, delta, max_rank, false, &U, &F);
+  const int r_new = svd_left(c, t->cores[k], rows, cols, delta, max_rank, false, &U, &F, nullptr, true);
   const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
   double* Nn = alloc_core(c, (size_t)r_new * ncols_n);
   gemm_rm(c, r_new, ncols_n, cols, F, cols, t->cores[k + 1], ncols_n, Nn, ncols_n);
@@ -533,12 +610,29 @@ void svd_step(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
   t->ranks[k + 1] = r_new;
 }
 
+// A certificate applies when max_rank cannot force a cut.
+bool cert_ok(int max_rank, int q) { return max_rank <= 0 || max_rank >= q; }
+
+// Last part of svd_step_right: core k-1 <- core k-1 F (rows x r_new), core k
+// <- Cn.
+void absorb_right_step(tta_ctx* c, tta_tt* t, int k, int rows, int r_new, double* Cn, const double* F)
+{
+  const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
+  double* Pn = alloc_core(c, (size_t)rows_p * r_new);
+  gemm_rm(c, rows_p, r_new, rows, t->cores[k - 1], rows, F, r_new, Pn, r_new);
+  replace_core(c, t, k - 1, Pn);
+  replace_core(c, t, k, Cn);
+  t->ranks[k] = r_new;
+}
+
 // svd_step_right (truncate.hpp:114-166): SVD of right_unfold(core k)
 // (r x n*rp) with V orthonormal (the new core's rows) and U diag(s)
 // absorbed into core k-1.  Jacobi runs on R^T as in svd_left:
 //   cols >= rows: M^T = Q R, R^T J = W -> M = W (Q J)^T: V = Q J, U s = W;
 //   cols <  rows: M = Q R,   R^T J = W -> M = (Q J) W^T: V = W diag(s)^-1,
 //                 U s = Q J diag(s).
+// As in svd_left, the absorbed factor is formed by one GEMM from the exactly
+// orthonormal one: U s = R^T J' (J' polar-corrected) or U s = M V.
 void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
 {
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
@@ -557,27 +651,30 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     qr_factor(c, A, cols, cols, rows, tau, Q, cols);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
+    if (cert_ok(max_rank, q) && certify_full_rank(c, X, q, delta))
+    {
+      // M = R^T Q^T: the new core is Q^T (the column-major Q read row-major),
+      // and R^T is absorbed into core k-1.
+      r_new = q;
+      Cn = alloc_core(c, (size_t)q * cols);
+      hip_check(hipMemcpyAsync(Cn, Q, (size_t)q * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+      F = ctx_ws(c, WS_M1P, (size_t)rows * q);
+      launch_transpose(X, q, q, q, F, q, c->stream);
+      absorb_right_step(c, t, k, rows, r_new, Cn, F);
+      return;
+    }
     double* J = ctx_ws(c, WS_J, (size_t)q * q);
     r_new = square_svd(c, X, q, J, delta, max_rank, false, &sv, &perm);
+    polar_correct(c, J, q);
     double* Jp = ctx_ws(c, WS_JP, (size_t)q * r_new);
     launch_gather_cols(J, q, q, perm, r_new, Jp, 1, q, c->stream);
     Cn = alloc_core(c, (size_t)r_new * cols);
-    gemm_desc g;
-    g.M = r_new;
-    g.N = cols;
-    g.K = q;
-    g.A = Jp;
-    g.sar = q;
-    g.sac = 1;
-    g.B = Q;
-    g.sbr = cols;
-    g.sbc = 1;
-    g.C = Cn;
-    g.scr = cols;
-    g.scc = 1;
-    run_gemm(c, g);
+    gemm_s(c, r_new, cols, q, Jp, q, 1, Q, cols, 1, Cn, cols, 1);  // Cn = J'_k^T Q^T
+    double* Rb = alloc_core(c, (size_t)q * q);
+    launch_extract_upper(A, cols, q, q, Rb, q, 1, c->stream);       // R row-major
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
-    launch_gather_cols(X, q, q, perm, r_new, F, r_new, 1, c->stream);
+    gemm_s(c, rows, r_new, q, Rb, 1, q, Jp, 1, q, F, r_new, 1);     // F = R^T J'_k
+    c->mem.put(Rb);
   }
   else
   {
@@ -589,34 +686,26 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     qr_factor(c, A, rows, rows, cols, tau, Q, rows);
     double* Y = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream);  // Y = R^T
+    if (cert_ok(max_rank, q) && certify_full_rank(c, Y, q, delta))
+    {
+      // M = M I: the new core is the identity and M is absorbed into core k-1.
+      r_new = q;
+      Cn = alloc_core(c, (size_t)q * q);
+      launch_eye(Cn, q, q, c->stream);
+      F = ctx_ws(c, WS_M1P, (size_t)rows * cols);
+      hip_check(hipMemcpyAsync(F, t->cores[k], (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream),
+                "hipMemcpyAsync");
+      absorb_right_step(c, t, k, rows, r_new, Cn, F);
+      return;
+    }
     double* J = ctx_ws(c, WS_J, (size_t)q * q);
     r_new = square_svd(c, Y, q, J, delta, max_rank, false, &sv, &perm);
     Cn = alloc_core(c, (size_t)r_new * cols);
     launch_gather_cols(Y, q, q, perm, r_new, Cn, 1, q, c->stream, sv, GATHER_DIV);
-    double* Wp = ctx_ws(c, WS_JP, (size_t)q * r_new);
-    launch_gather_cols(J, q, q, perm, r_new, Wp, 1, q, c->stream, sv, GATHER_MUL);
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
-    gemm_desc g;
-    g.M = rows;
-    g.N = r_new;
-    g.K = q;
-    g.A = Q;
-    g.sar = 1;
-    g.sac = rows;
-    g.B = Wp;
-    g.sbr = 1;
-    g.sbc = q;
-    g.C = F;
-    g.scr = r_new;
-    g.scc = 1;
-    run_gemm(c, g);
+    gemm_s(c, rows, r_new, cols, t->cores[k], cols, 1, Cn, 1, cols, F, r_new, 1);  // F = M Cn^T
   }
-  const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
-  double* Pn = alloc_core(c, (size_t)rows_p * r_new);
-  gemm_rm(c, rows_p, r_new, rows, t->cores[k - 1], rows, F, r_new, Pn, r_new);
-  replace_core(c, t, k - 1, Pn);
-  replace_core(c, t, k, Cn);
-  t->ranks[k] = r_new;
+  absorb_right_step(c, t, k, rows, r_new, Cn, F);
 }
 
 void right_orthogonalize_impl(tta_ctx* c, tta_tt* t)
@@ -2227,7 +2316,24 @@ int tta_prof_enable(tta_ctx* c, int on)
   }
   c->jac_calls = 0;
   c->jac_sweeps = 0;
+  c->cert_tried = 0;
+  c->cert_held = 0;
   TTA_GUARD_END(c)
+}
+
+int tta_set_rank_certificate(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->rank_certificate = on != 0;
+  return TTA_OK;
+}
+
+int tta_prof_rank_certificates(tta_ctx* c, long long* tried, long long* held)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (tried) *tried = c->cert_tried;
+  if (held) *held = c->cert_held;
+  return TTA_OK;
 }
 
 int tta_prof_jacobi_sweeps(tta_ctx* c, long long* calls, long long* sweeps)

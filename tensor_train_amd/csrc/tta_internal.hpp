@@ -104,6 +104,10 @@ struct tta_ctx
   // Block-Jacobi launches (q > 64) since the last tta_prof_enable and the
   // block sweeps they ran (host-driven, so counted whether or not timing is on).
   long long jac_calls = 0, jac_sweeps = 0;
+  // Full-rank certificate of the SVD steps (on by default): steps where it
+  // was tried, and where it held so that the Jacobi SVD was not needed.
+  bool rank_certificate = true;
+  long long cert_tried = 0, cert_held = 0;
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])

@@ -584,6 +584,98 @@ int grid_for(long n, int bs = 256)
   if (g > 65535L * 4) g = 65535L * 4;
   return (int)(g < 1 ? 1 : g);
 }
+
+// Certificate for the rank rule: workgroup J forms columns [16J, 16J + 16)
+// of X = R^-1 for an upper-triangular R (row-major, q x q) by blocked back
+// substitution, row blocks I = J down to 0.  The off-diagonal sums
+// B = E - R(I, I+1:) X(I+1:, J) run on all 256 threads from LDS (R staged in
+// 128-column chunks, X(:, J) resident); the 16 x 16 diagonal solve runs on
+// one thread per column.  out[2J] = ||X(:, J)||_F^2, out[2J+1] =
+// ||R(:, J)||_F^2; a zero or non-finite pivot propagates as inf / NaN.
+constexpr int kCertB = 16, kCertChunk = 128;
+
+__global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict__ R, int q, double* __restrict__ out)
+{
+  extern __shared__ double lds[];
+  const int J = blockIdx.x;
+  const int j0 = J * kCertB;
+  const int jn = min(kCertB, q - j0);
+  const int rows = j0 + jn;
+  double* Xs = lds;                              // rows x 16
+  double* Rs = Xs + (size_t)rows * kCertB;       // 16 x kCertChunk
+  double* Bs = Rs + kCertB * kCertChunk;         // 16 x 16
+  double* red = Bs + kCertB * kCertB;            // 4 waves
+  const int t = threadIdx.x, r = t / kCertB, cc = t % kCertB;
+  for (int e = t; e < rows * kCertB; e += 256) Xs[e] = 0.0;
+  __syncthreads();
+  for (int I = J; I >= 0; --I)
+  {
+    const int i0 = I * kCertB;
+    const int in = min(kCertB, q - i0);
+    const int i1 = i0 + in;
+    double acc = (r < in && cc < jn && i0 + r == j0 + cc) ? 1.0 : 0.0;
+    for (int k0 = i1; k0 < rows; k0 += kCertChunk)
+    {
+      const int kw = min(kCertChunk, rows - k0);
+      for (int e = t; e < kCertB * kCertChunk; e += 256)
+      {
+        const int rr = e / kCertChunk, kk = e % kCertChunk;
+        Rs[e] = (rr < in && kk < kw) ? R[(size_t)(i0 + rr) * q + k0 + kk] : 0.0;
+      }
+      __syncthreads();
+      if (r < in && cc < jn)
+        for (int kk = 0; kk < kw; ++kk) acc -= Rs[r * kCertChunk + kk] * Xs[(size_t)(k0 + kk) * kCertB + cc];
+      __syncthreads();
+    }
+    Bs[r * kCertB + cc] = acc;
+    __syncthreads();
+    if (t < jn)
+    {
+      for (int rr = in - 1; rr >= 0; --rr)
+      {
+        const double* Rrow = R + (size_t)(i0 + rr) * q + i0;
+        double v = Bs[rr * kCertB + t];
+        for (int k = rr + 1; k < in; ++k) v -= Rrow[k] * Xs[(size_t)(i0 + k) * kCertB + t];
+        Xs[(size_t)(i0 + rr) * kCertB + t] = v / Rrow[rr];
+      }
+    }
+    __syncthreads();
+  }
+  double sx = 0.0, sr = 0.0;
+  for (int e = t; e < rows * kCertB; e += 256)
+  {
+    const int row = e / kCertB, col = e % kCertB;
+    if (col < jn)
+    {
+      const double x = Xs[e];
+      sx += x * x;
+      if (row <= j0 + col)
+      {
+        const double v = R[(size_t)row * q + j0 + col];
+        sr += v * v;
+      }
+    }
+  }
+  sx = block_sum<256>(sx, red);
+  sr = block_sum<256>(sr, red);
+  if (t == 0)
+  {
+    out[2 * J] = sx;
+    out[2 * J + 1] = sr;
+  }
+}
+
+// dst (n x n, leading dimension ld) = identity.
+__global__ void eye_kernel(double* dst, int n, long ld)
+{
+  const long total = (long)n * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int i = (int)(e / n), j = (int)(e % n);
+    dst[(size_t)i * ld + j] = i == j ? 1.0 : 0.0;
+  }
+}
+
 }  // namespace
 
 // ------------------------------- launchers --------------------------------
@@ -641,6 +733,22 @@ void launch_gather_cols(const double* src, int lds_, int rows, const int* perm, 
 {
   hipLaunchKernelGGL(gather_cols_kernel, dim3(grid_for((long)rows * ncols)), dim3(256), 0, st, src, lds_, rows, perm,
                      ncols, dst, drs, dcs, scale, mode);
+}
+
+int rinv_cert_blocks(int q) { return (q + kCertB - 1) / kCertB; }
+
+void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st)
+{
+  const size_t lds = sizeof(double) * ((size_t)q * kCertB + kCertB * kCertChunk + kCertB * kCertB + 4);
+  hipLaunchKernelGGL(rinv_cert_kernel, dim3(rinv_cert_blocks(q)), dim3(256), lds, st, R, q, out);
+}
+
+void launch_eye(double* dst, int n, long ld, hipStream_t st)
+{
+  const long total = (long)n * n;
+  long g = (total + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(eye_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, dst, n, ld);
 }
 
 void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st)

@@ -231,6 +231,14 @@ void launch_gather_cols(const double* src, int lds, int rows, const int* perm, i
                         long dcs, hipStream_t st,
                         const double* scale = nullptr, int mode = GATHER_COPY);
 void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st);
+// Squared Frobenius norms of R^-1 and R per 16-column block (2 doubles per
+// block, rinv_cert_blocks(q) blocks) for an upper-triangular row-major R,
+// q <= kRinvCertMaxQ.
+constexpr int kRinvCertMaxQ = 1024;
+int rinv_cert_blocks(int q);
+void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st);
+// dst (n x n, leading dimension ld) = identity.
+void launch_eye(double* dst, int n, long ld, hipStream_t st);
 void launch_permute(const permute_desc& p, hipStream_t st);
 void launch_matvec_core(const double* A, const double* x, double* y, int r, int m, int n, int rp, int s, int sp,
                         hipStream_t st);
