@@ -316,6 +316,17 @@ int tta_set_rank_certificate(tta_ctx* ctx, int on);
  * tta_prof_enable. */
 int tta_prof_rank_certificates(tta_ctx* ctx, long long* tried, long long* held);
 
+/* ---- CholQR2 for the sweeps' QR factorizations (no reference counterpart) */
+/* Tall factorizations (m >= 2n, 32 <= n <= 512) of the QR and SVD sweeps run
+ * as CholQR2 (two Gram-Cholesky passes on the matrix cores) when it is
+ * accepted: no Cholesky breakdown and a second-pass factor within a factor 2
+ * of the identity, i.e. condition numbers up to about 1e7; otherwise, and
+ * with on = 0, blocked Householder.  Both give a backward-stable Q R. */
+int tta_set_cholqr(tta_ctx* ctx, int on);
+/* Factorizations where CholQR2 was tried and accepted, since the last
+ * tta_prof_enable. */
+int tta_prof_cholqr(tta_ctx* ctx, long long* tried, long long* used);
+
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks
  * (tensor cores and workspaces), bytes cached for reuse, device allocations

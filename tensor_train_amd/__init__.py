@@ -94,6 +94,17 @@ class Context:
         triangular factor provably keeps every singular value skips its SVD."""
         self.check(self.lib.tta_set_rank_certificate(self.h, 1 if on else 0))
 
+    def set_cholqr(self, on: bool) -> None:
+        """CholQR2 for the sweeps' tall QR factorizations (on by default; it
+        falls back to Householder wherever it would lose accuracy)."""
+        self.check(self.lib.tta_set_cholqr(self.h, 1 if on else 0))
+
+    def cholqr_counts(self):
+        """(tried, used) CholQR2 counts since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_cholqr(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def rank_certificates(self):
         """(tried, held) certificate counts since the last prof_enable."""
         a, b = ctypes.c_longlong(), ctypes.c_longlong()

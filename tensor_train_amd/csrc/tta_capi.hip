@@ -185,12 +185,87 @@ enum
   WS_CERT
 };
 
-// Householder QR of a column-major (m x n) matrix in place, thin Q (m x k)
-// into Q when non-null: blocked compact-WY kernels for m <= 2048, the
-// single-workgroup unblocked kernel beyond.
-void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq)
+// C = A B with element strides, without a profiling scope of its own (for
+// use inside a scope of another class).
+void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long sac, const double* B, long sbr,
+              long sbc, double* C, long scr, long scc)
+{
+  gemm_desc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sar = sar;
+  g.sac = sac;
+  g.B = B;
+  g.sbr = sbr;
+  g.sbc = sbc;
+  g.C = C;
+  g.scr = scr;
+  g.scc = scc;
+  launch_gemm(g, 1, c->stream);
+}
+
+// CholQR2 of a tall column-major (m x n) A, m >= 2n: G = A^T A = R1^T R1,
+// Q1 = A R1^-1, then once more on Q1 (G2 = Q1^T Q1 = R2^T R2, Q = Q1 R2^-1)
+// and R = R2 R1.  The Grams and products run on the matrix cores; the
+// Cholesky and the triangular inverses are single-workgroup kernels.  A is
+// read only until the factorization is accepted: no breakdown in either
+// pass and R2 within a factor 2 of the identity on its diagonal (Q1 well
+// conditioned, so the second pass restores orthogonality to rounding; this
+// holds for condition numbers up to about 1e7).  Then R goes to A's leading
+// n x n block (upper, zeros below) and Q to Q; otherwise nothing is written
+// and the caller factors A by Householder.
+bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
+{
+  const size_t nn = (size_t)n * n;
+  const int nb = rinv_cert_blocks(n);
+  double* G = alloc_core(c, nn);
+  double* X = alloc_core(c, nn);
+  double* R1 = alloc_core(c, nn);
+  double* Q1 = alloc_core(c, (size_t)m * n);
+  double* aux = alloc_core(c, 8 + 2 * (size_t)nb);
+  double h[8];
+  bool ok = false;
+  try
+  {
+    gemm_raw(c, n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
+    launch_chol(G, n, n, aux, c->stream);                    // G <- R1
+    launch_rinv_cert(G, n, aux + 8, c->stream, X, n);        // X = R1^-1
+    gemm_raw(c, m, n, n, A, 1, lda, X, n, 1, Q1, 1, m);      // Q1 = A X
+    hip_check(hipMemcpyAsync(R1, G, nn * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    gemm_raw(c, n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
+    launch_chol(G, n, n, aux + 4, c->stream);                // G <- R2
+    launch_rinv_cert(G, n, aux + 8, c->stream, X, n);        // X = R2^-1
+    if (Q) gemm_raw(c, m, n, n, Q1, 1, m, X, n, 1, Q, 1, ldq);  // Q = Q1 X
+    gemm_raw(c, n, n, n, G, n, 1, R1, n, 1, X, n, 1);        // X = R2 R1
+    hip_check(hipMemcpyAsync(h, aux, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    ok = h[0] == 0.0 && h[4] == 0.0 && h[5] >= 0.5 && h[6] <= 2.0 && std::isfinite(h[1]) && std::isfinite(h[2]);
+    if (ok) launch_transpose(X, n, n, n, A, lda, c->stream);
+  }
+  catch (...)
+  {
+    for (double* p : {G, X, R1, Q1, aux}) c->mem.put(p);
+    throw;
+  }
+  for (double* p : {G, X, R1, Q1, aux}) c->mem.put(p);
+  c->cholqr_tried += 1;
+  c->cholqr_used += ok ? 1 : 0;
+  return ok;
+}
+
+// QR of a column-major (m x n) matrix in place (R in the leading upper
+// triangle), thin Q (m x k) into Q when non-null.  With cholqr, tall
+// matrices (m >= 2n, 32 <= n <= chol_max_n()) go through CholQR2 first;
+// otherwise, and whenever CholQR2 declines, Householder: blocked compact-WY
+// kernels for m <= 2048, the single-workgroup unblocked kernel beyond.
+// Householder also leaves its reflectors below the diagonal and tau; no
+// caller of the CholQR2 form reads them.
+void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false)
 {
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
+  if (cholqr && c->cholqr && m >= 2 * n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq)) return;
   if (m <= qr_blocked_max_rows())
   {
     const int k = std::min(m, n);
@@ -267,7 +342,7 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   double* A = t->cores[k];
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
-  qr_factor(c, A, m, m, r, tau, Qn, m);
+  qr_factor(c, A, m, m, r, tau, Qn, m, true);
   if (!absorb)
   {
     replace_core(c, t, k, Qn);
@@ -295,7 +370,7 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   launch_transpose(t->cores[k], rp, m, rp, A, m, c->stream);
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(kk, 1));
   double* Q = ctx_ws(c, WS_Q, (size_t)m * kk);
-  qr_factor(c, A, m, m, rp, tau, Q, m);
+  qr_factor(c, A, m, m, rp, tau, Q, m, true);
   double* Cn = alloc_core(c, (size_t)m * kk);
   launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
   if (!absorb)
@@ -461,7 +536,7 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     launch_transpose(M, cols, rows, cols, A, rows, c->stream);
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows);
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
     if (try_cert && certify_full_rank(c, X, q, delta))
@@ -494,7 +569,7 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
   double* tau = ctx_ws(c, WS_F, (size_t)q);
   double* Q = ctx_ws(c, WS_Q, (size_t)cols * q);
-  qr_factor(c, A, cols, cols, rows, tau, Q, cols);
+  qr_factor(c, A, cols, cols, rows, tau, Q, cols, true);
   double* Y = ctx_ws(c, WS_X, (size_t)q * q);
   if (keep_all)
     launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);  // Y = R
@@ -648,7 +723,7 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     double* A = t->cores[k];  // M^T (cols x rows) column-major; factored in place
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)q * cols);
-    qr_factor(c, A, cols, cols, rows, tau, Q, cols);
+    qr_factor(c, A, cols, cols, rows, tau, Q, cols, true);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
     if (cert_ok(max_rank, q) && certify_full_rank(c, X, q, delta))
@@ -683,7 +758,7 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     launch_transpose(t->cores[k], cols, rows, cols, A, rows, c->stream);
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows);
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true);
     double* Y = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream);  // Y = R^T
     if (cert_ok(max_rank, q) && certify_full_rank(c, Y, q, delta))
@@ -1228,13 +1303,13 @@ int singular_values_dev(tta_ctx* c, const double* M, int rows, int cols, double*
   if (rows >= cols)
   {
     launch_transpose(M, cols, rows, cols, A, rows, c->stream);
-    qr_factor(c, A, rows, rows, cols, tau, nullptr, 0);
+    qr_factor(c, A, rows, rows, cols, tau, nullptr, 0, true);
     launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
   }
   else
   {
     hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
-    qr_factor(c, A, cols, cols, rows, tau, nullptr, 0);
+    qr_factor(c, A, cols, cols, rows, tau, nullptr, 0, true);
     launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
   }
   double* J = ctx_ws(c, WS_J, (size_t)q * q);
@@ -2318,6 +2393,8 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->jac_sweeps = 0;
   c->cert_tried = 0;
   c->cert_held = 0;
+  c->cholqr_tried = 0;
+  c->cholqr_used = 0;
   TTA_GUARD_END(c)
 }
 
@@ -2325,6 +2402,21 @@ int tta_set_rank_certificate(tta_ctx* c, int on)
 {
   if (!c) return TTA_ERR_ARG;
   c->rank_certificate = on != 0;
+  return TTA_OK;
+}
+
+int tta_set_cholqr(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->cholqr = on != 0;
+  return TTA_OK;
+}
+
+int tta_prof_cholqr(tta_ctx* c, long long* tried, long long* used)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (tried) *tried = c->cholqr_tried;
+  if (used) *used = c->cholqr_used;
   return TTA_OK;
 }
 

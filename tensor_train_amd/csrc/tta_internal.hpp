@@ -108,6 +108,10 @@ struct tta_ctx
   // was tried, and where it held so that the Jacobi SVD was not needed.
   bool rank_certificate = true;
   long long cert_tried = 0, cert_held = 0;
+  // CholQR2 for tall QR factorizations of the sweeps (on by default): tried
+  // and accepted counts.
+  bool cholqr = true;
+  long long cholqr_tried = 0, cholqr_used = 0;
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])

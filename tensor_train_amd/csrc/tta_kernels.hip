@@ -592,9 +592,12 @@ int grid_for(long n, int bs = 256)
 // 128-column chunks, X(:, J) resident); the 16 x 16 diagonal solve runs on
 // one thread per column.  out[2J] = ||X(:, J)||_F^2, out[2J+1] =
 // ||R(:, J)||_F^2; a zero or non-finite pivot propagates as inf / NaN.
+// With X non-null the columns of R^-1 are also written (row-major, leading
+// dimension ldx, zeros below the diagonal).
 constexpr int kCertB = 16, kCertChunk = 128;
 
-__global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict__ R, int q, double* __restrict__ out)
+__global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict__ R, int q, double* __restrict__ out,
+                                                        double* __restrict__ X, int ldx)
 {
   extern __shared__ double lds[];
   const int J = blockIdx.x;
@@ -604,7 +607,8 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
   double* Xs = lds;                              // rows x 16
   double* Rs = Xs + (size_t)rows * kCertB;       // 16 x kCertChunk
   double* Bs = Rs + kCertB * kCertChunk;         // 16 x 16
-  double* red = Bs + kCertB * kCertB;            // 4 waves
+  double* Rii = Bs + kCertB * kCertB;            // 16 x 16 diagonal block
+  double* red = Rii + kCertB * kCertB;           // 4 waves
   const int t = threadIdx.x, r = t / kCertB, cc = t % kCertB;
   for (int e = t; e < rows * kCertB; e += 256) Xs[e] = 0.0;
   __syncthreads();
@@ -628,12 +632,13 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
       __syncthreads();
     }
     Bs[r * kCertB + cc] = acc;
+    Rii[r * kCertB + cc] = (r < in && cc < in) ? R[(size_t)(i0 + r) * q + i0 + cc] : 0.0;
     __syncthreads();
     if (t < jn)
     {
       for (int rr = in - 1; rr >= 0; --rr)
       {
-        const double* Rrow = R + (size_t)(i0 + rr) * q + i0;
+        const double* Rrow = Rii + rr * kCertB;
         double v = Bs[rr * kCertB + t];
         for (int k = rr + 1; k < in; ++k) v -= Rrow[k] * Xs[(size_t)(i0 + k) * kCertB + t];
         Xs[(size_t)(i0 + rr) * kCertB + t] = v / Rrow[rr];
@@ -656,6 +661,12 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
       }
     }
   }
+  if (X)
+    for (int e = t; e < q * kCertB; e += 256)
+    {
+      const int row = e / kCertB, col = e % kCertB;
+      if (col < jn) X[(size_t)row * ldx + j0 + col] = row < rows ? Xs[e] : 0.0;
+    }
   sx = block_sum<256>(sx, red);
   sr = block_sum<256>(sr, red);
   if (t == 0)
@@ -737,10 +748,10 @@ void launch_gather_cols(const double* src, int lds_, int rows, const int* perm, 
 
 int rinv_cert_blocks(int q) { return (q + kCertB - 1) / kCertB; }
 
-void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st)
+void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X, int ldx)
 {
-  const size_t lds = sizeof(double) * ((size_t)q * kCertB + kCertB * kCertChunk + kCertB * kCertB + 4);
-  hipLaunchKernelGGL(rinv_cert_kernel, dim3(rinv_cert_blocks(q)), dim3(256), lds, st, R, q, out);
+  const size_t lds = sizeof(double) * ((size_t)q * kCertB + kCertB * kCertChunk + 2 * kCertB * kCertB + 4);
+  hipLaunchKernelGGL(rinv_cert_kernel, dim3(rinv_cert_blocks(q)), dim3(256), lds, st, R, q, out, X, ldx);
 }
 
 void launch_eye(double* dst, int n, long ld, hipStream_t st)

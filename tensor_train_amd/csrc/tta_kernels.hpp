@@ -212,6 +212,12 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
+// In-place Cholesky G = R^T R (row-major n x n, leading dimension ld, upper
+// triangle read, R written with the strict lower triangle zeroed), one
+// workgroup; n <= chol_max_n().  info[0] = 1 on breakdown, info[1..2] =
+// min / max diagonal of R.
+int chol_max_n();
+void launch_chol(double* G, int n, int ld, double* info, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
 // Returns the number of block sweeps the host ran (q > 64), or -1 when the
@@ -233,10 +239,10 @@ void launch_gather_cols(const double* src, int lds, int rows, const int* perm, i
 void launch_transpose(const double* in, int ldi, int rows, int cols, double* out, int ldo, hipStream_t st);
 // Squared Frobenius norms of R^-1 and R per 16-column block (2 doubles per
 // block, rinv_cert_blocks(q) blocks) for an upper-triangular row-major R,
-// q <= kRinvCertMaxQ.
+// q <= kRinvCertMaxQ; with X non-null also R^-1 itself (row-major, ldx).
 constexpr int kRinvCertMaxQ = 1024;
 int rinv_cert_blocks(int q);
-void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st);
+void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
 void launch_permute(const permute_desc& p, hipStream_t st);

@@ -5,7 +5,9 @@
 //     registers (thread t owns rows t, t + NT, ...), unblocked Householder
 //     with the dlarfg sign convention, then the triangular T of the block
 //     reflector H = H_0 ... H_{QB-1} = I - V T V^T (LAPACK dlarft, forward,
-//     columnwise) from the Gram V^T V.
+//     columnwise) from the Gram V^T V.  Column j's block reduction sums both
+//     v_j^T x_c (c > j, for the update) and v_c^T v_j (c < j, the Gram), so
+//     the Gram costs no pass of its own.
 //   * qr_update_kernel -- many workgroups per matrix (CT = 16 columns each):
 //     C <- (I - V T^T V^T) C on the trailing columns (factorisation) or
 //     C <- (I - V T V^T) C on the thin Q (formation, panels in reverse).
@@ -70,7 +72,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
   const int lda = P.lda;
   double* A = P.A;
   __shared__ double red[(NT / 64) * QB];
-  __shared__ double bc[QB * QB / 2 + QB];
+  __shared__ double bc[QB];
+  __shared__ double Gm[QB][QB + 1];
   __shared__ double Tl[QB][QB];
   __shared__ double taus[QB];
   __shared__ double sc[2];
@@ -125,15 +128,25 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
       const int r = tid + NT * i;
       vv[i] = (r == jj) ? 1.0 : ((r > jj && r < len) ? xj[i] * scal : 0.0);
     }
+    // part[c] = v_jj^T x_c for c > jj (update), v_c^T v_jj for c < jj (Gram;
+    // x holds v_c below the diagonal, the implicit 1 on it).
     double part[QB];
 #pragma unroll
     for (int c = 0; c < QB; ++c)
     {
       part[c] = 0.0;
 #pragma unroll
-      for (int i = 0; i < RPT; ++i) part[c] += vv[i] * x[i][c];
+      for (int i = 0; i < RPT; ++i)
+      {
+        const int r = tid + NT * i;
+        if (c > jj)
+          part[c] += vv[i] * x[i][c];
+        else if (c < jj)
+          part[c] += vv[i] * ((r == c) ? 1.0 : (r > c ? x[i][c] : 0.0));
+      }
     }
-    if (jj + 1 < bw) block_vsum<NT, QB>(part, jj + 1, bw, red, bc);
+    if (bw > 1) block_vsum<NT, QB>(part, 0, bw, red, bc);
+    if (tid < jj) Gm[tid][jj] = bc[tid];
 #pragma unroll
     for (int c = 0; c < QB; ++c)
     {
@@ -169,31 +182,6 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
     }
   }
   if (tid < bw) P.tau[j0 + tid] = taus[tid];
-  // Gram V^T V (strict upper part, packed) for T: one wave per pair, read
-  // back from global memory (the panel was just stored by this workgroup).
-  __syncthreads();
-  {
-    const int lane = tid & 63, wid = tid >> 6;
-    for (int idx = wid; idx < QB * (QB - 1) / 2; idx += NT / 64)
-    {
-      int a = 0, rem = idx;
-      while (rem >= QB - 1 - a)
-      {
-        rem -= QB - 1 - a;
-        ++a;
-      }
-      const int b = a + 1 + rem;
-      double s = 0.0;
-      if (b < bw)
-      {
-        const double* Va = A + (size_t)(j0 + a) * lda + j0;
-        const double* Vb = A + (size_t)(j0 + b) * lda + j0;
-        for (int r = b + lane; r < len; r += 64) s += ((r == b) ? 1.0 : Vb[r]) * Va[r];
-      }
-      s = wave_sum(s);
-      if (lane == 0) bc[idx] = s;
-    }
-  }
   __syncthreads();
   // T of the block reflector (LAPACK dlarft, forward, columnwise):
   //   T[a][a] = tau_a,  T[a][j] = -tau_j * sum_{b=a}^{j-1} T[a][b] (V^T V)[b][j].
@@ -215,7 +203,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
           double sacc = 0.0;
 #pragma unroll
           for (int bb = 0; bb < QB; ++bb)
-            if (bb >= a && bb < j) sacc += row[bb] * bc[bb * QB - bb * (bb + 1) / 2 + (j - bb - 1)];
+            if (bb >= a && bb < j) sacc += row[bb] * Gm[bb][j];
           row[j] = -taus[j] * sacc;
         }
       }
@@ -467,7 +455,168 @@ __global__ void qr_eye_kernel(qr_batch bt)
     P.Q[(size_t)c * P.ldq + r] = (r == c) ? 1.0 : 0.0;
   }
 }
+
+// ---------------------------------------------------------------------------
+// Cholesky G = R^T R of a symmetric positive definite n x n matrix (row-major,
+// leading dimension ld, upper triangle read; R written over it, strict lower
+// triangle zeroed) by one workgroup of 256 threads, right-looking in
+// 16-column blocks:
+//   * wave 0 factors the 16 x 16 diagonal block in registers (lane c holds
+//     column c; the pivot and the row of R go round by readlane);
+//   * one thread per trailing column solves R_JJ^T X = G(J, j1:) for the
+//     block row of R (forward substitution, R_JJ from LDS);
+//   * the trailing upper triangle takes the rank-16 update
+//     G(j1:, j1:) -= X^T X on the matrix cores, one 16 x 16 tile per wave
+//     at a time (four v_mfma_f64_16x16x4f64 steps over the block's 16 rows).
+// info[0] = 1 on a non-positive or non-finite pivot; info[1], info[2] =
+// smallest and largest diagonal entry of R.
+// ---------------------------------------------------------------------------
+constexpr int kCholMaxN = 512;
+
+__global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, double* info)
+{
+  __shared__ double Rd[16][17];
+  __shared__ double Ps[16][kCholMaxN];
+  __shared__ int bad;
+  __shared__ double dmin, dmax;
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid == 0)
+  {
+    bad = 0;
+    dmin = INFINITY;
+    dmax = 0.0;
+  }
+  for (int j0 = 0; j0 < n; j0 += 16)
+  {
+    const int jb = min(16, n - j0), j1 = j0 + jb, w = n - j1;
+    __syncthreads();
+    if (wid == 0)
+    {
+      double col[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r)
+        col[r] = (lane < jb && r <= lane) ? G[(size_t)(j0 + r) * ld + j0 + lane] : 0.0;
+      bool lbad = false;
+      double lmin = INFINITY, lmax = 0.0;
+#pragma unroll
+      for (int k = 0; k < 16; ++k)
+      {
+        if (k < jb)
+        {
+          const double piv = read_lane(col[k], k);
+          if (!(piv > 0.0) || !(piv < INFINITY)) lbad = true;
+          const double rkk = sqrt(piv);
+          lmin = fmin(lmin, rkk);
+          lmax = fmax(lmax, rkk);
+          if (lane == k)
+            col[k] = rkk;
+          else if (lane > k)
+            col[k] = col[k] / rkk;
+#pragma unroll
+          for (int r = k + 1; r < 16; ++r)
+          {
+            if (r < jb)
+            {
+              const double rkr = read_lane(col[k], r);
+              if (lane >= r && lane < jb) col[r] -= rkr * col[k];
+            }
+          }
+        }
+      }
+      if (lane < 16)
+      {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) Rd[r][lane] = (r <= lane && lane < jb) ? col[r] : 0.0;
+        if (lane < jb)
+        {
+#pragma unroll
+          for (int r = 0; r < 16; ++r)
+            if (r < jb) G[(size_t)(j0 + r) * ld + j0 + lane] = r <= lane ? col[r] : 0.0;
+        }
+      }
+      if (lane == 0)
+      {
+        if (lbad) bad = 1;
+        dmin = fmin(dmin, lmin);
+        dmax = fmax(dmax, lmax);
+      }
+    }
+    __syncthreads();
+    for (int c = tid; c < w; c += 256)
+    {
+      double x[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+      {
+        if (t < jb)
+        {
+          double v = G[(size_t)(j0 + t) * ld + j1 + c];
+#pragma unroll
+          for (int s2 = 0; s2 < 16; ++s2)
+            if (s2 < t) v -= Rd[s2][t] * x[s2];
+          x[t] = v / Rd[t][t];
+        }
+        else
+          x[t] = 0.0;
+      }
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+      {
+        Ps[t][c] = x[t];
+        if (t < jb) G[(size_t)(j0 + t) * ld + j1 + c] = x[t];
+      }
+    }
+    __syncthreads();
+    const int nt = (w + 15) / 16;
+    const int fl = lane & 15, fk = lane >> 4;
+    for (int tix = wid; tix < nt * nt; tix += 4)
+    {
+      const int I = tix / nt, J = tix % nt;
+      if (I > J) continue;
+      qdbl4 acc;
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const int gi = 16 * I + fk + 4 * r, gj = 16 * J + fl;
+        acc[r] = (gi < w && gj < w) ? G[(size_t)(j1 + gi) * ld + j1 + gj] : 0.0;
+      }
+#pragma unroll
+      for (int k0 = 0; k0 < 16; k0 += 4)
+      {
+        const int ai = 16 * I + fl, bj = 16 * J + fl, kk = k0 + fk;
+        const double a = ai < w ? -Ps[kk][ai] : 0.0;
+        const double b = bj < w ? Ps[kk][bj] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+      {
+        const int gi = 16 * I + fk + 4 * r, gj = 16 * J + fl;
+        if (gi < w && gj < w) G[(size_t)(j1 + gi) * ld + j1 + gj] = acc[r];
+      }
+    }
+  }
+  __syncthreads();
+  for (long e = tid; e < (long)n * n; e += 256)
+  {
+    const int i = (int)(e / n), j = (int)(e % n);
+    if (i > j) G[(size_t)i * ld + j] = 0.0;
+  }
+  if (tid == 0)
+  {
+    info[0] = bad ? 1.0 : 0.0;
+    info[1] = dmin;
+    info[2] = dmax;
+  }
+}
 }  // namespace
+
+int chol_max_n() { return kCholMaxN; }
+
+void launch_chol(double* G, int n, int ld, double* info, hipStream_t st)
+{
+  hipLaunchKernelGGL(chol_kernel, dim3(1), dim3(256), 0, st, G, n, ld, info);
+}
 
 int qr_blocked_max_rows() { return 4 * 512; }
 
