@@ -676,6 +676,56 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
   }
 }
 
+// G <- G - I (n x n, row-major) and out[0] = max |G - I| (one workgroup).
+__global__ __launch_bounds__(1024) void minus_ident_kernel(double* G, int n, double* out)
+{
+  __shared__ double red[16];
+  double m = 0.0;
+  for (long e = threadIdx.x; e < (long)n * n; e += 1024)
+  {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double v = G[e] - (i == j ? 1.0 : 0.0);
+    G[e] = v;
+    m = fmax(m, fabs(v) <= DBL_MAX ? fabs(v) : INFINITY);
+  }
+  // max over the block (non-finite entries counted as infinity).
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0)
+  {
+    double t = 0.0;
+    for (int w = 0; w < 16; ++w) t = fmax(t, red[w]);
+    out[0] = t;
+  }
+}
+
+// X = Phi(E - W) (n x n, row-major; W may be null): the strict upper part of
+// E - W, half its diagonal, zero below.
+__global__ void phi_kernel(const double* E, const double* W, double* X, int n)
+{
+  const long total = (long)n * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int i = (int)(e / n), j = (int)(e % n);
+    const double v = E[e] - (W ? W[e] : 0.0);
+    X[e] = i < j ? v : (i == j ? 0.5 * v : 0.0);
+  }
+}
+
+// Y = alpha I + beta X (n x n, row-major).
+__global__ void ident_axpy_kernel(const double* X, double* Y, int n, double alpha, double beta)
+{
+  const long total = (long)n * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int i = (int)(e / n), j = (int)(e % n);
+    Y[e] = (i == j ? alpha : 0.0) + beta * X[e];
+  }
+}
+
 // dst (n x n, leading dimension ld) = identity.
 __global__ void eye_kernel(double* dst, int n, long ld)
 {
@@ -752,6 +802,25 @@ void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, doubl
 {
   const size_t lds = sizeof(double) * ((size_t)q * kCertB + kCertB * kCertChunk + 2 * kCertB * kCertB + 4);
   hipLaunchKernelGGL(rinv_cert_kernel, dim3(rinv_cert_blocks(q)), dim3(256), lds, st, R, q, out, X, ldx);
+}
+
+void launch_minus_ident(double* G, int n, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(minus_ident_kernel, dim3(1), dim3(1024), 0, st, G, n, out);
+}
+
+void launch_phi(const double* E, const double* W, double* X, int n, hipStream_t st)
+{
+  long g = ((long)n * n + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(phi_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, E, W, X, n);
+}
+
+void launch_ident_axpy(const double* X, double* Y, int n, double alpha, double beta, hipStream_t st)
+{
+  long g = ((long)n * n + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(ident_axpy_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, X, Y, n, alpha, beta);
 }
 
 void launch_eye(double* dst, int n, long ld, hipStream_t st)

@@ -245,6 +245,12 @@ int rinv_cert_blocks(int q);
 void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
+// G <- G - I and out[0] = max |G - I| (row-major n x n).
+void launch_minus_ident(double* G, int n, double* out, hipStream_t st);
+// X = strict upper part of E - W plus half its diagonal (W may be null).
+void launch_phi(const double* E, const double* W, double* X, int n, hipStream_t st);
+// Y = alpha I + beta X.
+void launch_ident_axpy(const double* X, double* Y, int n, double alpha, double beta, hipStream_t st);
 void launch_permute(const permute_desc& p, hipStream_t st);
 void launch_matvec_core(const double* A, const double* x, double* y, int r, int m, int n, int rp, int s, int sp,
                         hipStream_t st);
