@@ -243,7 +243,8 @@ def main():
     ap.add_argument("--train-tts", type=int, default=650)
     ap.add_argument("--mlp-steps", type=int, default=1500)
     ap.add_argument("--full-svd-steps", type=int, default=2,
-                    help="steps timed again with the full-rank certificate off (every SVD step runs its SVD)")
+                    help="steps timed again with the full-rank certificate and CholQR2 off (every QR is "
+                         "Householder and every SVD step runs its SVD)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
@@ -267,18 +268,22 @@ def main():
     for _ in range(args.prof_steps):
         y = T.matvec_round(A, x, eps=EPS)
     ctx.synchronize()
-    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "apply", "other")}
+    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "apply", "chol", "rinv", "other")}
     cert_tried, cert_held = ctx.rank_certificates()
+    cholqr_tried, cholqr_used = ctx.cholqr_counts()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
-    # round kernel, the FP64 GEMM, the MPO core contraction); the roofline is
-    # quoted for the one with the most device time.
+    # round kernel, the FP64 GEMM, the MPO core contraction, CholQR2's
+    # Cholesky, the triangular inverse); the roofline is quoted for the one
+    # with the most device time.
     single = {"jacobi_round": ("bj_round_kernel", ctx.prof_read("jacobi_round")),
-              "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"])}
+              "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"]),
+              "chol": ("chol_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}
     ctx.prof_enable(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     full_svd_ms = None
     if args.full_svd_steps > 0:
         ctx.set_rank_certificate(False)
+        ctx.set_cholqr(False)
         y_full = T.matvec_round(A, x, eps=EPS)
         ctx.synchronize()
         t2 = time.perf_counter()
@@ -287,6 +292,7 @@ def main():
         ctx.synchronize()
         full_svd_ms = (time.perf_counter() - t2) * 1e3 / args.full_svd_steps
         ctx.set_rank_certificate(True)
+        ctx.set_cholqr(True)
         assert y_full.ranks == y.ranks
     alg_flops = algorithmic_flops(A, x, y.ranks)
 
@@ -326,9 +332,11 @@ def main():
         "kernel_ms_per_step": {k: v["ms"] / args.prof_steps for k, v in stats.items()},
         "algorithmic_gflop_per_step": alg_flops / 1e9,
         "achieved_gflops_per_step": alg_flops / ms_per_step / 1e6,
+        "cholqr2": {"tall_qr_per_step": cholqr_tried / args.prof_steps,
+                    "cholqr2_used_per_step": cholqr_used / args.prof_steps},
         "rank_certificate": {"svd_steps_per_step": cert_tried / args.prof_steps,
                              "certified_per_step": cert_held / args.prof_steps,
-                             "ms_per_step_certificate_off": full_svd_ms},
+                             "ms_per_step_householder_and_svd": full_svd_ms},
     }
     if not args.no_batched:
         out["batched"] = batched_section(T, ctx, args)

@@ -281,7 +281,7 @@ int tta_scale_core(tta_ctx* ctx, tta_tt* t, int k, double alpha);
 /* ---- kernel timing (HIP events on the context stream) -------------------- */
 enum tta_kernel_class
 {
-  TTA_K_QR = 0,     /* Householder QR + thin Q */
+  TTA_K_QR = 0,     /* Householder QR + thin Q (CholQR2 parts: see below) */
   TTA_K_JACOBI = 1, /* one-sided Jacobi SVD + sort + truncation rank */
   TTA_K_GEMM = 2,   /* FP64 contractions (absorb factors, U = Q J, inner) */
   TTA_K_APPLY = 3,  /* MPO core contraction (matvec / matmat cores) */
@@ -290,7 +290,13 @@ enum tta_kernel_class
    * share of its SVD call's textbook flops and bytes.  These launches are
    * also inside TTA_K_JACOBI, which times whole SVD calls. */
   TTA_K_JACOBI_ROUND = 5,
-  TTA_K_COUNT = 6
+  /* CholQR2's single-workgroup Cholesky (credited n^3/3 flops, 16 n^2 bytes)
+   * and the device triangular inverse of CholQR2 and of the full-rank
+   * certificate (n^3/3 flops, 16 n^2 bytes); CholQR2's products count as
+   * TTA_K_GEMM and its elementwise steps as TTA_K_OTHER. */
+  TTA_K_CHOL = 6,
+  TTA_K_RINV = 7,
+  TTA_K_COUNT = 8
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

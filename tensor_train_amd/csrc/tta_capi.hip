@@ -231,38 +231,59 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
   double* aux = alloc_core(c, 8 + 2 * (size_t)nb);
   double h[8];
   bool ok = false;
+  const double cn = (double)n * n * n / 3.0, cb = 16.0 * (double)nn;
+  auto gm = [&](int M, int N, int K, const double* Am, long sar, long sac, const double* Bm, long sbr, long sbc,
+                double* Cm, long scr, long scc)
+  {
+    prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
+    gemm_raw(c, M, N, K, Am, sar, sac, Bm, sbr, sbc, Cm, scr, scc);
+  };
   auto release = [&]()
   {
     for (double* p : {G, X, W, Y, R1, Q1, aux}) c->mem.put(p);
   };
   try
   {
-    gemm_raw(c, n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
-    launch_chol(G, n, n, aux, c->stream);                    // G <- R1
+    gm(n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
+    {
+      prof_scope ps(c, TTA_K_CHOL, cn, cb);
+      launch_chol(G, n, n, aux, c->stream);                  // G <- R1
+    }
     hip_check(hipMemcpyAsync(R1, G, nn * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
-    launch_rinv_cert(R1, n, aux + 8, c->stream, X, n);       // X = R1^-1
-    gemm_raw(c, m, n, n, A, 1, lda, X, n, 1, Q1, 1, m);      // Q1 = A X
-    gemm_raw(c, n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
-    launch_minus_ident(G, n, aux + 4, c->stream);            // G = E, aux[4] = max |E|
+    {
+      prof_scope ps(c, TTA_K_RINV, cn, cb);
+      launch_rinv_cert(R1, n, aux + 8, c->stream, X, n);     // X = R1^-1
+    }
+    gm(m, n, n, A, 1, lda, X, n, 1, Q1, 1, m);      // Q1 = A X
+    gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
+      launch_minus_ident(G, n, aux + 4, c->stream);          // G = E, aux[4] = max |E|
+    }
     hip_check(hipMemcpyAsync(h, aux, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     ok = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]) && h[4] * n <= 1.0e-4;
     if (ok)
     {
-      launch_phi(G, nullptr, X, n, c->stream);               // X = Phi(E)
+      auto ew = [&](auto&& launch)
+      {
+        prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
+        launch();
+      };
+      ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
       for (int it = 0; it < 3; ++it)
       {
-        gemm_raw(c, n, n, n, X, 1, n, X, n, 1, W, n, 1);     // W = X^T X
-        launch_phi(G, W, X, n, c->stream);                   // X = Phi(E - W)
+        gm(n, n, n, X, 1, n, X, n, 1, W, n, 1);                       // W = X^T X
+        ew([&] { launch_phi(G, W, X, n, c->stream); });              // X = Phi(E - W)
       }
-      launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream);       // W = R2 = I + X
-      launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream);      // G = Y0 = I - X
-      gemm_raw(c, n, n, n, W, n, 1, G, n, 1, X, n, 1);       // X = R2 Y0
-      launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream);      // X = 2 I - R2 Y0
-      gemm_raw(c, n, n, n, G, n, 1, X, n, 1, Y, n, 1);       // Y = R2^-1
-      if (Q) gemm_raw(c, m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);  // Q = Q1 Y
-      gemm_raw(c, n, n, n, W, n, 1, R1, n, 1, G, n, 1);      // G = R2 R1
-      launch_transpose(G, n, n, n, A, lda, c->stream);
+      ew([&] { launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream); });  // W = R2 = I + X
+      ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
+      gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
+      ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
+      gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
+      if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
+      gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R2 R1
+      ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });  // A's leading block = R
     }
   }
   catch (...)
@@ -285,8 +306,8 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
 // caller of the CholQR2 form reads them.
 void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false)
 {
-  prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (cholqr && c->cholqr && m >= 2 * n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq)) return;
+  prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (m <= qr_blocked_max_rows())
   {
     const int k = std::min(m, n);
@@ -500,7 +521,7 @@ bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta)
   double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
   c->cert_tried += 1;
   {
-    prof_scope ps(c, TTA_K_OTHER, (double)q * q * q / 3.0, 8.0 * 1.5 * (double)q * q);
+    prof_scope ps(c, TTA_K_RINV, (double)q * q * q / 3.0, 16.0 * (double)q * q);
     launch_rinv_cert(R, q, part, c->stream);
   }
   std::vector<double> h(2 * (size_t)nb);

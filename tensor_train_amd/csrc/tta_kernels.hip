@@ -632,17 +632,33 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
       __syncthreads();
     }
     Bs[r * kCertB + cc] = acc;
-    Rii[r * kCertB + cc] = (r < in && cc < in) ? R[(size_t)(i0 + r) * q + i0 + cc] : 0.0;
+    {
+      const double rv = (r < in && cc < in) ? R[(size_t)(i0 + r) * q + i0 + cc] : 0.0;
+      // The diagonal holds reciprocals: the substitution below multiplies.
+      Rii[r * kCertB + cc] = (r == cc && r < in) ? 1.0 / rv : rv;
+    }
     __syncthreads();
     if (t < jn)
     {
-      for (int rr = in - 1; rr >= 0; --rr)
+      // Back substitution for column t with the column in registers.
+      double xc[kCertB];
+#pragma unroll
+      for (int rr = kCertB - 1; rr >= 0; --rr)
       {
-        const double* Rrow = Rii + rr * kCertB;
-        double v = Bs[rr * kCertB + t];
-        for (int k = rr + 1; k < in; ++k) v -= Rrow[k] * Xs[(size_t)(i0 + k) * kCertB + t];
-        Xs[(size_t)(i0 + rr) * kCertB + t] = v / Rrow[rr];
+        double v = 0.0;
+        if (rr < in)
+        {
+          v = Bs[rr * kCertB + t];
+#pragma unroll
+          for (int k = rr + 1; k < kCertB; ++k)
+            if (k < in) v = fma(-Rii[rr * kCertB + k], xc[k], v);
+          v *= Rii[rr * kCertB + rr];
+        }
+        xc[rr] = v;
       }
+#pragma unroll
+      for (int rr = 0; rr < kCertB; ++rr)
+        if (rr < in) Xs[(size_t)(i0 + rr) * kCertB + t] = xc[rr];
     }
     __syncthreads();
   }

@@ -476,6 +476,7 @@ constexpr int kCholMaxN = 512;
 __global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, double* info)
 {
   __shared__ double Rd[16][17];
+  __shared__ double Rdi[16];
   __shared__ double Ps[16][kCholMaxN];
   __shared__ int bad;
   __shared__ double dmin, dmax;
@@ -527,6 +528,7 @@ __global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, dou
       {
 #pragma unroll
         for (int r = 0; r < 16; ++r) Rd[r][lane] = (r <= lane && lane < jb) ? col[r] : 0.0;
+        Rdi[lane] = lane < jb ? 1.0 / col[lane] : 0.0;
         if (lane < jb)
         {
 #pragma unroll
@@ -544,17 +546,20 @@ __global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, dou
     __syncthreads();
     for (int c = tid; c < w; c += 256)
     {
-      double x[16];
+      // All 16 loads first, then the substitution chain in registers.
+      double g[16], x[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) g[t] = t < jb ? G[(size_t)(j0 + t) * ld + j1 + c] : 0.0;
 #pragma unroll
       for (int t = 0; t < 16; ++t)
       {
         if (t < jb)
         {
-          double v = G[(size_t)(j0 + t) * ld + j1 + c];
+          double v = g[t];
 #pragma unroll
           for (int s2 = 0; s2 < 16; ++s2)
-            if (s2 < t) v -= Rd[s2][t] * x[s2];
-          x[t] = v / Rd[t][t];
+            if (s2 < t) v = fma(-Rd[s2][t], x[s2], v);
+          x[t] = v * Rdi[t];
         }
         else
           x[t] = 0.0;
@@ -569,32 +574,42 @@ __global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, dou
     __syncthreads();
     const int nt = (w + 15) / 16;
     const int fl = lane & 15, fk = lane >> 4;
-    for (int tix = wid; tix < nt * nt; tix += 4)
-    {
-      const int I = tix / nt, J = tix % nt;
-      if (I > J) continue;
-      qdbl4 acc;
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
+    // Wave wid takes tile rows I = wid, wid + 4, ...; along a row, four
+    // tiles at a time with all their loads issued before the first MFMA.
+    for (int I = wid; I < nt; I += 4)
+      for (int J0 = I; J0 < nt; J0 += 4)
       {
-        const int gi = 16 * I + fk + 4 * r, gj = 16 * J + fl;
-        acc[r] = (gi < w && gj < w) ? G[(size_t)(j1 + gi) * ld + j1 + gj] : 0.0;
-      }
+        qdbl4 acc[4];
 #pragma unroll
-      for (int k0 = 0; k0 < 16; k0 += 4)
-      {
-        const int ai = 16 * I + fl, bj = 16 * J + fl, kk = k0 + fk;
-        const double a = ai < w ? -Ps[kk][ai] : 0.0;
-        const double b = bj < w ? Ps[kk][bj] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-      }
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-      {
-        const int gi = 16 * I + fk + 4 * r, gj = 16 * J + fl;
-        if (gi < w && gj < w) G[(size_t)(j1 + gi) * ld + j1 + gj] = acc[r];
+          for (int r = 0; r < 4; ++r)
+          {
+            const int gi = 16 * I + fk + 4 * r, gj = 16 * (J0 + u) + fl;
+            acc[u][r] = (J0 + u < nt && gi < w && gj < w) ? G[(size_t)(j1 + gi) * ld + j1 + gj] : 0.0;
+          }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+        {
+          if (J0 + u >= nt) continue;
+#pragma unroll
+          for (int k0 = 0; k0 < 16; k0 += 4)
+          {
+            const int ai = 16 * I + fl, bj = 16 * (J0 + u) + fl, kk = k0 + fk;
+            const double a = ai < w ? -Ps[kk][ai] : 0.0;
+            const double b = bj < w ? Ps[kk][bj] : 0.0;
+            acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+          {
+            const int gi = 16 * I + fk + 4 * r, gj = 16 * (J0 + u) + fl;
+            if (J0 + u < nt && gi < w && gj < w) G[(size_t)(j1 + gi) * ld + j1 + gj] = acc[u][r];
+          }
       }
-    }
   }
   __syncthreads();
   for (long e = tid; e < (long)n * n; e += 256)
