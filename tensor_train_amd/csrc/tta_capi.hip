@@ -218,7 +218,16 @@ void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long s
 // bounds the pass-2 errors far below rounding.  Then R goes to A's leading
 // n x n block (upper, zeros below) and Q to Q; otherwise nothing is written
 // and the caller factors A by Householder.
-bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
+// Squared Frobenius norms of a factorization's triangular factor R and of
+// R^-1, when the factorization formed R^-1 anyway (CholQR2 does); the
+// full-rank certificate then needs no triangular inverse of its own.
+struct rinv_norms
+{
+  bool have = false;
+  double xf2 = 0.0, rf2 = 0.0;
+};
+
+bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, rinv_norms* cert = nullptr)
 {
   const size_t nn = (size_t)n * n;
   const int nb = rinv_cert_blocks(n);
@@ -229,6 +238,9 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
   double* R1 = alloc_core(c, nn);
   double* Q1 = alloc_core(c, (size_t)m * n);
   double* aux = alloc_core(c, 8 + 2 * (size_t)nb);
+  // R1^-1 stays live for the certificate's R^-1 = R1^-1 R2^-1; otherwise its
+  // buffer is reused by the second pass.
+  double* Xi = cert ? alloc_core(c, nn) : X;
   double h[8];
   bool ok = false;
   const double cn = (double)n * n * n / 3.0, cb = 16.0 * (double)nn;
@@ -241,6 +253,7 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
   auto release = [&]()
   {
     for (double* p : {G, X, W, Y, R1, Q1, aux}) c->mem.put(p);
+    if (Xi != X) c->mem.put(Xi);
   };
   try
   {
@@ -252,9 +265,9 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
     hip_check(hipMemcpyAsync(R1, G, nn * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
     {
       prof_scope ps(c, TTA_K_RINV, cn, cb);
-      launch_rinv_cert(R1, n, aux + 8, c->stream, X, n);     // X = R1^-1
+      launch_rinv_cert(R1, n, aux + 8, c->stream, Xi, n);    // Xi = R1^-1
     }
-    gm(m, n, n, A, 1, lda, X, n, 1, Q1, 1, m);      // Q1 = A X
+    gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
     gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
     {
       prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
@@ -284,6 +297,24 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
       if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
       gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R2 R1
       ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });  // A's leading block = R
+      if (cert)
+      {
+        // R^-1 = R1^-1 R2^-1, and the column norms of R and R^-1.
+        gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
+        double* cn2 = Q1;  // Q1 is no longer read
+        ew([&] { launch_col_norms(G, n, n, cn2, c->stream); });
+        ew([&] { launch_col_norms(W, n, n, cn2 + n, c->stream); });
+        std::vector<double> hn(2 * (size_t)n);
+        hip_check(hipMemcpyAsync(hn.data(), cn2, hn.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        cert->rf2 = cert->xf2 = 0.0;
+        for (int j = 0; j < n; ++j)
+        {
+          cert->rf2 += hn[j] * hn[j];
+          cert->xf2 += hn[n + j] * hn[n + j];
+        }
+        cert->have = true;
+      }
     }
   }
   catch (...)
@@ -304,9 +335,14 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq)
 // kernels for m <= 2048, the single-workgroup unblocked kernel beyond.
 // Householder also leaves its reflectors below the diagonal and tau; no
 // caller of the CholQR2 form reads them.
-void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false)
+// With cert non-null, a CholQR2 factorization also reports the norms of R and
+// R^-1 there (cert->have); a Householder one leaves cert->have false.
+void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false,
+               rinv_norms* cert = nullptr)
 {
-  if (cholqr && c->cholqr && m >= 2 * n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq)) return;
+  if (cert) cert->have = false;
+  if (cholqr && c->cholqr && m >= 2 * n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
+    return;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (m <= qr_blocked_max_rows())
   {
@@ -513,25 +549,36 @@ void polar_correct(tta_ctx* c, double* J, int q)
 // inverse's relative rounding error (<= q u kappa) and the difference between
 // this R and the one of any backward-stable factorization (<= q u ||M||,
 // i.e. <= 1e-5 sigma_min) stay far below the 1e-3 slack taken on each side.
-// Non-finite norms (a zero pivot) fail the certificate.
-bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta)
+// Non-finite norms (a zero pivot) fail the certificate.  When the QR formed
+// R^-1 itself (known holds its norms), they are taken from there: CholQR2's
+// R^-1 = R1^-1 R2^-1 carries the same relative error bound as the triangular
+// inverse.
+bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta, const rinv_norms* known = nullptr)
 {
   if (!c->rank_certificate || q < 1 || q > kRinvCertMaxQ) return false;
-  const int nb = rinv_cert_blocks(q);
-  double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
   c->cert_tried += 1;
-  {
-    prof_scope ps(c, TTA_K_RINV, (double)q * q * q / 3.0, 16.0 * (double)q * q);
-    launch_rinv_cert(R, q, part, c->stream);
-  }
-  std::vector<double> h(2 * (size_t)nb);
-  hip_check(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
-  hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
   double xf2 = 0.0, rf2 = 0.0;
-  for (int b = 0; b < nb; ++b)
+  if (known && known->have)
   {
-    xf2 += h[2 * (size_t)b];
-    rf2 += h[2 * (size_t)b + 1];
+    xf2 = known->xf2;
+    rf2 = known->rf2;
+  }
+  else
+  {
+    const int nb = rinv_cert_blocks(q);
+    double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
+    {
+      prof_scope ps(c, TTA_K_RINV, (double)q * q * q / 3.0, 16.0 * (double)q * q);
+      launch_rinv_cert(R, q, part, c->stream);
+    }
+    std::vector<double> h(2 * (size_t)nb);
+    hip_check(hipMemcpyAsync(h.data(), part, h.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    for (int b = 0; b < nb; ++b)
+    {
+      xf2 += h[2 * (size_t)b];
+      rf2 += h[2 * (size_t)b + 1];
+    }
   }
   if (!std::isfinite(xf2) || !std::isfinite(rf2) || !(xf2 > 0.0) || !(rf2 > 0.0)) return false;
   const double xf = std::sqrt(xf2), rf = std::sqrt(rf2);
@@ -578,10 +625,11 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     launch_transpose(M, cols, rows, cols, A, rows, c->stream);
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true);
+    rinv_norms rn;
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, try_cert && c->rank_certificate ? &rn : nullptr);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
-    if (try_cert && certify_full_rank(c, X, q, delta))
+    if (try_cert && certify_full_rank(c, X, q, delta, &rn))
     {
       double* U = alloc_core(c, (size_t)rows * q);
       launch_transpose(Q, rows, q, rows, U, q, c->stream);
@@ -611,13 +659,14 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
   double* tau = ctx_ws(c, WS_F, (size_t)q);
   double* Q = ctx_ws(c, WS_Q, (size_t)cols * q);
-  qr_factor(c, A, cols, cols, rows, tau, Q, cols, true);
+  rinv_norms rn;
+  qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, try_cert && c->rank_certificate ? &rn : nullptr);
   double* Y = ctx_ws(c, WS_X, (size_t)q * q);
   if (keep_all)
     launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);  // Y = R
   else
     launch_extract_upper(A, cols, q, q, Y, q, 1, c->stream);  // Y = R^T
-  if (try_cert && certify_full_rank(c, Y, q, delta))
+  if (try_cert && certify_full_rank(c, Y, q, delta, &rn))
   {
     double* U = alloc_core(c, (size_t)q * q);
     launch_eye(U, q, q, c->stream);
@@ -765,10 +814,11 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     double* A = t->cores[k];  // M^T (cols x rows) column-major; factored in place
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)q * cols);
-    qr_factor(c, A, cols, cols, rows, tau, Q, cols, true);
+    rinv_norms rn;
+    qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
-    if (cert_ok(max_rank, q) && certify_full_rank(c, X, q, delta))
+    if (cert_ok(max_rank, q) && certify_full_rank(c, X, q, delta, &rn))
     {
       // M = R^T Q^T: the new core is Q^T (the column-major Q read row-major),
       // and R^T is absorbed into core k-1.
@@ -800,10 +850,11 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     launch_transpose(t->cores[k], cols, rows, cols, A, rows, c->stream);
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true);
+    rinv_norms rn;
+    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
     double* Y = ctx_ws(c, WS_X, (size_t)q * q);
     launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream);  // Y = R^T
-    if (cert_ok(max_rank, q) && certify_full_rank(c, Y, q, delta))
+    if (cert_ok(max_rank, q) && certify_full_rank(c, Y, q, delta, &rn))
     {
       // M = M I: the new core is the identity and M is absorbed into core k-1.
       r_new = q;
