@@ -148,6 +148,8 @@ void prof_collect(tta_ctx* c)
   c->prof_pending.clear();
 }
 
+void gemm_go(tta_ctx* c, const gemm_desc& g);
+
 void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const double* B, long ldb, double* C,
              long ldc)
 {
@@ -165,7 +167,7 @@ void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const d
   g.scr = ldc;
   g.scc = 1;
   prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
-  launch_gemm(g, 1, c->stream);
+  gemm_go(c, g);
 }
 
 // Scratch slots (see tta_internal.hpp).
@@ -182,8 +184,21 @@ enum
   WS_PART,
   WS_SMALL,
   WS_T,
-  WS_CERT
+  WS_CERT,
+  WS_SPLITK
 };
+
+// One GEMM on the context stream, split along K when its output tiles
+// alone would leave the device mostly idle (the partial products go
+// through the WS_SPLITK scratch slot).
+void gemm_go(tta_ctx* c, const gemm_desc& g)
+{
+  const int S = gemm_splitk_slices(g.M, g.N, g.K);
+  if (S > 1)
+    launch_gemm_splitk(g, S, ctx_ws(c, WS_SPLITK, (size_t)S * g.M * g.N), c->stream);
+  else
+    launch_gemm(g, 1, c->stream);
+}
 
 // C = A B with element strides, without a profiling scope of its own (for
 // use inside a scope of another class).
@@ -203,7 +218,7 @@ void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long s
   g.C = C;
   g.scr = scr;
   g.scc = scc;
-  launch_gemm(g, 1, c->stream);
+  gemm_go(c, g);
 }
 
 // CholQR2 of a tall column-major (m x n) A, m >= 2n.  Pass 1: G = A^T A =
@@ -375,7 +390,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
 void run_gemm(tta_ctx* c, const gemm_desc& g)
 {
   prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K, 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
-  launch_gemm(g, 1, c->stream);
+  gemm_go(c, g);
 }
 
 int read_int(tta_ctx* c, const int* dev)

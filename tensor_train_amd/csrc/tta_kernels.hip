@@ -174,6 +174,36 @@ __global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
   gemm_tile(g, g.A + (size_t)blockIdx.z * g.bsa, g.B + (size_t)blockIdx.z * g.bsb, g.C + (size_t)blockIdx.z * g.bsc);
 }
 
+// Split-K: slice blockIdx.z multiplies the K range [z kper, (z+1) kper) into
+// its own partial product P_z (M x N, in C's orientation: row-major when C's
+// column stride is 1, column-major otherwise); splitk_reduce_kernel then
+// sums P_0 .. P_{S-1} in that order (a fixed order, so the result does not
+// depend on scheduling) and applies alpha and beta.
+__global__ __launch_bounds__(256) void gemm_f64_splitk_kernel(gemm_desc g, int kper, double* part)
+{
+  const int k0 = blockIdx.z * kper;
+  const int kn = min(g.K - k0, kper);
+  const bool rm = g.scc == 1;
+  double* P = part + (size_t)blockIdx.z * g.M * g.N;
+  gemm_tile_t<double, double, double, double>(g.M, g.N, kn, g.A + (size_t)k0 * g.sac, g.sar, g.sac,
+                                              g.B + (size_t)k0 * g.sbr, g.sbr, g.sbc, P, rm ? g.N : 1,
+                                              rm ? 1 : g.M, 1.0, 0.0);
+}
+
+__global__ __launch_bounds__(256) void splitk_reduce_kernel(gemm_desc g, int S, const double* part)
+{
+  const long mn = (long)g.M * g.N;
+  const bool rm = g.scc == 1;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < mn; e += (long)gridDim.x * 256)
+  {
+    double sum = 0.0;
+    for (int z = 0; z < S; ++z) sum += part[(size_t)z * mn + e];
+    const long i = rm ? e / g.N : e % g.M, j = rm ? e % g.N : e / g.M;
+    double* c = g.C + i * g.scr + j * g.scc;
+    *c = g.beta == 0.0 ? g.alpha * sum : g.alpha * sum + g.beta * *c;
+  }
+}
+
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
@@ -761,6 +791,32 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st)
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return;
   dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT, batch);
   hipLaunchKernelGGL(gemm_f64_kernel, grid, dim3(256), 0, st, g);
+}
+
+// Slices for a single GEMM whose 64 x 64 tiles alone would leave most of
+// the 256 compute units idle: enough to reach about 256 workgroups, each
+// slice at least 64 deep in K; 1 when the tiles already fill the device.
+int gemm_splitk_slices(int M, int N, int K)
+{
+  const long tiles = (long)((M + GT - 1) / GT) * ((N + GT - 1) / GT);
+  if (tiles >= 128 || K < 128) return 1;
+  long s = std::min<long>(256 / tiles, K / 64);
+  s = std::min<long>(s, 16);
+  return s < 2 ? 1 : (int)s;
+}
+
+void launch_gemm_splitk(const gemm_desc& g, int S, double* part, hipStream_t st)
+{
+  if (g.M <= 0 || g.N <= 0) return;
+  // K per slice: a multiple of the 32-deep slab; trailing empty slices are
+  // dropped.
+  const int kper = ((g.K + S - 1) / S + GK - 1) / GK * GK;
+  const int s_eff = (g.K + kper - 1) / kper;
+  dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT, s_eff);
+  hipLaunchKernelGGL(gemm_f64_splitk_kernel, grid, dim3(256), 0, st, g, kper, part);
+  const long mn = (long)g.M * g.N;
+  const int rb = (int)std::min<long>((mn + 255) / 256, 2048);
+  hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb), dim3(256), 0, st, g, s_eff, part);
 }
 
 void launch_qr(const qr_desc& q, int batch, hipStream_t st)
