@@ -221,8 +221,8 @@ void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long s
   gemm_go(c, g);
 }
 
-// CholQR2 of a tall column-major (m x n) A, m >= 2n.  Pass 1: G = A^T A =
-// R1^T R1 (single-workgroup Cholesky), Q1 = A R1^-1.  Pass 2: G2 = Q1^T Q1
+// CholQR2 of a column-major (m x n) A, m >= n.  Pass 1: G = A^T A =
+// R1^T R1 and R1^-1 (blocked Cholesky, chol_inv_blocked), Q1 = A R1^-1.  Pass 2: G2 = Q1^T Q1
 // = I + E with E of the order u kappa(A)^2; its Cholesky factor R2 = I + X
 // is the fixed point X = Phi(E - X^T X) (Phi: strict upper part plus half
 // the diagonal), reached from X = Phi(E) by three iterations that each gain
@@ -242,23 +242,78 @@ struct rinv_norms
   double xf2 = 0.0, rf2 = 0.0;
 };
 
+// Blocked Cholesky G = R^T R of an SPD n x n matrix (row-major, leading
+// dimension n) and the inverse X = R^-1, both upper triangular with zero
+// strict lower part, in 64-row block steps.  Step J: one wave factors and
+// inverts the diagonal block (potrf_inv_kernel; D = R_JJ^-1), then on the
+// matrix cores
+//   R(J, >J)       = D^T G(J, >J)              (block row of R)
+//   G(>J, >J)     -= R(J, >J)^T R(J, >J)       (trailing update)
+//   X(<J, J)       = -X(<J, <J) R(<J, J) D     (block column of R^-1).
+// G is overwritten by the trailing updates; T is scratch of n * 64 doubles;
+// info[0..2] as for launch_potrf_inv.
+void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double* T, double* info)
+{
+  const int B = potrf_block();
+  auto gm = [&](int M, int N, int K, const double* A, long sar, long sac, const double* Bm, long sbr, long sbc,
+                double* C, long scr, long scc, double alpha, double beta)
+  {
+    gemm_desc g;
+    g.M = M;
+    g.N = N;
+    g.K = K;
+    g.A = A;
+    g.sar = sar;
+    g.sac = sac;
+    g.B = Bm;
+    g.sbr = sbr;
+    g.sbc = sbc;
+    g.C = C;
+    g.scr = scr;
+    g.scc = scc;
+    g.alpha = alpha;
+    g.beta = beta;
+    prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
+    gemm_go(c, g);
+  };
+  for (int j0 = 0; j0 < n; j0 += B)
+  {
+    const int jb = std::min(B, n - j0), j1 = j0 + jb, w = n - j1;
+    {
+      prof_scope ps(c, TTA_K_CHOL, 2.0 * jb * jb * jb / 3.0, 24.0 * jb * jb);
+      launch_potrf_inv(G, n, R, n, X, n, jb, j0, info, c->stream);
+    }
+    const double* D = X + (size_t)j0 * n + j0;
+    if (w > 0)
+    {
+      double* Rrow = R + (size_t)j0 * n + j1;
+      gm(jb, w, jb, D, 1, n, G + (size_t)j0 * n + j1, n, 1, Rrow, n, 1, 1.0, 0.0);
+      gm(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0);
+    }
+    if (j0 > 0)
+    {
+      gm(j0, jb, j0, X, n, 1, R + j0, n, 1, T, jb, 1, 1.0, 0.0);
+      gm(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0);
+    }
+  }
+}
+
 bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, rinv_norms* cert = nullptr)
 {
   const size_t nn = (size_t)n * n;
-  const int nb = rinv_cert_blocks(n);
   double* G = alloc_core(c, nn);
   double* X = alloc_core(c, nn);
   double* W = alloc_core(c, nn);
   double* Y = alloc_core(c, nn);
   double* R1 = alloc_core(c, nn);
   double* Q1 = alloc_core(c, (size_t)m * n);
-  double* aux = alloc_core(c, 8 + 2 * (size_t)nb);
+  double* aux = alloc_core(c, 8 + (size_t)kMinusIdentParts);
   // R1^-1 stays live for the certificate's R^-1 = R1^-1 R2^-1; otherwise its
   // buffer is reused by the second pass.
   double* Xi = cert ? alloc_core(c, nn) : X;
-  double h[8];
+  double h[8 + kMinusIdentParts];
   bool ok = false;
-  const double cn = (double)n * n * n / 3.0, cb = 16.0 * (double)nn;
+  const double cb = 16.0 * (double)nn;
   auto gm = [&](int M, int N, int K, const double* Am, long sar, long sac, const double* Bm, long sbr, long sbc,
                 double* Cm, long scr, long scc)
   {
@@ -273,24 +328,19 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   try
   {
     gm(n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
-    {
-      prof_scope ps(c, TTA_K_CHOL, cn, cb);
-      launch_chol(G, n, n, aux, c->stream);                  // G <- R1
-    }
-    hip_check(hipMemcpyAsync(R1, G, nn * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
-    {
-      prof_scope ps(c, TTA_K_RINV, cn, cb);
-      launch_rinv_cert(R1, n, aux + 8, c->stream, Xi, n);    // Xi = R1^-1
-    }
+    chol_inv_blocked(c, G, n, R1, Xi, W, aux);       // R1, Xi = R1^-1
     gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
     gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
     {
       prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
-      launch_minus_ident(G, n, aux + 4, c->stream);          // G = E, aux[4] = max |E|
+      launch_minus_ident(G, n, aux + 8, c->stream);          // G = E, partial max |E|
     }
-    hip_check(hipMemcpyAsync(h, aux, sizeof(h), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    const int np = minus_ident_parts(n);
+    hip_check(hipMemcpyAsync(h, aux, (8 + (size_t)np) * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    ok = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]) && h[4] * n <= 1.0e-4;
+    double emax = 0.0;
+    for (int b = 0; b < np; ++b) emax = std::max(emax, h[8 + b]);
+    ok = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]) && emax * n <= 1.0e-4;
     if (ok)
     {
       auto ew = [&](auto&& launch)
@@ -345,7 +395,7 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
 
 // QR of a column-major (m x n) matrix in place (R in the leading upper
 // triangle), thin Q (m x k) into Q when non-null.  With cholqr, tall
-// matrices (m >= 2n, 32 <= n <= chol_max_n()) go through CholQR2 first;
+// matrices (m >= n, 32 <= n <= chol_max_n()) go through CholQR2 first;
 // otherwise, and whenever CholQR2 declines, Householder: blocked compact-WY
 // kernels for m <= 2048, the single-workgroup unblocked kernel beyond.
 // Householder also leaves its reflectors below the diagonal and tau; no
@@ -356,7 +406,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
                rinv_norms* cert = nullptr)
 {
   if (cert) cert->have = false;
-  if (cholqr && c->cholqr && m >= 2 * n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
+  if (cholqr && c->cholqr && m >= n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
     return;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (m <= qr_blocked_max_rows())

@@ -722,30 +722,26 @@ __global__ __launch_bounds__(256) void rinv_cert_kernel(const double* __restrict
   }
 }
 
-// G <- G - I (n x n, row-major) and out[0] = max |G - I| (one workgroup).
-__global__ __launch_bounds__(1024) void minus_ident_kernel(double* G, int n, double* out)
+// G <- G - I (n x n, row-major); block b writes the largest |G - I| of its
+// share to part[b] (non-finite entries counted as infinity).
+__global__ __launch_bounds__(256) void minus_ident_kernel(double* G, int n, double* part)
 {
-  __shared__ double red[16];
+  __shared__ double red[4];
   double m = 0.0;
-  for (long e = threadIdx.x; e < (long)n * n; e += 1024)
-  {
-    const int i = (int)(e / n), j = (int)(e % n);
-    const double v = G[e] - (i == j ? 1.0 : 0.0);
-    G[e] = v;
-    m = fmax(m, fabs(v) <= DBL_MAX ? fabs(v) : INFINITY);
-  }
-  // max over the block (non-finite entries counted as infinity).
+  for (int i = blockIdx.x; i < n; i += gridDim.x)
+    for (int j = threadIdx.x; j < n; j += 256)
+    {
+      const size_t e = (size_t)i * n + j;
+      const double v = G[e] - (i == j ? 1.0 : 0.0);
+      G[e] = v;
+      m = fmax(m, fabs(v) <= DBL_MAX ? fabs(v) : INFINITY);
+    }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
   if (lane == 0) red[wid] = m;
   __syncthreads();
-  if (threadIdx.x == 0)
-  {
-    double t = 0.0;
-    for (int w = 0; w < 16; ++w) t = fmax(t, red[w]);
-    out[0] = t;
-  }
+  if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 // X = Phi(E - W) (n x n, row-major; W may be null): the strict upper part of
@@ -876,9 +872,11 @@ void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, doubl
   hipLaunchKernelGGL(rinv_cert_kernel, dim3(rinv_cert_blocks(q)), dim3(256), lds, st, R, q, out, X, ldx);
 }
 
-void launch_minus_ident(double* G, int n, double* out, hipStream_t st)
+int minus_ident_parts(int n) { return std::max(1, std::min(n, kMinusIdentParts)); }
+
+void launch_minus_ident(double* G, int n, double* part, hipStream_t st)
 {
-  hipLaunchKernelGGL(minus_ident_kernel, dim3(1), dim3(1024), 0, st, G, n, out);
+  hipLaunchKernelGGL(minus_ident_kernel, dim3(minus_ident_parts(n)), dim3(256), 0, st, G, n, part);
 }
 
 void launch_phi(const double* E, const double* W, double* X, int n, hipStream_t st)

@@ -222,6 +222,14 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st);
 // workgroup; n <= chol_max_n().  info[0] = 1 on breakdown, info[1..2] =
 // min / max diagonal of R.
 int chol_max_n();
+// Diagonal block step of the blocked Cholesky: the jb x jb block (jb <=
+// potrf_block()) at (j0, j0) of G factored as R_b^T R_b, R_b into R and
+// R_b^-1 into X at the same place, rows j0 .. j0+jb-1 of R and X zeroed left
+// of the block; info[0..2] = breakdown flag, min and max diagonal of R
+// (started at j0 = 0, folded into afterwards).
+int potrf_block();
+void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, int ldx, int jb, int j0, double* info,
+                      hipStream_t st);
 void launch_chol(double* G, int n, int ld, double* info, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
@@ -250,8 +258,11 @@ int rinv_cert_blocks(int q);
 void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
-// G <- G - I and out[0] = max |G - I| (row-major n x n).
-void launch_minus_ident(double* G, int n, double* out, hipStream_t st);
+// G <- G - I (row-major n x n); max |G - I| over the entries, as
+// minus_ident_parts(n) <= kMinusIdentParts partial maxima in part.
+constexpr int kMinusIdentParts = 64;
+int minus_ident_parts(int n);
+void launch_minus_ident(double* G, int n, double* part, hipStream_t st);
 // X = strict upper part of E - W plus half its diagonal (W may be null).
 void launch_phi(const double* E, const double* W, double* X, int n, hipStream_t st);
 // Y = alpha I + beta X.
