@@ -268,7 +268,7 @@ def main():
     for _ in range(args.prof_steps):
         y = T.matvec_round(A, x, eps=EPS)
     ctx.synchronize()
-    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "apply", "chol", "rinv", "other")}
+    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "apply", "chol", "rinv", "other")}
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
@@ -277,7 +277,8 @@ def main():
     # with the most device time.
     single = {"jacobi_round": ("bj_round_kernel", ctx.prof_read("jacobi_round")),
               "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"]),
-              "chol": ("chol_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}
+              "gemm_splitk": ("gemm_f64_splitk_kernel", stats["gemm_splitk"]),
+              "chol": ("potrf_inv_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}
     ctx.prof_enable(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     full_svd_ms = None

@@ -283,20 +283,24 @@ enum tta_kernel_class
 {
   TTA_K_QR = 0,     /* Householder QR + thin Q (CholQR2 parts: see below) */
   TTA_K_JACOBI = 1, /* one-sided Jacobi SVD + sort + truncation rank */
-  TTA_K_GEMM = 2,   /* FP64 contractions (absorb factors, U = Q J, inner) */
+  TTA_K_GEMM = 2,   /* FP64 contractions in one tiled launch */
   TTA_K_APPLY = 3,  /* MPO core contraction (matvec / matmat cores) */
-  TTA_K_OTHER = 4,  /* transposes, gathers, norms, balancing */
+  TTA_K_OTHER = 4,  /* transposes, gathers, norms, balancing, split-K sums */
   /* Each launch of the block-Jacobi round kernel (q > 64), credited an equal
    * share of its SVD call's textbook flops and bytes.  These launches are
    * also inside TTA_K_JACOBI, which times whole SVD calls. */
   TTA_K_JACOBI_ROUND = 5,
-  /* CholQR2's single-workgroup Cholesky (credited n^3/3 flops, 16 n^2 bytes)
-   * and the device triangular inverse of CholQR2 and of the full-rank
-   * certificate (n^3/3 flops, 16 n^2 bytes); CholQR2's products count as
-   * TTA_K_GEMM and its elementwise steps as TTA_K_OTHER. */
+  /* The diagonal-block step of CholQR2's blocked Cholesky (factor and
+   * inverse of a b x b block, b <= 64: credited 2 b^3 / 3 flops, 24 b^2
+   * bytes) and the triangular inverse of the full-rank certificate where
+   * the QR was Householder (n^3/3 flops, 16 n^2 bytes).  The blocked
+   * Cholesky's products count as GEMMs, CholQR2's elementwise steps as
+   * TTA_K_OTHER. */
   TTA_K_CHOL = 6,
   TTA_K_RINV = 7,
-  TTA_K_COUNT = 8
+  /* The partial products of a GEMM split along K (their sum: TTA_K_OTHER). */
+  TTA_K_GEMM_SPLITK = 8,
+  TTA_K_COUNT = 9
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

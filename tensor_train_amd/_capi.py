@@ -107,7 +107,8 @@ SIGNATURES = {
     "tta_mem_trim": (ctypes.c_int, [_vp]),
 }
 
-KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4, "jacobi_round": 5, "chol": 6, "rinv": 7}
+KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4, "jacobi_round": 5, "chol": 6, "rinv": 7,
+                  "gemm_splitk": 8}
 
 _lib = None
 

@@ -166,7 +166,6 @@ void gemm_rm(tta_ctx* c, int M, int N, int K, const double* A, long lda, const d
   g.C = C;
   g.scr = ldc;
   g.scc = 1;
-  prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
   gemm_go(c, g);
 }
 
@@ -190,14 +189,31 @@ enum
 
 // One GEMM on the context stream, split along K when its output tiles
 // alone would leave the device mostly idle (the partial products go
-// through the WS_SPLITK scratch slot).
+// through the WS_SPLITK scratch slot).  Timed per kernel: the tiled GEMM
+// as TTA_K_GEMM, the split-K partial products as TTA_K_GEMM_SPLITK (both
+// credited 2 M N K flops and their operand and result bytes) and the
+// reduction of the partials as TTA_K_OTHER.
 void gemm_go(tta_ctx* c, const gemm_desc& g)
 {
+  const double mk = (double)g.M * g.K, kn = (double)g.K * g.N, mn = (double)g.M * g.N;
+  const double fl = 2.0 * mn * g.K;
   const int S = gemm_splitk_slices(g.M, g.N, g.K);
   if (S > 1)
-    launch_gemm_splitk(g, S, ctx_ws(c, WS_SPLITK, (size_t)S * g.M * g.N), c->stream);
+  {
+    double* part = ctx_ws(c, WS_SPLITK, (size_t)S * g.M * g.N);
+    int s_eff = 0;
+    {
+      prof_scope ps(c, TTA_K_GEMM_SPLITK, fl, 8.0 * (mk + kn + S * mn));
+      s_eff = launch_gemm_splitk_parts(g, S, part, c->stream);
+    }
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 8.0 * (s_eff + (g.beta != 0.0 ? 2.0 : 1.0)) * mn);
+    launch_splitk_reduce(g, s_eff, part, c->stream);
+  }
   else
+  {
+    prof_scope ps(c, TTA_K_GEMM, fl, 8.0 * (mk + kn + mn));
     launch_gemm(g, 1, c->stream);
+  }
 }
 
 // C = A B with element strides, without a profiling scope of its own (for
@@ -273,7 +289,6 @@ void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double
     g.scc = scc;
     g.alpha = alpha;
     g.beta = beta;
-    prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
     gemm_go(c, g);
   };
   for (int j0 = 0; j0 < n; j0 += B)
@@ -316,10 +331,7 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   const double cb = 16.0 * (double)nn;
   auto gm = [&](int M, int N, int K, const double* Am, long sar, long sac, const double* Bm, long sbr, long sbc,
                 double* Cm, long scr, long scc)
-  {
-    prof_scope ps(c, TTA_K_GEMM, 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + (double)M * N));
-    gemm_raw(c, M, N, K, Am, sar, sac, Bm, sbr, sbc, Cm, scr, scc);
-  };
+  { gemm_raw(c, M, N, K, Am, sar, sac, Bm, sbr, sbc, Cm, scr, scc); };
   auto release = [&]()
   {
     for (double* p : {G, X, W, Y, R1, Q1, aux}) c->mem.put(p);
@@ -437,11 +449,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
   launch_qr(q, 1, c->stream);
 }
 
-void run_gemm(tta_ctx* c, const gemm_desc& g)
-{
-  prof_scope ps(c, TTA_K_GEMM, 2.0 * g.M * g.N * g.K, 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N));
-  gemm_go(c, g);
-}
+void run_gemm(tta_ctx* c, const gemm_desc& g) { gemm_go(c, g); }
 
 int read_int(tta_ctx* c, const int* dev)
 {
@@ -2540,7 +2548,7 @@ int tta_prof_enable(tta_ctx* c, int on)
   TTA_GUARD_BEGIN
   prof_collect(c);
   c->prof = on != 0;
-  for (int i = 0; i < 8; ++i)
+  for (int i = 0; i < TTA_K_COUNT; ++i)
   {
     c->prof_ms[i] = 0.0;
     c->prof_n[i] = 0;

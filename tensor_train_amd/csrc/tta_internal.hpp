@@ -97,10 +97,11 @@ struct tta_ctx
   };
   std::vector<prof_rec> prof_pending;
   std::vector<hipEvent_t> prof_free;
-  double prof_ms[8] = {0};
-  long long prof_n[8] = {0};
-  double prof_flops[8] = {0};
-  double prof_bytes[8] = {0};
+  // Per kernel class (tta_kernel_class; room for 16).
+  double prof_ms[16] = {0};
+  long long prof_n[16] = {0};
+  double prof_flops[16] = {0};
+  double prof_bytes[16] = {0};
   // Block-Jacobi launches (q > 64) since the last tta_prof_enable and the
   // block sweeps they ran (host-driven, so counted whether or not timing is on).
   long long jac_calls = 0, jac_sweeps = 0;

@@ -801,15 +801,21 @@ int gemm_splitk_slices(int M, int N, int K)
   return s < 2 ? 1 : (int)s;
 }
 
-void launch_gemm_splitk(const gemm_desc& g, int S, double* part, hipStream_t st)
+int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st)
 {
-  if (g.M <= 0 || g.N <= 0) return;
+  if (g.M <= 0 || g.N <= 0) return 0;
   // K per slice: a multiple of the 32-deep slab; trailing empty slices are
   // dropped.
   const int kper = ((g.K + S - 1) / S + GK - 1) / GK * GK;
   const int s_eff = (g.K + kper - 1) / kper;
   dim3 grid((g.N + GT - 1) / GT, (g.M + GT - 1) / GT, s_eff);
   hipLaunchKernelGGL(gemm_f64_splitk_kernel, grid, dim3(256), 0, st, g, kper, part);
+  return s_eff;
+}
+
+void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hipStream_t st)
+{
+  if (g.M <= 0 || g.N <= 0 || s_eff <= 0) return;
   const long mn = (long)g.M * g.N;
   const int rb = (int)std::min<long>((mn + 255) / 256, 2048);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb), dim3(256), 0, st, g, s_eff, part);

@@ -208,10 +208,13 @@ struct permute_desc
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // Split-K form of a single GEMM for shapes with few output tiles:
-// gemm_splitk_slices gives the slice count S (1: use launch_gemm), and
-// launch_gemm_splitk needs S * M * N doubles of scratch in part.
+// gemm_splitk_slices gives the slice count S (1: use launch_gemm);
+// launch_gemm_splitk_parts writes the partial products of the slices it
+// uses (returned; at most S) into part (S * M * N doubles), and
+// launch_splitk_reduce sums them into C with alpha and beta.
 int gemm_splitk_slices(int M, int N, int K);
-void launch_gemm_splitk(const gemm_desc& g, int S, double* part, hipStream_t st);
+int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st);
+void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hipStream_t st);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st);
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,

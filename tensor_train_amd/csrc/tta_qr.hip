@@ -632,14 +632,17 @@ __global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, dou
 // at the same place, and rows j0 .. j0+jb-1 of R and X are zeroed left of
 // the block.
 // Gaussian elimination with symmetric pivots on the augmented block
-// [G_b | I] (64 x 128, padded with the identity where jb < 64): after pivot
-// j, row j scaled by 1/sqrt(pivot) is row j of [R_b | R_b^-T], and the rows
-// below take M(k, :) -= M(j, k) M(j, :) / pivot.  256 threads hold the block
-// in 4 x 8 sub-blocks (rows tr + 16a, columns tc + 16b); per pivot the
-// owning threads publish the unscaled pivot row in LDS (double-buffered, one
-// barrier per pivot) and every thread reads the 4 row and 8 column factors
-// it needs.  The R_b^-T rows go to LDS and leave transposed, as columns of
-// X, at the end.
+// [G_b | I] (64 x 128, padded with the identity where jb < 64): pivot row j
+// at its turn, scaled by 1/sqrt(pivot), is row j of [R_b | R_b^-T], and the
+// rows below take M(k, :) -= M(j, k) M(j, :) / pivot.  256 threads hold the
+// block in 4 x 8 sub-blocks (rows tr + 16a, columns tc + 16b).  At pivot j
+// the owning threads store the unscaled row into the LDS history H[j]; after
+// one barrier every thread reads its 4 row and 8 column factors from there
+// and updates.  The loop carries nothing else: 1/pivot comes from the
+// hardware reciprocal and two Newton steps, and the square roots, the
+// scaling of the rows, the diagonal statistics and all global stores happen
+// after the last pivot (R_b row by row, R_b^-T transposed as the columns of
+// X_b).
 // info[0] = 1 on a non-positive or non-finite pivot, info[1] / info[2] =
 // smallest / largest diagonal entry of R; the launch with j0 = 0 starts
 // them, later ones fold into them.
@@ -649,11 +652,10 @@ constexpr int kPotrfB = 64;
 __global__ __launch_bounds__(256) void potrf_inv_kernel(const double* G, int ldg, double* R, int ldr, double* X,
                                                         int ldx, int jb, int j0, double* info)
 {
-  __shared__ double rowbuf[2][2 * kPotrfB];
-  __shared__ double Lt[kPotrfB][kPotrfB + 1];  // Lt[j][c] = (R_b^-T)(j, c) = X_b(c, j)
+  __shared__ double H[kPotrfB][2 * kPotrfB + 1];  // unscaled pivot rows of [G_b | I]
+  __shared__ double sqv[kPotrfB], rsv[kPotrfB];
   const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
   const double* Gb = G + (size_t)j0 * ldg + j0;
-  double* Rb = R + (size_t)j0 * ldr + j0;
   double g[4][8];
 #pragma unroll
   for (int a = 0; a < 4; ++a)
@@ -666,9 +668,6 @@ __global__ __launch_bounds__(256) void potrf_inv_kernel(const double* G, int ldg
       else
         g[a][b] = row == col - kPotrfB ? 1.0 : 0.0;
     }
-  bool bad = false;
-  double dmin = INFINITY, dmax = 0.0;
-  int par = 0;
 #pragma unroll
   for (int a = 0; a < 4; ++a)
   {
@@ -676,63 +675,49 @@ __global__ __launch_bounds__(256) void potrf_inv_kernel(const double* G, int ldg
     {
       const int j = 16 * a + t;
       if (j >= jb) break;
-      double* rb = rowbuf[par];
+      double* h = H[j];
       if (tr == t)
       {
 #pragma unroll
-        for (int b = 0; b < 8; ++b) rb[tc + 16 * b] = g[a][b];
+        for (int b = 0; b < 8; ++b) h[tc + 16 * b] = g[a][b];
       }
       __syncthreads();
-      const double piv = rb[j];
-      const double sq = sqrt(piv);
-      const double rs = 1.0 / sq;
-      const double inv = rs * rs;
-      if (!(piv > 0.0) || !(piv < INFINITY)) bad = true;
-      dmin = fmin(dmin, sq);
-      dmax = fmax(dmax, sq);
+      const double piv = h[j];
+      double inv = __builtin_amdgcn_rcp(piv);
+      inv = fma(inv, fma(-piv, inv, 1.0), inv);
+      inv = fma(inv, fma(-piv, inv, 1.0), inv);
       double cf[8];
 #pragma unroll
-      for (int b = 0; b < 8; ++b) cf[b] = rb[tc + 16 * b];
-      if (tr == t)
-      {
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-        {
-          const int col = tc + 16 * b;
-          if (col < jb) Rb[(size_t)j * ldr + col] = col > j ? cf[b] * rs : (col == j ? sq : 0.0);
-        }
-#pragma unroll
-        for (int b = 4; b < 8; ++b) Lt[j][tc + 16 * (b - 4)] = cf[b] * rs;
-      }
-#pragma unroll
-      for (int b = 0; b < 8; ++b) cf[b] *= inv;
+      for (int b = 0; b < 8; ++b) cf[b] = h[tc + 16 * b];
 #pragma unroll
       for (int a2 = a; a2 < 4; ++a2)
       {
         if (a2 == a && tr <= t) continue;
-        const double rf = rb[tr + 16 * a2];
+        const double rf = h[tr + 16 * a2] * inv;
 #pragma unroll
         for (int b = 0; b < 8; ++b) g[a2][b] = fma(-rf, cf[b], g[a2][b]);
       }
-      par ^= 1;
     }
   }
   __syncthreads();
-  // X_b(c, j) = Lt[j][c]: row c of X_b from column c of Lt.
-  double* Xb = X + (size_t)j0 * ldx + j0;
-  for (int e = tid; e < jb * kPotrfB; e += 256)
+  if (tid < jb)
   {
-    const int c = e / kPotrfB, j = e % kPotrfB;
-    if (j < jb) Xb[(size_t)c * ldx + j] = j >= c ? Lt[j][c] : 0.0;
+    const double sq = sqrt(H[tid][tid]);
+    sqv[tid] = sq;
+    rsv[tid] = 1.0 / sq;
   }
-  for (int e = tid; e < jb * j0; e += 256)
-  {
-    const int t = e / j0, col = e % j0;
-    R[(size_t)(j0 + t) * ldr + col] = 0.0;
-    X[(size_t)(j0 + t) * ldx + col] = 0.0;
-  }
+  __syncthreads();
   if (tid == 0)
   {
+    bool bad = false;
+    double dmin = INFINITY, dmax = 0.0;
+    for (int j = 0; j < jb; ++j)
+    {
+      const double piv = H[j][j];
+      if (!(piv > 0.0) || !(piv < INFINITY)) bad = true;
+      dmin = fmin(dmin, sqv[j]);
+      dmax = fmax(dmax, sqv[j]);
+    }
     if (j0 == 0)
     {
       info[0] = bad ? 1.0 : 0.0;
@@ -745,6 +730,25 @@ __global__ __launch_bounds__(256) void potrf_inv_kernel(const double* G, int ldg
       info[1] = fmin(info[1], dmin);
       info[2] = fmax(info[2], dmax);
     }
+  }
+  // R_b(r, c) = H[r][c] / sqrt(pivot r) above the diagonal; X_b(r, c) =
+  // (R_b^-T)(c, r) = H[c][64 + r] / sqrt(pivot c) on and above it.
+  double* Rb = R + (size_t)j0 * ldr + j0;
+  double* Xb = X + (size_t)j0 * ldx + j0;
+  for (int e = tid; e < jb * kPotrfB; e += 256)
+  {
+    const int r = e / kPotrfB, c = e % kPotrfB;
+    if (c < jb)
+    {
+      Rb[(size_t)r * ldr + c] = c > r ? H[r][c] * rsv[r] : (c == r ? sqv[r] : 0.0);
+      Xb[(size_t)r * ldx + c] = c >= r ? H[c][kPotrfB + r] * rsv[c] : 0.0;
+    }
+  }
+  for (int e = tid; e < jb * j0; e += 256)
+  {
+    const int t = e / j0, col = e % j0;
+    R[(size_t)(j0 + t) * ldr + col] = 0.0;
+    X[(size_t)(j0 + t) * ldx + col] = 0.0;
   }
 }
 }  // namespace
