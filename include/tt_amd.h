@@ -315,8 +315,10 @@ int tta_prof_jacobi_sweeps(tta_ctx* ctx, long long* calls, long long* sweeps);
 
 /* ---- full-rank certificate of the SVD steps (no reference counterpart) ---- */
 /* Before the Jacobi SVD of an SVD step of round / round_right_canonical, the
- * step's triangular factor R is inverted on the device; when
- * 1/||R^-1||_F (a lower bound on sigma_min) exceeds delta with margin and
+ * norms of the step's triangular factor R and of R^-1 are formed on the
+ * device (R^-1 from CholQR2's two passes when that QR ran, else by a
+ * triangular inverse); when 1/||R^-1||_F (a lower bound on sigma_min)
+ * exceeds delta with margin and
  * ||R|| ||R^-1|| <= 1e8, the rank rule provably keeps every singular value,
  * and the step keeps the tensor as Q R without the SVD (same ranks, same
  * tensor, another orthogonal gauge of the cores).  On by default;
@@ -327,11 +329,13 @@ int tta_set_rank_certificate(tta_ctx* ctx, int on);
 int tta_prof_rank_certificates(tta_ctx* ctx, long long* tried, long long* held);
 
 /* ---- CholQR2 for the sweeps' QR factorizations (no reference counterpart) */
-/* Tall factorizations (m >= 2n, 32 <= n <= 512) of the QR and SVD sweeps run
- * as CholQR2 (two Gram-Cholesky passes on the matrix cores) when it is
- * accepted: no Cholesky breakdown and a second-pass factor within a factor 2
- * of the identity, i.e. condition numbers up to about 1e7; otherwise, and
- * with on = 0, blocked Householder.  Both give a backward-stable Q R. */
+/* Factorizations with m >= n and 32 <= n <= 512 in the QR and SVD sweeps of
+ * single TTs run as CholQR2 (Gram on the matrix cores, blocked Cholesky with
+ * inverse, then a second pass whose factor is solved as a perturbation of
+ * the identity) when it is accepted: no Cholesky breakdown and
+ * n max|Q1^T Q1 - I| <= 1e-4, i.e. condition numbers up to about 1e6;
+ * otherwise, and with on = 0, blocked Householder.  Both give a
+ * backward-stable Q R. */
 int tta_set_cholqr(tta_ctx* ctx, int on);
 /* Factorizations where CholQR2 was tried and accepted, since the last
  * tta_prof_enable. */
