@@ -322,11 +322,10 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   double* Y = alloc_core(c, nn);
   double* R1 = alloc_core(c, nn);
   double* Q1 = alloc_core(c, (size_t)m * n);
-  double* aux = alloc_core(c, 8 + (size_t)kMinusIdentParts);
+  double* aux = alloc_core(c, 8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
   // R1^-1 stays live for the certificate's R^-1 = R1^-1 R2^-1; otherwise its
   // buffer is reused by the second pass.
   double* Xi = cert ? alloc_core(c, nn) : X;
-  double h[8 + kMinusIdentParts];
   bool ok = false;
   const double cb = 16.0 * (double)nn;
   auto gm = [&](int M, int N, int K, const double* Am, long sar, long sac, const double* Bm, long sbr, long sbc,
@@ -343,47 +342,51 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
     chol_inv_blocked(c, G, n, R1, Xi, W, aux);       // R1, Xi = R1^-1
     gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
     gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
+    auto ew = [&](auto&& launch)
     {
       prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
-      launch_minus_ident(G, n, aux + 8, c->stream);          // G = E, partial max |E|
+      launch();
+    };
+    ew([&] { launch_minus_ident(G, n, aux + 8, c->stream); });     // G = E, partial max |E|
+    // The second pass, Q, R and the certificate's norms are formed before the
+    // acceptance test is read, so that each factorization waits on the
+    // stream once; Q is scratch for the caller either way, and A is written
+    // only after acceptance.
+    ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
+    for (int it = 0; it < 3; ++it)
+    {
+      gm(n, n, n, X, 1, n, X, n, 1, W, n, 1);                       // W = X^T X
+      ew([&] { launch_phi(G, W, X, n, c->stream); });              // X = Phi(E - W)
     }
+    ew([&] { launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream); });  // W = R2 = I + X
+    ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
+    gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
+    ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
+    gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
+    if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
+    gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R = R2 R1
     const int np = minus_ident_parts(n);
-    hip_check(hipMemcpyAsync(h, aux, (8 + (size_t)np) * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    double* cn2 = aux + 8 + kMinusIdentParts;
+    if (cert)
+    {
+      // R^-1 = R1^-1 R2^-1, and the column norms of R and R^-1.
+      gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
+      ew([&] { launch_col_norms(G, n, n, cn2, c->stream); });
+      ew([&] { launch_col_norms(W, n, n, cn2 + n, c->stream); });
+    }
+    std::vector<double> h(8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
+    const size_t nh = cert ? h.size() : 8 + (size_t)np;
+    hip_check(hipMemcpyAsync(h.data(), aux, nh * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
     hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
     double emax = 0.0;
     for (int b = 0; b < np; ++b) emax = std::max(emax, h[8 + b]);
     ok = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]) && emax * n <= 1.0e-4;
     if (ok)
     {
-      auto ew = [&](auto&& launch)
-      {
-        prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
-        launch();
-      };
-      ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
-      for (int it = 0; it < 3; ++it)
-      {
-        gm(n, n, n, X, 1, n, X, n, 1, W, n, 1);                       // W = X^T X
-        ew([&] { launch_phi(G, W, X, n, c->stream); });              // X = Phi(E - W)
-      }
-      ew([&] { launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream); });  // W = R2 = I + X
-      ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
-      gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
-      ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
-      gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
-      if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
-      gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R2 R1
       ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });  // A's leading block = R
       if (cert)
       {
-        // R^-1 = R1^-1 R2^-1, and the column norms of R and R^-1.
-        gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
-        double* cn2 = Q1;  // Q1 is no longer read
-        ew([&] { launch_col_norms(G, n, n, cn2, c->stream); });
-        ew([&] { launch_col_norms(W, n, n, cn2 + n, c->stream); });
-        std::vector<double> hn(2 * (size_t)n);
-        hip_check(hipMemcpyAsync(hn.data(), cn2, hn.size() * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
-        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+        const double* hn = h.data() + 8 + kMinusIdentParts;
         cert->rf2 = cert->xf2 = 0.0;
         for (int j = 0; j < n; ++j)
         {
