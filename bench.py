@@ -243,8 +243,8 @@ def main():
     ap.add_argument("--train-tts", type=int, default=650)
     ap.add_argument("--mlp-steps", type=int, default=1500)
     ap.add_argument("--full-svd-steps", type=int, default=2,
-                    help="steps timed again with the full-rank certificate and CholQR2 off (every QR is "
-                         "Householder and every SVD step runs its SVD)")
+                    help="steps timed again with the full-rank certificate, CholQR2 and speculative sweeps "
+                         "off (every QR is Householder and every SVD step runs its SVD)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
@@ -271,6 +271,7 @@ def main():
     stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "apply", "chol", "rinv", "other")}
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
+    spec_sweeps, spec_failed = ctx.speculation_counts()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
     # round kernel, the FP64 GEMM, the MPO core contraction, CholQR2's
     # Cholesky, the triangular inverse); the roofline is quoted for the one
@@ -285,6 +286,7 @@ def main():
     if args.full_svd_steps > 0:
         ctx.set_rank_certificate(False)
         ctx.set_cholqr(False)
+        ctx.set_speculation(False)
         y_full = T.matvec_round(A, x, eps=EPS)
         ctx.synchronize()
         t2 = time.perf_counter()
@@ -294,6 +296,7 @@ def main():
         full_svd_ms = (time.perf_counter() - t2) * 1e3 / args.full_svd_steps
         ctx.set_rank_certificate(True)
         ctx.set_cholqr(True)
+        ctx.set_speculation(True)
         assert y_full.ranks == y.ranks
     alg_flops = algorithmic_flops(A, x, y.ranks)
 
@@ -333,6 +336,8 @@ def main():
         "kernel_ms_per_step": {k: v["ms"] / args.prof_steps for k, v in stats.items()},
         "algorithmic_gflop_per_step": alg_flops / 1e9,
         "achieved_gflops_per_step": alg_flops / ms_per_step / 1e6,
+        "speculative_sweeps": {"per_step": spec_sweeps / args.prof_steps,
+                               "undone_per_step": spec_failed / args.prof_steps},
         "cholqr2": {"tall_qr_per_step": cholqr_tried / args.prof_steps,
                     "cholqr2_used_per_step": cholqr_used / args.prof_steps},
         "rank_certificate": {"svd_steps_per_step": cert_tried / args.prof_steps,

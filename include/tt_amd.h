@@ -341,6 +341,19 @@ int tta_set_cholqr(tta_ctx* ctx, int on);
  * tta_prof_enable. */
 int tta_prof_cholqr(tta_ctx* ctx, long long* tried, long long* used);
 
+/* ---- speculative sweeps (no reference counterpart) ------------------------- */
+/* The QR and SVD sweeps of round / round_right_canonical (and of the naive
+ * matvec_round / matmat_round built on them) decide CholQR2 acceptance and
+ * the rank certificates on the device, into one flag read once per sweep
+ * instead of once per factorization.  A sweep with any failed decision is
+ * undone from a copy of the TT taken before it and run again with host
+ * decisions, so results equal those of on = 0; after such a fallback the
+ * next sweep of that kind runs checked, and speculation resumes once a
+ * checked sweep needed no fallback.  On by default. */
+int tta_set_speculation(tta_ctx* ctx, int on);
+/* Speculative sweeps run and those undone, since the last tta_prof_enable. */
+int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
+
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks
  * (tensor cores and workspaces), bytes cached for reuse, device allocations

@@ -99,6 +99,18 @@ class Context:
         falls back to Householder wherever it would lose accuracy)."""
         self.check(self.lib.tta_set_cholqr(self.h, 1 if on else 0))
 
+    def set_speculation(self, on: bool) -> None:
+        """Speculative sweeps (on by default): CholQR2 acceptance and the rank
+        certificates decided on the device, one host read per sweep; a sweep
+        with a failed decision is undone and run again checked."""
+        self.check(self.lib.tta_set_speculation(self.h, 1 if on else 0))
+
+    def speculation_counts(self):
+        """(sweeps, failed) speculative sweep counts since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_speculation(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def cholqr_counts(self):
         """(tried, used) CholQR2 counts since the last prof_enable."""
         a, b = ctypes.c_longlong(), ctypes.c_longlong()

@@ -184,8 +184,12 @@ enum
   WS_SMALL,
   WS_T,
   WS_CERT,
-  WS_SPLITK
+  WS_SPLITK,
+  WS_SPEC
 };
+
+// Flag of the running speculative sweep (1.0 while every decision held).
+double* spec_flag(tta_ctx* c) { return ctx_ws(c, WS_SPEC, 1); }
 
 // One GEMM on the context stream, split along K when its output tiles
 // alone would leave the device mostly idle (the partial products go
@@ -256,6 +260,9 @@ struct rinv_norms
 {
   bool have = false;
   double xf2 = 0.0, rf2 = 0.0;
+  // In a speculative sweep: the column norms of R and then of R^-1 on the
+  // device (2n values), for the certificate decided there.
+  const double* dev = nullptr;
 };
 
 // Blocked Cholesky G = R^T R of an SPD n x n matrix (row-major, leading
@@ -367,12 +374,32 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
     gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R = R2 R1
     const int np = minus_ident_parts(n);
     double* cn2 = aux + 8 + kMinusIdentParts;
-    if (cert)
+    if (cert && !c->spec)
     {
       // R^-1 = R1^-1 R2^-1, and the column norms of R and R^-1.
       gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
       ew([&] { launch_col_norms(G, n, n, cn2, c->stream); });
       ew([&] { launch_col_norms(W, n, n, cn2 + n, c->stream); });
+    }
+    if (c->spec)
+    {
+      // Decided on the device; R goes to A now, and a failed acceptance
+      // undoes the whole sweep.
+      ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, spec_flag(c), c->stream); });
+      ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });
+      if (cert)
+      {
+        gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
+        double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
+        ew([&] { launch_col_norms(G, n, n, cd, c->stream); });
+        ew([&] { launch_col_norms(W, n, n, cd + n, c->stream); });
+        cert->dev = cd;
+      }
+      ok = true;
+      release();
+      c->cholqr_tried += 1;
+      c->cholqr_used += 1;
+      return true;
     }
     std::vector<double> h(8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
     const size_t nh = cert ? h.size() : 8 + (size_t)np;
@@ -420,7 +447,11 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
 void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false,
                rinv_norms* cert = nullptr)
 {
-  if (cert) cert->have = false;
+  if (cert)
+  {
+    cert->have = false;
+    cert->dev = nullptr;
+  }
   if (cholqr && c->cholqr && m >= n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
     return;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
@@ -633,6 +664,25 @@ bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta, const r
 {
   if (!c->rank_certificate || q < 1 || q > kRinvCertMaxQ) return false;
   c->cert_tried += 1;
+  if (c->spec)
+  {
+    // Decided on the device into the sweep's flag; the step goes on as
+    // certified.
+    if (known && known->dev)
+      launch_spec_cert(known->dev, q, 0, delta, spec_flag(c), c->stream);
+    else
+    {
+      const int nb = rinv_cert_blocks(q);
+      double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
+      {
+        prof_scope ps(c, TTA_K_RINV, (double)q * q * q / 3.0, 16.0 * (double)q * q);
+        launch_rinv_cert(R, q, part, c->stream);
+      }
+      launch_spec_cert(part, nb, 1, delta, spec_flag(c), c->stream);
+    }
+    c->cert_held += 1;
+    return true;
+  }
   double xf2 = 0.0, rf2 = 0.0;
   if (known && known->have)
   {
@@ -1001,6 +1051,72 @@ void balance(tta_ctx* c, tta_tt* t)
   }
 }
 
+// Copies of a TT's cores and its ranks, to undo a speculative sweep.
+struct tt_snapshot
+{
+  std::vector<double*> cores;
+  std::vector<int> ranks;
+};
+
+tt_snapshot take_snapshot(tta_ctx* c, const tta_tt* t)
+{
+  tt_snapshot s;
+  s.ranks = t->ranks;
+  for (int k = 0; k < t->d(); ++k)
+  {
+    const size_t n = core_numel(t, k);
+    double* p = alloc_core(c, n);
+    hip_check(hipMemcpyAsync(p, t->cores[k], n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
+    s.cores.push_back(p);
+  }
+  return s;
+}
+
+// Runs one sweep of round over t.  Speculative (c->speculate and the last
+// sweep of this kind needed no fallback): CholQR2 acceptance and the rank
+// certificates are decided on the device into one flag, the host reads it
+// once at the end, and if any decision failed the sweep is undone from a
+// snapshot and run again with host decisions, so the result is always that
+// of the checked sweep.  Otherwise the checked sweep runs directly, and
+// speculation resumes once such a sweep has needed no fallback.
+template <class F>
+void run_sweep(tta_ctx* c, tta_tt* t, bool& last_ok, F&& sweep)
+{
+  if (!c->speculate || !last_ok)
+  {
+    const long long ct = c->cholqr_tried, cu = c->cholqr_used, et = c->cert_tried, eh = c->cert_held;
+    sweep();
+    last_ok = c->cholqr_tried - ct == c->cholqr_used - cu && c->cert_tried - et == c->cert_held - eh;
+    return;
+  }
+  tt_snapshot snap = take_snapshot(c, t);
+  launch_set_value(spec_flag(c), 1.0, c->stream);
+  c->spec = true;
+  try
+  {
+    sweep();
+  }
+  catch (...)
+  {
+    c->spec = false;
+    for (double* p : snap.cores) c->mem.put(p);
+    throw;
+  }
+  c->spec = false;
+  c->spec_sweeps += 1;
+  if (read_double(c, spec_flag(c)) == 1.0)
+  {
+    for (double* p : snap.cores) c->mem.put(p);
+    return;
+  }
+  c->spec_failed += 1;
+  last_ok = false;
+  for (int k = 0; k < t->d(); ++k) c->mem.put(t->cores[k]);
+  t->cores = snap.cores;
+  t->ranks = snap.ranks;
+  sweep();
+}
+
 // detail::round (tt_round.hpp:36-127) / round_right_canonical (:151-191) on
 // a working copy `t` (consumed in place).
 void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
@@ -1010,7 +1126,7 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
   const double inv = 1.0 / std::sqrt((double)std::max(d - 1, 1));
   if (gauge == TTA_GAUGE_RIGHT_CANONICAL)
   {
-    left_orthogonalize_impl(c, t);
+    run_sweep(c, t, c->spec_qr_ok, [&] { left_orthogonalize_impl(c, t); });
     const double nrm = frob_norm_dev(c, t->cores[d - 1], core_numel(t, d - 1), eps, inv);
     if (nrm == 0.0)
     {
@@ -1018,10 +1134,10 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
       return;
     }
     const double delta = eps * nrm * inv;
-    for (int k = d - 1; k >= 1; --k) svd_step_right(c, t, k, delta, max_rank);
+    run_sweep(c, t, c->spec_svd_ok, [&] { for (int k = d - 1; k >= 1; --k) svd_step_right(c, t, k, delta, max_rank); });
     return;
   }
-  right_orthogonalize_impl(c, t);
+  run_sweep(c, t, c->spec_qr_ok, [&] { right_orthogonalize_impl(c, t); });
   const double nrm = frob_norm_dev(c, t->cores[0], core_numel(t, 0), eps, inv);
   if (nrm == 0.0)
   {
@@ -1029,7 +1145,7 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
     return;
   }
   const double delta = eps * nrm * inv;
-  for (int k = 0; k < d - 1; ++k) svd_step(c, t, k, delta, max_rank);
+  run_sweep(c, t, c->spec_svd_ok, [&] { for (int k = 0; k < d - 1; ++k) svd_step(c, t, k, delta, max_rank); });
   balance(c, t);
 }
 
@@ -2564,6 +2680,8 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->cert_held = 0;
   c->cholqr_tried = 0;
   c->cholqr_used = 0;
+  c->spec_sweeps = 0;
+  c->spec_failed = 0;
   TTA_GUARD_END(c)
 }
 
@@ -2578,6 +2696,22 @@ int tta_set_cholqr(tta_ctx* c, int on)
 {
   if (!c) return TTA_ERR_ARG;
   c->cholqr = on != 0;
+  return TTA_OK;
+}
+
+int tta_set_speculation(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->speculate = on != 0;
+  c->spec_qr_ok = c->spec_svd_ok = true;
+  return TTA_OK;
+}
+
+int tta_prof_speculation(tta_ctx* c, long long* sweeps, long long* failed)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (sweeps) *sweeps = c->spec_sweeps;
+  if (failed) *failed = c->spec_failed;
   return TTA_OK;
 }
 

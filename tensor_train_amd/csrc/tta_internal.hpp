@@ -113,6 +113,15 @@ struct tta_ctx
   // and accepted counts.
   bool cholqr = true;
   long long cholqr_tried = 0, cholqr_used = 0;
+  // Speculative sweeps of round (on by default): CholQR2 acceptance and the
+  // rank certificates are decided on the device into one flag that the host
+  // reads after the sweep; a failed sweep is undone and run again with host
+  // decisions.  spec marks a running speculative sweep; *_ok record whether
+  // the last sweep of each kind needed no fallback (speculation resumes
+  // after a checked sweep that needed none).
+  bool speculate = true, spec = false;
+  bool spec_qr_ok = true, spec_svd_ok = true;
+  long long spec_sweeps = 0, spec_failed = 0;
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])

@@ -744,6 +744,57 @@ __global__ __launch_bounds__(256) void minus_ident_kernel(double* G, int n, doub
   if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// Device-side decisions of a speculative sweep: flag[0] starts at 1 and is
+// cleared by the first factorization or certificate that does not hold.
+// CholQR2 acceptance: no breakdown in pass 1 (info[0] == 0, finite extreme
+// diagonal entries) and n max|E| <= 1e-4 over the partial maxima.
+__global__ void spec_cholqr_kernel(const double* info, const double* part, int np, int n, double* flag)
+{
+  if (threadIdx.x != 0) return;
+  double emax = 0.0;
+  for (int b = 0; b < np; ++b) emax = fmax(emax, part[b]);
+  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= 1.0e-4;
+  if (!ok) flag[0] = 0.0;
+}
+
+// Full-rank certificate from ||R||_F^2 and ||R^-1||_F^2: mode 0 reads the
+// column norms of R (a[0..n)) and of R^-1 (a[n..2n)), mode 1 the n (xf2,
+// rf2) pairs of rinv_cert_kernel.  Same test as the host's: both sums finite
+// and positive, ||R|| ||R^-1|| <= 1e8, 1/(||R^-1|| (1+1e-3)) > delta (1+1e-3).
+__global__ __launch_bounds__(64) void spec_cert_kernel(const double* a, int n, int mode, double delta, double* flag)
+{
+  const int lane = threadIdx.x;
+  double x = 0.0, r = 0.0;
+  for (int i = lane; i < n; i += 64)
+  {
+    if (mode == 0)
+    {
+      r = fma(a[i], a[i], r);
+      x = fma(a[n + i], a[n + i], x);
+    }
+    else
+    {
+      x += a[2 * i];
+      r += a[2 * i + 1];
+    }
+  }
+  x = wave_sum(x);
+  r = wave_sum(r);
+  if (lane != 0) return;
+  bool held = isfinite(x) && isfinite(r) && x > 0.0 && r > 0.0;
+  if (held)
+  {
+    const double xf = sqrt(x), rf = sqrt(r);
+    held = rf * xf <= 1.0e8 && 1.0 / (xf * (1.0 + 1.0e-3)) > delta * (1.0 + 1.0e-3);
+  }
+  if (!held) flag[0] = 0.0;
+}
+
+__global__ void set_value_kernel(double* p, double v)
+{
+  if (threadIdx.x == 0) p[0] = v;
+}
+
 // X = Phi(E - W) (n x n, row-major; W may be null): the strict upper part of
 // E - W, half its diagonal, zero below.
 __global__ void phi_kernel(const double* E, const double* W, double* X, int n)
@@ -897,6 +948,21 @@ void launch_ident_axpy(const double* X, double* Y, int n, double alpha, double b
   long g = ((long)n * n + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(ident_axpy_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, X, Y, n, alpha, beta);
+}
+
+void launch_spec_cholqr(const double* info, const double* part, int np, int n, double* flag, hipStream_t st)
+{
+  hipLaunchKernelGGL(spec_cholqr_kernel, dim3(1), dim3(64), 0, st, info, part, np, n, flag);
+}
+
+void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st)
+{
+  hipLaunchKernelGGL(spec_cert_kernel, dim3(1), dim3(64), 0, st, a, n, mode, delta, flag);
+}
+
+void launch_set_value(double* p, double v, hipStream_t st)
+{
+  hipLaunchKernelGGL(set_value_kernel, dim3(1), dim3(64), 0, st, p, v);
 }
 
 void launch_eye(double* dst, int n, long ld, hipStream_t st)

@@ -259,6 +259,15 @@ void launch_transpose(const double* in, int ldi, int rows, int cols, double* out
 constexpr int kRinvCertMaxQ = 1024;
 int rinv_cert_blocks(int q);
 void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
+// Decisions of a speculative sweep on the device (flag[0] cleared on the
+// first failure): CholQR2 acceptance from its info[0..2] and the np partial
+// maxima of |E|; the full-rank certificate from the column norms of R and
+// R^-1 (mode 0, a = 2n values) or the n (xf2, rf2) pairs of
+// launch_rinv_cert (mode 1).
+void launch_spec_cholqr(const double* info, const double* part, int np, int n, double* flag, hipStream_t st);
+void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
+// p[0] = v.
+void launch_set_value(double* p, double v, hipStream_t st);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
 // G <- G - I (row-major n x n); max |G - I| over the entries, as
