@@ -241,6 +241,38 @@ void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long s
   gemm_go(c, g);
 }
 
+// gemm_go with an epilogue folded into the split-K sum (splitk_epi); only
+// for products that split (gemm_splitk_slices > 1), so that the sum runs
+// after every partial product has read its operands and may overwrite one.
+void gemm_go_epi(tta_ctx* c, int M, int N, int K, const double* A, long sar, long sac, const double* B, long sbr,
+                 long sbc, double* C, long scr, long scc, double alpha, const splitk_epi& ep)
+{
+  gemm_desc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sar = sar;
+  g.sac = sac;
+  g.B = B;
+  g.sbr = sbr;
+  g.sbc = sbc;
+  g.C = C;
+  g.scr = scr;
+  g.scc = scc;
+  g.alpha = alpha;
+  const double mk = (double)M * K, kn = (double)K * N, mn = (double)M * N;
+  const int S = gemm_splitk_slices(M, N, K);
+  double* part = ctx_ws(c, WS_SPLITK, (size_t)S * mn);
+  int s_eff = 0;
+  {
+    prof_scope ps(c, TTA_K_GEMM_SPLITK, 2.0 * mn * K, 8.0 * (mk + kn + S * mn));
+    s_eff = launch_gemm_splitk_parts(g, S, part, c->stream);
+  }
+  prof_scope ps(c, TTA_K_OTHER, 0.0, 8.0 * (s_eff + 3.0) * mn);
+  launch_splitk_reduce_epi(g, s_eff, part, ep, c->stream);
+}
+
 // CholQR2 of a column-major (m x n) A, m >= n.  Pass 1: G = A^T A =
 // R1^T R1 and R1^-1 (blocked Cholesky, chol_inv_blocked), Q1 = A R1^-1.  Pass 2: G2 = Q1^T Q1
 // = I + E with E of the order u kappa(A)^2; its Cholesky factor R2 = I + X
@@ -348,28 +380,53 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
     gm(n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
     chol_inv_blocked(c, G, n, R1, Xi, W, aux);       // R1, Xi = R1^-1
     gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
-    gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);       // G = Q1^T Q1
     auto ew = [&](auto&& launch)
     {
       prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
       launch();
     };
-    ew([&] { launch_minus_ident(G, n, aux + 8, c->stream); });     // G = E, partial max |E|
     // The second pass, Q, R and the certificate's norms are formed before the
     // acceptance test is read, so that each factorization waits on the
     // stream once; Q is scratch for the caller either way, and A is written
-    // only after acceptance.
-    ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
-    for (int it = 0; it < 3; ++it)
+    // only after acceptance.  Where the products split along K, the
+    // elementwise steps ride on the sums of the partial products.
+    if (gemm_splitk_slices(n, n, m) > 1 && gemm_splitk_slices(n, n, n) > 1)
     {
-      gm(n, n, n, X, 1, n, X, n, 1, W, n, 1);                       // W = X^T X
-      ew([&] { launch_phi(G, W, X, n, c->stream); });              // X = Phi(E - W)
+      splitk_epi ep;
+      ep.mode = SPLITK_EPI_GRAM;
+      ep.out2 = X;
+      ep.part = aux + 8;
+      gemm_go_epi(c, n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1, 1.0, ep);  // G = E, X = Phi(E), max |E|
+      for (int it = 0; it < 3; ++it)
+      {
+        splitk_epi e2;
+        e2.mode = it < 2 ? SPLITK_EPI_PHI : SPLITK_EPI_PHI_LAST;
+        e2.E = G;
+        e2.out2 = W;
+        e2.out3 = G;
+        gemm_go_epi(c, n, n, n, X, 1, n, X, n, 1, X, n, 1, 1.0, e2);  // X = Phi(E - X^T X); last: W = I + X, G = I - X
+      }
+      splitk_epi e3;
+      e3.mode = SPLITK_EPI_DIAG;
+      e3.diag = 2.0;
+      gemm_go_epi(c, n, n, n, W, n, 1, G, n, 1, X, n, 1, -1.0, e3);   // X = 2 I - R2 Y0
     }
-    ew([&] { launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream); });  // W = R2 = I + X
-    ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
-    gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
-    ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
-    gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
+    else
+    {
+      gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);                       // G = Q1^T Q1
+      ew([&] { launch_minus_ident(G, n, aux + 8, c->stream); });     // G = E, partial max |E|
+      ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
+      for (int it = 0; it < 3; ++it)
+      {
+        gm(n, n, n, X, 1, n, X, n, 1, W, n, 1);                       // W = X^T X
+        ew([&] { launch_phi(G, W, X, n, c->stream); });              // X = Phi(E - W)
+      }
+      ew([&] { launch_ident_axpy(X, W, n, 1.0, 1.0, c->stream); });  // W = R2 = I + X
+      ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
+      gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
+      ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
+    }
+    gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                           // Y = R2^-1
     if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
     gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R = R2 R1
     const int np = minus_ident_parts(n);

@@ -204,6 +204,60 @@ __global__ __launch_bounds__(256) void splitk_reduce_kernel(gemm_desc g, int S, 
   }
 }
 
+// Split-K sum with one of the epilogues of splitk_epi (square row-major
+// n x n output).
+__global__ __launch_bounds__(256) void splitk_reduce_epi_kernel(gemm_desc g, int S, const double* part, splitk_epi ep)
+{
+  __shared__ double red[4];
+  const int n = g.N;
+  const long mn = (long)n * n;
+  double emax = 0.0;
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < mn; e += (long)gridDim.x * 256)
+  {
+    double sum = 0.0;
+    for (int z = 0; z < S; ++z) sum += part[(size_t)z * mn + e];
+    const int i = (int)(e / n), j = (int)(e % n);
+    double* c = g.C + (size_t)i * g.scr + j;
+    const double v = g.beta == 0.0 ? g.alpha * sum : g.alpha * sum + g.beta * *c;
+    const double id = i == j ? 1.0 : 0.0;
+    switch (ep.mode)
+    {
+      case SPLITK_EPI_GRAM:
+      {
+        const double x = v - id;
+        *c = x;
+        ep.out2[e] = i < j ? x : (i == j ? 0.5 * x : 0.0);
+        emax = fmax(emax, fabs(x) <= DBL_MAX ? fabs(x) : INFINITY);
+        break;
+      }
+      case SPLITK_EPI_PHI:
+      case SPLITK_EPI_PHI_LAST:
+      {
+        const double d = ep.E[e] - v;
+        const double x = i < j ? d : (i == j ? 0.5 * d : 0.0);
+        *c = x;
+        if (ep.mode == SPLITK_EPI_PHI_LAST)
+        {
+          ep.out2[e] = id + x;
+          ep.out3[e] = id - x;
+        }
+        break;
+      }
+      default:
+        *c = v + ep.diag * id;
+    }
+  }
+  if (ep.mode == SPLITK_EPI_GRAM)
+  {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    if (lane == 0) red[wid] = emax;
+    __syncthreads();
+    if (threadIdx.x == 0) ep.part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+  }
+}
+
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
@@ -870,6 +924,15 @@ void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hip
   const long mn = (long)g.M * g.N;
   const int rb = (int)std::min<long>((mn + 255) / 256, 2048);
   hipLaunchKernelGGL(splitk_reduce_kernel, dim3(rb), dim3(256), 0, st, g, s_eff, part);
+}
+
+void launch_splitk_reduce_epi(const gemm_desc& g, int s_eff, const double* part, const splitk_epi& ep,
+                              hipStream_t st)
+{
+  if (g.M <= 0 || g.N <= 0 || s_eff <= 0) return;
+  const long mn = (long)g.M * g.N;
+  const int rb = ep.mode == SPLITK_EPI_GRAM ? minus_ident_parts(g.N) : (int)std::min<long>((mn + 255) / 256, 2048);
+  hipLaunchKernelGGL(splitk_reduce_epi_kernel, dim3(rb), dim3(256), 0, st, g, s_eff, part, ep);
 }
 
 void launch_qr(const qr_desc& q, int batch, hipStream_t st)

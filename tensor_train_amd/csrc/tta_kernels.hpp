@@ -213,6 +213,34 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // uses (returned; at most S) into part (S * M * N doubles), and
 // launch_splitk_reduce sums them into C with alpha and beta.
 int gemm_splitk_slices(int M, int N, int K);
+// Epilogues folded into the split-K sum of a square row-major n x n product
+// v = alpha P + beta C (CholQR2's second pass):
+//   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E), part[b] = max |E| over
+//                    block b's entries (non-finite counted as infinity);
+//   SPLITK_EPI_PHI:  C = Phi(E - v) for the E given;
+//   SPLITK_EPI_PHI_LAST: as PHI, and out2 = I + C, out3 = I - C;
+//   SPLITK_EPI_DIAG: C = v + diag I.
+// Phi keeps the strict upper part and half the diagonal, zero below.
+enum
+{
+  SPLITK_EPI_GRAM = 1,
+  SPLITK_EPI_PHI = 2,
+  SPLITK_EPI_PHI_LAST = 3,
+  SPLITK_EPI_DIAG = 4
+};
+struct splitk_epi
+{
+  int mode = 0;
+  const double* E = nullptr;
+  double* out2 = nullptr;
+  double* out3 = nullptr;
+  double* part = nullptr;
+  double diag = 0.0;
+};
+// Sum of the partials with an epilogue; with SPLITK_EPI_GRAM the launch has
+// minus_ident_parts(n) blocks (one partial maximum each).
+void launch_splitk_reduce_epi(const gemm_desc& g, int s_eff, const double* part, const splitk_epi& ep,
+                              hipStream_t st);
 int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st);
 void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hipStream_t st);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
