@@ -273,18 +273,6 @@ void gemm_go_epi(tta_ctx* c, int M, int N, int K, const double* A, long sar, lon
   launch_splitk_reduce_epi(g, s_eff, part, ep, c->stream);
 }
 
-// CholQR2 of a column-major (m x n) A, m >= n.  Pass 1: G = A^T A =
-// R1^T R1 and R1^-1 (blocked Cholesky, chol_inv_blocked), Q1 = A R1^-1.  Pass 2: G2 = Q1^T Q1
-// = I + E with E of the order u kappa(A)^2; its Cholesky factor R2 = I + X
-// is the fixed point X = Phi(E - X^T X) (Phi: strict upper part plus half
-// the diagonal), reached from X = Phi(E) by three iterations that each gain
-// a factor ~2||E||, and R2^-1 is one Newton step from I - X (error
-// ||X||^4); Q = Q1 R2^-1, R = R2 R1.  Every product runs on the matrix
-// cores.  A is read only until the factorization is accepted: no breakdown
-// in pass 1 and n max|E| <= 1e-4 (condition numbers up to about 1e6), which
-// bounds the pass-2 errors far below rounding.  Then R goes to A's leading
-// n x n block (upper, zeros below) and Q to Q; otherwise nothing is written
-// and the caller factors A by Householder.
 // Squared Frobenius norms of a factorization's triangular factor R and of
 // R^-1, when the factorization formed R^-1 anyway (CholQR2 does); the
 // full-rank certificate then needs no triangular inverse of its own.
@@ -352,69 +340,122 @@ void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double
   }
 }
 
+// CholQR2 of a column-major (m x n) A, m >= n: A = Q R with Q^T Q = I.
+// Pass 1: G = A^T A = R1^T R1 and R1^-1 (blocked Cholesky, chol_inv_blocked),
+// Q1 = A R1^-1.  Pass 2 corrects Q1 from G2 = Q1^T Q1 = I + E, where E is of
+// the order u kappa(A)^2:
+//   * polar form (n max|E| <= 1e-5): P = I - E/2 + 3E^2/8 and
+//     Pi = I + E/2 - E^2/8, the second-order series of (I + E)^(-1/2) and
+//     (I + E)^(1/2); Q = Q1 P, R = Pi R1, R^-1 = R1^-1 P.  Q^T Q = I and
+//     Q R = A hold up to O(||E||^3) (below 1e-15) and rounding.  R is full,
+//     not triangular: Q's columns are the symmetric orthogonalization of
+//     Q1's.  Four products after G2.
+//   * triangular form (n max|E| <= 1e-4, condition numbers up to about 1e6):
+//     R2 = I + X with X the fixed point of X = Phi(E - X^T X) (Phi: strict
+//     upper part plus half the diagonal), reached from X = Phi(E) in three
+//     iterations that each gain a factor ~2||E||; R2^-1 by one Newton step
+//     from I - X (error ||X||^4); Q = Q1 R2^-1, R = R2 R1,
+//     R^-1 = R1^-1 R2^-1.
+// The polar form is formed before the acceptance test is read, so that each
+// factorization waits on the stream once, and a speculative sweep (which
+// accepts the polar form only, deciding on the device) not at all; the
+// triangular form runs only when the test asks for it.  A is read only until
+// the factorization is accepted; then R goes to A's leading n x n block (all
+// of it) and Q to Q (scratch for the caller either way).  Otherwise nothing
+// is written and the caller factors A by Householder.  Every product runs on
+// the matrix cores.
 bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, rinv_norms* cert = nullptr)
 {
   const size_t nn = (size_t)n * n;
-  double* G = alloc_core(c, nn);
+  double* G = alloc_core(c, nn);  // Gram, then E
   double* X = alloc_core(c, nn);
   double* W = alloc_core(c, nn);
   double* Y = alloc_core(c, nn);
   double* R1 = alloc_core(c, nn);
+  double* Rf = alloc_core(c, nn);  // R of the accepted form
+  double* Xi = alloc_core(c, nn);  // R1^-1
   double* Q1 = alloc_core(c, (size_t)m * n);
   double* aux = alloc_core(c, 8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
-  // R1^-1 stays live for the certificate's R^-1 = R1^-1 R2^-1; otherwise its
-  // buffer is reused by the second pass.
-  double* Xi = cert ? alloc_core(c, nn) : X;
   bool ok = false;
   const double cb = 16.0 * (double)nn;
+  const int np = minus_ident_parts(n);
+  double* cn2 = aux + 8 + kMinusIdentParts;
   auto gm = [&](int M, int N, int K, const double* Am, long sar, long sac, const double* Bm, long sbr, long sbc,
                 double* Cm, long scr, long scc)
   { gemm_raw(c, M, N, K, Am, sar, sac, Bm, sbr, sbc, Cm, scr, scc); };
+  auto ew = [&](auto&& launch)
+  {
+    prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
+    launch();
+  };
   auto release = [&]()
   {
-    for (double* p : {G, X, W, Y, R1, Q1, aux}) c->mem.put(p);
-    if (Xi != X) c->mem.put(Xi);
+    for (double* p : {G, X, W, Y, R1, Rf, Xi, Q1, aux}) c->mem.put(p);
   };
+  // Where the products split along K, the elementwise steps ride on the
+  // sums of their partial products.
+  const bool split = gemm_splitk_slices(n, n, m) > 1 && gemm_splitk_slices(n, n, n) > 1;
   try
   {
     gm(n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
     chol_inv_blocked(c, G, n, R1, Xi, W, aux);       // R1, Xi = R1^-1
     gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
-    auto ew = [&](auto&& launch)
-    {
-      prof_scope ps(c, TTA_K_OTHER, 0.0, cb);
-      launch();
-    };
-    // The second pass, Q, R and the certificate's norms are formed before the
-    // acceptance test is read, so that each factorization waits on the
-    // stream once; Q is scratch for the caller either way, and A is written
-    // only after acceptance.  Where the products split along K, the
-    // elementwise steps ride on the sums of the partial products.
-    if (gemm_splitk_slices(n, n, m) > 1 && gemm_splitk_slices(n, n, n) > 1)
+    if (split)
     {
       splitk_epi ep;
       ep.mode = SPLITK_EPI_GRAM;
-      ep.out2 = X;
       ep.part = aux + 8;
-      gemm_go_epi(c, n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1, 1.0, ep);  // G = E, X = Phi(E), max |E|
-      for (int it = 0; it < 3; ++it)
-      {
-        splitk_epi e2;
-        e2.mode = it < 2 ? SPLITK_EPI_PHI : SPLITK_EPI_PHI_LAST;
-        e2.E = G;
-        e2.out2 = W;
-        e2.out3 = G;
-        gemm_go_epi(c, n, n, n, X, 1, n, X, n, 1, X, n, 1, 1.0, e2);  // X = Phi(E - X^T X); last: W = I + X, G = I - X
-      }
-      splitk_epi e3;
-      e3.mode = SPLITK_EPI_DIAG;
-      e3.diag = 2.0;
-      gemm_go_epi(c, n, n, n, W, n, 1, G, n, 1, X, n, 1, -1.0, e3);   // X = 2 I - R2 Y0
+      gemm_go_epi(c, n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1, 1.0, ep);  // G = E, max |E|
+      splitk_epi pp;
+      pp.mode = SPLITK_EPI_POLAR;
+      pp.E = G;
+      pp.out2 = W;
+      gemm_go_epi(c, n, n, n, G, n, 1, G, n, 1, X, n, 1, 1.0, pp);    // X = P, W = Pi
     }
     else
     {
       gm(n, n, m, Q1, m, 1, Q1, 1, m, G, n, 1);                       // G = Q1^T Q1
-      ew([&] { launch_minus_ident(G, n, aux + 8, c->stream); });     // G = E, partial max |E|
+      ew([&] { launch_minus_ident(G, n, aux + 8, c->stream); });     // G = E, max |E|
+      gm(n, n, n, G, n, 1, G, n, 1, Y, n, 1);                         // Y = E E
+      ew([&] { launch_polar_series(G, Y, X, W, n, c->stream); });    // X = P, W = Pi
+    }
+    if (Q) gm(m, n, n, Q1, 1, m, X, n, 1, Q, 1, ldq);                // Q = Q1 P
+    gm(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1);                         // Rf = Pi R1
+    if (cert) gm(n, n, n, Xi, n, 1, X, n, 1, Y, n, 1);                // Y = R^-1 = R1^-1 P
+    if (c->spec)
+    {
+      // Decided on the device; R goes to A now, and a failed acceptance
+      // undoes the whole sweep.
+      ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), c->stream); });
+      ew([&] { launch_transpose(Rf, n, n, n, A, lda, c->stream); });
+      if (cert)
+      {
+        double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
+        ew([&] { launch_col_norms(Rf, n, n, cd, c->stream); });
+        ew([&] { launch_col_norms(Y, n, n, cd + n, c->stream); });
+        cert->dev = cd;
+      }
+      release();
+      c->cholqr_tried += 1;
+      c->cholqr_used += 1;
+      return true;
+    }
+    if (cert)
+    {
+      ew([&] { launch_col_norms(Rf, n, n, cn2, c->stream); });
+      ew([&] { launch_col_norms(Y, n, n, cn2 + n, c->stream); });
+    }
+    std::vector<double> h(8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
+    const size_t nh = cert ? h.size() : 8 + (size_t)np;
+    hip_check(hipMemcpyAsync(h.data(), aux, nh * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    double emax = 0.0;
+    for (int b = 0; b < np; ++b) emax = std::max(emax, h[8 + b]);
+    const bool pass1 = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]);
+    ok = pass1 && emax * n <= 1.0e-4;
+    if (ok && emax * n > 1.0e-5)
+    {
+      // Triangular form (the polar one is not accurate enough here).
       ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
       for (int it = 0; it < 3; ++it)
       {
@@ -425,49 +466,23 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
       ew([&] { launch_ident_axpy(X, G, n, 1.0, -1.0, c->stream); }); // G = Y0 = I - X
       gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
       ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
-    }
-    gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                           // Y = R2^-1
-    if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 Y
-    gm(n, n, n, W, n, 1, R1, n, 1, G, n, 1);                        // G = R = R2 R1
-    const int np = minus_ident_parts(n);
-    double* cn2 = aux + 8 + kMinusIdentParts;
-    if (cert && !c->spec)
-    {
-      // R^-1 = R1^-1 R2^-1, and the column norms of R and R^-1.
-      gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
-      ew([&] { launch_col_norms(G, n, n, cn2, c->stream); });
-      ew([&] { launch_col_norms(W, n, n, cn2 + n, c->stream); });
-    }
-    if (c->spec)
-    {
-      // Decided on the device; R goes to A now, and a failed acceptance
-      // undoes the whole sweep.
-      ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, spec_flag(c), c->stream); });
-      ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });
+      gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
+      if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 R2^-1
+      gm(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1);                       // Rf = R2 R1
       if (cert)
       {
-        gm(n, n, n, Xi, n, 1, Y, n, 1, W, n, 1);
-        double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
-        ew([&] { launch_col_norms(G, n, n, cd, c->stream); });
-        ew([&] { launch_col_norms(W, n, n, cd + n, c->stream); });
-        cert->dev = cd;
+        gm(n, n, n, Xi, n, 1, Y, n, 1, X, n, 1);                      // X = R^-1
+        ew([&] { launch_col_norms(Rf, n, n, cn2, c->stream); });
+        ew([&] { launch_col_norms(X, n, n, cn2 + n, c->stream); });
+        hip_check(hipMemcpyAsync(h.data() + 8 + kMinusIdentParts, cn2, 2 * (size_t)n * 8, hipMemcpyDeviceToHost,
+                                 c->stream),
+                  "hipMemcpyAsync");
+        hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
       }
-      ok = true;
-      release();
-      c->cholqr_tried += 1;
-      c->cholqr_used += 1;
-      return true;
     }
-    std::vector<double> h(8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
-    const size_t nh = cert ? h.size() : 8 + (size_t)np;
-    hip_check(hipMemcpyAsync(h.data(), aux, nh * 8, hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
-    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
-    double emax = 0.0;
-    for (int b = 0; b < np; ++b) emax = std::max(emax, h[8 + b]);
-    ok = h[0] == 0.0 && std::isfinite(h[1]) && std::isfinite(h[2]) && emax * n <= 1.0e-4;
     if (ok)
     {
-      ew([&] { launch_transpose(G, n, n, n, A, lda, c->stream); });  // A's leading block = R
+      ew([&] { launch_transpose(Rf, n, n, n, A, lda, c->stream); });  // A's leading block = R
       if (cert)
       {
         const double* hn = h.data() + 8 + kMinusIdentParts;
@@ -492,16 +507,18 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   return ok;
 }
 
-// QR of a column-major (m x n) matrix in place (R in the leading upper
-// triangle), thin Q (m x k) into Q when non-null.  With cholqr, tall
-// matrices (m >= n, 32 <= n <= chol_max_n()) go through CholQR2 first;
-// otherwise, and whenever CholQR2 declines, Householder: blocked compact-WY
-// kernels for m <= 2048, the single-workgroup unblocked kernel beyond.
-// Householder also leaves its reflectors below the diagonal and tau; no
-// caller of the CholQR2 form reads them.
+// QR of a column-major (m x n) matrix in place (R in the leading rows),
+// thin Q (m x k) into Q when non-null.  With cholqr, matrices with m >= n
+// and 32 <= n <= chol_max_n() go through CholQR2 first; otherwise, and
+// whenever CholQR2 declines, Householder: blocked compact-WY kernels for
+// m <= 2048, the single-workgroup unblocked kernel beyond.
+// Returns true when CholQR2 made the factorization: R then fills A's
+// leading n x n block and need not be triangular (read it with
+// launch_extract_upper(..., full = true)).  Householder leaves R in the upper
+// triangle with its reflectors below the diagonal and tau.
 // With cert non-null, a CholQR2 factorization also reports the norms of R and
 // R^-1 there (cert->have); a Householder one leaves cert->have false.
-void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false,
+bool qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false,
                rinv_norms* cert = nullptr)
 {
   if (cert)
@@ -510,7 +527,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
     cert->dev = nullptr;
   }
   if (cholqr && c->cholqr && m >= n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
-    return;
+    return true;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (m <= qr_blocked_max_rows())
   {
@@ -527,7 +544,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
     b.max_m = m;
     b.max_n = n;
     launch_qr_blocked(b, c->stream);
-    return;
+    return false;
   }
   qr_desc q;
   q.A = A;
@@ -538,6 +555,7 @@ void qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
   q.Q = Q;
   q.ldq = ldq;
   launch_qr(q, 1, c->stream);
+  return false;
 }
 
 void run_gemm(tta_ctx* c, const gemm_desc& g) { gemm_go(c, g); }
@@ -584,7 +602,7 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   double* A = t->cores[k];
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
-  qr_factor(c, A, m, m, r, tau, Qn, m, true);
+  const bool full = qr_factor(c, A, m, m, r, tau, Qn, m, true);
   if (!absorb)
   {
     replace_core(c, t, k, Qn);
@@ -592,7 +610,7 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
     return;
   }
   double* L = ctx_ws(c, WS_X, (size_t)r * r_eff);
-  launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream);
+  launch_extract_upper(A, m, r_eff, r, L, 1, r_eff, c->stream, full);
   const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
   double* Pn = alloc_core(c, (size_t)rows_p * r_eff);
   gemm_rm(c, rows_p, r_eff, r, t->cores[k - 1], r, L, r_eff, Pn, r_eff);
@@ -612,7 +630,7 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   launch_transpose(t->cores[k], rp, m, rp, A, m, c->stream);
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(kk, 1));
   double* Q = ctx_ws(c, WS_Q, (size_t)m * kk);
-  qr_factor(c, A, m, m, rp, tau, Q, m, true);
+  const bool full = qr_factor(c, A, m, m, rp, tau, Q, m, true);
   double* Cn = alloc_core(c, (size_t)m * kk);
   launch_transpose(Q, m, kk, m, Cn, kk, c->stream);
   if (!absorb)
@@ -622,7 +640,7 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
     return;
   }
   double* R = ctx_ws(c, WS_X, (size_t)kk * rp);
-  launch_extract_upper(A, m, kk, rp, R, rp, 1, c->stream);
+  launch_extract_upper(A, m, kk, rp, R, rp, 1, c->stream, full);
   const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
   double* Nn = alloc_core(c, (size_t)kk * ncols_n);
   gemm_rm(c, kk, ncols_n, rp, R, rp, t->cores[k + 1], ncols_n, Nn, ncols_n);
@@ -809,9 +827,10 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
     rinv_norms rn;
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, try_cert && c->rank_certificate ? &rn : nullptr);
+    const bool full =
+        qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, try_cert && c->rank_certificate ? &rn : nullptr);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream, full);
     if (try_cert && certify_full_rank(c, X, q, delta, &rn))
     {
       double* U = alloc_core(c, (size_t)rows * q);
@@ -828,7 +847,7 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
     double* U = alloc_core(c, (size_t)rows * r_new);
     gemm_s(c, rows, r_new, q, Q, 1, rows, Jp, 1, q, U, r_new, 1);  // U = Q J'_k
     double* Rb = alloc_core(c, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, Rb, q, 1, c->stream);      // R row-major
+    launch_extract_upper(A, rows, q, q, Rb, q, 1, c->stream, full);  // R row-major
     double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
     gemm_s(c, r_new, q, q, Jp, q, 1, Rb, q, 1, F, cols, 1);         // F = J'_k^T R
     c->mem.put(Rb);
@@ -843,12 +862,13 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   double* tau = ctx_ws(c, WS_F, (size_t)q);
   double* Q = ctx_ws(c, WS_Q, (size_t)cols * q);
   rinv_norms rn;
-  qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, try_cert && c->rank_certificate ? &rn : nullptr);
+  const bool full =
+      qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, try_cert && c->rank_certificate ? &rn : nullptr);
   double* Y = ctx_ws(c, WS_X, (size_t)q * q);
   if (keep_all)
-    launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream);  // Y = R
+    launch_extract_upper(A, cols, q, q, Y, 1, q, c->stream, full);  // Y = R
   else
-    launch_extract_upper(A, cols, q, q, Y, q, 1, c->stream);  // Y = R^T
+    launch_extract_upper(A, cols, q, q, Y, q, 1, c->stream, full);  // Y = R^T
   if (try_cert && certify_full_rank(c, Y, q, delta, &rn))
   {
     double* U = alloc_core(c, (size_t)q * q);
@@ -998,9 +1018,10 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)q * cols);
     rinv_norms rn;
-    qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
+    const bool full =
+        qr_factor(c, A, cols, cols, rows, tau, Q, cols, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
-    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
+    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream, full);
     if (cert_ok(max_rank, q) && certify_full_rank(c, X, q, delta, &rn))
     {
       // M = R^T Q^T: the new core is Q^T (the column-major Q read row-major),
@@ -1021,7 +1042,7 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     Cn = alloc_core(c, (size_t)r_new * cols);
     gemm_s(c, r_new, cols, q, Jp, q, 1, Q, cols, 1, Cn, cols, 1);  // Cn = J'_k^T Q^T
     double* Rb = alloc_core(c, (size_t)q * q);
-    launch_extract_upper(A, cols, q, q, Rb, q, 1, c->stream);       // R row-major
+    launch_extract_upper(A, cols, q, q, Rb, q, 1, c->stream, full);  // R row-major
     F = ctx_ws(c, WS_M1P, (size_t)rows * r_new);
     gemm_s(c, rows, r_new, q, Rb, 1, q, Jp, 1, q, F, r_new, 1);     // F = R^T J'_k
     c->mem.put(Rb);
@@ -1034,9 +1055,10 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
     double* tau = ctx_ws(c, WS_F, (size_t)q);
     double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
     rinv_norms rn;
-    qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
+    const bool full =
+        qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, cert_ok(max_rank, q) && c->rank_certificate ? &rn : nullptr);
     double* Y = ctx_ws(c, WS_X, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream);  // Y = R^T
+    launch_extract_upper(A, rows, q, q, Y, q, 1, c->stream, full);  // Y = R^T
     if (cert_ok(max_rank, q) && certify_full_rank(c, Y, q, delta, &rn))
     {
       // M = M I: the new core is the identity and M is absorbed into core k-1.
@@ -1645,14 +1667,14 @@ int singular_values_dev(tta_ctx* c, const double* M, int rows, int cols, double*
   if (rows >= cols)
   {
     launch_transpose(M, cols, rows, cols, A, rows, c->stream);
-    qr_factor(c, A, rows, rows, cols, tau, nullptr, 0, true);
-    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+    const bool full = qr_factor(c, A, rows, rows, cols, tau, nullptr, 0, true);
+    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream, full);
   }
   else
   {
     hip_check(hipMemcpyAsync(A, M, (size_t)rows * cols * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
-    qr_factor(c, A, cols, cols, rows, tau, nullptr, 0, true);
-    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream);
+    const bool full = qr_factor(c, A, cols, cols, rows, tau, nullptr, 0, true);
+    launch_extract_upper(A, cols, q, q, X, q, 1, c->stream, full);
   }
   double* J = ctx_ws(c, WS_J, (size_t)q * q);
   double* sv = nullptr;

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(256) void splitk_reduce_epi_kernel(gemm_desc g, int
       {
         const double x = v - id;
         *c = x;
-        ep.out2[e] = i < j ? x : (i == j ? 0.5 * x : 0.0);
+        if (ep.out2) ep.out2[e] = i < j ? x : (i == j ? 0.5 * x : 0.0);
         emax = fmax(emax, fabs(x) <= DBL_MAX ? fabs(x) : INFINITY);
         break;
       }
@@ -241,6 +241,13 @@ __global__ __launch_bounds__(256) void splitk_reduce_epi_kernel(gemm_desc g, int
           ep.out2[e] = id + x;
           ep.out3[e] = id - x;
         }
+        break;
+      }
+      case SPLITK_EPI_POLAR:
+      {
+        const double x = ep.E[e];
+        *c = id - 0.5 * x + 0.375 * v;
+        ep.out2[e] = id + 0.5 * x - 0.125 * v;
         break;
       }
       default:
@@ -321,13 +328,26 @@ __global__ __launch_bounds__(NT) void jacobi_kernel(jacobi_desc jd)
 }
 
 // dst[i*drs + c*dcs] = (i <= c) ? A[c*lda + i] : 0   for i < kr, c < nc.
-__global__ void extract_upper_kernel(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs)
+__global__ void extract_upper_kernel(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs,
+                                     int full)
 {
   const long total = (long)kr * nc;
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < total; t += (long)gridDim.x * blockDim.x)
   {
     const int c = (int)(t / kr), i = (int)(t % kr);
-    dst[i * drs + c * dcs] = (i <= c) ? A[(size_t)c * lda + i] : 0.0;
+    dst[i * drs + c * dcs] = (full || i <= c) ? A[(size_t)c * lda + i] : 0.0;
+  }
+}
+
+// P = I - E/2 + 3/8 V, Pi = I + E/2 - 1/8 V (n x n row-major).
+__global__ void polar_series_kernel(const double* E, const double* V, double* P, double* Pi, int n)
+{
+  const long total = (long)n * n;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const double id = e / n == e % n ? 1.0 : 0.0;
+    P[e] = id - 0.5 * E[e] + 0.375 * V[e];
+    Pi[e] = id + 0.5 * E[e] - 0.125 * V[e];
   }
 }
 
@@ -801,13 +821,13 @@ __global__ __launch_bounds__(256) void minus_ident_kernel(double* G, int n, doub
 // Device-side decisions of a speculative sweep: flag[0] starts at 1 and is
 // cleared by the first factorization or certificate that does not hold.
 // CholQR2 acceptance: no breakdown in pass 1 (info[0] == 0, finite extreme
-// diagonal entries) and n max|E| <= 1e-4 over the partial maxima.
-__global__ void spec_cholqr_kernel(const double* info, const double* part, int np, int n, double* flag)
+// diagonal entries) and n max|E| <= thr over the partial maxima.
+__global__ void spec_cholqr_kernel(const double* info, const double* part, int np, int n, double thr, double* flag)
 {
   if (threadIdx.x != 0) return;
   double emax = 0.0;
   for (int b = 0; b < np; ++b) emax = fmax(emax, part[b]);
-  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= 1.0e-4;
+  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= thr;
   if (!ok) flag[0] = 0.0;
 }
 
@@ -949,10 +969,16 @@ void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st)
   hipLaunchKernelGGL(jacobi_kernel<NT>, dim3(batch), dim3(NT), lds, st, j);
 }
 
-void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st)
+void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st,
+                          bool full)
 {
   hipLaunchKernelGGL(extract_upper_kernel, dim3(grid_for((long)kr * nc)), dim3(256), 0, st, A, lda, kr, nc, dst, drs,
-                     dcs);
+                     dcs, full ? 1 : 0);
+}
+
+void launch_polar_series(const double* E, const double* V, double* P, double* Pi, int n, hipStream_t st)
+{
+  hipLaunchKernelGGL(polar_series_kernel, dim3(grid_for((long)n * n)), dim3(256), 0, st, E, V, P, Pi, n);
 }
 
 __global__ void scale_copy_kernel(const double* src, int rows, int cols, long ld, const double* rowf, const double* colf,
@@ -1013,9 +1039,10 @@ void launch_ident_axpy(const double* X, double* Y, int n, double alpha, double b
   hipLaunchKernelGGL(ident_axpy_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, X, Y, n, alpha, beta);
 }
 
-void launch_spec_cholqr(const double* info, const double* part, int np, int n, double* flag, hipStream_t st)
+void launch_spec_cholqr(const double* info, const double* part, int np, int n, double thr, double* flag,
+                        hipStream_t st)
 {
-  hipLaunchKernelGGL(spec_cholqr_kernel, dim3(1), dim3(64), 0, st, info, part, np, n, flag);
+  hipLaunchKernelGGL(spec_cholqr_kernel, dim3(1), dim3(64), 0, st, info, part, np, n, thr, flag);
 }
 
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st)

@@ -215,18 +215,22 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 int gemm_splitk_slices(int M, int N, int K);
 // Epilogues folded into the split-K sum of a square row-major n x n product
 // v = alpha P + beta C (CholQR2's second pass):
-//   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E), part[b] = max |E| over
+//   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E) (if given), part[b] = max |E| over
 //                    block b's entries (non-finite counted as infinity);
 //   SPLITK_EPI_PHI:  C = Phi(E - v) for the E given;
 //   SPLITK_EPI_PHI_LAST: as PHI, and out2 = I + C, out3 = I - C;
-//   SPLITK_EPI_DIAG: C = v + diag I.
+//   SPLITK_EPI_DIAG: C = v + diag I;
+//   SPLITK_EPI_POLAR: for v = E E, C = P = I - E/2 + 3/8 v and out2 = Pi =
+//                    I + E/2 - 1/8 v, the second-order series of
+//                    (I + E)^(-1/2) and (I + E)^(1/2).
 // Phi keeps the strict upper part and half the diagonal, zero below.
 enum
 {
   SPLITK_EPI_GRAM = 1,
   SPLITK_EPI_PHI = 2,
   SPLITK_EPI_PHI_LAST = 3,
-  SPLITK_EPI_DIAG = 4
+  SPLITK_EPI_DIAG = 4,
+  SPLITK_EPI_POLAR = 5
 };
 struct splitk_epi
 {
@@ -241,6 +245,9 @@ struct splitk_epi
 // minus_ident_parts(n) blocks (one partial maximum each).
 void launch_splitk_reduce_epi(const gemm_desc& g, int s_eff, const double* part, const splitk_epi& ep,
                               hipStream_t st);
+// The SPLITK_EPI_POLAR epilogue on its own: P and Pi from E and V = E E
+// (n x n row-major).
+void launch_polar_series(const double* E, const double* V, double* P, double* Pi, int n, hipStream_t st);
 int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st);
 void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hipStream_t st);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
@@ -272,7 +279,8 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
                       double credit_bytes);
 void launch_qr(const qr_desc& q, int batch, hipStream_t st);
 void launch_jacobi(const jacobi_desc& j, int batch, hipStream_t st);
-void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st);
+void launch_extract_upper(const double* A, int lda, int kr, int nc, double* dst, long drs, long dcs, hipStream_t st,
+                          bool full = false);
 // dst (rows x cols, packed row-major) = src (row-major, leading dimension ld)
 // with row i scaled by rowf[i] and column j by colf[j] (either may be null).
 void launch_scale_copy(const double* src, int rows, int cols, long ld, const double* rowf, const double* colf,
@@ -289,10 +297,11 @@ int rinv_cert_blocks(int q);
 void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
 // Decisions of a speculative sweep on the device (flag[0] cleared on the
 // first failure): CholQR2 acceptance from its info[0..2] and the np partial
-// maxima of |E|; the full-rank certificate from the column norms of R and
+// maxima of |E| (accepted when n max|E| <= thr); the full-rank certificate from the column norms of R and
 // R^-1 (mode 0, a = 2n values) or the n (xf2, rf2) pairs of
 // launch_rinv_cert (mode 1).
-void launch_spec_cholqr(const double* info, const double* part, int np, int n, double* flag, hipStream_t st);
+void launch_spec_cholqr(const double* info, const double* part, int np, int n, double thr, double* flag,
+                        hipStream_t st);
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
 // p[0] = v.
 void launch_set_value(double* p, double v, hipStream_t st);
