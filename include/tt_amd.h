@@ -331,15 +331,18 @@ int tta_prof_rank_certificates(tta_ctx* ctx, long long* tried, long long* held);
 /* ---- CholQR2 for the sweeps' QR factorizations (no reference counterpart) */
 /* Factorizations with m >= n and 32 <= n <= 512 in the QR and SVD sweeps of
  * single TTs run as CholQR2 (Gram on the matrix cores, blocked Cholesky with
- * inverse, then a second pass whose factor is solved as a perturbation of
- * the identity) when it is accepted: no Cholesky breakdown and
- * n max|Q1^T Q1 - I| <= 1e-4, i.e. condition numbers up to about 1e6;
- * otherwise, and with on = 0, blocked Householder.  Both give a
- * backward-stable Q R. */
+ * inverse, then a second pass from E = Q1^T Q1 - I) when it is accepted: no
+ * Cholesky breakdown and n max|E| <= 1e-4, i.e. condition numbers up to
+ * about 1e6; otherwise, and with on = 0, blocked Householder.  Both give a
+ * backward-stable Q R.  The second pass is the polar form Q = Q1 (I+E)^(-1/2)
+ * (to second order; R then not triangular) for n max|E| <= 1e-5, the
+ * triangular form (Cholesky factor of I + E) above that. */
 int tta_set_cholqr(tta_ctx* ctx, int on);
 /* Factorizations where CholQR2 was tried and accepted, since the last
  * tta_prof_enable. */
 int tta_prof_cholqr(tta_ctx* ctx, long long* tried, long long* used);
+/* Of those, the ones accepted in the triangular form. */
+int tta_prof_cholqr_triangular(tta_ctx* ctx, long long* triangular);
 
 /* ---- speculative sweeps (no reference counterpart) ------------------------- */
 /* The QR and SVD sweeps of round / round_right_canonical (and of the naive

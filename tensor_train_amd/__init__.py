@@ -117,6 +117,13 @@ class Context:
         self.check(self.lib.tta_prof_cholqr(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def cholqr_triangular_count(self) -> int:
+        """CholQR2 factorizations accepted in the triangular form (condition
+        numbers beyond the polar form's range) since the last prof_enable."""
+        a = ctypes.c_longlong()
+        self.check(self.lib.tta_prof_cholqr_triangular(self.h, ctypes.byref(a)))
+        return a.value
+
     def rank_certificates(self):
         """(tried, held) certificate counts since the last prof_enable."""
         a, b = ctypes.c_longlong(), ctypes.c_longlong()

@@ -456,6 +456,7 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
     if (ok && emax * n > 1.0e-5)
     {
       // Triangular form (the polar one is not accurate enough here).
+      c->cholqr_triangular += 1;
       ew([&] { launch_phi(G, nullptr, X, n, c->stream); });          // X = Phi(E)
       for (int it = 0; it < 3; ++it)
       {
@@ -2759,6 +2760,7 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->cert_held = 0;
   c->cholqr_tried = 0;
   c->cholqr_used = 0;
+  c->cholqr_triangular = 0;
   c->spec_sweeps = 0;
   c->spec_failed = 0;
   TTA_GUARD_END(c)
@@ -2791,6 +2793,13 @@ int tta_prof_speculation(tta_ctx* c, long long* sweeps, long long* failed)
   if (!c) return TTA_ERR_ARG;
   if (sweeps) *sweeps = c->spec_sweeps;
   if (failed) *failed = c->spec_failed;
+  return TTA_OK;
+}
+
+int tta_prof_cholqr_triangular(tta_ctx* c, long long* triangular)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (triangular) *triangular = c->cholqr_triangular;
   return TTA_OK;
 }
 

@@ -113,6 +113,7 @@ struct tta_ctx
   // and accepted counts.
   bool cholqr = true;
   long long cholqr_tried = 0, cholqr_used = 0;
+  long long cholqr_triangular = 0;  // accepted in the triangular form
   // Speculative sweeps of round (on by default): CholQR2 acceptance and the
   // rank certificates are decided on the device into one flag that the host
   // reads after the sweep; a failed sweep is undone and run again with host

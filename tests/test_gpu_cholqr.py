@@ -94,3 +94,36 @@ def test_cholqr_declines_on_rank_deficient_and_graded():
         ref = O.round(g, eps)
         r, _ = _with(ctx, lambda: T.round(to_dev(g), eps=eps))
         assert r.ranks == ref.ranks and rel_err(cores3(r), ref) <= 1e-12
+
+
+def test_cholqr_forms_across_condition_numbers():
+    # Bond spectra 10^-(s j) over 64 values, condition numbers 1e2 .. 1e7:
+    # whichever form each factorization takes (the polar second pass for
+    # small E, the triangular one for n max|E| between 1e-5 and 1e-4,
+    # Householder beyond, the last covered by the graded test above), the
+    # result reproduces the oracle; this sweep reaches both CholQR2 forms.
+    ctx = T.Context.default()
+    forms = {"triangular": 0, "declined": 0, "polar": 0}
+    for logk in (2.0, 4.0, 4.5, 5.0, 5.5, 6.0, 7.0):
+        x = O.OTT.random([4] * 6, 64, 17)
+        O.left_orthogonalize(x)
+        cs = x.cores()
+        for k in range(len(cs) - 1):
+            cs[k] = cs[k] * (10.0 ** (-np.arange(cs[k].shape[2]) * logk / 63.0))[None, None, :]
+        g = O.OTT.from_cores(cs)
+        ref = O.round(g, 0.0)
+        ctx.set_cholqr(True)
+        ctx.set_speculation(False)
+        ctx.prof_enable(True)
+        try:
+            r = T.round(to_dev(g), eps=0.0)
+            tried, used = ctx.cholqr_counts()
+            tri = ctx.cholqr_triangular_count()
+        finally:
+            ctx.prof_enable(False)
+            ctx.set_speculation(True)
+        assert r.ranks == ref.ranks and rel_err(cores3(r), ref) <= 1e-12
+        forms["triangular"] += tri
+        forms["declined"] += tried - used
+        forms["polar"] += used - tri
+    assert forms["polar"] > 0 and forms["triangular"] > 0, forms
