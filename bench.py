@@ -268,7 +268,8 @@ def main():
     for _ in range(args.prof_steps):
         y = T.matvec_round(A, x, eps=EPS)
     ctx.synchronize()
-    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "apply", "chol", "rinv", "other")}
+    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "gemm_multi", "apply", "chol", "rinv",
+                                           "other")}
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     spec_sweeps, spec_failed = ctx.speculation_counts()
@@ -279,6 +280,7 @@ def main():
     single = {"jacobi_round": ("bj_round_kernel", ctx.prof_read("jacobi_round")),
               "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"]),
               "gemm_splitk": ("gemm_f64_splitk_kernel", stats["gemm_splitk"]),
+              "gemm_multi": ("gemm_f64_multi_kernel", stats["gemm_multi"]),
               "chol": ("potrf_inv_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}
     ctx.prof_enable(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps

@@ -300,7 +300,9 @@ enum tta_kernel_class
   TTA_K_RINV = 7,
   /* The partial products of a GEMM split along K (their sum: TTA_K_OTHER). */
   TTA_K_GEMM_SPLITK = 8,
-  TTA_K_COUNT = 9
+  /* Launches of up to four independent GEMMs together. */
+  TTA_K_GEMM_MULTI = 9,
+  TTA_K_COUNT = 10
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <initializer_list>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -241,6 +242,52 @@ void gemm_raw(tta_ctx* c, int M, int N, int K, const double* A, long sar, long s
   gemm_go(c, g);
 }
 
+// A GEMM descriptor for C = alpha A B + beta C with element strides.
+gemm_desc gdesc(int M, int N, int K, const double* A, long sar, long sac, const double* B, long sbr, long sbc, double* C,
+                long scr, long scc, double alpha = 1.0, double beta = 0.0)
+{
+  gemm_desc g;
+  g.M = M;
+  g.N = N;
+  g.K = K;
+  g.A = A;
+  g.sar = sar;
+  g.sac = sac;
+  g.B = B;
+  g.sbr = sbr;
+  g.sbc = sbc;
+  g.C = C;
+  g.scr = scr;
+  g.scc = scc;
+  g.alpha = alpha;
+  g.beta = beta;
+  return g;
+}
+
+// Independent GEMMs (empty ones, M or N or K = 0, skipped) in one launch
+// on the context stream; a single one goes through gemm_go.
+void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
+{
+  gemm_multi m;
+  double fl = 0.0, by = 0.0;
+  for (const gemm_desc& g : gs)
+  {
+    if (g.M <= 0 || g.N <= 0 || g.K <= 0) continue;
+    if (m.count == 4) throw tta::error(TTA_ERR_ARG, "gemm_multi_go: more than four products");
+    m.g[m.count++] = g;
+    fl += 2.0 * g.M * g.N * g.K;
+    by += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
+  }
+  if (m.count == 0) return;
+  if (m.count == 1)
+  {
+    gemm_go(c, m.g[0]);
+    return;
+  }
+  prof_scope ps(c, TTA_K_GEMM_MULTI, fl, by);
+  launch_gemm_multi(m, c->stream);
+}
+
 // gemm_go with an epilogue folded into the split-K sum (splitk_epi); only
 // for products that split (gemm_splitk_slices > 1), so that the sum runs
 // after every partial product has read its operands and may overwrite one.
@@ -298,26 +345,6 @@ struct rinv_norms
 void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double* T, double* info)
 {
   const int B = potrf_block();
-  auto gm = [&](int M, int N, int K, const double* A, long sar, long sac, const double* Bm, long sbr, long sbc,
-                double* C, long scr, long scc, double alpha, double beta)
-  {
-    gemm_desc g;
-    g.M = M;
-    g.N = N;
-    g.K = K;
-    g.A = A;
-    g.sar = sar;
-    g.sac = sac;
-    g.B = Bm;
-    g.sbr = sbr;
-    g.sbc = sbc;
-    g.C = C;
-    g.scr = scr;
-    g.scc = scc;
-    g.alpha = alpha;
-    g.beta = beta;
-    gemm_go(c, g);
-  };
   for (int j0 = 0; j0 < n; j0 += B)
   {
     const int jb = std::min(B, n - j0), j1 = j0 + jb, w = n - j1;
@@ -326,17 +353,13 @@ void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double
       launch_potrf_inv(G, n, R, n, X, n, jb, j0, info, c->stream);
     }
     const double* D = X + (size_t)j0 * n + j0;
-    if (w > 0)
-    {
-      double* Rrow = R + (size_t)j0 * n + j1;
-      gm(jb, w, jb, D, 1, n, G + (size_t)j0 * n + j1, n, 1, Rrow, n, 1, 1.0, 0.0);
-      gm(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0);
-    }
-    if (j0 > 0)
-    {
-      gm(j0, jb, j0, X, n, 1, R + j0, n, 1, T, jb, 1, 1.0, 0.0);
-      gm(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0);
-    }
+    double* Rrow = R + (size_t)j0 * n + j1;
+    // Block row of R beside the first product of R^-1's block column, then
+    // the trailing update beside the second (each pair independent).
+    gemm_multi_go(c, {gdesc(jb, w, jb, D, 1, n, G + (size_t)j0 * n + j1, n, 1, Rrow, n, 1),
+                      gdesc(j0, jb, j0, X, n, 1, R + j0, n, 1, T, jb, 1)});
+    gemm_multi_go(c, {gdesc(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0),
+                      gdesc(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0)});
   }
 }
 
@@ -419,9 +442,9 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
       gm(n, n, n, G, n, 1, G, n, 1, Y, n, 1);                         // Y = E E
       ew([&] { launch_polar_series(G, Y, X, W, n, c->stream); });    // X = P, W = Pi
     }
-    if (Q) gm(m, n, n, Q1, 1, m, X, n, 1, Q, 1, ldq);                // Q = Q1 P
-    gm(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1);                         // Rf = Pi R1
-    if (cert) gm(n, n, n, Xi, n, 1, X, n, 1, Y, n, 1);                // Y = R^-1 = R1^-1 P
+    // Q = Q1 P, Rf = Pi R1 and (for the certificate) Y = R^-1 = R1^-1 P.
+    gemm_multi_go(c, {gdesc(Q ? m : 0, n, n, Q1, 1, m, X, n, 1, Q, 1, ldq), gdesc(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1),
+                      gdesc(cert ? n : 0, n, n, Xi, n, 1, X, n, 1, Y, n, 1)});
     if (c->spec)
     {
       // Decided on the device; R goes to A now, and a failed acceptance

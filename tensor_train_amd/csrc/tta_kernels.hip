@@ -265,6 +265,14 @@ __global__ __launch_bounds__(256) void splitk_reduce_epi_kernel(gemm_desc g, int
   }
 }
 
+// Independent GEMMs of one launch, descriptors passed by value.
+__global__ __launch_bounds__(256) void gemm_f64_multi_kernel(gemm_multi m)
+{
+  const gemm_desc& g = m.g[blockIdx.z];
+  if ((int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
+  gemm_tile(g, g.A, g.B, g.C);
+}
+
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
@@ -924,6 +932,18 @@ int gemm_splitk_slices(int M, int N, int K)
   long s = std::min<long>(256 / tiles, K / 64);
   s = std::min<long>(s, 16);
   return s < 2 ? 1 : (int)s;
+}
+
+void launch_gemm_multi(const gemm_multi& m, hipStream_t st)
+{
+  int mt = 0, nt = 0;
+  for (int i = 0; i < m.count; ++i)
+  {
+    mt = std::max(mt, (m.g[i].M + GT - 1) / GT);
+    nt = std::max(nt, (m.g[i].N + GT - 1) / GT);
+  }
+  if (m.count <= 0 || mt <= 0 || nt <= 0) return;
+  hipLaunchKernelGGL(gemm_f64_multi_kernel, dim3(nt, mt, m.count), dim3(256), 0, st, m);
 }
 
 int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st)

@@ -213,6 +213,14 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // uses (returned; at most S) into part (S * M * N doubles), and
 // launch_splitk_reduce sums them into C with alpha and beta.
 int gemm_splitk_slices(int M, int N, int K);
+// Up to four independent GEMMs in one launch (descriptors by value;
+// blockIdx.z picks the problem, tiles beyond its M x N exit).
+struct gemm_multi
+{
+  gemm_desc g[4];
+  int count = 0;
+};
+void launch_gemm_multi(const gemm_multi& m, hipStream_t st);
 // Epilogues folded into the split-K sum of a square row-major n x n product
 // v = alpha P + beta C (CholQR2's second pass):
 //   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E) (if given), part[b] = max |E| over
