@@ -263,10 +263,7 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
-// In-place Cholesky G = R^T R (row-major n x n, leading dimension ld, upper
-// triangle read, R written with the strict lower triangle zeroed), one
-// workgroup; n <= chol_max_n().  info[0] = 1 on breakdown, info[1..2] =
-// min / max diagonal of R.
+// Largest n of a CholQR2 factorization.
 int chol_max_n();
 // Diagonal block step of the blocked Cholesky: the jb x jb block (jb <=
 // potrf_block()) at (j0, j0) of G factored as R_b^T R_b, R_b into R and
@@ -276,7 +273,6 @@ int chol_max_n();
 int potrf_block();
 void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, int ldx, int jb, int j0, double* info,
                       hipStream_t st);
-void launch_chol(double* G, int n, int ld, double* info, hipStream_t st);
 int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
 // Returns the number of block sweeps the host ran (q > 64), or -1 when the

@@ -456,174 +456,9 @@ __global__ void qr_eye_kernel(qr_batch bt)
   }
 }
 
-// ---------------------------------------------------------------------------
-// Cholesky G = R^T R of a symmetric positive definite n x n matrix (row-major,
-// leading dimension ld, upper triangle read; R written over it, strict lower
-// triangle zeroed) by one workgroup of 256 threads, right-looking in
-// 16-column blocks:
-//   * wave 0 factors the 16 x 16 diagonal block in registers (lane c holds
-//     column c; the pivot and the row of R go round by readlane);
-//   * one thread per trailing column solves R_JJ^T X = G(J, j1:) for the
-//     block row of R (forward substitution, R_JJ from LDS);
-//   * the trailing upper triangle takes the rank-16 update
-//     G(j1:, j1:) -= X^T X on the matrix cores, one 16 x 16 tile per wave
-//     at a time (four v_mfma_f64_16x16x4f64 steps over the block's 16 rows).
-// info[0] = 1 on a non-positive or non-finite pivot; info[1], info[2] =
-// smallest and largest diagonal entry of R.
-// ---------------------------------------------------------------------------
+// Largest n that CholQR2 takes (the blocked Cholesky itself has no limit;
+// beyond this the Grams cost more than Householder saves).
 constexpr int kCholMaxN = 512;
-
-__global__ __launch_bounds__(256) void chol_kernel(double* G, int n, int ld, double* info)
-{
-  __shared__ double Rd[16][17];
-  __shared__ double Rdi[16];
-  __shared__ double Ps[16][kCholMaxN];
-  __shared__ int bad;
-  __shared__ double dmin, dmax;
-  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-  if (tid == 0)
-  {
-    bad = 0;
-    dmin = INFINITY;
-    dmax = 0.0;
-  }
-  for (int j0 = 0; j0 < n; j0 += 16)
-  {
-    const int jb = min(16, n - j0), j1 = j0 + jb, w = n - j1;
-    __syncthreads();
-    if (wid == 0)
-    {
-      double col[16];
-#pragma unroll
-      for (int r = 0; r < 16; ++r)
-        col[r] = (lane < jb && r <= lane) ? G[(size_t)(j0 + r) * ld + j0 + lane] : 0.0;
-      bool lbad = false;
-      double lmin = INFINITY, lmax = 0.0;
-#pragma unroll
-      for (int k = 0; k < 16; ++k)
-      {
-        if (k < jb)
-        {
-          const double piv = read_lane(col[k], k);
-          if (!(piv > 0.0) || !(piv < INFINITY)) lbad = true;
-          const double rkk = sqrt(piv);
-          lmin = fmin(lmin, rkk);
-          lmax = fmax(lmax, rkk);
-          if (lane == k)
-            col[k] = rkk;
-          else if (lane > k)
-            col[k] = col[k] / rkk;
-#pragma unroll
-          for (int r = k + 1; r < 16; ++r)
-          {
-            if (r < jb)
-            {
-              const double rkr = read_lane(col[k], r);
-              if (lane >= r && lane < jb) col[r] -= rkr * col[k];
-            }
-          }
-        }
-      }
-      if (lane < 16)
-      {
-#pragma unroll
-        for (int r = 0; r < 16; ++r) Rd[r][lane] = (r <= lane && lane < jb) ? col[r] : 0.0;
-        Rdi[lane] = lane < jb ? 1.0 / col[lane] : 0.0;
-        if (lane < jb)
-        {
-#pragma unroll
-          for (int r = 0; r < 16; ++r)
-            if (r < jb) G[(size_t)(j0 + r) * ld + j0 + lane] = r <= lane ? col[r] : 0.0;
-        }
-      }
-      if (lane == 0)
-      {
-        if (lbad) bad = 1;
-        dmin = fmin(dmin, lmin);
-        dmax = fmax(dmax, lmax);
-      }
-    }
-    __syncthreads();
-    for (int c = tid; c < w; c += 256)
-    {
-      // All 16 loads first, then the substitution chain in registers.
-      double g[16], x[16];
-#pragma unroll
-      for (int t = 0; t < 16; ++t) g[t] = t < jb ? G[(size_t)(j0 + t) * ld + j1 + c] : 0.0;
-#pragma unroll
-      for (int t = 0; t < 16; ++t)
-      {
-        if (t < jb)
-        {
-          double v = g[t];
-#pragma unroll
-          for (int s2 = 0; s2 < 16; ++s2)
-            if (s2 < t) v = fma(-Rd[s2][t], x[s2], v);
-          x[t] = v * Rdi[t];
-        }
-        else
-          x[t] = 0.0;
-      }
-#pragma unroll
-      for (int t = 0; t < 16; ++t)
-      {
-        Ps[t][c] = x[t];
-        if (t < jb) G[(size_t)(j0 + t) * ld + j1 + c] = x[t];
-      }
-    }
-    __syncthreads();
-    const int nt = (w + 15) / 16;
-    const int fl = lane & 15, fk = lane >> 4;
-    // Wave wid takes tile rows I = wid, wid + 4, ...; along a row, four
-    // tiles at a time with all their loads issued before the first MFMA.
-    for (int I = wid; I < nt; I += 4)
-      for (int J0 = I; J0 < nt; J0 += 4)
-      {
-        qdbl4 acc[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-          {
-            const int gi = 16 * I + fk + 4 * r, gj = 16 * (J0 + u) + fl;
-            acc[u][r] = (J0 + u < nt && gi < w && gj < w) ? G[(size_t)(j1 + gi) * ld + j1 + gj] : 0.0;
-          }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-        {
-          if (J0 + u >= nt) continue;
-#pragma unroll
-          for (int k0 = 0; k0 < 16; k0 += 4)
-          {
-            const int ai = 16 * I + fl, bj = 16 * (J0 + u) + fl, kk = k0 + fk;
-            const double a = ai < w ? -Ps[kk][ai] : 0.0;
-            const double b = bj < w ? Ps[kk][bj] : 0.0;
-            acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int r = 0; r < 4; ++r)
-          {
-            const int gi = 16 * I + fk + 4 * r, gj = 16 * (J0 + u) + fl;
-            if (J0 + u < nt && gi < w && gj < w) G[(size_t)(j1 + gi) * ld + j1 + gj] = acc[u][r];
-          }
-      }
-  }
-  __syncthreads();
-  for (long e = tid; e < (long)n * n; e += 256)
-  {
-    const int i = (int)(e / n), j = (int)(e % n);
-    if (i > j) G[(size_t)i * ld + j] = 0.0;
-  }
-  if (tid == 0)
-  {
-    info[0] = bad ? 1.0 : 0.0;
-    info[1] = dmin;
-    info[2] = dmax;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Diagonal block of the blocked Cholesky (chol_inv_blocked in the C ABI
@@ -793,11 +628,6 @@ void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, i
 }
 
 int chol_max_n() { return kCholMaxN; }
-
-void launch_chol(double* G, int n, int ld, double* info, hipStream_t st)
-{
-  hipLaunchKernelGGL(chol_kernel, dim3(1), dim3(256), 0, st, G, n, ld, info);
-}
 
 int qr_blocked_max_rows() { return 4 * 512; }
 
