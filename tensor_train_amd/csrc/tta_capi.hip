@@ -387,7 +387,11 @@ void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double
 // of it) and Q to Q (scratch for the caller either way).  Otherwise nothing
 // is written and the caller factors A by Householder.  Every product runs on
 // the matrix cores.
-bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, rinv_norms* cert = nullptr)
+// The strided form: A(i, k) = A[i sar + k sac] is only read; Q(i, k) =
+// Q[i sqr + k sqc] (Q may be null) and R (n x n row-major, full) go to the
+// caller's buffers, so callers holding either layout need no transposes.
+bool cholqr2_strided(tta_ctx* c, const double* A, long sar, long sac, int m, int n, double* Q, long sqr, long sqc,
+                     double* Rf, rinv_norms* cert = nullptr)
 {
   const size_t nn = (size_t)n * n;
   double* G = alloc_core(c, nn);  // Gram, then E
@@ -395,7 +399,6 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   double* W = alloc_core(c, nn);
   double* Y = alloc_core(c, nn);
   double* R1 = alloc_core(c, nn);
-  double* Rf = alloc_core(c, nn);  // R of the accepted form
   double* Xi = alloc_core(c, nn);  // R1^-1
   double* Q1 = alloc_core(c, (size_t)m * n);
   double* aux = alloc_core(c, 8 + (size_t)kMinusIdentParts + 2 * (size_t)n);
@@ -413,16 +416,16 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   };
   auto release = [&]()
   {
-    for (double* p : {G, X, W, Y, R1, Rf, Xi, Q1, aux}) c->mem.put(p);
+    for (double* p : {G, X, W, Y, R1, Xi, Q1, aux}) c->mem.put(p);
   };
   // Where the products split along K, the elementwise steps ride on the
   // sums of their partial products.
   const bool split = gemm_splitk_slices(n, n, m) > 1 && gemm_splitk_slices(n, n, n) > 1;
   try
   {
-    gm(n, n, m, A, lda, 1, A, 1, lda, G, n, 1);     // G = A^T A
-    chol_inv_blocked(c, G, n, R1, Xi, W, aux);       // R1, Xi = R1^-1
-    gm(m, n, n, A, 1, lda, Xi, n, 1, Q1, 1, m);     // Q1 = A R1^-1
+    gm(n, n, m, A, sac, sar, A, sar, sac, G, n, 1);  // G = A^T A
+    chol_inv_blocked(c, G, n, R1, Xi, W, aux);        // R1, Xi = R1^-1
+    gm(m, n, n, A, sar, sac, Xi, n, 1, Q1, 1, m);    // Q1 = A R1^-1
     if (split)
     {
       splitk_epi ep;
@@ -443,14 +446,12 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
       ew([&] { launch_polar_series(G, Y, X, W, n, c->stream); });    // X = P, W = Pi
     }
     // Q = Q1 P, Rf = Pi R1 and (for the certificate) Y = R^-1 = R1^-1 P.
-    gemm_multi_go(c, {gdesc(Q ? m : 0, n, n, Q1, 1, m, X, n, 1, Q, 1, ldq), gdesc(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1),
+    gemm_multi_go(c, {gdesc(Q ? m : 0, n, n, Q1, 1, m, X, n, 1, Q, sqr, sqc), gdesc(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1),
                       gdesc(cert ? n : 0, n, n, Xi, n, 1, X, n, 1, Y, n, 1)});
     if (c->spec)
     {
-      // Decided on the device; R goes to A now, and a failed acceptance
-      // undoes the whole sweep.
+      // Decided on the device; a failed acceptance undoes the whole sweep.
       ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), c->stream); });
-      ew([&] { launch_transpose(Rf, n, n, n, A, lda, c->stream); });
       if (cert)
       {
         double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
@@ -491,7 +492,7 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
       gm(n, n, n, W, n, 1, G, n, 1, X, n, 1);                         // X = R2 Y0
       ew([&] { launch_ident_axpy(X, X, n, 2.0, -1.0, c->stream); }); // X = 2 I - R2 Y0
       gm(n, n, n, G, n, 1, X, n, 1, Y, n, 1);                         // Y = R2^-1
-      if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, 1, ldq);              // Q = Q1 R2^-1
+      if (Q) gm(m, n, n, Q1, 1, m, Y, n, 1, Q, sqr, sqc);            // Q = Q1 R2^-1
       gm(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1);                       // Rf = R2 R1
       if (cert)
       {
@@ -506,7 +507,6 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
     }
     if (ok)
     {
-      ew([&] { launch_transpose(Rf, n, n, n, A, lda, c->stream); });  // A's leading block = R
       if (cert)
       {
         const double* hn = h.data() + 8 + kMinusIdentParts;
@@ -531,6 +531,33 @@ bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, r
   return ok;
 }
 
+// CholQR2 in place on a column-major A (leading dimension lda): R to A's
+// leading n x n block (all of it), Q to Q (leading dimension ldq).
+bool cholqr2(tta_ctx* c, double* A, int lda, int m, int n, double* Q, int ldq, rinv_norms* cert = nullptr)
+{
+  double* Rf = alloc_core(c, (size_t)n * n);
+  bool ok = false;
+  try
+  {
+    ok = cholqr2_strided(c, A, 1, lda, m, n, Q, 1, ldq, Rf, cert);
+    if (ok)
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 16.0 * n * n);
+      launch_transpose(Rf, n, n, n, A, lda, c->stream);
+    }
+  }
+  catch (...)
+  {
+    c->mem.put(Rf);
+    throw;
+  }
+  c->mem.put(Rf);
+  return ok;
+}
+
+// Whether qr_factor(..., cholqr = true) tries CholQR2 on an m x n matrix.
+bool cholqr_applies(const tta_ctx* c, int m, int n) { return c->cholqr && m >= n && n >= 32 && n <= chol_max_n(); }
+
 // QR of a column-major (m x n) matrix in place (R in the leading rows),
 // thin Q (m x k) into Q when non-null.  With cholqr, matrices with m >= n
 // and 32 <= n <= chol_max_n() go through CholQR2 first; otherwise, and
@@ -550,8 +577,7 @@ bool qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
     cert->have = false;
     cert->dev = nullptr;
   }
-  if (cholqr && c->cholqr && m >= n && n >= 32 && n <= chol_max_n() && cholqr2(c, A, lda, m, n, Q, ldq, cert))
-    return true;
+  if (cholqr && cholqr_applies(c, m, n) && cholqr2(c, A, lda, m, n, Q, ldq, cert)) return true;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
   if (m <= qr_blocked_max_rows())
   {
@@ -626,7 +652,23 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   double* A = t->cores[k];
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
-  const bool full = qr_factor(c, A, m, m, r, tau, Qn, m, true);
+  const bool try_chol = absorb && cholqr_applies(c, m, r);
+  if (try_chol)
+  {
+    // CholQR2 leaves core k as it is and gives R row-major: core k-1 takes
+    // L = R^T by the GEMM's strides, with no extraction.
+    double* R = ctx_ws(c, WS_X, (size_t)r * r);
+    if (cholqr2_strided(c, A, 1, m, m, r, Qn, 1, m, R))
+    {
+      const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
+      double* Pn = alloc_core(c, (size_t)rows_p * r);
+      gemm_go(c, gdesc(rows_p, r, r, t->cores[k - 1], r, 1, R, 1, r, Pn, r, 1));  // core k-1 R^T
+      replace_core(c, t, k - 1, Pn);
+      replace_core(c, t, k, Qn);
+      return;
+    }
+  }
+  const bool full = qr_factor(c, A, m, m, r, tau, Qn, m, !try_chol);
   if (!absorb)
   {
     replace_core(c, t, k, Qn);
@@ -846,35 +888,65 @@ int svd_left(tta_ctx* c, const double* M, int rows, int cols, double delta, int 
   if (rows >= cols)
   {
     const int q = cols;
-    double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
-    launch_transpose(M, cols, rows, cols, A, rows, c->stream);
-    double* tau = ctx_ws(c, WS_F, (size_t)q);
-    double* Q = ctx_ws(c, WS_Q, (size_t)rows * q);
     rinv_norms rn;
-    const bool full =
-        qr_factor(c, A, rows, rows, cols, tau, Q, rows, true, try_cert && c->rank_certificate ? &rn : nullptr);
+    rinv_norms* rnp = try_cert && c->rank_certificate ? &rn : nullptr;
     double* X = ctx_ws(c, WS_X, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, X, q, 1, c->stream, full);
+    // Q with element strides (qrs, qcs).  CholQR2 reads M where it lies and
+    // writes Q row-major (the layout of U) and R into X; only Householder
+    // (CholQR2 not applicable, or declined) needs the column-major copy.
+    double* Q = nullptr;
+    long qrs = 0, qcs = 0;
+    bool own_q = false;
+    if (cholqr_applies(c, rows, cols))
+    {
+      double* Qd = alloc_core(c, (size_t)rows * q);
+      if (cholqr2_strided(c, M, cols, 1, rows, cols, Qd, q, 1, X, rnp))
+      {
+        Q = Qd;
+        qrs = q;
+        qcs = 1;
+        own_q = true;
+      }
+      else
+        c->mem.put(Qd);
+    }
+    if (!Q)
+    {
+      double* A = ctx_ws(c, WS_A, (size_t)rows * cols);
+      launch_transpose(M, cols, rows, cols, A, rows, c->stream);
+      double* tau = ctx_ws(c, WS_F, (size_t)q);
+      Q = ctx_ws(c, WS_Q, (size_t)rows * q);
+      qrs = 1;
+      qcs = rows;
+      qr_factor(c, A, rows, rows, cols, tau, Q, rows, false);
+      launch_extract_upper(A, rows, q, q, X, q, 1, c->stream);
+    }
     if (try_cert && certify_full_rank(c, X, q, delta, &rn))
     {
-      double* U = alloc_core(c, (size_t)rows * q);
-      launch_transpose(Q, rows, q, rows, U, q, c->stream);
+      double* U = Q;
+      if (!own_q)
+      {
+        U = alloc_core(c, (size_t)rows * q);
+        launch_transpose(Q, rows, q, rows, U, q, c->stream);
+      }
       *U_out = U;
       *F_out = X;
       return q;
     }
+    // The Jacobi iteration overwrites X; R is needed again for F.
+    double* Rb = alloc_core(c, (size_t)q * q);
+    hip_check(hipMemcpyAsync(Rb, X, (size_t)q * q * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
     double* J = ctx_ws(c, WS_J, (size_t)q * q);
     const int r_new = square_svd(c, X, q, J, delta, max_rank, keep_all, &sv, &perm);
     polar_correct(c, J, q);
     double* Jp = ctx_ws(c, WS_JP, (size_t)q * r_new);
     launch_gather_cols(J, q, q, perm, r_new, Jp, 1, q, c->stream);
     double* U = alloc_core(c, (size_t)rows * r_new);
-    gemm_s(c, rows, r_new, q, Q, 1, rows, Jp, 1, q, U, r_new, 1);  // U = Q J'_k
-    double* Rb = alloc_core(c, (size_t)q * q);
-    launch_extract_upper(A, rows, q, q, Rb, q, 1, c->stream, full);  // R row-major
+    gemm_s(c, rows, r_new, q, Q, qrs, qcs, Jp, 1, q, U, r_new, 1);  // U = Q J'_k
     double* F = ctx_ws(c, WS_M1P, (size_t)r_new * cols);
-    gemm_s(c, r_new, q, q, Jp, q, 1, Rb, q, 1, F, cols, 1);         // F = J'_k^T R
+    gemm_s(c, r_new, q, q, Jp, q, 1, Rb, q, 1, F, cols, 1);          // F = J'_k^T R
     c->mem.put(Rb);
+    if (own_q) c->mem.put(Q);
     *U_out = U;
     *F_out = F;
     if (s_out) *s_out = sv;
