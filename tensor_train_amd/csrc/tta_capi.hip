@@ -265,10 +265,19 @@ gemm_desc gdesc(int M, int N, int K, const double* A, long sar, long sac, const 
 }
 
 // Independent GEMMs (empty ones, M or N or K = 0, skipped) in one launch
-// on the context stream; a single one goes through gemm_go.
-void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
+// on the context stream; a single one goes through gemm_go, unless `fuse`
+// asks for the blocked Cholesky's next diagonal-block step in the same
+// launch (run after the first product's tile (0, 0), which must hold the
+// block: fuse->pG .. pj0 as for launch_potrf_inv).
+void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs, const gemm_multi* fuse = nullptr)
 {
   gemm_multi m;
+  if (fuse)
+  {
+    m = *fuse;
+    m.count = 0;
+    m.fuse = 0;
+  }
   double fl = 0.0, by = 0.0;
   for (const gemm_desc& g : gs)
   {
@@ -279,10 +288,15 @@ void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
     by += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
   }
   if (m.count == 0) return;
-  if (m.count == 1)
+  if (m.count == 1 && !fuse)
   {
     gemm_go(c, m.g[0]);
     return;
+  }
+  if (fuse)
+  {
+    fl += 2.0 * m.pjb * m.pjb * m.pjb / 3.0;
+    by += 24.0 * m.pjb * m.pjb;
   }
   prof_scope ps(c, TTA_K_GEMM_MULTI, fl, by);
   launch_gemm_multi(m, c->stream);
@@ -345,21 +359,34 @@ struct rinv_norms
 void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double* T, double* info)
 {
   const int B = potrf_block();
+  {
+    const int jb = std::min(B, n);
+    prof_scope ps(c, TTA_K_CHOL, 2.0 * jb * jb * jb / 3.0, 24.0 * jb * jb);
+    launch_potrf_inv(G, n, R, n, X, n, jb, 0, info, c->stream);
+  }
   for (int j0 = 0; j0 < n; j0 += B)
   {
     const int jb = std::min(B, n - j0), j1 = j0 + jb, w = n - j1;
-    {
-      prof_scope ps(c, TTA_K_CHOL, 2.0 * jb * jb * jb / 3.0, 24.0 * jb * jb);
-      launch_potrf_inv(G, n, R, n, X, n, jb, j0, info, c->stream);
-    }
     const double* D = X + (size_t)j0 * n + j0;
     double* Rrow = R + (size_t)j0 * n + j1;
     // Block row of R beside the first product of R^-1's block column, then
-    // the trailing update beside the second (each pair independent).
+    // the trailing update beside the second (each pair independent); the
+    // workgroup whose trailing-update tile is the next diagonal block goes
+    // on to factor it.
     gemm_multi_go(c, {gdesc(jb, w, jb, D, 1, n, G + (size_t)j0 * n + j1, n, 1, Rrow, n, 1),
                       gdesc(j0, jb, j0, X, n, 1, R + j0, n, 1, T, jb, 1)});
-    gemm_multi_go(c, {gdesc(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0),
-                      gdesc(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0)});
+    gemm_multi next;
+    next.pG = G;
+    next.pR = R;
+    next.pX = X;
+    next.pinfo = info;
+    next.pld = n;
+    next.pjb = std::min(B, w);
+    next.pj0 = j1;
+    gemm_multi_go(c,
+                  {gdesc(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0),
+                   gdesc(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0)},
+                  w > 0 ? &next : nullptr);
   }
 }
 
