@@ -248,7 +248,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
-              "independent TTs across GPUs, which this build does not run.", file=sys.stderr)
+              "independent TTs across GPUs, which this library does not do.", file=sys.stderr)
         sys.exit(2)
 
     import tensor_train_amd as T
