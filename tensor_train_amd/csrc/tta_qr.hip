@@ -492,7 +492,13 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
   const bool mfma_update = batch >= 16;
   for (int j0 = 0; j0 < kmax; j0 += QB)
   {
-    if (b.max_m <= NT)
+    // A single small matrix: one wave holding up to 4 rows per lane, so the
+    // per-column reductions need no cross-wave barrier.
+    if (batch == 1 && b.max_m <= 64)
+      hipLaunchKernelGGL((qr_panel_kernel<64, 1>), dim3(1), dim3(64), 0, st, b, j0);
+    else if (batch == 1 && b.max_m <= 256)
+      hipLaunchKernelGGL((qr_panel_kernel<64, 4>), dim3(1), dim3(64), 0, st, b, j0);
+    else if (b.max_m <= NT)
       hipLaunchKernelGGL((qr_panel_kernel<NT, 1>), dim3(batch), dim3(NT), 0, st, b, j0);
     else if (b.max_m <= 2 * NT)
       hipLaunchKernelGGL((qr_panel_kernel<NT, 2>), dim3(batch), dim3(NT), 0, st, b, j0);
