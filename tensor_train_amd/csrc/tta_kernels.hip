@@ -930,13 +930,15 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st)
 
 // Slices for a single GEMM whose 64 x 64 tiles alone would leave most of
 // the 256 compute units idle: enough to reach about 256 workgroups, each
-// slice at least 64 deep in K; 1 when the tiles already fill the device.
+// slice at least 64 deep in K, and at most 8 (beyond that the partials'
+// extra traffic costs as much as the shorter slices save); 1 when the tiles
+// already fill the device.
 int gemm_splitk_slices(int M, int N, int K)
 {
   const long tiles = (long)((M + GT - 1) / GT) * ((N + GT - 1) / GT);
   if (tiles >= 128 || K < 128) return 1;
   long s = std::min<long>(256 / tiles, K / 64);
-  s = std::min<long>(s, 16);
+  s = std::min<long>(s, 8);
   return s < 2 ? 1 : (int)s;
 }
 
