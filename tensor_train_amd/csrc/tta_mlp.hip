@@ -38,20 +38,24 @@ namespace
 constexpr int TS = 16;
 
 // C (M x N) = op(A) op(B) [+ bias] [relu], FP32.  opA: A(i,k) = trA ? A[k*lda+i] : A[i*lda+k].
+// Split over K: block z sums k in [z*ks, min(K, (z+1)*ks)) into C + z*cstr
+// (one z and ks = K for a plain product).
 __global__ __launch_bounds__(256) void sgemm_kernel(int M, int N, int K, const float* A, int lda, int trA,
                                                     const float* B, int ldb, int trB, float* C, int ldc,
-                                                    const float* bias, int relu, float beta)
+                                                    const float* bias, int relu, float beta, int ks, size_t cstr)
 {
   __shared__ float As[TS][TS + 1];
   __shared__ float Bs[TS][TS + 1];
   const int tx = threadIdx.x % TS, ty = threadIdx.x / TS;
   const int i = blockIdx.y * TS + ty, j = blockIdx.x * TS + tx;
+  const int kbeg = blockIdx.z * ks, kend = min(K, kbeg + ks);
+  C += blockIdx.z * cstr;
   float acc = 0.f;
-  for (int k0 = 0; k0 < K; k0 += TS)
+  for (int k0 = kbeg; k0 < kend; k0 += TS)
   {
     const int ka = k0 + tx, kb = k0 + ty;
-    As[ty][tx] = (i < M && ka < K) ? (trA ? A[(size_t)ka * lda + i] : A[(size_t)i * lda + ka]) : 0.f;
-    Bs[ty][tx] = (kb < K && j < N) ? (trB ? B[(size_t)j * ldb + kb] : B[(size_t)kb * ldb + j]) : 0.f;
+    As[ty][tx] = (i < M && ka < kend) ? (trA ? A[(size_t)ka * lda + i] : A[(size_t)i * lda + ka]) : 0.f;
+    Bs[ty][tx] = (kb < kend && j < N) ? (trB ? B[(size_t)j * ldb + kb] : B[(size_t)kb * ldb + j]) : 0.f;
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < TS; ++kk) acc = fmaf(As[ty][kk], Bs[kk][tx], acc);
@@ -94,15 +98,36 @@ __global__ __launch_bounds__(256) void mse_kernel(const float* out, const float*
   if (threadIdx.x == 0) atomicAdd(loss, red[0] / (float)n);
 }
 
-// db[j] = sum_i dY[i, j]  (one thread per column).
-__global__ void colsum_kernel(const float* dY, int n, int m, float* db)
+// Partial column sums of dY (n x m): block (x, z) sums rows
+// [z*ks, min(n, (z+1)*ks)) of columns [64x, 64x + 64) into part + z*pstr.
+__global__ __launch_bounds__(256) void colsum_part_kernel(const float* dY, int n, int m, int ks, float* part,
+                                                          size_t pstr)
 {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= m) return;
+  __shared__ float red[4][64];
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  const int j = blockIdx.x * 64 + tx;
+  const int r0 = blockIdx.y * ks, r1 = min(n, r0 + ks);
   float s = 0.f;
-  for (int i = 0; i < n; ++i) s += dY[(size_t)i * m + j];
-  db[j] = s;
+  if (j < m)
+    for (int i = r0 + ty; i < r1; i += 4) s += dY[(size_t)i * m + j];
+  red[ty][tx] = s;
+  __syncthreads();
+  if (ty == 0 && j < m) part[blockIdx.y * pstr + j] = (red[0][tx] + red[1][tx]) + (red[2][tx] + red[3][tx]);
 }
+
+// out[e] = sum_{z < np} part[z*pstr + e] in order z = 0, 1, ... (deterministic).
+__global__ void sum_parts_kernel(const float* part, int np, size_t pstr, long len, float* out)
+{
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < len; e += (long)gridDim.x * blockDim.x)
+  {
+    float s = 0.f;
+    for (int z = 0; z < np; ++z) s += part[z * pstr + e];
+    out[e] = s;
+  }
+}
+
+// Rows of the batch per slice of the batch reduction behind dW and db.
+int batch_slice(int n) { return std::max(512, (n + 127) / 128 + TS - 1) / TS * TS; }
 
 __global__ void adam_kernel(float* p, const float* g, float* m1, float* m2, long n, float lr, float b1, float b2,
                             float c1, float c2, float epsa)
@@ -123,7 +148,7 @@ void sgemm(hipStream_t st, int M, int N, int K, const float* A, int lda, int trA
 {
   dim3 grid((N + TS - 1) / TS, (M + TS - 1) / TS);
   hipLaunchKernelGGL(sgemm_kernel, grid, dim3(TS * TS), 0, st, M, N, K, A, lda, trA, B, ldb, trB, C, ldc, bias, relu,
-                     beta);
+                     beta, K, (size_t)0);
 }
 
 struct mlp_ws
@@ -132,6 +157,7 @@ struct mlp_ws
   std::vector<float*> dact;
   float* y = nullptr;
   float* loss = nullptr;
+  float* part = nullptr;  // per-slice partial dW and db of one layer
 };
 
 // Forward pass over n rows; activations kept for backward.
@@ -153,10 +179,18 @@ void backward(tta_ctx* c, tta_mlp* m, int n, mlp_ws& w)
   for (int l = L - 1; l >= 0; --l)
   {
     const int in = m->dims[l], out = m->dims[l + 1];
-    // dW = act_l^T dact_{l+1}; db = colsum(dact_{l+1}).
-    sgemm(c->stream, in, out, n, w.act[l], in, 1, w.dact[l + 1], out, 0, m->g + m->woff[l], out);
-    hipLaunchKernelGGL(colsum_kernel, dim3((out + 63) / 64), dim3(64), 0, c->stream, w.dact[l + 1], n, out,
-                       m->g + m->boff[l]);
+    // dW = act_l^T dact_{l+1}; db = colsum(dact_{l+1}).  Both reduce over the
+    // n rows of the batch: each slice of ks rows writes its partial [dW | db]
+    // (db follows dW in the parameter layout), then one pass sums the slices.
+    const int ks = batch_slice(n), np = (n + ks - 1) / ks;
+    const size_t len = (size_t)in * out + out;
+    dim3 grid((out + TS - 1) / TS, (in + TS - 1) / TS, np);
+    hipLaunchKernelGGL(sgemm_kernel, grid, dim3(TS * TS), 0, c->stream, in, out, n, w.act[l], in, 1, w.dact[l + 1],
+                       out, 0, w.part, out, nullptr, 0, 0.f, ks, len);
+    hipLaunchKernelGGL(colsum_part_kernel, dim3((out + 63) / 64, np), dim3(256), 0, c->stream, w.dact[l + 1], n, out,
+                       ks, w.part + (size_t)in * out, len);
+    hipLaunchKernelGGL(sum_parts_kernel, dim3((int)std::min<size_t>((len + 255) / 256, 1024)), dim3(256), 0,
+                       c->stream, w.part, np, len, (long)len, m->g + m->woff[l]);
     if (l > 0)
     {
       // dact_l = dact_{l+1} W^T, masked by relu'(act_l).
@@ -197,6 +231,13 @@ void make_ws(tta_ctx* c, tta_mlp* m, int n, ws_owner& O, mlp_ws& w, bool grads)
   }
   w.y = O.f((size_t)n);
   w.loss = O.f(1);
+  if (grads)
+  {
+    size_t len = 0;
+    for (int l = 0; l < L; ++l) len = std::max(len, (size_t)m->dims[l] * m->dims[l + 1] + m->dims[l + 1]);
+    const int ks = batch_slice(n);
+    w.part = O.f(len * (size_t)((n + ks - 1) / ks));
+  }
 }
 }  // namespace
 }  // namespace tta

@@ -11,14 +11,17 @@ import tensor_train_amd as T
 pytestmark = pytest.mark.gpu
 
 
-def test_mlp_gradients_match_cpu():
+# 300 rows fit one slice of the batch reduction behind dW and db; 5000 rows
+# split into several slices whose partial sums are added afterwards.
+@pytest.mark.parametrize("n", [300, 5000])
+def test_mlp_gradients_match_cpu(n):
     rng = np.random.default_rng(0)
     m = T.RankMLP(13, 64, 3, seed=5)
     dims = [13, 64, 64, 64, 1]
     p = m.get_params()
     assert p.size == sum(a * b + b for a, b in zip(dims[:-1], dims[1:]))
-    X = rng.standard_normal((300, 13)).astype(np.float32)
-    y = rng.uniform(0, 1, 300).astype(np.float32)
+    X = rng.standard_normal((n, 13)).astype(np.float32)
+    y = rng.uniform(0, 1, n).astype(np.float32)
     loss, g = m.grad(X, y)
     loss_ref, g_ref = R.loss_and_grad(p, dims, X, y)
     assert abs(loss - loss_ref) <= 1e-4 * max(1.0, loss_ref)
