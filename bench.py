@@ -268,8 +268,8 @@ def main():
     for _ in range(args.prof_steps):
         y = T.matvec_round(A, x, eps=EPS)
     ctx.synchronize()
-    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "gemm_multi", "apply", "chol", "rinv",
-                                           "other")}
+    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "gemm_multi", "chol_step", "apply",
+                                           "chol", "rinv", "other")}
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     spec_sweeps, spec_failed = ctx.speculation_counts()
@@ -281,6 +281,7 @@ def main():
               "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"]),
               "gemm_splitk": ("gemm_f64_splitk_kernel", stats["gemm_splitk"]),
               "gemm_multi": ("gemm_f64_multi_kernel", stats["gemm_multi"]),
+              "chol_step": ("chol_step_kernel", stats["chol_step"]),
               "chol": ("potrf_inv_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}
     ctx.prof_enable(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps

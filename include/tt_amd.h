@@ -294,15 +294,20 @@ enum tta_kernel_class
    * inverse of a b x b block, b <= 64: credited 2 b^3 / 3 flops, 24 b^2
    * bytes) and the triangular inverse of the full-rank certificate where
    * the QR was Householder (n^3/3 flops, 16 n^2 bytes).  The blocked
-   * Cholesky's products count as GEMMs, CholQR2's elementwise steps as
-   * TTA_K_OTHER. */
+   * Cholesky's later block steps are TTA_K_CHOL_STEP, CholQR2's elementwise
+   * steps TTA_K_OTHER. */
   TTA_K_CHOL = 6,
   TTA_K_RINV = 7,
   /* The partial products of a GEMM split along K (their sum: TTA_K_OTHER). */
   TTA_K_GEMM_SPLITK = 8,
   /* Launches of up to four independent GEMMs together. */
   TTA_K_GEMM_MULTI = 9,
-  TTA_K_COUNT = 10
+  /* One block step of CholQR2's blocked Cholesky with inverse in one launch:
+   * block row of R, trailing update, R^-1's block column and the next
+   * diagonal block (credited the flops and bytes of those products and of
+   * the block factorization). */
+  TTA_K_CHOL_STEP = 10,
+  TTA_K_COUNT = 11
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

@@ -111,7 +111,7 @@ SIGNATURES = {
 }
 
 KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4, "jacobi_round": 5, "chol": 6, "rinv": 7,
-                  "gemm_splitk": 8, "gemm_multi": 9}
+                  "gemm_splitk": 8, "gemm_multi": 9, "chol_step": 10}
 
 _lib = None
 

@@ -356,7 +356,7 @@ struct rinv_norms
 //   X(<J, J)       = -X(<J, <J) R(<J, J) D     (block column of R^-1).
 // G is overwritten by the trailing updates; T is scratch of n * 64 doubles;
 // info[0..2] as for launch_potrf_inv.
-void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double* T, double* info)
+void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double* info)
 {
   const int B = potrf_block();
   {
@@ -364,30 +364,38 @@ void chol_inv_blocked(tta_ctx* c, double* G, int n, double* R, double* X, double
     prof_scope ps(c, TTA_K_CHOL, 2.0 * jb * jb * jb / 3.0, 24.0 * jb * jb);
     launch_potrf_inv(G, n, R, n, X, n, jb, 0, info, c->stream);
   }
-  for (int j0 = 0; j0 < n; j0 += B)
+  if (n <= B) return;
+  double* S = alloc_core(c, chol_step_scratch(n));
+  try
   {
-    const int jb = std::min(B, n - j0), j1 = j0 + jb, w = n - j1;
-    const double* D = X + (size_t)j0 * n + j0;
-    double* Rrow = R + (size_t)j0 * n + j1;
-    // Block row of R beside the first product of R^-1's block column, then
-    // the trailing update beside the second (each pair independent); the
-    // workgroup whose trailing-update tile is the next diagonal block goes
-    // on to factor it.
-    gemm_multi_go(c, {gdesc(jb, w, jb, D, 1, n, G + (size_t)j0 * n + j1, n, 1, Rrow, n, 1),
-                      gdesc(j0, jb, j0, X, n, 1, R + j0, n, 1, T, jb, 1)});
-    gemm_multi next;
-    next.pG = G;
-    next.pR = R;
-    next.pX = X;
-    next.pinfo = info;
-    next.pld = n;
-    next.pjb = std::min(B, w);
-    next.pj0 = j1;
-    gemm_multi_go(c,
-                  {gdesc(w, w, jb, Rrow, 1, n, Rrow, n, 1, G + (size_t)j1 * n + j1, n, 1, -1.0, 1.0),
-                   gdesc(j0, jb, jb, T, jb, 1, D, n, 1, X + j0, n, 1, -1.0, 0.0)},
-                  w > 0 ? &next : nullptr);
+    for (int j0 = 0; j0 < n; j0 += B)
+    {
+      chol_step st;
+      int nx = 0;
+      chol_step_shape(n, j0, st.jb, st.nt, st.ntri, nx);
+      const int jb = st.jb, w = n - j0 - jb;
+      st.G = G;
+      st.R = R;
+      st.X = X;
+      st.S = S;
+      st.info = info;
+      st.n = n;
+      st.j0 = j0;
+      // Flops and bytes of the plain block step: block row of R, trailing
+      // update, X's block column (T and -T D) and the next diagonal block.
+      const int nb = std::min(B, w);
+      const double fl = 2.0 * jb * jb * w + 2.0 * w * w * jb + 2.0 * j0 * jb * (j0 + jb) + 2.0 * nb * nb * nb / 3.0;
+      const double by = 8.0 * (2.0 * jb * w + 2.0 * w * w + 2.0 * j0 * j0 + 3.0 * j0 * jb + 3.0 * nb * nb);
+      prof_scope ps(c, TTA_K_CHOL_STEP, fl, by);
+      launch_chol_step(st, c->stream);
+    }
   }
+  catch (...)
+  {
+    c->mem.put(S);
+    throw;
+  }
+  c->mem.put(S);
 }
 
 // CholQR2 of a column-major (m x n) A, m >= n: A = Q R with Q^T Q = I.
@@ -451,7 +459,7 @@ bool cholqr2_strided(tta_ctx* c, const double* A, long sar, long sac, int m, int
   try
   {
     gm(n, n, m, A, sac, sar, A, sar, sac, G, n, 1);  // G = A^T A
-    chol_inv_blocked(c, G, n, R1, Xi, W, aux);        // R1, Xi = R1^-1
+    chol_inv_blocked(c, G, n, R1, Xi, aux);           // R1, Xi = R1^-1
     gm(m, n, n, A, sar, sac, Xi, n, 1, Q1, 1, m);    // Q1 = A R1^-1
     if (split)
     {

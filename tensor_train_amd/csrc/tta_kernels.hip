@@ -77,12 +77,14 @@ struct mfma_step<float>
   __device__ static int row(int fk, int r) { return 4 * fk + r; }
 };
 
-// Tile of C = alpha A B + beta C with A of type TA, B of type TB, C of type
-// TC, multiplied on the matrix cores in CT (operands converted on the way
-// into LDS).
+// The 64 x 64 tile at (i0, j0) of C = alpha A B + beta C with A of type TA,
+// B of type TB, C of type TC, multiplied on the matrix cores in CT (operands
+// converted on the way into LDS).  Every use of LDS is followed by a
+// barrier, so a workgroup may run several in a row.
 template <class TA, class TB, class TC, class CT>
 __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, long sar, long sac, const TB* B, long sbr,
-                                            long sbc, TC* C, long scr, long scc, double alpha, double beta)
+                                            long sbc, TC* C, long scr, long scc, double alpha, double beta, int i0,
+                                            int j0)
 {
   using S = mfma_step<CT>;
   __shared__ CT As[GK][GT + 1];
@@ -90,7 +92,6 @@ __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, lo
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wr = wid >> 1, wc = wid & 1;
-  const int i0 = blockIdx.y * GT, j0 = blockIdx.x * GT;
   const bool a_kc = (sac == 1);  // A contiguous along k
   const bool b_jc = (sbc == 1);  // B contiguous along j
   typename S::acc_t acc[2][2];
@@ -167,7 +168,7 @@ __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, lo
 __device__ __forceinline__ void gemm_tile(const gemm_desc& g, const double* A, const double* B, double* C)
 {
   gemm_tile_t<double, double, double, double>(g.M, g.N, g.K, A, g.sar, g.sac, B, g.sbr, g.sbc, C, g.scr, g.scc, g.alpha,
-                                              g.beta);
+                                              g.beta, blockIdx.y * GT, blockIdx.x * GT);
 }
 
 __global__ __launch_bounds__(256) void gemm_f64_kernel(gemm_desc g)
@@ -188,7 +189,7 @@ __global__ __launch_bounds__(256) void gemm_f64_splitk_kernel(gemm_desc g, int k
   double* P = part + (size_t)blockIdx.z * g.M * g.N;
   gemm_tile_t<double, double, double, double>(g.M, g.N, kn, g.A + (size_t)k0 * g.sac, g.sar, g.sac,
                                               g.B + (size_t)k0 * g.sbr, g.sbr, g.sbc, P, rm ? g.N : 1,
-                                              rm ? 1 : g.M, 1.0, 0.0);
+                                              rm ? 1 : g.M, 1.0, 0.0, blockIdx.y * GT, blockIdx.x * GT);
 }
 
 __global__ __launch_bounds__(256) void splitk_reduce_kernel(gemm_desc g, int S, const double* part)
@@ -279,6 +280,69 @@ __global__ __launch_bounds__(256) void gemm_f64_multi_kernel(gemm_multi m)
   }
 }
 
+// One block step of the blocked Cholesky with inverse (chol_step in the
+// header), one workgroup per output tile:
+//   * trailing tile (a, b), a <= b, of the upper triangle (tile (0, 0)
+//     first): R_a = D^T G(jb rows from j0, columns of block a) and likewise
+//     R_b (the block row of R; tile (a, a) writes its R_a into R, others
+//     into their scratch), then G(block a, block b) -= R_a^T R_b.  Tile
+//     (0, 0) then factors the next diagonal block (potrf_inv_block).
+//     Lower trailing tiles are left stale: only the upper block rows and the
+//     diagonal blocks of G are read afterwards.
+//   * X tile i (rows 64 i .. of the first j0): T = X(rows, 0:j0) R(0:j0,
+//     j0:j1) into scratch, then X(rows, j0:j1) = -T D.
+// D = R_jj^-1, the step's diagonal block of X, factored by the launch before.
+__global__ __launch_bounds__(256) void chol_step_kernel(chol_step s)
+{
+  const int n = s.n, j0 = s.j0, jb = s.jb, j1 = j0 + jb, w = n - j1;
+  const double* D = s.X + (size_t)j0 * n + j0;
+  const int t = blockIdx.x;
+  double* S = s.S + (size_t)t * 2 * GT * GT;
+  if (t < s.ntri)
+  {
+    int a = 0, rem = t;
+    while (rem >= s.nt - a)
+    {
+      rem -= s.nt - a;
+      ++a;
+    }
+    const int b = a + rem;
+    const int ca = j1 + GT * a, cb = j1 + GT * b;
+    const int wa = min(GT, n - ca), wb = min(GT, n - cb);
+    double* Ra = a == b ? s.R + (size_t)j0 * n + ca : S;
+    const long la = a == b ? n : GT;
+    gemm_tile_t<double, double, double, double>(jb, wa, jb, D, 1, n, s.G + (size_t)j0 * n + ca, n, 1, Ra, la, 1, 1.0,
+                                                0.0, 0, 0);
+    const double* Rb = Ra;
+    long lb = la;
+    if (a != b)
+    {
+      double* Rw = S + GT * GT;
+      gemm_tile_t<double, double, double, double>(jb, wb, jb, D, 1, n, s.G + (size_t)j0 * n + cb, n, 1, Rw, GT, 1,
+                                                  1.0, 0.0, 0, 0);
+      Rb = Rw;
+      lb = GT;
+    }
+    __syncthreads();  // R_a and R_b, stored by the whole workgroup, are read by the whole workgroup
+    gemm_tile_t<double, double, double, double>(wa, wb, jb, Ra, 1, la, Rb, lb, 1, s.G + (size_t)ca * n + cb, n, 1, -1.0,
+                                                1.0, 0, 0);
+    if (t == 0 && w > 0)
+    {
+      __syncthreads();  // the updated diagonal block is visible to the whole workgroup
+      potrf_inv_block(s.G, n, s.R, n, s.X, n, min(GT, w), j1, s.info);
+    }
+  }
+  else
+  {
+    const int ri = GT * (t - s.ntri), hi = min(GT, j0 - ri);
+    gemm_tile_t<double, double, double, double>(hi, jb, j0, s.X + (size_t)ri * n, n, 1, s.R + j0, n, 1, S, GT, 1, 1.0,
+                                                0.0, 0, 0);
+    __syncthreads();  // T is read by the whole workgroup
+    gemm_tile_t<double, double, double, double>(hi, jb, jb, S, GT, 1, D, n, 1, s.X + (size_t)ri * n + j0, n, 1, -1.0,
+                                                0.0, 0, 0);
+  }
+}
+
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
@@ -297,7 +361,8 @@ __global__ __launch_bounds__(256) void xgemm_grouped_kernel(const xgemm_desc* gs
   const xgemm_desc& g = gs[blockIdx.z];
   if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
   gemm_tile_t<TA, TB, TC, CT>(g.M, g.N, g.K, static_cast<const TA*>(g.A), g.sar, g.sac, static_cast<const TB*>(g.B),
-                              g.sbr, g.sbc, static_cast<TC*>(g.C), g.scr, g.scc, 1.0, 0.0);
+                              g.sbr, g.sbc, static_cast<TC*>(g.C), g.scr, g.scc, 1.0, 0.0, blockIdx.y * GT,
+                              blockIdx.x * GT);
 }
 
 // Householder QR (+ optional thin Q) of one column-major matrix per block.
@@ -940,6 +1005,34 @@ int gemm_splitk_slices(int M, int N, int K)
   long s = std::min<long>(256 / tiles, K / 64);
   s = std::min<long>(s, 8);
   return s < 2 ? 1 : (int)s;
+}
+
+void chol_step_shape(int n, int j0, int& jb, int& nt, int& ntri, int& nx)
+{
+  jb = std::min(kPotrfB, n - j0);
+  const int w = n - j0 - jb;
+  nt = (w + GT - 1) / GT;
+  ntri = nt * (nt + 1) / 2;
+  nx = (j0 + GT - 1) / GT;
+}
+
+size_t chol_step_scratch(int n)
+{
+  size_t most = 0;
+  for (int j0 = 0; j0 < n; j0 += kPotrfB)
+  {
+    int jb, nt, ntri, nx;
+    chol_step_shape(n, j0, jb, nt, ntri, nx);
+    most = std::max(most, (size_t)(ntri + nx) * 2 * GT * GT);
+  }
+  return most;
+}
+
+void launch_chol_step(const chol_step& s, hipStream_t st)
+{
+  const int wgs = s.ntri + (s.j0 + GT - 1) / GT;
+  if (wgs <= 0) return;
+  hipLaunchKernelGGL(chol_step_kernel, dim3(wgs), dim3(256), 0, st, s);
 }
 
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st)

@@ -230,6 +230,27 @@ struct gemm_multi
   int pld = 0, pjb = 0, pj0 = 0;
 };
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st);
+// One block step of the blocked Cholesky with inverse of the n x n
+// row-major G (upper triangle and diagonal blocks read): with the diagonal
+// block [j0, j0 + jb) already factored into R and X (its R_jj^-1), the block
+// row of R right of it, the trailing update of G's upper block rows, the
+// factorization of the next diagonal block, and X's block column above the
+// diagonal block, in one launch.  S: chol_step_scratch(n) doubles.
+struct chol_step
+{
+  double* G = nullptr;
+  double* R = nullptr;
+  double* X = nullptr;
+  double* S = nullptr;
+  double* info = nullptr;
+  int n = 0, j0 = 0, jb = 0;
+  int nt = 0, ntri = 0;  // trailing block columns and upper-triangle tiles
+};
+// Block size, trailing block columns, trailing upper-triangle tiles and X
+// row blocks of the step at j0.
+void chol_step_shape(int n, int j0, int& jb, int& nt, int& ntri, int& nx);
+size_t chol_step_scratch(int n);
+void launch_chol_step(const chol_step& s, hipStream_t st);
 // Epilogues folded into the split-K sum of a square row-major n x n product
 // v = alpha P + beta C (CholQR2's second pass):
 //   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E) (if given), part[b] = max |E| over
