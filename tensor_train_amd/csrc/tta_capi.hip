@@ -1272,12 +1272,21 @@ tt_snapshot take_snapshot(tta_ctx* c, const tta_tt* t)
 {
   tt_snapshot s;
   s.ranks = t->ranks;
+  copy_batch b;
+  double total = 0.0;
+  for (int k = 0; k < t->d(); ++k) total += (double)core_numel(t, k);
+  prof_scope ps(c, TTA_K_OTHER, 0.0, 16.0 * total);
   for (int k = 0; k < t->d(); ++k)
   {
     const size_t n = core_numel(t, k);
     double* p = alloc_core(c, n);
-    hip_check(hipMemcpyAsync(p, t->cores[k], n * 8, hipMemcpyDeviceToDevice, c->stream), "hipMemcpyAsync");
     s.cores.push_back(p);
+    b.it[b.count++] = {t->cores[k], p, (long)n};
+    if (b.count == kCopyBatch || k + 1 == t->d())
+    {
+      launch_copy_batch(b, c->stream);
+      b.count = 0;
+    }
   }
   return s;
 }

@@ -901,11 +901,14 @@ __global__ __launch_bounds__(256) void minus_ident_kernel(double* G, int n, doub
 // cleared by the first factorization or certificate that does not hold.
 // CholQR2 acceptance: no breakdown in pass 1 (info[0] == 0, finite extreme
 // diagonal entries) and n max|E| <= thr over the partial maxima.
-__global__ void spec_cholqr_kernel(const double* info, const double* part, int np, int n, double thr, double* flag)
+__global__ __launch_bounds__(64) void spec_cholqr_kernel(const double* info, const double* part, int np, int n,
+                                                         double thr, double* flag)
 {
-  if (threadIdx.x != 0) return;
   double emax = 0.0;
-  for (int b = 0; b < np; ++b) emax = fmax(emax, part[b]);
+  for (int b = threadIdx.x; b < np; b += 64) emax = fmax(emax, part[b]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+  if (threadIdx.x != 0) return;
   const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= thr;
   if (!ok) flag[0] = 0.0;
 }
@@ -1169,6 +1172,28 @@ void launch_spec_cholqr(const double* info, const double* part, int np, int n, d
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st)
 {
   hipLaunchKernelGGL(spec_cert_kernel, dim3(1), dim3(64), 0, st, a, n, mode, delta, flag);
+}
+
+// Copy z of the batch: workgroups of blockIdx.y = z stride over its n
+// doubles, two per thread per step.
+__global__ __launch_bounds__(256) void copy_batch_kernel(copy_batch b)
+{
+  const copy_batch::item& it = b.it[blockIdx.y];
+  const long n = it.n;
+  for (long e = 2 * (blockIdx.x * 256L + threadIdx.x); e < n; e += 2L * gridDim.x * 256)
+  {
+    it.dst[e] = it.src[e];
+    if (e + 1 < n) it.dst[e + 1] = it.src[e + 1];
+  }
+}
+
+void launch_copy_batch(const copy_batch& b, hipStream_t st)
+{
+  long most = 0;
+  for (int z = 0; z < b.count; ++z) most = std::max(most, b.it[z].n);
+  if (b.count <= 0 || most <= 0) return;
+  const int gx = (int)std::min<long>((most + 511) / 512, 256);
+  hipLaunchKernelGGL(copy_batch_kernel, dim3(gx, b.count), dim3(256), 0, st, b);
 }
 
 void launch_set_value(double* p, double v, hipStream_t st)

@@ -339,11 +339,26 @@ void launch_spec_cholqr(const double* info, const double* part, int np, int n, d
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
 // p[0] = v.
 void launch_set_value(double* p, double v, hipStream_t st);
+// Up to kCopyBatch device-to-device copies of doubles in one launch
+// (descriptors by value).
+constexpr int kCopyBatch = 32;
+struct copy_batch
+{
+  struct item
+  {
+    const double* src;
+    double* dst;
+    long n;
+  };
+  item it[kCopyBatch];
+  int count = 0;
+};
+void launch_copy_batch(const copy_batch& b, hipStream_t st);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
 // G <- G - I (row-major n x n); max |G - I| over the entries, as
 // minus_ident_parts(n) <= kMinusIdentParts partial maxima in part.
-constexpr int kMinusIdentParts = 64;
+constexpr int kMinusIdentParts = 256;
 int minus_ident_parts(int n);
 void launch_minus_ident(double* G, int n, double* part, hipStream_t st);
 // X = strict upper part of E - W plus half its diagonal (W may be null).
