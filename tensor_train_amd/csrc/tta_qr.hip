@@ -504,7 +504,11 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
       hipLaunchKernelGGL((qr_panel_kernel<NT, 2>), dim3(batch), dim3(NT), 0, st, b, j0);
     else
       hipLaunchKernelGGL((qr_panel_kernel<NT, 4>), dim3(batch), dim3(NT), 0, st, b, j0);
-    const int ncols = b.max_n - j0 - 1;
+    // Columns right of the panel: for the single problem `one`, exactly
+    // those beyond its panel width (min(m, n) - j0 capped at QB); otherwise
+    // the update runs whenever any column may remain.
+    int ncols = b.max_n - j0 - 1;
+    if (!b.descs) ncols = b.one.n - j0 - std::min(std::min(b.one.m, b.one.n) - j0, QB);
     if (ncols > 0)
     {
       if (mfma_update)

@@ -23,7 +23,10 @@ def test_qr_golden():
         assert np.abs(R - g[f"R{i}"]).max() <= 1e-13 * max(1.0, np.abs(M).max()), (i, M.shape)
 
 
-@pytest.mark.parametrize("shape", [(1024, 256), (2048, 100), (300, 300), (96, 24), (640, 33), (17, 70), (4, 256)])
+# Includes shapes factored in one 16-column panel (no trailing update, or
+# one for the columns of a wide matrix) and shapes with a short last panel.
+@pytest.mark.parametrize("shape", [(1024, 256), (2048, 100), (300, 300), (96, 24), (640, 33), (17, 70), (4, 256),
+                                   (16, 16), (64, 4), (256, 16), (200, 13), (10, 40), (64, 64), (100, 70)])
 def test_qr_large_vs_oracle(shape):
     rng = np.random.default_rng(shape[0] * 7 + shape[1])
     M = rng.standard_normal(shape)
