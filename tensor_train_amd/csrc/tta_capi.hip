@@ -486,14 +486,16 @@ bool cholqr2_strided(tta_ctx* c, const double* A, long sar, long sac, int m, int
     if (c->spec)
     {
       // Decided on the device; a failed acceptance undoes the whole sweep.
-      ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), c->stream); });
+      // With a certificate, the column norms of R and R^-1 ride on the
+      // same launch.
       if (cert)
       {
         double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
-        ew([&] { launch_col_norms(Rf, n, n, cd, c->stream); });
-        ew([&] { launch_col_norms(Y, n, n, cd + n, c->stream); });
+        ew([&] { launch_spec_cholqr_norms(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), Rf, Y, cd, c->stream); });
         cert->dev = cd;
       }
+      else
+        ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), c->stream); });
       release();
       c->cholqr_tried += 1;
       c->cholqr_used += 1;

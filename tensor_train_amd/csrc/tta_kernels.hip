@@ -913,6 +913,36 @@ __global__ __launch_bounds__(64) void spec_cholqr_kernel(const double* info, con
   if (!ok) flag[0] = 0.0;
 }
 
+// spec_cholqr_kernel's decision in block 0 (its first wave) and, in blocks
+// 1 .., the column norms of the n x n row-major A and B into out[0, n) and
+// out[n, 2n) (col_norms_kernel's work, four columns per block).
+__global__ __launch_bounds__(256) void spec_cholqr_norms_kernel(const double* info, const double* part, int np, int n,
+                                                               double thr, double* flag, const double* A,
+                                                               const double* B, double* out)
+{
+  const int lane = threadIdx.x & 63;
+  if (blockIdx.x == 0)
+  {
+    if (threadIdx.x >= 64) return;
+    double emax = 0.0;
+    for (int b = lane; b < np; b += 64) emax = fmax(emax, part[b]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    if (lane != 0) return;
+    const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= thr;
+    if (!ok) flag[0] = 0.0;
+    return;
+  }
+  const int col = (blockIdx.x - 1) * 4 + (threadIdx.x >> 6);
+  if (col >= 2 * n) return;
+  const double* M = col < n ? A : B;
+  const int c = col < n ? col : col - n;
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 64) acc = fma(M[(size_t)i * n + c], M[(size_t)i * n + c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) out[col] = sqrt(acc);
+}
+
 // Full-rank certificate from ||R||_F^2 and ||R^-1||_F^2: mode 0 reads the
 // column norms of R (a[0..n)) and of R^-1 (a[n..2n)), mode 1 the n (xf2,
 // rf2) pairs of rinv_cert_kernel.  Same test as the host's: both sums finite
@@ -1167,6 +1197,13 @@ void launch_spec_cholqr(const double* info, const double* part, int np, int n, d
                         hipStream_t st)
 {
   hipLaunchKernelGGL(spec_cholqr_kernel, dim3(1), dim3(64), 0, st, info, part, np, n, thr, flag);
+}
+
+void launch_spec_cholqr_norms(const double* info, const double* part, int np, int n, double thr, double* flag,
+                              const double* A, const double* B, double* out, hipStream_t st)
+{
+  hipLaunchKernelGGL(spec_cholqr_norms_kernel, dim3(1 + (2 * n + 3) / 4), dim3(256), 0, st, info, part, np, n, thr,
+                     flag, A, B, out);
 }
 
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st)

@@ -337,6 +337,10 @@ void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, doubl
 void launch_spec_cholqr(const double* info, const double* part, int np, int n, double thr, double* flag,
                         hipStream_t st);
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
+// launch_spec_cholqr and the column norms of the n x n row-major A and B
+// (into out[0, n) and out[n, 2n)) in one launch.
+void launch_spec_cholqr_norms(const double* info, const double* part, int np, int n, double thr, double* flag,
+                              const double* A, const double* B, double* out, hipStream_t st);
 // p[0] = v.
 void launch_set_value(double* p, double v, hipStream_t st);
 // Up to kCopyBatch device-to-device copies of doubles in one launch
