@@ -265,19 +265,10 @@ gemm_desc gdesc(int M, int N, int K, const double* A, long sar, long sac, const 
 }
 
 // Independent GEMMs (empty ones, M or N or K = 0, skipped) in one launch
-// on the context stream; a single one goes through gemm_go, unless `fuse`
-// asks for the blocked Cholesky's next diagonal-block step in the same
-// launch (run after the first product's tile (0, 0), which must hold the
-// block: fuse->pG .. pj0 as for launch_potrf_inv).
-void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs, const gemm_multi* fuse = nullptr)
+// on the context stream; a single one goes through gemm_go.
+void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
 {
   gemm_multi m;
-  if (fuse)
-  {
-    m = *fuse;
-    m.count = 0;
-    m.fuse = 0;
-  }
   double fl = 0.0, by = 0.0;
   for (const gemm_desc& g : gs)
   {
@@ -288,15 +279,10 @@ void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs, const gemm_m
     by += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
   }
   if (m.count == 0) return;
-  if (m.count == 1 && !fuse)
+  if (m.count == 1)
   {
     gemm_go(c, m.g[0]);
     return;
-  }
-  if (fuse)
-  {
-    fl += 2.0 * m.pjb * m.pjb * m.pjb / 3.0;
-    by += 24.0 * m.pjb * m.pjb;
   }
   prof_scope ps(c, TTA_K_GEMM_MULTI, fl, by);
   launch_gemm_multi(m, c->stream);

@@ -273,11 +273,6 @@ __global__ __launch_bounds__(256) void gemm_f64_multi_kernel(gemm_multi m)
   const gemm_desc& g = m.g[blockIdx.z];
   if ((int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
   gemm_tile(g, g.A, g.B, g.C);
-  if ((int)blockIdx.z == m.fuse && blockIdx.x == 0 && blockIdx.y == 0)
-  {
-    __syncthreads();  // the tile's stores are visible to the whole workgroup
-    potrf_inv_block(m.pG, m.pld, m.pR, m.pld, m.pX, m.pld, m.pjb, m.pj0, m.pinfo);
-  }
 }
 
 // One block step of the blocked Cholesky with inverse (chol_step in the

@@ -215,19 +215,10 @@ void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 int gemm_splitk_slices(int M, int N, int K);
 // Up to four independent GEMMs in one launch (descriptors by value;
 // blockIdx.z picks the problem, tiles beyond its M x N exit).
-// With fuse >= 0, the workgroup of tile (0, 0) of problem `fuse` then runs
-// the blocked Cholesky's diagonal-block step (launch_potrf_inv's arguments
-// below) on the block that tile has just written.
 struct gemm_multi
 {
   gemm_desc g[4];
   int count = 0;
-  int fuse = -1;
-  const double* pG = nullptr;
-  double* pR = nullptr;
-  double* pX = nullptr;
-  double* pinfo = nullptr;
-  int pld = 0, pjb = 0, pj0 = 0;
 };
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st);
 // One block step of the blocked Cholesky with inverse of the n x n
