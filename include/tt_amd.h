@@ -375,9 +375,10 @@ int tta_mem_stats(tta_ctx* ctx, unsigned long long* live_bytes, unsigned long lo
 int tta_mem_trim(tta_ctx* ctx);
 
 /* ---- self-checks --------------------------------------------------------- */
-/* Compares the device's DPP wave reduction with a ds_bpermute reduction on n
- * host values (rounded up to whole 64-lane waves); *mismatches = 0 when they
- * agree to rounding.  No reference counterpart: a hardware sanity check. */
+/* Compares the device's DPP wave reductions (one value per lane, and 16
+ * values per lane at once) with ds_bpermute reductions on n host values
+ * (rounded up to whole 64-lane waves); *mismatches = 0 when they agree to
+ * rounding.  No reference counterpart: a hardware sanity check. */
 int tta_selfcheck_wave_sum(tta_ctx* ctx, const double* host, int n, int* mismatches);
 
 #ifdef __cplusplus

@@ -52,6 +52,32 @@ __device__ __forceinline__ double wave_sum(double v)
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// Sums of 16 values over the 64 lanes of a wave at once: lane l returns the
+// sum over all lanes of v[l & 15].  Within each 16-lane row a butterfly
+// halves the values at each step, with partners l ^ 15, l ^ 7, l ^ 3 and
+// l ^ 1 (DPP row_mirror, row_half_mirror, quad_perm [3,2,1,0] and
+// [1,0,3,2]): a lane keeps the half picked by its own bit 3, 2, 1, 0 and
+// adds the partner's copy of that half, which the partner gives away because
+// its bit is the opposite.  The four rows are then added by two lane
+// exchanges (xor 16, xor 32).  15 DPP moves instead of 16 separate
+// reductions.
+__device__ __forceinline__ double wave_vsum16(const double (&v)[16])
+{
+  const int lane = threadIdx.x & (kWave - 1);
+  double a[8], b[4], c[2];
+  const bool h3 = lane & 8, h2 = lane & 4, h1 = lane & 2, h0 = lane & 1;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) a[i] = (h3 ? v[8 + i] : v[i]) + dpp_mov<0x140>(h3 ? v[i] : v[8 + i]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b[i] = (h2 ? a[4 + i] : a[i]) + dpp_mov<0x141>(h2 ? a[i] : a[4 + i]);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) c[i] = (h1 ? b[2 + i] : b[i]) + dpp_mov<0x1B>(h1 ? b[i] : b[2 + i]);
+  double r = (h0 ? c[1] : c[0]) + dpp_mov<0xB1>(h0 ? c[0] : c[1]);
+  r += __shfl_xor(r, 16, kWave);
+  r += __shfl_xor(r, 32, kWave);
+  return r;
+}
+
 // Reference reduction through ds_bpermute, kept for cross-checking the DPP
 // path in the tests.
 __device__ __forceinline__ double wave_sum_shfl(double v)

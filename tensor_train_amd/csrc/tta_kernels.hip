@@ -631,8 +631,10 @@ __global__ void scale_kernel(double* x, long n, double alpha)
   for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < n; t += (long)gridDim.x * blockDim.x) x[t] *= alpha;
 }
 
-// Self-check of the DPP wave reduction against the ds_bpermute one: each
-// wave sums 64 values of x; mismatches (beyond 4 ulp-scaled) are counted.
+// Self-check of the DPP wave reductions against the ds_bpermute one: each
+// wave sums 64 values of x (wave_sum), and 16 vectors of 64 values taken
+// from x at strides of 64 (wave_vsum16); mismatches (beyond 8 ulp of the
+// sum of magnitudes) are counted.
 __global__ __launch_bounds__(256) void wave_sum_check_kernel(const double* x, int n, int* bad)
 {
   const int i = blockIdx.x * 256 + threadIdx.x;
@@ -641,6 +643,22 @@ __global__ __launch_bounds__(256) void wave_sum_check_kernel(const double* x, in
   double mag = wave_sum(fabs(v));
   if ((threadIdx.x & 63) == 0 && fabs(a - b) > 8.0 * 2.220446049250313e-16 * mag) atomicAdd(bad, 1);
   if (fabs(a - __shfl(a, 0)) != 0.0) atomicAdd(bad, 1000);
+  double v16[16];
+#pragma unroll
+  for (int c = 0; c < 16; ++c) v16[c] = i < n ? x[(i + 64 * c) % n] : 0.0;
+  const double s16 = wave_vsum16(v16);
+  double ref = 0.0, m16 = 0.0;
+#pragma unroll
+  for (int c = 0; c < 16; ++c)
+  {
+    const double rc = wave_sum_shfl(v16[c]), mc = wave_sum_shfl(fabs(v16[c]));
+    if ((threadIdx.x & 15) == c)
+    {
+      ref = rc;
+      m16 = mc;
+    }
+  }
+  if (fabs(s16 - ref) > 8.0 * 2.220446049250313e-16 * m16) atomicAdd(bad, 1);
 }
 
 // ---- descriptor-batched helpers (problem = blockIdx.y / blockIdx.z) ----

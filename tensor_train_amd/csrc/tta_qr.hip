@@ -41,13 +41,22 @@ template <int NT, int NV>
 __device__ __forceinline__ void block_vsum(const double (&v)[NV], int lo, int hi, double* red, double* out)
 {
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-  for (int c = 0; c < NV; ++c)
+  if constexpr (NV == 16)
   {
-    if (c >= lo && c < hi)
+    // All 16 at once; lane c holds the wave's sum of v[c].
+    const double s = wave_vsum16(v);
+    if (lane < 16 && lane >= lo && lane < hi) red[wid * NV + lane] = s;
+  }
+  else
+  {
+#pragma unroll
+    for (int c = 0; c < NV; ++c)
     {
-      const double s = wave_sum(v[c]);
-      if (lane == 0) red[wid * NV + c] = s;
+      if (c >= lo && c < hi)
+      {
+        const double s = wave_sum(v[c]);
+        if (lane == 0) red[wid * NV + c] = s;
+      }
     }
   }
   __syncthreads();
