@@ -101,10 +101,13 @@ def test_cholqr_forms_across_condition_numbers():
     # whichever form each factorization takes (the polar second pass for
     # small E, the triangular one for n max|E| between 1e-5 and 1e-4,
     # Householder beyond, the last covered by the graded test above), the
-    # result reproduces the oracle; this sweep reaches both CholQR2 forms.
+    # result reproduces the oracle.  Here the triangular form is taken for
+    # decay exponents in a window about 0.25 wide near 6.4 (measured), so the
+    # sweep steps by 0.125 through 6 .. 7 to reach both CholQR2 forms whatever
+    # the last bits of the factorizations before them.
     ctx = T.Context.default()
     forms = {"triangular": 0, "declined": 0, "polar": 0}
-    for logk in (2.0, 4.0, 4.5, 5.0, 5.5, 6.0, 7.0):
+    for logk in (2.0, 4.0, 4.5, 5.0, 5.5, 6.0, 6.125, 6.25, 6.375, 6.5, 6.625, 6.75, 6.875, 7.0):
         x = O.OTT.random([4] * 6, 64, 17)
         O.left_orthogonalize(x)
         cs = x.cores()
