@@ -278,7 +278,7 @@ def main():
     # Cholesky, the triangular inverse); the roofline is quoted for the one
     # with the most device time.
     single = {"jacobi_round": ("bj_round_kernel", ctx.prof_read("jacobi_round")),
-              "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_kernel", stats["apply"]),
+              "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_batch_kernel", stats["apply"]),
               "gemm_splitk": ("gemm_f64_splitk_kernel", stats["gemm_splitk"]),
               "gemm_multi": ("gemm_f64_multi_kernel", stats["gemm_multi"]),
               "chol_step": ("chol_step_kernel", stats["chol_step"]),

@@ -1237,16 +1237,26 @@ void balance(tta_ctx* c, tta_tt* t)
     total += (size_t)t->ranks[k + 1];
   }
   double* nrm = ctx_ws(c, WS_JP, total + 1);
-  for (int k = 0; k < d - 1; ++k)
-    launch_col_norms(t->cores[k], t->ranks[k] * t->modes[k], t->ranks[k + 1], nrm + off[k], c->stream);
+  // Core k's right column norms at off[k] (none for the last core), its
+  // left ones at off[k - 1] (none for the first); kCoreBatch cores a launch.
+  auto batches = [&](auto&& launch)
+  {
+    core_batch b;
+    for (int k = 0; k < d; ++k)
+    {
+      b.it[b.count++] = {t->cores[k], t->ranks[k], t->modes[k], t->ranks[k + 1], k < d - 1 ? (int)off[k] : -1,
+                         k > 0 ? (int)off[k - 1] : -1};
+      if (b.count == kCoreBatch || k + 1 == d)
+      {
+        launch(b);
+        b.count = 0;
+      }
+    }
+  };
+  batches([&](const core_batch& b) { launch_core_norms_batch(b, nrm, c->stream); });
   double* geo = small_dbl(c) + 2;
   launch_geo_mean(nrm, (int)total, geo, c->stream);
-  for (int k = 0; k < d; ++k)
-  {
-    const double* nl = k > 0 ? nrm + off[k - 1] : nullptr;
-    const double* nr = k < d - 1 ? nrm + off[k] : nullptr;
-    launch_balance(t->cores[k], t->ranks[k], t->modes[k], t->ranks[k + 1], nl, nr, geo, c->stream);
-  }
+  batches([&](const core_batch& b) { launch_balance_batch(b, nrm, geo, c->stream); });
 }
 
 // Copies of a TT's cores and its ranks, to undo a speculative sweep.
@@ -1371,9 +1381,17 @@ tta_tt* matvec_impl(tta_ctx* c, const tta_tt* A, const tta_tt* x)
     by += 8.0 * ((double)core_numel(A, k) + (double)core_numel(x, k) + (double)ranks[k] * modes[k] * ranks[k + 1]);
   }
   prof_scope ps(c, TTA_K_APPLY, fl, by);
+  matvec_batch b;
   for (int k = 0; k < d; ++k)
-    launch_matvec_core(A->cores[k], x->cores[k], y->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
-                       A->ranks[k + 1], x->ranks[k], x->ranks[k + 1], c->stream);
+  {
+    b.it[b.count++] = {A->cores[k], x->cores[k], y->cores[k], A->ranks[k], A->row_modes[k], A->col_modes[k],
+                       A->ranks[k + 1], x->ranks[k], x->ranks[k + 1]};
+    if (b.count == kCoreBatch || k + 1 == d)
+    {
+      launch_matvec_core_batch(b, c->stream);
+      b.count = 0;
+    }
+  }
   return y;
 }
 

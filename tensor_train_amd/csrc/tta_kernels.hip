@@ -612,6 +612,68 @@ __global__ void balance_kernel(double* x, int r, int n, int rp, const double* nl
   }
 }
 
+// Per-core launches folded into one (descriptors by value, blockIdx.y picks
+// the core); each core's work is that of the single-core kernel above.
+__global__ __launch_bounds__(256) void matvec_core_batch_kernel(matvec_batch b)
+{
+  const matvec_batch::item& it = b.it[blockIdx.y];
+  const int r = it.r, m = it.m, n = it.n, rp = it.rp, s = it.s, sp = it.sp;
+  const long total = (long)r * s * m * rp * sp;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    long t = o;
+    const int bx = (int)(t % sp);
+    t /= sp;
+    const int bb = (int)(t % rp);
+    t /= rp;
+    const int i = (int)(t % m);
+    t /= m;
+    const int ax = (int)(t % s);
+    const int a = (int)(t / s);
+    const double* Ar = it.A + (((size_t)a * m + i) * n) * rp + bb;
+    const double* xr = it.x + ((size_t)ax * n) * sp + bx;
+    double acc = 0.0;
+    for (int j = 0; j < n; ++j) acc = fma(Ar[(size_t)j * rp], xr[(size_t)j * sp], acc);
+    it.y[o] = acc;
+  }
+}
+
+// Column norms of each core's left unfolding (r n x rp) into nrm + off.
+__global__ __launch_bounds__(256) void core_norms_batch_kernel(core_batch b, double* nrm)
+{
+  const core_batch::item& it = b.it[blockIdx.y];
+  if (it.off < 0) return;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= it.rp) return;
+  const int rows = it.r * it.n;
+  double acc = 0.0;
+  for (int i = lane; i < rows; i += 64) acc = fma(it.x[(size_t)i * it.rp + c], it.x[(size_t)i * it.rp + c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) nrm[it.off + c] = sqrt(acc);
+}
+
+// balance_kernel for each core: left norms at nrm + offl, right ones at
+// nrm + off (negative: none).
+__global__ __launch_bounds__(256) void balance_batch_kernel(core_batch b, const double* nrm, const double* geo)
+{
+  const core_batch::item& it = b.it[blockIdx.y];
+  const long total = (long)it.r * it.n * it.rp;
+  const double g = *geo;
+  if (g == 0.0) return;
+  const double* nl = it.offl >= 0 ? nrm + it.offl : nullptr;
+  const double* nr = it.off >= 0 ? nrm + it.off : nullptr;
+  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  {
+    const int bb = (int)(o % it.rp);
+    const int a = (int)(o / ((long)it.n * it.rp));
+    double f = 1.0;
+    if (nr && nr[bb] > 0.0) f *= g / nr[bb];
+    if (nl && nl[a] > 0.0) f /= g / nl[a];
+    it.x[o] *= f;
+  }
+}
+
 // dst[i*ldd + j] = f * src[i*lds + j] + keep * dst[i*ldd + j].
 __global__ void copy2d_kernel(const double* src, int lds_, double* dst, int ldd, int rows, int cols, double f,
                               double keep)
@@ -1304,6 +1366,34 @@ void launch_col_norms(const double* M, int rows, int cols, double* out, hipStrea
 void launch_geo_mean(const double* v, int n, double* out, hipStream_t st)
 {
   hipLaunchKernelGGL(geo_mean_kernel, dim3(1), dim3(256), 0, st, v, n, out);
+}
+
+void launch_matvec_core_batch(const matvec_batch& b, hipStream_t st)
+{
+  long most = 0;
+  for (int z = 0; z < b.count; ++z)
+  {
+    const matvec_batch::item& it = b.it[z];
+    most = std::max(most, (long)it.r * it.s * it.m * it.rp * it.sp);
+  }
+  if (b.count <= 0 || most <= 0) return;
+  hipLaunchKernelGGL(matvec_core_batch_kernel, dim3(grid_for(most), b.count), dim3(256), 0, st, b);
+}
+
+void launch_core_norms_batch(const core_batch& b, double* nrm, hipStream_t st)
+{
+  int most = 0;
+  for (int z = 0; z < b.count; ++z) most = std::max(most, b.it[z].rp);
+  if (b.count <= 0 || most <= 0) return;
+  hipLaunchKernelGGL(core_norms_batch_kernel, dim3((most + 3) / 4, b.count), dim3(256), 0, st, b, nrm);
+}
+
+void launch_balance_batch(const core_batch& b, const double* nrm, const double* geo, hipStream_t st)
+{
+  long most = 0;
+  for (int z = 0; z < b.count; ++z) most = std::max(most, (long)b.it[z].r * b.it[z].n * b.it[z].rp);
+  if (b.count <= 0 || most <= 0) return;
+  hipLaunchKernelGGL(balance_batch_kernel, dim3(grid_for(most), b.count), dim3(256), 0, st, b, nrm, geo);
 }
 
 void launch_balance(double* x, int r, int n, int rp, const double* nl, const double* nr, const double* geo,

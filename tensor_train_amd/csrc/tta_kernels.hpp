@@ -349,6 +349,38 @@ struct copy_batch
   int count = 0;
 };
 void launch_copy_batch(const copy_batch& b, hipStream_t st);
+// Up to kCoreBatch per-core launches of one TT folded into one (descriptors
+// by value): the MPO core contraction of launch_matvec_core, and the column
+// norms and balancing of launch_col_norms / launch_balance.
+constexpr int kCoreBatch = 32;
+struct matvec_batch
+{
+  struct item
+  {
+    const double* A;
+    const double* x;
+    double* y;
+    int r, m, n, rp, s, sp;
+  };
+  item it[kCoreBatch];
+  int count = 0;
+};
+void launch_matvec_core_batch(const matvec_batch& b, hipStream_t st);
+// Core x (r, n, rp); off / offl: offsets of its right / left column norms in
+// the norm array (negative: none; the norm kernel writes at off).
+struct core_batch
+{
+  struct item
+  {
+    double* x;
+    int r, n, rp;
+    int off, offl;
+  };
+  item it[kCoreBatch];
+  int count = 0;
+};
+void launch_core_norms_batch(const core_batch& b, double* nrm, hipStream_t st);
+void launch_balance_batch(const core_batch& b, const double* nrm, const double* geo, hipStream_t st);
 // dst (n x n, leading dimension ld) = identity.
 void launch_eye(double* dst, int n, long ld, hipStream_t st);
 // G <- G - I (row-major n x n); max |G - I| over the entries, as
