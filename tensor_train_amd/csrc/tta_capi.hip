@@ -266,9 +266,20 @@ gemm_desc gdesc(int M, int N, int K, const double* A, long sar, long sac, const 
 
 // Independent GEMMs (empty ones, M or N or K = 0, skipped) in one launch
 // on the context stream; a single one goes through gemm_go.
-void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
+// With `decide` (its dinfo .. dflag set), the speculative CholQR2
+// acceptance decision rides along in one more workgroup of the launch.
+void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs, const gemm_multi* decide = nullptr)
 {
   gemm_multi m;
+  if (decide)
+  {
+    m.dinfo = decide->dinfo;
+    m.dpart = decide->dpart;
+    m.dnp = decide->dnp;
+    m.dn = decide->dn;
+    m.dthr = decide->dthr;
+    m.dflag = decide->dflag;
+  }
   double fl = 0.0, by = 0.0;
   for (const gemm_desc& g : gs)
   {
@@ -278,8 +289,8 @@ void gemm_multi_go(tta_ctx* c, std::initializer_list<gemm_desc> gs)
     fl += 2.0 * g.M * g.N * g.K;
     by += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
   }
-  if (m.count == 0) return;
-  if (m.count == 1)
+  if (m.count == 0 && !decide) return;
+  if (m.count == 1 && !decide)
   {
     gemm_go(c, m.g[0]);
     return;
@@ -467,21 +478,30 @@ bool cholqr2_strided(tta_ctx* c, const double* A, long sar, long sac, int m, int
       ew([&] { launch_polar_series(G, Y, X, W, n, c->stream); });    // X = P, W = Pi
     }
     // Q = Q1 P, Rf = Pi R1 and (for the certificate) Y = R^-1 = R1^-1 P.
-    gemm_multi_go(c, {gdesc(Q ? m : 0, n, n, Q1, 1, m, X, n, 1, Q, sqr, sqc), gdesc(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1),
-                      gdesc(cert ? n : 0, n, n, Xi, n, 1, X, n, 1, Y, n, 1)});
+    // In a speculative sweep without a certificate, the acceptance decision
+    // rides on this launch.
+    gemm_multi dec;
+    dec.dinfo = aux;
+    dec.dpart = aux + 8;
+    dec.dnp = np;
+    dec.dn = n;
+    dec.dthr = 1.0e-5;
+    dec.dflag = spec_flag(c);
+    gemm_multi_go(c,
+                  {gdesc(Q ? m : 0, n, n, Q1, 1, m, X, n, 1, Q, sqr, sqc), gdesc(n, n, n, W, n, 1, R1, n, 1, Rf, n, 1),
+                   gdesc(cert ? n : 0, n, n, Xi, n, 1, X, n, 1, Y, n, 1)},
+                  c->spec && !cert ? &dec : nullptr);
     if (c->spec)
     {
       // Decided on the device; a failed acceptance undoes the whole sweep.
-      // With a certificate, the column norms of R and R^-1 ride on the
-      // same launch.
+      // With a certificate, the decision and the column norms of R and
+      // R^-1 share one launch after the products (which the norms read).
       if (cert)
       {
         double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
         ew([&] { launch_spec_cholqr_norms(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), Rf, Y, cd, c->stream); });
         cert->dev = cd;
       }
-      else
-        ew([&] { launch_spec_cholqr(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), c->stream); });
       release();
       c->cholqr_tried += 1;
       c->cholqr_used += 1;

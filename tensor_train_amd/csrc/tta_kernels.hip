@@ -270,6 +270,19 @@ __global__ __launch_bounds__(256) void splitk_reduce_epi_kernel(gemm_desc g, int
 // Independent GEMMs of one launch, descriptors passed by value.
 __global__ __launch_bounds__(256) void gemm_f64_multi_kernel(gemm_multi m)
 {
+  if ((int)blockIdx.z == m.count)
+  {
+    // The speculative CholQR2 acceptance decision riding along.
+    if (blockIdx.x != 0 || blockIdx.y != 0 || threadIdx.x >= 64) return;
+    double emax = 0.0;
+    for (int b = threadIdx.x; b < m.dnp; b += 64) emax = fmax(emax, m.dpart[b]);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    if (threadIdx.x != 0) return;
+    const bool ok = m.dinfo[0] == 0.0 && isfinite(m.dinfo[1]) && isfinite(m.dinfo[2]) && emax * m.dn <= m.dthr;
+    if (!ok) m.dflag[0] = 0.0;
+    return;
+  }
   const gemm_desc& g = m.g[blockIdx.z];
   if ((int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
   gemm_tile(g, g.A, g.B, g.C);
@@ -974,23 +987,12 @@ __global__ __launch_bounds__(256) void minus_ident_kernel(double* G, int n, doub
 
 // Device-side decisions of a speculative sweep: flag[0] starts at 1 and is
 // cleared by the first factorization or certificate that does not hold.
-// CholQR2 acceptance: no breakdown in pass 1 (info[0] == 0, finite extreme
-// diagonal entries) and n max|E| <= thr over the partial maxima.
-__global__ __launch_bounds__(64) void spec_cholqr_kernel(const double* info, const double* part, int np, int n,
-                                                         double thr, double* flag)
-{
-  double emax = 0.0;
-  for (int b = threadIdx.x; b < np; b += 64) emax = fmax(emax, part[b]);
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
-  if (threadIdx.x != 0) return;
-  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * n <= thr;
-  if (!ok) flag[0] = 0.0;
-}
-
-// spec_cholqr_kernel's decision in block 0 (its first wave) and, in blocks
-// 1 .., the column norms of the n x n row-major A and B into out[0, n) and
-// out[n, 2n) (col_norms_kernel's work, four columns per block).
+// CholQR2 acceptance (here in block 0's first wave, or in one extra
+// workgroup of gemm_f64_multi_kernel): no breakdown in pass 1 (info[0] == 0,
+// finite extreme diagonal entries) and n max|E| <= thr over the np partial
+// maxima.  Blocks 1 .. of this kernel compute the column norms of the n x n
+// row-major A and B into out[0, n) and out[n, 2n) (col_norms_kernel's work,
+// four columns per block).
 __global__ __launch_bounds__(256) void spec_cholqr_norms_kernel(const double* info, const double* part, int np, int n,
                                                                double thr, double* flag, const double* A,
                                                                const double* B, double* out)
@@ -1151,8 +1153,9 @@ void launch_gemm_multi(const gemm_multi& m, hipStream_t st)
     mt = std::max(mt, (m.g[i].M + GT - 1) / GT);
     nt = std::max(nt, (m.g[i].N + GT - 1) / GT);
   }
-  if (m.count <= 0 || mt <= 0 || nt <= 0) return;
-  hipLaunchKernelGGL(gemm_f64_multi_kernel, dim3(nt, mt, m.count), dim3(256), 0, st, m);
+  if (!m.dflag && (m.count <= 0 || mt <= 0 || nt <= 0)) return;
+  hipLaunchKernelGGL(gemm_f64_multi_kernel, dim3(std::max(nt, 1), std::max(mt, 1), m.count + (m.dflag ? 1 : 0)),
+                     dim3(256), 0, st, m);
 }
 
 int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st)
@@ -1266,12 +1269,6 @@ void launch_ident_axpy(const double* X, double* Y, int n, double alpha, double b
   long g = ((long)n * n + 255) / 256;
   if (g > 1024) g = 1024;
   hipLaunchKernelGGL(ident_axpy_kernel, dim3((int)(g < 1 ? 1 : g)), dim3(256), 0, st, X, Y, n, alpha, beta);
-}
-
-void launch_spec_cholqr(const double* info, const double* part, int np, int n, double thr, double* flag,
-                        hipStream_t st)
-{
-  hipLaunchKernelGGL(spec_cholqr_kernel, dim3(1), dim3(64), 0, st, info, part, np, n, thr, flag);
 }
 
 void launch_spec_cholqr_norms(const double* info, const double* part, int np, int n, double thr, double* flag,

@@ -219,6 +219,13 @@ struct gemm_multi
 {
   gemm_desc g[4];
   int count = 0;
+  // Optional: the speculative CholQR2 acceptance decision in one more
+  // workgroup of the launch (blockIdx.z == count).
+  const double* dinfo = nullptr;
+  const double* dpart = nullptr;
+  int dnp = 0, dn = 0;
+  double dthr = 0.0;
+  double* dflag = nullptr;
 };
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st);
 // One block step of the blocked Cholesky with inverse of the n x n
@@ -321,15 +328,14 @@ constexpr int kRinvCertMaxQ = 1024;
 int rinv_cert_blocks(int q);
 void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, double* X = nullptr, int ldx = 0);
 // Decisions of a speculative sweep on the device (flag[0] cleared on the
-// first failure): CholQR2 acceptance from its info[0..2] and the np partial
-// maxima of |E| (accepted when n max|E| <= thr); the full-rank certificate from the column norms of R and
+// first failure): the full-rank certificate from the column norms of R and
 // R^-1 (mode 0, a = 2n values) or the n (xf2, rf2) pairs of
-// launch_rinv_cert (mode 1).
-void launch_spec_cholqr(const double* info, const double* part, int np, int n, double thr, double* flag,
-                        hipStream_t st);
+// launch_rinv_cert (mode 1).  CholQR2 acceptance (from its info[0..2] and
+// the np partial maxima of |E|, accepted when n max|E| <= thr) rides on
+// launch_spec_cholqr_norms or on a gemm_multi launch (its dinfo .. dflag).
 void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
-// launch_spec_cholqr and the column norms of the n x n row-major A and B
-// (into out[0, n) and out[n, 2n)) in one launch.
+// The CholQR2 acceptance decision and the column norms of the n x n
+// row-major A and B (into out[0, n) and out[n, 2n)) in one launch.
 void launch_spec_cholqr_norms(const double* info, const double* part, int np, int n, double thr, double* flag,
                               const double* A, const double* B, double* out, hipStream_t st);
 // p[0] = v.
