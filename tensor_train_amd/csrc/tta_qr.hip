@@ -229,6 +229,8 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
 // C <- (I - V op(T) V^T) C for columns [cbeg, cend) of C, rows j0 .. m-1.
 // mode 0: factorisation (C = A, op(T) = T^T, cbeg = j0 + bw, cend = n).
 // mode 1: Q formation (C = Q, op(T) = T, cbeg = j0, cend = k).
+// mode 2: as mode 1 with C taken to be the identity (Q is only written; the
+//         sums are the same as mode 1's after an identity fill).
 __global__ __launch_bounds__(256) void qr_update_kernel(qr_batch bt, int j0, int mode)
 {
   const qrp& P = pick(bt);
@@ -280,7 +282,7 @@ __global__ __launch_bounds__(256) void qr_update_kernel(qr_batch bt, int j0, int
       {
         const int r = r0 + rr;
         if (r >= len) break;
-        const double a = col[r];
+        const double a = mode == 2 ? (j0 + r == c ? 1.0 : 0.0) : col[r];
 #pragma unroll
         for (int jj = 0; jj < QB; ++jj) acc[u][jj] = fma(Vs[rr][jj], a, acc[u][jj]);
       }
@@ -337,7 +339,10 @@ __global__ __launch_bounds__(256) void qr_update_kernel(qr_batch bt, int j0, int
         double s = 0.0;
 #pragma unroll
         for (int jj = 0; jj < QB; ++jj) s = fma(Vs[rr][jj], W2[jj][cc], s);
-        col[r] -= s;
+        if (mode == 2)
+          col[r] = (j0 + r == c ? 1.0 : 0.0) - s;
+        else
+          col[r] -= s;
       }
     }
   }
@@ -528,6 +533,13 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
   }
   if (b.descs ? b.form_q : b.one.Q != nullptr)
   {
+    if (!mfma_update && kmax <= QB)
+    {
+      // One panel: its update reads C as the identity (mode 2), so Q needs
+      // no separate fill.
+      hipLaunchKernelGGL(qr_update_kernel, dim3((kmax + QCT - 1) / QCT, batch), dim3(256), 0, st, b, 0, 2);
+      return;
+    }
     const long total = (long)b.max_m * kmax;
     long g = (total + 255) / 256;
     if (g > 4096) g = 4096;
