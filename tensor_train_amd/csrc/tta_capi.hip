@@ -498,7 +498,9 @@ bool cholqr2_strided(tta_ctx* c, const double* A, long sar, long sac, int m, int
       // R^-1 share one launch after the products (which the norms read).
       if (cert)
       {
-        double* cd = ctx_ws(c, WS_CERT, 2 * (size_t)n);
+        // Kept until the sweep's certificates are decided together.
+        double* cd = alloc_core(c, 2 * (size_t)n);
+        c->spec_cert_bufs.push_back(cd);
         ew([&] { launch_spec_cholqr_norms(aux, aux + 8, np, n, 1.0e-5, spec_flag(c), Rf, Y, cd, c->stream); });
         cert->dev = cd;
       }
@@ -850,19 +852,21 @@ bool certify_full_rank(tta_ctx* c, const double* R, int q, double delta, const r
   c->cert_tried += 1;
   if (c->spec)
   {
-    // Decided on the device into the sweep's flag; the step goes on as
-    // certified.
+    // Decided on the device into the sweep's flag, together with the
+    // sweep's other certificates before the flag is read (run_sweep); the
+    // step goes on as certified.
     if (known && known->dev)
-      launch_spec_cert(known->dev, q, 0, delta, spec_flag(c), c->stream);
+      c->spec_certs.push_back({known->dev, q, 0, delta});
     else
     {
       const int nb = rinv_cert_blocks(q);
-      double* part = ctx_ws(c, WS_CERT, 2 * (size_t)nb);
+      double* part = alloc_core(c, 2 * (size_t)nb);
+      c->spec_cert_bufs.push_back(part);
       {
         prof_scope ps(c, TTA_K_RINV, (double)q * q * q / 3.0, 16.0 * (double)q * q);
         launch_rinv_cert(R, q, part, c->stream);
       }
-      launch_spec_cert(part, nb, 1, delta, spec_flag(c), c->stream);
+      c->spec_certs.push_back({part, nb, 1, delta});
     }
     c->cert_held += 1;
     return true;
@@ -1316,6 +1320,36 @@ tt_snapshot take_snapshot(tta_ctx* c, const tta_tt* t)
 // snapshot and run again with host decisions, so the result is always that
 // of the checked sweep.  Otherwise the checked sweep runs directly, and
 // speculation resumes once such a sweep has needed no fallback.
+// The recorded certificate decisions of a speculative sweep, kCertBatch per
+// launch, then their input buffers back to the pool (stream-ordered).
+void flush_spec_certs(tta_ctx* c)
+{
+  cert_batch b;
+  {
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 16.0 * (double)c->spec_certs.size());
+    for (size_t i = 0; i < c->spec_certs.size(); ++i)
+    {
+      const tta_ctx::spec_cert_rec& r = c->spec_certs[i];
+      b.it[b.count++] = {r.a, r.n, r.mode, r.delta};
+      if (b.count == kCertBatch || i + 1 == c->spec_certs.size())
+      {
+        launch_spec_cert_batch(b, spec_flag(c), c->stream);
+        b.count = 0;
+      }
+    }
+  }
+  c->spec_certs.clear();
+  for (void* p : c->spec_cert_bufs) c->mem.put(p);
+  c->spec_cert_bufs.clear();
+}
+
+void drop_spec_certs(tta_ctx* c)
+{
+  c->spec_certs.clear();
+  for (void* p : c->spec_cert_bufs) c->mem.put(p);
+  c->spec_cert_bufs.clear();
+}
+
 template <class F>
 void run_sweep(tta_ctx* c, tta_tt* t, bool& last_ok, F&& sweep)
 {
@@ -1332,10 +1366,12 @@ void run_sweep(tta_ctx* c, tta_tt* t, bool& last_ok, F&& sweep)
   try
   {
     sweep();
+    flush_spec_certs(c);
   }
   catch (...)
   {
     c->spec = false;
+    drop_spec_certs(c);
     for (double* p : snap.cores) c->mem.put(p);
     throw;
   }

@@ -123,6 +123,17 @@ struct tta_ctx
   bool speculate = true, spec = false;
   bool spec_qr_ok = true, spec_svd_ok = true;
   long long spec_sweeps = 0, spec_failed = 0;
+  // Certificate decisions of the running speculative sweep, made together
+  // before its flag is read: their inputs (n values in mode 1, 2n in mode
+  // 0), and the pool buffers that hold those inputs until then.
+  struct spec_cert_rec
+  {
+    const double* a;
+    int n, mode;
+    double delta;
+  };
+  std::vector<spec_cert_rec> spec_certs;
+  std::vector<void*> spec_cert_bufs;
 };
 
 // A device TT or TT-matrix.  Core k holds (ranks[k], modes[k], ranks[k+1])

@@ -1024,7 +1024,7 @@ __global__ __launch_bounds__(256) void spec_cholqr_norms_kernel(const double* in
 // column norms of R (a[0..n)) and of R^-1 (a[n..2n)), mode 1 the n (xf2,
 // rf2) pairs of rinv_cert_kernel.  Same test as the host's: both sums finite
 // and positive, ||R|| ||R^-1|| <= 1e8, 1/(||R^-1|| (1+1e-3)) > delta (1+1e-3).
-__global__ __launch_bounds__(64) void spec_cert_kernel(const double* a, int n, int mode, double delta, double* flag)
+__device__ __forceinline__ void spec_cert_decide(const double* a, int n, int mode, double delta, double* flag)
 {
   const int lane = threadIdx.x;
   double x = 0.0, r = 0.0;
@@ -1051,6 +1051,14 @@ __global__ __launch_bounds__(64) void spec_cert_kernel(const double* a, int n, i
     held = rf * xf <= 1.0e8 && 1.0 / (xf * (1.0 + 1.0e-3)) > delta * (1.0 + 1.0e-3);
   }
   if (!held) flag[0] = 0.0;
+}
+
+// The certificates of a speculative sweep, decided together at its end: block
+// z decides item z (any failure clears the flag).
+__global__ __launch_bounds__(64) void spec_cert_batch_kernel(cert_batch b, double* flag)
+{
+  const cert_batch::item& it = b.it[blockIdx.x];
+  spec_cert_decide(it.a, it.n, it.mode, it.delta, flag);
 }
 
 __global__ void set_value_kernel(double* p, double v)
@@ -1278,9 +1286,10 @@ void launch_spec_cholqr_norms(const double* info, const double* part, int np, in
                      flag, A, B, out);
 }
 
-void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st)
+void launch_spec_cert_batch(const cert_batch& b, double* flag, hipStream_t st)
 {
-  hipLaunchKernelGGL(spec_cert_kernel, dim3(1), dim3(64), 0, st, a, n, mode, delta, flag);
+  if (b.count <= 0) return;
+  hipLaunchKernelGGL(spec_cert_batch_kernel, dim3(b.count), dim3(64), 0, st, b, flag);
 }
 
 // Copy z of the batch: workgroups of blockIdx.y = z stride over its n

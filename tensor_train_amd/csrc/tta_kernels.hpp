@@ -333,7 +333,21 @@ void launch_rinv_cert(const double* R, int q, double* out, hipStream_t st, doubl
 // launch_rinv_cert (mode 1).  CholQR2 acceptance (from its info[0..2] and
 // the np partial maxima of |E|, accepted when n max|E| <= thr) rides on
 // launch_spec_cholqr_norms or on a gemm_multi launch (its dinfo .. dflag).
-void launch_spec_cert(const double* a, int n, int mode, double delta, double* flag, hipStream_t st);
+// Up to kCertBatch certificate decisions (inputs a, n, mode, delta as
+// described above) in one launch.
+constexpr int kCertBatch = 32;
+struct cert_batch
+{
+  struct item
+  {
+    const double* a;
+    int n, mode;
+    double delta;
+  };
+  item it[kCertBatch];
+  int count = 0;
+};
+void launch_spec_cert_batch(const cert_batch& b, double* flag, hipStream_t st);
 // The CholQR2 acceptance decision and the column norms of the n x n
 // row-major A and B (into out[0, n) and out[n, 2n)) in one launch.
 void launch_spec_cholqr_norms(const double* info, const double* part, int np, int n, double thr, double* flag,
