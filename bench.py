@@ -229,8 +229,28 @@ def batched_section(T, ctx, args):
     return out
 
 
+def copy_calibration(mib: int) -> None:
+    """Copies a float64 device buffer of `mib` MiB into another five times
+    with torch's elementwise copy: each dispatch reads and writes exactly
+    `mib` MiB.  The per-dispatch FETCH_SIZE / WRITE_SIZE that rocprofv3
+    reports for it, divided by mib * 1024 KiB, is the factor by which those
+    counters over- or under-count HBM bytes on this device."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB x 5", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run the copy kernel of known size used to calibrate rocprofv3's "
+                         "FETCH_SIZE / WRITE_SIZE counters, then exit")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
@@ -246,6 +266,9 @@ def main():
                     help="steps timed again with the full-rank certificate, CholQR2 and speculative sweeps "
                          "off (every QR is Householder and every SVD step runs its SVD)")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
               "independent TTs across GPUs, which this library does not do.", file=sys.stderr)

@@ -319,6 +319,12 @@ int tta_prof_read(tta_ctx* ctx, int kernel_class, long long* launches, double* t
 /* Block-Jacobi SVD calls (matrices above 64 columns) since the last
  * tta_prof_enable and the sweeps they ran. */
 int tta_prof_jacobi_sweeps(tta_ctx* ctx, long long* calls, long long* sweeps);
+/* Block-Jacobi problems still rotating after the sweep limit (30 block
+ * sweeps) since the last tta_prof_enable: their singular vectors are
+ * orthogonal to working precision, but pairs of columns may remain above
+ * the rotation tolerance.  No reference counterpart (Eigen's BDCSVD has no
+ * sweep limit to report). */
+int tta_prof_jacobi_unconverged(tta_ctx* ctx, long long* problems);
 
 /* ---- full-rank certificate of the SVD steps (no reference counterpart) ---- */
 /* Before the Jacobi SVD of an SVD step of round / round_right_canonical, the

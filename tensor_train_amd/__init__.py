@@ -89,6 +89,13 @@ class Context:
         self.check(self.lib.tta_prof_jacobi_sweeps(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def jacobi_unconverged(self) -> int:
+        """Block-Jacobi problems still rotating at the sweep limit since the
+        last prof_enable (0 when every SVD converged)."""
+        a = ctypes.c_longlong()
+        self.check(self.lib.tta_prof_jacobi_unconverged(self.h, ctypes.byref(a)))
+        return a.value
+
     def set_rank_certificate(self, on: bool) -> None:
         """Full-rank certificate of the SVD steps (on by default): a step whose
         triangular factor provably keeps every singular value skips its SVD."""

@@ -606,8 +606,7 @@ bool cholqr_applies(const tta_ctx* c, int m, int n) { return c->cholqr && m >= n
 // QR of a column-major (m x n) matrix in place (R in the leading rows),
 // thin Q (m x k) into Q when non-null.  With cholqr, matrices with m >= n
 // and 32 <= n <= chol_max_n() go through CholQR2 first; otherwise, and
-// whenever CholQR2 declines, Householder: blocked compact-WY kernels for
-// m <= 2048, the single-workgroup unblocked kernel beyond.
+// whenever CholQR2 declines, blocked compact-WY Householder (any m).
 // Returns true when CholQR2 made the factorization: R then fills A's
 // leading n x n block and need not be triangular (read it with
 // launch_extract_upper(..., full = true)).  Householder leaves R in the upper
@@ -624,32 +623,19 @@ bool qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
   }
   if (cholqr && cholqr_applies(c, m, n) && cholqr2(c, A, lda, m, n, Q, ldq, cert)) return true;
   prof_scope ps(c, TTA_K_QR, qr_flops(m, n), 8.0 * ((double)m * n + (double)m * std::min(m, n)));
-  if (m <= qr_blocked_max_rows())
-  {
-    const int k = std::min(m, n);
-    qr_batch b;
-    b.one.A = A;
-    b.one.lda = lda;
-    b.one.m = m;
-    b.one.n = n;
-    b.one.tau = tau;
-    b.one.T = ctx_ws(c, WS_T, (size_t)((k + 15) / 16) * 256 + 1);
-    b.one.Q = Q;
-    b.one.ldq = ldq;
-    b.max_m = m;
-    b.max_n = n;
-    launch_qr_blocked(b, c->stream);
-    return false;
-  }
-  qr_desc q;
-  q.A = A;
-  q.lda = lda;
-  q.m = m;
-  q.n = n;
-  q.tau = tau;
-  q.Q = Q;
-  q.ldq = ldq;
-  launch_qr(q, 1, c->stream);
+  const int k = std::min(m, n);
+  qr_batch b;
+  b.one.A = A;
+  b.one.lda = lda;
+  b.one.m = m;
+  b.one.n = n;
+  b.one.tau = tau;
+  b.one.T = ctx_ws(c, WS_T, (size_t)((k + 15) / 16) * 256 + 1);
+  b.one.Q = Q;
+  b.one.ldq = ldq;
+  b.max_m = m;
+  b.max_n = n;
+  launch_qr_blocked(b, c->stream);
   return false;
 }
 
@@ -2957,6 +2943,7 @@ int tta_prof_enable(tta_ctx* c, int on)
   }
   c->jac_calls = 0;
   c->jac_sweeps = 0;
+  c->jac_unconverged = 0;
   c->cert_tried = 0;
   c->cert_held = 0;
   c->cholqr_tried = 0;
@@ -3025,6 +3012,13 @@ int tta_prof_jacobi_sweeps(tta_ctx* c, long long* calls, long long* sweeps)
   if (!c) return TTA_ERR_ARG;
   if (calls) *calls = c->jac_calls;
   if (sweeps) *sweeps = c->jac_sweeps;
+  return TTA_OK;
+}
+
+int tta_prof_jacobi_unconverged(tta_ctx* c, long long* problems)
+{
+  if (!c || !problems) return TTA_ERR_ARG;
+  *problems = c->jac_unconverged;
   return TTA_OK;
 }
 

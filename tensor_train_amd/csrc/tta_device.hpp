@@ -346,14 +346,19 @@ __device__ void column_norms_sorted(const double* __restrict__ W, int ldw, int p
     if (lane == 0) nrm[c] = sqrt(acc);
   }
   __syncthreads();
+  // A total order, so perm is always a permutation of 0..q-1 and every s[o]
+  // is written: norms descending, a NaN norm (from a breakdown upstream)
+  // ordered after every number (key -1; norms are >= 0), ties by index.
   for (int c = tid; c < q; c += NT)
   {
     const double v = nrm[c];
+    const double kv = (v == v) ? v : -1.0;
     int rank = 0;
     for (int o = 0; o < q; ++o)
     {
       const double u = nrm[o];
-      rank += (u > v) || (u == v && o < c);
+      const double ku = (u == u) ? u : -1.0;
+      rank += (ku > kv) || (ku == kv && o < c);
     }
     perm[rank] = c;
     s[rank] = v;

@@ -105,6 +105,8 @@ struct tta_ctx
   // Block-Jacobi launches (q > 64) since the last tta_prof_enable and the
   // block sweeps they ran (host-driven, so counted whether or not timing is on).
   long long jac_calls = 0, jac_sweeps = 0;
+  // Block-Jacobi problems that had not converged after the sweep limit.
+  long long jac_unconverged = 0;
   // Full-rank certificate of the SVD steps (on by default): steps where it
   // was tried, and where it held so that the Jacobi SVD was not needed.
   bool rank_certificate = true;

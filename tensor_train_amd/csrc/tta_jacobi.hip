@@ -521,6 +521,8 @@ __global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
 
 int jacobi_block_threshold() { return 64; }
 
+constexpr int kMaxBlockSweeps = 30;
+
 int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, tta_ctx* c, double credit_flops,
                       double credit_bytes)
 {
@@ -550,8 +552,11 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     }
     if ((size_t)32 * 8 * max_q > (size_t)max_lds || max_q > 640)
     {
-      // Too tall to stage a block pair: single-workgroup Jacobi in global memory.
-      for (int i = 0; i < batch && !b.descs; ++i)
+      // Too tall to stage a block pair: single-workgroup Jacobi in global
+      // memory, which serves one problem at a time.
+      if (b.descs)
+        fail(TTA_ERR_UNSUPPORTED, "batched Jacobi SVD: %d columns exceed the staged block-pair limit", max_q);
+      for (int i = 0; i < batch; ++i)
       {
         jacobi_desc jd;
         jd.X = b.one.X;
@@ -589,7 +594,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
     }
     const size_t rec0 = c->prof_pending.size();
-    for (int sweep = 0; sweep < 30; ++sweep)
+    for (int sweep = 0; sweep < kMaxBlockSweeps; ++sweep)
     {
       sweeps_run = sweep + 1;
       for (int s = 0; s < nbb - 1; ++s)
@@ -613,11 +618,15 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       }
       (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
       hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
-      if (sweep >= 2)
+      if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
       {
         (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);
         (void)hipStreamSynchronize(st);
         if (*done_host >= batch) break;
+        // Problems still rotating after the last sweep: their W = X J is
+        // orthogonal only to the last sweep's rotation size, not to the
+        // tolerance; counted for tta_prof_jacobi_unconverged.
+        if (sweep == kMaxBlockSweeps - 1) c->jac_unconverged += batch - *done_host;
       }
     }
     // Equal share of the call's credit for every round launch recorded.

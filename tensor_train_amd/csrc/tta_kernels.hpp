@@ -301,7 +301,6 @@ int chol_max_n();
 int potrf_block();
 void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, int ldx, int jb, int j0, double* info,
                       hipStream_t st);
-int qr_blocked_max_rows();
 // done_dev: one device int of scratch; done_host: one pinned host int.
 // Returns the number of block sweeps the host ran (q > 64), or -1 when the
 // sweeps ran inside one kernel.

@@ -226,6 +226,136 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
   for (int e = tid; e < QB * QB; e += NT) Tout[e] = Tl[e / QB][e % QB];
 }
 
+// The same panel factorization for panels of any height: the panel stays in
+// place in global memory (in L2 for the heights this serves) and each
+// column costs two passes over the rows a thread owns (tid, tid + NT, ...):
+//   pass A  -- v_jj^T x_c for c > jj (update) and v_c^T v_jj for c < jj (Gram),
+//   pass B  -- x_c -= tau v_jj (v_jj^T x_c) for c > jj, column jj <- (beta, v_jj),
+//              and the tail norm and diagonal entry of column jj + 1 from the
+//              updated values (so the next column needs no pass of its own).
+// The reflector formulas are those of qr_panel_kernel; only the summation
+// order differs.
+template <int NT>
+__global__ __launch_bounds__(NT) void qr_panel_tall_kernel(qr_batch bt, int j0)
+{
+  const qrp& P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
+  const int m = P.m, n = P.n;
+  const int k = m < n ? m : n;
+  if (j0 >= k) return;
+  const int bw = (k - j0) < QB ? (k - j0) : QB;
+  const int len = m - j0;
+  const int lda = P.lda;
+  double* A = P.A + (size_t)j0 * lda + j0;  // panel origin: A(j0, j0)
+  __shared__ double red[(NT / 64) * QB];
+  __shared__ double bc[QB];
+  __shared__ double Gm[QB][QB + 1];
+  __shared__ double Tl[QB][QB];
+  __shared__ double taus[QB];
+  __shared__ double sc[2];
+  const int tid = threadIdx.x;
+  if (tid < QB) taus[tid] = 0.0;
+  // Tail norm and diagonal entry of column 0.
+  {
+    double acc[1] = {0.0};
+    for (int r = tid; r < len; r += NT)
+    {
+      const double v = A[r];
+      if (r > 0) acc[0] += v * v;
+      else sc[1] = v;
+    }
+    block_vsum<NT, 1>(acc, 0, 1, red, sc);
+  }
+  for (int jj = 0; jj < bw; ++jj)
+  {
+    const double tail = sc[0];
+    const double alpha = sc[1];
+    double tj = 0.0, scal = 0.0, beta = alpha;
+    if (tail > DBL_MIN)
+    {
+      beta = sqrt(alpha * alpha + tail);
+      if (alpha >= 0.0) beta = -beta;
+      tj = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+    if (tid == 0) taus[jj] = tj;
+    const double* xj = A + (size_t)jj * lda;
+    // Pass A.
+    double part[QB];
+#pragma unroll
+    for (int c = 0; c < QB; ++c) part[c] = 0.0;
+    for (int r = jj + tid; r < len; r += NT)
+    {
+      const double v = (r == jj) ? 1.0 : xj[r] * scal;
+#pragma unroll
+      for (int c = 0; c < QB; ++c)
+      {
+        if (c >= bw || c == jj) continue;
+        const double* xc = A + (size_t)c * lda;
+        // Column c < jj holds v_c below its diagonal (implicit 1 on it); rows
+        // r >= jj > c are all below c's diagonal.
+        part[c] += v * xc[r];
+      }
+    }
+    if (bw > 1) block_vsum<NT, QB>(part, 0, bw, red, bc);
+    if (tid < jj) Gm[tid][jj] = bc[tid];
+    // Pass B.
+    double acc[1] = {0.0};
+    const int nx = jj + 1;
+    for (int r = jj + tid; r < len; r += NT)
+    {
+      const double xv = xj[r];
+      const double v = (r == jj) ? 1.0 : xv * scal;
+#pragma unroll
+      for (int c = 0; c < QB; ++c)
+      {
+        if (c <= jj || c >= bw) continue;
+        double* xc = A + (size_t)c * lda;
+        const double u = xc[r] - tj * bc[c] * v;
+        xc[r] = u;
+        if (c == nx)
+        {
+          if (r > nx) acc[0] += u * u;
+          else if (r == nx) sc[1] = u;
+        }
+      }
+      A[(size_t)jj * lda + r] = (r == jj) ? beta : v;
+    }
+    __syncthreads();  // sc[1] written above is read after the sum
+    if (nx < bw) block_vsum<NT, 1>(acc, 0, 1, red, sc);
+    else __syncthreads();
+  }
+  if (tid < bw) P.tau[j0 + tid] = taus[tid];
+  __syncthreads();
+  if (tid < QB)
+  {
+    const int a = tid;
+    double row[QB];
+#pragma unroll
+    for (int j = 0; j < QB; ++j) row[j] = 0.0;
+    if (a < bw)
+    {
+#pragma unroll
+      for (int j = 0; j < QB; ++j)
+      {
+        if (j == a) row[j] = taus[a];
+        if (j > a && j < bw)
+        {
+          double sacc = 0.0;
+#pragma unroll
+          for (int bb = 0; bb < QB; ++bb)
+            if (bb >= a && bb < j) sacc += row[bb] * Gm[bb][j];
+          row[j] = -taus[j] * sacc;
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < QB; ++j) Tl[a][j] = row[j];
+  }
+  __syncthreads();
+  double* Tout = P.T + (size_t)(j0 / QB) * QB * QB;
+  for (int e = tid; e < QB * QB; e += NT) Tout[e] = Tl[e / QB][e % QB];
+}
+
 // C <- (I - V op(T) V^T) C for columns [cbeg, cend) of C, rows j0 .. m-1.
 // mode 0: factorisation (C = A, op(T) = T^T, cbeg = j0 + bw, cend = n).
 // mode 1: Q formation (C = Q, op(T) = T, cbeg = j0, cend = k).
@@ -492,7 +622,6 @@ void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, i
 
 int chol_max_n() { return kCholMaxN; }
 
-int qr_blocked_max_rows() { return 4 * 512; }
 
 void launch_qr_blocked(const qr_batch& b, hipStream_t st)
 {
@@ -508,7 +637,9 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
   {
     // A single small matrix: one wave holding up to 4 rows per lane, so the
     // per-column reductions need no cross-wave barrier.
-    if (batch == 1 && b.max_m <= 64)
+    if (b.max_m > 4 * NT)
+      hipLaunchKernelGGL((qr_panel_tall_kernel<NT>), dim3(batch), dim3(NT), 0, st, b, j0);
+    else if (batch == 1 && b.max_m <= 64)
       hipLaunchKernelGGL((qr_panel_kernel<64, 1>), dim3(1), dim3(64), 0, st, b, j0);
     else if (batch == 1 && b.max_m <= 256)
       hipLaunchKernelGGL((qr_panel_kernel<64, 4>), dim3(1), dim3(64), 0, st, b, j0);

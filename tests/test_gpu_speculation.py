@@ -104,3 +104,23 @@ def test_speculation_falls_back_on_cholqr_breakdown():
     assert sw == 2 and failed == 2
     _same(on, off)
     assert on.ranks == ref.ranks and rel_err(cores3(on), ref) <= 1e-12
+
+
+def test_speculation_rank_deficient_sum_with_cap():
+    """round(x + x, max_rank < 2r) with bonds >= 32: the sum's Grams are
+    singular, so a speculative CholQR2 breaks down and its (discarded) factors
+    hold non-finite values while the sweep continues; the capped SVD steps
+    run the Jacobi SVD on them before the sweep's flag is read.  The sorted
+    singular values must still index every column exactly once (no
+    out-of-range gather), and the undone sweep must give the checked result."""
+    x = O.OTT.random([4] * 8, 40, 91)
+    s = O.add(x, x)
+    cap = 30
+    ref = O.round(s, 1e-12, cap, 0)
+    ctx = T.Context.default()
+    on, (sw, failed) = _run(ctx, lambda: T.round(to_dev(s), eps=1e-12, max_rank=cap), True)
+    off, _ = _run(ctx, lambda: T.round(to_dev(s), eps=1e-12, max_rank=cap), False)
+    _same(on, off)
+    assert on.ranks == ref.ranks
+    assert max(on.ranks) == cap
+    assert rel_err(cores3(on), ref) <= 1e-12
