@@ -217,13 +217,52 @@ def batched_section(T, ctx, args):
     ctx.synchronize()
     T.round_batched(c3[:16], eps=eps)
     ctx.synchronize()
+    ctx.prof_enable(True)
     t0 = time.perf_counter()
     p3 = T.round_batched(c3, eps=eps)
     ctx.synchronize()
     t3 = time.perf_counter() - t0
+    p3_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "other")}
+    p3_sweeps = ctx.jacobi_sweeps()
+    ctx.prof_enable(False)
+    # Mixed: rank-revealing CholQR2 steps where certified.
+    T.round_batched(c3[:16], eps=eps, precision="mixed")
+    ctx.synchronize()
+    ctx.prof_enable(True)
+    t0 = time.perf_counter()
+    m3 = T.round_batched(c3, eps=eps, precision="mixed")
+    ctx.synchronize()
+    t3m = time.perf_counter() - t0
+    m3_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "chol", "other")}
+    m3_rr = ctx.rank_revealing_counts()
+    ctx.prof_enable(False)
+    sample3 = list(range(0, n3, max(1, n3 // 16)))
+
+    def rel3(a, b):
+        dif = T.sub(a, b)
+        T.right_orthogonalize(dif)
+        ref = b.copy()
+        T.right_orthogonalize(ref)
+        return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
+
+    m3_err_input = max(rel3(m3[i], c3[i]) for i in sample3)
+    m3_diff_plain = max(rel3(m3[i], p3[i]) for i in sample3)
+    p3_err_input = max(rel3(p3[i], c3[i]) for i in sample3)
+    ranks_equal = sum(int(a.ranks == b.ranks) for a, b in zip(m3, p3))
     out["config3"] = {
         "workload": f"{n3} TTs, d~U{{8..40}}, n in {{2,4,8}}, add(u, add(0.5u, v)), r_u, r_v ~ U{{8..64}}, eps=1e-6",
         "plain_fp64_rounds_per_s": n3 / t3,
+        "plain_kernel_ms": p3_ms,
+        "plain_jacobi_calls_sweeps": list(p3_sweeps),
+        "mixed_rounds_per_s": n3 / t3m,
+        "speedup_mixed_vs_plain": t3 / t3m,
+        "mixed_kernel_ms": m3_ms,
+        "mixed_certified_steps": {"tried": m3_rr[0], "accepted": m3_rr[1]},
+        "mixed_ranks_equal_plain": f"{ranks_equal}/{n3}",
+        "mean_rank_mixed_over_plain": float(np.mean([np.mean(a.ranks[1:-1]) / np.mean(b.ranks[1:-1])
+                                                     for a, b in zip(m3, p3)])),
+        "max_rel_err_vs_input_sampled": {"mixed": m3_err_input, "plain": p3_err_input},
+        "max_rel_diff_mixed_vs_plain_sampled": m3_diff_plain,
         "mean_out_rank": float(np.mean([np.mean(p.ranks[1:-1]) for p in p3])),
     }
     return out
@@ -258,6 +297,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="skip the matvec_round workload and the CPU baseline (profiling the batched paths)")
     ap.add_argument("--cfg4-count", type=int, default=2048)
     ap.add_argument("--cfg3-count", type=int, default=4096)
     ap.add_argument("--train-tts", type=int, default=650)
@@ -277,6 +318,9 @@ def main():
     import tensor_train_amd as T
 
     ctx = T.Context.default(0)
+    if args.batched_only:
+        print(json.dumps({"batched": batched_section(T, ctx, args)}))
+        return
     A, x = build_inputs(T)
     for _ in range(args.warmup):
         y = T.matvec_round(A, x, eps=EPS)

@@ -167,13 +167,24 @@ int tta_scale(tta_ctx* ctx, const tta_tt* a, double alpha, tta_tt** out);
 enum tta_precision
 {
   TTA_PREC_FP64 = 0,   /* plain: the reference algorithm in FP64 */
-  TTA_PREC_MIXED = 1,  /* FP32 contractions, FP64 Gram / Cholesky / small SVD */
-  TTA_PREC_LEARNED = 2 /* learned-rank randomized rounding with plain fallback */
+  TTA_PREC_MIXED = 1,  /* SVD steps by rank-revealing CholQR2 where certified */
+  TTA_PREC_LEARNED = 2 /* learned-rank randomized rounding with plain fallback
+                          (tta_round_learned_batched; not a tta_round_batched mode) */
 };
 /* round(a_i, {eps, max_rank}) for count TTs of any d, modes and ranks
  * (detail::round, include/core/round/tt_round.hpp:36-127, per TT), the
  * TTs advanced in lockstep so each launch covers the batch.  out[i] are
- * fresh tensors.  No reference counterpart for the batching itself. */
+ * fresh tensors.  No reference counterpart for the batching itself.
+ * precision TTA_PREC_FP64: the plain algorithm, results as tta_round.
+ * precision TTA_PREC_MIXED: each tall SVD-sweep step (unfolding M of at most
+ * 192 columns) is first tried as a rank-revealing CholQR2: FP64 Gram M^T M
+ * and its Cholesky with diagonal pivoting stopped where the residual of the
+ * p pivot columns fits (0.4 delta)^2, CholQR2 of those columns (FP64), then
+ * the FP64 residual correction R~ = Q^T M and E = M - Q R~.  The step keeps
+ * (Q, R~ core_{k+1}) at rank p when ||E||_F <= delta / 2; otherwise it runs as
+ * in the plain mode.  Each accepted step discards at most delta / 2, so the
+ * round's error bound holds and ranks are never below the plain ranks; on
+ * exactly rank-deficient unfoldings (sums) the ranks are the plain ranks. */
 int tta_round_batched(tta_ctx* ctx, int count, tta_tt* const* in, double eps, int max_rank, int precision,
                       tta_tt** out);
 
@@ -325,6 +336,10 @@ int tta_prof_jacobi_sweeps(tta_ctx* ctx, long long* calls, long long* sweeps);
  * the rotation tolerance.  No reference counterpart (Eigen's BDCSVD has no
  * sweep limit to report). */
 int tta_prof_jacobi_unconverged(tta_ctx* ctx, long long* problems);
+/* SVD-sweep steps of TTA_PREC_MIXED batched rounding tried by the
+ * rank-revealing CholQR2 certificate and those accepted, since the last
+ * tta_prof_enable. */
+int tta_prof_rank_revealing(tta_ctx* ctx, long long* tried, long long* held);
 
 /* ---- full-rank certificate of the SVD steps (no reference counterpart) ---- */
 /* Before the Jacobi SVD of an SVD step of round / round_right_canonical, the

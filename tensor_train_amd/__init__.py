@@ -89,6 +89,13 @@ class Context:
         self.check(self.lib.tta_prof_jacobi_sweeps(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def rank_revealing_counts(self):
+        """(tried, accepted) SVD steps of the mixed batched rounding's
+        rank-revealing CholQR2 since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_rank_revealing(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def jacobi_unconverged(self) -> int:
         """Block-Jacobi problems still rotating at the sweep limit since the
         last prof_enable (0 when every SVD converged)."""
@@ -645,7 +652,8 @@ PRECISIONS = {"fp64": 0, "mixed": 1, "learned": 2}
 
 
 def round_batched(tts, eps=0.0, max_rank=0, precision="fp64"):
-    """Round a list of device TTs in one batched call (tta_round_batched)."""
+    """Round a list of device TTs in one batched call (tta_round_batched);
+    precision "fp64" (plain) or "mixed" (rank-revealing CholQR2 steps)."""
     if not tts:
         return []
     ctx = tts[0]._ctx

@@ -102,6 +102,8 @@ SIGNATURES = {
                                                   ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_jacobi_sweeps": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_jacobi_unconverged": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    "tta_prof_rank_revealing": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                               ctypes.POINTER(ctypes.c_longlong)]),
     "tta_soft_threshold": (ctypes.c_int, [_vp, _vp, ctypes.c_double, ctypes.POINTER(ctypes.c_void_p)]),
     "tta_from_dense": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),

@@ -2944,6 +2944,8 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->jac_calls = 0;
   c->jac_sweeps = 0;
   c->jac_unconverged = 0;
+  c->rr_tried = 0;
+  c->rr_held = 0;
   c->cert_tried = 0;
   c->cert_held = 0;
   c->cholqr_tried = 0;
@@ -3019,6 +3021,14 @@ int tta_prof_jacobi_unconverged(tta_ctx* c, long long* problems)
 {
   if (!c || !problems) return TTA_ERR_ARG;
   *problems = c->jac_unconverged;
+  return TTA_OK;
+}
+
+int tta_prof_rank_revealing(tta_ctx* c, long long* tried, long long* held)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (tried) *tried = c->rr_tried;
+  if (held) *held = c->rr_held;
   return TTA_OK;
 }
 

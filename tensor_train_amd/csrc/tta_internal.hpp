@@ -107,6 +107,9 @@ struct tta_ctx
   long long jac_calls = 0, jac_sweeps = 0;
   // Block-Jacobi problems that had not converged after the sweep limit.
   long long jac_unconverged = 0;
+  // SVD-sweep steps of the mixed batched rounding tried by the rank-revealing
+  // CholQR2 certificate, and those it accepted.
+  long long rr_tried = 0, rr_held = 0;
   // Full-rank certificate of the SVD steps (on by default): steps where it
   // was tried, and where it held so that the Jacobi SVD was not needed.
   bool rank_certificate = true;

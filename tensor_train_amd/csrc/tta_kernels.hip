@@ -445,7 +445,8 @@ __global__ void polar_series_kernel(const double* E, const double* V, double* P,
 __device__ __forceinline__ double gathered(const double* src, int lds_, int t, int o, const int* perm,
                                            const double* scale, int mode)
 {
-  const double v = src[t + (size_t)perm[o] * lds_];
+  const double v = (mode & GATHER_ROWMAJOR) ? src[(size_t)t * lds_ + perm[o]] : src[t + (size_t)perm[o] * lds_];
+  mode &= ~GATHER_ROWMAJOR;
   if (mode == GATHER_MUL) return v * scale[o];
   if (mode == GATHER_DIV) return scale[o] != 0.0 ? v / scale[o] : (t == o ? 1.0 : 0.0);
   return v;

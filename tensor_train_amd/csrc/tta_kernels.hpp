@@ -150,7 +150,10 @@ enum
 {
   GATHER_COPY = 0,
   GATHER_MUL = 1,
-  GATHER_DIV = 2
+  GATHER_DIV = 2,
+  // Flag added to a mode: src is row-major, column perm[o] read as
+  // src[t*lds + perm[o]].
+  GATHER_ROWMAJOR = 4
 };
 struct ga_desc
 {
@@ -205,6 +208,30 @@ struct permute_desc
   long n[6] = {1, 1, 1, 1, 1, 1};
   long ss[6] = {0, 0, 0, 0, 0, 0};
 };
+
+// One problem of the batched pivoted Cholesky (tta_rrqr.hip): G (q x q,
+// row-major, ldg) factored with diagonal pivoting and a trace stop at
+// stop_f * (*delta)^2 (pivot = 1), or in order without pivoting (pivot = 0;
+// p_out = -1 on a non-positive pivot).  Writes the pivot order piv (q; the
+// identity without pivoting, may be null then), the pivot count p_out, the
+// factor's columns L (q x q scratch, column-major, original row index), and
+// R^-1 (p x p upper triangular, row-major, ldr) for R = L11^T in pivot order.
+struct pchol_desc
+{
+  const double* G = nullptr;
+  int ldg = 0, q = 0;
+  const double* delta = nullptr;
+  double stop_f = 0.0;
+  int pivot = 1;
+  int* piv = nullptr;
+  int* p_out = nullptr;
+  double* L = nullptr;
+  double* Rinv = nullptr;
+  int ldr = 0;
+};
+constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
+size_t pchol_lds_bytes(int q);
+void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, hipStream_t st);
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // Split-K form of a single GEMM for shapes with few output tiles:

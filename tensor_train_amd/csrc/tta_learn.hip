@@ -24,7 +24,7 @@
 
 namespace tta
 {
-void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int max_rank);
+void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int max_rank, bool mixed);
 }
 
 using namespace tta;

@@ -32,7 +32,7 @@ namespace tta
 {
 void launch_gauss_batched(void* const* dst, const int* m, const int* nc, const unsigned long long* seed, const int* k,
                           int count, bool f32, tta_ctx* c);
-void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int max_rank);
+void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int max_rank, bool mixed);
 
 namespace
 {
@@ -482,7 +482,7 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
       upload_async(c, d, cp2.data(), cp2.size() * sizeof(cp_desc));
       launch_copy_batched(static_cast<cp_desc*>(d), (int)cp2.size(), me, c->stream);
       c->mem.put(d);
-      round_batched_plain(c, again, eps, 0);
+      round_batched_plain(c, again, eps, 0, false);
     }
     for (int i = 0; i < count; ++i) out[i] = ts[i];
   }
