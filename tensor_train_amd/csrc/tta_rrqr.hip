@@ -3,7 +3,10 @@
 //
 // One workgroup per problem.  The symmetric q x q matrix G (row-major) is
 // held as a packed lower triangle in LDS (q <= kPcholMaxQ, about 147 KiB at
-// q = 192) and factored right-looking, one pivot per step:
+// q = 192) and factored right-looking, one pivot per step, the pivot moved
+// to the front by a symmetric swap (as LAPACK's dpstf2) so the trailing
+// Schur complement stays a contiguous triangle whose update is spread
+// evenly over the workgroup:
 //   * pivoting on: the pivot is the largest remaining diagonal entry of the
 //     Schur complement S (ties to the smaller index), and the factorization
 //     stops before step j when trace(S) <= stop_f * delta^2 or the largest
@@ -30,7 +33,7 @@ namespace
 {
 __device__ __forceinline__ int pk(int i, int l) { return i * (i + 1) / 2 + l; }  // i >= l
 
-// Sum and arg-max (largest value, smallest index on ties) over the block;
+// Sum and arg-max (largest value, smallest position on ties) over the block;
 // every thread gets the results.  `red`: LDS of 3 * NT / 64 doubles.
 template <int NT>
 __device__ __forceinline__ void block_sum_argmax(double& sum, double& mv, int& mi, double* red)
@@ -72,6 +75,15 @@ __device__ __forceinline__ void block_sum_argmax(double& sum, double& mv, int& m
   }
 }
 
+// Row and column of entry t of a packed lower triangle (t = i (i + 1) / 2 + l).
+__device__ __forceinline__ void unpack(int t, int& i, int& l)
+{
+  i = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while (pk(i + 1, 0) <= t) ++i;
+  while (pk(i, 0) > t) --i;
+  l = t - pk(i, 0);
+}
+
 template <int NT>
 __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
 {
@@ -80,10 +92,10 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
   const int tid = threadIdx.x;
   extern __shared__ double sm[];
   const int npk = q * (q + 1) / 2;
-  double* S = sm;
-  double* lv = S + npk;
+  double* S = sm;            // packed lower triangle, by position
+  double* lv = S + npk;      // the current column of the factor, by position
   double* red = lv + q;
-  int* done = reinterpret_cast<int*>(red + 3 * (NT / 64));
+  int* ord = reinterpret_cast<int*>(red + 3 * (NT / 64));  // position -> original index
   if (q <= 0)
   {
     if (tid == 0) *d.p_out = 0;
@@ -91,13 +103,11 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
   }
   for (int e = tid; e < npk; e += NT)
   {
-    int i = (int)((sqrt(8.0 * e + 1.0) - 1.0) * 0.5);
-    while (pk(i + 1, 0) <= e) ++i;
-    while (pk(i, 0) > e) --i;
-    const int l = e - pk(i, 0);
+    int i, l;
+    unpack(e, i, l);
     S[e] = d.G[(size_t)i * d.ldg + l];
   }
-  for (int i = tid; i < q; i += NT) done[i] = 0;
+  for (int i = tid; i < q; i += NT) ord[i] = i;
   __syncthreads();
   double dmax = 0.0;
   for (int i = 0; i < q; ++i) dmax = fmax(dmax, S[pk(i, i)]);
@@ -112,21 +122,21 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
   bool broke = false;
   for (int j = 0; j < q; ++j)
   {
+    // Trace and largest diagonal entry of the trailing Schur complement.
     double tr = 0.0, mv = -1.0;
     int mi = 0x7fffffff;
-    for (int i = tid; i < q; i += NT)
-      if (!done[i])
+    for (int i = j + tid; i < q; i += NT)
+    {
+      const double v = S[pk(i, i)];
+      tr += v > 0.0 ? v : 0.0;
+      if (v > mv)
       {
-        const double v = S[pk(i, i)];
-        tr += v > 0.0 ? v : 0.0;
-        if (v > mv || (v == mv && i < mi))
-        {
-          mv = v;
-          mi = i;
-        }
+        mv = v;
+        mi = i;
       }
+    }
     block_sum_argmax<NT>(tr, mv, mi, red);
-    int pj;
+    int pj = j;
     if (d.pivot)
     {
       if (tr <= stop2 || !(mv > floor_piv)) break;
@@ -134,7 +144,6 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
     }
     else
     {
-      pj = j;
       mv = S[pk(j, j)];
       if (!(mv > 0.0))
       {
@@ -142,34 +151,64 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
         break;
       }
     }
+    // Symmetric swap of positions j and pj (pj > j) in the trailing part.
+    if (pj != j)
+    {
+      for (int l = j + 1 + tid; l < q; l += NT)
+      {
+        if (l < pj)
+        {
+          const double x = S[pk(l, j)];
+          S[pk(l, j)] = S[pk(pj, l)];
+          S[pk(pj, l)] = x;
+        }
+        else if (l > pj)
+        {
+          const double x = S[pk(l, j)];
+          S[pk(l, j)] = S[pk(l, pj)];
+          S[pk(l, pj)] = x;
+        }
+      }
+      if (tid == 0)
+      {
+        const double x = S[pk(j, j)];
+        S[pk(j, j)] = S[pk(pj, pj)];
+        S[pk(pj, pj)] = x;
+        const int o = ord[j];
+        ord[j] = ord[pj];
+        ord[pj] = o;
+      }
+      __syncthreads();
+    }
     const double dj = sqrt(mv);
     const double inv = 1.0 / dj;
-    for (int i = tid; i < q; i += NT)
+    for (int i = j + tid; i < q; i += NT)
     {
-      double li = 0.0;
-      if (i == pj)
-        li = dj;
-      else if (!done[i])
-        li = (i > pj ? S[pk(i, pj)] : S[pk(pj, i)]) * inv;
+      const double li = i == j ? dj : S[pk(i, j)] * inv;
       lv[i] = li;
-      d.L[(size_t)j * q + i] = li;
+      d.L[(size_t)j * q + ord[i]] = li;
     }
     __syncthreads();
-    // Rank-1 update of the remaining Schur complement (rows and columns of
-    // finished pivots hold lv = 0; the pivot's own row becomes dead).
-    for (int i = tid; i < q; i += NT)
+    // Trailing update S(i, l) -= lv_i lv_l, j < l <= i < q, spread evenly
+    // over the block by the packed index of the trailing triangle.
+    const int nt = q - j - 1;
+    const int cnt = nt * (nt + 1) / 2;
+    if (tid < cnt)
     {
-      const double li = lv[i];
-      if (li == 0.0 || i == pj) continue;
-      double* row = S + pk(i, 0);
-      for (int l = 0; l <= i; ++l)
-        if (l != pj) row[l] = fma(-li, lv[l], row[l]);
-    }
-    __syncthreads();
-    if (tid == 0)
-    {
-      done[pj] = 1;
-      if (d.piv) d.piv[j] = pj;
+      int ii, ll;
+      unpack(tid, ii, ll);
+      for (int t = tid; t < cnt; t += NT)
+      {
+        const int i = j + 1 + ii, l = j + 1 + ll;
+        double* e = S + pk(i, l);
+        *e = fma(-lv[i], lv[l], *e);
+        ll += NT;
+        while (ll > ii)
+        {
+          ll -= ii + 1;
+          ++ii;
+        }
+      }
     }
     __syncthreads();
     p = j + 1;
@@ -180,14 +219,17 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
     return;
   }
   if (tid == 0) *d.p_out = p;
-  if (d.piv && !d.pivot)
-    for (int i = tid; i < q; i += NT) d.piv[i] = i;
+  if (d.piv)
+    for (int i = tid; i < q; i += NT) d.piv[i] = ord[i];
+  // Leading block of the factor in pivot order, packed: T(a, b) = L(ord_a, b).
+  // The L columns were written by this workgroup in this launch; the barrier
+  // orders those stores before the loads.
   __syncthreads();
-  // Leading block in pivot order, packed: T(a, b) = L(piv_a, b), a >= b.
-  for (int a = tid; a < p; a += NT)
+  for (int e = tid; e < p * (p + 1) / 2; e += NT)
   {
-    const int ra = d.pivot ? d.piv[a] : a;
-    for (int b = 0; b <= a; ++b) S[pk(a, b)] = d.L[(size_t)b * q + ra];
+    int a, b;
+    unpack(e, a, b);
+    S[e] = d.L[(size_t)b * q + ord[a]];
   }
   __syncthreads();
   // In-place inverse of the lower triangle, last column first:
@@ -203,7 +245,6 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
     __syncthreads();
     const double xjj = 1.0 / S[pk(j, j)];
     for (int i = j + 1 + tid; i < p; i += NT) S[pk(i, j)] = -lv[i] * xjj;
-    __syncthreads();
     if (tid == 0) S[pk(j, j)] = xjj;
     __syncthreads();
   }
