@@ -131,9 +131,112 @@ def cpu_baseline(reps: int, y_dev_cores):
     }, rel, yref.ranks
 
 
+def _rel(T, a, b):
+    """||a - b|| / ||b|| as ||core 0|| of the right-orthogonalized difference
+    TT (QR on the device, no squaring; resolves about 1e-15)."""
+    import numpy as np
+
+    dif = T.sub(a, b)
+    T.right_orthogonalize(dif)
+    ref = b.copy()
+    T.right_orthogonalize(ref)
+    return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
+
+
+def _timed(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
+    """fn() timed on the wall clock (stream drained before and after), with
+    the library's per-class kernel milliseconds from a second, event-timed
+    run of the same call."""
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    out = fn()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    return out, wall
+
+
+def _kernel_ms(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
+    ctx.prof_enable(True)
+    out = fn()
+    ctx.synchronize()
+    ms = {k: ctx.prof_read(k)["ms"] for k in classes}
+    extra = {"rank_revealing": ctx.rank_revealing_counts(), "jacobi_calls_sweeps": ctx.jacobi_sweeps()}
+    ctx.prof_enable(False)
+    del out
+    return ms, extra
+
+
+def _train_mlp(T, ctx, train, eps, steps):
+    import numpy as np
+
+    plain_tr = T.round_batched(train, eps=eps)
+    X = T.bond_features(train, eps)
+    y = np.array([p.ranks[k] / t.ranks[k] for t, p in zip(train, plain_tr) for k in range(1, t.d)], np.float32)
+    mlp = T.RankMLP(X.shape[1], 64, 3, seed=1)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    loss = mlp.train(X, y, steps=steps, lr=2e-3)
+    ctx.synchronize()
+    return mlp, {"train_bonds": int(len(y)), "steps": steps, "final_mse": loss, "train_s": time.perf_counter() - t0}
+
+
+def _three_paths(T, ctx, batch, mlp, eps, label):
+    """Plain FP64, mixed and mixed + learned (randomized sketch rounding with
+    learned ranks) batched rounding of one workload: rounds/s, kernel time
+    per class, sampled errors (against the input and against plain)."""
+    import numpy as np
+
+    n = len(batch)
+    T.round_batched(batch[:16], eps=eps)
+    T.round_batched(batch[:16], eps=eps, precision="mixed")
+    T.round_learned_batched(mlp, batch[:16], eps, oversample=6, slack=0.1, seed=3)
+    plain, t_plain = _timed(ctx, lambda: T.round_batched(batch, eps=eps))
+    mixed, t_mixed = _timed(ctx, lambda: T.round_batched(batch, eps=eps, precision="mixed"))
+    (learned, err, fb, ranks), t_learned = _timed(
+        ctx, lambda: T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3))
+    sample = list(range(0, n, max(1, n // 16)))
+    res = {
+        "plain_fp64_rounds_per_s": n / t_plain,
+        "mixed_rounds_per_s": n / t_mixed,
+        "mixed_learned_rounds_per_s": n / t_learned,
+        "speedup_mixed_vs_plain": t_plain / t_mixed,
+        "speedup_mixed_learned_vs_plain": t_plain / t_learned,
+        "learned_fallback_fraction": float(fb.mean()),
+        "learned_max_discarded_rel_norm": float(err[fb == 0].max()) if (fb == 0).any() else None,
+        "max_rel_err_vs_input_sampled": {
+            "plain": max(_rel(T, plain[i], batch[i]) for i in sample),
+            "mixed": max(_rel(T, mixed[i], batch[i]) for i in sample),
+            "mixed_learned": max(_rel(T, learned[i], batch[i]) for i in sample)},
+        "max_rel_diff_vs_plain_sampled": {
+            "mixed": max(_rel(T, mixed[i], plain[i]) for i in sample),
+            "mixed_learned": max(_rel(T, learned[i], plain[i]) for i in sample)},
+        "mean_rank": {"plain": float(np.mean([np.mean(p.ranks[1:-1]) for p in plain])),
+                      "mixed": float(np.mean([np.mean(p.ranks[1:-1]) for p in mixed])),
+                      "learned_target": float(np.mean([np.mean(r[1:-1]) for r in ranks]))},
+        "mixed_ranks_equal_plain": f"{sum(int(a.ranks == b.ranks) for a, b in zip(mixed, plain))}/{n}",
+    }
+    del plain, mixed, learned
+    res["kernel_ms"] = {}
+    for name, fn in (("plain", lambda: T.round_batched(batch, eps=eps)),
+                     ("mixed", lambda: T.round_batched(batch, eps=eps, precision="mixed")),
+                     ("mixed_learned", lambda: T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1,
+                                                                      seed=3)[0])):
+        ms, extra = _kernel_ms(ctx, fn)
+        res["kernel_ms"][name] = ms
+        if name == "plain":
+            res["plain_jacobi_calls_sweeps"] = list(extra["jacobi_calls_sweeps"])
+        if name == "mixed":
+            res["mixed_certified_steps"] = {"tried": extra["rank_revealing"][0],
+                                            "accepted": extra["rank_revealing"][1]}
+    print(f"[bench] {label}: plain {res['plain_fp64_rounds_per_s']:.0f}, mixed {res['mixed_rounds_per_s']:.0f}, "
+          f"mixed+learned {res['mixed_learned_rounds_per_s']:.0f} rounds/s", file=sys.stderr, flush=True)
+    return res
+
+
 def batched_section(T, ctx, args):
-    """Batched workloads: config 4 (learned-rank randomized vs plain FP64
-    batched rounding) and config 3 (plain FP64 batched rounding)."""
+    """Batched workloads (config 4 and config 3): plain FP64, mixed, and
+    mixed + learned batched rounding, and for config 4 the randomized
+    rounding without the network (stored ranks as targets)."""
     import numpy as np
 
     out = {}
@@ -144,127 +247,41 @@ def batched_section(T, ctx, args):
     n4 = args.cfg4_count
     s_train = rng.uniform(0.05, 0.3, args.train_tts)
     train = T.build_decay_batch(args.train_tts, 32, 4, 128, s_train, 10_000_000 + np.arange(args.train_tts))
-    plain_tr = T.round_batched(train, eps=eps)
-    X = T.bond_features(train, eps)
-    y = np.array([p.ranks[k] / t.ranks[k] for t, p in zip(train, plain_tr) for k in range(1, t.d)], np.float32)
-    del plain_tr
-    mlp = T.RankMLP(X.shape[1], 64, 3, seed=1)
-    t0 = time.perf_counter()
-    loss = mlp.train(X, y, steps=args.mlp_steps, lr=2e-3)
-    ctx.synchronize()
-    train_s = time.perf_counter() - t0
+    mlp4, mlp4_info = _train_mlp(T, ctx, train, eps, args.mlp_steps)
     del train
     s4 = rng.uniform(0.05, 0.3, n4)
     batch = T.build_decay_batch(n4, 32, 4, 128, s4, np.arange(n4))
-    ctx.synchronize()
-    # warm up both paths on a slice
-    T.round_batched(batch[:32], eps=eps)
-    T.round_learned_batched(mlp, batch[:32], eps, oversample=6, slack=0.1, seed=3)
-    ctx.synchronize()
-    ctx.prof_enable(True)
-    t0 = time.perf_counter()
-    plain = T.round_batched(batch, eps=eps)
-    ctx.synchronize()
-    t_plain = time.perf_counter() - t0
-    plain_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "other")}
-    ctx.prof_enable(True)
-    t0 = time.perf_counter()
-    learned, err, fb, ranks = T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3)
-    ctx.synchronize()
-    t_learned = time.perf_counter() - t0
-    learned_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "other")}
-    ctx.prof_enable(False)
-    # Errors on a sample, each norm as ||core 0|| of the right-orthogonalized
-    # difference TT (QR based; resolves ~1e-15): learned vs input, plain vs
-    # input, learned vs plain.
-    sample = list(range(0, n4, max(1, n4 // 16)))
-
-    def rel(a, b):
-        dif = T.sub(a, b)
-        T.right_orthogonalize(dif)
-        ref = b.copy()
-        T.right_orthogonalize(ref)
-        return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
-
-    err_learned = [rel(learned[i], batch[i]) for i in sample]
-    err_plain = [rel(plain[i], batch[i]) for i in sample]
-    diffs = [rel(learned[i], plain[i]) for i in sample]
-    out["config4"] = {
-        "workload": f"{n4} TTs, d=32, n=4, ranks min(4^k, 4^(32-k), 128), cores Q_k diag(10^(-j s)), "
-                    "s~U[0.05,0.3], eps=1e-6",
-        "plain_fp64_rounds_per_s": n4 / t_plain,
-        "learned_rounds_per_s": n4 / t_learned,
-        "speedup_learned_vs_plain": t_plain / t_learned,
-        "fallback_fraction": float(fb.mean()),
-        "max_discarded_rel_norm": float(err[fb == 0].max()) if (fb == 0).any() else None,
-        "max_rel_err_vs_input_sampled": {"learned": max(err_learned), "plain": max(err_plain)},
-        "max_rel_diff_vs_plain_sampled": max(diffs),
-        "plain_kernel_ms": plain_ms,
-        "learned_kernel_ms": learned_ms,
-        "mean_target_rank": float(np.mean([np.mean(r[1:-1]) for r in ranks])),
-        "mean_plain_rank": float(np.mean([np.mean(p.ranks[1:-1]) for p in plain])),
-        "mlp": {"train_bonds": int(len(y)), "steps": args.mlp_steps, "final_mse": loss, "train_s": train_s},
-    }
-    del batch, plain, learned
+    r4 = _three_paths(T, ctx, batch, mlp4, eps, "config 4")
+    # The method without the network: randomized rounding with the stored
+    # ranks as targets (no truncation but the final projection's), same
+    # fallback rule.
+    stored = [t.ranks for t in batch]
+    T.round_randomized_batched(batch[:16], stored[:16], eps, seed=3)
+    (nolearn, err0, fb0), t0s = _timed(ctx, lambda: T.round_randomized_batched(batch, stored, eps, seed=3))
+    r4["randomized_stored_ranks_rounds_per_s"] = n4 / t0s
+    r4["randomized_stored_ranks_fallback_fraction"] = float(fb0.mean())
+    del nolearn
+    r4.update({"workload": f"{n4} TTs, d=32, n=4, ranks min(4^k, 4^(32-k), 128), cores Q_k diag(10^(-j s)), "
+                           "s~U[0.05,0.3], eps=1e-6", "mlp": mlp4_info})
+    out["config4"] = r4
+    del batch
 
     # ---- config 3: 4096 TTs, d~U{8..40}, n in {2,4,8}, add(u, add(0.5u, v)).
-    n3 = args.cfg3_count
-    d = rng.integers(8, 41, n3)
-    nn = rng.choice([2, 4, 8], n3)
-    ru = rng.integers(8, 65, n3)
-    rv = rng.integers(8, 65, n3)
-    c3 = T.build_sum_batch(d, nn, ru, rv, 1000 + np.arange(n3), 500000 + np.arange(n3))
-    ctx.synchronize()
-    T.round_batched(c3[:16], eps=eps)
-    ctx.synchronize()
-    ctx.prof_enable(True)
-    t0 = time.perf_counter()
-    p3 = T.round_batched(c3, eps=eps)
-    ctx.synchronize()
-    t3 = time.perf_counter() - t0
-    p3_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "other")}
-    p3_sweeps = ctx.jacobi_sweeps()
-    ctx.prof_enable(False)
-    # Mixed: rank-revealing CholQR2 steps where certified.
-    T.round_batched(c3[:16], eps=eps, precision="mixed")
-    ctx.synchronize()
-    ctx.prof_enable(True)
-    t0 = time.perf_counter()
-    m3 = T.round_batched(c3, eps=eps, precision="mixed")
-    ctx.synchronize()
-    t3m = time.perf_counter() - t0
-    m3_ms = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "chol", "other")}
-    m3_rr = ctx.rank_revealing_counts()
-    ctx.prof_enable(False)
-    sample3 = list(range(0, n3, max(1, n3 // 16)))
+    def sums(count, seed0):
+        d = rng.integers(8, 41, count)
+        nn = rng.choice([2, 4, 8], count)
+        ru = rng.integers(8, 65, count)
+        rv = rng.integers(8, 65, count)
+        return T.build_sum_batch(d, nn, ru, rv, seed0 + np.arange(count), seed0 + 500000 + np.arange(count))
 
-    def rel3(a, b):
-        dif = T.sub(a, b)
-        T.right_orthogonalize(dif)
-        ref = b.copy()
-        T.right_orthogonalize(ref)
-        return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
-
-    m3_err_input = max(rel3(m3[i], c3[i]) for i in sample3)
-    m3_diff_plain = max(rel3(m3[i], p3[i]) for i in sample3)
-    p3_err_input = max(rel3(p3[i], c3[i]) for i in sample3)
-    ranks_equal = sum(int(a.ranks == b.ranks) for a, b in zip(m3, p3))
-    out["config3"] = {
-        "workload": f"{n3} TTs, d~U{{8..40}}, n in {{2,4,8}}, add(u, add(0.5u, v)), r_u, r_v ~ U{{8..64}}, eps=1e-6",
-        "plain_fp64_rounds_per_s": n3 / t3,
-        "plain_kernel_ms": p3_ms,
-        "plain_jacobi_calls_sweeps": list(p3_sweeps),
-        "mixed_rounds_per_s": n3 / t3m,
-        "speedup_mixed_vs_plain": t3 / t3m,
-        "mixed_kernel_ms": m3_ms,
-        "mixed_certified_steps": {"tried": m3_rr[0], "accepted": m3_rr[1]},
-        "mixed_ranks_equal_plain": f"{ranks_equal}/{n3}",
-        "mean_rank_mixed_over_plain": float(np.mean([np.mean(a.ranks[1:-1]) / np.mean(b.ranks[1:-1])
-                                                     for a, b in zip(m3, p3)])),
-        "max_rel_err_vs_input_sampled": {"mixed": m3_err_input, "plain": p3_err_input},
-        "max_rel_diff_mixed_vs_plain_sampled": m3_diff_plain,
-        "mean_out_rank": float(np.mean([np.mean(p.ranks[1:-1]) for p in p3])),
-    }
+    train3 = sums(args.train_tts_cfg3, 50_000_000)
+    mlp3, mlp3_info = _train_mlp(T, ctx, train3, eps, args.mlp_steps)
+    del train3
+    c3 = sums(args.cfg3_count, 1000)
+    r3 = _three_paths(T, ctx, c3, mlp3, eps, "config 3")
+    r3.update({"workload": f"{args.cfg3_count} TTs, d~U{{8..40}}, n in {{2,4,8}}, add(u, add(0.5u, v)), "
+                           "r_u, r_v ~ U{8..64}, eps=1e-6", "mlp": mlp3_info})
+    out["config3"] = r3
     return out
 
 
@@ -302,6 +319,7 @@ def main():
     ap.add_argument("--cfg4-count", type=int, default=2048)
     ap.add_argument("--cfg3-count", type=int, default=4096)
     ap.add_argument("--train-tts", type=int, default=650)
+    ap.add_argument("--train-tts-cfg3", type=int, default=850)
     ap.add_argument("--mlp-steps", type=int, default=1500)
     ap.add_argument("--full-svd-steps", type=int, default=2,
                     help="steps timed again with the full-rank certificate, CholQR2 and speculative sweeps "

@@ -411,6 +411,10 @@ struct rr_prob
   double stop_f = 0.0;
   long sur = 0, suc = 0;
   int max_p = 0;  // > 0: a pivot count above it is not carried to stage 2
+  // > 0: not carried to stage 2 when the trace left at the stop exceeds
+  // max_trace_f * (*delta)^2 (the step would be rejected anyway).
+  double max_trace_f = 0.0;
+  double hdelta = 0.0;  // host copy of *delta
 };
 struct rr_out
 {
@@ -429,6 +433,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   std::vector<double*> R1(nb);
   std::vector<int*> pivs(nb);
   int* pout = S.alloc<int>((size_t)nb);
+  double* trl = S.alloc<double>((size_t)nb);
   std::vector<gemm_desc> g1;
   std::vector<pchol_desc> pc;
   int max_q = 0, max_m = 0;
@@ -456,6 +461,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     d.L = S.alloc<double>((size_t)P.q * P.q);
     d.Rinv = R1[a];
     d.ldr = P.q;
+    d.tr_out = trl + a;
     pc.push_back(d);
     max_q = std::max(max_q, P.q);
     max_m = std::max(max_m, P.m);
@@ -470,12 +476,19 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     launch_pchol_batch(S.upload(pc), nb, max_q, c->stream);
   }
   std::vector<int> hp(nb);
+  std::vector<double> htr(nb);
   hip_check(hipMemcpyAsync(hp.data(), pout, nb * sizeof(int), hipMemcpyDeviceToHost, c->stream), "pivots");
+  hip_check(hipMemcpyAsync(htr.data(), trl, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream), "trace");
   sync(c);
 
   std::vector<int> al;
   for (int a = 0; a < nb; ++a)
-    if (hp[a] >= 1 && (pr[a].max_p <= 0 || hp[a] <= pr[a].max_p)) al.push_back(a);
+  {
+    const rr_prob& P = pr[a];
+    if (hp[a] < 1 || (P.max_p > 0 && hp[a] > P.max_p)) continue;
+    if (P.max_trace_f > 0.0 && htr[a] > P.max_trace_f * P.hdelta * P.hdelta) continue;
+    al.push_back(a);
+  }
   const int nc = (int)al.size();
   if (nc == 0) return out;
   std::vector<ga_desc> ga;
@@ -634,14 +647,15 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
 // is at rounding level and p is the plain rank.  Returns the TTs whose step
 // was not accepted; their cores are untouched.
 std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>& sel, int k,
-                                    int max_rank, const double* deltas, const std::vector<double>& hdelta)
+                                    int max_rank, const double* deltas, const std::vector<double>& hdelta,
+                                    std::vector<char>& off)
 {
   std::vector<int> rest, el;
   for (int i : sel)
   {
     const tta_tt* t = ts[i];
     const int rows = t->ranks[k] * t->modes[k], q = t->ranks[k + 1];
-    if (rows >= q && q <= kPcholMaxQ && hdelta[i] > 0.0)
+    if (!off[i] && rows >= q && q <= kPcholMaxQ && hdelta[i] > 0.0)
       el.push_back(i);
     else
       rest.push_back(i);
@@ -664,6 +678,8 @@ std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const 
     P.sur = -1;  // U row-major (m x p): row stride p, known after stage 1
     P.suc = 1;
     P.max_p = max_rank;
+    P.hdelta = hdelta[i];
+    P.max_trace_f = 0.5;  // a residual near sqrt(0.5) delta would fail the delta / 2 test
     pr.push_back(P);
   }
   std::vector<rr_out> o = rr_cholqr2(c, S, pr);
@@ -680,6 +696,7 @@ std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const 
     {
       if (o[a].U) c->mem.put(o[a].U);
       rest.push_back(i);
+      off[i] = 1;
       continue;
     }
     const int p = o[a].p, q = t->ranks[k + 1];
@@ -730,7 +747,7 @@ std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const 
 // right-orthonormal), core k-1 becomes core_{k-1} R~^T.  Returns the TTs
 // whose step was not accepted.
 std::vector<int> right_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>& sel, int j,
-                                      double eps)
+                                      double eps, std::vector<char>& off)
 {
   const double tol = std::min(1e-13, 1e-3 * eps);
   if (!(tol > 0.0)) return sel;
@@ -739,7 +756,7 @@ std::vector<int> right_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, cons
   {
     const tta_tt* t = ts[i];
     const int k = t->d() - 1 - j;
-    if (t->ranks[k] <= kPcholMaxQ)
+    if (!off[i] && t->ranks[k] <= kPcholMaxQ)
       el.push_back(i);
     else
       rest.push_back(i);
@@ -779,6 +796,7 @@ std::vector<int> right_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, cons
     {
       if (o[a].U) c->mem.put(o[a].U);
       rest.push_back(i);
+      off[i] = 1;
       continue;
     }
     const int p = o[a].p, r = t->ranks[k];
@@ -828,6 +846,12 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
   for (auto* t : ts) maxd = std::max(maxd, t->d());
   if (maxd <= 1) return;
 
+  // A TT whose certified step was rejected takes the plain steps for the
+  // rest of that sweep: the rejections come where the spectrum runs on below
+  // the Gram's resolution (sqrt(q u) of the largest singular value), which
+  // persists along the sweep.
+  std::vector<char> rr_off(count, 0);
+
   // ------------------------------------------------------------ right sweep
   for (int j = 0; j <= maxd - 2; ++j)
   {
@@ -835,7 +859,7 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
     for (int i = 0; i < count; ++i)
       if (ts[i]->d() - 1 - j >= 1) sel.push_back(i);
     if (sel.empty()) continue;
-    if (mixed) sel = right_step_certified(c, ts, sel, j, eps);
+    if (mixed) sel = right_step_certified(c, ts, sel, j, eps, rr_off);
     if (!sel.empty()) right_step_plain(c, ts, sel, j);
   }
 
@@ -895,13 +919,14 @@ void round_batched_plain(tta_ctx* c, std::vector<tta_tt*>& ts, double eps, int m
   }
 
   // ------------------------------------------------------------ SVD sweep
+  std::fill(rr_off.begin(), rr_off.end(), 0);
   for (int k = 0; k <= maxd - 2; ++k)
   {
     std::vector<int> sel;
     for (int i = 0; i < count; ++i)
       if (live[i] && k < ts[i]->d() - 1) sel.push_back(i);
     if (sel.empty()) continue;
-    if (mixed) sel = svd_step_certified(c, ts, sel, k, max_rank, deltas, hdelta);
+    if (mixed) sel = svd_step_certified(c, ts, sel, k, max_rank, deltas, hdelta, rr_off);
     if (!sel.empty()) svd_step_plain(c, ts, sel, k, max_rank, rk, deltas, flags);
   }
 

@@ -228,6 +228,7 @@ struct pchol_desc
   double* L = nullptr;
   double* Rinv = nullptr;
   int ldr = 0;
+  double* tr_out = nullptr;  // optional: trace of the Schur complement left at the stop
 };
 constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
 size_t pchol_lds_bytes(int q);

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
   }
   int p = 0;
   bool broke = false;
+  double tr_left = 0.0;
   for (int j = 0; j < q; ++j)
   {
     // Trace and largest diagonal entry of the trailing Schur complement.
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
       }
     }
     block_sum_argmax<NT>(tr, mv, mi, red);
+    tr_left = tr;
     int pj = j;
     if (d.pivot)
     {
@@ -218,7 +220,11 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
     if (tid == 0) *d.p_out = -1;
     return;
   }
-  if (tid == 0) *d.p_out = p;
+  if (tid == 0)
+  {
+    *d.p_out = p;
+    if (d.tr_out) *d.tr_out = p == q ? 0.0 : tr_left;
+  }
   if (d.piv)
     for (int i = tid; i < q; i += NT) d.piv[i] = ord[i];
   // Leading block of the factor in pivot order, packed: T(a, b) = L(ord_a, b).
