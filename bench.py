@@ -92,28 +92,110 @@ def measured_traffic(kernel: str):
     return None if rec is None else float(rec["bytes_per_launch"])
 
 
-def cpu_baseline(reps: int, y_dev_cores):
-    """Times the FP64 CPU oracle (single thread) on the same inputs; also
-    returns the relative error of the GPU result against the oracle run with
-    its extended-precision SVD accumulation."""
+def _harness(fn, iters):
+    """The reference benchmark harness (tests/bench/tt_bench_utils.hpp:38-67):
+    max(3, min(32, iters / 10)) untimed calls, then `iters` timed calls;
+    minimum and median in ms."""
+    warm = max(3, min(32, iters // 10))
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(iters):
+        t0 = time.perf_counter()
+        fn()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    ts.sort()
+    med = ts[len(ts) // 2] if len(ts) % 2 else 0.5 * (ts[len(ts) // 2 - 1] + ts[len(ts) // 2])
+    return {"warmup": warm, "iters": iters, "min_ms": ts[0], "median_ms": med}
+
+
+def _host_threads():
+    """Threads for the batched CPU baseline: the host cores this process may
+    run on, at most 16 (the CPU share of one GPU on the measuring hosts)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_leg(args):
+    """One build of the FP64 CPU oracle (TT_ORACLE_VARIANT selects -O2 or
+    -O3 -march=native) timed on the CPU only: the matvec_round workload with
+    the reference harness, and bounded samples of the two batch workloads
+    rounded by a std::thread pool.  Prints one JSON object."""
+    import numpy as np
+
+    import oracle as O
+
+    O.set_svd_extended(False)  # timed in FP64, the reference's arithmetic
+    A = O.OTT.random_matrix([4] * D, [4] * D, RA, 3)
+    cores = [c / np.sqrt(c.shape[0] * 16.0) for c in A.cores()]
+    A = O.OTT.from_cores(cores, [4] * D, [4] * D)
+    x = O.OTT.random([4] * D, RX, 4)
+    res = {"matvec_round": _harness(lambda: O.matvec_round(A, x, EPS, 0, 0), args.cpu_reps)}
+    thr = _host_threads()
+    rng = np.random.default_rng(2024)
+    # Config 4 sample: the same construction (left-orthogonal Q_k
+    # diag(10^(-j s)) cores from QR of Gaussian matrices, unit last core) with
+    # numpy's generator instead of the device's.
+    c4 = []
+    for t in range(args.cpu_batch_cfg4):
+        s = rng.uniform(0.05, 0.3)
+        ranks = [1] + [min(4 ** k, 4 ** (32 - k), 128) for k in range(1, 32)] + [1]
+        cs = []
+        for k in range(32):
+            G = rng.standard_normal((ranks[k] * 4, ranks[k + 1]))
+            if k < 31:
+                Q, _ = np.linalg.qr(G)
+                cs.append((Q * 10.0 ** (-s * np.arange(ranks[k + 1])))[:, :].reshape(ranks[k], 4, ranks[k + 1]))
+            else:
+                cs.append((G / np.linalg.norm(G)).reshape(ranks[k], 4, 1))
+        c4.append(O.OTT.from_cores(cs))
+    # Config 3 sample: add(u, add(scale(u, 0.5), v)) with the reference's
+    # random() (mt19937_64) for u and v, shapes drawn as in the GPU batch.
+    c3 = []
+    for t in range(args.cpu_batch_cfg3):
+        d = int(rng.integers(8, 41))
+        n = int(rng.choice([2, 4, 8]))
+        u = O.OTT.random([n] * d, int(rng.integers(8, 65)), 1000 + t)
+        v = O.OTT.random([n] * d, int(rng.integers(8, 65)), 501000 + t)
+        c3.append(O.add(u, O.add(O.scale(u, 0.5), v)))
+    for name, batch in (("config4", c4), ("config3", c3)):
+        O.round_batch(batch[:1], 1e-6, 0, threads=1)  # warm-up of one TT
+        t0 = time.perf_counter()
+        O.round_batch(batch, 1e-6, 0, threads=thr)
+        dt = time.perf_counter() - t0
+        res[name] = {"rounds_per_s": len(batch) / dt, "tts": len(batch), "threads": thr, "seconds": dt}
+    print(json.dumps(res))
+
+
+def cpu_baseline(args, y_dev_cores):
+    """The FP64 CPU oracle (the restated reference algorithm; the reference's
+    Eigen build cannot be compiled here) timed on this host in two builds, as
+    separate processes: -O2 without -march (the reference's flags) and
+    -O3 -march=native.  Also returns the relative error of the GPU result
+    against the oracle run with its extended-precision SVD accumulation."""
+    import subprocess
+
     import numpy as np
 
     import oracle as O
 
     if not os.path.exists(O._LIB_PATH):
         O.build()
+    legs = {}
+    for variant in ("o2", "native"):
+        env = dict(os.environ, TT_ORACLE_VARIANT="native" if variant == "native" else "")
+        cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", "--cpu-reps", str(args.cpu_reps),
+               "--cpu-batch-cfg3", str(args.cpu_batch_cfg3), "--cpu-batch-cfg4", str(args.cpu_batch_cfg4)]
+        print(f"[bench] CPU baseline ({variant}) ...", file=sys.stderr, flush=True)
+        r = subprocess.run(cmd, env=env, capture_output=True, text=True, check=True)
+        legs[variant] = json.loads(r.stdout.strip().splitlines()[-1])
     A = O.OTT.random_matrix([4] * D, [4] * D, RA, 3)
     cores = [c / np.sqrt(c.shape[0] * 16.0) for c in A.cores()]
     A = O.OTT.from_cores(cores, [4] * D, [4] * D)
     x = O.OTT.random([4] * D, RX, 4)
-    times = []
-    # Timed in FP64 (the reference's arithmetic); the accuracy check uses
-    # the oracle's extended-precision SVD accumulation.
-    O.set_svd_extended(False)
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        O.matvec_round(A, x, EPS, 0, 0)
-        times.append(time.perf_counter() - t0)
     O.set_svd_extended(True)
     yref = O.matvec_round(A, x, EPS, 0, 0)
     diff = O.axpby(1.0, O.OTT.from_cores(y_dev_cores), -1.0, yref)
@@ -121,13 +203,26 @@ def cpu_baseline(reps: int, y_dev_cores):
     r = yref.copy()
     O.right_orthogonalize(r)
     rel = float(np.linalg.norm(diff.cores()[0]) / np.linalg.norm(r.cores()[0]))
+    o2 = legs["o2"]["matvec_round"]
+    try:
+        nproc = os.cpu_count()
+    except Exception:  # noqa: BLE001
+        nproc = None
     return {
-        "value": reps / sum(times),
+        "value": 1e3 / o2["median_ms"],
         "unit": "TT-rounds/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{reps} full matvec_round(A, x, eps=1e-8) of the d=20 workload by the FP64 CPU oracle "
-                  f"(restated reference algorithm, g++ -O2, one thread); min {min(times) * 1e3:.0f} ms",
+        "sample": f"matvec_round(A, x, eps=1e-8) of the d=20 workload by the FP64 CPU oracle (restated reference "
+                  f"algorithm, g++ -O2, one thread), reference harness: {o2['warmup']} warm-up + {o2['iters']} "
+                  f"timed calls, median {o2['median_ms']:.0f} ms, min {o2['min_ms']:.0f} ms",
+        "host_nproc": nproc,
+        "o2": legs["o2"],
+        "o3_march_native": legs["native"],
+        "batched_note": "config 3 / config 4 samples rounded by a std::thread pool (one TT per thread at a time, "
+                        "threads = the host cores available, at most 16); inputs drawn with the same constructions "
+                        "on the CPU (numpy / mt19937_64) rather than the device generators; one warm-up TT, one "
+                        "timed pass",
     }, rel, yref.ranks
 
 
@@ -310,7 +405,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--cpu-reps", type=int, default=2)
+    ap.add_argument("--cpu-reps", type=int, default=2,
+                    help="timed matvec_round calls of the CPU baseline (after the harness's warm-up calls)")
+    ap.add_argument("--cpu-batch-cfg3", type=int, default=128)
+    ap.add_argument("--cpu-batch-cfg4", type=int, default=64)
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--prof-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -327,6 +426,9 @@ def main():
     args = ap.parse_args()
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+        return
+    if args.cpu_leg:
+        cpu_leg(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batched path would shard "
@@ -435,7 +537,7 @@ def main():
     if not args.no_batched:
         out["batched"] = batched_section(T, ctx, args)
     if not args.no_cpu:
-        cb, rel, yref_ranks = cpu_baseline(args.cpu_reps, y.cores)
+        cb, rel, yref_ranks = cpu_baseline(args, y.cores)
         out["cpu_baseline"] = cb
         out["rel_err_vs_cpu"] = rel
         out["ranks_match_cpu"] = yref_ranks == y.ranks

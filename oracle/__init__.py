@@ -20,7 +20,10 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libtt_oracle.so")
+# TT_ORACLE_VARIANT=native selects the -O3 -march=native build (the CPU
+# baseline's second line); the default is the -O2 build.
+_VARIANT = os.environ.get("TT_ORACLE_VARIANT", "")
+_LIB_PATH = os.path.join(_HERE, "_build", "libtt_oracle_native.so" if _VARIANT == "native" else "libtt_oracle.so")
 _lib = None
 
 _i32p = ctypes.POINTER(ctypes.c_int)
@@ -60,6 +63,9 @@ def lib():
         L.ora_scale.restype = _vp
         L.ora_round.argtypes = [_vp, ctypes.c_double, ctypes.c_int, ctypes.c_int]
         L.ora_round.restype = _vp
+        L.ora_round_batch.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(_vp)]
+        L.ora_round_batch.restype = None
         L.ora_right_orthogonalize.argtypes = [_vp]
         L.ora_left_orthogonalize.argtypes = [_vp]
         L.ora_matvec.argtypes = [_vp, _vp]
@@ -198,6 +204,16 @@ def scale(a, alpha):
 
 def round(a, eps=0.0, max_rank=0, gauge=0):  # noqa: A001 - mirrors the reference name
     return OTT(lib().ora_round(a.h, eps, max_rank, gauge), a.m, a.n)
+
+
+def round_batch(tts, eps=0.0, max_rank=0, threads=1):
+    """round(t, eps, max_rank) for every TT, on `threads` std::threads of the
+    oracle (one TT per thread at a time)."""
+    n = len(tts)
+    arr = (_vp * n)(*[t.h for t in tts])
+    outs = (_vp * n)()
+    lib().ora_round_batch(arr, n, eps, max_rank, threads, outs)
+    return [OTT(outs[i]) for i in range(n)]
 
 
 def right_orthogonalize(a):

@@ -31,7 +31,9 @@
 #include <cstring>
 #include <limits>
 #include <numeric>
+#include <atomic>
 #include <random>
+#include <thread>
 #include <vector>
 
 namespace ora
@@ -1603,6 +1605,22 @@ extern "C"
   {
     const tt& in = *static_cast<tt*>(a);
     return new tt(gauge == 1 ? ora::round_right_canonical(in, eps, max_rank) : ora::round_tt(in, eps, max_rank));
+  }
+  // The CPU baseline of the batched workloads: outs[i] = round(in[i], eps,
+  // max_rank) for count TTs on `threads` std::threads, each taking the next
+  // index from an atomic counter.  Every TT is rounded single-threaded, as
+  // the reference rounds one TT (it has no threads of its own).
+  void ora_round_batch(void* const* in, int count, double eps, int max_rank, int threads, void** outs)
+  {
+    std::atomic<int> next{0};
+    auto work = [&]() {
+      for (int i = next.fetch_add(1); i < count; i = next.fetch_add(1))
+        outs[i] = new tt(ora::round_tt(*static_cast<tt*>(in[i]), eps, max_rank));
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < threads; ++t) pool.emplace_back(work);
+    work();
+    for (auto& th : pool) th.join();
   }
   void ora_right_orthogonalize(void* a) { ora::right_orthogonalize(*static_cast<tt*>(a)); }
   void ora_left_orthogonalize(void* a) { ora::left_orthogonalize(*static_cast<tt*>(a)); }

@@ -441,3 +441,18 @@ def test_round_without_truncation_reproduces_input():
         r = O.round(Y, 0.0, 0, gauge)
         assert r.ranks[1:-1] == [4, 16, 64, 256, 64, 16, 4]
         assert rel_err(r.cores(), Y) <= 1e-13
+
+
+def test_round_batch_threads_equal_round():
+    """The threaded batch entry (the CPU baseline of the batched workloads)
+    returns exactly what round returns for each TT."""
+    tts = []
+    for t in range(6):
+        u = O.OTT.random([3] * 6, 5 + t, 70 + t)
+        tts.append(O.add(u, O.OTT.random([3] * 6, 4, 90 + t)))
+    outs = O.round_batch(tts, 1e-8, 0, threads=3)
+    for x, y in zip(tts, outs):
+        ref = O.round(x, 1e-8, 0, 0)
+        assert y.ranks == ref.ranks
+        for a, b in zip(y.cores(), ref.cores()):
+            assert np.array_equal(a, b)
