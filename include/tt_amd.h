@@ -151,6 +151,17 @@ int tta_matmat_round_iterative(tta_ctx* ctx, const tta_tt* A, const tta_tt* B, d
 /* inner(a, b) (include/core/algebra/tt_inner.hpp:33-48); frob_inner for
  * TT-matrices (tt_matrix_inner.hpp:31-36). */
 int tta_inner(tta_ctx* ctx, const tta_tt* a, const tta_tt* b, double* out);
+/* inner(a, b) as tta_inner, allocating nothing: no reference counterpart;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  The result goes to out, a device double the
+ * caller provides; the call only enqueues work on the context's stream (no
+ * host read, no synchronization), so the value is ready once the stream
+ * reaches it.  The environments live in a fixed library-owned device array of
+ * 4 Mi doubles (32 MiB per device, held until the process ends): operands
+ * needing more (2 max_k ra_k rb_k + max_k rb_k n_k ra_{k+1} doubles, each
+ * part rounded up to 32) are rejected with TTA_ERR_UNSUPPORTED.  Calls that
+ * overlap on one device share that array and race. */
+int tta_inner_noalloc(tta_ctx* ctx, const tta_tt* a, const tta_tt* b, double* out);
 /* norm(a) = sqrt(max(inner(a, a), 0)) (tt_inner.hpp:55-59; tt_matrix_inner.hpp:44-48). */
 int tta_norm(tta_ctx* ctx, const tta_tt* a, double* out);
 /* frob_norm_apply(A, x) / frob_norm_apply(A, B)

@@ -55,6 +55,7 @@ SIGNATURES = {
                                                   _i32p, _f64p, _i32p]),
     "tta_inner": (ctypes.c_int, [_vp, _vp, _vp, _f64p]),
     "tta_norm": (ctypes.c_int, [_vp, _vp, _f64p]),
+    "tta_inner_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "tta_frob_norm_apply": (ctypes.c_int, [_vp, _vp, _vp, _f64p]),
     "tta_axpby": (ctypes.c_int, [_vp, ctypes.c_double, _vp, ctypes.c_double, _vp, _vpp]),
     "tta_scale": (ctypes.c_int, [_vp, _vp, ctypes.c_double, _vpp]),

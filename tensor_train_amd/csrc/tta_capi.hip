@@ -2556,6 +2556,17 @@ int tta_ctx_create(int device, tta_ctx** out)
     return TTA_ERR_HIP;
   }
   c->h_dbl = reinterpret_cast<double*>(c->h_int + 8);
+  try
+  {
+    tta::noalloc_init(c);
+  }
+  catch (...)
+  {
+    (void)hipHostFree(c->h_int);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return TTA_ERR_HIP;
+  }
   *out = c;
   return TTA_OK;
 }

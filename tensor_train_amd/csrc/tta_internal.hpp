@@ -110,6 +110,9 @@ struct tta_ctx
   // SVD-sweep steps of the mixed batched rounding tried by the rank-revealing
   // CholQR2 certificate, and those it accepted.
   long long rr_tried = 0, rr_held = 0;
+  // Device address of the allocation-free entry point's scratch array on
+  // this context's device (looked up at context creation).
+  double* noalloc_scratch = nullptr;
   // Full-rank certificate of the SVD steps (on by default): steps where it
   // was tried, and where it held so that the Jacobi SVD was not needed.
   bool rank_certificate = true;
@@ -156,6 +159,9 @@ struct tta_tt
 namespace tta
 {
 double* ctx_ws(tta_ctx* c, int slot, size_t n);
+// Looks up the device address of tta_noalloc.hip's scratch array for the
+// current device (the context's) and keeps it in the context.
+void noalloc_init(tta_ctx* c);
 }
 
 namespace tta
