@@ -137,7 +137,6 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
   }
   const int nb = (int)idx.size();
   if (nb == 0) return;
-  if (max_q > 640) fail(TTA_ERR_UNSUPPORTED, "round_batched: bond rank %d above 640", max_q);
   {
     prof_scope ps(c, TTA_K_OTHER);
     if (!tr.empty()) launch_transpose_batched(S.upload(tr), (int)tr.size(), max_tr_r, max_tr_c, c->stream);

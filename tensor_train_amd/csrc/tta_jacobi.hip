@@ -449,6 +449,255 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
   if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
+// ---------------------------------------------------------------------------
+// Block Jacobi for q above the staged kernels' limit (the X and J columns of a
+// block pair no longer fit in LDS).  Blocks of kLW = 16 columns, round-robin
+// over block pairs as above; one round is three launches:
+//   bjl_gram_kernel  -- partial Grams X_p^T X_p of the pair's 32 columns over
+//                       256-row chunks (matrix cores, chunk staged in LDS),
+//   bjl_solve_kernel -- the chunks summed in order, then up to two cyclic
+//                       Jacobi sweeps on the 32 x 32 Gram in LDS, each rotation
+//                       computed by the one-sided rule from the Gram entries
+//                       (the same tolerance and negligible-column rule) and
+//                       applied as G <- R^T G R, accumulated into V;
+//   bjl_apply_kernel -- X_p <- X_p V and J_p <- J_p V by 256-row chunks on the
+//                       matrix cores (skipped where V = I).
+// The Gram is recomputed from the columns in every round, so the rotations'
+// inputs never carry updates from earlier rounds; a pair counts as rotated
+// when a rotation of its first inner sweep passed the tolerance.
+constexpr int kLW = 16;
+constexpr int kLP = 2 * kLW;
+constexpr int kLRC = 256;
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
+
+struct bjl_work
+{
+  double* gpart = nullptr;  // [problem][pair][chunk][kLP * kLP]
+  double* V = nullptr;      // [problem][pair][kLP * kLP]
+  int* pflag = nullptr;     // [problem][pair]: V != I
+  int pairs = 0, nch = 0;
+};
+
+// Global column of local column c (< kLP) of pair `pair` in round s, or -1
+// (padding block, or past q).
+__device__ __forceinline__ int bjl_col(int s, int pair, int nbb, int c, int q)
+{
+  int bi, bj;
+  rr_pair(s, pair, nbb, bi, bj);
+  const int col = (c < kLW ? bi : bj) * kLW + (c % kLW);
+  return col < q ? col : -1;
+}
+
+__global__ __launch_bounds__(256) void bjl_gram_kernel(jac_batch bt, bjl_work w, int s, int nbb)
+{
+  const int prob = blockIdx.z;
+  const jac_prob& P = jpick(bt, prob);
+  if (P.flag && P.flag[1]) return;
+  const int q = P.q, pair = blockIdx.y, ch = blockIdx.x;
+  const int r0 = ch * kLRC;
+  if (r0 >= q) return;
+  __shared__ double Xs[kLRC][kLP + 1];
+  __shared__ int cols[kLP];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  if (tid < kLP) cols[tid] = bjl_col(s, pair, nbb, tid, q);
+  __syncthreads();
+  for (int e = tid; e < kLRC * kLP; e += 256)
+  {
+    const int c = e / kLRC, r = e % kLRC;
+    Xs[r][c] = (cols[c] >= 0 && r0 + r < q) ? P.X[(size_t)cols[c] * P.ldx + r0 + r] : 0.0;
+  }
+  __syncthreads();
+  const int fl = lane & 15, fk = lane >> 4;
+  const int qi = wid >> 1, qj = wid & 1;
+  dbl4_t acc = {0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < kLRC; k0 += 4)
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[k0 + fk][qi * 16 + fl], Xs[k0 + fk][qj * 16 + fl], acc, 0, 0, 0);
+  double* G = w.gpart + ((size_t)(prob * w.pairs + pair) * w.nch + ch) * (kLP * kLP);
+#pragma unroll
+  for (int r = 0; r < 4; ++r) G[(qi * 16 + fk + 4 * r) * kLP + qj * 16 + fl] = acc[r];
+}
+
+__global__ __launch_bounds__(256) void bjl_solve_kernel(jac_batch bt, bjl_work w, int s, int nbb)
+{
+  const int prob = blockIdx.y;
+  const jac_prob& P = jpick(bt, prob);
+  const int pair = blockIdx.x;
+  int* pf = w.pflag + (size_t)prob * w.pairs + pair;
+  if (P.flag && P.flag[1])
+  {
+    if (threadIdx.x == 0) *pf = 0;
+    return;
+  }
+  const int q = P.q;
+  __shared__ double G[kLP][kLP + 1];
+  __shared__ double V[kLP][kLP + 1];
+  __shared__ double rc[kLP / 2], rs[kLP / 2];
+  __shared__ int rot_flag;
+  const int tid = threadIdx.x;
+  const int nch = (q + kLRC - 1) / kLRC;
+  const double* Gp = w.gpart + (size_t)(prob * w.pairs + pair) * w.nch * (kLP * kLP);
+  for (int e = tid; e < kLP * kLP; e += 256)
+  {
+    double acc = 0.0;
+    for (int ch = 0; ch < nch; ++ch) acc += Gp[(size_t)ch * (kLP * kLP) + e];
+    G[e / kLP][e % kLP] = acc;
+    V[e / kLP][e % kLP] = (e / kLP == e % kLP) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const double negl2 = jac_negl2(P, P.s[0]);
+  const double tol = jac_tol(q);
+  int any = 0, first = 0;  // V != I; a rotation in the first inner sweep
+  for (int inner = 0; inner < 2; ++inner)
+  {
+    if (tid == 0) rot_flag = 0;
+    __syncthreads();
+    int rotated = 0;  // thread-local, any pair of this inner sweep
+    for (int r2 = 0; r2 < kLP - 1; ++r2)
+    {
+      if (tid < kLP / 2)
+      {
+        int li, lj;
+        rr_pair(r2, tid, kLP, li, lj);
+        if (li > lj)
+        {
+          const int t = li;
+          li = lj;
+          lj = t;
+        }
+        double cs = 1.0, sn = 0.0;
+        const double a = G[li][li], b = G[lj][lj], g = G[li][lj];
+        if (!(a <= negl2 || b <= negl2) && jacobi_rotation(a, b, g, tol, cs, sn))
+          rotated = 1;
+        else
+        {
+          cs = 1.0;
+          sn = 0.0;
+        }
+        rc[tid] = cs;
+        rs[tid] = sn;
+      }
+      __syncthreads();
+      // Columns: G <- G R, V <- V R (pair = tid / 16, two rows per thread).
+      {
+        const int pr = tid >> 4, l16 = tid & 15;
+        int li, lj;
+        rr_pair(r2, pr, kLP, li, lj);
+        if (li > lj)
+        {
+          const int t = li;
+          li = lj;
+          lj = t;
+        }
+        const double cs = rc[pr], sn = rs[pr];
+        if (sn != 0.0)
+          for (int row = 2 * l16; row < 2 * l16 + 2; ++row)
+          {
+            const double x = G[row][li], y = G[row][lj];
+            G[row][li] = cs * x - sn * y;
+            G[row][lj] = sn * x + cs * y;
+            const double u = V[row][li], v = V[row][lj];
+            V[row][li] = cs * u - sn * v;
+            V[row][lj] = sn * u + cs * v;
+          }
+      }
+      __syncthreads();
+      // Rows: G <- R^T G.
+      {
+        const int pr = tid >> 4, l16 = tid & 15;
+        int li, lj;
+        rr_pair(r2, pr, kLP, li, lj);
+        if (li > lj)
+        {
+          const int t = li;
+          li = lj;
+          lj = t;
+        }
+        const double cs = rc[pr], sn = rs[pr];
+        if (sn != 0.0)
+          for (int col = 2 * l16; col < 2 * l16 + 2; ++col)
+          {
+            const double x = G[li][col], y = G[lj][col];
+            G[li][col] = cs * x - sn * y;
+            G[lj][col] = sn * x + cs * y;
+          }
+      }
+      __syncthreads();
+    }
+    if (rotated) atomicOr(&rot_flag, 1);
+    __syncthreads();
+    const int r = rot_flag;
+    __syncthreads();
+    if (!r) break;
+    any = 1;
+    if (inner == 0) first = 1;
+  }
+  double* Vg = w.V + (size_t)(prob * w.pairs + pair) * (kLP * kLP);
+  for (int e = tid; e < kLP * kLP; e += 256) Vg[e] = V[e / kLP][e % kLP];
+  if (tid == 0)
+  {
+    *pf = any;
+    if (first && P.flag) atomicOr(P.flag, 1);
+  }
+}
+
+__global__ __launch_bounds__(256) void bjl_apply_kernel(jac_batch bt, bjl_work w, int s, int nbb, int nchx)
+{
+  const int prob = blockIdx.z;
+  const jac_prob& P = jpick(bt, prob);
+  if (P.flag && P.flag[1]) return;
+  const int pair = blockIdx.y;
+  if (!w.pflag[(size_t)prob * w.pairs + pair]) return;
+  const bool isJ = (int)blockIdx.x >= nchx;
+  const int ch = isJ ? (int)blockIdx.x - nchx : (int)blockIdx.x;
+  const int q = P.q;
+  const int r0 = ch * kLRC;
+  if (r0 >= q) return;
+  double* M = isJ ? P.J : P.X;
+  const int ld = isJ ? P.ldj : P.ldx;
+  __shared__ double Xs[kLRC][kLP + 1];
+  __shared__ double Vs[kLP][kLP + 1];
+  __shared__ int cols[kLP];
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  const double* Vg = w.V + (size_t)(prob * w.pairs + pair) * (kLP * kLP);
+  for (int e = tid; e < kLP * kLP; e += 256) Vs[e / kLP][e % kLP] = Vg[e];
+  if (tid < kLP) cols[tid] = bjl_col(s, pair, nbb, tid, q);
+  __syncthreads();
+  for (int e = tid; e < kLRC * kLP; e += 256)
+  {
+    const int c = e / kLRC, r = e % kLRC;
+    Xs[r][c] = (cols[c] >= 0 && r0 + r < q) ? M[(size_t)cols[c] * ld + r0 + r] : 0.0;
+  }
+  __syncthreads();
+  const int fl = lane & 15, fk = lane >> 4;
+  // 32 output tiles of 16 x 16 (16 row tiles x 2 column tiles), 8 per wave.
+  dbl4_t acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+  {
+    const int tile = wid + 4 * t;
+    const int rb = (tile >> 1) * 16, cb = (tile & 1) * 16;
+    acc[t] = dbl4_t{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int k0 = 0; k0 < kLP; k0 += 4)
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(Xs[rb + fl][k0 + fk], Vs[k0 + fk][cb + fl], acc[t], 0, 0, 0);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 8; ++t)
+  {
+    const int tile = wid + 4 * t;
+    const int rb = (tile >> 1) * 16, cb = (tile & 1) * 16;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Xs[rb + fk + 4 * r][cb + fl] = acc[t][r];
+  }
+  __syncthreads();
+  for (int e = tid; e < kLRC * kLP; e += 256)
+  {
+    const int c = e / kLRC, r = e % kLRC;
+    if (cols[c] >= 0 && r0 + r < q) M[(size_t)cols[c] * ld + r0 + r] = Xs[r][c];
+  }
+}
+
 // J = I for every problem; flags cleared.
 __global__ void jac_init_kernel(jac_batch bt)
 {
@@ -552,28 +801,59 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     }
     if ((size_t)32 * 8 * max_q > (size_t)max_lds || max_q > 640)
     {
-      // Too tall to stage a block pair: single-workgroup Jacobi in global
-      // memory, which serves one problem at a time.
-      if (b.descs)
-        fail(TTA_ERR_UNSUPPORTED, "batched Jacobi SVD: %d columns exceed the staged block-pair limit", max_q);
-      for (int i = 0; i < batch; ++i)
+      // Too tall to stage a block pair: the Gram-based block rounds.
+      const int nb = (max_q + kLW - 1) / kLW;
+      const int nbb = (nb + 1) & ~1;
+      bjl_work w;
+      w.pairs = nbb / 2;
+      w.nch = (max_q + kLRC - 1) / kLRC;
+      const size_t ng = (size_t)batch * w.pairs * w.nch * kLP * kLP;
+      const size_t nv = (size_t)batch * w.pairs * kLP * kLP;
+      w.gpart = static_cast<double*>(c->mem.get(ng * sizeof(double)));
+      w.V = static_cast<double*>(c->mem.get(nv * sizeof(double)));
+      w.pflag = static_cast<int*>(c->mem.get((size_t)batch * w.pairs * sizeof(int)));
       {
-        jacobi_desc jd;
-        jd.X = b.one.X;
-        jd.ldx = b.one.ldx;
-        jd.p = b.one.q;
-        jd.q = b.one.q;
-        jd.J = b.one.J;
-        jd.ldj = b.one.ldj;
-        jd.s = b.one.s;
-        jd.perm = b.one.perm;
-        jd.rank_out = b.one.rank_out;
-        jd.delta_ptr = b.one.delta_ptr;
-        jd.delta = b.one.delta;
-        jd.max_rank = b.one.max_rank;
-        launch_jacobi(jd, 1, st);
+        long g = ((long)max_q * max_q + 255) / 256;
+        if (g > 1024) g = 1024;
+        hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+        hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
       }
-      return -1;
+      const size_t rec0 = c->prof_pending.size();
+      for (int sweep = 0; sweep < kMaxBlockSweeps; ++sweep)
+      {
+        sweeps_run = sweep + 1;
+        for (int r = 0; r < nbb - 1; ++r)
+        {
+          prof_scope ps(c, TTA_K_JACOBI_ROUND);
+          hipLaunchKernelGGL(bjl_gram_kernel, dim3(w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
+          hipLaunchKernelGGL(bjl_solve_kernel, dim3(w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
+          hipLaunchKernelGGL(bjl_apply_kernel, dim3(2 * w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb,
+                             w.nch);
+        }
+        (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
+        hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+        if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
+        {
+          (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+          (void)hipStreamSynchronize(st);
+          if (*done_host >= batch) break;
+          if (sweep == kMaxBlockSweeps - 1) c->jac_unconverged += batch - *done_host;
+        }
+      }
+      size_t nrec = 0;
+      for (size_t r = rec0; r < c->prof_pending.size(); ++r) nrec += c->prof_pending[r].cls == TTA_K_JACOBI_ROUND;
+      for (size_t r = rec0; r < c->prof_pending.size(); ++r)
+        if (c->prof_pending[r].cls == TTA_K_JACOBI_ROUND)
+        {
+          c->prof_pending[r].flops = credit_flops / (double)nrec;
+          c->prof_pending[r].bytes = credit_bytes / (double)nrec;
+        }
+      c->mem.put(w.gpart);
+      c->mem.put(w.V);
+      c->mem.put(w.pflag);
+      constexpr int NF = 256;
+      hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);
+      return sweeps_run;
     }
     // Batches of q <= 256 fill the chip with 256-thread workgroups, where the
     // grouped kernel's lower VALU count and deferred J win; few or larger
