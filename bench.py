@@ -328,6 +328,53 @@ def _three_paths(T, ctx, batch, mlp, eps, label):
     return res
 
 
+def qtt_section(T, ctx, args):
+    """The large single-instance QTT workload at full size: d = 50, n = 2, x
+    of rank 256 (Gaussian cores, left-orthogonalized, bond spectrum rescaled
+    to 0.9^j), MPO of rank 16 (N(0, 1/(4 r)) cores), plain FP64 matvec_round
+    and round at eps = 1e-10 (product ranks up to 4096).  One timed call each
+    (each is tens of seconds); with --qtt-prof a second, event-timed
+    matvec_round gives the time per kernel class."""
+    import numpy as np
+
+    d = 50
+    x = T.random([2] * d, 256, 11)
+    T.left_orthogonalize(x)
+    cores = x.cores
+    for k in range(d - 1):
+        cores[k] = cores[k] * (0.9 ** np.arange(cores[k].shape[2]))[None, None, :]
+    x = T.TensorTrain.from_cores(cores)
+    A = T.random_matrix([2] * d, [2] * d, 16, 12)
+    rk = A.ranks
+    for k in range(d):
+        T.scale_core(A, k, 1.0 / np.sqrt(rk[k] * 4.0))
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    y = T.matvec_round(A, x, eps=1e-10)
+    ctx.synchronize()
+    t_mvr = time.perf_counter() - t0
+    yfull = T.matvec(A, x)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    y2 = T.round(yfull, eps=1e-10)
+    ctx.synchronize()
+    t_round = time.perf_counter() - t0
+    out = {"workload": "QTT d=50, n=2, x rank 256 (0.9^j bond spectrum), MPO rank 16, eps=1e-10, plain FP64",
+           "matvec_round_s": t_mvr, "round_of_product_s": t_round, "product_max_rank": max(yfull.ranks),
+           "out_max_rank": max(y.ranks), "out_ranks_equal": y.ranks == y2.ranks}
+    del yfull, y2
+    if args.qtt_prof:
+        ctx.prof_enable(True)
+        y = T.matvec_round(A, x, eps=1e-10)
+        ctx.synchronize()
+        out["kernel_ms"] = {k: ctx.prof_read(k)["ms"] for k in ("qr", "jacobi", "gemm", "gemm_splitk", "chol",
+                                                                "chol_step", "apply", "other")}
+        out["jacobi_calls_sweeps"] = list(ctx.jacobi_sweeps())
+        ctx.prof_enable(False)
+    print(f"[bench] QTT d=50: matvec_round {t_mvr:.1f} s, round {t_round:.1f} s", file=sys.stderr, flush=True)
+    return out
+
+
 def batched_section(T, ctx, args):
     """Batched workloads (config 4 and config 3): plain FP64, mixed, and
     mixed + learned batched rounding, and for config 4 the randomized
@@ -413,6 +460,9 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-qtt", action="store_true", help="skip the full-size QTT (d=50, ranks 4096) workload")
+    ap.add_argument("--qtt-only", action="store_true")
+    ap.add_argument("--qtt-prof", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="skip the matvec_round workload and the CPU baseline (profiling the batched paths)")
     ap.add_argument("--cfg4-count", type=int, default=2048)
@@ -440,6 +490,9 @@ def main():
     ctx = T.Context.default(0)
     if args.batched_only:
         print(json.dumps({"batched": batched_section(T, ctx, args)}))
+        return
+    if args.qtt_only:
+        print(json.dumps({"qtt_full": qtt_section(T, ctx, args)}))
         return
     A, x = build_inputs(T)
     for _ in range(args.warmup):
@@ -536,6 +589,8 @@ def main():
     }
     if not args.no_batched:
         out["batched"] = batched_section(T, ctx, args)
+    if not args.no_qtt:
+        out["qtt_full"] = qtt_section(T, ctx, args)
     if not args.no_cpu:
         cb, rel, yref_ranks = cpu_baseline(args, y.cores)
         out["cpu_baseline"] = cb

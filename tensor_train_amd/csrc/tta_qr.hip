@@ -629,10 +629,11 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
   if (batch <= 0) return;
   const int kmax = b.max_m < b.max_n ? b.max_m : b.max_n;
   constexpr int NT = 512;
-  // Batches: 16-column matrix-core updates (enough workgroups anyway);
-  // a single matrix keeps the 8-column VALU update, whose twice as many
-  // workgroups matter more there than the per-column cost.
-  const bool mfma_update = batch >= 16;
+  // Batches and wide matrices: 16-column matrix-core updates (enough
+  // workgroups anyway); a single narrower matrix keeps the 8-column VALU
+  // update, whose twice as many workgroups matter more there than the
+  // per-column cost.
+  const bool mfma_update = batch >= 16 || b.max_n >= 512;
   for (int j0 = 0; j0 < kmax; j0 += QB)
   {
     // A single small matrix: one wave holding up to 4 rows per lane, so the
