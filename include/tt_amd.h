@@ -396,6 +396,15 @@ int tta_set_speculation(tta_ctx* ctx, int on);
 /* Speculative sweeps run and those undone, since the last tta_prof_enable. */
 int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
 
+/* ---- HIP graphs (no reference counterpart) ------------------------------ */
+/* Block-Jacobi SVD sweeps (more than 64 columns, single problems and
+ * batches) captured into a HIP graph at the first sweep of each call and
+ * replayed for the later sweeps, instead of enqueuing every round launch.
+ * Results are identical either way (the same launches run in the same order).
+ * Not used while per-launch event timing (tta_prof_enable) is on.  Off by
+ * default. */
+int tta_set_graphs(tta_ctx* ctx, int on);
+
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks
  * (tensor cores and workspaces), bytes cached for reuse, device allocations

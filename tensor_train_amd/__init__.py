@@ -113,6 +113,11 @@ class Context:
         falls back to Householder wherever it would lose accuracy)."""
         self.check(self.lib.tta_set_cholqr(self.h, 1 if on else 0))
 
+    def set_graphs(self, on: bool) -> None:
+        """Block-Jacobi sweeps captured into a HIP graph and replayed (off by
+        default; identical results)."""
+        self.check(self.lib.tta_set_graphs(self.h, 1 if on else 0))
+
     def set_speculation(self, on: bool) -> None:
         """Speculative sweeps (on by default): CholQR2 acceptance and the rank
         certificates decided on the device, one host read per sweep; a sweep

@@ -160,6 +160,16 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
   jac_batch jb;
   jb.descs = S.upload(jp);
   jb.batch = nb;
+  if (max_q <= 64)
+  {
+    // Whole-problem Jacobi kernels of uneven sizes: a persistent launch fed
+    // from a largest-first work queue.
+    std::vector<int> ord(nb);
+    for (int a = 0; a < nb; ++a) ord[a] = a;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return jp[x].q > jp[y].q; });
+    jb.order = S.upload(ord);
+    jb.queue = S.alloc<int>(1);
+  }
   int* done = S.alloc<int>(1);
   {
     const credit w = jacobi_credit(jp.data(), jp.size());
@@ -424,6 +434,16 @@ struct rr_out
   double* Rt = nullptr;  // in the caller's arena
 };
 
+// The work-queue order of a batched pivoted Cholesky: problems by decreasing
+// size q (work ~ q^3), so the largest start first and the small ones fill in.
+std::vector<int> largest_first(const std::vector<pchol_desc>& pc)
+{
+  std::vector<int> o(pc.size());
+  for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
+  std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return pc[a].q > pc[b].q; });
+  return o;
+}
+
 std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob>& pr)
 {
   const int nb = (int)pr.size();
@@ -470,9 +490,10 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
     launch_gemm_grouped(S.upload(g1), nb, max_q, max_q, c->stream);
   }
+  int* queue = S.alloc<int>(2);
   {
     prof_scope ps(c, TTA_K_CHOL);
-    launch_pchol_batch(S.upload(pc), nb, max_q, c->stream);
+    launch_pchol_batch(S.upload(pc), nb, max_q, S.upload(largest_first(pc)), queue, c->stream);
   }
   std::vector<int> hp(nb);
   std::vector<double> htr(nb);
@@ -599,7 +620,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   }
   {
     prof_scope ps(c, TTA_K_CHOL);
-    launch_pchol_batch(S.upload(pc2), nc, mx_p, c->stream);
+    launch_pchol_batch(S.upload(pc2), nc, mx_p, S.upload(largest_first(pc2)), queue + 1, c->stream);
   }
   {
     credit w;

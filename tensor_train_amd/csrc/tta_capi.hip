@@ -3035,6 +3035,13 @@ int tta_prof_jacobi_unconverged(tta_ctx* c, long long* problems)
   return TTA_OK;
 }
 
+int tta_set_graphs(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->graphs = on != 0;
+  return TTA_OK;
+}
+
 int tta_prof_rank_revealing(tta_ctx* c, long long* tried, long long* held)
 {
   if (!c) return TTA_ERR_ARG;

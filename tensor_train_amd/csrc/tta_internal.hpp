@@ -110,6 +110,8 @@ struct tta_ctx
   // SVD-sweep steps of the mixed batched rounding tried by the rank-revealing
   // CholQR2 certificate, and those it accepted.
   long long rr_tried = 0, rr_held = 0;
+  // Block-Jacobi sweeps captured in a HIP graph and replayed (tta_set_graphs).
+  bool graphs = false;
   // Device address of the allocation-free entry point's scratch array on
   // this context's device (looked up at context creation).
   double* noalloc_scratch = nullptr;

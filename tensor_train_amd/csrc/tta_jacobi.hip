@@ -23,6 +23,8 @@
 // every one of them; rotating them only stirs rounding noise between
 // columns that are discarded, which costs sweeps.  delta = 0 (no truncation,
 // full SVD) disables the rule.
+#include <algorithm>
+
 #include "../../include/tt_amd.h"
 #include "tta_device.hpp"
 #include "tta_internal.hpp"
@@ -147,9 +149,8 @@ __device__ __forceinline__ void apply_rotations(double (&jr)[2 * W], const doubl
 
 // Whole Jacobi in one workgroup with X, J in LDS (q <= 64).
 template <int NT>
-__global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_sweeps)
+__device__ void jacobi_small_one(const jac_prob& P, int max_sweeps)
 {
-  const jac_prob& P = jpick(bt, blockIdx.x);
   const int q = P.q;
   extern __shared__ double lds[];
   double* Xs = lds;          // q*q
@@ -206,6 +207,31 @@ __global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_
   }
   if (tid == 0 && P.sweeps) *P.sweeps = sweep;
 }
+
+// One problem per workgroup, or, with a work queue, a persistent grid whose
+// workgroups take problems in bt.order (largest first) until none is left.
+template <int NT>
+__global__ __launch_bounds__(NT) void jacobi_small_kernel(jac_batch bt, int max_sweeps)
+{
+  if (!bt.queue)
+  {
+    jacobi_small_one<NT>(jpick(bt, blockIdx.x), max_sweeps);
+    return;
+  }
+  __shared__ int item;
+  for (;;)
+  {
+    if (threadIdx.x == 0) item = atomicAdd(bt.queue, 1);
+    __syncthreads();
+    const int it = item;
+    __syncthreads();
+    if (it >= bt.batch) break;
+    jacobi_small_one<NT>(jpick(bt, bt.order[it]), max_sweeps);
+    __syncthreads();
+  }
+}
+
+__global__ void jac_zero_int_kernel(int* p) { *p = 0; }
 
 // Block pair of bj_group_kernel: block pair (bi, bj) of round s
 // of the round-robin over nbb (even) blocks of W columns; a block paired with
@@ -772,6 +798,42 @@ int jacobi_block_threshold() { return 64; }
 
 constexpr int kMaxBlockSweeps = 30;
 
+// One block sweep's launches captured into a HIP graph on first use and
+// replayed for every later sweep of the same call (the launches and their
+// arguments are the same in every sweep).  Off, or with per-launch event
+// timing on, the launches are enqueued directly.  The stream is drained
+// before the graph is destroyed (every sweep loop ends on a stream wait).
+struct sweep_graph
+{
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  sweep_graph() = default;
+  sweep_graph(const sweep_graph&) = delete;
+  sweep_graph& operator=(const sweep_graph&) = delete;
+  ~sweep_graph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
+  template <class F>
+  void run(hipStream_t st, bool on, F&& enqueue)
+  {
+    if (!on)
+    {
+      enqueue();
+      return;
+    }
+    if (!exec)
+    {
+      hip_check(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+      enqueue();
+      hip_check(hipStreamEndCapture(st, &graph), "hipStreamEndCapture");
+      hip_check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+    }
+    hip_check(hipGraphLaunch(exec, st), "hipGraphLaunch");
+  }
+};
+
 int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_host, tta_ctx* c, double credit_flops,
                       double credit_bytes)
 {
@@ -783,7 +845,24 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
   {
     constexpr int NT = 512;
     const size_t lds = 2 * sizeof(double) * (size_t)max_q * max_q;
-    hipLaunchKernelGGL(jacobi_small_kernel<NT>, dim3(batch), dim3(NT), lds, st, b, 60);
+    if (b.descs && b.order && b.queue)
+    {
+      // Persistent: resident workgroups by LDS (at most 4 of 512 threads per
+      // compute unit), fed from the work queue.
+      static int cus = 0;
+      if (cus == 0)
+      {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+          cus = 256;
+      }
+      const int per_cu = std::max(1, std::min(4, (int)((size_t)(160 * 1024) / std::max<size_t>(lds, 1))));
+      hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, b.queue);
+      hipLaunchKernelGGL(jacobi_small_kernel<NT>, dim3(std::min(batch, cus * per_cu)), dim3(NT), lds, st, b, 60);
+    }
+    else
+      hipLaunchKernelGGL(jacobi_small_kernel<NT>, dim3(batch), dim3(NT), lds, st, b, 60);
   }
   else
   {
@@ -819,19 +898,23 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
         hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
       }
       const size_t rec0 = c->prof_pending.size();
+      sweep_graph sg;
+      const bool use_graph = c->graphs && !c->prof;
       for (int sweep = 0; sweep < kMaxBlockSweeps; ++sweep)
       {
         sweeps_run = sweep + 1;
-        for (int r = 0; r < nbb - 1; ++r)
-        {
-          prof_scope ps(c, TTA_K_JACOBI_ROUND);
-          hipLaunchKernelGGL(bjl_gram_kernel, dim3(w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
-          hipLaunchKernelGGL(bjl_solve_kernel, dim3(w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
-          hipLaunchKernelGGL(bjl_apply_kernel, dim3(2 * w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb,
-                             w.nch);
-        }
-        (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
-        hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+        sg.run(st, use_graph, [&]() {
+          for (int r = 0; r < nbb - 1; ++r)
+          {
+            prof_scope ps(c, TTA_K_JACOBI_ROUND);
+            hipLaunchKernelGGL(bjl_gram_kernel, dim3(w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
+            hipLaunchKernelGGL(bjl_solve_kernel, dim3(w.pairs, batch), dim3(256), 0, st, b, w, r, nbb);
+            hipLaunchKernelGGL(bjl_apply_kernel, dim3(2 * w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb,
+                               w.nch);
+          }
+          hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
+          hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+        });
         if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
         {
           (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);
@@ -874,30 +957,34 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
     }
     const size_t rec0 = c->prof_pending.size();
+    sweep_graph sg;
+    const bool use_graph = c->graphs && !c->prof;
     for (int sweep = 0; sweep < kMaxBlockSweeps; ++sweep)
     {
       sweeps_run = sweep + 1;
-      for (int s = 0; s < nbb - 1; ++s)
-      {
-        const dim3 grid(nbb / 2, batch);
-        prof_scope ps(c, TTA_K_JACOBI_ROUND);
-        if (grouped && max_q <= 128)
-          hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
-        else if (grouped)
-          hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
-        else if (narrow)
-          hipLaunchKernelGGL((bj_round_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb, 1);
-        else if (W == 16 && max_q <= 128)
-          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
-        else if (W == 16 && max_q <= 256)
-          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1);
-        else if (W == 16)
-          hipLaunchKernelGGL((bj_round_kernel<NT, 16, 5>), grid, dim3(NT), lds, st, b, s, nbb, 1);
-        else
-          hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1);
-      }
-      (void)hipMemsetAsync(done_dev, 0, sizeof(int), st);
-      hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+      sg.run(st, use_graph, [&]() {
+        for (int s = 0; s < nbb - 1; ++s)
+        {
+          const dim3 grid(nbb / 2, batch);
+          prof_scope ps(c, TTA_K_JACOBI_ROUND);
+          if (grouped && max_q <= 128)
+            hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
+          else if (grouped)
+            hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
+          else if (narrow)
+            hipLaunchKernelGGL((bj_round_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb, 1);
+          else if (W == 16 && max_q <= 128)
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+          else if (W == 16 && max_q <= 256)
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+          else if (W == 16)
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 5>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+          else
+            hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+        }
+        hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
+        hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+      });
       if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
       {
         (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);

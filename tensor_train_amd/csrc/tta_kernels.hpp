@@ -127,6 +127,11 @@ struct jac_batch
   const jac_prob* descs = nullptr;
   jac_prob one;
   int batch = 1;
+  // Optional work queue for the whole-problem kernel (q <= 64): the problems
+  // in the order they are taken (device, `batch` ints, largest first) and one
+  // device int of scratch; with both set the launch is persistent.
+  const int* order = nullptr;
+  int* queue = nullptr;
 };
 
 // Descriptor-batched helpers (one entry per problem, device arrays).
@@ -232,7 +237,10 @@ struct pchol_desc
 };
 constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
 size_t pchol_lds_bytes(int q);
-void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, hipStream_t st);
+// Persistent launch fed from a work queue: `order` (count device ints) lists
+// the problems in the order they are taken (largest first); `queue` is one
+// device int of scratch (zeroed by the launch).
+void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st);
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // Split-K form of a single GEMM for shapes with few output tiles:

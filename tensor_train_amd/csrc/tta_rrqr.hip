@@ -25,6 +25,8 @@
 #include "tta_internal.hpp"
 #include "tta_kernels.hpp"
 
+#include <algorithm>
+
 namespace tta
 {
 using namespace dev;
@@ -85,9 +87,8 @@ __device__ __forceinline__ void unpack(int t, int& i, int& l)
 }
 
 template <int NT>
-__global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
+__device__ void pchol_one(const pchol_desc& d)
 {
-  const pchol_desc& d = ds[blockIdx.x];
   const int q = d.q;
   const int tid = threadIdx.x;
   extern __shared__ double sm[];
@@ -261,6 +262,27 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds)
     d.Rinv[(size_t)a * d.ldr + b] = b >= a ? S[pk(b, a)] : 0.0;
   }
 }
+// Persistent form: a grid sized to the device's resident capacity; each
+// workgroup takes the next problem of `order` (the problems by decreasing
+// size, so the largest start first) from the work-queue counter *queue,
+// which the launch wrapper zeroes beforehand, until the queue is empty.
+template <int NT>
+__global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds, int count, const int* order, int* queue)
+{
+  __shared__ int item;
+  for (;;)
+  {
+    if (threadIdx.x == 0) item = atomicAdd(queue, 1);
+    __syncthreads();
+    const int it = item;
+    __syncthreads();
+    if (it >= count) break;
+    pchol_one<NT>(ds[order[it]]);
+    __syncthreads();
+  }
+}
+
+__global__ void zero_int_kernel(int* p) { *p = 0; }
 }  // namespace
 
 size_t pchol_lds_bytes(int q)
@@ -269,11 +291,26 @@ size_t pchol_lds_bytes(int q)
   return sizeof(double) * ((size_t)q * (q + 1) / 2 + q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
 }
 
-void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, hipStream_t st)
+void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st)
 {
   if (count <= 0) return;
   if (max_q > kPcholMaxQ) fail(TTA_ERR_UNSUPPORTED, "pivoted Cholesky: %d columns above %d", max_q, kPcholMaxQ);
-  hipLaunchKernelGGL(pchol_batch_kernel<256>, dim3(count), dim3(256), pchol_lds_bytes(max_q), st, ds);
+  const size_t lds = pchol_lds_bytes(max_q);
+  static int cus = 0, lds_cu = 0;
+  if (cus == 0)
+  {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess ||
+        lds_cu <= 0)
+      lds_cu = 160 * 1024;
+  }
+  // Resident workgroups: by LDS, at most 8 of 256 threads per compute unit.
+  const int per_cu = std::max(1, std::min(8, (int)((size_t)lds_cu / std::max<size_t>(lds, 1))));
+  const int grid = std::min(count, cus * per_cu);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, queue);
+  hipLaunchKernelGGL(pchol_batch_kernel<256>, dim3(grid), dim3(256), lds, st, ds, count, order, queue);
 }
 
 }  // namespace tta
