@@ -282,9 +282,14 @@ def _three_paths(T, ctx, batch, mlp, eps, label):
     import numpy as np
 
     n = len(batch)
-    T.round_batched(batch[:16], eps=eps)
-    T.round_batched(batch[:16], eps=eps, precision="mixed")
-    T.round_learned_batched(mlp, batch[:16], eps, oversample=6, slack=0.1, seed=3)
+    # One untimed pass of each path over the whole batch: allocator segments
+    # and the GEMM tile autotuner's per-shape decisions are in place before
+    # the timed passes, as in a long-running process.
+    for _ in range(2):
+        T.round_batched(batch, eps=eps)
+        T.round_batched(batch, eps=eps, precision="mixed")
+        T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3)
+    ctx.synchronize()
     plain, t_plain = _timed(ctx, lambda: T.round_batched(batch, eps=eps))
     mixed, t_mixed = _timed(ctx, lambda: T.round_batched(batch, eps=eps, precision="mixed"))
     (learned, err, fb, ranks), t_learned = _timed(
@@ -398,7 +403,7 @@ def batched_section(T, ctx, args):
     # ranks as targets (no truncation but the final projection's), same
     # fallback rule.
     stored = [t.ranks for t in batch]
-    T.round_randomized_batched(batch[:16], stored[:16], eps, seed=3)
+    T.round_randomized_batched(batch, stored, eps, seed=3)
     (nolearn, err0, fb0), t0s = _timed(ctx, lambda: T.round_randomized_batched(batch, stored, eps, seed=3))
     r4["randomized_stored_ranks_rounds_per_s"] = n4 / t0s
     r4["randomized_stored_ranks_fallback_fraction"] = float(fb0.mean())

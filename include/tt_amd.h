@@ -405,6 +405,17 @@ int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
  * default. */
 int tta_set_graphs(tta_ctx* ctx, int on);
 
+/* ---- autotuner (no reference counterpart) --------------------------------- */
+/* The grouped FP64 GEMMs of the batched paths choose their tile size (64 x 64
+ * or 32 x 32 outputs per workgroup) per shape class (device, log2 of the
+ * largest M, of the largest N, of the problem count) by timing one launch
+ * of each on the first two launches of the class and keeping the faster per
+ * unit of work.  Both tiles give identical results.  The table is per
+ * process; on by default (off: the 64 x 64 tile always). */
+int tta_set_autotune(tta_ctx* ctx, int on);
+/* Shape classes seen, decided, and decided for the 32 x 32 tile. */
+int tta_prof_autotune(tta_ctx* ctx, long long* keys, long long* decided, long long* chose_small);
+
 /* ---- device memory pool --------------------------------------------------- */
 /* Statistics of the context's caching allocator: bytes held by live blocks
  * (tensor cores and workspaces), bytes cached for reuse, device allocations

@@ -113,6 +113,17 @@ class Context:
         falls back to Householder wherever it would lose accuracy)."""
         self.check(self.lib.tta_set_cholqr(self.h, 1 if on else 0))
 
+    def set_autotune(self, on: bool) -> None:
+        """Tile-size autotuner of the grouped GEMMs (on by default; identical
+        results either way)."""
+        self.check(self.lib.tta_set_autotune(self.h, 1 if on else 0))
+
+    def autotune_stats(self):
+        """(shape classes seen, decided, decided for the 32 x 32 tile)."""
+        v = [ctypes.c_longlong() for _ in range(3)]
+        self.check(self.lib.tta_prof_autotune(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
     def set_graphs(self, on: bool) -> None:
         """Block-Jacobi sweeps captured into a HIP graph and replayed (off by
         default; identical results)."""

@@ -3035,6 +3035,20 @@ int tta_prof_jacobi_unconverged(tta_ctx* c, long long* problems)
   return TTA_OK;
 }
 
+int tta_set_autotune(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  tta::set_gemm_autotune(on != 0);
+  return TTA_OK;
+}
+
+int tta_prof_autotune(tta_ctx* c, long long* keys, long long* decided, long long* chose_small)
+{
+  if (!c) return TTA_ERR_ARG;
+  tta::gemm_autotune_stats(keys, decided, chose_small);
+  return TTA_OK;
+}
+
 int tta_set_graphs(tta_ctx* c, int on)
 {
   if (!c) return TTA_ERR_ARG;

@@ -1,6 +1,9 @@
 // HIP kernels (gfx950) for the FP64 plain path of TT rounding and MPO
 // apply-and-round.  Launch wrappers are declared in tta_kernels.hpp.
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 
 #include "tta_device.hpp"
 #include "tta_kernels.hpp"
@@ -26,9 +29,11 @@ namespace
 constexpr int GT = 64, GK = 32;
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// Element e (0..7) of this thread's share of a K slab: A(i0 + ii, k0 + kk)
-// and B(k0 + kk, j0 + jj), with the thread order matched to the operand's
-// contiguous direction so that loads coalesce.
+// Element e (0..7, or 0..3 for 32-wide tiles) of this thread's share of a K
+// slab of a TT-wide tile: A(i0 + ii, k0 + kk) and B(k0 + kk, j0 + jj), with
+// the thread order matched to the operand's contiguous direction so that
+// loads coalesce.
+template <int TT = GT>
 __device__ __forceinline__ void slab_coords_a(bool kc, int idx, int& ii, int& kk)
 {
   if (kc)
@@ -38,16 +43,17 @@ __device__ __forceinline__ void slab_coords_a(bool kc, int idx, int& ii, int& kk
   }
   else
   {
-    ii = idx & 63;
-    kk = idx >> 6;
+    ii = idx & (TT - 1);
+    kk = idx / TT;
   }
 }
+template <int TT = GT>
 __device__ __forceinline__ void slab_coords_b(bool jc, int idx, int& kk, int& jj)
 {
   if (jc)
   {
-    kk = idx >> 6;
-    jj = idx & 63;
+    kk = idx / TT;
+    jj = idx & (TT - 1);
   }
   else
   {
@@ -77,41 +83,46 @@ struct mfma_step<float>
   __device__ static int row(int fk, int r) { return 4 * fk + r; }
 };
 
-// The 64 x 64 tile at (i0, j0) of C = alpha A B + beta C with A of type TA,
-// B of type TB, C of type TC, multiplied on the matrix cores in CT (operands
-// converted on the way into LDS).  Every use of LDS is followed by a
-// barrier, so a workgroup may run several in a row.
-template <class TA, class TB, class TC, class CT>
+// The TT x TT tile (TT = 64, or 32 for small outputs) at (i0, j0) of
+// C = alpha A B + beta C with A of type TA, B of type TB, C of type TC,
+// multiplied on the matrix cores in CT (operands converted on the way into
+// LDS).  Each wave owns a (TT/2)^2 quarter as blocks of 16 x 16.  Every
+// output element accumulates the same sequence of 4-deep matrix-core steps
+// over K whatever TT is, so both tile sizes give identical results.  Every
+// use of LDS is followed by a barrier, so a workgroup may run several in a
+// row.
+template <class TA, class TB, class TC, class CT, int TT = GT>
 __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, long sar, long sac, const TB* B, long sbr,
                                             long sbc, TC* C, long scr, long scc, double alpha, double beta, int i0,
                                             int j0)
 {
   using S = mfma_step<CT>;
-  __shared__ CT As[GK][GT + 1];
-  __shared__ CT Bs[GK][GT + 1];
+  constexpr int WS = TT / 2, NB = WS / 16, NL = GK * TT / 256;
+  __shared__ CT As[GK][TT + 1];
+  __shared__ CT Bs[GK][TT + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wr = wid >> 1, wc = wid & 1;
   const bool a_kc = (sac == 1);  // A contiguous along k
   const bool b_jc = (sbc == 1);  // B contiguous along j
-  typename S::acc_t acc[2][2];
+  typename S::acc_t acc[NB][NB];
 #pragma unroll
-  for (int u = 0; u < 2; ++u)
+  for (int u = 0; u < NB; ++u)
 #pragma unroll
-    for (int v = 0; v < 2; ++v) acc[u][v] = typename S::acc_t{0, 0, 0, 0};
+    for (int v = 0; v < NB; ++v) acc[u][v] = typename S::acc_t{0, 0, 0, 0};
   const int fl = lane & 15, fk = lane >> 4;
   // Registers holding the next slab while the current one is multiplied.
-  CT pa[8], pb[8];
+  CT pa[NL], pb[NL];
   auto fetch = [&](int k0) {
 #pragma unroll
-    for (int l = 0; l < 8; ++l)
+    for (int l = 0; l < NL; ++l)
     {
       const int idx = tid + 256 * l;
       int ii, kk, jj, kb;
-      slab_coords_a(a_kc, idx, ii, kk);
+      slab_coords_a<TT>(a_kc, idx, ii, kk);
       const int gi = i0 + ii, gk = k0 + kk;
       pa[l] = (gi < M && gk < K) ? (CT)A[(size_t)gi * sar + (size_t)gk * sac] : (CT)0;
-      slab_coords_b(b_jc, idx, kb, jj);
+      slab_coords_b<TT>(b_jc, idx, kb, jj);
       const int gj = j0 + jj, gk2 = k0 + kb;
       pb[l] = (gj < N && gk2 < K) ? (CT)B[(size_t)gk2 * sbr + (size_t)gj * sbc] : (CT)0;
     }
@@ -120,13 +131,13 @@ __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, lo
   for (int k0 = 0; k0 < K; k0 += GK)
   {
 #pragma unroll
-    for (int l = 0; l < 8; ++l)
+    for (int l = 0; l < NL; ++l)
     {
       const int idx = tid + 256 * l;
       int ii, kk, jj, kb;
-      slab_coords_a(a_kc, idx, ii, kk);
+      slab_coords_a<TT>(a_kc, idx, ii, kk);
       As[kk][ii] = pa[l];
-      slab_coords_b(b_jc, idx, kb, jj);
+      slab_coords_b<TT>(b_jc, idx, kb, jj);
       Bs[kb][jj] = pb[l];
     }
     __syncthreads();
@@ -134,29 +145,29 @@ __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, lo
 #pragma unroll
     for (int ks = 0; ks < GK; ks += 4)
     {
-      CT a[2], b[2];
+      CT a[NB], b[NB];
 #pragma unroll
-      for (int u = 0; u < 2; ++u) a[u] = As[ks + fk][wr * 32 + u * 16 + fl];
+      for (int u = 0; u < NB; ++u) a[u] = As[ks + fk][wr * WS + u * 16 + fl];
 #pragma unroll
-      for (int v = 0; v < 2; ++v) b[v] = Bs[ks + fk][wc * 32 + v * 16 + fl];
+      for (int v = 0; v < NB; ++v) b[v] = Bs[ks + fk][wc * WS + v * 16 + fl];
 #pragma unroll
-      for (int u = 0; u < 2; ++u)
+      for (int u = 0; u < NB; ++u)
 #pragma unroll
-        for (int v = 0; v < 2; ++v) acc[u][v] = S::run(a[u], b[v], acc[u][v]);
+        for (int v = 0; v < NB; ++v) acc[u][v] = S::run(a[u], b[v], acc[u][v]);
     }
     __syncthreads();
   }
 #pragma unroll
-  for (int u = 0; u < 2; ++u)
+  for (int u = 0; u < NB; ++u)
 #pragma unroll
-    for (int v = 0; v < 2; ++v)
+    for (int v = 0; v < NB; ++v)
     {
-      const int gj = j0 + wc * 32 + v * 16 + fl;
+      const int gj = j0 + wc * WS + v * 16 + fl;
       if (gj >= N) continue;
 #pragma unroll
       for (int r = 0; r < 4; ++r)
       {
-        const int gi = i0 + wr * 32 + u * 16 + S::row(fk, r);
+        const int gi = i0 + wr * WS + u * 16 + S::row(fk, r);
         if (gi >= M) continue;
         TC* c = C + (size_t)gi * scr + (size_t)gj * scc;
         const double x = (double)acc[u][v][r];
@@ -352,12 +363,14 @@ __global__ __launch_bounds__(256) void chol_step_kernel(chol_step s)
 }
 
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
-// M x N exit.
+// M x N exit.  TT: the tile size (64, or 32 for small outputs).
+template <int TT>
 __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* gs)
 {
   const gemm_desc& g = gs[blockIdx.z];
-  if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * GT >= g.M || (int)blockIdx.x * GT >= g.N) return;
-  gemm_tile(g, g.A, g.B, g.C);
+  if (g.M <= 0 || g.N <= 0 || (int)blockIdx.y * TT >= g.M || (int)blockIdx.x * TT >= g.N) return;
+  gemm_tile_t<double, double, double, double, TT>(g.M, g.N, g.K, g.A, g.sar, g.sac, g.B, g.sbr, g.sbc, g.C, g.scr,
+                                                  g.scc, g.alpha, g.beta, blockIdx.y * TT, blockIdx.x * TT);
 }
 
 // Grouped GEMM with per-operand precision (xgemm_desc.types): A, B and C
@@ -1442,11 +1455,128 @@ void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int
     hipLaunchKernelGGL((xgemm_grouped_kernel<double, double, double, double>), grid, dim3(256), 0, st, descs_dev);
 }
 
+// ---------------------------------------------------------------------------
+// Autotuner of the grouped GEMM's tile size.  The 64 x 64 and 32 x 32 tiles
+// give identical results (see gemm_tile_t), so the choice only moves time:
+// small outputs waste most of a 64 x 64 tile, large ones lose operand reuse
+// with 32 x 32.  Launches are keyed by (device, log2 of max M, of max N, of
+// the problem count); the first launches of a key alternate the two tiles,
+// each between two events, until each has two timings (read without
+// waiting, at later launches of the key); the key then keeps the 32 tile
+// when its best time per unit of tile work (count x max M x max N) is at
+// least 10 % below the 64 tile's.
+// The table is per process and guarded by a mutex.
+// ---------------------------------------------------------------------------
+namespace
+{
+struct tune_entry
+{
+  int trials[2] = {0, 0};
+  double ms_per_work[2] = {0.0, 0.0};
+  bool pending[2] = {false, false};
+  double pending_work[2] = {0.0, 0.0};
+  hipEvent_t ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+  int choice = -1;
+};
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, tune_entry> g_tune;
+bool g_tune_on = true;
+constexpr int kTuneTrials = 2;  // timed launches per tile; the faster minimum wins
+
+int log2_bucket(long v)
+{
+  int b = 0;
+  while ((1L << b) < v) ++b;
+  return b;
+}
+}  // namespace
+
+void set_gemm_autotune(bool on)
+{
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_on = on;
+}
+
+void gemm_autotune_stats(long long* keys, long long* decided, long long* chose_small)
+{
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  long long k = 0, d = 0, sm = 0;
+  for (const auto& kv : g_tune)
+  {
+    ++k;
+    if (kv.second.choice >= 0) ++d;
+    if (kv.second.choice == 1) ++sm;
+  }
+  if (keys) *keys = k;
+  if (decided) *decided = d;
+  if (chose_small) *chose_small = sm;
+}
+
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st)
 {
   if (count <= 0 || max_m <= 0 || max_n <= 0) return;
-  dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, count);
-  hipLaunchKernelGGL(gemm_f64_grouped_kernel, grid, dim3(256), 0, st, descs_dev);
+  int variant = 0;
+  bool timed = false;
+  tune_entry* e = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (g_tune_on)
+    {
+      int dev = 0;
+      (void)hipGetDevice(&dev);
+      e = &g_tune[{dev, log2_bucket(max_m), log2_bucket(max_n), log2_bucket(count)}];
+      for (int v = 0; v < 2; ++v)
+        if (e->pending[v] && hipEventQuery(e->ev[v][1]) == hipSuccess)
+        {
+          float ms = 0.0f;
+          if (hipEventElapsedTime(&ms, e->ev[v][0], e->ev[v][1]) == hipSuccess)
+          {
+            const double t = ms / e->pending_work[v];
+            e->ms_per_work[v] = e->trials[v] == 0 ? t : std::min(e->ms_per_work[v], t);
+            e->trials[v] += 1;
+          }
+          e->pending[v] = false;
+        }
+      if (e->choice < 0 && e->trials[0] >= kTuneTrials && e->trials[1] >= kTuneTrials)
+        e->choice = e->ms_per_work[1] < 0.9 * e->ms_per_work[0] ? 1 : 0;
+      if (e->choice >= 0)
+        variant = e->choice;
+      else
+      {
+        // Alternate the two tiles until each has kTuneTrials timings.
+        const int n0 = e->trials[0] + (e->pending[0] ? 1 : 0), n1 = e->trials[1] + (e->pending[1] ? 1 : 0);
+        const int want = n0 <= n1 ? 0 : 1;
+        variant = want;
+        timed = !e->pending[want] && (want == 0 ? n0 : n1) < kTuneTrials;
+      }
+      if (timed)
+      {
+        for (int t = 0; t < 2; ++t)
+          if (!e->ev[variant][t] && hipEventCreate(&e->ev[variant][t]) != hipSuccess) timed = false;
+        if (timed)
+        {
+          e->pending[variant] = true;
+          e->pending_work[variant] = (double)count * max_m * max_n;
+          (void)hipEventRecord(e->ev[variant][0], st);
+        }
+      }
+    }
+  }
+  if (variant == 1)
+  {
+    dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, count);
+    hipLaunchKernelGGL(gemm_f64_grouped_kernel<32>, grid, dim3(256), 0, st, descs_dev);
+  }
+  else
+  {
+    dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, count);
+    hipLaunchKernelGGL(gemm_f64_grouped_kernel<GT>, grid, dim3(256), 0, st, descs_dev);
+  }
+  if (timed)
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    (void)hipEventRecord(e->ev[variant][1], st);
+  }
 }
 
 void launch_transpose_batched(const tr_desc* ds, int count, int max_rows, int max_cols, hipStream_t st)

@@ -322,6 +322,11 @@ void launch_splitk_reduce_epi(const gemm_desc& g, int s_eff, const double* part,
 void launch_polar_series(const double* E, const double* V, double* P, double* Pi, int n, hipStream_t st);
 int launch_gemm_splitk_parts(const gemm_desc& g, int S, double* part, hipStream_t st);
 void launch_splitk_reduce(const gemm_desc& g, int s_eff, const double* part, hipStream_t st);
+// The grouped GEMM's tile-size autotuner (per process): on / off (off: the
+// 64 x 64 tile always), and the keys seen, decided, and decided for the
+// 32 x 32 tile.
+void set_gemm_autotune(bool on);
+void gemm_autotune_stats(long long* keys, long long* decided, long long* chose_small);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st);
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,

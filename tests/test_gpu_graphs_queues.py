@@ -80,3 +80,33 @@ def test_persistent_queue_small_problems_against_oracle():
             assert y.ranks == ref.ranks
             d = y.to_dense() - ref.to_dense()
             assert np.linalg.norm(d) <= 1e-12 * np.linalg.norm(ref.to_dense())
+
+
+def test_autotuned_tiles_identical():
+    """The grouped GEMM's 64 and 32 tiles accumulate every output element in
+    the same order: batched rounding gives bit-identical cores with the tuner
+    exploring both tiles (its first launches of each shape class), with the
+    tuner's choices, and with the tuner off (64 tile only)."""
+    rng = np.random.default_rng(31)
+    d = rng.integers(4, 12, 30)
+    n = rng.choice([2, 4], 30)
+    ru = rng.integers(2, 24, 30)
+    rv = rng.integers(2, 24, 30)
+    batch = T.build_sum_batch(d, n, ru, rv, 10 + np.arange(30), 90 + np.arange(30))
+    ctx = T.Context.default(0)
+    ctx.set_autotune(False)
+    try:
+        ref = T.round_batched(batch, eps=1e-9)
+        ref_m = T.round_batched(batch, eps=1e-9, precision="mixed")
+    finally:
+        ctx.set_autotune(True)
+    for _ in range(3):
+        got = T.round_batched(batch, eps=1e-9)
+        got_m = T.round_batched(batch, eps=1e-9, precision="mixed")
+        ctx.synchronize()
+        for a, b in zip(got, ref):
+            _same(a, b)
+        for a, b in zip(got_m, ref_m):
+            _same(a, b)
+    keys, decided, small = ctx.autotune_stats()
+    assert keys > 0 and decided > 0
