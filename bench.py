@@ -272,55 +272,77 @@ def _train_mlp(T, ctx, train, eps, steps):
     t0 = time.perf_counter()
     loss = mlp.train(X, y, steps=steps, lr=2e-3)
     ctx.synchronize()
-    return mlp, {"train_bonds": int(len(y)), "steps": steps, "final_mse": loss, "train_s": time.perf_counter() - t0}
+    t_train = time.perf_counter() - t0
+    # FP32 training, BF16 inference.
+    mlp.set_bf16_inference(True)
+    # The policy's gate: the learned rounding is used for this workload only
+    # when it falls back on at most half of the training TTs (a fallback costs
+    # the sketch plus the deterministic rounding).
+    _, _, fb, _ = T.round_learned_batched(mlp, train, eps, seed=5, fallback="mixed")
+    return mlp, {"train_bonds": int(len(y)), "steps": steps, "final_mse": loss, "train_s": t_train,
+                 "inference": "bf16", "train_set_fallback_fraction": float(fb.mean()),
+                 "gate_open": bool(fb.mean() <= 0.5)}
 
 
-def _three_paths(T, ctx, batch, mlp, eps, label):
-    """Plain FP64, mixed and mixed + learned (randomized sketch rounding with
-    learned ranks) batched rounding of one workload: rounds/s, kernel time
-    per class, sampled errors (against the input and against plain)."""
+def _three_paths(T, ctx, batch, mlp, gate_open, eps, label):
+    """Plain FP64, mixed and mixed + learned batched rounding of one
+    workload: rounds/s, kernel time per class, sampled errors (against the
+    input and against plain).  The learned rounding (randomized sketch
+    rounding with the network's target ranks, mixed fallback) is timed on
+    its own; "mixed + learned" is the policy: the learned rounding where its
+    gate is open (it fell back on at most half of the training TTs), the
+    mixed rounding otherwise."""
     import numpy as np
 
     n = len(batch)
+
+    def learned_fn():
+        return T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3, fallback="mixed")
+
     # One untimed pass of each path over the whole batch: allocator segments
     # and the GEMM tile autotuner's per-shape decisions are in place before
     # the timed passes, as in a long-running process.
     for _ in range(2):
         T.round_batched(batch, eps=eps)
         T.round_batched(batch, eps=eps, precision="mixed")
-        T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3)
+        learned_fn()
     ctx.synchronize()
     plain, t_plain = _timed(ctx, lambda: T.round_batched(batch, eps=eps))
     mixed, t_mixed = _timed(ctx, lambda: T.round_batched(batch, eps=eps, precision="mixed"))
-    (learned, err, fb, ranks), t_learned = _timed(
-        ctx, lambda: T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1, seed=3))
+    (learned, err, fb, ranks), t_learned = _timed(ctx, learned_fn)
+    t_policy = t_learned if gate_open else t_mixed
+    policy = learned if gate_open else mixed
     sample = list(range(0, n, max(1, n // 16)))
     res = {
         "plain_fp64_rounds_per_s": n / t_plain,
         "mixed_rounds_per_s": n / t_mixed,
-        "mixed_learned_rounds_per_s": n / t_learned,
+        "learned_rounds_per_s": n / t_learned,
+        "mixed_learned_policy": "learned" if gate_open else "mixed (learned gate closed)",
+        "mixed_learned_rounds_per_s": n / t_policy,
         "speedup_mixed_vs_plain": t_plain / t_mixed,
-        "speedup_mixed_learned_vs_plain": t_plain / t_learned,
+        "speedup_learned_vs_plain": t_plain / t_learned,
+        "speedup_mixed_learned_vs_plain": t_plain / t_policy,
         "learned_fallback_fraction": float(fb.mean()),
         "learned_max_discarded_rel_norm": float(err[fb == 0].max()) if (fb == 0).any() else None,
         "max_rel_err_vs_input_sampled": {
             "plain": max(_rel(T, plain[i], batch[i]) for i in sample),
             "mixed": max(_rel(T, mixed[i], batch[i]) for i in sample),
-            "mixed_learned": max(_rel(T, learned[i], batch[i]) for i in sample)},
+            "learned": max(_rel(T, learned[i], batch[i]) for i in sample),
+            "mixed_learned": max(_rel(T, policy[i], batch[i]) for i in sample)},
         "max_rel_diff_vs_plain_sampled": {
             "mixed": max(_rel(T, mixed[i], plain[i]) for i in sample),
-            "mixed_learned": max(_rel(T, learned[i], plain[i]) for i in sample)},
+            "learned": max(_rel(T, learned[i], plain[i]) for i in sample),
+            "mixed_learned": max(_rel(T, policy[i], plain[i]) for i in sample)},
         "mean_rank": {"plain": float(np.mean([np.mean(p.ranks[1:-1]) for p in plain])),
                       "mixed": float(np.mean([np.mean(p.ranks[1:-1]) for p in mixed])),
                       "learned_target": float(np.mean([np.mean(r[1:-1]) for r in ranks]))},
         "mixed_ranks_equal_plain": f"{sum(int(a.ranks == b.ranks) for a, b in zip(mixed, plain))}/{n}",
     }
-    del plain, mixed, learned
+    del plain, mixed, learned, policy
     res["kernel_ms"] = {}
     for name, fn in (("plain", lambda: T.round_batched(batch, eps=eps)),
                      ("mixed", lambda: T.round_batched(batch, eps=eps, precision="mixed")),
-                     ("mixed_learned", lambda: T.round_learned_batched(mlp, batch, eps, oversample=6, slack=0.1,
-                                                                      seed=3)[0])):
+                     ("learned", lambda: learned_fn()[0])):
         ms, extra = _kernel_ms(ctx, fn)
         res["kernel_ms"][name] = ms
         if name == "plain":
@@ -329,7 +351,8 @@ def _three_paths(T, ctx, batch, mlp, eps, label):
             res["mixed_certified_steps"] = {"tried": extra["rank_revealing"][0],
                                             "accepted": extra["rank_revealing"][1]}
     print(f"[bench] {label}: plain {res['plain_fp64_rounds_per_s']:.0f}, mixed {res['mixed_rounds_per_s']:.0f}, "
-          f"mixed+learned {res['mixed_learned_rounds_per_s']:.0f} rounds/s", file=sys.stderr, flush=True)
+          f"learned {res['learned_rounds_per_s']:.0f}, mixed+learned {res['mixed_learned_rounds_per_s']:.0f} "
+          f"rounds/s", file=sys.stderr, flush=True)
     return res
 
 
@@ -398,16 +421,22 @@ def batched_section(T, ctx, args):
     del train
     s4 = rng.uniform(0.05, 0.3, n4)
     batch = T.build_decay_batch(n4, 32, 4, 128, s4, np.arange(n4))
-    r4 = _three_paths(T, ctx, batch, mlp4, eps, "config 4")
-    # The method without the network: randomized rounding with the stored
-    # ranks as targets (no truncation but the final projection's), same
-    # fallback rule.
-    stored = [t.ranks for t in batch]
-    T.round_randomized_batched(batch, stored, eps, seed=3)
-    (nolearn, err0, fb0), t0s = _timed(ctx, lambda: T.round_randomized_batched(batch, stored, eps, seed=3))
-    r4["randomized_stored_ranks_rounds_per_s"] = n4 / t0s
-    r4["randomized_stored_ranks_fallback_fraction"] = float(fb0.mean())
-    del nolearn
+    r4 = _three_paths(T, ctx, batch, mlp4, mlp4_info["gate_open"], eps, "config 4")
+    # The method without the network: the same randomized rounding and
+    # fallback with non-learned target ranks, the stored ranks (no truncation
+    # but the final projection's) and fixed fractions of them plus 6.
+    policies = {"stored_ranks": [t.ranks for t in batch]}
+    for f in (0.6, 0.8):
+        policies[f"fraction_{f}"] = [[1] + [min(t.ranks[k], int(np.ceil(f * t.ranks[k])) + 6)
+                                            for k in range(1, t.d)] + [1] for t in batch]
+    r4["non_learned_policies"] = {}
+    for name, rk in policies.items():
+        T.round_randomized_batched(batch, rk, eps, seed=3, fallback="mixed")
+        (nolearn, err0, fb0), t0s = _timed(
+            ctx, lambda: T.round_randomized_batched(batch, rk, eps, seed=3, fallback="mixed"))
+        r4["non_learned_policies"][name] = {"rounds_per_s": n4 / t0s, "fallback_fraction": float(fb0.mean()),
+                                            "mean_target_rank": float(np.mean([np.mean(r[1:-1]) for r in rk]))}
+        del nolearn
     r4.update({"workload": f"{n4} TTs, d=32, n=4, ranks min(4^k, 4^(32-k), 128), cores Q_k diag(10^(-j s)), "
                            "s~U[0.05,0.3], eps=1e-6", "mlp": mlp4_info})
     out["config4"] = r4
@@ -425,7 +454,7 @@ def batched_section(T, ctx, args):
     mlp3, mlp3_info = _train_mlp(T, ctx, train3, eps, args.mlp_steps)
     del train3
     c3 = sums(args.cfg3_count, 1000)
-    r3 = _three_paths(T, ctx, c3, mlp3, eps, "config 3")
+    r3 = _three_paths(T, ctx, c3, mlp3, mlp3_info["gate_open"], eps, "config 3")
     r3.update({"workload": f"{args.cfg3_count} TTs, d~U{{8..40}}, n in {{2,4,8}}, add(u, add(0.5u, v)), "
                            "r_u, r_v ~ U{8..64}, eps=1e-6", "mlp": mlp3_info})
     out["config3"] = r3

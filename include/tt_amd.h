@@ -199,12 +199,20 @@ enum tta_precision
 int tta_round_batched(tta_ctx* ctx, int count, tta_tt* const* in, double eps, int max_rank, int precision,
                       tta_tt** out);
 
+/* Deterministic rounding that replaces a randomized result whose discarded
+ * norm exceeds eps. */
+enum tta_fallback
+{
+  TTA_FALLBACK_NONE = 0,  /* keep the randomized result */
+  TTA_FALLBACK_FP64 = 1,  /* tta_round_batched with TTA_PREC_FP64 */
+  TTA_FALLBACK_MIXED = 2  /* tta_round_batched with TTA_PREC_MIXED */
+};
 /* Randomized rounding with given target ranks (Randomize-then-Orthogonalize:
  * Gaussian sketch TT, right-to-left sketch contractions, one left-to-right
  * Householder QR sweep).  ranks: for TT i, d_i + 1 entries (ends ignored),
  * concatenated.  The discarded norm ||y - yhat|| = sqrt(||y||^2 - ||yhat||^2)
- * is reported in rel_err[i] (relative); with fallback != 0 every TT whose
- * discarded norm exceeds eps is rounded by the plain path instead
+ * is reported in rel_err[i] (relative); every TT whose discarded norm exceeds
+ * eps is rounded by the deterministic path that `fallback` names instead
  * (fell_back[i] = 1).  No reference counterpart. */
 int tta_round_randomized_batched(tta_ctx* ctx, int count, tta_tt* const* in, const int* ranks, double eps,
                                  unsigned long long seed, int fallback, tta_tt** out, double* rel_err,
@@ -222,10 +230,17 @@ int tta_mlp_get_params(tta_ctx* ctx, const tta_mlp* m, float* host);
 int tta_mlp_set_params(tta_ctx* ctx, tta_mlp* m, const float* host);
 /* MSE loss and its gradient (flat, parameter layout) on n rows X (n x n_in), y (n). */
 int tta_mlp_grad(tta_ctx* ctx, tta_mlp* m, const float* X, const float* y, int n, float* grad, float* loss);
-/* `steps` full-batch Adam steps (lr, 0.9, 0.999); *loss = final loss. */
+/* `steps` full-batch Adam steps (betas 0.9, 0.999), the step size decayed
+ * from lr to lr / 100 on a half cosine over the call; the parameters of the
+ * lowest loss seen are kept and *loss is their loss. */
 int tta_mlp_train(tta_ctx* ctx, tta_mlp* m, const float* X, const float* y, int n, int steps, float lr,
                   float* loss);
 int tta_mlp_predict(tta_ctx* ctx, tta_mlp* m, const float* X, int n, float* y);
+/* Inference (tta_mlp_predict and the learned rounding's rank prediction)
+ * with weights, biases and activations rounded to BF16 (nearest, ties to
+ * even) and products accumulated in FP32 (on = 1), or in FP32 throughout
+ * (on = 0, the default).  Training always runs in FP32. */
+int tta_mlp_set_bf16_inference(tta_ctx* ctx, tta_mlp* m, int on);
 /* Number of per-bond features and the features of every bond k = 1..d_i-1 of
  * every TT (rows concatenated; the feature list is documented with
  * tta_learn.hip). */
@@ -233,13 +248,13 @@ int tta_num_bond_features(void);
 int tta_bond_features(tta_ctx* ctx, int count, tta_tt* const* in, double eps, float* feats);
 /* Learned-rank rounding: target rank of each bond =
  * ceil(prediction * r_k * (1 + slack)) + oversample (clipped to [1, r_k]),
- * randomized rounding with those ranks, plain fallback when the discarded norm
- * exceeds eps.  The relative slack covers the sketch's error factor over the
+ * randomized rounding with those ranks, deterministic fallback (FP64 or
+ * mixed; TTA_FALLBACK_NONE is rejected) when the discarded norm exceeds eps.  The relative slack covers the sketch's error factor over the
  * optimal truncation, which grows as the spectrum decays more slowly.
  * ranks_used (optional) gets the d_i + 1 target ranks per TT. */
 int tta_round_learned_batched(tta_ctx* ctx, tta_mlp* m, int count, tta_tt* const* in, double eps, int oversample,
-                              double slack, unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
-                              int* ranks_used);
+                              double slack, unsigned long long seed, int fallback, tta_tt** out, double* rel_err,
+                              int* fell_back, int* ranks_used);
 
 /* ---- dense kernels of the path (host buffers, row-major) --------------- */
 /* qr_thin (include/core/types/tt_eigen_bridge.hpp:42-60): Householder QR,

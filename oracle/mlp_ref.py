@@ -43,3 +43,24 @@ def loss_and_grad(p, dims, X, y):
             d = (d @ Ws[l].T) * (acts[l] > 0)
     flat = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return loss, flat
+
+
+def round_bf16(x):
+    """Nearest bfloat16 value of each float32 entry (ties to even), returned
+    as float32: the upper 16 bits of the IEEE single after adding 0x7fff plus
+    the lowest kept bit; infinities and NaNs pass through."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
+    finite = (b & 0x7F800000) != 0x7F800000
+    r = np.where(finite, b + 0x7FFF + ((b >> 16) & 1), b) & 0xFFFF0000
+    return r.astype(np.uint32).view(np.float32)
+
+
+def forward_bf16(p, dims, X):
+    """Inference output with weights, biases and layer inputs rounded to
+    bfloat16 and products accumulated exactly (float64)."""
+    Ws, bs = unpack(round_bf16(np.asarray(p, np.float32)), dims)
+    a = X
+    for i, (W, b) in enumerate(zip(Ws, bs)):
+        z = round_bf16(np.asarray(a, np.float32)).astype(np.float64) @ W + b
+        a = np.maximum(z, 0.0) if i < len(Ws) - 1 else z
+    return a[:, 0]

@@ -705,10 +705,23 @@ def build_decay_batch(count, d, n, rmax, s, seeds, ctx: Context | None = None):
     return [TensorTrain(ctypes.c_void_p(out[i]), ctx) for i in range(count)]
 
 
-def round_randomized_batched(tts, ranks, eps, seed=1, fallback=True):
-    """Randomized rounding with target ranks (list per TT of d+1 ints), plain
-    fallback when the discarded norm exceeds eps.  Returns (outputs,
-    rel_err array, fell_back array)."""
+_FALLBACK = {"none": 0, "fp64": 1, "mixed": 2}
+
+
+def _fallback_code(fallback):
+    if fallback is True:
+        return 1
+    if fallback is False:
+        return 0
+    return _enum(_FALLBACK, fallback, "fallback")
+
+
+def round_randomized_batched(tts, ranks, eps, seed=1, fallback="fp64"):
+    """Randomized rounding with target ranks (list per TT of d+1 ints);
+    TTs whose discarded norm exceeds eps are rounded again by the
+    deterministic path `fallback` names ("fp64" plain, "mixed", or "none" to
+    keep the randomized result; True / False mean "fp64" / "none").  Returns
+    (outputs, rel_err array, fell_back array)."""
     if not tts:
         return [], np.zeros(0), np.zeros(0, np.int32)
     ctx = tts[0]._ctx
@@ -719,7 +732,8 @@ def round_randomized_batched(tts, ranks, eps, seed=1, fallback=True):
     err = np.zeros(n)
     fb = np.zeros(n, np.int32)
     ctx.check(ctx.lib.tta_round_randomized_batched(ctx.h, n, arr_in, flat.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
-                                                   ctypes.c_double(eps), ctypes.c_ulonglong(seed), 1 if fallback else 0,
+                                                   ctypes.c_double(eps), ctypes.c_ulonglong(seed),
+                                                   _fallback_code(fallback),
                                                    arr_out, err.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                    fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
     return [type(tts[i])(ctypes.c_void_p(arr_out[i]), ctx) for i in range(n)], err, fb
@@ -779,6 +793,12 @@ class RankMLP:
                                                     ctypes.byref(loss)))
         return loss.value
 
+    def set_bf16_inference(self, on=True):
+        """Run inference (predict and the learned rounding's rank prediction)
+        with BF16 weights, biases and activations and FP32 accumulation;
+        training stays FP32."""
+        self._ctx.check(self._ctx.lib.tta_mlp_set_bf16_inference(self._ctx.h, self._h, 1 if on else 0))
+
     def predict(self, X):
         Xa, Xp = _f32(X)
         out = np.zeros(Xa.shape[0], np.float32)
@@ -797,10 +817,11 @@ def bond_features(tts, eps):
     return out[:nb]
 
 
-def round_learned_batched(mlp, tts, eps, oversample=6, seed=1, slack=0.1):
-    """Learned-rank randomized rounding with plain fallback; target rank per
-    bond = ceil(pred * r_k * (1 + slack)) + oversample.  Returns (outputs,
-    rel_err, fell_back, target ranks per TT)."""
+def round_learned_batched(mlp, tts, eps, oversample=6, seed=1, slack=0.1, fallback="fp64"):
+    """Learned-rank randomized rounding with deterministic fallback ("fp64"
+    plain or "mixed"); target rank per bond = ceil(pred * r_k * (1 + slack))
+    + oversample.  Returns (outputs, rel_err, fell_back, target ranks per
+    TT)."""
     ctx = tts[0]._ctx
     n = len(tts)
     arr_in = (ctypes.c_void_p * n)(*[t._h.value for t in tts])
@@ -809,7 +830,8 @@ def round_learned_batched(mlp, tts, eps, oversample=6, seed=1, slack=0.1):
     fb = np.zeros(n, np.int32)
     rk = np.zeros(sum(t.d + 1 for t in tts), np.int32)
     ctx.check(ctx.lib.tta_round_learned_batched(ctx.h, mlp._h, n, arr_in, ctypes.c_double(eps), oversample,
-                                                ctypes.c_double(slack), ctypes.c_ulonglong(seed), arr_out,
+                                                ctypes.c_double(slack), ctypes.c_ulonglong(seed),
+                                                _fallback_code(fallback), arr_out,
                                                 err.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
                                                 fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                                 rk.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))

@@ -72,11 +72,12 @@ SIGNATURES = {
     "tta_mlp_grad": (ctypes.c_int, [_vp, _vp, _f32p, _f32p, ctypes.c_int, _f32p, _f32p]),
     "tta_mlp_train": (ctypes.c_int, [_vp, _vp, _f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_float, _f32p]),
     "tta_mlp_predict": (ctypes.c_int, [_vp, _vp, _f32p, ctypes.c_int, _f32p]),
+    "tta_mlp_set_bf16_inference": (ctypes.c_int, [_vp, _vp, ctypes.c_int]),
     "tta_num_bond_features": (ctypes.c_int, []),
     "tta_bond_features": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), ctypes.c_double, _f32p]),
     "tta_round_learned_batched": (ctypes.c_int, [_vp, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p),
                                                  ctypes.c_double, ctypes.c_int, ctypes.c_double, ctypes.c_ulonglong,
-                                                 ctypes.POINTER(ctypes.c_void_p), _f64p, _i32p, _i32p]),
+                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_void_p), _f64p, _i32p, _i32p]),
     "tta_qr_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p]),
     "tta_svd_thin": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, ctypes.c_int, _f64p, _f64p, _f64p]),
     "tta_selfcheck_wave_sum": (ctypes.c_int, [_vp, _f64p, ctypes.c_int, _i32p]),

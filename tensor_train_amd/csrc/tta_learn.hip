@@ -169,11 +169,12 @@ int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const* in, const
                                  int* fell_back);
 
 int tta_round_learned_batched(tta_ctx* c, tta_mlp* m, int count, tta_tt* const* in, double eps, int oversample,
-                              double slack, unsigned long long seed, tta_tt** out, double* rel_err, int* fell_back,
-                              int* ranks_used)
+                              double slack, unsigned long long seed, int fallback, tta_tt** out, double* rel_err,
+                              int* fell_back, int* ranks_used)
 {
   if (!c || !m || count < 0 || (count > 0 && (!in || !out)) || !(eps > 0.0) || oversample < 0 || !(slack >= 0.0))
     return TTA_ERR_ARG;
+  if (fallback != TTA_FALLBACK_FP64 && fallback != TTA_FALLBACK_MIXED) return TTA_ERR_ARG;
   try
   {
     std::vector<float> pred;
@@ -201,7 +202,7 @@ int tta_round_learned_batched(tta_ctx* c, tta_mlp* m, int count, tta_tt* const* 
       if (d >= 1) ranks.push_back(1);
     }
     if (ranks_used) std::copy(ranks.begin(), ranks.end(), ranks_used);
-    return tta_round_randomized_batched(c, count, in, ranks.data(), eps, seed, 1, out, rel_err, fell_back);
+    return tta_round_randomized_batched(c, count, in, ranks.data(), eps, seed, fallback, out, rel_err, fell_back);
   }
   catch (const tta::error& e)
   {

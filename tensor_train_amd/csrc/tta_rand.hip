@@ -388,16 +388,20 @@ void round_batched_randomized(tta_ctx* c, std::vector<tta_tt*>& ts, const std::v
 
 using namespace tta;
 
-// Batched randomized rounding with per-TT target ranks, plain fallback for the
-// TTs whose discarded norm exceeds eps.  `ranks`: for TT i, d_i + 1 entries
-// (first and last ignored), concatenated.  `rel_err` and `fell_back` (each
-// `count` entries, may be null) report the discarded norm estimate and whether
-// the plain rounding replaced the randomized result.
+// Batched randomized rounding with per-TT target ranks, deterministic
+// fallback for the TTs whose discarded norm exceeds eps (fallback =
+// TTA_FALLBACK_FP64: the plain rounding, TTA_FALLBACK_MIXED: the mixed one,
+// TTA_FALLBACK_NONE: keep the randomized result).  `ranks`: for TT i, d_i + 1
+// entries (first and last ignored), concatenated.  `rel_err` and `fell_back`
+// (each `count` entries, may be null) report the discarded norm estimate and
+// whether the deterministic rounding replaced the randomized result.
 extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const* in, const int* ranks, double eps,
                                             unsigned long long seed, int fallback, tta_tt** out, double* rel_err,
                                             int* fell_back)
 {
   if (!c || count < 0 || (count > 0 && (!in || !ranks || !out))) return TTA_ERR_ARG;
+  if (fallback != TTA_FALLBACK_NONE && fallback != TTA_FALLBACK_FP64 && fallback != TTA_FALLBACK_MIXED)
+    return TTA_ERR_ARG;
   std::vector<tta_tt*> ts;
   // TTs with d >= 2 start from the caller's cores (read only; every core is
   // replaced); shorter ones pass through as copies.
@@ -459,7 +463,7 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
     }
     if (!redo.empty())
     {
-      // Plain rounding of the rejected TTs from their inputs.
+      // Deterministic rounding of the rejected TTs from their inputs.
       std::vector<tta_tt*> again;
       std::vector<cp_desc> cp2;
       long me = 0;
@@ -482,7 +486,7 @@ extern "C" int tta_round_randomized_batched(tta_ctx* c, int count, tta_tt* const
       upload_async(c, d, cp2.data(), cp2.size() * sizeof(cp_desc));
       launch_copy_batched(static_cast<cp_desc*>(d), (int)cp2.size(), me, c->stream);
       c->mem.put(d);
-      round_batched_plain(c, again, eps, 0, false);
+      round_batched_plain(c, again, eps, 0, fallback == TTA_FALLBACK_MIXED);
     }
     for (int i = 0; i < count; ++i) out[i] = ts[i];
   }

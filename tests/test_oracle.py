@@ -456,3 +456,19 @@ def test_round_batch_threads_equal_round():
         assert y.ranks == ref.ranks
         for a, b in zip(y.cores(), ref.cores()):
             assert np.array_equal(a, b)
+
+
+def test_round_bf16_matches_torch():
+    """The MLP restatement's bfloat16 rounding against torch's (nearest, ties
+    to even), on random values, exact ties and specials."""
+    import torch
+
+    import oracle.mlp_ref as R
+
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(10000).astype(np.float32) * np.float32(10.0) ** rng.integers(-30, 30, 10000),
+                        np.array([1 + 2 ** -8, 1 + 3 * 2 ** -8, -(1 + 2 ** -8), 0.0, -0.0, np.inf, -np.inf,
+                                  3.4e38, 1e-40], np.float32)]).astype(np.float32)
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    got = R.round_bf16(x)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
