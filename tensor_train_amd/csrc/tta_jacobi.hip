@@ -37,7 +37,10 @@ using namespace dev;
 namespace
 {
 
-__device__ __forceinline__ const jac_prob& jpick(const jac_batch& b, int idx)
+// By value: a reference that may point at the kernel argument made the
+// compiler copy the argument to scratch memory and read every field through
+// generic (flat) loads.
+__device__ __forceinline__ jac_prob jpick(const jac_batch& b, int idx)
 {
   return b.descs ? b.descs[idx] : b.one;
 }

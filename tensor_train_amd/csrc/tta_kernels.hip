@@ -33,33 +33,23 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // slab of a TT-wide tile: A(i0 + ii, k0 + kk) and B(k0 + kk, j0 + jj), with
 // the thread order matched to the operand's contiguous direction so that
 // loads coalesce.
+// (Selects rather than branches: with stores to the two outputs in
+// branches the compiler kept them in scratch memory.)
 template <int TT = GT>
 __device__ __forceinline__ void slab_coords_a(bool kc, int idx, int& ii, int& kk)
 {
-  if (kc)
-  {
-    ii = idx >> 5;
-    kk = idx & 31;
-  }
-  else
-  {
-    ii = idx & (TT - 1);
-    kk = idx / TT;
-  }
+  constexpr int SH = TT == 64 ? 6 : 5;
+  static_assert(TT == 64 || TT == 32, "tile width");
+  ii = kc ? (idx >> 5) : (idx & (TT - 1));
+  kk = kc ? (idx & 31) : (idx >> SH);
 }
 template <int TT = GT>
 __device__ __forceinline__ void slab_coords_b(bool jc, int idx, int& kk, int& jj)
 {
-  if (jc)
-  {
-    kk = idx / TT;
-    jj = idx & (TT - 1);
-  }
-  else
-  {
-    jj = idx >> 5;
-    kk = idx & 31;
-  }
+  constexpr int SH = TT == 64 ? 6 : 5;
+  static_assert(TT == 64 || TT == 32, "tile width");
+  kk = jc ? (idx >> SH) : (idx & 31);
+  jj = jc ? (idx & (TT - 1)) : (idx >> 5);
 }
 
 typedef float flt4 __attribute__((ext_vector_type(4)));

@@ -29,7 +29,10 @@ constexpr int QCT = 8;   // columns per update workgroup (2 per wave)
 constexpr int QCPW = QCT / 4;
 constexpr int QRC = 128; // rows per V chunk in the update kernel
 
-__device__ __forceinline__ const qrp& pick(const qr_batch& b)
+// By value: a reference that may point at the kernel argument made the
+// compiler copy the argument to scratch memory and read every field through
+// generic (flat) loads.
+__device__ __forceinline__ qrp pick(const qr_batch& b)
 {
   return b.descs ? b.descs[blockIdx.y] : b.one;
 }
@@ -73,7 +76,7 @@ __device__ __forceinline__ void block_vsum(const double (&v)[NV], int lo, int hi
 template <int NT, int RPT>
 __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
 {
-  const qrp& P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
+  const qrp P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
   const int m = P.m, n = P.n;
   const int k = m < n ? m : n;
   if (j0 >= k) return;
@@ -238,7 +241,7 @@ __global__ __launch_bounds__(NT) void qr_panel_kernel(qr_batch bt, int j0)
 template <int NT>
 __global__ __launch_bounds__(NT) void qr_panel_tall_kernel(qr_batch bt, int j0)
 {
-  const qrp& P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
+  const qrp P = bt.descs ? bt.descs[blockIdx.x] : bt.one;
   const int m = P.m, n = P.n;
   const int k = m < n ? m : n;
   if (j0 >= k) return;
