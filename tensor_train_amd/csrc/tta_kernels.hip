@@ -5,7 +5,9 @@
 #include <map>
 #include <mutex>
 
+#include "../../include/tt_amd.h"
 #include "tta_device.hpp"
+#include "tta_internal.hpp"
 #include "tta_kernels.hpp"
 #include "tta_potrf.hpp"
 
@@ -506,28 +508,84 @@ __global__ void permute_kernel(permute_desc p)
   }
 }
 
-// y(a*s+ax, i, b*sp+bx) = sum_j A(a,i,j,b) x(ax,j,bx)   (one thread per output).
-__global__ void matvec_core_kernel(const double* __restrict__ A, const double* __restrict__ x, double* __restrict__ y,
-                                   int r, int m, int n, int rp, int s, int sp)
+// y(a*s+ax, i, b*sp+bx) = sum_j A(a,i,j,b) x(ax,j,bx), the MPO core
+// contraction, one thread per column of outputs: thread t = ((a m + i) s +
+// ax) sp + bx keeps x(ax, :, bx) in registers (up to kMvN of them; larger
+// mode sizes read x again per output) and writes the rp outputs
+// y(a, ax, i, :, bx); lanes with consecutive bx write adjacent doubles.
+// When the 64 lanes of a wave share (a, i) (s sp a multiple of 64 in the
+// interior of a core) and A(a, i, :, :) has at most 64 entries, the wave
+// loads it once, one entry per lane, and each product takes its factor by
+// readlane, so the vector memory pipeline carries only x and the stores.
+// 32-bit indices: the launcher checks that the core has fewer than 2^31
+// entries.  Launch: one thread per column, cols = r s m sp.
+constexpr int kMvN = 8;
+__device__ __forceinline__ double read_lane(double v, int k)
 {
-  const long total = (long)r * s * m * rp * sp;
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, k), hi = __builtin_amdgcn_readlane((int)(b >> 32), k);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__device__ __forceinline__ void matvec_core_cols(const double* __restrict__ A, const double* __restrict__ x,
+                                                 double* __restrict__ y, unsigned r, unsigned m, unsigned n,
+                                                 unsigned rp, unsigned s, unsigned sp, unsigned t)
+{
+  const unsigned cols = r * s * m * sp, gs = s * sp;
+  const unsigned lane = threadIdx.x & 63, t0 = t - lane;
+  if (t0 >= cols) return;  // the whole wave is past the core
+  const unsigned tl = min(t0 + 63, cols - 1), tt = min(t, cols - 1);
+  const unsigned g = tt / gs, rem = tt - g * gs, ax = rem / sp, bx = rem - ax * sp;
+  const unsigned a = g / m, i = g - a * m;
+  const double* xr = x + (size_t)ax * n * sp + bx;
+  double* yr = y + ((size_t)(a * s + ax) * m + i) * rp * sp + bx;
+  const bool live = t < cols;
+  if (n > (unsigned)kMvN)
   {
-    long t = o;
-    const int bx = (int)(t % sp);
-    t /= sp;
-    const int b = (int)(t % rp);
-    t /= rp;
-    const int i = (int)(t % m);
-    t /= m;
-    const int ax = (int)(t % s);
-    const int a = (int)(t / s);
-    const double* Ar = A + (((size_t)a * m + i) * n) * rp + b;
-    const double* xr = x + ((size_t)ax * n) * sp + bx;
-    double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc = fma(Ar[(size_t)j * rp], xr[(size_t)j * sp], acc);
-    y[o] = acc;
+    const double* Ar = A + (size_t)g * n * rp;
+    for (unsigned bb = 0; bb < rp; ++bb)
+    {
+      double acc = 0.0;
+      for (unsigned j = 0; j < n; ++j) acc = fma(Ar[(size_t)j * rp + bb], xr[(size_t)j * sp], acc);
+      if (live) yr[(size_t)bb * sp] = acc;
+    }
+    return;
   }
+  double xv[kMvN];
+#pragma unroll
+  for (int j = 0; j < kMvN; ++j) xv[j] = j < (int)n ? xr[(size_t)j * sp] : 0.0;
+  if (t0 / gs == tl / gs && n * rp <= 64)
+  {
+    const double* Ag = A + (size_t)(t0 / gs) * n * rp;
+    const double mine = lane < n * rp ? Ag[lane] : 0.0;
+    for (unsigned bb = 0; bb < rp; ++bb)
+    {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < kMvN; ++j)
+        if (j < (int)n) acc = fma(read_lane(mine, j * rp + bb), xv[j], acc);
+      if (live) yr[(size_t)bb * sp] = acc;
+    }
+  }
+  else
+  {
+    const double* Ar = A + (size_t)g * n * rp;
+    for (unsigned bb = 0; bb < rp; ++bb)
+    {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < kMvN; ++j)
+        if (j < (int)n) acc = fma(Ar[(size_t)j * rp + bb], xv[j], acc);
+      if (live) yr[(size_t)bb * sp] = acc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void matvec_core_kernel(const double* __restrict__ A, const double* __restrict__ x,
+                                                          double* __restrict__ y, int r, int m, int n, int rp, int s,
+                                                          int sp)
+{
+  matvec_core_cols(A, x, y, r, m, n, rp, s, sp, blockIdx.x * 256u + threadIdx.x);
 }
 
 // C(a*s+ax, i, j, b*sp+bx) = sum_q A(a,i,q,b) B(ax,q,j,bx).
@@ -581,16 +639,105 @@ __global__ __launch_bounds__(256) void finish_sumsq_kernel(const double* part, i
   }
 }
 
-// Column norms of a row-major (rows x cols) matrix: one wave per column.
+// Norms of columns [c0, c0 + kCnCols) of a row-major (rows x cols) matrix by
+// a workgroup of 256 threads: cw = min(kCnCols, cols rounded up to a power of
+// two) adjacent columns (one 128-byte line) of 256 / cw rows at a time, eight
+// such loads in flight per thread, so every entry is read once and the
+// workgroup keeps enough requests outstanding to cover memory latency; each
+// column's partial sums (one per thread that read it) are then added in a
+// fixed order.
+constexpr int kCnCols = 16;
+__device__ __forceinline__ void col_norms_tile(const double* __restrict__ M, unsigned rows, int cols,
+                                               double* __restrict__ out, int c0)
+{
+  __shared__ double red[256];
+  int cw = 1;
+  while (cw < kCnCols && cw < cols) cw <<= 1;
+  const unsigned per = 256 / cw;
+  const int tid = threadIdx.x;
+  const int c = c0 + (tid % cw);
+  double acc = 0.0;
+  if (c < cols)
+  {
+    unsigned i = tid / cw;
+    for (; i + 7 * per < rows; i += 8 * per)
+    {
+      double v[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = M[(size_t)(i + k * per) * cols + c];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc = fma(v[k], v[k], acc);
+    }
+    for (; i < rows; i += per)
+    {
+      const double v = M[(size_t)i * cols + c];
+      acc = fma(v, v, acc);
+    }
+  }
+  red[tid] = acc;
+  __syncthreads();
+  if (tid < cw && c < cols)
+  {
+    double sum = 0.0;
+    for (unsigned k = 0; k < per; ++k) sum += red[k * cw + tid];
+    out[c] = sqrt(sum);
+  }
+}
+
+// Balancing of one core x (r, n, rp) by a grid-stride loop over groups of 4
+// consecutive entries: x(a, i, b) *= (g / nr(b)) / (g / nl(a)), a factor 1
+// where a norm is zero or its array null.  One 32-bit decomposition of each
+// group's first index (cores have fewer than 2^31 entries), carries for the
+// rest, one vector load and store for whole groups of a 32-byte aligned core.
+__device__ __forceinline__ void balance_core(double* __restrict__ x, unsigned r, unsigned n, unsigned rp,
+                                             const double* __restrict__ nl, const double* __restrict__ nr, double g,
+                                             unsigned first, unsigned stride)
+{
+  const unsigned total = r * n * rp;
+  const bool aligned = (reinterpret_cast<uintptr_t>(x) & 31) == 0;
+  for (unsigned o = first * 4; o < total; o += stride * 4)
+  {
+    unsigned row = o / rp, bb = o - row * rp, a = row / n;
+    double v[4];
+    const bool whole = aligned && o + 4 <= total;
+    if (whole)
+    {
+      const dbl4 w = *reinterpret_cast<const dbl4*>(x + o);
+      v[0] = w[0], v[1] = w[1], v[2] = w[2], v[3] = w[3];
+    }
+    else
+    {
+      for (int k = 0; k < 4; ++k) v[k] = o + k < total ? x[o + k] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+    {
+      double f = 1.0;
+      if (nr && nr[bb] > 0.0) f *= g / nr[bb];
+      if (nl && nl[a] > 0.0) f /= g / nl[a];
+      v[k] *= f;
+      if (++bb == rp)
+      {
+        bb = 0;
+        a = ++row / n;
+      }
+    }
+    if (whole)
+    {
+      const dbl4 w = {v[0], v[1], v[2], v[3]};
+      *reinterpret_cast<dbl4*>(x + o) = w;
+    }
+    else
+    {
+      for (int k = 0; k < 4 && o + k < total; ++k) x[o + k] = v[k];
+    }
+  }
+}
+
+// Column norms of a row-major (rows x cols) matrix (col_norms_tile).
 __global__ __launch_bounds__(256) void col_norms_kernel(const double* M, int rows, int cols, double* out)
 {
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= cols) return;
-  double acc = 0.0;
-  for (int i = lane; i < rows; i += 64) acc = fma(M[(size_t)i * cols + c], M[(size_t)i * cols + c], acc);
-  acc = wave_sum(acc);
-  if (lane == 0) out[c] = sqrt(acc);
+  col_norms_tile(M, rows, cols, out, blockIdx.x * kCnCols);
 }
 
 // Geometric mean of the nonzero norms (single block): out = exp(mean(log)).
@@ -612,21 +759,12 @@ __global__ __launch_bounds__(256) void geo_mean_kernel(const double* v, int n, d
 // Balancing of one core (r, n, rp): x(a,i,b) *= f_right(b) / f_left(a), with
 // f(c) = geo / norm(c) for norm(c) > 0 and 1 otherwise; a null norm array
 // means all factors are 1.
-__global__ void balance_kernel(double* x, int r, int n, int rp, const double* nl, const double* nr,
-                               const double* geo)
+__global__ __launch_bounds__(256) void balance_kernel(double* x, int r, int n, int rp, const double* nl,
+                                                      const double* nr, const double* geo)
 {
-  const long total = (long)r * n * rp;
   const double g = *geo;
   if (g == 0.0) return;
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
-  {
-    const int b = (int)(o % rp);
-    const int a = (int)(o / ((long)n * rp));
-    double f = 1.0;
-    if (nr && nr[b] > 0.0) f *= g / nr[b];
-    if (nl && nl[a] > 0.0) f /= g / nl[a];
-    x[o] *= f;
-  }
+  balance_core(x, r, n, rp, nl, nr, g, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
 // Per-core launches folded into one (descriptors by value, blockIdx.y picks
@@ -634,40 +772,16 @@ __global__ void balance_kernel(double* x, int r, int n, int rp, const double* nl
 __global__ __launch_bounds__(256) void matvec_core_batch_kernel(matvec_batch b)
 {
   const matvec_batch::item& it = b.it[blockIdx.y];
-  const int r = it.r, m = it.m, n = it.n, rp = it.rp, s = it.s, sp = it.sp;
-  const long total = (long)r * s * m * rp * sp;
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
-  {
-    long t = o;
-    const int bx = (int)(t % sp);
-    t /= sp;
-    const int bb = (int)(t % rp);
-    t /= rp;
-    const int i = (int)(t % m);
-    t /= m;
-    const int ax = (int)(t % s);
-    const int a = (int)(t / s);
-    const double* Ar = it.A + (((size_t)a * m + i) * n) * rp + bb;
-    const double* xr = it.x + ((size_t)ax * n) * sp + bx;
-    double acc = 0.0;
-    for (int j = 0; j < n; ++j) acc = fma(Ar[(size_t)j * rp], xr[(size_t)j * sp], acc);
-    it.y[o] = acc;
-  }
+  matvec_core_cols(it.A, it.x, it.y, it.r, it.m, it.n, it.rp, it.s, it.sp, blockIdx.x * 256u + threadIdx.x);
 }
 
-// Column norms of each core's left unfolding (r n x rp) into nrm + off.
+// Column norms of each core's left unfolding (r n x rp, row-major) into
+// nrm + off (col_norms_tile; workgroup (x, core) takes columns kCnCols x ..).
 __global__ __launch_bounds__(256) void core_norms_batch_kernel(core_batch b, double* nrm)
 {
   const core_batch::item& it = b.it[blockIdx.y];
-  if (it.off < 0) return;
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= it.rp) return;
-  const int rows = it.r * it.n;
-  double acc = 0.0;
-  for (int i = lane; i < rows; i += 64) acc = fma(it.x[(size_t)i * it.rp + c], it.x[(size_t)i * it.rp + c], acc);
-  acc = wave_sum(acc);
-  if (lane == 0) nrm[it.off + c] = sqrt(acc);
+  if (it.off < 0 || (int)blockIdx.x * kCnCols >= it.rp) return;
+  col_norms_tile(it.x, (unsigned)it.r * it.n, it.rp, nrm + it.off, blockIdx.x * kCnCols);
 }
 
 // balance_kernel for each core: left norms at nrm + offl, right ones at
@@ -675,20 +789,10 @@ __global__ __launch_bounds__(256) void core_norms_batch_kernel(core_batch b, dou
 __global__ __launch_bounds__(256) void balance_batch_kernel(core_batch b, const double* nrm, const double* geo)
 {
   const core_batch::item& it = b.it[blockIdx.y];
-  const long total = (long)it.r * it.n * it.rp;
   const double g = *geo;
   if (g == 0.0) return;
-  const double* nl = it.offl >= 0 ? nrm + it.offl : nullptr;
-  const double* nr = it.off >= 0 ? nrm + it.off : nullptr;
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
-  {
-    const int bb = (int)(o % it.rp);
-    const int a = (int)(o / ((long)it.n * it.rp));
-    double f = 1.0;
-    if (nr && nr[bb] > 0.0) f *= g / nr[bb];
-    if (nl && nl[a] > 0.0) f /= g / nl[a];
-    it.x[o] *= f;
-  }
+  balance_core(it.x, it.r, it.n, it.rp, it.offl >= 0 ? nrm + it.offl : nullptr, it.off >= 0 ? nrm + it.off : nullptr,
+               g, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
 // dst[i*ldd + j] = f * src[i*lds + j] + keep * dst[i*ldd + j].
@@ -806,17 +910,12 @@ __global__ __launch_bounds__(256) void frob_batched_kernel(const sq_desc* ds)
   }
 }
 
-// Column norms of a row-major (rows x cols) core per problem; one wave per column.
+// Column norms of a row-major (rows x cols) core per problem (col_norms_tile).
 __global__ __launch_bounds__(256) void col_norms_batched_kernel(const cn_desc* ds)
 {
   const cn_desc& d = ds[blockIdx.y];
-  const int lane = threadIdx.x & 63;
-  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (c >= d.cols) return;
-  double acc = 0.0;
-  for (int i = lane; i < d.rows; i += 64) acc = fma(d.M[(size_t)i * d.cols + c], d.M[(size_t)i * d.cols + c], acc);
-  acc = wave_sum(acc);
-  if (lane == 0) d.out[c] = sqrt(acc);
+  if ((int)blockIdx.x * kCnCols >= d.cols) return;
+  col_norms_tile(d.M, d.rows, d.cols, d.out, blockIdx.x * kCnCols);
 }
 
 // Geometric mean of the nonzero entries of v[0..n) per problem.
@@ -836,21 +935,12 @@ __global__ __launch_bounds__(256) void geo_mean_batched_kernel(const gm_desc* ds
   if (threadIdx.x == 0) d.out[0] = c > 0.0 ? exp(s / c) : 0.0;
 }
 
-__global__ void balance_batched_kernel(const bal_desc* ds)
+__global__ __launch_bounds__(256) void balance_batched_kernel(const bal_desc* ds)
 {
   const bal_desc& d = ds[blockIdx.y];
-  const long total = (long)d.r * d.n * d.rp;
   const double g = *d.geo;
   if (g == 0.0) return;
-  for (long o = blockIdx.x * (long)blockDim.x + threadIdx.x; o < total; o += (long)gridDim.x * blockDim.x)
-  {
-    const int b = (int)(o % d.rp);
-    const int a = (int)(o / ((long)d.n * d.rp));
-    double f = 1.0;
-    if (d.nr && d.nr[b] > 0.0) f *= g / d.nr[b];
-    if (d.nl && d.nl[a] > 0.0) f /= g / d.nl[a];
-    d.x[o] *= f;
-  }
+  balance_core(d.x, d.r, d.n, d.rp, d.nl, d.nr, g, blockIdx.x * 256u + threadIdx.x, gridDim.x * 256u);
 }
 
 int grid_for(long n, int bs = 256)
@@ -1348,7 +1438,11 @@ void launch_matvec_core(const double* A, const double* x, double* y, int r, int 
                         hipStream_t st)
 {
   const long total = (long)r * s * m * rp * sp;
-  hipLaunchKernelGGL(matvec_core_kernel, dim3(grid_for(total)), dim3(256), 0, st, A, x, y, r, m, n, rp, s, sp);
+  if (total <= 0) return;
+  if (total >= (1L << 31)) fail(TTA_ERR_UNSUPPORTED, "matvec core of %ld entries (at most 2^31 - 1)", total);
+  const long cols = total / rp;
+  hipLaunchKernelGGL(matvec_core_kernel, dim3((unsigned)((cols + 255) / 256)), dim3(256), 0, st, A, x, y, r, m, n, rp,
+                     s, sp);
 }
 
 void launch_matmat_core(const double* A, const double* B, double* C, int r, int m, int p, int rp, int s, int n,
@@ -1370,7 +1464,8 @@ void launch_frob_norm(const double* x, long n, double* part, int nparts, double*
 
 void launch_col_norms(const double* M, int rows, int cols, double* out, hipStream_t st)
 {
-  hipLaunchKernelGGL(col_norms_kernel, dim3((cols + 3) / 4), dim3(256), 0, st, M, rows, cols, out);
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(col_norms_kernel, dim3((cols + kCnCols - 1) / kCnCols), dim3(256), 0, st, M, rows, cols, out);
 }
 
 void launch_geo_mean(const double* v, int n, double* out, hipStream_t st)
@@ -1380,14 +1475,17 @@ void launch_geo_mean(const double* v, int n, double* out, hipStream_t st)
 
 void launch_matvec_core_batch(const matvec_batch& b, hipStream_t st)
 {
-  long most = 0;
+  long most = 0, most_cols = 0;
   for (int z = 0; z < b.count; ++z)
   {
     const matvec_batch::item& it = b.it[z];
     most = std::max(most, (long)it.r * it.s * it.m * it.rp * it.sp);
+    most_cols = std::max(most_cols, (long)it.r * it.s * it.m * it.sp);
   }
   if (b.count <= 0 || most <= 0) return;
-  hipLaunchKernelGGL(matvec_core_batch_kernel, dim3(grid_for(most), b.count), dim3(256), 0, st, b);
+  if (most >= (1L << 31)) fail(TTA_ERR_UNSUPPORTED, "matvec core of %ld entries (at most 2^31 - 1)", most);
+  hipLaunchKernelGGL(matvec_core_batch_kernel, dim3((unsigned)((most_cols + 255) / 256), b.count), dim3(256), 0, st,
+                     b);
 }
 
 void launch_core_norms_batch(const core_batch& b, double* nrm, hipStream_t st)
@@ -1395,7 +1493,8 @@ void launch_core_norms_batch(const core_batch& b, double* nrm, hipStream_t st)
   int most = 0;
   for (int z = 0; z < b.count; ++z) most = std::max(most, b.it[z].rp);
   if (b.count <= 0 || most <= 0) return;
-  hipLaunchKernelGGL(core_norms_batch_kernel, dim3((most + 3) / 4, b.count), dim3(256), 0, st, b, nrm);
+  hipLaunchKernelGGL(core_norms_batch_kernel, dim3((most + kCnCols - 1) / kCnCols, b.count), dim3(256), 0, st, b,
+                     nrm);
 }
 
 void launch_balance_batch(const core_batch& b, const double* nrm, const double* geo, hipStream_t st)
@@ -1403,13 +1502,17 @@ void launch_balance_batch(const core_batch& b, const double* nrm, const double* 
   long most = 0;
   for (int z = 0; z < b.count; ++z) most = std::max(most, (long)b.it[z].r * b.it[z].n * b.it[z].rp);
   if (b.count <= 0 || most <= 0) return;
-  hipLaunchKernelGGL(balance_batch_kernel, dim3(grid_for(most), b.count), dim3(256), 0, st, b, nrm, geo);
+  if (most >= (1L << 31)) fail(TTA_ERR_UNSUPPORTED, "core of %ld entries (at most 2^31 - 1)", most);
+  hipLaunchKernelGGL(balance_batch_kernel, dim3(grid_for((most + 3) / 4), b.count), dim3(256), 0, st, b, nrm, geo);
 }
 
 void launch_balance(double* x, int r, int n, int rp, const double* nl, const double* nr, const double* geo,
                     hipStream_t st)
 {
-  hipLaunchKernelGGL(balance_kernel, dim3(grid_for((long)r * n * rp)), dim3(256), 0, st, x, r, n, rp, nl, nr, geo);
+  const long total = (long)r * n * rp;
+  if (total <= 0) return;
+  if (total >= (1L << 31)) fail(TTA_ERR_UNSUPPORTED, "core of %ld entries (at most 2^31 - 1)", total);
+  hipLaunchKernelGGL(balance_kernel, dim3(grid_for((total + 3) / 4)), dim3(256), 0, st, x, r, n, rp, nl, nr, geo);
 }
 
 void launch_copy2d(const double* src, int lds_, double* dst, int ldd, int rows, int cols, double f, double keep,
@@ -1606,7 +1709,8 @@ void launch_frob_batched(const sq_desc* ds, int count, hipStream_t st)
 void launch_col_norms_batched(const cn_desc* ds, int count, int max_cols, hipStream_t st)
 {
   if (count <= 0 || max_cols <= 0) return;
-  hipLaunchKernelGGL(col_norms_batched_kernel, dim3((max_cols + 3) / 4, count), dim3(256), 0, st, ds);
+  hipLaunchKernelGGL(col_norms_batched_kernel, dim3((max_cols + kCnCols - 1) / kCnCols, count), dim3(256), 0, st,
+                     ds);
 }
 
 void launch_geo_mean_batched(const gm_desc* ds, int count, hipStream_t st)
@@ -1618,7 +1722,8 @@ void launch_geo_mean_batched(const gm_desc* ds, int count, hipStream_t st)
 void launch_balance_batched(const bal_desc* ds, int count, long max_elems, hipStream_t st)
 {
   if (count <= 0) return;
-  int g = (int)std::min<long>((max_elems + 255) / 256, 256);
+  if (max_elems >= (1L << 31)) fail(TTA_ERR_UNSUPPORTED, "core of %ld entries (at most 2^31 - 1)", max_elems);
+  int g = (int)std::min<long>((max_elems + 1023) / 1024, 256);
   hipLaunchKernelGGL(balance_batched_kernel, dim3(g < 1 ? 1 : g, count), dim3(256), 0, st, ds);
 }
 
