@@ -238,10 +238,9 @@ def _rel(T, a, b):
     return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
 
 
-def _timed(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
-    """fn() timed on the wall clock (stream drained before and after), with
-    the library's per-class kernel milliseconds from a second, event-timed
-    run of the same call."""
+def _timed(ctx, fn):
+    """(fn(), seconds): fn timed on the wall clock, the stream drained before
+    and after."""
     ctx.synchronize()
     t0 = time.perf_counter()
     out = fn()
@@ -251,6 +250,8 @@ def _timed(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
 
 
 def _kernel_ms(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
+    """Per-class device milliseconds of one more call of fn, timed by the
+    library's HIP events, and its Jacobi and rank-revealing counters."""
     ctx.prof_enable(True)
     out = fn()
     ctx.synchronize()
@@ -504,6 +505,10 @@ def main():
     ap.add_argument("--train-tts", type=int, default=650)
     ap.add_argument("--train-tts-cfg3", type=int, default=850)
     ap.add_argument("--mlp-steps", type=int, default=1500)
+    ap.add_argument("--plain-decay-only", type=int, default=0, metavar="COUNT",
+                    help="only round COUNT decaying-spectrum TTs (d=32, n=4, ranks <= 128, eps=1e-6) by the plain "
+                         "FP64 batched path, once untimed and once timed (a short run for counter profiles of the "
+                         "batched QR and Jacobi kernels), then exit")
     ap.add_argument("--full-svd-steps", type=int, default=2,
                     help="steps timed again with the full-rank certificate, CholQR2 and speculative sweeps "
                          "off (every QR is Householder and every SVD step runs its SVD)")
@@ -522,6 +527,17 @@ def main():
     import tensor_train_amd as T
 
     ctx = T.Context.default(0)
+    if args.plain_decay_only > 0:
+        import numpy as np
+
+        n = args.plain_decay_only
+        s = np.random.default_rng(2024).uniform(0.05, 0.3, n)
+        batch = T.build_decay_batch(n, 32, 4, 128, s, np.arange(n))
+        T.round_batched(batch, eps=1e-6)
+        out, t = _timed(ctx, lambda: T.round_batched(batch, eps=1e-6))
+        print(json.dumps({"workload": f"plain FP64 batched rounding of {n} decaying-spectrum TTs (d=32, n=4, "
+                                      "ranks <= 128, eps=1e-6)", "rounds_per_s": n / t}))
+        return
     if args.batched_only:
         print(json.dumps({"batched": batched_section(T, ctx, args)}))
         return
