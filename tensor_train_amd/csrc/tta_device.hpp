@@ -207,6 +207,23 @@ __device__ void org2r(const double* __restrict__ A, int lda, int m, int k, const
   }
 }
 
+// Hardware reciprocal and reciprocal square root, each refined by two
+// Newton steps (within a few ulps).
+__device__ __forceinline__ double rcp_newton(double x)
+{
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
+// 1 / sqrt(y) for 1 <= y <= 1e300.
+__device__ __forceinline__ double rsqrt_newton(double y)
+{
+  double r = __builtin_amdgcn_rsq(y);
+  r = r * fma(-0.5 * y * r, r, 1.5);
+  return r * fma(-0.5 * y * r, r, 1.5);
+}
+
 // Jacobi rotation that orthogonalises columns x_i, x_j from a = |x_i|^2,
 // b = |x_j|^2, g = x_i . x_j (Rutishauser's formulas).  Returns false (no
 // rotation) when |g| <= tol |x_i| |x_j|, when a squared norm underflowed to
@@ -214,13 +231,39 @@ __device__ void org2r(const double* __restrict__ A, int lda, int m, int k, const
 // g is not finite, or when the angle rounds to zero; so every rotation
 // reported changes the columns.  For |zeta| > 1e150 the tangent takes its
 // asymptotic form 1 / (2 zeta), where zeta^2 would overflow.
+// The rotation sits on the critical path of every inner Jacobi round, so its
+// divisions and square roots are rcp_newton / rsqrt_newton (the rotation
+// only has to be accurate: cs^2 + sn^2 = 1 to rounding), and the threshold
+// compares squares, for squared norms within 1e+-140 (a b can neither
+// overflow nor underflow); other scales keep the correctly rounded form.
 __device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn)
 {
-  if (a == 0.0 || b == 0.0 || !(fabs(g) > tol * sqrt(a) * sqrt(b))) return false;
-  const double zeta = (b - a) / (2.0 * g);
+  if (a == 0.0 || b == 0.0) return false;
+  if (!(a > 1.0e-140 && a < 1.0e140 && b > 1.0e-140 && b < 1.0e140))
+  {
+    // Extreme scales: the correctly rounded operations throughout.
+    if (!(fabs(g) > tol * sqrt(a) * sqrt(b))) return false;
+    const double zeta = (b - a) / (2.0 * g);
+    const double az = fabs(zeta);
+    const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + sqrt(1.0 + zeta * zeta));
+    cs = 1.0 / sqrt(1.0 + t * t);
+    sn = cs * t;
+    return sn != 0.0;
+  }
+  // Here |g| > tol sqrt(a b) > 1e-155 once past the test: no denormal g.
+  if (!(g * g > (tol * tol) * (a * b))) return false;
+  const double zeta = 0.5 * (b - a) * rcp_newton(g);
   const double az = fabs(zeta);
-  const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + sqrt(1.0 + zeta * zeta));
-  cs = 1.0 / sqrt(1.0 + t * t);
+  double den;
+  if (az > 1.0e150)
+    den = 2.0 * az;
+  else
+  {
+    const double y = fma(zeta, zeta, 1.0);
+    den = az + y * rsqrt_newton(y);
+  }
+  const double t = (zeta >= 0.0 ? 1.0 : -1.0) * rcp_newton(den);
+  cs = rsqrt_newton(fma(t, t, 1.0));
   sn = cs * t;
   return sn != 0.0;
 }
