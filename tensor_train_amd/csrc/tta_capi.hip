@@ -613,6 +613,103 @@ bool cholqr_applies(const tta_ctx* c, int m, int n) { return c->cholqr && m >= n
 // triangle with its reflectors below the diagonal and tau.
 // With cert non-null, a CholQR2 factorization also reports the norms of R and
 // R^-1 there (cert->have); a Householder one leaves cert->have false.
+// Two-level blocked Householder QR of one tall matrix (column-major m x n,
+// in place, thin Q into Q when non-null): the 16-column panels of each
+// 64-column block update only the rest of that block
+// (launch_qr_update_block); the trailing matrix then takes the block's
+// reflectors at once, C <- (I - V T^T V^T) C as W = V^T C, W <- T^T W and
+// C -= V W on the matrix cores (gemm_go, split along K where the output is
+// small), and Q is formed from the identity block by block in reverse order
+// with C <- (I - V T V^T) C.  Same reflectors, R and Q as the one-level
+// form up to rounding.
+void qr_two_level(tta_ctx* c, const qr_batch& b)
+{
+  const int m = b.one.m, n = b.one.n, lda = b.one.lda, k = std::min(m, n);
+  constexpr int NB = 64;
+  const int nblk = (k + NB - 1) / NB;
+  double* V = alloc_core(c, (size_t)m * NB);
+  double* G = alloc_core(c, (size_t)NB * NB);
+  double* Tb = alloc_core(c, (size_t)nblk * NB * NB);
+  double* W = alloc_core(c, (size_t)NB * std::max(n, k));
+  double* W2 = alloc_core(c, (size_t)NB * std::max(n, k));
+  auto release = [&]() {
+    c->mem.put(V);
+    c->mem.put(G);
+    c->mem.put(Tb);
+    c->mem.put(W);
+    c->mem.put(W2);
+  };
+  try
+  {
+    // C (len x nc, column-major, ldc) <- (I - V op(T) V^T) C; op(T) = T^T
+    // when trans, else T.
+    auto apply = [&](int len, int nb, const double* T, bool trans, double* C, int ldc, int nc) {
+      gemm_desc g;
+      g.M = nb, g.N = nc, g.K = len;  // W = V^T C
+      g.A = V, g.sar = len, g.sac = 1;
+      g.B = C, g.sbr = 1, g.sbc = ldc;
+      g.C = W, g.scr = nc, g.scc = 1;
+      gemm_go(c, g);
+      g = gemm_desc();
+      g.M = nb, g.N = nc, g.K = nb;  // W2 = op(T) W
+      g.A = T, g.sar = trans ? 1 : nb, g.sac = trans ? nb : 1;
+      g.B = W, g.sbr = nc, g.sbc = 1;
+      g.C = W2, g.scr = nc, g.scc = 1;
+      gemm_go(c, g);
+      g = gemm_desc();
+      g.M = len, g.N = nc, g.K = nb;  // C -= V W2
+      g.A = V, g.sar = 1, g.sac = len;
+      g.B = W2, g.sbr = nc, g.sbc = 1;
+      g.C = C, g.scr = 1, g.scc = ldc;
+      g.alpha = -1.0;
+      g.beta = 1.0;
+      gemm_go(c, g);
+    };
+    // V and T of the block at column j0 (nb columns).
+    auto block_vt = [&](int j0, int nb, double* T) {
+      const int len = m - j0;
+      launch_qr_vblock(b.one.A, lda, j0, len, nb, V, c->stream);
+      gemm_desc g;
+      g.M = nb, g.N = nb, g.K = len;  // G = V^T V (row-major)
+      g.A = V, g.sar = len, g.sac = 1;
+      g.B = V, g.sbr = 1, g.sbc = len;
+      g.C = G, g.scr = nb, g.scc = 1;
+      gemm_go(c, g);
+      launch_qr_tblock(G, b.one.tau + j0, nb, T, c->stream);
+    };
+    for (int J0 = 0; J0 < k; J0 += NB)
+    {
+      const int nb = std::min(NB, k - J0);
+      for (int j0 = J0; j0 < J0 + nb; j0 += 16)
+      {
+        launch_qr_panel_one(b, j0, c->stream);
+        launch_qr_update_block(b, j0, J0 + nb, c->stream);
+      }
+      double* T = Tb + (size_t)(J0 / NB) * NB * NB;
+      block_vt(J0, nb, T);
+      if (J0 + nb < n) apply(m - J0, nb, T, true, b.one.A + (size_t)(J0 + nb) * lda + J0, lda, n - J0 - nb);
+    }
+    if (b.one.Q)
+    {
+      launch_qr_eye(b, c->stream);
+      for (int J0 = ((k - 1) / NB) * NB; J0 >= 0; J0 -= NB)
+      {
+        const int nb = std::min(NB, k - J0);
+        launch_qr_vblock(b.one.A, lda, J0, m - J0, nb, V, c->stream);
+        apply(m - J0, nb, Tb + (size_t)(J0 / NB) * NB * NB, false, b.one.Q + (size_t)J0 * b.one.ldq + J0, b.one.ldq,
+              k - J0);
+      }
+    }
+    hip_check(hipGetLastError(), "qr_two_level");
+  }
+  catch (...)
+  {
+    release();
+    throw;
+  }
+  release();
+}
+
 bool qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double* Q, int ldq, bool cholqr = false,
                rinv_norms* cert = nullptr)
 {
@@ -635,7 +732,12 @@ bool qr_factor(tta_ctx* c, double* A, int lda, int m, int n, double* tau, double
   b.one.ldq = ldq;
   b.max_m = m;
   b.max_n = n;
-  launch_qr_blocked(b, c->stream);
+  // Tall matrices with many columns: the trailing updates dominate, and the
+  // two-level form reads the trailing matrix a quarter as often.
+  if (m >= 2048 && std::min(m, n) >= 256)
+    qr_two_level(c, b);
+  else
+    launch_qr_blocked(b, c->stream);
   return false;
 }
 

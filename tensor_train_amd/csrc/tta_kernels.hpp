@@ -332,6 +332,15 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
+// Pieces of the two-level blocked QR of one matrix (b.one, no descriptors):
+// the 16-column panel at j0; its update of the columns before col_end only;
+// Q = I; V (len x nb, column-major) of the block of reflectors at column j0;
+// T (nb x nb, row-major, nb <= 64) of a block from G = V^T V and tau.
+void launch_qr_panel_one(const qr_batch& b, int j0, hipStream_t st);
+void launch_qr_update_block(const qr_batch& b, int j0, int col_end, hipStream_t st);
+void launch_qr_eye(const qr_batch& b, hipStream_t st);
+void launch_qr_vblock(const double* A, int lda, int j0, int len, int nb, double* V, hipStream_t st);
+void launch_qr_tblock(const double* G, const double* tau, int nb, double* T, hipStream_t st);
 // Largest n of a CholQR2 factorization.
 int chol_max_n();
 // Diagonal block step of the blocked Cholesky: the jb x jb block (jb <=

@@ -14,6 +14,8 @@
 //     V is staged through LDS in 128-row chunks.
 // The thin Q (m x k, k = min(m, n)) is formed from Q = I by the same update
 // kernel (LAPACK dorgqr order).
+#include <algorithm>
+
 #include "tta_device.hpp"
 #include "tta_kernels.hpp"
 #include "tta_potrf.hpp"
@@ -492,7 +494,7 @@ typedef double qdbl4 __attribute__((ext_vector_type(4)));
 constexpr int UCT = 16;   // columns per workgroup
 constexpr int URC = 64;   // rows per LDS chunk (16 per wave)
 
-__global__ __launch_bounds__(256) void qr_update_mfma_kernel(qr_batch bt, int j0, int mode)
+__global__ __launch_bounds__(256) void qr_update_mfma_kernel(qr_batch bt, int j0, int mode, int col_end)
 {
   const qrp& P = pick(bt);
   const int m = P.m, n = P.n;
@@ -501,7 +503,7 @@ __global__ __launch_bounds__(256) void qr_update_mfma_kernel(qr_batch bt, int j0
   const int bw = (k - j0) < QB ? (k - j0) : QB;
   const int len = m - j0;
   const int cbeg = mode == 0 ? j0 + bw : j0;
-  const int cend = mode == 0 ? n : k;
+  const int cend = mode == 0 ? (col_end > 0 && col_end < n ? col_end : n) : k;
   const int c0 = cbeg + blockIdx.x * UCT;
   if (c0 >= cend) return;
   const int nc = (cend - c0) < UCT ? (cend - c0) : UCT;
@@ -604,6 +606,60 @@ __global__ void qr_eye_kernel(qr_batch bt)
   }
 }
 
+// ---------------------------------------------------------------------------
+// Two-level blocking for single tall matrices: the 16-column panels of a
+// 64-column block update only the rest of that block; the trailing matrix
+// then takes one update with the block's compact-WY form, V (len x nb, unit
+// lower trapezoidal) and T (nb x nb upper triangular, H_1 .. H_nb = I - V T
+// V^T), as three GEMMs on the matrix cores.  The trailing matrix is then read
+// and written once per 64 columns instead of once per 16.
+// ---------------------------------------------------------------------------
+// V (column-major, len x nb) of the block at column j0 from the reflectors
+// stored below A's diagonal: 1 on the diagonal, 0 above.
+__global__ void qr_vblock_kernel(const double* A, int lda, int j0, int len, int nb, double* V)
+{
+  const long total = (long)len * nb;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+  {
+    const int c = (int)(e / len), r = (int)(e % len);
+    V[e] = r == c ? 1.0 : (r > c ? A[(size_t)(j0 + c) * lda + j0 + r] : 0.0);
+  }
+}
+
+// T of the block (row-major nb x nb, nb <= 64) from G = V^T V (row-major)
+// and tau (LAPACK dlarft, forward, columnwise): T(i, i) = tau_i and
+// T(0:i, i) = -tau_i T(0:i, 0:i) G(0:i, i).  One workgroup of 64 threads;
+// thread r keeps row r of T in registers.
+__global__ __launch_bounds__(64) void qr_tblock_kernel(const double* G, const double* tau, int nb, double* T)
+{
+  __shared__ double Gs[64][65];
+  __shared__ double taus[64];
+  const int r = threadIdx.x;
+  for (int e = r; e < nb * nb; e += 64) Gs[e / nb][e % nb] = G[e];
+  if (r < nb) taus[r] = tau[r];
+  __syncthreads();
+  double t[64];
+#pragma unroll
+  for (int c = 0; c < 64; ++c) t[c] = 0.0;
+  for (int i = 0; i < nb; ++i)
+  {
+    double z = 0.0;
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (c >= r && c < i) z = fma(t[c], Gs[c][i], z);
+    const double v = r < i ? -taus[i] * z : (r == i ? taus[i] : 0.0);
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (c == i) t[c] = v;
+  }
+  if (r < nb)
+  {
+#pragma unroll
+    for (int c = 0; c < 64; ++c)
+      if (c < nb) T[(size_t)r * nb + c] = t[c];
+  }
+}
+
 // Largest n that CholQR2 takes (the blocked Cholesky itself has no limit;
 // beyond this the Grams cost more than Householder saves).
 constexpr int kCholMaxN = 512;
@@ -625,6 +681,52 @@ void launch_potrf_inv(const double* G, int ldg, double* R, int ldr, double* X, i
 
 int chol_max_n() { return kCholMaxN; }
 
+
+void launch_qr_panel_one(const qr_batch& b, int j0, hipStream_t st)
+{
+  constexpr int NT = 512;
+  if (b.max_m > 4 * NT)
+    hipLaunchKernelGGL((qr_panel_tall_kernel<NT>), dim3(1), dim3(NT), 0, st, b, j0);
+  else if (b.max_m <= 64)
+    hipLaunchKernelGGL((qr_panel_kernel<64, 1>), dim3(1), dim3(64), 0, st, b, j0);
+  else if (b.max_m <= 256)
+    hipLaunchKernelGGL((qr_panel_kernel<64, 4>), dim3(1), dim3(64), 0, st, b, j0);
+  else if (b.max_m <= NT)
+    hipLaunchKernelGGL((qr_panel_kernel<NT, 1>), dim3(1), dim3(NT), 0, st, b, j0);
+  else if (b.max_m <= 2 * NT)
+    hipLaunchKernelGGL((qr_panel_kernel<NT, 2>), dim3(1), dim3(NT), 0, st, b, j0);
+  else
+    hipLaunchKernelGGL((qr_panel_kernel<NT, 4>), dim3(1), dim3(NT), 0, st, b, j0);
+}
+
+void launch_qr_update_block(const qr_batch& b, int j0, int col_end, hipStream_t st)
+{
+  const int k = std::min(b.one.m, b.one.n);
+  const int bw = std::min(k - j0, QB);
+  const int ncols = std::min(col_end, b.one.n) - (j0 + bw);
+  if (ncols <= 0) return;
+  hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((ncols + UCT - 1) / UCT, 1), dim3(256), 0, st, b, j0, 0, col_end);
+}
+
+void launch_qr_eye(const qr_batch& b, hipStream_t st)
+{
+  const long total = (long)b.max_m * std::min(b.max_m, b.max_n);
+  long g = (total + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(qr_eye_kernel, dim3((int)g, 1), dim3(256), 0, st, b);
+}
+
+void launch_qr_vblock(const double* A, int lda, int j0, int len, int nb, double* V, hipStream_t st)
+{
+  long g = ((long)len * nb + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(qr_vblock_kernel, dim3((int)g), dim3(256), 0, st, A, lda, j0, len, nb, V);
+}
+
+void launch_qr_tblock(const double* G, const double* tau, int nb, double* T, hipStream_t st)
+{
+  hipLaunchKernelGGL(qr_tblock_kernel, dim3(1), dim3(64), 0, st, G, tau, nb, T);
+}
 
 void launch_qr_blocked(const qr_batch& b, hipStream_t st)
 {
@@ -661,7 +763,7 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
     if (ncols > 0)
     {
       if (mfma_update)
-        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((ncols + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 0);
+        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((ncols + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 0, 0);
       else
         hipLaunchKernelGGL(qr_update_kernel, dim3((ncols + QCT - 1) / QCT, batch), dim3(256), 0, st, b, j0, 0);
     }
@@ -683,7 +785,8 @@ void launch_qr_blocked(const qr_batch& b, hipStream_t st)
     for (int j0 = last; j0 >= 0; j0 -= QB)
     {
       if (mfma_update)
-        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((kmax - j0 + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 1);
+        hipLaunchKernelGGL(qr_update_mfma_kernel, dim3((kmax - j0 + UCT - 1) / UCT, batch), dim3(256), 0, st, b, j0, 1,
+                           0);
       else
         hipLaunchKernelGGL(qr_update_kernel, dim3((kmax - j0 + QCT - 1) / QCT, batch), dim3(256), 0, st, b, j0, 1);
     }
