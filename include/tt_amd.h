@@ -411,6 +411,20 @@ int tta_set_speculation(tta_ctx* ctx, int on);
 /* Speculative sweeps run and those undone, since the last tta_prof_enable. */
 int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
 
+/* ---- chained sweep blocks (no reference counterpart) ----------------------- */
+/* Inside a speculative sweep, a run of two or more consecutive QR steps of
+ * CholQR2's shape (and, in the SVD sweeps, steps whose full-rank certificate
+ * max_rank cannot overrule) is factored as one block: the steps' Gram
+ * matrices follow one another by two products each (G_k = sum_i C_i G_{k+1}
+ * C_i^T), and the Choleskys, the orthonormalizing products and a second
+ * pass run for all steps of the block at once.  Exact arithmetic gives the
+ * sequential sweep's result; any failed decision undoes the sweep as above.
+ * On by default. */
+int tta_set_chain(tta_ctx* ctx, int on);
+/* Chained blocks run and the steps they covered, since the last
+ * tta_prof_enable. */
+int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps);
+
 /* ---- HIP graphs (no reference counterpart) ------------------------------ */
 /* Block-Jacobi SVD sweeps (more than 64 columns, single problems and
  * batches) captured into a HIP graph at the first sweep of each call and

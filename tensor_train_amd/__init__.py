@@ -135,6 +135,18 @@ class Context:
         with a failed decision is undone and run again checked."""
         self.check(self.lib.tta_set_speculation(self.h, 1 if on else 0))
 
+    def set_chain(self, on: bool) -> None:
+        """Chained sweep blocks inside speculative sweeps (on by default): runs
+        of CholQR2-shaped steps factored together, with the Grams chained by
+        products and the Choleskys batched off the critical path."""
+        self.check(self.lib.tta_set_chain(self.h, 1 if on else 0))
+
+    def chain_counts(self):
+        """(blocks, steps) of chained sweep blocks since the last prof_enable."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_chain(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def speculation_counts(self):
         """(sweeps, failed) speculative sweep counts since the last prof_enable."""
         a, b = ctypes.c_longlong(), ctypes.c_longlong()

@@ -771,6 +771,310 @@ double frob_norm_dev(tta_ctx* c, const double* x, size_t n, double eps, double i
 }
 
 // ---------------------------------------------------------------------------
+// Chained sweep blocks.
+//
+// A run of consecutive QR steps of a sweep, each of them CholQR2's shape, is
+// factored with the Choleskys off the sweep's critical path.  Right sweep
+// (dir 0; steps kb, kb-1, .., ka): step k factors the right unfolding of
+// core k after core k+1's factor R_{k+1} has been absorbed, whose Gram is
+//   G_k = sum_i C_k[:, i, :] G_{k+1} C_k[:, i, :]^T     (G_{k+1} = R^T R),
+// so the Grams follow from the cores alone, one after another, by two
+// products each, with no factorization between them.  Left sweep (dir 1;
+// steps ka .. kb, left unfoldings) is the mirror:
+//   H_k = sum_i C_k[:, i, :]^T H_{k-1} C_k[:, i, :].
+// Then, all steps at once:
+//   pass 1  S_k = chol(G_k) and S_k^-1 (batched blocked Cholesky), and the
+//           cores N_k = S_k^-T C_k S_{k+1}^T (dir 0) or S_{k-1} C_k S_k^-1
+//           (dir 1), whose unfoldings are orthonormal up to the first pass's
+//           error E ~ u kappa^2;
+//   pass 2  the same chain of Grams over the N_k (G2_k = I + E_k), then
+//           P_k = (I + E_k)^(-1/2) and Pi_k = (I + E_k)^(1/2) by their
+//           second-order series (CholQR2's polar form, exact to O(|E|^3)),
+//           and the final cores P_k N_k Pi_{k+1} (dir 0) or Pi_{k-1} N_k P_k
+//           (dir 1); the factor Rf = Pi S of the block's last step is
+//           absorbed into the core past the block.
+// In exact arithmetic S_k is the R of step k's QR and E = 0, so the result
+// is the sequential sweep's.  Rounding: the chain's G_k and the factor
+// S_{k+1} it is used with are consistent to the Cholesky's backward error,
+// as in the sequential CholQR's Gram of the absorbed core, so E has the
+// sequential first pass's size.  The acceptance (every Cholesky held, every
+// n max|E| <= 1e-5, the polar form's range) and, for the SVD sweep, every
+// step's full-rank certificate (column norms of Rf = Pi S and Rf^-1 =
+// S^-1 P) are decided on the device into the speculative sweep's flag: a
+// block runs only inside a speculative sweep, which is undone and run again
+// step by step when any decision fails.
+// ---------------------------------------------------------------------------
+constexpr int kChainMin = 2;
+
+bool cert_ok(int max_rank, int q);
+
+// One batched launch of independent GEMMs (grouped kernel, descriptors
+// uploaded through the arena).
+void gemm_batch_go(tta_ctx* c, arena& ar, const std::vector<gemm_desc>& gs)
+{
+  if (gs.empty()) return;
+  int mm = 0, mn = 0;
+  const credit cr = gemm_credit(gs.data(), gs.size());
+  for (const gemm_desc& g : gs)
+  {
+    mm = std::max(mm, g.M);
+    mn = std::max(mn, g.N);
+  }
+  prof_scope ps(c, TTA_K_GEMM_MULTI, cr.flops, cr.bytes);
+  launch_gemm_grouped(ar.upload(gs), (int)gs.size(), mm, mn, c->stream);
+}
+
+// Whether step k of a sweep in direction dir may join a chained block: a
+// speculative sweep, CholQR2's shape, a core to absorb into, and (with
+// cert) a certificate that max_rank cannot overrule.
+bool chain_step_ok(const tta_ctx* c, const tta_tt* t, int k, int dir, bool cert, int max_rank)
+{
+  if (!c->chain || !c->spec) return false;
+  const int d = t->d();
+  if (dir == 0 ? k < 1 : k > d - 2) return false;
+  const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+  const int m = dir == 0 ? n * rp : r * n, q = dir == 0 ? r : rp;
+  if (!cholqr_applies(c, m, q)) return false;
+  if (cert && (!c->rank_certificate || !cert_ok(max_rank, q) || q > kRinvCertMaxQ)) return false;
+  return true;
+}
+
+// The chained block of steps ka .. kb (see above).  With cert, each step's
+// full-rank certificate for delta is recorded for the sweep's end.
+void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, double delta)
+{
+  const int L = kb - ka + 1;
+  std::vector<int> ks(L), qs(L);
+  for (int p = 0; p < L; ++p)
+  {
+    ks[p] = dir == 0 ? kb - p : ka + p;
+    qs[p] = dir == 0 ? t->ranks[ks[p]] : t->ranks[ks[p] + 1];
+  }
+  arena ar(c);
+  std::vector<chain_prob> pr(L);
+  int max_n = 0;
+  size_t max_core = 0;
+  for (int p = 0; p < L; ++p)
+  {
+    const size_t qq = (size_t)qs[p] * qs[p];
+    chain_prob& b = pr[p];
+    b.n = qs[p];
+    for (double** m : {&b.G, &b.R, &b.X, &b.E, &b.V, &b.P, &b.Pi, &b.Rf, &b.Y}) *m = ar.alloc<double>(qq);
+    b.S = ar.alloc<double>(chol_step_scratch(qs[p]));
+    b.info = ar.alloc<double>(8);
+    b.part = ar.alloc<double>(kChainParts);
+    if (cert)
+    {
+      // Read by the sweep's certificate launch at its end.
+      b.norms = alloc_core(c, 2 * (size_t)qs[p]);
+      c->spec_cert_bufs.push_back(b.norms);
+    }
+    max_n = std::max(max_n, qs[p]);
+    max_core = std::max(max_core, core_numel(t, ks[p]));
+  }
+  double* T = ar.alloc<double>(max_core);
+  std::vector<double*> C(L), N(L), U(L), O(L);
+  for (int p = 0; p < L; ++p)
+  {
+    C[p] = t->cores[ks[p]];
+    N[p] = ar.alloc<double>(core_numel(t, ks[p]));
+    U[p] = ar.alloc<double>(core_numel(t, ks[p]));
+  }
+  // The chain of Grams over the cores X[p] into out(p).
+  auto chain = [&](const std::vector<double*>& X, double* chain_prob::*out)
+  {
+    for (int p = 0; p < L; ++p)
+    {
+      const int k = ks[p], r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+      const double* src = X[p];
+      if (dir == 0)
+      {
+        if (p > 0)
+        {
+          gemm_raw(c, r * n, rp, rp, X[p], rp, 1, pr[p - 1].*out, rp, 1, T, rp, 1);  // T = X G_{k+1}
+          src = T;
+        }
+        gemm_raw(c, r, r, n * rp, src, (long)n * rp, 1, X[p], 1, (long)n * rp, pr[p].*out, r, 1);  // T X^T
+      }
+      else
+      {
+        if (p > 0)
+        {
+          gemm_raw(c, r, n * rp, r, pr[p - 1].*out, r, 1, X[p], (long)n * rp, 1, T, (long)n * rp, 1);  // H_{k-1} X
+          src = T;
+        }
+        gemm_raw(c, rp, rp, r * n, src, 1, rp, X[p], rp, 1, pr[p].*out, rp, 1);  // T^T X (left unfoldings)
+      }
+    }
+  };
+  chain(C, &chain_prob::G);
+  chain_prob* dps = ar.upload(pr);
+  const int B = potrf_block();
+  {
+    double fl = 0.0, by = 0.0;
+    for (int q : qs)
+    {
+      const double jb = std::min(B, q);
+      fl += 2.0 * jb * jb * jb / 3.0;
+      by += 24.0 * jb * jb;
+    }
+    prof_scope ps(c, TTA_K_CHOL, fl, by);
+    launch_chain_potrf(dps, L, c->stream);
+  }
+  for (int j0 = 0; j0 < max_n; j0 += B)
+  {
+    // Work of each problem's block step, as chol_inv_blocked credits it.
+    double fl = 0.0, by = 0.0;
+    for (int q : qs)
+    {
+      if (j0 >= q) continue;
+      const double jb = std::min(B, q - j0), w = q - j0 - jb, nb = std::min<double>(B, w);
+      fl += 2.0 * jb * jb * w + 2.0 * w * w * jb + 2.0 * j0 * jb * (j0 + jb) + 2.0 * nb * nb * nb / 3.0;
+      by += 8.0 * (2.0 * jb * w + 2.0 * w * w + 2.0 * j0 * j0 + 3.0 * j0 * jb + 3.0 * nb * nb);
+    }
+    prof_scope ps(c, TTA_K_CHOL_STEP, fl, by);
+    launch_chain_chol_step(dps, L, max_n, j0, c->stream);
+  }
+  // Pass 1 cores.
+  {
+    std::vector<gemm_desc> g1, g2;
+    for (int p = 0; p < L; ++p)
+    {
+      const int k = ks[p], r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+      const long w = (long)n * rp;
+      const double* src = C[p];
+      if (dir == 0)
+      {
+        if (p > 0)
+        {
+          g1.push_back(gdesc(r * n, rp, rp, C[p], rp, 1, pr[p - 1].R, 1, rp, U[p], rp, 1));  // C S_{k+1}^T
+          src = U[p];
+        }
+        g2.push_back(gdesc(r, n * rp, r, pr[p].X, 1, r, src, w, 1, N[p], w, 1));  // S_k^-T (.)
+      }
+      else
+      {
+        if (p > 0)
+        {
+          g1.push_back(gdesc(r, n * rp, r, pr[p - 1].R, r, 1, C[p], w, 1, U[p], w, 1));  // S_{k-1} C
+          src = U[p];
+        }
+        g2.push_back(gdesc(r * n, rp, rp, src, rp, 1, pr[p].X, rp, 1, N[p], rp, 1));  // (.) S_k^-1
+      }
+    }
+    gemm_batch_go(c, ar, g1);
+    gemm_batch_go(c, ar, g2);
+  }
+  // Pass 2.
+  chain(N, &chain_prob::E);
+  {
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+    launch_chain_minus_eye(dps, L, max_n, c->stream);
+  }
+  {
+    std::vector<gemm_desc> gv;
+    for (int p = 0; p < L; ++p)
+    {
+      const int q = qs[p];
+      gv.push_back(gdesc(q, q, q, pr[p].E, q, 1, pr[p].E, q, 1, pr[p].V, q, 1));  // V = E E
+    }
+    gemm_batch_go(c, ar, gv);
+  }
+  {
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+    launch_chain_polar(dps, L, max_n, c->stream);
+  }
+  for (int p = 0; p < L; ++p) O[p] = alloc_core(c, core_numel(t, ks[p]));
+  {
+    std::vector<gemm_desc> g1, g2, gr;
+    for (int p = 0; p < L; ++p)
+    {
+      const int k = ks[p], r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1], q = qs[p];
+      const long w = (long)n * rp;
+      const double* src = N[p];
+      if (dir == 0)
+      {
+        if (p > 0)
+        {
+          g1.push_back(gdesc(r * n, rp, rp, N[p], rp, 1, pr[p - 1].Pi, rp, 1, U[p], rp, 1));  // N Pi_{k+1}
+          src = U[p];
+        }
+        g2.push_back(gdesc(r, n * rp, r, pr[p].P, r, 1, src, w, 1, O[p], w, 1));  // P_k (.)
+      }
+      else
+      {
+        if (p > 0)
+        {
+          g1.push_back(gdesc(r, n * rp, r, pr[p - 1].Pi, r, 1, N[p], w, 1, U[p], w, 1));  // Pi_{k-1} N
+          src = U[p];
+        }
+        g2.push_back(gdesc(r * n, rp, rp, src, rp, 1, pr[p].P, rp, 1, O[p], rp, 1));  // (.) P_k
+      }
+      if (cert || p == L - 1)
+      {
+        gr.push_back(gdesc(q, q, q, pr[p].Pi, q, 1, pr[p].R, q, 1, pr[p].Rf, q, 1));  // Rf = Pi S
+        if (cert) gr.push_back(gdesc(q, q, q, pr[p].X, q, 1, pr[p].P, q, 1, pr[p].Y, q, 1));  // Rf^-1 = S^-1 P
+      }
+    }
+    gemm_batch_go(c, ar, g1);
+    gemm_batch_go(c, ar, g2);
+    gemm_batch_go(c, ar, gr);
+  }
+  {
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+    launch_chain_decide(dps, L, 1.0e-5, spec_flag(c), c->stream);
+    if (cert) launch_chain_norms(dps, L, max_n, c->stream);
+  }
+  // The last step's factor into the core past the block.
+  const double* Rf = pr[L - 1].Rf;
+  const int q = qs[L - 1];
+  if (dir == 0)
+  {
+    const int kp = ka - 1, rows_p = t->ranks[kp] * t->modes[kp];
+    double* Pn = alloc_core(c, (size_t)rows_p * q);
+    gemm_go(c, gdesc(rows_p, q, q, t->cores[kp], q, 1, Rf, 1, q, Pn, q, 1));  // core Rf^T
+    replace_core(c, t, kp, Pn);
+  }
+  else
+  {
+    const int kn = kb + 1, cols = t->modes[kn] * t->ranks[kn + 1];
+    double* Nn = alloc_core(c, (size_t)q * cols);
+    gemm_go(c, gdesc(q, cols, q, Rf, q, 1, t->cores[kn], cols, 1, Nn, cols, 1));  // Rf core
+    replace_core(c, t, kn, Nn);
+  }
+  for (int p = 0; p < L; ++p)
+  {
+    replace_core(c, t, ks[p], O[p]);
+    if (cert)
+    {
+      c->spec_certs.push_back({pr[p].norms, qs[p], 0, delta});
+      c->cert_tried += 1;
+      c->cert_held += 1;
+    }
+  }
+  c->cholqr_tried += L;
+  c->cholqr_used += L;
+  c->chain_blocks += 1;
+  c->chain_steps += L;
+}
+
+// The end of the run of chainable steps that starts at step k of a sweep in
+// direction dir (dir 0 runs downward), or k itself when the run is shorter
+// than kChainMin.
+int chain_run_end(const tta_ctx* c, const tta_tt* t, int k, int dir, bool cert, int max_rank)
+{
+  const int d = t->d();
+  int e = k;
+  while (true)
+  {
+    const int nx = dir == 0 ? e - 1 : e + 1;
+    if (nx < 0 || nx > d - 1 || !chain_step_ok(c, t, nx, dir, cert, max_rank)) break;
+    e = nx;
+  }
+  return std::abs(e - k) + 1 >= kChainMin ? e : k;
+}
+
+// ---------------------------------------------------------------------------
 // right_qr_step (include/core/detail/orthogonalize.hpp:63-100): QR of the
 // column-major (n*rp x r) view of core k (its own bytes, factored in place),
 // thin Q -> new core (r_eff, n, rp), L = R^T absorbed into core k-1.  With
@@ -1312,18 +1616,42 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
   absorb_right_step(c, t, k, rows, r_new, Cn, F);
 }
 
+// Runs of chainable steps (chain_block) inside speculative sweeps, single
+// steps otherwise.
 void right_orthogonalize_impl(tta_ctx* c, tta_tt* t)
 {
   const int d = t->d();
   if (d <= 1) return;
-  for (int k = d - 1; k >= 1; --k) right_qr_step(c, t, k);
+  for (int k = d - 1; k >= 1;)
+  {
+    const int e = chain_step_ok(c, t, k, 0, false, 0) ? chain_run_end(c, t, k, 0, false, 0) : k;
+    if (e != k)
+    {
+      chain_block(c, t, e, k, 0, false, 0.0);
+      k = e - 1;
+      continue;
+    }
+    right_qr_step(c, t, k);
+    --k;
+  }
 }
 
 void left_orthogonalize_impl(tta_ctx* c, tta_tt* t)
 {
   const int d = t->d();
   if (d <= 1) return;
-  for (int k = 0; k < d - 1; ++k) left_qr_step(c, t, k);
+  for (int k = 0; k < d - 1;)
+  {
+    const int e = chain_step_ok(c, t, k, 1, false, 0) ? chain_run_end(c, t, k, 1, false, 0) : k;
+    if (e != k)
+    {
+      chain_block(c, t, k, e, 1, false, 0.0);
+      k = e + 1;
+      continue;
+    }
+    left_qr_step(c, t, k);
+    ++k;
+  }
 }
 
 void make_rank1_zero(tta_ctx* c, tta_tt* t)
@@ -1495,7 +1823,23 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
       return;
     }
     const double delta = eps * nrm * inv;
-    run_sweep(c, t, c->spec_svd_ok, [&] { for (int k = d - 1; k >= 1; --k) svd_step_right(c, t, k, delta, max_rank); });
+    run_sweep(c, t, c->spec_svd_ok,
+              [&]
+              {
+                for (int k = d - 1; k >= 1;)
+                {
+                  const int e =
+                      chain_step_ok(c, t, k, 0, true, max_rank) ? chain_run_end(c, t, k, 0, true, max_rank) : k;
+                  if (e != k)
+                  {
+                    chain_block(c, t, e, k, 0, true, delta);
+                    k = e - 1;
+                    continue;
+                  }
+                  svd_step_right(c, t, k, delta, max_rank);
+                  --k;
+                }
+              });
     return;
   }
   run_sweep(c, t, c->spec_qr_ok, [&] { right_orthogonalize_impl(c, t); });
@@ -1506,7 +1850,22 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
     return;
   }
   const double delta = eps * nrm * inv;
-  run_sweep(c, t, c->spec_svd_ok, [&] { for (int k = 0; k < d - 1; ++k) svd_step(c, t, k, delta, max_rank); });
+  run_sweep(c, t, c->spec_svd_ok,
+            [&]
+            {
+              for (int k = 0; k < d - 1;)
+              {
+                const int e = chain_step_ok(c, t, k, 1, true, max_rank) ? chain_run_end(c, t, k, 1, true, max_rank) : k;
+                if (e != k)
+                {
+                  chain_block(c, t, k, e, 1, true, delta);
+                  k = e + 1;
+                  continue;
+                }
+                svd_step(c, t, k, delta, max_rank);
+                ++k;
+              }
+            });
   balance(c, t);
 }
 
@@ -3066,6 +3425,8 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->cholqr_triangular = 0;
   c->spec_sweeps = 0;
   c->spec_failed = 0;
+  c->chain_blocks = 0;
+  c->chain_steps = 0;
   TTA_GUARD_END(c)
 }
 
@@ -3096,6 +3457,21 @@ int tta_prof_speculation(tta_ctx* c, long long* sweeps, long long* failed)
   if (!c) return TTA_ERR_ARG;
   if (sweeps) *sweeps = c->spec_sweeps;
   if (failed) *failed = c->spec_failed;
+  return TTA_OK;
+}
+
+int tta_set_chain(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->chain = on != 0;
+  return TTA_OK;
+}
+
+int tta_prof_chain(tta_ctx* c, long long* blocks, long long* steps)
+{
+  if (!c) return TTA_ERR_ARG;
+  if (blocks) *blocks = c->chain_blocks;
+  if (steps) *steps = c->chain_steps;
   return TTA_OK;
 }
 

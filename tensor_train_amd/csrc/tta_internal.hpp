@@ -131,6 +131,10 @@ struct tta_ctx
   // the last sweep of each kind needed no fallback (speculation resumes
   // after a checked sweep that needed none).
   bool speculate = true, spec = false;
+  // Chained sweep blocks inside speculative sweeps (on by default): blocks
+  // run and the steps they covered.
+  bool chain = true;
+  long long chain_blocks = 0, chain_steps = 0;
   bool spec_qr_ok = true, spec_svd_ok = true;
   long long spec_sweeps = 0, spec_failed = 0;
   // Certificate decisions of the running speculative sweep, made together

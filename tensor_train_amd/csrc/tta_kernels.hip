@@ -303,11 +303,10 @@ __global__ __launch_bounds__(256) void gemm_f64_multi_kernel(gemm_multi m)
 //   * X tile i (rows 64 i .. of the first j0): T = X(rows, 0:j0) R(0:j0,
 //     j0:j1) into scratch, then X(rows, j0:j1) = -T D.
 // D = R_jj^-1, the step's diagonal block of X, factored by the launch before.
-__global__ __launch_bounds__(256) void chol_step_kernel(chol_step s)
+__device__ __forceinline__ void chol_step_run(const chol_step& s, int t)
 {
   const int n = s.n, j0 = s.j0, jb = s.jb, j1 = j0 + jb, w = n - j1;
   const double* D = s.X + (size_t)j0 * n + j0;
-  const int t = blockIdx.x;
   double* S = s.S + (size_t)t * 2 * GT * GT;
   if (t < s.ntri)
   {
@@ -353,6 +352,119 @@ __global__ __launch_bounds__(256) void chol_step_kernel(chol_step s)
                                                 0.0, 0, 0);
   }
 }
+
+__global__ __launch_bounds__(256) void chol_step_kernel(chol_step s) { chol_step_run(s, blockIdx.x); }
+
+// ---------------------------------------------------------------------------
+// Batched pieces of a chained sweep (chain_prob in the header): problem
+// blockIdx.y (blockIdx.x for the first diagonal blocks) of a device array.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void chain_potrf_kernel(const chain_prob* ps)
+{
+  const int p = blockIdx.x;
+  double* G = ps[p].G;
+  double* R = ps[p].R;
+  double* X = ps[p].X;
+  double* info = ps[p].info;
+  const int n = ps[p].n;
+  potrf_inv_block(G, n, R, n, X, n, min(kPotrfB, n), 0, info);
+}
+
+__global__ __launch_bounds__(256) void chain_chol_step_kernel(const chain_prob* ps, int j0)
+{
+  const int p = blockIdx.y;
+  const int n = ps[p].n;
+  if (j0 >= n) return;
+  chol_step s;
+  s.n = n;
+  s.j0 = j0;
+  s.jb = min(kPotrfB, n - j0);
+  const int w = n - j0 - s.jb;
+  s.nt = (w + GT - 1) / GT;
+  s.ntri = s.nt * (s.nt + 1) / 2;
+  const int nx = (j0 + GT - 1) / GT;
+  if ((int)blockIdx.x >= s.ntri + nx) return;
+  s.G = ps[p].G;
+  s.R = ps[p].R;
+  s.X = ps[p].X;
+  s.S = ps[p].S;
+  s.info = ps[p].info;
+  chol_step_run(s, blockIdx.x);
+}
+
+// E <- E - I; block b of problem p writes the largest |E| of rows b, b +
+// kChainParts, ... to part[b].
+__global__ __launch_bounds__(256) void chain_minus_eye_kernel(const chain_prob* ps)
+{
+  __shared__ double red[4];
+  const int p = blockIdx.y, n = ps[p].n;
+  double* E = ps[p].E;
+  double m = 0.0;
+  for (int i = blockIdx.x; i < n; i += kChainParts)
+    for (int j = threadIdx.x; j < n; j += 256)
+    {
+      const size_t e = (size_t)i * n + j;
+      const double v = E[e] - (i == j ? 1.0 : 0.0);
+      E[e] = v;
+      m = fmax(m, fabs(v) <= DBL_MAX ? fabs(v) : INFINITY);
+    }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if (lane == 0) red[wid] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) ps[p].part[blockIdx.x] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+}
+
+__global__ __launch_bounds__(256) void chain_polar_kernel(const chain_prob* ps)
+{
+  const int p = blockIdx.y, n = ps[p].n;
+  const double* E = ps[p].E;
+  const double* V = ps[p].V;
+  double* P = ps[p].P;
+  double* Pi = ps[p].Pi;
+  const int total = n * n;
+  for (int e = blockIdx.x * 256 + threadIdx.x; e < total; e += gridDim.x * 256)
+  {
+    const double id = (e / n == e % n) ? 1.0 : 0.0;
+    const double x = E[e], v = V[e];
+    P[e] = id - 0.5 * x + 0.375 * v;
+    Pi[e] = id + 0.5 * x - 0.125 * v;
+  }
+}
+
+// Column c < 2n of problem p (four per block, one wave each): Rf's columns,
+// then Y's.
+__global__ __launch_bounds__(256) void chain_norms_kernel(const chain_prob* ps)
+{
+  const int p = blockIdx.y, n = ps[p].n;
+  const int lane = threadIdx.x & 63;
+  const int col = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (col >= 2 * n) return;
+  const double* M = col < n ? ps[p].Rf : ps[p].Y;
+  const int c = col < n ? col : col - n;
+  double acc = 0.0;
+  for (int i = lane; i < n; i += 64) acc = fma(M[(size_t)i * n + c], M[(size_t)i * n + c], acc);
+  acc = wave_sum(acc);
+  if (lane == 0) ps[p].norms[col] = sqrt(acc);
+}
+
+// One wave per problem: the Cholesky held and n max|E| <= thr, else the
+// flag is cleared.
+__global__ __launch_bounds__(64) void chain_decide_kernel(const chain_prob* ps, double thr, double* flag)
+{
+  const int p = blockIdx.x, lane = threadIdx.x;
+  const double* part = ps[p].part;
+  double emax = part[lane];
+  emax = fmax(emax, __shfl_xor(emax, 32, 64));
+#pragma unroll
+  for (int o = 16; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+  if (lane != 0) return;
+  const double* info = ps[p].info;
+  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * ps[p].n <= thr;
+  if (!ok) flag[0] = 0.0;
+}
+static_assert(kChainParts == 64, "chain_decide_kernel reads one partial maximum per lane");
 
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.  TT: the tile size (64, or 32 for small outputs).
@@ -1245,6 +1357,49 @@ void launch_chol_step(const chol_step& s, hipStream_t st)
   const int wgs = s.ntri + (s.j0 + GT - 1) / GT;
   if (wgs <= 0) return;
   hipLaunchKernelGGL(chol_step_kernel, dim3(wgs), dim3(256), 0, st, s);
+}
+
+void launch_chain_potrf(const chain_prob* ps, int count, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(chain_potrf_kernel, dim3(count), dim3(256), 0, st, ps);
+}
+
+void launch_chain_chol_step(const chain_prob* ps, int count, int max_n, int j0, hipStream_t st)
+{
+  if (count <= 0 || j0 >= max_n) return;
+  // The most workgroups any problem's step at j0 uses: the largest n has the
+  // most trailing tiles, and every problem with n > j0 has the same X tiles.
+  int jb, nt, ntri, nx;
+  chol_step_shape(max_n, j0, jb, nt, ntri, nx);
+  const int wgs = ntri + nx;
+  if (wgs <= 0) return;
+  hipLaunchKernelGGL(chain_chol_step_kernel, dim3(wgs, count), dim3(256), 0, st, ps, j0);
+}
+
+void launch_chain_minus_eye(const chain_prob* ps, int count, int max_n, hipStream_t st)
+{
+  if (count <= 0 || max_n <= 0) return;
+  hipLaunchKernelGGL(chain_minus_eye_kernel, dim3(kChainParts, count), dim3(256), 0, st, ps);
+}
+
+void launch_chain_polar(const chain_prob* ps, int count, int max_n, hipStream_t st)
+{
+  if (count <= 0 || max_n <= 0) return;
+  const int blocks = std::min((max_n * max_n + 255) / 256, 256);
+  hipLaunchKernelGGL(chain_polar_kernel, dim3(blocks, count), dim3(256), 0, st, ps);
+}
+
+void launch_chain_norms(const chain_prob* ps, int count, int max_n, hipStream_t st)
+{
+  if (count <= 0 || max_n <= 0) return;
+  hipLaunchKernelGGL(chain_norms_kernel, dim3((2 * max_n + 3) / 4, count), dim3(256), 0, st, ps);
+}
+
+void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(chain_decide_kernel, dim3(count), dim3(64), 0, st, ps, thr, flag);
 }
 
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st)

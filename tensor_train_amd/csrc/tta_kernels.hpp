@@ -285,6 +285,50 @@ struct chol_step
 void chol_step_shape(int n, int j0, int& jb, int& nt, int& ntri, int& nx);
 size_t chol_step_scratch(int n);
 void launch_chol_step(const chol_step& s, hipStream_t st);
+
+// One factorization of a chained sweep (device arrays of these; every
+// matrix n x n row-major):
+//   G     SPD Gram, overwritten by the Cholesky's trailing updates;
+//   R, X  its Cholesky factor (G = R^T R, upper) and R^-1;
+//   S     chol_step_scratch(n) doubles; info 3 doubles (as launch_potrf_inv);
+//   E     second-pass Gram on entry to launch_chain_minus_eye, E = G2 - I
+//         after it, with kChainParts partial maxima of |E| in part;
+//   V     E E; P, Pi the second-order series of (I + E)^(-1/2) and
+//         (I + E)^(1/2) (launch_chain_polar);
+//   Rf, Y the factor Pi R and its inverse X P, whose column norms go to
+//         norms (2n doubles: Rf's, then Y's) for the full-rank certificate.
+struct chain_prob
+{
+  double* G = nullptr;
+  double* R = nullptr;
+  double* X = nullptr;
+  double* S = nullptr;
+  double* info = nullptr;
+  double* E = nullptr;
+  double* V = nullptr;
+  double* P = nullptr;
+  double* Pi = nullptr;
+  double* part = nullptr;
+  double* Rf = nullptr;
+  double* Y = nullptr;
+  double* norms = nullptr;
+  int n = 0;
+};
+constexpr int kChainParts = 64;
+// The blocked Cholesky with inverse (chol_inv_blocked's steps) of `count`
+// problems at once: the first diagonal block of each, then block step j0 of
+// every problem with n > j0 (max_n bounds the n's).
+void launch_chain_potrf(const chain_prob* ps, int count, hipStream_t st);
+void launch_chain_chol_step(const chain_prob* ps, int count, int max_n, int j0, hipStream_t st);
+// E <- E - I with the partial maxima of |E| (non-finite as infinity).
+void launch_chain_minus_eye(const chain_prob* ps, int count, int max_n, hipStream_t st);
+// P = I - E/2 + 3/8 V, Pi = I + E/2 - 1/8 V.
+void launch_chain_polar(const chain_prob* ps, int count, int max_n, hipStream_t st);
+// Column norms of Rf and Y into norms.
+void launch_chain_norms(const chain_prob* ps, int count, int max_n, hipStream_t st);
+// flag[0] = 0 unless every problem's Cholesky held (info[0] == 0, finite
+// extreme diagonal entries) and n max|E| <= thr.
+void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st);
 // Epilogues folded into the split-K sum of a square row-major n x n product
 // v = alpha P + beta C (CholQR2's second pass):
 //   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E) (if given), part[b] = max |E| over

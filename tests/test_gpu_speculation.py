@@ -8,7 +8,8 @@ decisions.  Checked here: speculation on and off give bit-identical cores
 product, both gauges, and the naive matvec_round built on round) and where
 some fail (config 1, whose bonds the rank rule cuts; a zero-padded TT, on
 which CholQR2 breaks down); the counters show each fallback, and after one
-the next sweep of that kind runs checked."""
+the next sweep of that kind runs checked.  Chained blocks are switched off
+here (their own tests are in test_gpu_chain.py)."""
 import numpy as np
 import pytest
 
@@ -25,6 +26,9 @@ def cores3(t):
 
 
 def _run(ctx, fn, on):
+    # Chained blocks (which run only in speculative sweeps) switched off, so
+    # that both runs factor step by step and their cores must be identical.
+    ctx.set_chain(False)
     ctx.set_speculation(on)
     ctx.prof_enable(True)
     try:
@@ -33,6 +37,7 @@ def _run(ctx, fn, on):
     finally:
         ctx.prof_enable(False)
         ctx.set_speculation(True)
+        ctx.set_chain(True)
 
 
 def _same(a, b):
