@@ -1086,6 +1086,24 @@ void right_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int m = n * rp;
   const int r_eff = std::min(m, r);
+  if (absorb && m <= r)
+  {
+    // The unfolding's r columns span at most R^m, and Q = I_m is an
+    // orthonormal basis of R^m: R = Q^T M^T = M^T, so core k-1 takes core
+    // k whole (one product) and core k becomes the identity, r_eff = m.
+    const int rows_p = t->ranks[k - 1] * t->modes[k - 1];
+    double* Pn = alloc_core(c, (size_t)rows_p * m);
+    gemm_rm(c, rows_p, m, r, t->cores[k - 1], r, t->cores[k], m, Pn, m);
+    double* Cn = alloc_core(c, (size_t)m * m);
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 8.0 * (double)m * m);
+      launch_eye(Cn, m, m, c->stream);
+    }
+    replace_core(c, t, k - 1, Pn);
+    replace_core(c, t, k, Cn);
+    t->ranks[k] = m;
+    return;
+  }
   double* A = t->cores[k];
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(r_eff, 1));
   double* Qn = alloc_core(c, (size_t)r_eff * m);
@@ -1129,6 +1147,23 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int m = r * n;
   const int kk = std::min(m, rp);
+  if (absorb && m <= rp)
+  {
+    // As in right_qr_step: Q = I_m, R = the left unfolding, so core k+1
+    // takes core k whole and core k becomes the identity, rank m.
+    const int ncols_n = t->modes[k + 1] * t->ranks[k + 2];
+    double* Nn = alloc_core(c, (size_t)m * ncols_n);
+    gemm_rm(c, m, ncols_n, rp, t->cores[k], rp, t->cores[k + 1], ncols_n, Nn, ncols_n);
+    double* Cn = alloc_core(c, (size_t)m * m);
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 8.0 * (double)m * m);
+      launch_eye(Cn, m, m, c->stream);
+    }
+    replace_core(c, t, k + 1, Nn);
+    replace_core(c, t, k, Cn);
+    t->ranks[k + 1] = m;
+    return;
+  }
   double* A = ctx_ws(c, WS_A, (size_t)m * rp);
   launch_transpose(t->cores[k], rp, m, rp, A, m, c->stream);
   double* tau = ctx_ws(c, WS_F, (size_t)std::max(kk, 1));

@@ -97,18 +97,19 @@ def test_cholqr_declines_on_rank_deficient_and_graded():
 
 
 def test_cholqr_forms_across_condition_numbers():
-    # Bond spectra 10^-(s j) over 64 values, condition numbers 1e2 .. 1e7:
+    # Bond spectra 10^-(s j) over 64 values (d = 8, bonds 64, so that the
+    # QR sweep factors two tall 256 x 64 unfoldings and the SVD sweep three):
     # whichever form each factorization takes (the polar second pass for
     # small E, the triangular one for n max|E| between 1e-5 and 1e-4,
     # Householder beyond, the last covered by the graded test above), the
     # result reproduces the oracle.  Here the triangular form is taken for
-    # decay exponents in a window about 0.25 wide near 6.4 (measured), so the
-    # sweep steps by 0.125 through 6 .. 7 to reach both CholQR2 forms whatever
+    # decay exponents in a narrow window near 8.5 (measured), so the sweep
+    # steps by 1/32 through 8.25 .. 8.75 to reach both CholQR2 forms whatever
     # the last bits of the factorizations before them.
     ctx = T.Context.default()
     forms = {"triangular": 0, "declined": 0, "polar": 0}
-    for logk in (2.0, 4.0, 4.5, 5.0, 5.5, 6.0, 6.125, 6.25, 6.375, 6.5, 6.625, 6.75, 6.875, 7.0):
-        x = O.OTT.random([4] * 6, 64, 17)
+    for logk in [2.0, 4.0, 6.0] + list(8.25 + np.arange(17) / 32.0):
+        x = O.OTT.random([4] * 8, 64, 17)
         O.left_orthogonalize(x)
         cs = x.cores()
         for k in range(len(cs) - 1):
