@@ -563,6 +563,7 @@ def main():
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     spec_sweeps, spec_failed = ctx.speculation_counts()
+    chain_blocks, chain_steps, chain_one_pass = ctx.chain_stats()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
     # round kernel, the FP64 GEMM, the MPO core contraction, CholQR2's
     # Cholesky, the triangular inverse); the roofline is quoted for the one
@@ -576,7 +577,22 @@ def main():
     ctx.prof_enable(False)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     full_svd_ms = None
+    svd_every_ms = None
     if args.full_svd_steps > 0:
+        # The truncating path's cost at this shape: the certificate off, so
+        # every SVD step runs its Jacobi SVD (QR sweep, CholQR2, chained
+        # blocks and speculation as by default).
+        ctx.set_rank_certificate(False)
+        y_full = T.matvec_round(A, x, eps=EPS)
+        ctx.synchronize()
+        t2 = time.perf_counter()
+        for _ in range(args.full_svd_steps):
+            y_full = T.matvec_round(A, x, eps=EPS)
+        ctx.synchronize()
+        svd_every_ms = (time.perf_counter() - t2) * 1e3 / args.full_svd_steps
+        ctx.set_rank_certificate(True)
+        assert y_full.ranks == y.ranks
+        # The same with everything by Householder and Jacobi.
         ctx.set_rank_certificate(False)
         ctx.set_cholqr(False)
         ctx.set_speculation(False)
@@ -633,8 +649,12 @@ def main():
                                "undone_per_step": spec_failed / args.prof_steps},
         "cholqr2": {"tall_qr_per_step": cholqr_tried / args.prof_steps,
                     "cholqr2_used_per_step": cholqr_used / args.prof_steps},
+        "chained_blocks": {"per_step": chain_blocks / args.prof_steps,
+                           "steps_per_step": chain_steps / args.prof_steps,
+                           "one_pass_per_step": chain_one_pass / args.prof_steps},
         "rank_certificate": {"svd_steps_per_step": cert_tried / args.prof_steps,
                              "certified_per_step": cert_held / args.prof_steps,
+                             "ms_per_step_every_svd": svd_every_ms,
                              "ms_per_step_householder_and_svd": full_svd_ms},
     }
     if not args.no_batched:

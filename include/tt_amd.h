@@ -421,9 +421,13 @@ int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
  * sequential sweep's result; any failed decision undoes the sweep as above.
  * On by default. */
 int tta_set_chain(tta_ctx* ctx, int on);
-/* Chained blocks run and the steps they covered, since the last
- * tta_prof_enable. */
-int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps);
+/* In the QR sweep of round (whose cores only feed the SVD sweep) the
+ * second pass is left out when the first pass's cores are provably
+ * orthonormal to 1e-10 already (a bound on every step's Gram error from the
+ * cores' own Grams, read once per block).
+ * Chained blocks run, the steps they covered, and the blocks that needed
+ * only the first pass, since the last tta_prof_enable. */
+int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long* one_pass);
 
 /* ---- HIP graphs (no reference counterpart) ------------------------------ */
 /* Block-Jacobi SVD sweeps (more than 64 columns, single problems and

@@ -143,9 +143,14 @@ class Context:
 
     def chain_counts(self):
         """(blocks, steps) of chained sweep blocks since the last prof_enable."""
-        a, b = ctypes.c_longlong(), ctypes.c_longlong()
-        self.check(self.lib.tta_prof_chain(self.h, ctypes.byref(a), ctypes.byref(b)))
-        return a.value, b.value
+        return self.chain_stats()[:2]
+
+    def chain_stats(self):
+        """(blocks, steps, one_pass) of chained sweep blocks since the last
+        prof_enable; one_pass counts the blocks that needed no second pass."""
+        a, b, o = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
+        self.check(self.lib.tta_prof_chain(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(o)))
+        return a.value, b.value, o.value
 
     def speculation_counts(self):
         """(sweeps, failed) speculative sweep counts since the last prof_enable."""

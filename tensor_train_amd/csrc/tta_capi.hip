@@ -805,6 +805,11 @@ double frob_norm_dev(tta_ctx* c, const double* x, size_t n, double eps, double i
 // step by step when any decision fails.
 // ---------------------------------------------------------------------------
 constexpr int kChainMin = 2;
+// Orthonormality to which the cores of a sweep that only feeds the next one
+// (the QR sweep of round) are left after one pass: the next sweep's singular
+// values then carry a relative error of at most about 5e-11, far inside the
+// rank rule's and the certificate's margins.
+constexpr double kChainSkipTol = 1.0e-10;
 
 bool cert_ok(int max_rank, int q);
 
@@ -835,13 +840,24 @@ bool chain_step_ok(const tta_ctx* c, const tta_tt* t, int k, int dir, bool cert,
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
   const int m = dir == 0 ? n * rp : r * n, q = dir == 0 ? r : rp;
   if (!cholqr_applies(c, m, q)) return false;
+  // A QR step with no more rows than columns takes the identity basis
+  // (right_qr_step / left_qr_step); the SVD steps keep square ones, whose
+  // certificate needs the factorization.
+  if (!cert && m <= q) return false;
   if (cert && (!c->rank_certificate || !cert_ok(max_rank, q) || q > kRinvCertMaxQ)) return false;
   return true;
 }
 
 // The chained block of steps ka .. kb (see above).  With cert, each step's
 // full-rank certificate for delta is recorded for the sweep's end.
-void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, double delta)
+// skip_tol > 0 (for sweeps whose cores only serve the next sweep): when the
+// first pass is already orthonormal to skip_tol, the second is left out.
+// Its E_k obey E_k = F_k + sum_i N_i E_{k+1} N_i^T with F_k = N_k N_k^T - I
+// (the core's own Gram), and ||sum_i N_i E N_i^T|| <= ||E|| (1 + ||F_k||),
+// so ||E_k|| <= sum_j n_j max|F_j| (to first order) for every k; the host
+// reads that sum (one synchronisation) and skips the second pass when it is
+// at most skip_tol.
+void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, double delta, double skip_tol = 0.0)
 {
   const int L = kb - ka + 1;
   std::vector<int> ks(L), qs(L);
@@ -874,10 +890,11 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
   }
   double* T = ar.alloc<double>(max_core);
   std::vector<double*> C(L), N(L), U(L), O(L);
+  // N from the pool: with the second pass skipped they become the cores.
   for (int p = 0; p < L; ++p)
   {
     C[p] = t->cores[ks[p]];
-    N[p] = ar.alloc<double>(core_numel(t, ks[p]));
+    N[p] = alloc_core(c, core_numel(t, ks[p]));
     U[p] = ar.alloc<double>(core_numel(t, ks[p]));
   }
   // The chain of Grams over the cores X[p] into out(p).
@@ -965,6 +982,61 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
     gemm_batch_go(c, ar, g1);
     gemm_batch_go(c, ar, g2);
   }
+  bool skip2 = false;
+  if (skip_tol > 0.0)
+  {
+    // F_k (into E) and its partial maxima, then the bound on every E_k.
+    std::vector<gemm_desc> gf;
+    for (int p = 0; p < L; ++p)
+    {
+      const int k = ks[p], r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+      const long w = (long)n * rp;
+      if (dir == 0)
+        gf.push_back(gdesc(r, r, n * rp, N[p], w, 1, N[p], 1, w, pr[p].E, r, 1));  // N N^T (right unfoldings)
+      else
+        gf.push_back(gdesc(rp, rp, r * n, N[p], 1, rp, N[p], rp, 1, pr[p].E, rp, 1));  // N^T N (left unfoldings)
+    }
+    gemm_batch_go(c, ar, gf);
+    double* bound = small_dbl(c) + 10;
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+      launch_chain_minus_eye(dps, L, max_n, c->stream);
+      launch_chain_bound(dps, L, bound, c->stream);
+    }
+    skip2 = read_double(c, bound) <= skip_tol;
+  }
+  if (skip2)
+  {
+    // The first pass's cores are the block's cores; the last step's factor
+    // is its Cholesky factor.
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+      launch_chain_decide(dps, L, 1.0e-5, spec_flag(c), c->stream);
+    }
+    const double* Rs = pr[L - 1].R;
+    const int q = qs[L - 1];
+    if (dir == 0)
+    {
+      const int kp = ka - 1, rows_p = t->ranks[kp] * t->modes[kp];
+      double* Pn = alloc_core(c, (size_t)rows_p * q);
+      gemm_go(c, gdesc(rows_p, q, q, t->cores[kp], q, 1, Rs, 1, q, Pn, q, 1));  // core S^T
+      replace_core(c, t, kp, Pn);
+    }
+    else
+    {
+      const int kn = kb + 1, cols = t->modes[kn] * t->ranks[kn + 1];
+      double* Nn = alloc_core(c, (size_t)q * cols);
+      gemm_go(c, gdesc(q, cols, q, Rs, q, 1, t->cores[kn], cols, 1, Nn, cols, 1));  // S core
+      replace_core(c, t, kn, Nn);
+    }
+    for (int p = 0; p < L; ++p) replace_core(c, t, ks[p], N[p]);
+    c->cholqr_tried += L;
+    c->cholqr_used += L;
+    c->chain_blocks += 1;
+    c->chain_steps += L;
+    c->chain_one_pass += 1;
+    return;
+  }
   // Pass 2.
   chain(N, &chain_prob::E);
   {
@@ -1045,6 +1117,7 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
   for (int p = 0; p < L; ++p)
   {
     replace_core(c, t, ks[p], O[p]);
+    c->mem.put(N[p]);
     if (cert)
     {
       c->spec_certs.push_back({pr[p].norms, qs[p], 0, delta});
@@ -1662,7 +1735,7 @@ void right_orthogonalize_impl(tta_ctx* c, tta_tt* t)
     const int e = chain_step_ok(c, t, k, 0, false, 0) ? chain_run_end(c, t, k, 0, false, 0) : k;
     if (e != k)
     {
-      chain_block(c, t, e, k, 0, false, 0.0);
+      chain_block(c, t, e, k, 0, false, 0.0, kChainSkipTol);
       k = e - 1;
       continue;
     }
@@ -1680,7 +1753,7 @@ void left_orthogonalize_impl(tta_ctx* c, tta_tt* t)
     const int e = chain_step_ok(c, t, k, 1, false, 0) ? chain_run_end(c, t, k, 1, false, 0) : k;
     if (e != k)
     {
-      chain_block(c, t, k, e, 1, false, 0.0);
+      chain_block(c, t, k, e, 1, false, 0.0, kChainSkipTol);
       k = e + 1;
       continue;
     }
@@ -3462,6 +3535,7 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->spec_failed = 0;
   c->chain_blocks = 0;
   c->chain_steps = 0;
+  c->chain_one_pass = 0;
   TTA_GUARD_END(c)
 }
 
@@ -3502,11 +3576,12 @@ int tta_set_chain(tta_ctx* c, int on)
   return TTA_OK;
 }
 
-int tta_prof_chain(tta_ctx* c, long long* blocks, long long* steps)
+int tta_prof_chain(tta_ctx* c, long long* blocks, long long* steps, long long* one_pass)
 {
   if (!c) return TTA_ERR_ARG;
   if (blocks) *blocks = c->chain_blocks;
   if (steps) *steps = c->chain_steps;
+  if (one_pass) *one_pass = c->chain_one_pass;
   return TTA_OK;
 }
 

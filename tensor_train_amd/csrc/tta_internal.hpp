@@ -135,6 +135,7 @@ struct tta_ctx
   // run and the steps they covered.
   bool chain = true;
   long long chain_blocks = 0, chain_steps = 0;
+  long long chain_one_pass = 0;  // blocks whose second pass was not needed
   bool spec_qr_ok = true, spec_svd_ok = true;
   long long spec_sweeps = 0, spec_failed = 0;
   // Certificate decisions of the running speculative sweep, made together

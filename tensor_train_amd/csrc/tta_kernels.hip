@@ -466,6 +466,22 @@ __global__ __launch_bounds__(64) void chain_decide_kernel(const chain_prob* ps, 
 }
 static_assert(kChainParts == 64, "chain_decide_kernel reads one partial maximum per lane");
 
+// out[0] = sum over the problems of n max|E| (non-finite as infinity), in
+// problem order: a bound on the spectral norm of every chained E.
+__global__ __launch_bounds__(64) void chain_bound_kernel(const chain_prob* ps, int count, double* out)
+{
+  const int lane = threadIdx.x;
+  double total = 0.0;
+  for (int p = 0; p < count; ++p)
+  {
+    double emax = ps[p].part[lane];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    total += emax * ps[p].n;
+  }
+  if (lane == 0) out[0] = total <= DBL_MAX ? total : INFINITY;
+}
+
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.  TT: the tile size (64, or 32 for small outputs).
 template <int TT>
@@ -1394,6 +1410,12 @@ void launch_chain_norms(const chain_prob* ps, int count, int max_n, hipStream_t 
 {
   if (count <= 0 || max_n <= 0) return;
   hipLaunchKernelGGL(chain_norms_kernel, dim3((2 * max_n + 3) / 4, count), dim3(256), 0, st, ps);
+}
+
+void launch_chain_bound(const chain_prob* ps, int count, double* out, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(chain_bound_kernel, dim3(1), dim3(64), 0, st, ps, count, out);
 }
 
 void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st)

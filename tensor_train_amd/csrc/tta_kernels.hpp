@@ -326,6 +326,8 @@ void launch_chain_minus_eye(const chain_prob* ps, int count, int max_n, hipStrea
 void launch_chain_polar(const chain_prob* ps, int count, int max_n, hipStream_t st);
 // Column norms of Rf and Y into norms.
 void launch_chain_norms(const chain_prob* ps, int count, int max_n, hipStream_t st);
+// out[0] = sum over the problems of n max|E| from the partial maxima.
+void launch_chain_bound(const chain_prob* ps, int count, double* out, hipStream_t st);
 // flag[0] = 0 unless every problem's Cholesky held (info[0] == 0, finite
 // extreme diagonal entries) and n max|E| <= thr.
 void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st);

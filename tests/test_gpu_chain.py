@@ -28,7 +28,7 @@ def _run(ctx, fn, chain):
     ctx.prof_enable(True)
     try:
         out = fn()
-        return out, ctx.chain_counts(), ctx.speculation_counts()
+        return out, ctx.chain_stats(), ctx.speculation_counts()
     finally:
         ctx.prof_enable(False)
         ctx.set_chain(True)
@@ -54,11 +54,13 @@ def test_chain_on_mpo_product(gauge):
     ref = O.round(O.matvec(A, x), 1e-8, 0, 1 if gauge == "right_canonical" else 0)
     ctx = T.Context.default()
     y = T.matvec(to_dev(A), to_dev(x))
-    on, (blocks, steps), (sw, failed) = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), True)
-    off, (b0, _), _ = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), False)
+    on, (blocks, steps, one), (sw, failed) = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), True)
+    off, (b0, _, _), _ = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), False)
     # One block per sweep over the rank-256 interior (product ranks
-    # 1, 32, 128, 256, .., 256, 128, 32, 1; the ends take single steps).
-    assert (blocks, sw, failed, b0) == (2, 2, 0, 0)
+    # 1, 32, 128, 256, .., 256, 128, 32, 1; the ends take single steps);
+    # the QR sweep's block is orthonormal after its first pass (condition
+    # numbers about 10), the SVD sweep's always takes both.
+    assert (blocks, one, sw, failed, b0) == (2, 1, 2, 0, 0)
     assert steps >= 10
     assert on.ranks == ref.ranks == off.ranks
     assert rel_err(cores3(on), ref) <= 1e-13
@@ -72,8 +74,8 @@ def test_chain_matvec_round_full_size():
     ref = O.round(O.matvec(A, x), 1e-8)
     ctx = T.Context.default()
     Ad, xd = to_dev(A), to_dev(x)
-    y, (blocks, steps), (sw, failed) = _run(ctx, lambda: T.matvec_round(Ad, xd, eps=1e-8), True)
-    assert (blocks, sw, failed) == (2, 2, 0)
+    y, (blocks, steps, one), (sw, failed) = _run(ctx, lambda: T.matvec_round(Ad, xd, eps=1e-8), True)
+    assert (blocks, one, sw, failed) == (2, 1, 2, 0)
     assert steps >= 25
     assert y.ranks == ref.ranks
     assert rel_err(cores3(y), ref) <= 1e-12
@@ -95,7 +97,7 @@ def test_chain_unequal_ranks(gauge):
     ref = O.round(a, 1e-12, 0, 1 if gauge == "right_canonical" else 0)
     ctx = T.Context.default()
     ad = to_dev(a)
-    on, (blocks, _), (sw, failed) = _run(ctx, lambda: T.round(ad, eps=1e-12, gauge=gauge), True)
+    on, (blocks, _, _), (sw, failed) = _run(ctx, lambda: T.round(ad, eps=1e-12, gauge=gauge), True)
     off, _, _ = _run(ctx, lambda: T.round(ad, eps=1e-12, gauge=gauge), False)
     assert blocks >= 2 and failed == 0
     assert on.ranks == ref.ranks == off.ranks
@@ -115,7 +117,7 @@ def test_chain_falls_back_where_the_rule_cuts():
     assert max(ref.ranks) < 64
     ctx = T.Context.default()
     ctx.set_speculation(True)
-    r, (blocks, _), (sw, failed) = _run(ctx, lambda: T.round(t, eps=1e-2), True)
+    r, (blocks, _, _), (sw, failed) = _run(ctx, lambda: T.round(t, eps=1e-2), True)
     ctx.set_speculation(True)
     assert blocks >= 2 and failed == 1
     assert r.ranks == ref.ranks
@@ -131,7 +133,7 @@ def test_chain_falls_back_on_singular_grams():
     ref = O.round(s, 1e-12)
     ctx = T.Context.default()
     ctx.set_speculation(True)
-    on, (blocks, _), (sw, failed) = _run(ctx, lambda: T.round(to_dev(s), eps=1e-12), True)
+    on, (blocks, _, _), (sw, failed) = _run(ctx, lambda: T.round(to_dev(s), eps=1e-12), True)
     ctx.set_speculation(True)
     assert blocks >= 1 and failed >= 1
     assert on.ranks == ref.ranks
