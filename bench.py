@@ -563,7 +563,7 @@ def main():
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     spec_sweeps, spec_failed = ctx.speculation_counts()
-    chain_blocks, chain_steps, chain_one_pass = ctx.chain_stats()
+    chain_blocks, chain_steps, chain_one_pass, chain_rejected = ctx.chain_stats()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
     # round kernel, the FP64 GEMM, the MPO core contraction, CholQR2's
     # Cholesky, the triangular inverse); the roofline is quoted for the one
@@ -651,7 +651,8 @@ def main():
                     "cholqr2_used_per_step": cholqr_used / args.prof_steps},
         "chained_blocks": {"per_step": chain_blocks / args.prof_steps,
                            "steps_per_step": chain_steps / args.prof_steps,
-                           "one_pass_per_step": chain_one_pass / args.prof_steps},
+                           "one_pass_per_step": chain_one_pass / args.prof_steps,
+                           "rejected_per_step": chain_rejected / args.prof_steps},
         "rank_certificate": {"svd_steps_per_step": cert_tried / args.prof_steps,
                              "certified_per_step": cert_held / args.prof_steps,
                              "ms_per_step_every_svd": svd_every_ms,

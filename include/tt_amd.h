@@ -399,35 +399,40 @@ int tta_prof_cholqr(tta_ctx* ctx, long long* tried, long long* used);
 int tta_prof_cholqr_triangular(tta_ctx* ctx, long long* triangular);
 
 /* ---- speculative sweeps (no reference counterpart) ------------------------- */
-/* The QR and SVD sweeps of round / round_right_canonical (and of the naive
+/* The SVD sweeps of round / round_right_canonical (and of the naive
  * matvec_round / matmat_round built on them) decide CholQR2 acceptance and
  * the rank certificates on the device, into one flag read once per sweep
  * instead of once per factorization.  A sweep with any failed decision is
  * undone from a copy of the TT taken before it and run again with host
  * decisions, so results equal those of on = 0; after such a fallback the
- * next sweep of that kind runs checked, and speculation resumes once a
- * checked sweep needed no fallback.  On by default. */
+ * next SVD sweep runs checked, and speculation resumes once a checked
+ * sweep needed no fallback.  (The QR sweeps decide on the host: a declined
+ * CholQR2 falls back to Householder for its own step.)  On by default. */
 int tta_set_speculation(tta_ctx* ctx, int on);
 /* Speculative sweeps run and those undone, since the last tta_prof_enable. */
 int tta_prof_speculation(tta_ctx* ctx, long long* sweeps, long long* failed);
 
 /* ---- chained sweep blocks (no reference counterpart) ----------------------- */
-/* Inside a speculative sweep, a run of two or more consecutive QR steps of
- * CholQR2's shape (and, in the SVD sweeps, steps whose full-rank certificate
- * max_rank cannot overrule) is factored as one block: the steps' Gram
+/* In round's sweeps (for the SVD sweeps, speculative ones), a run of two or
+ * more consecutive QR steps of CholQR2's shape (and, in the SVD sweeps,
+ * steps whose full-rank certificate max_rank cannot overrule) is factored
+ * as one block: the steps' Gram
  * matrices follow one another by two products each (G_k = sum_i C_i G_{k+1}
  * C_i^T), and the Choleskys, the orthonormalizing products and a second
  * pass run for all steps of the block at once.  Exact arithmetic gives the
- * sequential sweep's result; any failed decision undoes the sweep as above.
- * On by default. */
+ * sequential sweep's result.  A block is accepted on the host (every
+ * Cholesky held, the second pass within the polar form's range); a
+ * rejected block leaves the TT untouched and its steps run one by one.
+ * The certificates are decided with the sweep's, as above.  On by default. */
 int tta_set_chain(tta_ctx* ctx, int on);
 /* In the QR sweep of round (whose cores only feed the SVD sweep) the
  * second pass is left out when the first pass's cores are provably
  * orthonormal to 1e-10 already (a bound on every step's Gram error from the
- * cores' own Grams, read once per block).
- * Chained blocks run, the steps they covered, and the blocks that needed
- * only the first pass, since the last tta_prof_enable. */
-int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long* one_pass);
+ * cores' own Grams, read with the block's acceptance).
+ * Chained blocks accepted, the steps they covered, the accepted blocks that
+ * needed only the first pass, and the blocks rejected, since the last
+ * tta_prof_enable. */
+int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long* one_pass, long long* rejected);
 
 /* ---- HIP graphs (no reference counterpart) ------------------------------ */
 /* Block-Jacobi SVD sweeps (more than 64 columns, single problems and

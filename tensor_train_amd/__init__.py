@@ -146,11 +146,13 @@ class Context:
         return self.chain_stats()[:2]
 
     def chain_stats(self):
-        """(blocks, steps, one_pass) of chained sweep blocks since the last
-        prof_enable; one_pass counts the blocks that needed no second pass."""
-        a, b, o = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_longlong()
-        self.check(self.lib.tta_prof_chain(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(o)))
-        return a.value, b.value, o.value
+        """(blocks, steps, one_pass, rejected) of chained sweep blocks since
+        the last prof_enable: blocks accepted, the steps they covered, those
+        that needed no second pass, and blocks rejected (their steps then
+        ran one by one)."""
+        v = [ctypes.c_longlong() for _ in range(4)]
+        self.check(self.lib.tta_prof_chain(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
 
     def speculation_counts(self):
         """(sweeps, failed) speculative sweep counts since the last prof_enable."""

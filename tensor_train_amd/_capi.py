@@ -101,7 +101,7 @@ SIGNATURES = {
     "tta_set_speculation": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_set_chain": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_chain": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
-                                      ctypes.POINTER(ctypes.c_longlong)]),
+                                      ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_cholqr_triangular": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_speculation": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
     "tta_prof_cholqr": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),

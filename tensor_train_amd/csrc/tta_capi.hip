@@ -797,14 +797,26 @@ double frob_norm_dev(tta_ctx* c, const double* x, size_t n, double eps, double i
 // is the sequential sweep's.  Rounding: the chain's G_k and the factor
 // S_{k+1} it is used with are consistent to the Cholesky's backward error,
 // as in the sequential CholQR's Gram of the absorbed core, so E has the
-// sequential first pass's size.  The acceptance (every Cholesky held, every
-// n max|E| <= 1e-5, the polar form's range) and, for the SVD sweep, every
-// step's full-rank certificate (column norms of Rf = Pi S and Rf^-1 =
-// S^-1 P) are decided on the device into the speculative sweep's flag: a
-// block runs only inside a speculative sweep, which is undone and run again
-// step by step when any decision fails.
+// sequential first pass's size in the measure u ||C_k||^2 ||G_{k+1}|| /
+// lambda_min(G_k), which can exceed the sequential u kappa(A_k)^2 where
+// graded spectra of neighbouring cores meet.  So the block is accepted on
+// the host (one synchronisation per block): every Cholesky held and every
+// n max|E| <= 1e-5 (the polar form's range); otherwise nothing has been
+// written and its steps run one by one.  For the SVD sweep, every step's
+// full-rank certificate (column norms of Rf = Pi S and Rf^-1 = S^-1 P) is
+// recorded for the speculative sweep's end, like a single step's; blocks
+// run only inside speculative sweeps.
 // ---------------------------------------------------------------------------
 constexpr int kChainMin = 2;
+
+// Host copy of the device sum item (launch_chain_sum).
+struct chain_sum_item_h
+{
+  const double* part;
+  double* out;
+  int s;
+  long n;
+};
 // Orthonormality to which the cores of a sweep that only feeds the next one
 // (the QR sweep of round) are left after one pass: the next sweep's singular
 // values then carry a relative error of at most about 5e-11, far inside the
@@ -834,7 +846,9 @@ void gemm_batch_go(tta_ctx* c, arena& ar, const std::vector<gemm_desc>& gs)
 // cert) a certificate that max_rank cannot overrule.
 bool chain_step_ok(const tta_ctx* c, const tta_tt* t, int k, int dir, bool cert, int max_rank)
 {
-  if (!c->chain || !c->spec) return false;
+  // SVD-sweep blocks record their certificates for a speculative sweep's
+  // end; QR-sweep blocks are decided on the host and need no speculation.
+  if (!c->chain || (cert && !c->spec)) return false;
   const int d = t->d();
   if (dir == 0 ? k < 1 : k > d - 2) return false;
   const int r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
@@ -857,7 +871,7 @@ bool chain_step_ok(const tta_ctx* c, const tta_tt* t, int k, int dir, bool cert,
 // so ||E_k|| <= sum_j n_j max|F_j| (to first order) for every k; the host
 // reads that sum (one synchronisation) and skips the second pass when it is
 // at most skip_tol.
-void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, double delta, double skip_tol = 0.0)
+bool chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, double delta, double skip_tol = 0.0)
 {
   const int L = kb - ka + 1;
   std::vector<int> ks(L), qs(L);
@@ -897,32 +911,47 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
     N[p] = alloc_core(c, core_numel(t, ks[p]));
     U[p] = ar.alloc<double>(core_numel(t, ks[p]));
   }
-  // The chain of Grams over the cores X[p] into out(p).
+  // The chain of Grams over the cores X[p] into out(p).  Each step is two
+  // launches on 32 x 32 tiles: the product with the previous Gram, read as
+  // the sum of its split-K partials, and the new Gram's partials; the Grams
+  // themselves are summed for all steps at once afterwards.
   auto chain = [&](const std::vector<double*>& X, double* chain_prob::*out)
   {
+    std::vector<double*> gp(L);
+    std::vector<int> gs(L);
+    std::vector<chain_sum_item_h> items(L);
+    long max_qq = 0;
     for (int p = 0; p < L; ++p)
     {
       const int k = ks[p], r = t->ranks[k], n = t->modes[k], rp = t->ranks[k + 1];
+      const long w = (long)n * rp;
+      const int q = qs[p];
       const double* src = X[p];
-      if (dir == 0)
+      if (p > 0)
       {
-        if (p > 0)
-        {
-          gemm_raw(c, r * n, rp, rp, X[p], rp, 1, pr[p - 1].*out, rp, 1, T, rp, 1);  // T = X G_{k+1}
-          src = T;
-        }
-        gemm_raw(c, r, r, n * rp, src, (long)n * rp, 1, X[p], 1, (long)n * rp, pr[p].*out, r, 1);  // T X^T
+        const int qp = qs[p - 1];
+        gemm_desc g = dir == 0 ? gdesc(r * n, rp, rp, X[p], rp, 1, gp[p - 1], rp, 1, T, rp, 1)   // X G_{k+1}
+                               : gdesc(r, n * rp, r, gp[p - 1], r, 1, X[p], w, 1, T, w, 1);     // H_{k-1} X
+        prof_scope ps(c, TTA_K_GEMM_SPLITK, 2.0 * g.M * g.N * (double)g.K,
+                      8.0 * ((double)g.M * g.K + (double)gs[p - 1] * qp * qp + (double)g.M * g.N));
+        if (dir == 0)
+          launch_gemm_chain(g, 1, 0, gs[p - 1], (long)qp * qp, 1, nullptr, c->stream);
+        else
+          launch_gemm_chain(g, gs[p - 1], (long)qp * qp, 1, 0, 1, nullptr, c->stream);
+        src = T;
       }
-      else
-      {
-        if (p > 0)
-        {
-          gemm_raw(c, r, n * rp, r, pr[p - 1].*out, r, 1, X[p], (long)n * rp, 1, T, (long)n * rp, 1);  // H_{k-1} X
-          src = T;
-        }
-        gemm_raw(c, rp, rp, r * n, src, 1, rp, X[p], rp, 1, pr[p].*out, rp, 1);  // T^T X (left unfoldings)
-      }
+      gemm_desc g = dir == 0 ? gdesc(q, q, n * rp, src, w, 1, X[p], 1, w, nullptr, q, 1)      // (.) X^T
+                             : gdesc(q, q, r * n, src, 1, rp, X[p], rp, 1, nullptr, q, 1);  // (.)^T X, left unfoldings
+      const int sl = chain_gemm_slices(g.M, g.N, g.K);
+      gp[p] = ar.alloc<double>((size_t)sl * q * q);
+      prof_scope ps(c, TTA_K_GEMM_SPLITK, 2.0 * g.M * g.N * (double)g.K,
+                    8.0 * (2.0 * (double)g.M * g.K + (double)sl * q * q));
+      gs[p] = launch_gemm_chain(g, 1, 0, 1, 0, sl, gp[p], c->stream);
+      items[p] = {gp[p], pr[p].*out, gs[p], (long)q * q};
+      max_qq = std::max(max_qq, (long)q * q);
     }
+    prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+    launch_chain_sum(ar.upload(items), L, max_qq, c->stream);
   };
   chain(C, &chain_prob::G);
   chain_prob* dps = ar.upload(pr);
@@ -982,10 +1011,34 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
     gemm_batch_go(c, ar, g1);
     gemm_batch_go(c, ar, g2);
   }
+  // The block's status on the host (one synchronisation): E <- E - I with
+  // its partial maxima, then the sum and the largest n max|E| over the
+  // steps, infinity on a Cholesky breakdown.
+  auto status = [&](double& sum, double& most)
+  {
+    double* st = small_dbl(c) + 10;
+    {
+      prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
+      launch_chain_minus_eye(dps, L, max_n, c->stream);
+      launch_chain_bound(dps, L, st, c->stream);
+    }
+    hip_check(hipMemcpyAsync(c->h_dbl, st, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream), "hipMemcpyAsync");
+    hip_check(hipStreamSynchronize(c->stream), "hipStreamSynchronize");
+    sum = c->h_dbl[0];
+    most = c->h_dbl[1];
+  };
+  // A rejected block leaves the TT as it was; its steps then run one by
+  // one.
+  auto reject = [&]()
+  {
+    for (int p = 0; p < L; ++p) c->mem.put(N[p]);
+    c->chain_rejected += 1;
+    return false;
+  };
   bool skip2 = false;
   if (skip_tol > 0.0)
   {
-    // F_k (into E) and its partial maxima, then the bound on every E_k.
+    // F_k (into E), then the bound on every E_k.
     std::vector<gemm_desc> gf;
     for (int p = 0; p < L; ++p)
     {
@@ -997,22 +1050,15 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
         gf.push_back(gdesc(rp, rp, r * n, N[p], 1, rp, N[p], rp, 1, pr[p].E, rp, 1));  // N^T N (left unfoldings)
     }
     gemm_batch_go(c, ar, gf);
-    double* bound = small_dbl(c) + 10;
-    {
-      prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
-      launch_chain_minus_eye(dps, L, max_n, c->stream);
-      launch_chain_bound(dps, L, bound, c->stream);
-    }
-    skip2 = read_double(c, bound) <= skip_tol;
+    double sum = 0.0, most = 0.0;
+    status(sum, most);
+    if (!(sum < INFINITY)) return reject();
+    skip2 = sum <= skip_tol;
   }
   if (skip2)
   {
     // The first pass's cores are the block's cores; the last step's factor
     // is its Cholesky factor.
-    {
-      prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
-      launch_chain_decide(dps, L, 1.0e-5, spec_flag(c), c->stream);
-    }
     const double* Rs = pr[L - 1].R;
     const int q = qs[L - 1];
     if (dir == 0)
@@ -1035,13 +1081,15 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
     c->chain_blocks += 1;
     c->chain_steps += L;
     c->chain_one_pass += 1;
-    return;
+    return true;
   }
-  // Pass 2.
+  // Pass 2, accepted when every step's n max|E| is within the polar form's
+  // range (as a single CholQR2's).
   chain(N, &chain_prob::E);
   {
-    prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
-    launch_chain_minus_eye(dps, L, max_n, c->stream);
+    double sum = 0.0, most = 0.0;
+    status(sum, most);
+    if (!(most <= 1.0e-5)) return reject();
   }
   {
     std::vector<gemm_desc> gv;
@@ -1092,10 +1140,10 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
     gemm_batch_go(c, ar, g2);
     gemm_batch_go(c, ar, gr);
   }
+  if (cert)
   {
     prof_scope ps(c, TTA_K_OTHER, 0.0, 0.0);
-    launch_chain_decide(dps, L, 1.0e-5, spec_flag(c), c->stream);
-    if (cert) launch_chain_norms(dps, L, max_n, c->stream);
+    launch_chain_norms(dps, L, max_n, c->stream);
   }
   // The last step's factor into the core past the block.
   const double* Rf = pr[L - 1].Rf;
@@ -1129,6 +1177,7 @@ void chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
   c->cholqr_used += L;
   c->chain_blocks += 1;
   c->chain_steps += L;
+  return true;
 }
 
 // The end of the run of chainable steps that starts at step k of a sweep in
@@ -1724,41 +1773,40 @@ void svd_step_right(tta_ctx* c, tta_tt* t, int k, double delta, int max_rank)
   absorb_right_step(c, t, k, rows, r_new, Cn, F);
 }
 
-// Runs of chainable steps (chain_block) inside speculative sweeps, single
-// steps otherwise.
-void right_orthogonalize_impl(tta_ctx* c, tta_tt* t)
+// The QR sweeps.  With in_round (the first sweep of round, whose cores
+// only feed its second sweep) runs of chainable steps are factored as
+// chained blocks whose second pass may be left out (kChainSkipTol); every
+// decision is made on the host, so a declined CholQR2 falls back to
+// Householder for its step alone.  Otherwise single steps throughout.
+void right_orthogonalize_impl(tta_ctx* c, tta_tt* t, bool in_round = false)
 {
   const int d = t->d();
   if (d <= 1) return;
   for (int k = d - 1; k >= 1;)
   {
-    const int e = chain_step_ok(c, t, k, 0, false, 0) ? chain_run_end(c, t, k, 0, false, 0) : k;
-    if (e != k)
+    const int e = in_round && chain_step_ok(c, t, k, 0, false, 0) ? chain_run_end(c, t, k, 0, false, 0) : k;
+    if (e != k && chain_block(c, t, e, k, 0, false, 0.0, kChainSkipTol))
     {
-      chain_block(c, t, e, k, 0, false, 0.0, kChainSkipTol);
       k = e - 1;
       continue;
     }
-    right_qr_step(c, t, k);
-    --k;
+    for (; k >= e; --k) right_qr_step(c, t, k);
   }
 }
 
-void left_orthogonalize_impl(tta_ctx* c, tta_tt* t)
+void left_orthogonalize_impl(tta_ctx* c, tta_tt* t, bool in_round = false)
 {
   const int d = t->d();
   if (d <= 1) return;
   for (int k = 0; k < d - 1;)
   {
-    const int e = chain_step_ok(c, t, k, 1, false, 0) ? chain_run_end(c, t, k, 1, false, 0) : k;
-    if (e != k)
+    const int e = in_round && chain_step_ok(c, t, k, 1, false, 0) ? chain_run_end(c, t, k, 1, false, 0) : k;
+    if (e != k && chain_block(c, t, k, e, 1, false, 0.0, kChainSkipTol))
     {
-      chain_block(c, t, k, e, 1, false, 0.0, kChainSkipTol);
       k = e + 1;
       continue;
     }
-    left_qr_step(c, t, k);
-    ++k;
+    for (; k <= e; ++k) left_qr_step(c, t, k);
   }
 }
 
@@ -1923,7 +1971,7 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
   const double inv = 1.0 / std::sqrt((double)std::max(d - 1, 1));
   if (gauge == TTA_GAUGE_RIGHT_CANONICAL)
   {
-    run_sweep(c, t, c->spec_qr_ok, [&] { left_orthogonalize_impl(c, t); });
+    left_orthogonalize_impl(c, t, true);
     const double nrm = frob_norm_dev(c, t->cores[d - 1], core_numel(t, d - 1), eps, inv);
     if (nrm == 0.0)
     {
@@ -1938,19 +1986,17 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
                 {
                   const int e =
                       chain_step_ok(c, t, k, 0, true, max_rank) ? chain_run_end(c, t, k, 0, true, max_rank) : k;
-                  if (e != k)
+                  if (e != k && chain_block(c, t, e, k, 0, true, delta))
                   {
-                    chain_block(c, t, e, k, 0, true, delta);
                     k = e - 1;
                     continue;
                   }
-                  svd_step_right(c, t, k, delta, max_rank);
-                  --k;
+                  for (; k >= e; --k) svd_step_right(c, t, k, delta, max_rank);
                 }
               });
     return;
   }
-  run_sweep(c, t, c->spec_qr_ok, [&] { right_orthogonalize_impl(c, t); });
+  right_orthogonalize_impl(c, t, true);
   const double nrm = frob_norm_dev(c, t->cores[0], core_numel(t, 0), eps, inv);
   if (nrm == 0.0)
   {
@@ -1964,14 +2010,12 @@ void round_inplace(tta_ctx* c, tta_tt* t, double eps, int max_rank, int gauge)
               for (int k = 0; k < d - 1;)
               {
                 const int e = chain_step_ok(c, t, k, 1, true, max_rank) ? chain_run_end(c, t, k, 1, true, max_rank) : k;
-                if (e != k)
+                if (e != k && chain_block(c, t, k, e, 1, true, delta))
                 {
-                  chain_block(c, t, k, e, 1, true, delta);
                   k = e + 1;
                   continue;
                 }
-                svd_step(c, t, k, delta, max_rank);
-                ++k;
+                for (; k <= e; ++k) svd_step(c, t, k, delta, max_rank);
               }
             });
   balance(c, t);
@@ -3536,6 +3580,7 @@ int tta_prof_enable(tta_ctx* c, int on)
   c->chain_blocks = 0;
   c->chain_steps = 0;
   c->chain_one_pass = 0;
+  c->chain_rejected = 0;
   TTA_GUARD_END(c)
 }
 
@@ -3557,7 +3602,7 @@ int tta_set_speculation(tta_ctx* c, int on)
 {
   if (!c) return TTA_ERR_ARG;
   c->speculate = on != 0;
-  c->spec_qr_ok = c->spec_svd_ok = true;
+  c->spec_svd_ok = true;
   return TTA_OK;
 }
 
@@ -3576,12 +3621,13 @@ int tta_set_chain(tta_ctx* c, int on)
   return TTA_OK;
 }
 
-int tta_prof_chain(tta_ctx* c, long long* blocks, long long* steps, long long* one_pass)
+int tta_prof_chain(tta_ctx* c, long long* blocks, long long* steps, long long* one_pass, long long* rejected)
 {
   if (!c) return TTA_ERR_ARG;
   if (blocks) *blocks = c->chain_blocks;
   if (steps) *steps = c->chain_steps;
   if (one_pass) *one_pass = c->chain_one_pass;
+  if (rejected) *rejected = c->chain_rejected;
   return TTA_OK;
 }
 

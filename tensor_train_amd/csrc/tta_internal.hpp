@@ -124,19 +124,21 @@ struct tta_ctx
   bool cholqr = true;
   long long cholqr_tried = 0, cholqr_used = 0;
   long long cholqr_triangular = 0;  // accepted in the triangular form
-  // Speculative sweeps of round (on by default): CholQR2 acceptance and the
-  // rank certificates are decided on the device into one flag that the host
-  // reads after the sweep; a failed sweep is undone and run again with host
-  // decisions.  spec marks a running speculative sweep; *_ok record whether
-  // the last sweep of each kind needed no fallback (speculation resumes
-  // after a checked sweep that needed none).
+  // Speculative SVD sweeps of round (on by default): CholQR2 acceptance and
+  // the rank certificates are decided on the device into one flag that the
+  // host reads after the sweep; a failed sweep is undone and run again with
+  // host decisions.  spec marks a running speculative sweep; spec_svd_ok
+  // records whether the last SVD sweep needed no fallback (speculation
+  // resumes after a checked sweep that needed none).
   bool speculate = true, spec = false;
-  // Chained sweep blocks inside speculative sweeps (on by default): blocks
-  // run and the steps they covered.
+  // Chained sweep blocks in round's sweeps (on by default): blocks
+  // accepted, the steps they covered, those accepted after one pass, and
+  // those rejected.
   bool chain = true;
   long long chain_blocks = 0, chain_steps = 0;
   long long chain_one_pass = 0;  // blocks whose second pass was not needed
-  bool spec_qr_ok = true, spec_svd_ok = true;
+  long long chain_rejected = 0;  // blocks whose steps then ran one by one
+  bool spec_svd_ok = true;
   long long spec_sweeps = 0, spec_failed = 0;
   // Certificate decisions of the running speculative sweep, made together
   // before its flag is read: their inputs (n values in mode 1, 2n in mode

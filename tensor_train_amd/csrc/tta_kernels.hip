@@ -37,21 +37,24 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 // loads coalesce.
 // (Selects rather than branches: with stores to the two outputs in
 // branches the compiler kept them in scratch memory.)
-template <int TT = GT>
+// KD: the slab depth (32, or 64 for the Gram chain's products).
+template <int TT = GT, int KD = GK>
 __device__ __forceinline__ void slab_coords_a(bool kc, int idx, int& ii, int& kk)
 {
-  constexpr int SH = TT == 64 ? 6 : 5;
+  constexpr int SH = TT == 64 ? 6 : 5, KS = KD == 64 ? 6 : 5;
   static_assert(TT == 64 || TT == 32, "tile width");
-  ii = kc ? (idx >> 5) : (idx & (TT - 1));
-  kk = kc ? (idx & 31) : (idx >> SH);
+  static_assert(KD == 64 || KD == 32, "slab depth");
+  ii = kc ? (idx >> KS) : (idx & (TT - 1));
+  kk = kc ? (idx & (KD - 1)) : (idx >> SH);
 }
-template <int TT = GT>
+template <int TT = GT, int KD = GK>
 __device__ __forceinline__ void slab_coords_b(bool jc, int idx, int& kk, int& jj)
 {
-  constexpr int SH = TT == 64 ? 6 : 5;
+  constexpr int SH = TT == 64 ? 6 : 5, KS = KD == 64 ? 6 : 5;
   static_assert(TT == 64 || TT == 32, "tile width");
-  kk = jc ? (idx >> SH) : (idx & 31);
-  jj = jc ? (idx & (TT - 1)) : (idx >> 5);
+  static_assert(KD == 64 || KD == 32, "slab depth");
+  kk = jc ? (idx >> SH) : (idx & (KD - 1));
+  jj = jc ? (idx & (TT - 1)) : (idx >> KS);
 }
 
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -83,15 +86,18 @@ struct mfma_step<float>
 // over K whatever TT is, so both tile sizes give identical results.  Every
 // use of LDS is followed by a barrier, so a workgroup may run several in a
 // row.
-template <class TA, class TB, class TC, class CT, int TT = GT>
+// NSA / NSB > 1: the operand is the sum of na <= NSA (nb <= NSB) equally
+// laid out buffers psa (psb) elements apart, all loaded at once and summed
+// in order (split-K partial products read without a separate sum).
+template <class TA, class TB, class TC, class CT, int TT = GT, int NSA = 1, int NSB = 1, int KD = GK>
 __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, long sar, long sac, const TB* B, long sbr,
                                             long sbc, TC* C, long scr, long scc, double alpha, double beta, int i0,
-                                            int j0)
+                                            int j0, int na = 1, long psa = 0, int nb = 1, long psb = 0)
 {
   using S = mfma_step<CT>;
-  constexpr int WS = TT / 2, NB = WS / 16, NL = GK * TT / 256;
-  __shared__ CT As[GK][TT + 1];
-  __shared__ CT Bs[GK][TT + 1];
+  constexpr int WS = TT / 2, NB = WS / 16, NL = KD * TT / 256;
+  __shared__ CT As[KD][TT + 1];
+  __shared__ CT Bs[KD][TT + 1];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wid = tid >> 6;
   const int wr = wid >> 1, wc = wid & 1;
@@ -111,31 +117,57 @@ __device__ __forceinline__ void gemm_tile_t(int M, int N, int K, const TA* A, lo
     {
       const int idx = tid + 256 * l;
       int ii, kk, jj, kb;
-      slab_coords_a<TT>(a_kc, idx, ii, kk);
+      slab_coords_a<TT, KD>(a_kc, idx, ii, kk);
       const int gi = i0 + ii, gk = k0 + kk;
-      pa[l] = (gi < M && gk < K) ? (CT)A[(size_t)gi * sar + (size_t)gk * sac] : (CT)0;
-      slab_coords_b<TT>(b_jc, idx, kb, jj);
+      if constexpr (NSA > 1)
+      {
+        const bool in = gi < M && gk < K;
+        const size_t o = (size_t)gi * sar + (size_t)gk * sac;
+        CT vs[NSA];
+#pragma unroll
+        for (int t = 0; t < NSA; ++t) vs[t] = (in && t < na) ? (CT)A[o + (size_t)t * psa] : (CT)0;
+        CT v = vs[0];
+#pragma unroll
+        for (int t = 1; t < NSA; ++t) v += vs[t];
+        pa[l] = v;
+      }
+      else
+        pa[l] = (gi < M && gk < K) ? (CT)A[(size_t)gi * sar + (size_t)gk * sac] : (CT)0;
+      slab_coords_b<TT, KD>(b_jc, idx, kb, jj);
       const int gj = j0 + jj, gk2 = k0 + kb;
-      pb[l] = (gj < N && gk2 < K) ? (CT)B[(size_t)gk2 * sbr + (size_t)gj * sbc] : (CT)0;
+      if constexpr (NSB > 1)
+      {
+        const bool in = gj < N && gk2 < K;
+        const size_t o = (size_t)gk2 * sbr + (size_t)gj * sbc;
+        CT vs[NSB];
+#pragma unroll
+        for (int t = 0; t < NSB; ++t) vs[t] = (in && t < nb) ? (CT)B[o + (size_t)t * psb] : (CT)0;
+        CT v = vs[0];
+#pragma unroll
+        for (int t = 1; t < NSB; ++t) v += vs[t];
+        pb[l] = v;
+      }
+      else
+        pb[l] = (gj < N && gk2 < K) ? (CT)B[(size_t)gk2 * sbr + (size_t)gj * sbc] : (CT)0;
     }
   };
   fetch(0);
-  for (int k0 = 0; k0 < K; k0 += GK)
+  for (int k0 = 0; k0 < K; k0 += KD)
   {
 #pragma unroll
     for (int l = 0; l < NL; ++l)
     {
       const int idx = tid + 256 * l;
       int ii, kk, jj, kb;
-      slab_coords_a<TT>(a_kc, idx, ii, kk);
+      slab_coords_a<TT, KD>(a_kc, idx, ii, kk);
       As[kk][ii] = pa[l];
-      slab_coords_b<TT>(b_jc, idx, kb, jj);
+      slab_coords_b<TT, KD>(b_jc, idx, kb, jj);
       Bs[kb][jj] = pb[l];
     }
     __syncthreads();
-    if (k0 + GK < K) fetch(k0 + GK);
+    if (k0 + KD < K) fetch(k0 + KD);
 #pragma unroll
-    for (int ks = 0; ks < GK; ks += 4)
+    for (int ks = 0; ks < KD; ks += 4)
     {
       CT a[NB], b[NB];
 #pragma unroll
@@ -355,6 +387,50 @@ __device__ __forceinline__ void chol_step_run(const chol_step& s, int t)
 
 __global__ __launch_bounds__(256) void chol_step_kernel(chol_step s) { chol_step_run(s, blockIdx.x); }
 
+// One product of a Gram chain on 32 x 32 tiles: an operand given as the
+// sum of split-K partials (na / nb buffers) is summed while loaded; with
+// part set, slice blockIdx.z of the K range [z kper, (z+1) kper) writes its
+// partial product P_z (row-major M x N, M N apart) instead of C.
+template <int SA, int SB>
+__global__ __launch_bounds__(256) void gemm_chain_kernel(gemm_desc g, int na, long psa, int nb, long psb, int kper,
+                                                         double* part)
+{
+  constexpr int TT = 32;
+  if (!part)
+  {
+    gemm_tile_t<double, double, double, double, TT, SA, SB, 64>(g.M, g.N, g.K, g.A, g.sar, g.sac, g.B, g.sbr, g.sbc, g.C,
+                                                            g.scr, g.scc, g.alpha, g.beta, blockIdx.y * TT,
+                                                            blockIdx.x * TT, na, psa, nb, psb);
+    return;
+  }
+  const int k0 = blockIdx.z * kper;
+  const int kn = min(g.K - k0, kper);
+  double* P = part + (size_t)blockIdx.z * g.M * g.N;
+  gemm_tile_t<double, double, double, double, TT, SA, SB, 64>(g.M, g.N, kn, g.A + (size_t)k0 * g.sac, g.sar, g.sac,
+                                                          g.B + (size_t)k0 * g.sbr, g.sbr, g.sbc, P, g.N, 1, 1.0, 0.0,
+                                                          blockIdx.y * TT, blockIdx.x * TT, na, psa, nb, psb);
+}
+
+// out = the sum of the s partials P_0 .. P_{s-1} (n elements each), in
+// order, for each item of a batch (blockIdx.y).
+struct chain_sum_item
+{
+  const double* part;
+  double* out;
+  int s;
+  long n;
+};
+__global__ __launch_bounds__(256) void chain_sum_kernel(const chain_sum_item* items)
+{
+  const chain_sum_item it = items[blockIdx.y];
+  for (long e = blockIdx.x * 256L + threadIdx.x; e < it.n; e += (long)gridDim.x * 256)
+  {
+    double v = it.part[e];
+    for (int t = 1; t < it.s; ++t) v += it.part[(size_t)t * it.n + e];
+    it.out[e] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Batched pieces of a chained sweep (chain_prob in the header): problem
 // blockIdx.y (blockIdx.x for the first diagonal blocks) of a device array.
@@ -449,38 +525,32 @@ __global__ __launch_bounds__(256) void chain_norms_kernel(const chain_prob* ps)
   if (lane == 0) ps[p].norms[col] = sqrt(acc);
 }
 
-// One wave per problem: the Cholesky held and n max|E| <= thr, else the
-// flag is cleared.
-__global__ __launch_bounds__(64) void chain_decide_kernel(const chain_prob* ps, double thr, double* flag)
-{
-  const int p = blockIdx.x, lane = threadIdx.x;
-  const double* part = ps[p].part;
-  double emax = part[lane];
-  emax = fmax(emax, __shfl_xor(emax, 32, 64));
-#pragma unroll
-  for (int o = 16; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
-  if (lane != 0) return;
-  const double* info = ps[p].info;
-  const bool ok = info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]) && emax * ps[p].n <= thr;
-  if (!ok) flag[0] = 0.0;
-}
-static_assert(kChainParts == 64, "chain_decide_kernel reads one partial maximum per lane");
-
-// out[0] = sum over the problems of n max|E| (non-finite as infinity), in
-// problem order: a bound on the spectral norm of every chained E.
+// out[0] = the sum and out[1] = the largest over the problems of n max|E|
+// (from the partial maxima, non-finite as infinity), both infinity when
+// any problem's Cholesky broke down (info[0] != 0 or a non-finite extreme
+// diagonal entry).  The sum bounds the spectral norm of every chained E.
 __global__ __launch_bounds__(64) void chain_bound_kernel(const chain_prob* ps, int count, double* out)
 {
   const int lane = threadIdx.x;
-  double total = 0.0;
+  double total = 0.0, most = 0.0;
+  bool ok = true;
   for (int p = 0; p < count; ++p)
   {
     double emax = ps[p].part[lane];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) emax = fmax(emax, __shfl_xor(emax, o, 64));
+    const double* info = ps[p].info;
+    ok = ok && info[0] == 0.0 && isfinite(info[1]) && isfinite(info[2]);
     total += emax * ps[p].n;
+    most = fmax(most, emax * ps[p].n);
   }
-  if (lane == 0) out[0] = total <= DBL_MAX ? total : INFINITY;
+  if (lane == 0)
+  {
+    out[0] = ok && total <= DBL_MAX ? total : INFINITY;
+    out[1] = ok && most <= DBL_MAX ? most : INFINITY;
+  }
 }
+static_assert(kChainParts == 64, "chain_bound_kernel reads one partial maximum per lane");
 
 // Grouped GEMM: problem blockIdx.z has its own descriptor; tiles beyond its
 // M x N exit.  TT: the tile size (64, or 32 for small outputs).
@@ -1375,6 +1445,58 @@ void launch_chol_step(const chol_step& s, hipStream_t st)
   hipLaunchKernelGGL(chol_step_kernel, dim3(wgs), dim3(256), 0, st, s);
 }
 
+int chain_gemm_slices(int M, int N, int K)
+{
+  const long tiles = (long)((M + 31) / 32) * ((N + 31) / 32);
+  if (tiles >= 128 || K < 128) return 1;
+  long s = std::min<long>(256 / tiles, K / 64);
+  s = std::min<long>(s, 8);
+  return s < 2 ? 1 : (int)s;
+}
+
+int launch_gemm_chain(const gemm_desc& g, int na, long psa, int nb, long psb, int slices, double* part,
+                      hipStream_t st)
+{
+  if (g.M <= 0 || g.N <= 0) return 0;
+  int kper = g.K, s_eff = 1;
+  if (part)
+  {
+    kper = ((g.K + slices - 1) / slices + 63) / 64 * 64;
+    s_eff = (g.K + kper - 1) / kper;
+  }
+  dim3 grid((g.N + 31) / 32, (g.M + 31) / 32, part ? s_eff : 1);
+  if (na > 8 || nb > 8 || (na > 1 && nb > 1)) throw error(TTA_ERR_ARG, "launch_gemm_chain: summed operands");
+  // Summand capacity rounded up to 2, 4 or 8 (unused ones load nothing).
+  auto cap = [](int x) { return x <= 1 ? 1 : (x <= 2 ? 2 : (x <= 4 ? 4 : 8)); };
+  const int ca = cap(na), cb = cap(nb);
+#define TTA_CHAIN_LAUNCH(A_, B_) \
+  hipLaunchKernelGGL((gemm_chain_kernel<A_, B_>), grid, dim3(256), 0, st, g, na, psa, nb, psb, kper, part)
+  if (ca == 2)
+    TTA_CHAIN_LAUNCH(2, 1);
+  else if (ca == 4)
+    TTA_CHAIN_LAUNCH(4, 1);
+  else if (ca == 8)
+    TTA_CHAIN_LAUNCH(8, 1);
+  else if (cb == 2)
+    TTA_CHAIN_LAUNCH(1, 2);
+  else if (cb == 4)
+    TTA_CHAIN_LAUNCH(1, 4);
+  else if (cb == 8)
+    TTA_CHAIN_LAUNCH(1, 8);
+  else
+    TTA_CHAIN_LAUNCH(1, 1);
+#undef TTA_CHAIN_LAUNCH
+  return part ? s_eff : 0;
+}
+
+void launch_chain_sum(const void* items_dev, int count, long max_n, hipStream_t st)
+{
+  if (count <= 0 || max_n <= 0) return;
+  const int blocks = (int)std::min<long>((max_n + 255) / 256, 512);
+  hipLaunchKernelGGL(chain_sum_kernel, dim3(blocks, count), dim3(256), 0, st,
+                     static_cast<const chain_sum_item*>(items_dev));
+}
+
 void launch_chain_potrf(const chain_prob* ps, int count, hipStream_t st)
 {
   if (count <= 0) return;
@@ -1418,11 +1540,6 @@ void launch_chain_bound(const chain_prob* ps, int count, double* out, hipStream_
   hipLaunchKernelGGL(chain_bound_kernel, dim3(1), dim3(64), 0, st, ps, count, out);
 }
 
-void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st)
-{
-  if (count <= 0) return;
-  hipLaunchKernelGGL(chain_decide_kernel, dim3(count), dim3(64), 0, st, ps, thr, flag);
-}
 
 void launch_gemm_multi(const gemm_multi& m, hipStream_t st)
 {

@@ -315,6 +315,18 @@ struct chain_prob
   int n = 0;
 };
 constexpr int kChainParts = 64;
+// One product of a Gram chain on 32 x 32 tiles.  One operand may be the
+// sum of na (or nb) <= 8 buffers psa (psb) elements apart, summed in order
+// as it is loaded.  With part null, C = alpha A B + beta C; otherwise the K range is
+// cut into at most `slices` slices (chain_gemm_slices) whose partial
+// products go to part (row-major M x N each, M N apart), and the number of
+// slices used is returned.
+int chain_gemm_slices(int M, int N, int K);
+int launch_gemm_chain(const gemm_desc& g, int na, long psa, int nb, long psb, int slices, double* part,
+                      hipStream_t st);
+// For each of `count` items (a device array of {const double* part;
+// double* out; int s; long n}): out = P_0 + ... + P_{s-1}, in order.
+void launch_chain_sum(const void* items_dev, int count, long max_n, hipStream_t st);
 // The blocked Cholesky with inverse (chol_inv_blocked's steps) of `count`
 // problems at once: the first diagonal block of each, then block step j0 of
 // every problem with n > j0 (max_n bounds the n's).
@@ -326,11 +338,10 @@ void launch_chain_minus_eye(const chain_prob* ps, int count, int max_n, hipStrea
 void launch_chain_polar(const chain_prob* ps, int count, int max_n, hipStream_t st);
 // Column norms of Rf and Y into norms.
 void launch_chain_norms(const chain_prob* ps, int count, int max_n, hipStream_t st);
-// out[0] = sum over the problems of n max|E| from the partial maxima.
+// out[0] = the sum and out[1] = the largest over the problems of n max|E|
+// (from the partial maxima); both infinity when any problem's Cholesky
+// broke down (info[0] != 0 or non-finite extreme diagonal entries).
 void launch_chain_bound(const chain_prob* ps, int count, double* out, hipStream_t st);
-// flag[0] = 0 unless every problem's Cholesky held (info[0] == 0, finite
-// extreme diagonal entries) and n max|E| <= thr.
-void launch_chain_decide(const chain_prob* ps, int count, double thr, double* flag, hipStream_t st);
 // Epilogues folded into the split-K sum of a square row-major n x n product
 // v = alpha P + beta C (CholQR2's second pass):
 //   SPLITK_EPI_GRAM: C = E = v - I, out2 = Phi(E) (if given), part[b] = max |E| over

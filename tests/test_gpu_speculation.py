@@ -1,9 +1,9 @@
 """Speculative sweeps (tta_set_speculation).
 
-The QR and SVD sweeps of round decide CholQR2 acceptance and the rank
-certificates on the device, into one flag read once per sweep; a sweep with a
-failed decision is undone from a copy of the TT and run again with host
-decisions.  Checked here: speculation on and off give bit-identical cores
+The SVD sweep of round decides CholQR2 acceptance and the rank certificates
+on the device, into one flag read once per sweep; a sweep with a failed
+decision is undone from a copy of the TT and run again with host decisions
+(the QR sweep decides on the host step by step).  Checked here: speculation on and off give bit-identical cores
 (and the oracle's ranks and tensor) where every decision holds (the MPO
 product, both gauges, and the naive matvec_round built on round) and where
 some fail (config 1, whose bonds the rank rule cuts; a zero-padded TT, on
@@ -54,7 +54,7 @@ def test_speculation_identical_on_mpo_product(gauge):
     y = T.matvec(to_dev(A), to_dev(x))
     on, (sw, failed) = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), True)
     off, (sw0, _) = _run(ctx, lambda: T.round(y, eps=1e-8, gauge=gauge), False)
-    assert (sw, failed, sw0) == (2, 0, 0)
+    assert (sw, failed, sw0) == (1, 0, 0)
     _same(on, off)
     assert on.ranks == ref.ranks and rel_err(cores3(on), ref) <= 1e-13
 
@@ -66,7 +66,7 @@ def test_speculation_identical_on_matvec_round():
     Ad, xd = to_dev(A), to_dev(x)
     on, (sw, failed) = _run(ctx, lambda: T.matvec_round(Ad, xd, eps=1e-8), True)
     off, _ = _run(ctx, lambda: T.matvec_round(Ad, xd, eps=1e-8), False)
-    assert sw == 2 and failed == 0
+    assert sw == 1 and failed == 0
     _same(on, off)
     assert on.ranks == ref.ranks and rel_err(cores3(on), ref) <= 1e-13
 
@@ -74,14 +74,14 @@ def test_speculation_identical_on_matvec_round():
 def test_speculation_falls_back_where_the_rule_cuts():
     # Every interior bond of config 1 truncates 24 -> 16: the speculative SVD
     # sweep fails its certificates and is run again; the next call's SVD
-    # sweep runs checked (its QR sweep stays speculative).
+    # sweep runs checked.
     c, _, _ = cfg1_input()
     ref = O.round(c, 1e-10)
     ctx = T.Context.default()
     cd = to_dev(c)
     (r1, r2), (sw, failed) = _run(ctx, lambda: (T.round(cd, eps=1e-10), T.round(cd, eps=1e-10)), True)
     off, _ = _run(ctx, lambda: T.round(cd, eps=1e-10), False)
-    assert (sw, failed) == (3, 1)
+    assert (sw, failed) == (1, 1)
     _same(r1, off)
     _same(r2, off)
     assert r1.ranks == ref.ranks and rel_err(cores3(r1), ref) <= 1e-12
@@ -89,8 +89,8 @@ def test_speculation_falls_back_where_the_rule_cuts():
 
 def test_speculation_falls_back_on_cholqr_breakdown():
     # Exact zero padding of a rank-4 TT to bond 40: the Grams are singular,
-    # so the speculative QR sweep fails CholQR2's acceptance, and the padding
-    # is cut, so the SVD sweep fails its certificates.
+    # so the QR sweep's CholQR2 declines (Householder takes those steps), and
+    # the padding is cut, so the SVD sweep fails its certificates.
     a = O.OTT.random([4, 4, 4, 4, 4, 4], 4, 9)
     pad = []
     cores = a.cores()
@@ -106,7 +106,7 @@ def test_speculation_falls_back_on_cholqr_breakdown():
     bd = to_dev(b)
     on, (sw, failed) = _run(ctx, lambda: T.round(bd, eps=1e-10), True)
     off, _ = _run(ctx, lambda: T.round(bd, eps=1e-10), False)
-    assert sw == 2 and failed == 2
+    assert sw == 1 and failed == 1
     _same(on, off)
     assert on.ranks == ref.ranks and rel_err(cores3(on), ref) <= 1e-12
 
