@@ -1309,12 +1309,35 @@ void left_qr_step(tta_ctx* c, tta_tt* t, int k, bool absorb = true)
   t->ranks[k + 1] = kk;
 }
 
+void flush_spec_certs(tta_ctx* c);
+
+// Thrown when a speculative sweep is found to have failed before its end.
+struct spec_abort
+{
+};
+
+// A step of a speculative sweep that needs its SVD (no certificate applies)
+// synchronises anyway, to read the truncation rank; there the decisions
+// recorded so far (the sweep's earlier certificates and CholQR2
+// acceptances, this step's included) are made at once.  If all held, the
+// rest of the sweep runs checked and nothing needs undoing; otherwise the
+// sweep is abandoned at once (run_sweep undoes it and runs it checked), so
+// no further SVDs are spent on a sweep that will be discarded.
+void spec_checkpoint(tta_ctx* c)
+{
+  if (!c->spec) return;
+  flush_spec_certs(c);
+  if (read_double(c, spec_flag(c)) != 1.0) throw spec_abort{};
+  c->spec = false;
+}
+
 // Jacobi SVD of the square q x q column-major X (ld q), rotations into J
 // (ld q): X J = W.  Leaves sorted norms s and order perm in scratch and
 // returns the truncation rank decided on the device (or q with keep_all).
 int square_svd(tta_ctx* c, double* X, int q, double* J, double delta, int max_rank, bool keep_all, double** s_out,
                int** perm_out)
 {
+  spec_checkpoint(c);
   int* ints = small_int(c);
   double* sv = ctx_ws(c, WS_PART, 1024 + 2 * (size_t)q + 8) + 1024;
   int* perm = reinterpret_cast<int*>(sv + q);
@@ -1939,6 +1962,20 @@ void run_sweep(tta_ctx* c, tta_tt* t, bool& last_ok, F&& sweep)
   {
     sweep();
     flush_spec_certs(c);
+  }
+  catch (const spec_abort&)
+  {
+    // Failed before its end (spec_checkpoint): undone and run checked.
+    c->spec = false;
+    drop_spec_certs(c);
+    c->spec_sweeps += 1;
+    c->spec_failed += 1;
+    last_ok = false;
+    for (int k = 0; k < t->d(); ++k) c->mem.put(t->cores[k]);
+    t->cores = snap.cores;
+    t->ranks = snap.ranks;
+    sweep();
+    return;
   }
   catch (...)
   {
