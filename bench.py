@@ -558,19 +558,22 @@ def main():
     for _ in range(args.prof_steps):
         y = T.matvec_round(A, x, eps=EPS)
     ctx.synchronize()
-    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "gemm_multi", "chol_step", "apply",
-                                           "chol", "rinv", "other")}
+    stats = {k: ctx.prof_read(k) for k in ("qr", "jacobi", "gemm", "gemm_splitk", "gemm_chain", "gemm_multi",
+                                           "gemm_grouped", "chol_step", "apply", "chol", "rinv", "other")}
     cert_tried, cert_held = ctx.rank_certificates()
     cholqr_tried, cholqr_used = ctx.cholqr_counts()
     spec_sweeps, spec_failed = ctx.speculation_counts()
     chain_blocks, chain_steps, chain_one_pass, chain_rejected = ctx.chain_stats()
     # Single-kernel classes: every timed launch is one kernel (the Jacobi
     # round kernel, the FP64 GEMM, the MPO core contraction, CholQR2's
-    # Cholesky, the triangular inverse); the roofline is quoted for the one
+    # Cholesky, the triangular inverse, the Gram chains' products, the
+    # chained blocks' grouped products); the roofline is quoted for the one
     # with the most device time.
     single = {"jacobi_round": ("bj_round_kernel", ctx.prof_read("jacobi_round")),
               "gemm": ("gemm_f64_kernel", stats["gemm"]), "apply": ("matvec_core_batch_kernel", stats["apply"]),
               "gemm_splitk": ("gemm_f64_splitk_kernel", stats["gemm_splitk"]),
+              "gemm_chain": ("gemm_chain_kernel", stats["gemm_chain"]),
+              "gemm_grouped": ("gemm_f64_grouped_kernel", stats["gemm_grouped"]),
               "gemm_multi": ("gemm_f64_multi_kernel", stats["gemm_multi"]),
               "chol_step": ("chol_step_kernel", stats["chol_step"]),
               "chol": ("potrf_inv_kernel", stats["chol"]), "rinv": ("rinv_cert_kernel", stats["rinv"])}

@@ -344,7 +344,14 @@ enum tta_kernel_class
    * diagonal block (credited the flops and bytes of those products and of
    * the block factorization). */
   TTA_K_CHOL_STEP = 10,
-  TTA_K_COUNT = 11
+  /* The products of the Gram chains of chained sweep blocks (32 x 32 tiles,
+   * an operand read as the sum of split-K partials; credited 2 M N K flops
+   * and each operand read once, the partials written once). */
+  TTA_K_GEMM_CHAIN = 11,
+  /* The batched products of chained sweep blocks (grouped launches over the
+   * block's steps). */
+  TTA_K_GEMM_GROUPED = 12,
+  TTA_K_COUNT = 13
 };
 /* Enable (1) / disable (0) per-launch event timing; enabling resets totals. */
 int tta_prof_enable(tta_ctx* ctx, int on);

@@ -122,7 +122,8 @@ SIGNATURES = {
 }
 
 KERNEL_CLASSES = {"qr": 0, "jacobi": 1, "gemm": 2, "apply": 3, "other": 4, "jacobi_round": 5, "chol": 6, "rinv": 7,
-                  "gemm_splitk": 8, "gemm_multi": 9, "chol_step": 10}
+                  "gemm_splitk": 8, "gemm_multi": 9, "chol_step": 10, "gemm_chain": 11,
+                  "gemm_grouped": 12}
 
 _lib = None
 

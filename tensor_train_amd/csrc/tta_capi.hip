@@ -837,7 +837,7 @@ void gemm_batch_go(tta_ctx* c, arena& ar, const std::vector<gemm_desc>& gs)
     mm = std::max(mm, g.M);
     mn = std::max(mn, g.N);
   }
-  prof_scope ps(c, TTA_K_GEMM_MULTI, cr.flops, cr.bytes);
+  prof_scope ps(c, TTA_K_GEMM_GROUPED, cr.flops, cr.bytes);
   launch_gemm_grouped(ar.upload(gs), (int)gs.size(), mm, mn, c->stream);
 }
 
@@ -932,7 +932,7 @@ bool chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
         const int qp = qs[p - 1];
         gemm_desc g = dir == 0 ? gdesc(r * n, rp, rp, X[p], rp, 1, gp[p - 1], rp, 1, T, rp, 1)   // X G_{k+1}
                                : gdesc(r, n * rp, r, gp[p - 1], r, 1, X[p], w, 1, T, w, 1);     // H_{k-1} X
-        prof_scope ps(c, TTA_K_GEMM_SPLITK, 2.0 * g.M * g.N * (double)g.K,
+        prof_scope ps(c, TTA_K_GEMM_CHAIN, 2.0 * g.M * g.N * (double)g.K,
                       8.0 * ((double)g.M * g.K + (double)gs[p - 1] * qp * qp + (double)g.M * g.N));
         if (dir == 0)
           launch_gemm_chain(g, 1, 0, gs[p - 1], (long)qp * qp, 1, nullptr, c->stream);
@@ -944,7 +944,7 @@ bool chain_block(tta_ctx* c, tta_tt* t, int ka, int kb, int dir, bool cert, doub
                              : gdesc(q, q, r * n, src, 1, rp, X[p], rp, 1, nullptr, q, 1);  // (.)^T X, left unfoldings
       const int sl = chain_gemm_slices(g.M, g.N, g.K);
       gp[p] = ar.alloc<double>((size_t)sl * q * q);
-      prof_scope ps(c, TTA_K_GEMM_SPLITK, 2.0 * g.M * g.N * (double)g.K,
+      prof_scope ps(c, TTA_K_GEMM_CHAIN, 2.0 * g.M * g.N * (double)g.K,
                     8.0 * (2.0 * (double)g.M * g.K + (double)sl * q * q));
       gs[p] = launch_gemm_chain(g, 1, 0, 1, 0, sl, gp[p], c->stream);
       items[p] = {gp[p], pr[p].*out, gs[p], (long)q * q};
