@@ -377,6 +377,91 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
+// One round of block Jacobi for few problems of q <= 64 PL: workgroup =
+// block pair, one wave per column pair, X of the pair staged in LDS as in
+// bj_round_kernel, but J not staged: thread t < q keeps row t of the pair's
+// 2W columns of J in registers, the inner sweep's rotations are recorded in
+// LDS (the identity where a pair does not rotate) and applied to those rows
+// in order at its end (apply_rotations).  Each inner round then reads and
+// writes only X, and J's updates leave the rotation chain.
+template <int NT, int W, int PL>
+__global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int nbb)
+{
+  static_assert(NT == 64 * W, "one wave per column pair");
+  const jac_prob& P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  if (P.flag && P.flag[1]) return;  // converged in an earlier sweep
+  bj_pair<W> bp;
+  if (!bp.setup(P, s, nbb)) return;
+  extern __shared__ double lds[];
+  double* Xs = lds;
+  double* rcs = lds + 2 * W * q;
+  double* rsn = rcs + (2 * W - 1) * W;
+  double jr[2 * W];
+  bp.template load<NT>(P, Xs, jr);
+  __syncthreads();
+  const int tid = threadIdx.x, lane = tid & 63, pr = tid >> 6;
+  const double tol = jac_tol(q);
+  const double negl2 = jac_negl2(P, P.s[0]);  // ||X||_F^2 from jac_init_kernel
+  int rot = 0;
+  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+  {
+    int li, lj;
+    rr_pair(r2, pr, 2 * W, li, lj);
+    if (li > lj)
+    {
+      const int t = li;
+      li = lj;
+      lj = t;
+    }
+    const bool real = bp.real(li) && bp.real(lj);
+    double* xi = Xs + li * q;
+    double* xj = Xs + lj * q;
+    double u[PL], v[PL];
+    double a = 0.0, b = 0.0, g = 0.0;
+#pragma unroll
+    for (int e = 0; e < PL; ++e)
+    {
+      const int t = lane + 64 * e;
+      u[e] = t < q ? xi[t] : 0.0;
+      v[e] = t < q ? xj[t] : 0.0;
+      a = fma(u[e], u[e], a);
+      b = fma(v[e], v[e], b);
+      g = fma(u[e], v[e], g);
+    }
+    wave_sum3(a, b, g);
+    double cs = 1.0, sn = 0.0;
+    if (real && !(a <= negl2 || b <= negl2) && jacobi_rotation(a, b, g, tol, cs, sn))
+    {
+      rot = 1;
+#pragma unroll
+      for (int e = 0; e < PL; ++e)
+      {
+        const int t = lane + 64 * e;
+        if (t < q)
+        {
+          xi[t] = cs * u[e] - sn * v[e];
+          xj[t] = sn * u[e] + cs * v[e];
+        }
+      }
+    }
+    else
+    {
+      cs = 1.0;
+      sn = 0.0;
+    }
+    if (lane == 0)
+    {
+      rcs[r2 * W + pr] = cs;
+      rsn[r2 * W + pr] = sn;
+    }
+    __syncthreads();
+  }
+  if (tid < q) apply_rotations<W>(jr, rcs, rsn);
+  bp.template store<NT>(P, Xs, jr);
+  if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
+}
+
 // One round of block Jacobi with 16-lane groups: each 16-lane row of a wave
 // owns one column pair of the inner round (4 pairs per wave, W / 4 waves),
 // holds PL = ceil(q / 16) elements of each column in registers and reduces
@@ -952,7 +1037,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     const int W = grouped ? 16 : (narrow ? 8 : ((size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8));
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
-    const size_t lds = grouped ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
+    const size_t lds = (grouped || narrow) ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
     {
       long g = ((long)max_q * max_q + 255) / 256;
       if (g > 1024) g = 1024;
@@ -975,7 +1060,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
           else if (grouped)
             hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
           else if (narrow)
-            hipLaunchKernelGGL((bj_round_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb, 1);
+            hipLaunchKernelGGL((bj_defer_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb);
           else if (W == 16 && max_q <= 128)
             hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
           else if (W == 16 && max_q <= 256)
