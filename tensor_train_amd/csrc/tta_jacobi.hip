@@ -7,8 +7,10 @@
 // q  > 64: block Jacobi.  The columns are cut into blocks of W; each launch
 //   of bj_round_kernel is one round of a round-robin over blocks, one
 //   workgroup per block pair: the pair's 2W columns of X and J are staged
-//   in LDS and orthogonalised among themselves by one inner sweep, then
-//   written back.  A sweep is nb - 1 rounds; the host stops after a sweep
+//   in LDS, rotated by one inner sweep over pairs of them (by default the
+//   cross schedule: the pairs across the two blocks, and each block's own
+//   pairs in a sweep's first round), then written back.  A sweep is nb - 1
+//   rounds and rotates every column pair once; the host stops after a sweep
 //   with no rotation.
 // Rotation rule: rotate while |x_i . x_j| > tol ||x_i|| ||x_j|| with
 // tol = max(1e-15, q u), the worst-case rounding bound of a length-q dot
@@ -24,6 +26,7 @@
 // columns that are discarded, which costs sweeps.  delta = 0 (no truncation,
 // full SVD) disables the rule.
 #include <algorithm>
+#include <type_traits>
 
 #include "../../include/tt_amd.h"
 #include "tta_device.hpp"
@@ -33,6 +36,9 @@
 namespace tta
 {
 using namespace dev;
+
+bool g_jac_cross = true;
+bool g_jac_cross_group = true;
 
 namespace
 {
@@ -123,31 +129,79 @@ __device__ __forceinline__ int rotate_pair(double* xi, double* xj, double* ji, d
   return 1;
 }
 
+// Schedules of a block pair's inner sweep (local columns 0..W-1 of block i,
+// W..2W-1 of block j; W disjoint pairs per inner round, 2W - 1 rounds or W):
+//   kFullSweep: all pairs of the 2W columns by the round-robin over 2W;
+//   kCrossIntra / kCross: W - 1 intra rounds pairing each block's own
+//     columns by the round-robin over W (pairs 0..W/2-1 in block i,
+//     W/2..W-1 in block j; only in a sweep's first round, where every block
+//     meets another once), then W cross rounds, round h pairing column a of
+//     block i with column (a + h) mod W of block j.
+// The cross schedules rotate every column pair of the matrix once per
+// sweep, as cyclic Jacobi does; the full inner sweep repeats each block's own
+// pairs in every round.  Sweeps to convergence come out the same or a few
+// more, for about half the inner rounds per launch.
+constexpr int kFullSweep = 0, kCrossIntra = 1, kCross = 2;
+template <int W, int SCHED>
+constexpr int inner_rounds() { return SCHED == kCross ? W : 2 * W - 1; }
+
+template <int W, int SCHED>
+__device__ __forceinline__ void inner_pair(int r2, int pr, int& li, int& lj)
+{
+  if constexpr (SCHED == kFullSweep)
+  {
+    int x, y;
+    rr_pair(r2, pr, 2 * W, x, y);
+    li = min(x, y);
+    lj = max(x, y);
+  }
+  else if (SCHED == kCrossIntra && r2 < W - 1)
+  {
+    const int blk = pr >= W / 2 ? W : 0;
+    int x, y;
+    rr_pair(r2, pr - (blk ? W / 2 : 0), W, x, y);
+    li = blk + min(x, y);
+    lj = blk + max(x, y);
+  }
+  else
+  {
+    const int h = SCHED == kCrossIntra ? r2 - (W - 1) : r2;
+    li = pr;
+    lj = W + ((pr + h) % W);
+  }
+}
+
 // Deferred J update of one block-pair inner sweep: jr holds one row of the
-// pair's 2W columns of J; the sweep's rotations (round-robin order, rounds
+// pair's 2W columns of J; the sweep's rotations (schedule SCHED, rounds
 // applied in sequence) come from rcs / rsn[r2 * W + pair].  Fully unrolled so
 // the column indices are constants and jr stays in registers.
-template <int W>
+template <int W, int SCHED>
 __device__ __forceinline__ void apply_rotations(double (&jr)[2 * W], const double* rcs, const double* rsn)
 {
 #pragma unroll
-  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+  for (int r2 = 0; r2 < inner_rounds<W, SCHED>(); ++r2)
 #pragma unroll
     for (int pr = 0; pr < W; ++pr)
     {
       int li, lj;
-      rr_pair(r2, pr, 2 * W, li, lj);
-      if (li > lj)
-      {
-        const int t = li;
-        li = lj;
-        lj = t;
-      }
+      inner_pair<W, SCHED>(r2, pr, li, lj);
       const double c = rcs[r2 * W + pr], sv = rsn[r2 * W + pr];
       const double y1 = jr[li], y2 = jr[lj];
       jr[li] = c * y1 - sv * y2;
       jr[lj] = sv * y1 + c * y2;
     }
+}
+
+// Calls f with std::integral_constant<int, sched> for the runtime schedule.
+template <class F>
+void launch_sched(int sched, F&& f)
+{
+  if (sched == kFullSweep)
+    f(std::integral_constant<int, kFullSweep>{});
+  else if (sched == kCrossIntra)
+    f(std::integral_constant<int, kCrossIntra>{});
+  else
+    f(std::integral_constant<int, kCross>{});
 }
 
 // Whole Jacobi in one workgroup with X, J in LDS (q <= 64).
@@ -302,9 +356,10 @@ constexpr size_t bj_lds_bytes(int W, int q) { return sizeof(double) * (2 * (size
 // round-robin over nbb (even) blocks of W columns, one wave per column pair,
 // X and J of the pair staged in LDS ([2W][q] each) and rotated together.
 // With few problems (or q > 128) its 16 waves per workgroup hide the latency
-// of each rotation chain better than deferring J would save.
+// of each rotation chain better than deferring J would save.  cross: the
+// inner sweep follows the cross schedules (inner_pair), else kFullSweep.
 template <int NT, int W, int PL>
-__global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int nbb, int inner_sweeps)
+__global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int nbb, int inner_sweeps, int cross)
 {
   const jac_prob& P = jpick(bt, blockIdx.y);
   const int q = P.q;
@@ -343,20 +398,21 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
   const double tol = jac_tol(q);
   const double negl2 = jac_negl2(P, P.s[0]);  // ||X||_F^2 from jac_init_kernel
   int rot = 0;
+  const bool intra = s == 0;
+  const int nr = (!cross || intra) ? 2 * W - 1 : W;
   for (int it = 0; it < inner_sweeps; ++it)
   {
-    for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+    for (int r2 = 0; r2 < nr; ++r2)
     {
       for (int pr = wid; pr < W; pr += NT / 64)
       {
         int li, lj;
-        rr_pair(r2, pr, 2 * W, li, lj);
-        if (li > lj)
-        {
-          const int t = li;
-          li = lj;
-          lj = t;
-        }
+        if (!cross)
+          inner_pair<W, kFullSweep>(r2, pr, li, lj);
+        else if (intra)
+          inner_pair<W, kCrossIntra>(r2, pr, li, lj);
+        else
+          inner_pair<W, kCross>(r2, pr, li, lj);
         if ((li % W) >= ncol[li / W] || (lj % W) >= ncol[lj / W]) continue;
         rot |= rotate_pair<PL>(Xs + li * q, Xs + lj * q, Js + li * q, Js + lj * q, q, q, tol, negl2, lane);
       }
@@ -384,7 +440,8 @@ __global__ __launch_bounds__(NT) void bj_round_kernel(jac_batch bt, int s, int n
 // LDS (the identity where a pair does not rotate) and applied to those rows
 // in order at its end (apply_rotations).  Each inner round then reads and
 // writes only X, and J's updates leave the rotation chain.
-template <int NT, int W, int PL>
+// The inner sweep follows schedule SCHED (inner_pair).
+template <int NT, int W, int PL, int SCHED>
 __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int nbb)
 {
   static_assert(NT == 64 * W, "one wave per column pair");
@@ -404,16 +461,10 @@ __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int n
   const double tol = jac_tol(q);
   const double negl2 = jac_negl2(P, P.s[0]);  // ||X||_F^2 from jac_init_kernel
   int rot = 0;
-  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+  for (int r2 = 0; r2 < inner_rounds<W, SCHED>(); ++r2)
   {
     int li, lj;
-    rr_pair(r2, pr, 2 * W, li, lj);
-    if (li > lj)
-    {
-      const int t = li;
-      li = lj;
-      lj = t;
-    }
+    inner_pair<W, SCHED>(r2, pr, li, lj);
     const bool real = bp.real(li) && bp.real(lj);
     double* xi = Xs + li * q;
     double* xj = Xs + lj * q;
@@ -457,7 +508,7 @@ __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int n
     }
     __syncthreads();
   }
-  if (tid < q) apply_rotations<W>(jr, rcs, rsn);
+  if (tid < q) apply_rotations<W, SCHED>(jr, rcs, rsn);
   bp.template store<NT>(P, Xs, jr);
   if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
 }
@@ -468,10 +519,10 @@ __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int n
 // its three dot products within the row (4 DPP stages, no cross-row step).
 // The reduction and the rotation formulas are thus shared by the four pairs
 // of a wave instead of being paid per pair, which cuts the VALU work per
-// rotation about threefold against one pair per wave.  Same pairing,
-// rotation rule and convergence flag as bj_round_kernel; J deferred
-// (bj_pair).
-template <int W, int PL>
+// rotation about threefold against one pair per wave.  Inner sweep by
+// schedule SCHED, rotation rule and convergence flag as bj_round_kernel; J
+// deferred (bj_pair).
+template <int W, int PL, int SCHED>
 __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, int nbb)
 {
   constexpr int NT = W * 16;
@@ -493,16 +544,10 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
   const int pr = tid >> 4;  // pair slot of this 16-lane group
   const int li16 = tid & 15;
   int rot = 0;
-  for (int r2 = 0; r2 < 2 * W - 1; ++r2)
+  for (int r2 = 0; r2 < inner_rounds<W, SCHED>(); ++r2)
   {
     int li, lj;
-    rr_pair(r2, pr, 2 * W, li, lj);
-    if (li > lj)
-    {
-      const int t = li;
-      li = lj;
-      lj = t;
-    }
+    inner_pair<W, SCHED>(r2, pr, li, lj);
     const bool real = bp.real(li) && bp.real(lj);
     double* xi = Xs + li * q;
     double* xj = Xs + lj * q;
@@ -558,7 +603,7 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
     }
     __syncthreads();
   }
-  if (tid < q) apply_rotations<W>(jr, rcs, rsn);
+  if (tid < q) apply_rotations<W, SCHED>(jr, rcs, rsn);
   bp.template store<NT>(P, Xs, jr);
   if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
 }
@@ -1035,6 +1080,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     // VALU throughput on the rotations, not by the round count.
     const bool narrow = !grouped && batch < 32 && max_q > 128 && max_q <= 256;
     const int W = grouped ? 16 : (narrow ? 8 : ((size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8));
+    const bool cross = grouped ? g_jac_cross_group : g_jac_cross;
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
     const size_t lds = (grouped || narrow) ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
@@ -1055,20 +1101,28 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
         {
           const dim3 grid(nbb / 2, batch);
           prof_scope ps(c, TTA_K_JACOBI_ROUND);
+          const int sched = !cross ? kFullSweep : (s == 0 ? kCrossIntra : kCross);
           if (grouped && max_q <= 128)
-            hipLaunchKernelGGL((bj_group_kernel<16, 8>), grid, dim3(256), lds, st, b, s, nbb);
+            launch_sched(sched, [&](auto sc) {
+              hipLaunchKernelGGL((bj_group_kernel<16, 8, decltype(sc)::value>), grid, dim3(256), lds, st, b, s, nbb);
+            });
           else if (grouped)
-            hipLaunchKernelGGL((bj_group_kernel<16, 16>), grid, dim3(256), lds, st, b, s, nbb);
+            launch_sched(sched, [&](auto sc) {
+              hipLaunchKernelGGL((bj_group_kernel<16, 16, decltype(sc)::value>), grid, dim3(256), lds, st, b, s, nbb);
+            });
           else if (narrow)
-            hipLaunchKernelGGL((bj_defer_kernel<512, 8, 4>), grid, dim3(512), lds, st, b, s, nbb);
+            launch_sched(sched, [&](auto sc) {
+              hipLaunchKernelGGL((bj_defer_kernel<512, 8, 4, decltype(sc)::value>), grid, dim3(512), lds, st, b, s,
+                                 nbb);
+            });
           else if (W == 16 && max_q <= 128)
-            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 2>), grid, dim3(NT), lds, st, b, s, nbb, 1, cross ? 1 : 0);
           else if (W == 16 && max_q <= 256)
-            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 4>), grid, dim3(NT), lds, st, b, s, nbb, 1, cross ? 1 : 0);
           else if (W == 16)
-            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 5>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+            hipLaunchKernelGGL((bj_round_kernel<NT, 16, 5>), grid, dim3(NT), lds, st, b, s, nbb, 1, cross ? 1 : 0);
           else
-            hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1);
+            hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1, cross ? 1 : 0);
         }
         hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
         hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);

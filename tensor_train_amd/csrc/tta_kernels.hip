@@ -387,28 +387,58 @@ __device__ __forceinline__ void chol_step_run(const chol_step& s, int t)
 
 __global__ __launch_bounds__(256) void chol_step_kernel(chol_step s) { chol_step_run(s, blockIdx.x); }
 
+// The tile (bx, by, bz) this workgroup computes.  With bh = 0 it is its
+// block index.  Otherwise the grid (whose size is a multiple of 8) is
+// renumbered so that the workgroups the dispatcher deals to one XCD (every
+// eighth, in launch order) form one contiguous run of tiles, and that run
+// is walked in column strips bh tiles tall: each XCD then covers a compact
+// block of bh x (run / bh) tiles of one slice and fetches only the operand
+// strips of that block into its own L2.  A bijection whenever gridDim.y is
+// a multiple of bh.
+__device__ __forceinline__ void xcd_tile(int bh, int& bx, int& by, int& bz)
+{
+  bx = blockIdx.x;
+  by = blockIdx.y;
+  bz = blockIdx.z;
+  if (bh <= 0) return;
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int per = gx * gy;
+  const int L = bx + gx * (by + gy * bz);
+  const int run = per * gridDim.z / 8;
+  const int l = (L % 8) * run + L / 8;
+  bz = l / per;
+  int r = l - bz * per;
+  const int grp = r / (bh * gx);
+  r -= grp * bh * gx;
+  bx = r / bh;
+  by = grp * bh + r % bh;
+}
+
 // One product of a Gram chain on 32 x 32 tiles: an operand given as the
 // sum of split-K partials (na / nb buffers) is summed while loaded; with
-// part set, slice blockIdx.z of the K range [z kper, (z+1) kper) writes its
-// partial product P_z (row-major M x N, M N apart) instead of C.
+// part set, slice z of the K range [z kper, (z+1) kper) writes its partial
+// product P_z (row-major M x N, M N apart) instead of C.  Tiles are ordered
+// per XCD as xcd_tile describes.
 template <int SA, int SB>
 __global__ __launch_bounds__(256) void gemm_chain_kernel(gemm_desc g, int na, long psa, int nb, long psb, int kper,
-                                                         double* part)
+                                                         double* part, int bh)
 {
   constexpr int TT = 32;
+  int bx, by, bz;
+  xcd_tile(bh, bx, by, bz);
   if (!part)
   {
     gemm_tile_t<double, double, double, double, TT, SA, SB, 64>(g.M, g.N, g.K, g.A, g.sar, g.sac, g.B, g.sbr, g.sbc, g.C,
-                                                            g.scr, g.scc, g.alpha, g.beta, blockIdx.y * TT,
-                                                            blockIdx.x * TT, na, psa, nb, psb);
+                                                            g.scr, g.scc, g.alpha, g.beta, by * TT, bx * TT, na, psa,
+                                                            nb, psb);
     return;
   }
-  const int k0 = blockIdx.z * kper;
+  const int k0 = bz * kper;
   const int kn = min(g.K - k0, kper);
-  double* P = part + (size_t)blockIdx.z * g.M * g.N;
+  double* P = part + (size_t)bz * g.M * g.N;
   gemm_tile_t<double, double, double, double, TT, SA, SB, 64>(g.M, g.N, kn, g.A + (size_t)k0 * g.sac, g.sar, g.sac,
                                                           g.B + (size_t)k0 * g.sbr, g.sbr, g.sbc, P, g.N, 1, 1.0, 0.0,
-                                                          blockIdx.y * TT, blockIdx.x * TT, na, psa, nb, psb);
+                                                          by * TT, bx * TT, na, psa, nb, psb);
 }
 
 // out = the sum of the s partials P_0 .. P_{s-1} (n elements each), in
@@ -1445,6 +1475,34 @@ void launch_chol_step(const chol_step& s, hipStream_t st)
   hipLaunchKernelGGL(chol_step_kernel, dim3(wgs), dim3(256), 0, st, s);
 }
 
+bool g_chain_xcd = true;
+
+int xcd_strip_height(int gx, int gy, int gz, double wa, double wb)
+{
+  const long per = (long)gx * gy, total = per * gz;
+  if (total < 16 || total % 8 != 0) return 0;
+  const long run = total / 8;
+  if (run >= per) return gy;  // each XCD holds whole slices
+  // A run of bh x bw tiles: the A strips (weight wa per tile row) are
+  // fetched by gx / bw XCDs each, the B strips (wb per tile column) by
+  // gy / bh; the cheapest shape that tiles the grid exactly wins.
+  int best = 0;
+  double best_cost = 0.0;
+  for (int bh = 1; bh <= gy; ++bh)
+  {
+    if (gy % bh != 0 || run % bh != 0) continue;
+    const long bw = run / bh;
+    if (bw > gx || gx % bw != 0) continue;
+    const double cost = wa * (double)(gx / bw) + wb * (double)(gy / bh);
+    if (!best || cost < best_cost)
+    {
+      best = bh;
+      best_cost = cost;
+    }
+  }
+  return best;
+}
+
 int chain_gemm_slices(int M, int N, int K)
 {
   const long tiles = (long)((M + 31) / 32) * ((N + 31) / 32);
@@ -1466,11 +1524,14 @@ int launch_gemm_chain(const gemm_desc& g, int na, long psa, int nb, long psb, in
   }
   dim3 grid((g.N + 31) / 32, (g.M + 31) / 32, part ? s_eff : 1);
   if (na > 8 || nb > 8 || (na > 1 && nb > 1)) throw error(TTA_ERR_ARG, "launch_gemm_chain: summed operands");
+  const int bh = g_chain_xcd ? xcd_strip_height((int)grid.x, (int)grid.y, (int)grid.z, (double)g.M * na,
+                                                (double)g.N * nb)
+                             : 0;
   // Summand capacity rounded up to 2, 4 or 8 (unused ones load nothing).
   auto cap = [](int x) { return x <= 1 ? 1 : (x <= 2 ? 2 : (x <= 4 ? 4 : 8)); };
   const int ca = cap(na), cb = cap(nb);
 #define TTA_CHAIN_LAUNCH(A_, B_) \
-  hipLaunchKernelGGL((gemm_chain_kernel<A_, B_>), grid, dim3(256), 0, st, g, na, psa, nb, psb, kper, part)
+  hipLaunchKernelGGL((gemm_chain_kernel<A_, B_>), grid, dim3(256), 0, st, g, na, psa, nb, psb, kper, part, bh)
   if (ca == 2)
     TTA_CHAIN_LAUNCH(2, 1);
   else if (ca == 4)

@@ -315,6 +315,17 @@ struct chain_prob
   int n = 0;
 };
 constexpr int kChainParts = 64;
+// Cross schedule of the staged block-Jacobi kernels (on by default): each
+// launch rotates only the pairs across its two blocks, and a block's own
+// pairs once per sweep.
+extern bool g_jac_cross;
+// The same for the grouped kernel of batches.
+extern bool g_jac_cross_group;
+// XCD-aware tile order of the chain products (on by default), and the
+// strip height it picks for a gx x gy x gz tile grid whose A and B operands
+// weigh wa and wb in all (0: block order).
+extern bool g_chain_xcd;
+int xcd_strip_height(int gx, int gy, int gz, double wa, double wb);
 // One product of a Gram chain on 32 x 32 tiles.  One operand may be the
 // sum of na (or nb) <= 8 buffers psa (psb) elements apart, summed in order
 // as it is loaded.  With part null, C = alpha A B + beta C; otherwise the K range is
