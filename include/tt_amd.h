@@ -450,6 +450,18 @@ int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long*
  * default. */
 int tta_set_graphs(tta_ctx* ctx, int on);
 
+/* ---- Gram-driven block Jacobi (no reference counterpart) ------------------- */
+/* Block-Jacobi SVDs of fewer than 32 problems of 65..512 columns run each
+ * round on the block pair's Gram matrix: the pair's 16 columns are staged
+ * once, their 16 x 16 Gram formed on the matrix cores, the inner sweep's
+ * rotations decided and applied on the Gram alone (the one-sided rule, the
+ * same tolerance), and the columns of X and J updated once by the
+ * accumulated 16 x 16 rotation.  The Gram is formed afresh from the columns
+ * at every launch, so convergence is judged on the columns' own dot
+ * products.  The singular values and vectors agree with on = 0 to rounding
+ * (different rounding, not bit-identical).  On by default. */
+int tta_set_jacobi_gram(tta_ctx* ctx, int on);
+
 /* ---- autotuner (no reference counterpart) --------------------------------- */
 /* The grouped FP64 GEMMs of the batched paths choose their tile size (64 x 64
  * or 32 x 32 outputs per workgroup) per shape class (device, log2 of the

@@ -129,6 +129,12 @@ class Context:
         default; identical results)."""
         self.check(self.lib.tta_set_graphs(self.h, 1 if on else 0))
 
+    def set_jacobi_gram(self, on: bool) -> None:
+        """Block-Jacobi rounds of few problems (65..512 columns) driven by the
+        block pair's 16 x 16 Gram matrix (on by default; equal results up to
+        rounding)."""
+        self.check(self.lib.tta_set_jacobi_gram(self.h, 1 if on else 0))
+
     def set_speculation(self, on: bool) -> None:
         """Speculative sweeps (on by default): CholQR2 acceptance and the rank
         certificates decided on the device, one host read per sweep; a sweep

@@ -96,6 +96,7 @@ SIGNATURES = {
     "tta_prof_read": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _f64p, _f64p, _f64p]),
     "tta_set_cholqr": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_set_graphs": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "tta_set_jacobi_gram": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_set_autotune": (ctypes.c_int, [_vp, ctypes.c_int]),
     "tta_prof_autotune": (ctypes.c_int, [_vp] + [ctypes.POINTER(ctypes.c_longlong)] * 3),
     "tta_set_speculation": (ctypes.c_int, [_vp, ctypes.c_int]),

@@ -3727,6 +3727,13 @@ int tta_set_graphs(tta_ctx* c, int on)
   return TTA_OK;
 }
 
+int tta_set_jacobi_gram(tta_ctx* c, int on)
+{
+  if (!c) return TTA_ERR_ARG;
+  c->jac_gram = on != 0;
+  return TTA_OK;
+}
+
 int tta_prof_rank_revealing(tta_ctx* c, long long* tried, long long* held)
 {
   if (!c) return TTA_ERR_ARG;

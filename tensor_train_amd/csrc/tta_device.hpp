@@ -230,13 +230,15 @@ __device__ __forceinline__ double rsqrt_newton(double y)
 // zero (the column is negligible and its products would never settle), when
 // g is not finite, or when the angle rounds to zero; so every rotation
 // reported changes the columns.  For |zeta| > 1e150 the tangent takes its
-// asymptotic form 1 / (2 zeta), where zeta^2 would overflow.
+// asymptotic form 1 / (2 zeta), where zeta^2 would overflow.  tan_out
+// receives the tangent t (sn = cs t) of a rotation reported.
 // The rotation sits on the critical path of every inner Jacobi round, so its
 // divisions and square roots are rcp_newton / rsqrt_newton (the rotation
 // only has to be accurate: cs^2 + sn^2 = 1 to rounding), and the threshold
 // compares squares, for squared norms within 1e+-140 (a b can neither
 // overflow nor underflow); other scales keep the correctly rounded form.
-__device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn)
+__device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn,
+                                                double& tan_out)
 {
   if (a == 0.0 || b == 0.0) return false;
   if (!(a > 1.0e-140 && a < 1.0e140 && b > 1.0e-140 && b < 1.0e140))
@@ -248,6 +250,7 @@ __device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, do
     const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (az > 1.0e150 ? 2.0 * az : az + sqrt(1.0 + zeta * zeta));
     cs = 1.0 / sqrt(1.0 + t * t);
     sn = cs * t;
+    tan_out = t;
     return sn != 0.0;
   }
   // Here |g| > tol sqrt(a b) > 1e-155 once past the test: no denormal g.
@@ -265,7 +268,15 @@ __device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, do
   const double t = (zeta >= 0.0 ? 1.0 : -1.0) * rcp_newton(den);
   cs = rsqrt_newton(fma(t, t, 1.0));
   sn = cs * t;
+  tan_out = t;
   return sn != 0.0;
+}
+
+// The same without the tangent t = sn / cs.
+__device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn)
+{
+  double t;
+  return jacobi_rotation(a, b, g, tol, cs, sn, t);
 }
 
 // Circle-method round-robin pairing of qq (even) players: round s in

@@ -112,6 +112,9 @@ struct tta_ctx
   long long rr_tried = 0, rr_held = 0;
   // Block-Jacobi sweeps captured in a HIP graph and replayed (tta_set_graphs).
   bool graphs = false;
+  // Block-Jacobi rounds of few problems driven by the block pair's Gram
+  // matrix (bj_gram_kernel; tta_set_jacobi_gram, on by default).
+  bool jac_gram = true;
   // Device address of the allocation-free entry point's scratch array on
   // this context's device (looked up at context creation).
   double* noalloc_scratch = nullptr;

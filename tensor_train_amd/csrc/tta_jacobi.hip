@@ -82,6 +82,15 @@ __device__ __forceinline__ void wave_sum3(double& a, double& b, double& g)
   g = (read_lane(g, 0) + read_lane(g, 16)) + (read_lane(g, 32) + read_lane(g, 48));
 }
 
+// Orders one wave's LDS accesses: the wave's LDS writes before it have
+// completed, and are seen by every lane's reads after it (the memory clobber
+// keeps the compiler from moving accesses across).
+__device__ __forceinline__ void wave_sync_lds()
+{
+  asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Rotate columns (xi, xj) of length p and (ji, jj) of length q in LDS by the
 // Jacobi rotation that orthogonalises xi and xj.  One wave; returns 1 when a
 // rotation was applied.  PL = elements per lane held in registers (p and q
@@ -608,6 +617,215 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
   if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
+
+// LDS bytes of bj_gram_kernel<*, *, 8, *> for problems of up to q columns.
+constexpr size_t bjg_lds_bytes(int q) { return sizeof(double) * (32 * (size_t)q + 5 * 256 + 2 * 15 * 8 + 8); }
+
+// One round of block Jacobi whose rotations are decided on the block pair's
+// Gram matrix, for q <= NH = NT RPT / 2: workgroup = block pair (bi, bj) of
+// the round-robin over nbb (even) blocks of W = 8 columns.  Row slot
+// t + k NT (k < RPT) of thread t loads row t + k NT of the pair's 16 columns
+// of X below NH, row t + k NT - NH of J from NH on (every load in flight at
+// once), into LDS.
+//   1. X_p is staged in LDS and its Gram G = X_p^T X_p formed on the matrix
+//      cores by four waves (one 16 x 16 tile, each wave a share of the rows,
+//      the partials summed in wave order; G kept exactly symmetric).
+//   2. One wave runs the inner sweep (schedule SCHED) on G alone: lane
+//      8 P + Q holds the 2 x 2 block of G at the round's row pair P and
+//      column pair Q.  The diagonal lanes decide and compute their pair's
+//      rotation from their own block by the one-sided rule (the tolerance
+//      and negligible-column rule of the staged kernels) and record it;
+//      every lane takes the rotations of its two pairs by lane shuffles and
+//      updates its block, G <- R^T G R (the rotated pair's entry set to
+//      zero, the new diagonal a - t g, b + t g); the blocks are exchanged
+//      through LDS for the next round's pairing.  No workgroup barrier per
+//      round.
+//   3. Every row of X_p and J_p takes the recorded rotations in order
+//      (apply_rotations), exactly as the staged kernels rotate the columns,
+//      and is written back only when a rotation passed.
+// The Gram only decides the rotations: the columns are rotated pair by
+// pair, never multiplied by an accumulated 16 x 16 product, so that small
+// columns keep the accuracy of the one-sided method (a product's rounding,
+// relative to the largest column it mixes in, left graded spectra slower to
+// converge and less accurate).  The Gram is formed afresh from the columns at
+// every launch, so a sweep with no rotation passing was decided on the
+// columns' own dot products, as in the staged kernels.
+template <int NT, int RPT, int W, int SCHED>
+__global__ __launch_bounds__(NT) void bj_gram_kernel(jac_batch bt, int s, int nbb)
+{
+  constexpr int C = 2 * W, NH = NT * RPT / 2, NR = inner_rounds<W, SCHED>();
+  static_assert(W == 8, "one lane per (row pair, column pair): W * W = 64");
+  const jac_prob P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  bj_pair<W> bp;
+  if (!bp.setup(P, s, nbb)) return;
+  extern __shared__ double lds[];
+  double* Xs = lds;              // [C][q]
+  double* Js = Xs + C * q;       // [C][q]
+  double* Gp = Js + C * q;       // [4][C * C]: per-wave partial Grams
+  double* G = Gp + 4 * C * C;    // [C * C]
+  double* rcs = G + C * C;       // [NR][W]: the round's rotations
+  double* rsn = rcs + 15 * W;
+  int* rot_lds = (int*)(rsn + 15 * W);
+  const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+  // Slot tid + k NT (k < RPT): row slot of X below NH, of J from NH on;
+  // both are staged in LDS ([C][q] each) until the rotations are known.
+  {
+    double r[RPT][C];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+    {
+      // Loads issued before the convergence flag is read, so that their
+      // latencies overlap.
+      const int slot = tid + k * NT;
+      const bool isj = slot >= NH;
+      const int row = isj ? slot - NH : slot;
+      const double* src = isj ? P.J : P.X;
+      const int ld = isj ? P.ldj : P.ldx;
+#pragma unroll
+      for (int l = 0; l < C; ++l)
+      {
+        const bool ok = row < q && bp.real(l);
+        const size_t gc = ok ? bp.global_col(l) : 0;
+        r[k][l] = ok ? src[gc * ld + row] : 0.0;
+      }
+    }
+    if (P.flag && P.flag[1]) return;  // converged in an earlier sweep
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+    {
+      const int slot = tid + k * NT;
+      const bool isj = slot >= NH;
+      const int row = isj ? slot - NH : slot;
+      if (row < q)
+      {
+        double* dst = isj ? Js : Xs;
+#pragma unroll
+        for (int l = 0; l < C; ++l) dst[l * q + row] = r[k][l];
+      }
+    }
+  }
+  const double xnorm2 = P.s[0];  // ||X||_F^2 from jac_init_kernel
+  __syncthreads();
+  if (wid < 4)
+  {
+    // A(i, k) = X(r + k, i) in lane i + 16 k and B(k, j) = X(r + k, j) in
+    // lane j + 16 k: the same element; accumulator r of lane l is
+    // G(l / 16 + 4 r, l % 16).  Two accumulators halve the dependent chain.
+    dbl4_t acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    const double* xc = Xs + (lane & 15) * q;
+    const int kk = lane >> 4;
+    for (int r0 = 8 * wid; r0 < q; r0 += 32)
+    {
+      const int ra = r0 + kk, rb = r0 + 4 + kk;
+      const double a = ra < q ? xc[ra] : 0.0;
+      const double b = rb < q ? xc[rb] : 0.0;
+      acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a, a, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(b, b, acc1, 0, 0, 0);
+    }
+    double* gw = Gp + wid * C * C;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gw[(kk + 4 * k) * C + (lane & 15)] = acc0[k] + acc1[k];
+  }
+  __syncthreads();
+  if (wid == 0)
+  {
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+    {
+      const int e = lane + 64 * m;
+      const int k = e / C, l = e % C;
+      const int o = min(k, l) * C + max(k, l);
+      G[e] = ((Gp[o] + Gp[C * C + o]) + Gp[2 * C * C + o]) + Gp[3 * C * C + o];
+    }
+    wave_sync_lds();
+    const double tol = jac_tol(q);
+    const double negl2 = jac_negl2(P, xnorm2);
+    const int pp = lane >> 3, qp = lane & 7;  // row pair, column pair
+    int rot = 0;
+#pragma unroll
+    for (int r2 = 0; r2 < NR; ++r2)
+    {
+      int i0, j0, i1, j1;
+      inner_pair<W, SCHED>(r2, pp, i0, j0);
+      inner_pair<W, SCHED>(r2, qp, i1, j1);
+      const double g00 = G[i0 * C + i1], g01 = G[i0 * C + j1];
+      const double g10 = G[j0 * C + i1], g11 = G[j0 * C + j1];
+      // The diagonal lanes compute their pair's rotation; every lane takes
+      // the rotations of its row and column pairs from them.
+      double cs = 1.0, sn = 0.0, tP = 0.0;
+      if (pp == qp)
+      {
+        if (!(bp.real(i0) && bp.real(j0) && !(g00 <= negl2 || g11 <= negl2) &&
+              jacobi_rotation(g00, g11, g01, tol, cs, sn, tP)))
+        {
+          cs = 1.0;
+          sn = 0.0;
+          tP = 0.0;
+        }
+        rot |= sn != 0.0;
+        rcs[r2 * W + pp] = cs;
+        rsn[r2 * W + pp] = sn;
+      }
+      const double cP = __shfl(cs, 9 * pp), sP = __shfl(sn, 9 * pp);
+      const double cQ = __shfl(cs, 9 * qp), sQ = __shfl(sn, 9 * qp);
+      double n00, n01, n10, n11;
+      {
+        // G' = R_P^T G R_Q with R = [[c, s], [-s, c]], each entry summed as
+        // (t00 + t11) + (t01 + t10) with no contraction, so that the lane
+        // holding the transposed block rounds every entry alike.
+#pragma clang fp contract(off)
+        n00 = ((cP * cQ) * g00 + (sP * sQ) * g11) + ((cP * -sQ) * g01 + (-sP * cQ) * g10);
+        n01 = ((cP * sQ) * g00 + (-sP * cQ) * g11) + ((cP * cQ) * g01 + (-sP * sQ) * g10);
+        n10 = ((sP * cQ) * g00 + (cP * -sQ) * g11) + ((sP * -sQ) * g01 + (cP * cQ) * g10);
+        n11 = ((sP * sQ) * g00 + (cP * cQ) * g11) + ((sP * cQ) * g01 + (cP * sQ) * g10);
+      }
+      if (pp == qp && sP != 0.0)
+      {
+        n00 = fma(-tP, g01, g00);
+        n11 = fma(tP, g01, g11);
+        n01 = 0.0;
+        n10 = 0.0;
+      }
+      G[i0 * C + i1] = n00;
+      G[i0 * C + j1] = n01;
+      G[j0 * C + i1] = n10;
+      G[j0 * C + j1] = n11;
+      wave_sync_lds();
+    }
+    // The vote runs on every lane of the wave (a vote inside a branch
+    // taken by lane 0 alone would see only that lane's pair).
+    const int any_rot = __any(rot);
+    if (lane == 0) *rot_lds = any_rot;
+  }
+  __syncthreads();
+  if (*rot_lds)
+  {
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+    {
+      const int slot = tid + k * NT;
+      const bool isj = slot >= NH;
+      const int row = isj ? slot - NH : slot;
+      if (row < q)
+      {
+        const double* src = isj ? Js : Xs;
+        double r[C];
+#pragma unroll
+        for (int l = 0; l < C; ++l) r[l] = src[l * q + row];
+        apply_rotations<W, SCHED>(r, rcs, rsn);
+        double* dst = isj ? P.J : P.X;
+        const int ld = isj ? P.ldj : P.ldx;
+#pragma unroll
+        for (int l = 0; l < C; ++l)
+          if (bp.real(l)) dst[(size_t)bp.global_col(l) * ld + row] = r[l];
+      }
+    }
+    if (tid == 0 && P.flag) atomicOr(P.flag, 1);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Block Jacobi for q above the staged kernels' limit (the X and J columns of a
 // block pair no longer fit in LDS).  Blocks of kLW = 16 columns, round-robin
@@ -627,7 +845,6 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
 constexpr int kLW = 16;
 constexpr int kLP = 2 * kLW;
 constexpr int kLRC = 256;
-typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
 struct bjl_work
 {
@@ -1079,11 +1296,18 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     // over twice as many workgroups (CUs); each CU is bound by its LDS and
     // VALU throughput on the rotations, not by the round count.
     const bool narrow = !grouped && batch < 32 && max_q > 128 && max_q <= 256;
-    const int W = grouped ? 16 : (narrow ? 8 : ((size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8));
+    // Few problems of q <= 512: rotations decided on the block pair's Gram
+    // matrix (blocks of 8 columns; 2 q row slots of X and J on 256 or 512
+    // threads).
+    const bool gram = !grouped && c->jac_gram && batch < 32 && max_q <= 512 &&
+                      bjg_lds_bytes(max_q) <= (size_t)max_lds;
+    const int gnt = max_q <= 128 ? 256 : (max_q <= 256 ? 512 : 1024);
+    const int W = grouped ? 16 : ((narrow || gram) ? 8 : ((size_t)32 * 16 * max_q <= (size_t)max_lds ? 16 : 8));
     const bool cross = grouped ? g_jac_cross_group : g_jac_cross;
     const int nb = (max_q + W - 1) / W;
     const int nbb = (nb + 1) & ~1;
-    const size_t lds = (grouped || narrow) ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q;
+    const size_t lds = gram ? bjg_lds_bytes(max_q)
+                       : ((grouped || narrow) ? bj_lds_bytes(W, max_q) : 4 * sizeof(double) * (size_t)W * max_q);
     {
       long g = ((long)max_q * max_q + 255) / 256;
       if (g > 1024) g = 1024;
@@ -1102,7 +1326,17 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
           const dim3 grid(nbb / 2, batch);
           prof_scope ps(c, TTA_K_JACOBI_ROUND);
           const int sched = !cross ? kFullSweep : (s == 0 ? kCrossIntra : kCross);
-          if (grouped && max_q <= 128)
+          if (gram)
+            launch_sched(sched, [&](auto sc) {
+              constexpr int SC = decltype(sc)::value;
+              if (gnt == 256)
+                hipLaunchKernelGGL((bj_gram_kernel<256, 1, 8, SC>), grid, dim3(256), lds, st, b, s, nbb);
+              else if (gnt == 512)
+                hipLaunchKernelGGL((bj_gram_kernel<512, 1, 8, SC>), grid, dim3(512), lds, st, b, s, nbb);
+              else
+                hipLaunchKernelGGL((bj_gram_kernel<512, 2, 8, SC>), grid, dim3(512), lds, st, b, s, nbb);
+            });
+          else if (grouped && max_q <= 128)
             launch_sched(sched, [&](auto sc) {
               hipLaunchKernelGGL((bj_group_kernel<16, 8, decltype(sc)::value>), grid, dim3(256), lds, st, b, s, nbb);
             });
