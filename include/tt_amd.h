@@ -451,15 +451,17 @@ int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long*
 int tta_set_graphs(tta_ctx* ctx, int on);
 
 /* ---- Gram-driven block Jacobi (no reference counterpart) ------------------- */
-/* Block-Jacobi SVDs of fewer than 32 problems of 65..512 columns run each
- * round on the block pair's Gram matrix: the pair's 16 columns are staged
- * once, their 16 x 16 Gram formed on the matrix cores, the inner sweep's
- * rotations decided and applied on the Gram alone (the one-sided rule, the
- * same tolerance), and the columns of X and J updated once by the
- * accumulated 16 x 16 rotation.  The Gram is formed afresh from the columns
- * at every launch, so convergence is judged on the columns' own dot
- * products.  The singular values and vectors agree with on = 0 to rounding
- * (different rounding, not bit-identical).  On by default. */
+/* Block-Jacobi SVDs of fewer than 32 problems of 65..512 columns decide
+ * each round's rotations on the block pair's Gram matrix: the pair's 16
+ * columns are staged once, their 16 x 16 Gram formed on the matrix cores,
+ * the inner sweep's rotations decided and computed on the Gram alone (the
+ * one-sided rule, the same tolerance, two-sided updates of the Gram), and
+ * the rows of X and J then rotated pair by pair by the recorded rotations.
+ * The Gram is formed afresh from the columns at every launch, so
+ * convergence is judged on the columns' own dot products.  The singular
+ * values and vectors agree with on = 0 to rounding (not bit-identical).  On
+ * by default; on = 0 runs the staged rounds, which compute every rotation
+ * from the columns' dot products. */
 int tta_set_jacobi_gram(tta_ctx* ctx, int on);
 
 /* ---- autotuner (no reference counterpart) --------------------------------- */

@@ -130,9 +130,9 @@ class Context:
         self.check(self.lib.tta_set_graphs(self.h, 1 if on else 0))
 
     def set_jacobi_gram(self, on: bool) -> None:
-        """Block-Jacobi rounds of few problems (65..512 columns) driven by the
-        block pair's 16 x 16 Gram matrix (on by default; equal results up to
-        rounding)."""
+        """Block-Jacobi rounds of few problems (65..512 columns) whose
+        rotations are decided on the block pair's 16 x 16 Gram matrix (on by
+        default; equal results up to rounding)."""
         self.check(self.lib.tta_set_jacobi_gram(self.h, 1 if on else 0))
 
     def set_speculation(self, on: bool) -> None:
