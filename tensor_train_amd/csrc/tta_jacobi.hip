@@ -1096,20 +1096,46 @@ __global__ void jac_init_kernel(jac_batch bt)
   }
 }
 
-// s[0] = ||X||_F^2 (one workgroup per problem); s is not otherwise used
-// before jac_finish_kernel writes the singular values.
+// Squared column norms of X into s[0..q): one wave per column, four
+// columns per workgroup, the column's loads in flight eight at a time
+// (the loads of one workgroup per problem used to run one dependent column
+// after another).  Each lane sums rows lane, lane + 64, ... in order, as
+// column_norms_sorted does.
+__global__ __launch_bounds__(256) void jac_colnorm2_kernel(jac_batch bt)
+{
+  const jac_prob P = jpick(bt, blockIdx.y);
+  const int q = P.q;
+  const int lane = threadIdx.x & 63;
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (c >= q) return;
+  const double* w = P.X + (size_t)c * P.ldx;
+  double acc = 0.0;
+  for (int t0 = 0; t0 < q; t0 += 64 * 8)
+  {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+    {
+      const int t = t0 + lane + 64 * k;
+      v[k] = t < q ? w[t] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += v[k] * v[k];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) P.s[c] = acc;
+}
+
+// s[0] = ||X||_F^2 from the squared column norms in s (one workgroup per
+// problem; every read precedes the write through block_sum's barrier); s
+// is not otherwise used before jac_finish_kernel writes the singular values.
 template <int NT>
 __global__ __launch_bounds__(NT) void jac_norm_kernel(jac_batch bt)
 {
-  const jac_prob& P = jpick(bt, blockIdx.x);
+  const jac_prob P = jpick(bt, blockIdx.x);
   __shared__ double red[NT / 64];
-  const int q = P.q;
   double part = 0.0;
-  for (long e = threadIdx.x; e < (long)q * q; e += NT)
-  {
-    const double v = P.X[(size_t)(e / q) * P.ldx + (e % q)];
-    part = fma(v, v, part);
-  }
+  for (int c = threadIdx.x; c < P.q; c += NT) part += P.s[c];
   const double t = block_sum<NT>(part, red);
   if (threadIdx.x == 0) P.s[0] = t;
 }
@@ -1129,17 +1155,40 @@ __global__ void jac_sweep_end_kernel(jac_batch bt, int batch, int* done)
   }
 }
 
-// Sorted column norms and truncation rank (one workgroup per problem).
+// Sorted column norms and truncation rank (one workgroup per problem) from
+// the squared column norms in s (jac_colnorm2_kernel): norms descending, a
+// NaN norm (from a breakdown upstream) after every number, ties by index, so
+// perm is always a permutation of 0..q-1 and every s[o] is written.  The
+// rank is decided on the sorted values in LDS.  LDS: 2 q doubles.
 template <int NT>
 __global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
 {
-  const jac_prob& P = jpick(bt, blockIdx.x);
+  const jac_prob P = jpick(bt, blockIdx.x);
   extern __shared__ double nrm[];
-  column_norms_sorted<NT>(P.X, P.ldx, P.q, P.q, nrm, P.s, P.perm);
+  const int q = P.q;
+  double* srt = nrm + q;
+  for (int c = threadIdx.x; c < q; c += NT) nrm[c] = sqrt(P.s[c]);
+  __syncthreads();
+  for (int c = threadIdx.x; c < q; c += NT)
+  {
+    const double v = nrm[c];
+    const double kv = (v == v) ? v : -1.0;
+    int rank = 0;
+    for (int o = 0; o < q; ++o)
+    {
+      const double u = nrm[o];
+      const double ku = (u == u) ? u : -1.0;
+      rank += (ku > kv) || (ku == kv && o < c);
+    }
+    P.perm[rank] = c;
+    P.s[rank] = v;
+    srt[rank] = v;
+  }
+  __syncthreads();
   if (threadIdx.x == 0 && P.rank_out)
   {
     const double delta = P.delta_ptr ? *P.delta_ptr : P.delta;
-    *P.rank_out = truncate_rank(P.s, P.q, delta, P.max_rank);
+    *P.rank_out = truncate_rank(srt, q, delta, P.max_rank);
   }
 }
 }  // namespace
@@ -1191,7 +1240,10 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
   const int batch = b.descs ? b.batch : 1;
   if (batch <= 0 || max_q <= 0) return 0;
   int sweeps_run = -1;
-  if (max_q <= 64)
+  // Few problems of 16 < q <= 64 take the Gram-driven block rounds below
+  // (a whole-problem workgroup runs each of its 63 rounds per sweep as four
+  // dependent pairs per wave).
+  if (max_q <= 64 && !(c->jac_gram && batch < 32 && max_q > 16))
   {
     constexpr int NT = 512;
     const size_t lds = 2 * sizeof(double) * (size_t)max_q * max_q;
@@ -1245,6 +1297,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
         long g = ((long)max_q * max_q + 255) / 256;
         if (g > 1024) g = 1024;
         hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+        hipLaunchKernelGGL(jac_colnorm2_kernel, dim3((max_q + 3) / 4, batch), dim3(256), 0, st, b);
         hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
       }
       const size_t rec0 = c->prof_pending.size();
@@ -1285,7 +1338,8 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       c->mem.put(w.V);
       c->mem.put(w.pflag);
       constexpr int NF = 256;
-      hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);
+      hipLaunchKernelGGL(jac_colnorm2_kernel, dim3((max_q + 3) / 4, batch), dim3(256), 0, st, b);
+      hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), 2 * sizeof(double) * (size_t)max_q, st, b);
       return sweeps_run;
     }
     // Batches of q <= 256 fill the chip with 256-thread workgroups, where the
@@ -1312,6 +1366,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       long g = ((long)max_q * max_q + 255) / 256;
       if (g > 1024) g = 1024;
       hipLaunchKernelGGL(jac_init_kernel, dim3((int)g, batch), dim3(256), 0, st, b);
+      hipLaunchKernelGGL(jac_colnorm2_kernel, dim3((max_q + 3) / 4, batch), dim3(256), 0, st, b);
       hipLaunchKernelGGL(jac_norm_kernel<256>, dim3(batch), dim3(256), 0, st, b);
     }
     const size_t rec0 = c->prof_pending.size();
@@ -1383,7 +1438,8 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
       }
   }
   constexpr int NF = 256;
-  hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), sizeof(double) * (size_t)max_q, st, b);
+  hipLaunchKernelGGL(jac_colnorm2_kernel, dim3((max_q + 3) / 4, batch), dim3(256), 0, st, b);
+  hipLaunchKernelGGL(jac_finish_kernel<NF>, dim3(batch), dim3(NF), 2 * sizeof(double) * (size_t)max_q, st, b);
   return sweeps_run;
 }
 

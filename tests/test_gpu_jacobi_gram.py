@@ -1,4 +1,4 @@
-"""The Gram-driven block-Jacobi rounds (few problems of 65..512 columns,
+"""The Gram-driven block-Jacobi rounds (few problems of 17..512 columns,
 tta_set_jacobi_gram) against LAPACK and against the staged rounds: the
 singular values to backward-stable accuracy (absolute errors ~ u s_max, as
 LAPACK and Eigen's BDCSVD give), orthonormal U, the product reproduced, the
@@ -22,7 +22,7 @@ def _graded(q, decades, zero_from, seed):
     return (U0 * sv) @ V0.T
 
 
-@pytest.mark.parametrize("q,decades", [(72, 6), (130, 4), (256, 12), (300, 8), (512, 4)])
+@pytest.mark.parametrize("q,decades", [(24, 5), (40, 8), (64, 6), (72, 6), (130, 4), (256, 12), (300, 8), (512, 4)])
 def test_gram_rounds_vs_lapack_and_staged(q, decades):
     ctx = T.Context.default()
     M = _graded(q, decades, q - q // 5, q)
@@ -44,8 +44,10 @@ def test_gram_rounds_vs_lapack_and_staged(q, decades):
         keep = s > 1e-12
         assert np.abs((U[:, keep] * s[keep]) @ Vt[keep] - M).max() <= 1e-13, gram
     # The rounds decide the same rotations up to rounding: sweep counts agree
-    # within one.
-    assert abs(out[True][3] - out[False][3]) <= 1
+    # within one (for q <= 64 the staged path is the whole-problem kernel,
+    # whose sweeps are not counted).
+    if q > 64:
+        assert abs(out[True][3] - out[False][3]) <= 1
 
 
 def test_gram_rounds_matvec_round_every_svd_vs_oracle():
