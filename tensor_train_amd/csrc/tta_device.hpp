@@ -216,7 +216,7 @@ __device__ __forceinline__ double rcp_newton(double x)
   return fma(r, fma(-x, r, 1.0), r);
 }
 
-// 1 / sqrt(y) for 1 <= y <= 1e300.
+// 1 / sqrt(y) for 1e-300 <= y <= 1e300 (y r and y r^2 stay in range).
 __device__ __forceinline__ double rsqrt_newton(double y)
 {
   double r = __builtin_amdgcn_rsq(y);
