@@ -451,7 +451,8 @@ int tta_prof_chain(tta_ctx* ctx, long long* blocks, long long* steps, long long*
 int tta_set_graphs(tta_ctx* ctx, int on);
 
 /* ---- Gram-driven block Jacobi (no reference counterpart) ------------------- */
-/* Block-Jacobi SVDs of fewer than 32 problems of 65..512 columns decide
+/* Block-Jacobi SVDs of fewer than 32 problems of 65..512 columns (and of
+ * one problem of 41..512 columns) decide
  * each round's rotations on the block pair's Gram matrix: the pair's 16
  * columns are staged once, their 16 x 16 Gram formed on the matrix cores,
  * the inner sweep's rotations decided and computed on the Gram alone (the

@@ -1266,10 +1266,12 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
   const int batch = b.descs ? b.batch : 1;
   if (batch <= 0 || max_q <= 0) return 0;
   int sweeps_run = -1;
-  // Few problems of 16 < q <= 64 take the Gram-driven block rounds below
-  // (a whole-problem workgroup runs each of its 63 rounds per sweep as four
-  // dependent pairs per wave).
-  if (max_q <= 64 && !(c->jac_gram && batch < 32 && max_q > 16))
+  // A single problem of 40 < q <= 64 takes the Gram-driven block rounds
+  // below (a whole-problem workgroup runs each of its 63 rounds per sweep as
+  // four dependent pairs per wave: 2 ms at q = 64 against about 0.6 ms).
+  // Smaller problems, and batches, stay in the one-launch kernel, whose
+  // sweeps need no host read.
+  if (max_q <= 64 && !(c->jac_gram && batch == 1 && max_q > 40))
   {
     constexpr int NT = 512;
     const size_t lds = 2 * sizeof(double) * (size_t)max_q * max_q;
