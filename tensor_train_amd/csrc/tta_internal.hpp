@@ -105,6 +105,11 @@ struct tta_ctx
   // Block-Jacobi launches (q > 64) since the last tta_prof_enable and the
   // block sweeps they ran (host-driven, so counted whether or not timing is on).
   long long jac_calls = 0, jac_sweeps = 0;
+  // Sweeps the last single-problem block Jacobi needed: the next call reads
+  // the convergence flag from three sweeps before that on (each read drains
+  // the stream; a sweep of a converged problem costs only early-exiting
+  // launches).
+  int jac_sweep_hint = 0;
   // Block-Jacobi problems that had not converged after the sweep limit.
   long long jac_unconverged = 0;
   // SVD-sweep steps of the mixed batched rounding tried by the rank-revealing

@@ -791,11 +791,16 @@ __global__ __launch_bounds__(NT) void bj_gram_kernel(jac_batch bt, int s, int nb
           tP = 0.0;
         }
         rot |= sn != 0.0;
-        rcs[r2 * W + pp] = cs;
-        rsn[r2 * W + pp] = sn;
       }
       const double cP = __shfl(cs, 9 * pp), sP = __shfl(sn, 9 * pp);
       const double cQ = __shfl(cs, 9 * qp), sQ = __shfl(sn, 9 * qp);
+      // Recorded after the shuffles are issued, so that waiting for their
+      // results does not wait for these writes too.
+      if (pp == qp)
+      {
+        rcs[r2 * W + pp] = cs;
+        rsn[r2 * W + pp] = sn;
+      }
       double n00, n01, n10, n11;
       {
         // G' = R_P^T G R_Q with R = [[c, s], [-s, c]], each entry summed as
@@ -1167,7 +1172,7 @@ __global__ __launch_bounds__(NT) void jac_norm_kernel(jac_batch bt)
 }
 
 // After a sweep: flag[1] = (no rotation in the sweep); flag[0] reset;
-// counts converged problems into *done.
+// counts converged problems into *done (zeroed before by jac_zero_int_kernel).
 __global__ void jac_sweep_end_kernel(jac_batch bt, int batch, int* done)
 {
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < batch; i += gridDim.x * blockDim.x)
@@ -1179,6 +1184,27 @@ __global__ void jac_sweep_end_kernel(jac_batch bt, int batch, int* done)
     if (P.flag[1]) atomicAdd(done, 1);
     if (P.sweeps && !P.flag[1]) *P.sweeps += 1;
   }
+}
+
+// The same for batches of at most 256 problems in one workgroup, which
+// writes the count itself (no separate zeroing launch).
+__global__ __launch_bounds__(256) void jac_sweep_end_one_kernel(jac_batch bt, int batch, int* done)
+{
+  const int i = threadIdx.x;
+  int conv = 0;
+  if (i < batch)
+  {
+    const jac_prob& P = jpick(bt, i);
+    if (P.flag)
+    {
+      if (!P.flag[1] && P.flag[0] == 0) P.flag[1] = 1;
+      P.flag[0] = 0;
+      conv = P.flag[1] != 0;
+      if (P.sweeps && !conv) *P.sweeps += 1;
+    }
+  }
+  const int n = __syncthreads_count(conv);
+  if (i == 0) *done = n;
 }
 
 // Sorted column norms and truncation rank (one workgroup per problem) from
@@ -1216,6 +1242,17 @@ __global__ __launch_bounds__(NT) void jac_finish_kernel(jac_batch bt)
     const double delta = P.delta_ptr ? *P.delta_ptr : P.delta;
     *P.rank_out = truncate_rank(srt, q, delta, P.max_rank);
   }
+}
+// End of a block sweep: flags updated, converged problems counted into done.
+void launch_sweep_end(const jac_batch& b, int batch, int* done_dev, hipStream_t st)
+{
+  if (batch <= 256)
+  {
+    hipLaunchKernelGGL(jac_sweep_end_one_kernel, dim3(1), dim3(256), 0, st, b, batch, done_dev);
+    return;
+  }
+  hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
+  hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
 }
 }  // namespace
 
@@ -1343,8 +1380,7 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
             hipLaunchKernelGGL(bjl_apply_kernel, dim3(2 * w.nch, w.pairs, batch), dim3(256), 0, st, b, w, r, nbb,
                                w.nch);
           }
-          hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
-          hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+          launch_sweep_end(b, batch, done_dev, st);
         });
         if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
         {
@@ -1400,6 +1436,11 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
     const size_t rec0 = c->prof_pending.size();
     sweep_graph sg;
     const bool use_graph = c->graphs && !c->prof;
+    // One problem whose sweep counter sits next to the done count: the flag
+    // is read from the hint on, and the counter with it gives the sweeps the
+    // problem needed, whatever sweeps ran after its convergence.
+    const bool hinted = batch == 1 && !b.descs && b.one.sweeps && b.one.sweeps == done_dev + 1;
+    const int check_from = hinted ? std::max(2, c->jac_sweep_hint - 3) : 2;
     for (int sweep = 0; sweep < kMaxBlockSweeps; ++sweep)
     {
       sweeps_run = sweep + 1;
@@ -1441,20 +1482,25 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
           else
             hipLaunchKernelGGL((bj_round_kernel<NT, 8, 10>), grid, dim3(NT), lds, st, b, s, nbb, 1, cross ? 1 : 0);
         }
-        hipLaunchKernelGGL(jac_zero_int_kernel, dim3(1), dim3(1), 0, st, done_dev);
-        hipLaunchKernelGGL(jac_sweep_end_kernel, dim3((batch + 255) / 256), dim3(256), 0, st, b, batch, done_dev);
+        launch_sweep_end(b, batch, done_dev, st);
       });
-      if (sweep >= 2 || sweep == kMaxBlockSweeps - 1)
+      if (sweep >= check_from || sweep == kMaxBlockSweeps - 1)
       {
-        (void)hipMemcpyAsync(done_host, done_dev, sizeof(int), hipMemcpyDeviceToHost, st);
+        (void)hipMemcpyAsync(done_host, done_dev, (hinted ? 2 : 1) * sizeof(int), hipMemcpyDeviceToHost, st);
         (void)hipStreamSynchronize(st);
-        if (*done_host >= batch) break;
+        if (*done_host >= batch)
+        {
+          // The sweeps with a rotation, and the one that found none.
+          if (hinted) sweeps_run = done_host[1] + 1;
+          break;
+        }
         // Problems still rotating after the last sweep: their W = X J is
         // orthogonal only to the last sweep's rotation size, not to the
         // tolerance; counted for tta_prof_jacobi_unconverged.
         if (sweep == kMaxBlockSweeps - 1) c->jac_unconverged += batch - *done_host;
       }
     }
+    if (hinted) c->jac_sweep_hint = sweeps_run;
     // Equal share of the call's credit for every round launch recorded.
     size_t nrec = 0;
     for (size_t r = rec0; r < c->prof_pending.size(); ++r) nrec += c->prof_pending[r].cls == TTA_K_JACOBI_ROUND;
