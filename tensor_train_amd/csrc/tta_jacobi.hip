@@ -326,27 +326,44 @@ struct bj_pair
   }
   __device__ bool real(int l) const { return (l % W) < (l < W ? ncol0 : ncol1); }
   __device__ int global_col(int l) const { return (l < W ? cbeg0 : cbeg1) + l % W; }
-  template <int NT>
+  // X's columns into LDS ([2W][q]) and row t of J's into jr, for q <= R
+  // (a power of two dividing 2 W NT): thread t takes row t % R of
+  // 2 W R / NT columns, with every load issued before any LDS store (an
+  // element-strided loop with a division by q per element serialized them).
+  template <int NT, int R>
   __device__ void load(const jac_prob& P, double* Xs, double (&jr)[2 * W]) const
   {
+    static_assert(R <= NT && NT % R == 0 && (2 * W * R) % NT == 0, "row slots");
+    constexpr int CPT = 2 * W * R / NT;  // columns per thread
     const int q = P.q;
-    for (int e = threadIdx.x; e < 2 * W * q; e += NT)
+    const int r = threadIdx.x % R, l0 = (threadIdx.x / R) * CPT;
+    double v[CPT];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
     {
-      const int l = e / q, r = e % q;
-      Xs[e] = real(l) ? P.X[(size_t)global_col(l) * P.ldx + r] : 0.0;
+      const bool ok = r < q && real(l0 + k);
+      v[k] = ok ? P.X[(size_t)global_col(l0 + k) * P.ldx + r] : 0.0;
     }
     const int t = threadIdx.x;
 #pragma unroll
     for (int l = 0; l < 2 * W; ++l) jr[l] = (t < q && real(l)) ? P.J[(size_t)global_col(l) * P.ldj + t] : 0.0;
+    if (r < q)
+    {
+#pragma unroll
+      for (int k = 0; k < CPT; ++k) Xs[(l0 + k) * q + r] = v[k];
+    }
   }
-  template <int NT>
+  template <int NT, int R>
   __device__ void store(const jac_prob& P, const double* Xs, const double (&jr)[2 * W]) const
   {
+    constexpr int CPT = 2 * W * R / NT;
     const int q = P.q;
-    for (int e = threadIdx.x; e < 2 * W * q; e += NT)
+    const int r = threadIdx.x % R, l0 = (threadIdx.x / R) * CPT;
+    if (r < q)
     {
-      const int l = e / q, r = e % q;
-      if (real(l)) P.X[(size_t)global_col(l) * P.ldx + r] = Xs[e];
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (real(l0 + k)) P.X[(size_t)global_col(l0 + k) * P.ldx + r] = Xs[(l0 + k) * q + r];
     }
     const int t = threadIdx.x;
     if (t < q)
@@ -464,7 +481,7 @@ __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int n
   double* rcs = lds + 2 * W * q;
   double* rsn = rcs + (2 * W - 1) * W;
   double jr[2 * W];
-  bp.template load<NT>(P, Xs, jr);
+  bp.template load<NT, 64 * PL>(P, Xs, jr);
   __syncthreads();
   const int tid = threadIdx.x, lane = tid & 63, pr = tid >> 6;
   const double tol = jac_tol(q);
@@ -518,7 +535,7 @@ __global__ __launch_bounds__(NT) void bj_defer_kernel(jac_batch bt, int s, int n
     __syncthreads();
   }
   if (tid < q) apply_rotations<W, SCHED>(jr, rcs, rsn);
-  bp.template store<NT>(P, Xs, jr);
+  bp.template store<NT, 64 * PL>(P, Xs, jr);
   if (rot && lane == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
@@ -545,7 +562,7 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
   double* rcs = lds + 2 * W * q;
   double* rsn = rcs + (2 * W - 1) * W;
   double jr[2 * W];
-  bp.template load<NT>(P, Xs, jr);
+  bp.template load<NT, 16 * PL>(P, Xs, jr);
   __syncthreads();
   const int tid = threadIdx.x;
   const double tol = jac_tol(q);
@@ -613,7 +630,7 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
     __syncthreads();
   }
   if (tid < q) apply_rotations<W, SCHED>(jr, rcs, rsn);
-  bp.template store<NT>(P, Xs, jr);
+  bp.template store<NT, 16 * PL>(P, Xs, jr);
   if (rot && li16 == 0 && P.flag) atomicOr(P.flag, 1);
 }
 
