@@ -225,22 +225,43 @@ __device__ __forceinline__ double rsqrt_newton(double y)
 }
 
 // Jacobi rotation that orthogonalises columns x_i, x_j from a = |x_i|^2,
-// b = |x_j|^2, g = x_i . x_j (Rutishauser's formulas).  Returns false (no
-// rotation) when |g| <= tol |x_i| |x_j|, when a squared norm underflowed to
-// zero (the column is negligible and its products would never settle), when
-// g is not finite, or when the angle rounds to zero; so every rotation
-// reported changes the columns.  For |zeta| > 1e150 the tangent takes its
-// asymptotic form 1 / (2 zeta), where zeta^2 would overflow.  tan_out
-// receives the tangent t (sn = cs t) of a rotation reported.
-// The rotation sits on the critical path of every inner Jacobi round, so its
-// divisions and square roots are rcp_newton / rsqrt_newton (the rotation
-// only has to be accurate: cs^2 + sn^2 = 1 to rounding), and the threshold
-// compares squares, for squared norms within 1e+-140 (a b can neither
-// overflow nor underflow); other scales keep the correctly rounded form.
+// b = |x_j|^2, g = x_i . x_j: the rotation x_i' = cs x_i - sn x_j,
+// x_j' = sn x_i + cs x_j with tangent t the smaller root of
+// t^2 + 2 zeta t - 1 = 0, zeta = (b - a) / (2 g) (Rutishauser's rotation).
+// Returns false (no rotation) when |g| <= tol |x_i| |x_j|, when a squared
+// norm underflowed to zero (the column is negligible and its products would
+// never settle), when g is not finite, or when the angle rounds to zero; so
+// every rotation reported changes the columns.  tan_out receives t.
+// The rotation sits on the critical path of every inner Jacobi round:
+//   * squared norms within 1e+-100: two reciprocal square roots.  With
+//     d = b - a and h = sqrt(d^2 + 4 g^2) (in [4e-230, 5e200] there),
+//     cos 2theta = |d| / h, cs^2 = x = (1 + |d| / h) / 2,
+//     sn = sign(d) g / (h cs) and t = sn / cs; cs^2 + sn^2 = 1 to rounding
+//     (at d = 0 the angle is 45 degrees with t = sign(g));
+//   * within 1e+-140: t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)),
+//     cs = 1 / sqrt(1 + t^2), by the hardware reciprocal and reciprocal
+//     square root with Newton steps, the threshold on squares (a b can
+//     neither overflow nor underflow);
+//   * other scales: the same formulas correctly rounded; for
+//     |zeta| > 1e150 the tangent takes its asymptotic form 1 / (2 zeta),
+//     where zeta^2 would overflow.
 __device__ __forceinline__ bool jacobi_rotation(double a, double b, double g, double tol, double& cs, double& sn,
                                                 double& tan_out)
 {
   if (a == 0.0 || b == 0.0) return false;
+  if (a > 1.0e-100 && a < 1.0e100 && b > 1.0e-100 && b < 1.0e100)
+  {
+    if (!(g * g > (tol * tol) * (a * b))) return false;
+    const double d = b - a;
+    const double r = rsqrt_newton(fma(d, d, 4.0 * (g * g)));  // 1 / h
+    const double x = fma(0.5 * fabs(d), r, 0.5);                // cs^2
+    const double ri = rsqrt_newton(x);                         // 1 / cs
+    const double gs = d >= 0.0 ? g : -g;
+    cs = x * ri;
+    sn = (gs * r) * ri;
+    tan_out = sn * ri;
+    return sn != 0.0;
+  }
   if (!(a > 1.0e-140 && a < 1.0e140 && b > 1.0e-140 && b < 1.0e140))
   {
     // Extreme scales: the correctly rounded operations throughout.

@@ -636,32 +636,6 @@ __global__ __launch_bounds__(W * 16) void bj_group_kernel(jac_batch bt, int s, i
 
 typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
-// The rotation of jacobi_rotation (same threshold, same angle up to
-// rounding) by two reciprocal square roots instead of four reciprocals and
-// reciprocal square roots in sequence, for squared norms within 1e+-100
-// (d^2 + 4 g^2 then lies in [4e-230, 5e200]); other scales take
-// jacobi_rotation.  With d = b - a, h = sqrt(d^2 + 4 g^2): cos 2theta =
-// |d| / h, cs^2 = x = (1 + |d| / h) / 2, sn = sign(d) g / (h cs) and
-// t = sn / cs = sign(d) g / (h x); cs^2 + sn^2 = x + g^2 / (h^2 x) = 1.  At
-// d = 0 the angle is 45 degrees with t = sign(g) (jacobi_rotation takes
-// t = 1; both zero the pair's dot product).  The diagonal lanes of the Gram
-// rounds compute one rotation per inner round on their critical path.
-__device__ __forceinline__ bool gram_rotation(double a, double b, double g, double tol, double& cs, double& sn,
-                                              double& t)
-{
-  if (!(a > 1.0e-100 && a < 1.0e100 && b > 1.0e-100 && b < 1.0e100)) return jacobi_rotation(a, b, g, tol, cs, sn, t);
-  if (!(g * g > (tol * tol) * (a * b))) return false;
-  const double d = b - a;
-  const double r = rsqrt_newton(fma(d, d, 4.0 * (g * g)));  // 1 / h
-  const double x = fma(0.5 * fabs(d), r, 0.5);                // cs^2
-  const double ri = rsqrt_newton(x);                         // 1 / cs
-  const double gs = d >= 0.0 ? g : -g;
-  cs = x * ri;
-  sn = (gs * r) * ri;
-  t = sn * ri;
-  return sn != 0.0;
-}
-
 // LDS bytes of bj_gram_kernel<*, *, 8, *> for problems of up to q columns.
 constexpr size_t bjg_lds_bytes(int q) { return sizeof(double) * (32 * (size_t)q + 5 * 256 + 2 * 15 * 8 + 8); }
 
@@ -801,7 +775,7 @@ __global__ __launch_bounds__(NT) void bj_gram_kernel(jac_batch bt, int s, int nb
       if (pp == qp)
       {
         if (!(bp.real(i0) && bp.real(j0) && !(g00 <= negl2 || g11 <= negl2) &&
-              gram_rotation(g00, g11, g01, tol, cs, sn, tP)))
+              jacobi_rotation(g00, g11, g01, tol, cs, sn, tP)))
         {
           cs = 1.0;
           sn = 0.0;
