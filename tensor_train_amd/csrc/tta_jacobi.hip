@@ -201,6 +201,83 @@ __device__ __forceinline__ void apply_rotations(double (&jr)[2 * W], const doubl
     }
 }
 
+// apply_rotations for the cross schedules with runtime loops over the
+// inner rounds and compile-time register indices: the row's registers are
+// renamed instead of the pairing.  Cross rounds pair column a of block i
+// with column (a + h) mod W of block j, so block j's registers shift down by
+// one after every round and the pairs stay (a, W + a).  Intra rounds follow
+// the circle method (pair 0: W - 1 with s; pair p: (s + p) and (s - p)
+// mod (W - 1)), so positions 0..W-2 of each block shift down by one after
+// every round and the pairs stay (W - 1, 0) and (p, W - 1 - p), the smaller
+// original index taking the first role of the rotation.  W shifts of a
+// W-cycle (W - 1 of a (W - 1)-cycle) restore the order.  The fully unrolled
+// form kept hundreds of values live (every round's registers and rotations
+// scheduled together).
+__device__ __forceinline__ void rot2(double& x, double& y, double c, double sv)
+{
+  const double a = x, b = y;
+  x = c * a - sv * b;
+  y = sv * a + c * b;
+}
+
+template <int W, int SCHED>
+__device__ __forceinline__ void apply_rotations_rt(double (&jr)[2 * W], const double* rcs, const double* rsn)
+{
+  static_assert(SCHED == kCrossIntra || SCHED == kCross, "cross schedules");
+  constexpr int M = W - 1;
+  int r2 = 0;
+  if constexpr (SCHED == kCrossIntra)
+  {
+#pragma unroll 1
+    for (int st = 0; st < M; ++st, ++r2)
+    {
+#pragma unroll
+      for (int blk = 0; blk < 2; ++blk)
+      {
+        double* v = jr + blk * W;
+        const int pr0 = blk * (W / 2);
+        // Pair 0: original (M, st), st < M: st first.
+        rot2(v[0], v[M], rcs[r2 * W + pr0], rsn[r2 * W + pr0]);
+#pragma unroll
+        for (int p = 1; p < W / 2; ++p)
+        {
+          const int x = (st + p) % M, y = (st - p + M) % M;  // originals at p, M - p
+          const bool first = x < y;
+          double u = first ? v[p] : v[M - p];
+          double w = first ? v[M - p] : v[p];
+          rot2(u, w, rcs[r2 * W + pr0 + p], rsn[r2 * W + pr0 + p]);
+          v[p] = first ? u : w;
+          v[M - p] = first ? w : u;
+        }
+        const double t0 = v[0];
+#pragma unroll
+        for (int k = 0; k < M - 1; ++k) v[k] = v[k + 1];
+        v[M - 1] = t0;
+      }
+    }
+  }
+#pragma unroll 1
+  for (int h = 0; h < W; ++h, ++r2)
+  {
+#pragma unroll
+    for (int a = 0; a < W; ++a) rot2(jr[a], jr[W + a], rcs[r2 * W + a], rsn[r2 * W + a]);
+    const double t0 = jr[W];
+#pragma unroll
+    for (int k = 0; k < W - 1; ++k) jr[W + k] = jr[W + k + 1];
+    jr[2 * W - 1] = t0;
+  }
+}
+
+// The same for the two cross schedules only.
+template <class F>
+void launch_cross_sched(int sched, F&& f)
+{
+  if (sched == kCrossIntra)
+    f(std::integral_constant<int, kCrossIntra>{});
+  else
+    f(std::integral_constant<int, kCross>{});
+}
+
 // Calls f with std::integral_constant<int, sched> for the runtime schedule.
 template <class F>
 void launch_sched(int sched, F&& f)
@@ -836,7 +913,7 @@ __global__ __launch_bounds__(NT) void bj_gram_kernel(jac_batch bt, int s, int nb
         double r[C];
 #pragma unroll
         for (int l = 0; l < C; ++l) r[l] = src[l * q + row];
-        apply_rotations<W, SCHED>(r, rcs, rsn);
+        apply_rotations_rt<W, SCHED>(r, rcs, rsn);
         double* dst = isj ? P.J : P.X;
         const int ld = isj ? P.ldj : P.ldx;
 #pragma unroll
@@ -1441,8 +1518,10 @@ int launch_jacobi_svd(const jac_batch& b, int max_q, int* done_dev, int* done_ho
           const dim3 grid(nbb / 2, batch);
           prof_scope ps(c, TTA_K_JACOBI_ROUND);
           const int sched = !cross ? kFullSweep : (s == 0 ? kCrossIntra : kCross);
+          // The Gram rounds always take the cross schedules.
+          const int gsched = s == 0 ? kCrossIntra : kCross;
           if (gram)
-            launch_sched(sched, [&](auto sc) {
+            launch_cross_sched(gsched, [&](auto sc) {
               constexpr int SC = decltype(sc)::value;
               if (gnt == 256)
                 hipLaunchKernelGGL((bj_gram_kernel<256, 1, 8, SC>), grid, dim3(256), lds, st, b, s, nbb);
