@@ -517,6 +517,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   std::vector<cp_desc> cp;
   std::vector<sq_desc> sq;
   int* p2 = S.alloc<int>((size_t)nc);
+  int* skip2 = S.alloc<int>((size_t)nc);
   double* en = S.alloc<double>((size_t)nc * 2);
   int mx_p = 0;
   long mx_ga = 0, mx_cp = 0;
@@ -564,6 +565,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     d.L = S.alloc<double>((size_t)p * p);
     d.Rinv = Ri2;
     d.ldr = p;
+    d.skip = skip2 + b;
     pc2.push_back(d);
     // U = Q1 R2^-1 in the caller's layout.
     g = gemm_desc();
@@ -619,8 +621,13 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     launch_gemm_grouped(S.upload(gg2), nc, mx_p, mx_p, c->stream);
   }
   {
+    // Second pass: one Newton-Schulz step where Q1 is already orthonormal to
+    // 1e-8 / p (rank-deficient sums: Q1's columns are well conditioned),
+    // the Cholesky of G2 for the rest.
     prof_scope ps(c, TTA_K_CHOL);
-    launch_pchol_batch(S.upload(pc2), nc, mx_p, S.upload(largest_first(pc2)), queue + 1, c->stream);
+    const pchol_desc* pc2_dev = S.upload(pc2);
+    launch_ns_pass2(pc2_dev, nc, skip2, c->stream);
+    launch_pchol_batch(pc2_dev, nc, mx_p, S.upload(largest_first(pc2)), queue + 1, c->stream);
   }
   {
     credit w;

@@ -234,6 +234,9 @@ struct pchol_desc
   double* Rinv = nullptr;
   int ldr = 0;
   double* tr_out = nullptr;  // optional: trace of the Schur complement left at the stop
+  // Optional: the problem is skipped when *skip != 0 (its outputs were
+  // written by launch_ns_pass2).
+  const int* skip = nullptr;
 };
 constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
 size_t pchol_lds_bytes(int q);
@@ -241,6 +244,13 @@ size_t pchol_lds_bytes(int q);
 // the problems in the order they are taken (largest first); `queue` is one
 // device int of scratch (zeroed by the launch).
 void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st);
+// Second pass of CholQR2 by one Newton-Schulz step where it suffices: for
+// each problem (descriptor of the unpivoted pass, device array), with
+// G2 = Q1^T Q1 = I + E2 in d.G: when p max|E2| <= 1e-8, Rinv = (3 I - G2) / 2
+// (row-major p x p, so that Q1 Rinv = Q1 (I - E2 / 2) is orthonormal to
+// O(E2^2) <= 1e-16 plus rounding), *p_out = p and *skip = 1; otherwise
+// *skip = 0 and the pivoted-Cholesky launch factors it.
+void launch_ns_pass2(const pchol_desc* ds, int count, int* skip, hipStream_t st);
 
 void launch_gemm(const gemm_desc& g, int batch, hipStream_t st);
 // Split-K form of a single GEMM for shapes with few output tiles:

@@ -89,6 +89,7 @@ __device__ __forceinline__ void unpack(int t, int& i, int& l)
 template <int NT>
 __device__ void pchol_one(const pchol_desc& d)
 {
+  if (d.skip && *d.skip) return;  // second pass taken by the Newton-Schulz step
   const int q = d.q;
   const int tid = threadIdx.x;
   extern __shared__ double sm[];
@@ -283,7 +284,56 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds, i
 }
 
 __global__ void zero_int_kernel(int* p) { *p = 0; }
+
+__device__ __forceinline__ double wave_max_all(double v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One workgroup per problem: max |G2 - I|, then either Rinv = (3 I - G2) / 2
+// with the problem marked done, or the problem left to the Cholesky.
+template <int NT>
+__global__ __launch_bounds__(NT) void ns_pass2_kernel(const pchol_desc* ds, int* skip)
+{
+  const pchol_desc d = ds[blockIdx.x];
+  const int p = d.q;
+  __shared__ double red[NT / 64];
+  double mx = 0.0;
+  for (int e = threadIdx.x; e < p * p; e += NT)
+  {
+    const int i = e / p, j = e % p;
+    const double v = d.G[(size_t)i * d.ldg + j] - (i == j ? 1.0 : 0.0);
+    mx = fmax(mx, fabs(v));  // a NaN leaves mx, and fails the test below
+    if (v != v) mx = 1.0e300;
+  }
+  mx = wave_max_all(mx);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  double m = 0.0;
+#pragma unroll
+  for (int w = 0; w < NT / 64; ++w) m = fmax(m, red[w]);
+  const bool ns = p > 0 && (double)p * m <= 1.0e-8;
+  if (ns)
+    for (int e = threadIdx.x; e < p * p; e += NT)
+    {
+      const int i = e / p, j = e % p;
+      d.Rinv[(size_t)i * d.ldr + j] = 0.5 * ((i == j ? 3.0 : 0.0) - d.G[(size_t)i * d.ldg + j]);
+    }
+  if (threadIdx.x == 0)
+  {
+    skip[blockIdx.x] = ns ? 1 : 0;
+    if (ns) *d.p_out = p;
+  }
+}
 }  // namespace
+
+void launch_ns_pass2(const pchol_desc* ds, int count, int* skip, hipStream_t st)
+{
+  if (count <= 0) return;
+  hipLaunchKernelGGL(ns_pass2_kernel<256>, dim3(count), dim3(256), 0, st, ds, skip);
+}
 
 size_t pchol_lds_bytes(int q)
 {
