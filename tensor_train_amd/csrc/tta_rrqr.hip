@@ -96,7 +96,8 @@ __device__ void pchol_one(const pchol_desc& d)
   const int npk = q * (q + 1) / 2;
   double* S = sm;            // packed lower triangle, by position
   double* lv = S + npk;      // the current column of the factor, by position
-  double* red = lv + q;
+  double* lv2 = lv + q;      // the second column of a pivot pair (pivoted form)
+  double* red = lv2 + q;
   int* ord = reinterpret_cast<int*>(red + 3 * (NT / 64));  // position -> original index
   if (q <= 0)
   {
@@ -123,11 +124,47 @@ __device__ void pchol_one(const pchol_desc& d)
   int p = 0;
   bool broke = false;
   double tr_left = 0.0;
-  for (int j = 0; j < q; ++j)
-  {
-    // Trace and largest diagonal entry of the trailing Schur complement.
-    double tr = 0.0, mv = -1.0;
-    int mi = 0x7fffffff;
+  // Symmetric swap of positions j and pj (pj > j) in the trailing part of S
+  // (entries of positions above j and the diagonal), the permutation and,
+  // when given, a column buffer indexed by position.
+  auto swap_pos = [&](int j, int pj, double* col) {
+    for (int l = j + 1 + tid; l < q; l += NT)
+    {
+      if (l < pj)
+      {
+        const double x = S[pk(l, j)];
+        S[pk(l, j)] = S[pk(pj, l)];
+        S[pk(pj, l)] = x;
+      }
+      else if (l > pj)
+      {
+        const double x = S[pk(l, j)];
+        S[pk(l, j)] = S[pk(l, pj)];
+        S[pk(l, pj)] = x;
+      }
+    }
+    if (tid == 0)
+    {
+      const double x = S[pk(j, j)];
+      S[pk(j, j)] = S[pk(pj, pj)];
+      S[pk(pj, pj)] = x;
+      const int o = ord[j];
+      ord[j] = ord[pj];
+      ord[pj] = o;
+      if (col)
+      {
+        const double y = col[j];
+        col[j] = col[pj];
+        col[pj] = y;
+      }
+    }
+  };
+  // Trace and largest diagonal entry of the trailing Schur complement from
+  // position j on (every thread returns them).
+  auto diag_scan = [&](int j, double& tr, double& mv, int& mi) {
+    tr = 0.0;
+    mv = -1.0;
+    mi = 0x7fffffff;
     for (int i = j + tid; i < q; i += NT)
     {
       const double v = S[pk(i, i)];
@@ -139,83 +176,182 @@ __device__ void pchol_one(const pchol_desc& d)
       }
     }
     block_sum_argmax<NT>(tr, mv, mi, red);
-    tr_left = tr;
-    int pj = j;
-    if (d.pivot)
+  };
+  if (d.pivot)
+  {
+    // Pivots in pairs: the first column updates only the diagonal; the
+    // second pivot is chosen on that updated diagonal, its column taken from
+    // the raw entries minus the first column's term; then one pass over the
+    // trailing triangle applies both rank-1 updates (half the LDS
+    // read-modify-writes of one pass per pivot).  The stop and the pivot
+    // choices are those of the one-at-a-time form.
+    for (int j = 0; j < q; j += 2)
     {
+      double tr, mv;
+      int mi;
+      diag_scan(j, tr, mv, mi);
+      tr_left = tr;
       if (tr <= stop2 || !(mv > floor_piv)) break;
-      pj = mi;
-    }
-    else
-    {
-      mv = S[pk(j, j)];
-      if (!(mv > 0.0))
+      if (mi != j)
       {
-        broke = true;
-        break;
+        swap_pos(j, mi, nullptr);
+        __syncthreads();
       }
-    }
-    // Symmetric swap of positions j and pj (pj > j) in the trailing part.
-    if (pj != j)
-    {
-      for (int l = j + 1 + tid; l < q; l += NT)
       {
-        if (l < pj)
+        const double dj = sqrt(mv), inv = 1.0 / dj;
+        for (int i = j + tid; i < q; i += NT)
         {
-          const double x = S[pk(l, j)];
-          S[pk(l, j)] = S[pk(pj, l)];
-          S[pk(pj, l)] = x;
-        }
-        else if (l > pj)
-        {
-          const double x = S[pk(l, j)];
-          S[pk(l, j)] = S[pk(l, pj)];
-          S[pk(l, pj)] = x;
+          const double li = i == j ? dj : S[pk(i, j)] * inv;
+          lv[i] = li;
+          d.L[(size_t)j * q + ord[i]] = li;
+          if (i > j) S[pk(i, i)] = fma(-li, li, S[pk(i, i)]);
         }
       }
-      if (tid == 0)
+      __syncthreads();
+      p = j + 1;
+      const int j2 = j + 1;
+      if (j2 >= q) break;
+      diag_scan(j2, tr, mv, mi);
+      tr_left = tr;
+      if (tr <= stop2 || !(mv > floor_piv)) break;
+      if (mi != j2)
       {
-        const double x = S[pk(j, j)];
-        S[pk(j, j)] = S[pk(pj, pj)];
-        S[pk(pj, pj)] = x;
-        const int o = ord[j];
-        ord[j] = ord[pj];
-        ord[pj] = o;
+        swap_pos(j2, mi, lv);
+        __syncthreads();
+      }
+      {
+        const double dj = sqrt(mv), inv = 1.0 / dj;
+        const double a = lv[j2];
+        for (int i = j2 + tid; i < q; i += NT)
+        {
+          const double li = i == j2 ? dj : fma(-lv[i], a, S[pk(i, j2)]) * inv;
+          lv2[i] = li;
+          d.L[(size_t)j2 * q + ord[i]] = li;
+          if (i > j2) S[pk(i, i)] = fma(-li, li, S[pk(i, i)]);
+        }
+      }
+      __syncthreads();
+      p = j2 + 1;
+      // Off-diagonal trailing entries S(i, l), j2 < l < i < q, spread evenly
+      // over the block by the packed index of the strictly lower triangle.
+      const int nt = q - j2 - 1;
+      const int cnt = nt * (nt - 1) / 2;
+      if (tid < cnt)
+      {
+        int ii, ll;
+        unpack(tid, ii, ll);
+        for (int t = tid; t < cnt; t += NT)
+        {
+          const int i = j2 + 2 + ii, l = j2 + 1 + ll;
+          double* e = S + pk(i, l);
+          *e = fma(-lv2[i], lv2[l], fma(-lv[i], lv[l], *e));
+          ll += NT;
+          while (ll > ii)
+          {
+            ll -= ii + 1;
+            ++ii;
+          }
+        }
       }
       __syncthreads();
     }
-    const double dj = sqrt(mv);
-    const double inv = 1.0 / dj;
-    for (int i = j + tid; i < q; i += NT)
+  }
+  else
+  {
+    for (int j = 0; j < q; ++j)
     {
-      const double li = i == j ? dj : S[pk(i, j)] * inv;
-      lv[i] = li;
-      d.L[(size_t)j * q + ord[i]] = li;
-    }
-    __syncthreads();
-    // Trailing update S(i, l) -= lv_i lv_l, j < l <= i < q, spread evenly
-    // over the block by the packed index of the trailing triangle.
-    const int nt = q - j - 1;
-    const int cnt = nt * (nt + 1) / 2;
-    if (tid < cnt)
-    {
-      int ii, ll;
-      unpack(tid, ii, ll);
-      for (int t = tid; t < cnt; t += NT)
+      // Trace and largest diagonal entry of the trailing Schur complement.
+      double tr = 0.0, mv = -1.0;
+      int mi = 0x7fffffff;
+      for (int i = j + tid; i < q; i += NT)
       {
-        const int i = j + 1 + ii, l = j + 1 + ll;
-        double* e = S + pk(i, l);
-        *e = fma(-lv[i], lv[l], *e);
-        ll += NT;
-        while (ll > ii)
+        const double v = S[pk(i, i)];
+        tr += v > 0.0 ? v : 0.0;
+        if (v > mv)
         {
-          ll -= ii + 1;
-          ++ii;
+          mv = v;
+          mi = i;
         }
       }
+      block_sum_argmax<NT>(tr, mv, mi, red);
+      tr_left = tr;
+      int pj = j;
+      if (d.pivot)
+      {
+        if (tr <= stop2 || !(mv > floor_piv)) break;
+        pj = mi;
+      }
+      else
+      {
+        mv = S[pk(j, j)];
+        if (!(mv > 0.0))
+        {
+          broke = true;
+          break;
+        }
+      }
+      // Symmetric swap of positions j and pj (pj > j) in the trailing part.
+      if (pj != j)
+      {
+        for (int l = j + 1 + tid; l < q; l += NT)
+        {
+          if (l < pj)
+          {
+            const double x = S[pk(l, j)];
+            S[pk(l, j)] = S[pk(pj, l)];
+            S[pk(pj, l)] = x;
+          }
+          else if (l > pj)
+          {
+            const double x = S[pk(l, j)];
+            S[pk(l, j)] = S[pk(l, pj)];
+            S[pk(l, pj)] = x;
+          }
+        }
+        if (tid == 0)
+        {
+          const double x = S[pk(j, j)];
+          S[pk(j, j)] = S[pk(pj, pj)];
+          S[pk(pj, pj)] = x;
+          const int o = ord[j];
+          ord[j] = ord[pj];
+          ord[pj] = o;
+        }
+        __syncthreads();
+      }
+      const double dj = sqrt(mv);
+      const double inv = 1.0 / dj;
+      for (int i = j + tid; i < q; i += NT)
+      {
+        const double li = i == j ? dj : S[pk(i, j)] * inv;
+        lv[i] = li;
+        d.L[(size_t)j * q + ord[i]] = li;
+      }
+      __syncthreads();
+      // Trailing update S(i, l) -= lv_i lv_l, j < l <= i < q, spread evenly
+      // over the block by the packed index of the trailing triangle.
+      const int nt = q - j - 1;
+      const int cnt = nt * (nt + 1) / 2;
+      if (tid < cnt)
+      {
+        int ii, ll;
+        unpack(tid, ii, ll);
+        for (int t = tid; t < cnt; t += NT)
+        {
+          const int i = j + 1 + ii, l = j + 1 + ll;
+          double* e = S + pk(i, l);
+          *e = fma(-lv[i], lv[l], *e);
+          ll += NT;
+          while (ll > ii)
+          {
+            ll -= ii + 1;
+            ++ii;
+          }
+        }
+      }
+      __syncthreads();
+      p = j + 1;
     }
-    __syncthreads();
-    p = j + 1;
   }
   if (broke)
   {
@@ -338,7 +474,7 @@ void launch_ns_pass2(const pchol_desc* ds, int count, int* skip, hipStream_t st)
 size_t pchol_lds_bytes(int q)
 {
   constexpr int NT = 256;
-  return sizeof(double) * ((size_t)q * (q + 1) / 2 + q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
+  return sizeof(double) * ((size_t)q * (q + 1) / 2 + 2 * (size_t)q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
 }
 
 void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st)
