@@ -239,6 +239,7 @@ struct pchol_desc
   const int* skip = nullptr;
 };
 constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
+constexpr int kPcholBlock = 4;   // pivots per trailing update (pivoted form)
 size_t pchol_lds_bytes(int q);
 // Persistent launch fed from a work queue: `order` (count device ints) lists
 // the problems in the order they are taken (largest first); `queue` is one

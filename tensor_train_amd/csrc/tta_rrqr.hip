@@ -2,11 +2,11 @@
 // rank-revealing CholQR2 step of the mixed batched rounding.
 //
 // One workgroup per problem.  The symmetric q x q matrix G (row-major) is
-// held as a packed lower triangle in LDS (q <= kPcholMaxQ, about 147 KiB at
-// q = 192) and factored right-looking, one pivot per step, the pivot moved
-// to the front by a symmetric swap (as LAPACK's dpstf2) so the trailing
-// Schur complement stays a contiguous triangle whose update is spread
-// evenly over the workgroup:
+// held as a packed lower triangle in LDS (q <= kPcholMaxQ, about 152 KiB at
+// q = 192) and factored right-looking, each pivot moved to the front by a
+// symmetric swap (as LAPACK's dpstf2) so the trailing Schur complement stays
+// a contiguous triangle; with pivoting the trailing update is applied once
+// per block of kPcholBlock pivots, spread evenly over the workgroup:
 //   * pivoting on: the pivot is the largest remaining diagonal entry of the
 //     Schur complement S (ties to the smaller index), and the factorization
 //     stops before step j when trace(S) <= stop_f * delta^2 or the largest
@@ -95,9 +95,8 @@ __device__ void pchol_one(const pchol_desc& d)
   extern __shared__ double sm[];
   const int npk = q * (q + 1) / 2;
   double* S = sm;            // packed lower triangle, by position
-  double* lv = S + npk;      // the current column of the factor, by position
-  double* lv2 = lv + q;      // the second column of a pivot pair (pivoted form)
-  double* red = lv2 + q;
+  double* lv = S + npk;      // columns of the factor, by position (kPcholBlock of q)
+  double* red = lv + kPcholBlock * q;
   int* ord = reinterpret_cast<int*>(red + 3 * (NT / 64));  // position -> original index
   if (q <= 0)
   {
@@ -125,9 +124,9 @@ __device__ void pchol_one(const pchol_desc& d)
   bool broke = false;
   double tr_left = 0.0;
   // Symmetric swap of positions j and pj (pj > j) in the trailing part of S
-  // (entries of positions above j and the diagonal), the permutation and,
-  // when given, a column buffer indexed by position.
-  auto swap_pos = [&](int j, int pj, double* col) {
+  // (entries of positions above j and the diagonal), the permutation and
+  // the first ncol factor columns held in lv.
+  auto swap_pos = [&](int j, int pj, int ncol) {
     for (int l = j + 1 + tid; l < q; l += NT)
     {
       if (l < pj)
@@ -151,11 +150,11 @@ __device__ void pchol_one(const pchol_desc& d)
       const int o = ord[j];
       ord[j] = ord[pj];
       ord[pj] = o;
-      if (col)
+      for (int c = 0; c < ncol; ++c)
       {
-        const double y = col[j];
-        col[j] = col[pj];
-        col[pj] = y;
+        const double y = lv[c * q + j];
+        lv[c * q + j] = lv[c * q + pj];
+        lv[c * q + pj] = y;
       }
     }
   };
@@ -179,62 +178,53 @@ __device__ void pchol_one(const pchol_desc& d)
   };
   if (d.pivot)
   {
-    // Pivots in pairs: the first column updates only the diagonal; the
-    // second pivot is chosen on that updated diagonal, its column taken from
-    // the raw entries minus the first column's term; then one pass over the
-    // trailing triangle applies both rank-1 updates (half the LDS
-    // read-modify-writes of one pass per pivot).  The stop and the pivot
-    // choices are those of the one-at-a-time form.
-    for (int j = 0; j < q; j += 2)
+    // Pivots in blocks of kPcholBlock: each pivot of a block is chosen on the
+    // diagonal updated by the block's earlier columns, its column taken from
+    // the raw entries minus those columns' terms; then one pass over the
+    // trailing triangle applies the whole block's rank-1 updates (a quarter
+    // of the passes, and of their barriers, of one pass per pivot).  The stop
+    // and the pivot choices are those of the one-at-a-time form.
+    for (int j = 0; j < q; j += kPcholBlock)
     {
-      double tr, mv;
-      int mi;
-      diag_scan(j, tr, mv, mi);
-      tr_left = tr;
-      if (tr <= stop2 || !(mv > floor_piv)) break;
-      if (mi != j)
+      bool stop = false;
+      for (int m = 0; m < kPcholBlock && j + m < q; ++m)
       {
-        swap_pos(j, mi, nullptr);
-        __syncthreads();
-      }
-      {
-        const double dj = sqrt(mv), inv = 1.0 / dj;
-        for (int i = j + tid; i < q; i += NT)
+        const int jm = j + m;
+        double tr, mv;
+        int mi;
+        diag_scan(jm, tr, mv, mi);
+        tr_left = tr;
+        if (tr <= stop2 || !(mv > floor_piv))
         {
-          const double li = i == j ? dj : S[pk(i, j)] * inv;
-          lv[i] = li;
-          d.L[(size_t)j * q + ord[i]] = li;
-          if (i > j) S[pk(i, i)] = fma(-li, li, S[pk(i, i)]);
+          stop = true;
+          break;
         }
-      }
-      __syncthreads();
-      p = j + 1;
-      const int j2 = j + 1;
-      if (j2 >= q) break;
-      diag_scan(j2, tr, mv, mi);
-      tr_left = tr;
-      if (tr <= stop2 || !(mv > floor_piv)) break;
-      if (mi != j2)
-      {
-        swap_pos(j2, mi, lv);
-        __syncthreads();
-      }
-      {
-        const double dj = sqrt(mv), inv = 1.0 / dj;
-        const double a = lv[j2];
-        for (int i = j2 + tid; i < q; i += NT)
+        if (mi != jm)
         {
-          const double li = i == j2 ? dj : fma(-lv[i], a, S[pk(i, j2)]) * inv;
-          lv2[i] = li;
-          d.L[(size_t)j2 * q + ord[i]] = li;
-          if (i > j2) S[pk(i, i)] = fma(-li, li, S[pk(i, i)]);
+          swap_pos(jm, mi, m);
+          __syncthreads();
         }
+        const double dj = sqrt(mv), inv = 1.0 / dj;
+        for (int i = jm + tid; i < q; i += NT)
+        {
+          double li = dj;
+          if (i > jm)
+          {
+            double x = S[pk(i, jm)];
+            for (int c = 0; c < m; ++c) x = fma(-lv[c * q + i], lv[c * q + jm], x);
+            li = x * inv;
+            S[pk(i, i)] = fma(-li, li, S[pk(i, i)]);
+          }
+          lv[m * q + i] = li;
+          d.L[(size_t)jm * q + ord[i]] = li;
+        }
+        __syncthreads();
+        p = jm + 1;
       }
-      __syncthreads();
-      p = j2 + 1;
-      // Off-diagonal trailing entries S(i, l), j2 < l < i < q, spread evenly
+      if (stop || p == q) break;
+      // Off-diagonal trailing entries S(i, l), p <= l < i < q, spread evenly
       // over the block by the packed index of the strictly lower triangle.
-      const int nt = q - j2 - 1;
+      const int nt = q - p;
       const int cnt = nt * (nt - 1) / 2;
       if (tid < cnt)
       {
@@ -242,9 +232,12 @@ __device__ void pchol_one(const pchol_desc& d)
         unpack(tid, ii, ll);
         for (int t = tid; t < cnt; t += NT)
         {
-          const int i = j2 + 2 + ii, l = j2 + 1 + ll;
+          const int i = p + 1 + ii, l = p + ll;
           double* e = S + pk(i, l);
-          *e = fma(-lv2[i], lv2[l], fma(-lv[i], lv[l], *e));
+          double x = *e;
+#pragma unroll
+          for (int c = 0; c < kPcholBlock; ++c) x = fma(-lv[c * q + i], lv[c * q + l], x);
+          *e = x;
           ll += NT;
           while (ll > ii)
           {
@@ -474,7 +467,7 @@ void launch_ns_pass2(const pchol_desc* ds, int count, int* skip, hipStream_t st)
 size_t pchol_lds_bytes(int q)
 {
   constexpr int NT = 256;
-  return sizeof(double) * ((size_t)q * (q + 1) / 2 + 2 * (size_t)q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
+  return sizeof(double) * ((size_t)q * (q + 1) / 2 + kPcholBlock * (size_t)q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
 }
 
 void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st)
