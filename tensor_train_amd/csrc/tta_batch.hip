@@ -190,7 +190,6 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
   std::vector<gemm_desc> m1, m2;
   std::vector<double*> Us, Nn;
   long max_g1 = 0, max_g2 = 0;
-  int m1_m = 0, m1_n = 0, m2_m = 0, m2_n = 0;
   for (int a = 0; a < nb; ++a)
   {
     tta_tt* t = ts[idx[a]];
@@ -223,8 +222,6 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
       g.scr = r_new;
       g.scc = 1;
       m1.push_back(g);
-      m1_m = std::max(m1_m, rows);
-      m1_n = std::max(m1_n, r_new);
     }
     else
     {
@@ -248,8 +245,6 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
       g.scr = cols;
       g.scc = 1;
       m1.push_back(g);
-      m1_m = std::max(m1_m, r_new);
-      m1_n = std::max(m1_n, cols);
     }
     max_g1 = std::max(max_g1, (long)q * r_new);
     max_g2 = std::max(max_g2, (long)q * r_new);
@@ -269,8 +264,6 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
     g.scc = 1;
     Nn.push_back(g.C);
     m2.push_back(g);
-    m2_m = std::max(m2_m, r_new);
-    m2_n = std::max(m2_n, ncols_n);
   }
   {
     prof_scope ps(c, TTA_K_OTHER);
@@ -283,8 +276,8 @@ void svd_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<int>
     w.flops += w2.flops;
     w.bytes += w2.bytes;
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-    launch_gemm_grouped(S.upload(m1), nb, m1_m, m1_n, c->stream);
-    launch_gemm_grouped(S.upload(m2), nb, m2_m, m2_n, c->stream);
+    launch_gemm_grouped(S, m1, c->stream);
+    launch_gemm_grouped(S, m2, c->stream);
   }
   for (int a = 0; a < nb; ++a)
   {
@@ -307,7 +300,7 @@ void right_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<in
   std::vector<qrp> qr;
   std::vector<ex_desc> ex;
   std::vector<gemm_desc> gm;
-  int max_m = 0, max_n = 0, max_gm = 0, max_gn = 0;
+  int max_m = 0, max_n = 0;
   long max_ex = 0;
   std::vector<double*> newQ, newP;
   for (int i : sel)
@@ -358,8 +351,6 @@ void right_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<in
     g.scc = 1;
     newP.push_back(g.C);
     gm.push_back(g);
-    max_gm = std::max(max_gm, rows_p);
-    max_gn = std::max(max_gn, r_eff);
   }
   const int nb = (int)idx.size();
   if (nb == 0) return;
@@ -381,7 +372,7 @@ void right_step_plain(tta_ctx* c, std::vector<tta_tt*>& ts, const std::vector<in
   {
     const credit w = gemm_credit(gm.data(), gm.size());
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-    launch_gemm_grouped(S.upload(gm), nb, max_gm, max_gn, c->stream);
+    launch_gemm_grouped(S, gm, c->stream);
   }
   for (int a = 0; a < nb; ++a)
   {
@@ -488,7 +479,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   {
     const credit w = gemm_credit(g1.data(), g1.size());
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-    launch_gemm_grouped(S.upload(g1), nb, max_q, max_q, c->stream);
+    launch_gemm_grouped(S, g1, c->stream);
   }
   int* queue = S.alloc<int>(2);
   {
@@ -617,8 +608,8 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
       w.bytes += w2.bytes;
     }
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-    launch_gemm_grouped(S.upload(gq1), nc, max_m, mx_p, c->stream);
-    launch_gemm_grouped(S.upload(gg2), nc, mx_p, mx_p, c->stream);
+    launch_gemm_grouped(S, gq1, c->stream);
+    launch_gemm_grouped(S, gg2, c->stream);
   }
   {
     // Second pass: one Newton-Schulz step where Q1 is already orthonormal to
@@ -638,9 +629,9 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
       w.bytes += w2.bytes;
     }
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-    launch_gemm_grouped(S.upload(gu), nc, max_m, mx_p, c->stream);
-    launch_gemm_grouped(S.upload(grt), nc, mx_p, max_q, c->stream);
-    launch_gemm_grouped(S.upload(ge), nc, max_m, max_q, c->stream);
+    launch_gemm_grouped(S, gu, c->stream);
+    launch_gemm_grouped(S, grt, c->stream);
+    launch_gemm_grouped(S, ge, c->stream);
   }
   {
     prof_scope ps(c, TTA_K_OTHER);
@@ -713,7 +704,6 @@ std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const 
   std::vector<gemm_desc> gn;
   std::vector<int> acc;
   std::vector<double*> Nn;
-  int mx_p = 0, mx_nn = 0;
   for (size_t a = 0; a < el.size(); ++a)
   {
     const int i = el[a];
@@ -736,15 +726,13 @@ std::vector<int> svd_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, const 
     Nn.push_back(g.C);
     gn.push_back(g);
     acc.push_back((int)a);
-    mx_p = std::max(mx_p, p);
-    mx_nn = std::max(mx_nn, ncols_n);
   }
   if (!gn.empty())
   {
     {
       const credit w = gemm_credit(gn.data(), gn.size());
       prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-      launch_gemm_grouped(S.upload(gn), (int)gn.size(), mx_p, mx_nn, c->stream);
+      launch_gemm_grouped(S, gn, c->stream);
     }
     for (size_t x = 0; x < acc.size(); ++x)
     {
@@ -812,7 +800,6 @@ std::vector<int> right_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, cons
   std::vector<gemm_desc> gn;
   std::vector<int> acc;
   std::vector<double*> Nn;
-  int mx_rows = 0, mx_p = 0;
   for (size_t a = 0; a < el.size(); ++a)
   {
     const int i = el[a];
@@ -837,15 +824,13 @@ std::vector<int> right_step_certified(tta_ctx* c, std::vector<tta_tt*>& ts, cons
     Nn.push_back(g.C);
     gn.push_back(g);
     acc.push_back((int)a);
-    mx_rows = std::max(mx_rows, rows_p);
-    mx_p = std::max(mx_p, p);
   }
   if (!gn.empty())
   {
     {
       const credit w = gemm_credit(gn.data(), gn.size());
       prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-      launch_gemm_grouped(S.upload(gn), (int)gn.size(), mx_rows, mx_p, c->stream);
+      launch_gemm_grouped(S, gn, c->stream);
     }
     for (size_t x = 0; x < acc.size(); ++x)
     {

@@ -830,15 +830,9 @@ bool cert_ok(int max_rank, int q);
 void gemm_batch_go(tta_ctx* c, arena& ar, const std::vector<gemm_desc>& gs)
 {
   if (gs.empty()) return;
-  int mm = 0, mn = 0;
   const credit cr = gemm_credit(gs.data(), gs.size());
-  for (const gemm_desc& g : gs)
-  {
-    mm = std::max(mm, g.M);
-    mn = std::max(mn, g.N);
-  }
   prof_scope ps(c, TTA_K_GEMM_GROUPED, cr.flops, cr.bytes);
-  launch_gemm_grouped(ar.upload(gs), (int)gs.size(), mm, mn, c->stream);
+  launch_gemm_grouped(ar, gs, c->stream);
 }
 
 // Whether step k of a sweep in direction dir may join a chained block: a

@@ -593,6 +593,20 @@ __global__ __launch_bounds__(256) void gemm_f64_grouped_kernel(const gemm_desc* 
                                                   g.scc, g.alpha, g.beta, blockIdx.y * TT, blockIdx.x * TT);
 }
 
+// Grouped GEMM over a list of the output tiles that exist: workgroup b
+// computes tile tiles[b].y (row-major over the problem's tile grid) of
+// problem tiles[b].x.
+template <int TT>
+__global__ __launch_bounds__(256) void gemm_f64_tiles_kernel(const gemm_desc* gs, const int2* tiles)
+{
+  const int2 t = tiles[blockIdx.x];
+  const gemm_desc& g = gs[t.x];
+  const int tn = (g.N + TT - 1) / TT;
+  const int ty = t.y / tn, tx = t.y - ty * tn;
+  gemm_tile_t<double, double, double, double, TT>(g.M, g.N, g.K, g.A, g.sar, g.sac, g.B, g.sbr, g.sbc, g.C, g.scr,
+                                                  g.scc, g.alpha, g.beta, ty * TT, tx * TT);
+}
+
 // Grouped GEMM with per-operand precision (xgemm_desc.types): A, B and C
 // each FP64 or FP32, multiplied in FP32 on the matrix cores when
 // types.compute is FP32, in FP64 otherwise.
@@ -1960,9 +1974,19 @@ void gemm_autotune_stats(long long* keys, long long* decided, long long* chose_s
   if (chose_small) *chose_small = sm;
 }
 
-void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st)
+namespace
 {
-  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+// Tile choice of a grouped FP64 GEMM (64, or 32 for small outputs), timed
+// per shape bucket until each tile has kTuneTrials timings.
+struct gemm_pick
+{
+  int variant = 0;
+  bool timed = false;
+  tune_entry* e = nullptr;
+};
+
+gemm_pick pick_gemm_tile(int count, int max_m, int max_n, hipStream_t st)
+{
   int variant = 0;
   bool timed = false;
   tune_entry* e = nullptr;
@@ -2010,7 +2034,22 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
       }
     }
   }
-  if (variant == 1)
+  return gemm_pick{variant, timed, e};
+}
+
+void end_gemm_pick(const gemm_pick& t, hipStream_t st)
+{
+  if (!t.timed) return;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  (void)hipEventRecord(t.e->ev[t.variant][1], st);
+}
+}  // namespace
+
+void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st)
+{
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  const gemm_pick t = pick_gemm_tile(count, max_m, max_n, st);
+  if (t.variant == 1)
   {
     dim3 grid((max_n + 31) / 32, (max_m + 31) / 32, count);
     hipLaunchKernelGGL(gemm_f64_grouped_kernel<32>, grid, dim3(256), 0, st, descs_dev);
@@ -2020,11 +2059,59 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
     dim3 grid((max_n + GT - 1) / GT, (max_m + GT - 1) / GT, count);
     hipLaunchKernelGGL(gemm_f64_grouped_kernel<GT>, grid, dim3(256), 0, st, descs_dev);
   }
-  if (timed)
+  end_gemm_pick(t, st);
+}
+
+void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t st)
+{
+  const int count = (int)v.size();
+  int max_m = 0, max_n = 0;
+  for (const gemm_desc& g : v)
   {
-    std::lock_guard<std::mutex> lk(g_tune_mu);
-    (void)hipEventRecord(e->ev[variant][1], st);
+    max_m = std::max(max_m, g.M);
+    max_n = std::max(max_n, g.N);
   }
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  const gemm_desc* dv = S.upload(v);
+  const gemm_pick t = pick_gemm_tile(count, max_m, max_n, st);
+  const int T = t.variant == 1 ? 32 : GT;
+  const long per = (long)((max_m + T - 1) / T) * ((max_n + T - 1) / T);
+  long used = 0;
+  for (const gemm_desc& g : v)
+    if (g.M > 0 && g.N > 0) used += (long)((g.M + T - 1) / T) * ((g.N + T - 1) / T);
+  if (used == 0)
+  {
+    end_gemm_pick(t, st);
+    return;
+  }
+  if ((4 * used > 3 * per * count && count <= 65535) || used > 0x7fffffffL)
+  {
+    // Shapes close to the largest: the grid over it wastes little.
+    dim3 grid((max_n + T - 1) / T, (max_m + T - 1) / T, count);
+    if (T == 32)
+      hipLaunchKernelGGL(gemm_f64_grouped_kernel<32>, grid, dim3(256), 0, st, dv);
+    else
+      hipLaunchKernelGGL(gemm_f64_grouped_kernel<GT>, grid, dim3(256), 0, st, dv);
+  }
+  else
+  {
+    // One workgroup per output tile that exists.
+    std::vector<int2> tiles;
+    tiles.reserve(used);
+    for (int z = 0; z < count; ++z)
+    {
+      const gemm_desc& g = v[z];
+      if (g.M <= 0 || g.N <= 0) continue;
+      const int nt = ((g.M + T - 1) / T) * ((g.N + T - 1) / T);
+      for (int k = 0; k < nt; ++k) tiles.push_back(make_int2(z, k));
+    }
+    const int2* td = S.upload(tiles);
+    if (T == 32)
+      hipLaunchKernelGGL(gemm_f64_tiles_kernel<32>, dim3((unsigned)used), dim3(256), 0, st, dv, td);
+    else
+      hipLaunchKernelGGL(gemm_f64_tiles_kernel<GT>, dim3((unsigned)used), dim3(256), 0, st, dv, td);
+  }
+  end_gemm_pick(t, st);
 }
 
 void launch_transpose_batched(const tr_desc* ds, int count, int max_rows, int max_cols, hipStream_t st)

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <utility>
+#include <vector>
 
 struct tta_ctx;
 
@@ -408,6 +409,12 @@ void set_gemm_autotune(bool on);
 void gemm_autotune_stats(long long* keys, long long* decided, long long* chose_small);
 // Grouped GEMM over `count` device descriptors; max_m / max_n bound them.
 void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int max_n, hipStream_t st);
+// The same from host descriptors: uploads them through the arena and, when
+// the problems' output shapes differ so much that a grid over the largest
+// would be mostly idle workgroups, launches one workgroup per output tile
+// that exists.
+class arena;
+void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t st);
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);

@@ -73,15 +73,9 @@ void run_xgrouped(tta_ctx* c, arena& O, const std::vector<xgemm_desc>& gs, xgemm
 void run_grouped(tta_ctx* c, arena& O, const std::vector<gemm_desc>& gs)
 {
   if (gs.empty()) return;
-  int mm = 0, nn = 0;
-  for (auto& g : gs)
-  {
-    mm = std::max(mm, g.M);
-    nn = std::max(nn, g.N);
-  }
   const credit w = gemm_credit(gs.data(), gs.size());
   prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-  launch_gemm_grouped(O.upload(gs), (int)gs.size(), mm, nn, c->stream);
+  launch_gemm_grouped(O, gs, c->stream);
 }
 }  // namespace
 
