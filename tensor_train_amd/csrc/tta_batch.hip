@@ -425,16 +425,6 @@ struct rr_out
   double* Rt = nullptr;  // in the caller's arena
 };
 
-// The work-queue order of a batched pivoted Cholesky: problems by decreasing
-// size q (work ~ q^3), so the largest start first and the small ones fill in.
-std::vector<int> largest_first(const std::vector<pchol_desc>& pc)
-{
-  std::vector<int> o(pc.size());
-  for (size_t i = 0; i < o.size(); ++i) o[i] = (int)i;
-  std::stable_sort(o.begin(), o.end(), [&](int a, int b) { return pc[a].q > pc[b].q; });
-  return o;
-}
-
 std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob>& pr)
 {
   const int nb = (int)pr.size();
@@ -446,7 +436,6 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   double* trl = S.alloc<double>((size_t)nb);
   std::vector<gemm_desc> g1;
   std::vector<pchol_desc> pc;
-  int max_q = 0, max_m = 0;
   for (int a = 0; a < nb; ++a)
   {
     const rr_prob& P = pr[a];
@@ -473,18 +462,15 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     d.ldr = P.q;
     d.tr_out = trl + a;
     pc.push_back(d);
-    max_q = std::max(max_q, P.q);
-    max_m = std::max(max_m, P.m);
   }
   {
     const credit w = gemm_credit(g1.data(), g1.size());
     prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
     launch_gemm_grouped(S, g1, c->stream);
   }
-  int* queue = S.alloc<int>(2);
   {
     prof_scope ps(c, TTA_K_CHOL);
-    launch_pchol_batch(S.upload(pc), nb, max_q, S.upload(largest_first(pc)), queue, c->stream);
+    launch_pchol_batch(S, S.upload(pc), pc, c->stream);
   }
   std::vector<int> hp(nb);
   std::vector<double> htr(nb);
@@ -510,7 +496,6 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
   int* p2 = S.alloc<int>((size_t)nc);
   int* skip2 = S.alloc<int>((size_t)nc);
   double* en = S.alloc<double>((size_t)nc * 2);
-  int mx_p = 0;
   long mx_ga = 0, mx_cp = 0;
   for (int b = 0; b < nc; ++b)
   {
@@ -592,7 +577,6 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     sd.x = P.M;
     sd.out_norm = en + 2 * b + 1;
     sq.push_back(sd);
-    mx_p = std::max(mx_p, p);
   }
   {
     prof_scope ps(c, TTA_K_OTHER);
@@ -618,7 +602,7 @@ std::vector<rr_out> rr_cholqr2(tta_ctx* c, scratch& S, const std::vector<rr_prob
     prof_scope ps(c, TTA_K_CHOL);
     const pchol_desc* pc2_dev = S.upload(pc2);
     launch_ns_pass2(pc2_dev, nc, skip2, c->stream);
-    launch_pchol_batch(pc2_dev, nc, mx_p, S.upload(largest_first(pc2)), queue + 1, c->stream);
+    launch_pchol_batch(S, pc2_dev, pc2, c->stream);
   }
   {
     credit w;

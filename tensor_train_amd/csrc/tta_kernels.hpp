@@ -10,6 +10,7 @@ struct tta_ctx;
 
 namespace tta
 {
+class arena;  // bump allocator of a step's device buffers (tta_internal.hpp)
 
 // C(i,j) = alpha * sum_k A(i,k) B(k,j) + beta * C(i,j), element strides per
 // operand, batch strides bs* (elements) over blockIdx.z.
@@ -242,10 +243,12 @@ struct pchol_desc
 constexpr int kPcholMaxQ = 192;  // packed FP64 triangle in LDS
 constexpr int kPcholBlock = 4;   // pivots per trailing update (pivoted form)
 size_t pchol_lds_bytes(int q);
-// Persistent launch fed from a work queue: `order` (count device ints) lists
-// the problems in the order they are taken (largest first); `queue` is one
-// device int of scratch (zeroed by the launch).
-void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st);
+// Persistent launches fed from work queues (problems taken largest first),
+// one per residency class: the problems are grouped by how many of their
+// workgroups fit a compute unit's LDS (1, 2, 4 or 8), and each group runs
+// with the LDS of its own largest problem, so small problems are not held
+// to the residency of the largest.  ds: the descriptors `host` as uploaded.
+void launch_pchol_batch(arena& S, const pchol_desc* ds, const std::vector<pchol_desc>& host, hipStream_t st);
 // Second pass of CholQR2 by one Newton-Schulz step where it suffices: for
 // each problem (descriptor of the unpivoted pass, device array), with
 // G2 = Q1^T Q1 = I + E2 in d.G: when p max|E2| <= 1e-8, Rinv = (3 I - G2) / 2
@@ -413,7 +416,6 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 // the problems' output shapes differ so much that a grid over the largest
 // would be mostly idle workgroups, launches one workgroup per output tile
 // that exists.
-class arena;
 void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t st);
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);

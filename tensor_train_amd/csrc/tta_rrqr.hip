@@ -412,7 +412,10 @@ __global__ __launch_bounds__(NT) void pchol_batch_kernel(const pchol_desc* ds, i
   }
 }
 
-__global__ void zero_int_kernel(int* p) { *p = 0; }
+__global__ void zero_int_kernel(int* p, int n)
+{
+  if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
 
 __device__ __forceinline__ double wave_max_all(double v)
 {
@@ -470,11 +473,10 @@ size_t pchol_lds_bytes(int q)
   return sizeof(double) * ((size_t)q * (q + 1) / 2 + kPcholBlock * (size_t)q + 3 * (NT / 64)) + sizeof(int) * (size_t)q;
 }
 
-void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* order, int* queue, hipStream_t st)
+void launch_pchol_batch(arena& S, const pchol_desc* ds, const std::vector<pchol_desc>& host, hipStream_t st)
 {
+  const int count = (int)host.size();
   if (count <= 0) return;
-  if (max_q > kPcholMaxQ) fail(TTA_ERR_UNSUPPORTED, "pivoted Cholesky: %d columns above %d", max_q, kPcholMaxQ);
-  const size_t lds = pchol_lds_bytes(max_q);
   static int cus = 0, lds_cu = 0;
   if (cus == 0)
   {
@@ -485,11 +487,42 @@ void launch_pchol_batch(const pchol_desc* ds, int count, int max_q, const int* o
         lds_cu <= 0)
       lds_cu = 160 * 1024;
   }
-  // Resident workgroups: by LDS, at most 8 of 256 threads per compute unit.
-  const int per_cu = std::max(1, std::min(8, (int)((size_t)lds_cu / std::max<size_t>(lds, 1))));
-  const int grid = std::min(count, cus * per_cu);
-  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(1), 0, st, queue);
-  hipLaunchKernelGGL(pchol_batch_kernel<256>, dim3(grid), dim3(256), lds, st, ds, count, order, queue);
+  std::vector<int> order(count);
+  for (int i = 0; i < count; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return host[a].q > host[b].q; });
+  const int max_q = host[order[0]].q;
+  if (max_q > kPcholMaxQ) fail(TTA_ERR_UNSUPPORTED, "pivoted Cholesky: %d columns above %d", max_q, kPcholMaxQ);
+  // Resident workgroups of 256 threads per compute unit that a problem's LDS
+  // allows, rounded down to 1, 2, 4 or 8.  It never decreases along the
+  // largest-first order, so its runs are the classes.
+  auto resident = [&](int q) {
+    const int r = std::max(1, std::min(8, (int)((size_t)lds_cu / std::max<size_t>(pchol_lds_bytes(q), 1))));
+    int b = 1;
+    while (2 * b <= r) b *= 2;
+    return b;
+  };
+  std::vector<int> start, per_cu;
+  for (int i = 0; i < count; ++i)
+  {
+    const int r = resident(host[order[i]].q);
+    if (per_cu.empty() || r != per_cu.back())
+    {
+      start.push_back(i);
+      per_cu.push_back(r);
+    }
+  }
+  start.push_back(count);
+  const int nclass = (int)per_cu.size();
+  const int* od = S.upload(order);
+  int* queues = S.alloc<int>(nclass);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, st, queues, nclass);
+  for (int k = 0; k < nclass; ++k)
+  {
+    const int n = start[k + 1] - start[k];
+    const size_t lds = pchol_lds_bytes(host[order[start[k]]].q);
+    const int grid = std::min(n, cus * per_cu[k]);
+    hipLaunchKernelGGL(pchol_batch_kernel<256>, dim3(grid), dim3(256), lds, st, ds, n, od + start[k], queues + k);
+  }
 }
 
 }  // namespace tta
