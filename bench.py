@@ -238,15 +238,20 @@ def _rel(T, a, b):
     return float(np.linalg.norm(dif.cores[0]) / np.linalg.norm(ref.cores[0]))
 
 
-def _timed(ctx, fn):
+def _timed(ctx, fn, reps=1):
     """(fn(), seconds): fn timed on the wall clock, the stream drained before
-    and after."""
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    out = fn()
-    ctx.synchronize()
-    wall = time.perf_counter() - t0
-    return out, wall
+    and after; with reps > 1 the median of that many calls (the last call's
+    result is returned)."""
+    walls = []
+    out = None
+    for _ in range(reps):
+        del out
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        out = fn()
+        ctx.synchronize()
+        walls.append(time.perf_counter() - t0)
+    return out, sorted(walls)[len(walls) // 2]
 
 
 def _kernel_ms(ctx, fn, classes=("qr", "jacobi", "gemm", "chol", "other")):
@@ -308,9 +313,10 @@ def _three_paths(T, ctx, batch, mlp, gate_open, eps, label):
         T.round_batched(batch, eps=eps, precision="mixed")
         learned_fn()
     ctx.synchronize()
-    plain, t_plain = _timed(ctx, lambda: T.round_batched(batch, eps=eps))
-    mixed, t_mixed = _timed(ctx, lambda: T.round_batched(batch, eps=eps, precision="mixed"))
-    (learned, err, fb, ranks), t_learned = _timed(ctx, learned_fn)
+    # Median of three timed passes of each path.
+    plain, t_plain = _timed(ctx, lambda: T.round_batched(batch, eps=eps), reps=3)
+    mixed, t_mixed = _timed(ctx, lambda: T.round_batched(batch, eps=eps, precision="mixed"), reps=3)
+    (learned, err, fb, ranks), t_learned = _timed(ctx, learned_fn, reps=3)
     t_policy = t_learned if gate_open else t_mixed
     policy = learned if gate_open else mixed
     sample = list(range(0, n, max(1, n // 16)))

@@ -620,6 +620,19 @@ __global__ __launch_bounds__(256) void xgemm_grouped_kernel(const xgemm_desc* gs
                               blockIdx.x * GT);
 }
 
+// The same over a list of the output tiles that exist (as
+// gemm_f64_tiles_kernel).
+template <class TA, class TB, class TC, class CT>
+__global__ __launch_bounds__(256) void xgemm_tiles_kernel(const xgemm_desc* gs, const int2* tiles)
+{
+  const int2 w = tiles[blockIdx.x];
+  const xgemm_desc& g = gs[w.x];
+  const int tn = (g.N + GT - 1) / GT;
+  const int ty = w.y / tn, tx = w.y - ty * tn;
+  gemm_tile_t<TA, TB, TC, CT>(g.M, g.N, g.K, static_cast<const TA*>(g.A), g.sar, g.sac, static_cast<const TB*>(g.B),
+                              g.sbr, g.sbc, static_cast<TC*>(g.C), g.scr, g.scc, 1.0, 0.0, ty * GT, tx * GT);
+}
+
 // Householder QR (+ optional thin Q) of one column-major matrix per block.
 template <int NT>
 __global__ __launch_bounds__(NT) void qr_kernel(qr_desc q)
@@ -1917,6 +1930,70 @@ void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int
     hipLaunchKernelGGL((xgemm_grouped_kernel<double, double, double, double>), grid, dim3(256), 0, st, descs_dev);
 }
 
+namespace
+{
+// Output tiles (problem, tile index) of the problems of v that have any, for
+// a T x T tile, and their count.
+template <class D>
+long count_tiles(const std::vector<D>& v, int T)
+{
+  long used = 0;
+  for (const D& g : v)
+    if (g.M > 0 && g.N > 0) used += (long)((g.M + T - 1) / T) * ((g.N + T - 1) / T);
+  return used;
+}
+template <class D>
+std::vector<int2> tile_list(const std::vector<D>& v, int T, long used)
+{
+  std::vector<int2> tiles;
+  tiles.reserve(used);
+  for (int z = 0; z < (int)v.size(); ++z)
+  {
+    const D& g = v[z];
+    if (g.M <= 0 || g.N <= 0) continue;
+    const int nt = ((g.M + T - 1) / T) * ((g.N + T - 1) / T);
+    for (int k = 0; k < nt; ++k) tiles.push_back(make_int2(z, k));
+  }
+  return tiles;
+}
+// Whether a grid over the largest shape (per tiles per problem) would hold
+// few idle workgroups, or cannot be replaced by a one-dimensional grid.
+bool full_grid_ok(long used, long per, int count)
+{
+  return (4 * used > 3 * per * count && count <= 65535) || used > 0x7fffffffL;
+}
+}  // namespace
+
+void launch_xgemm_grouped(arena& S, const std::vector<xgemm_desc>& v, xgemm_types t, hipStream_t st)
+{
+  const int count = (int)v.size();
+  int max_m = 0, max_n = 0;
+  for (const xgemm_desc& g : v)
+  {
+    max_m = std::max(max_m, g.M);
+    max_n = std::max(max_n, g.N);
+  }
+  if (count <= 0 || max_m <= 0 || max_n <= 0) return;
+  const xgemm_desc* dv = S.upload(v);
+  const long used = count_tiles(v, GT);
+  if (used == 0) return;
+  if (full_grid_ok(used, (long)((max_m + GT - 1) / GT) * ((max_n + GT - 1) / GT), count))
+  {
+    launch_xgemm_grouped(dv, count, max_m, max_n, t, st);
+    return;
+  }
+  const int2* td = S.upload(tile_list(v, GT, used));
+  const dim3 grid((unsigned)used);
+  if (t == XG_D_S_S_S)
+    hipLaunchKernelGGL((xgemm_tiles_kernel<double, float, float, float>), grid, dim3(256), 0, st, dv, td);
+  else if (t == XG_S_S_S_S)
+    hipLaunchKernelGGL((xgemm_tiles_kernel<float, float, float, float>), grid, dim3(256), 0, st, dv, td);
+  else if (t == XG_D_S_D_D)
+    hipLaunchKernelGGL((xgemm_tiles_kernel<double, float, double, double>), grid, dim3(256), 0, st, dv, td);
+  else
+    hipLaunchKernelGGL((xgemm_tiles_kernel<double, double, double, double>), grid, dim3(256), 0, st, dv, td);
+}
+
 // ---------------------------------------------------------------------------
 // Autotuner of the grouped GEMM's tile size.  The 64 x 64 and 32 x 32 tiles
 // give identical results (see gemm_tile_t), so the choice only moves time:
@@ -2075,16 +2152,13 @@ void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t 
   const gemm_desc* dv = S.upload(v);
   const gemm_pick t = pick_gemm_tile(count, max_m, max_n, st);
   const int T = t.variant == 1 ? 32 : GT;
-  const long per = (long)((max_m + T - 1) / T) * ((max_n + T - 1) / T);
-  long used = 0;
-  for (const gemm_desc& g : v)
-    if (g.M > 0 && g.N > 0) used += (long)((g.M + T - 1) / T) * ((g.N + T - 1) / T);
+  const long used = count_tiles(v, T);
   if (used == 0)
   {
     end_gemm_pick(t, st);
     return;
   }
-  if ((4 * used > 3 * per * count && count <= 65535) || used > 0x7fffffffL)
+  if (full_grid_ok(used, (long)((max_m + T - 1) / T) * ((max_n + T - 1) / T), count))
   {
     // Shapes close to the largest: the grid over it wastes little.
     dim3 grid((max_n + T - 1) / T, (max_m + T - 1) / T, count);
@@ -2096,16 +2170,7 @@ void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t 
   else
   {
     // One workgroup per output tile that exists.
-    std::vector<int2> tiles;
-    tiles.reserve(used);
-    for (int z = 0; z < count; ++z)
-    {
-      const gemm_desc& g = v[z];
-      if (g.M <= 0 || g.N <= 0) continue;
-      const int nt = ((g.M + T - 1) / T) * ((g.N + T - 1) / T);
-      for (int k = 0; k < nt; ++k) tiles.push_back(make_int2(z, k));
-    }
-    const int2* td = S.upload(tiles);
+    const int2* td = S.upload(tile_list(v, T, used));
     if (T == 32)
       hipLaunchKernelGGL(gemm_f64_tiles_kernel<32>, dim3((unsigned)used), dim3(256), 0, st, dv, td);
     else

@@ -419,6 +419,9 @@ void launch_gemm_grouped(const gemm_desc* descs_dev, int count, int max_m, int m
 void launch_gemm_grouped(arena& S, const std::vector<gemm_desc>& v, hipStream_t st);
 void launch_xgemm_grouped(const xgemm_desc* descs_dev, int count, int max_m, int max_n, xgemm_types t,
                           hipStream_t st);
+// The same from host descriptors, over the output tiles that exist when a
+// grid over the largest shape would be mostly idle (as launch_gemm_grouped).
+void launch_xgemm_grouped(arena& S, const std::vector<xgemm_desc>& v, xgemm_types t, hipStream_t st);
 void launch_qr_blocked(const qr_batch& b, hipStream_t st);
 // Pieces of the two-level blocked QR of one matrix (b.one, no descriptors):
 // the 16-column panel at j0; its update of the columns before col_end only;

@@ -57,17 +57,14 @@ gemm_desc rm_gemm(int M, int N, int K, const double* A, long lda, const double* 
 void run_xgrouped(tta_ctx* c, arena& O, const std::vector<xgemm_desc>& gs, xgemm_types t)
 {
   if (gs.empty()) return;
-  int mm = 0, nn = 0;
   credit w;
   for (auto& g : gs)
   {
-    mm = std::max(mm, g.M);
-    nn = std::max(nn, g.N);
     w.flops += 2.0 * g.M * g.N * (double)g.K;
     w.bytes += 8.0 * ((double)g.M * g.K + (double)g.K * g.N + (double)g.M * g.N);
   }
   prof_scope ps(c, TTA_K_GEMM, w.flops, w.bytes);
-  launch_xgemm_grouped(O.upload(gs), (int)gs.size(), mm, nn, t, c->stream);
+  launch_xgemm_grouped(O, gs, t, c->stream);
 }
 
 void run_grouped(tta_ctx* c, arena& O, const std::vector<gemm_desc>& gs)
